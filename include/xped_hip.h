@@ -1,0 +1,145 @@
+/*
+ * xped_hip.h -- device layer of the MI355X block-sparse tensor path (C ABI).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 (XH_OK) or an error code; the
+ * message of the last failure on a context is available from xh_last_error().  One context
+ * per host thread; calls on a context are stream-ordered on the context's HIP stream and are
+ * not re-entrant.  Plans are immutable after creation and may be shared read-only.
+ *
+ * The reference performs each of these operations as a loop of per-block CPU library calls
+ * behind its compile-time plugin API Xped::PlainInterface (include/Xped/Interfaces/
+ * PlainInterface.hpp:102-110).  Each entry point below cites the loop it replaces.
+ */
+#ifndef XPED_HIP_H_
+#define XPED_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    XH_OK = 0,
+    XH_EINVAL = 1,   /* invalid argument or plan                                   */
+    XH_ENOMEM = 2,   /* device allocation failed (reference: StorageType_contiguous.hpp:91-96 rethrows bad_alloc) */
+    XH_EHIP = 3,     /* HIP runtime error                                           */
+    XH_ENOTCONV = 4, /* an iterative factorisation did not converge                 */
+    XH_ENODEV = 5    /* no usable GPU                                               */
+};
+
+enum { XH_FP64 = 0, XH_FP32 = 1, XH_BF16X3 = 2 };
+
+typedef struct xh_ctx xh_ctx;
+typedef struct xh_permute_plan xh_permute_plan;
+typedef struct xh_gemm_plan xh_gemm_plan;
+
+/* ---- context and buffers ------------------------------------------------------------ */
+int xh_create(int device, xh_ctx** out);
+int xh_destroy(xh_ctx* ctx);
+const char* xh_last_error(const xh_ctx* ctx);
+int xh_device_count(int* count);
+int xh_alloc(xh_ctx* ctx, size_t bytes, void** dptr);
+int xh_free(xh_ctx* ctx, void* dptr);
+int xh_upload(xh_ctx* ctx, void* dst, const void* src, size_t bytes);   /* host -> device, synchronous */
+int xh_download(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device -> host, synchronous */
+int xh_synchronize(xh_ctx* ctx);
+void* xh_stream(xh_ctx* ctx); /* the context's hipStream_t */
+int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value);
+
+/* ---- grouped permute ------------------------------------------------------------------
+ * Replaces the subblock loop of Tensor::permute_impl (src/Core/Tensor.cpp:83-146, 148-214):
+ * for every output subblock, dst = sum_s coeff_s * shuffle_p(src_s), accumulated in list order.
+ * Output leg k of a task reads source leg p[k]; its stride inside the source subblock is
+ * sstride[k] in the source row index (is_col[k] == 0) or column index (is_col[k] == 1). */
+#define XH_MAXLEGS 20
+typedef struct {
+    int64_t dst_off, dst_ld, rows, cols;
+    int32_t nsrc, src_begin;
+    uint32_t ext[XH_MAXLEGS];
+    int64_t sstride[XH_MAXLEGS];
+    uint8_t is_col[XH_MAXLEGS];
+} xh_permute_task;
+typedef struct {
+    int64_t src_off, src_ld;
+    double coeff;
+} xh_permute_source;
+typedef struct {
+    int32_t nlegs, new_rank;
+    int64_t ntasks, nsources;
+    const xh_permute_task* tasks;
+    const xh_permute_source* sources;
+} xh_permute_desc;
+int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, xh_permute_plan** out);
+int xh_permute_plan_destroy(xh_ctx* ctx, xh_permute_plan* plan);
+int xh_permute(xh_ctx* ctx, const xh_permute_plan* plan, const double* src, double* dst);
+/* bytes moved by one xh_permute (8 * (read + written elements)) */
+int xh_permute_plan_bytes(const xh_permute_plan* plan, double* bytes);
+
+/* ---- grouped GEMM ---------------------------------------------------------------------
+ * Replaces the matched-sector loop of TensorBase::operator* (src/Core/TensorBase.cpp:262-279)
+ * calling MatrixInterface::prod (src/Interfaces/MatrixInterface_Eigen_impl.cpp:155-160):
+ * C_g = alpha * op(A_g) * op(B_g) + beta * C_g for every item g in one persistent launch.
+ * Blocks are column-major; lda = m (or k when transposed), ldb = k (or n), ldc = m. */
+typedef struct {
+    int64_t a_off, b_off, c_off;
+    int32_t m, n, k;
+} xh_gemm_item;
+int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, int trans_a, int trans_b, xh_gemm_plan** out);
+int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan);
+int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
+int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops);
+
+/* ---- batched factorisations -------------------------------------------------------------
+ * Replace the per-sector loops of Tensor::full_svd_impl (src/Core/Tensor.cpp:510-519) over
+ * PlainInterface::svd (src/Interfaces/PlainInterface_Eigen_impl.cpp:116-127), Tensor::tQR
+ * (Tensor.cpp:757-780) over MatrixInterface::qr, and Tensor::teigh (Tensor.cpp:782-803) over
+ * MatrixInterface::eigh.  Block g is m x n column-major at a_off; outputs are thin:
+ * U (m x r) at u_off, S (r) at s_off, Vt (r x n) at vt_off with r = min(m, n); singular values
+ * descending.  eigh: square blocks, eigenvalues ascending, eigenvectors in columns. */
+typedef struct {
+    int64_t a_off, u_off, s_off, vt_off;
+    int32_t m, n;
+} xh_factor_item;
+int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt);
+/* QR: Q (m x r) at u_off, R (r x n) at vt_off, Householder, R with the sign convention of
+ * Eigen::HouseholderQR (diagonal may be negative). */
+int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* Q, double* R);
+int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* W, double* V);
+
+/* ---- elementwise / reductions on a whole tensor buffer or its blocks ---------------------- */
+int xh_scale(xh_ctx* ctx, double* x, size_t n, double alpha);                          /* TensorBase.hpp:197-209 */
+int xh_axpby(xh_ctx* ctx, double* y, const double* x, size_t n, double a, double b);    /* y = a*x + b*y (CoeffBinaryOp) */
+int xh_max_abs(xh_ctx* ctx, const double* x, size_t n, double* out);                   /* maxNorm, TensorBase.cpp:48-56 */
+int xh_dot(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out);       /* squaredNorm building block */
+/* Per block g (m_g x n_g at off_g): out[g] = trace of the block (TensorBase.cpp:26-46). */
+int xh_block_trace(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* x, double* out);
+/* Per row of a set of subblocks: x[rows] *= sign (twist, Tensor.cpp:357-377) expressed as a
+ * list of (offset, rows, cols, ld, factor) rectangles. */
+typedef struct {
+    int64_t off, ld;
+    int32_t rows, cols;
+    double factor;
+} xh_rect;
+int xh_scale_rects(xh_ctx* ctx, double* x, const xh_rect* rects, int64_t nrects);
+/* diagonal maps of square blocks (DiagCoeffUnaryOp, TensorBase.cpp:191-201): mode 0 = sqrt, 1 = inverse, 2 = inverse sqrt */
+int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, double* x, int mode);
+/* copy rectangles: dst block (rows x cols at dst_off, ld dst_ld) = src block (src_off, src_ld) */
+typedef struct {
+    int64_t src_off, src_ld, dst_off, dst_ld;
+    int32_t rows, cols;
+    double factor;
+    int32_t transpose; /* 1: dst = factor * src^T (src is cols x rows) */
+} xh_copy_rect;
+int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects);
+
+/* ---- timing helpers (events on the context stream) --------------------------------------- */
+int xh_event_record(xh_ctx* ctx, int slot);
+int xh_event_elapsed_ms(xh_ctx* ctx, int slot0, int slot1, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPED_HIP_H_ */

@@ -1,0 +1,123 @@
+/*
+ * xped_tensor.h -- tensor-level C ABI of the MI355X block-sparse tensor path.
+ *
+ * This is the drop-in surface for the operations of Xped::Tensor<Scalar=double, Rank, CoRank,
+ * Symmetry> (include/Xped/Core/Tensor.hpp:59-444).  Block structure (sectors, fusion trees,
+ * offsets) is built on the host in integer arithmetic exactly as the reference builds it; the
+ * data lives in GPU memory and every data operation runs as HIP kernels.  A runtime created
+ * with device < 0 is structure-only: all structural queries work, data operations return
+ * XH_ENODEV.
+ *
+ * Symmetry kinds (two components at most): 1 = U(1) (SpinU1/ChargeU1), 2 = fermionic U(1)
+ * (FChargeU1), 3 = SU(2) (SpinSU2, labels q = 2S+1); e.g. (1,0) = U1<SpinU1>,
+ * (2,1) = Combined<U1<FChargeU1>, U1<SpinU1>>, (3,0) = SU2<SpinSU2>.
+ *
+ * Status codes are those of xped_hip.h.  Handles returned through `out` parameters are owned by
+ * the caller and released with the matching *_destroy function.
+ */
+#ifndef XPED_TENSOR_H_
+#define XPED_TENSOR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "xped_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xt_rt xt_rt;
+typedef struct xt_basis xt_basis;
+typedef struct xt_tensor xt_tensor;
+
+/* ---- runtime ------------------------------------------------------------------------- */
+int xt_rt_create(int device, xt_rt** out);
+int xt_rt_destroy(xt_rt* rt);
+const char* xt_last_error(const xt_rt* rt);
+xh_ctx* xt_rt_ctx(xt_rt* rt);
+/* counters: total GEMM flops and permute bytes issued, number of launches */
+int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, long* n_gemm, long* n_permute);
+int xt_rt_clear_caches(xt_rt* rt);
+
+/* ---- bases: Qbasis<Symmetry, depth> (include/Xped/Core/Qbasis.hpp) -------------------- */
+int xt_basis_create(int kind0, int kind1, xt_basis** out);                 /* empty depth-1 basis */
+int xt_basis_destroy(xt_basis* b);
+int xt_basis_copy(const xt_basis* b, xt_basis** out);
+int xt_basis_push_back(xt_basis* b, const int* q, size_t dim);             /* Qbasis.cpp:34-73 */
+int xt_basis_conj(const xt_basis* b, xt_basis** out);                      /* Qbasis.cpp:418-439 */
+int xt_basis_combine(const xt_basis* a, const xt_basis* b, xt_basis** out); /* Qbasis.cpp:441-495 */
+int xt_basis_forget_history(const xt_basis* b, xt_basis** out);            /* Qbasis.cpp:410-416 */
+int xt_basis_sort(xt_basis* b);                                            /* Qbasis.cpp:306-331 */
+int xt_basis_sym(const xt_basis* b, int* kind0, int* kind1);
+/* depth, number of sectors, dim(), fullDim(), IS_CONJ() */
+int xt_basis_info(const xt_basis* b, int* depth, int* nsec, size_t* dim, size_t* full_dim, int* conj);
+/* sector i: q (2 ints), num (offset), inner dim, number of fusion trees */
+int xt_basis_entry(const xt_basis* b, int i, int* q, size_t* num, size_t* dim, int* ntrees);
+/* tree t of sector i.  rank; unc/inter are 2 ints per q (inter has max(rank-2,0) entries);
+ * dims (rank uint32), dual (rank ints), tree dim */
+int xt_basis_tree(const xt_basis* b, int i, int t, int* rank, int* unc, int* inter, int* coupled, uint32_t* dims, int* dual, uint64_t* dim);
+/* FusionTree::swap / permute (FusionTree.cpp:326-412) applied to tree t of sector i of a coupled
+ * basis.  Results: up to `cap` trees given as (unc, inter, coupled) in the layout above and a
+ * coefficient each; *n receives the number of results. */
+int xt_tree_permute(const xt_basis* b, int i, int t, const int* p, int np, int cap, int* n, int* unc, int* inter, int* coupled, double* coeff);
+int xt_tree_swap(const xt_basis* b, int i, int t, int pos, int cap, int* n, int* unc, int* inter, int* coupled, double* coeff);
+
+/* ---- Wigner symbols (arguments are 2j) -------------------------------------------------- */
+double xt_wigner_3j(int tj1, int tj2, int tj3, int tm1, int tm2, int tm3);
+double xt_wigner_6j(int tj1, int tj2, int tj3, int tj4, int tj5, int tj6);
+double xt_wigner_9j(int tj1, int tj2, int tj3, int tj4, int tj5, int tj6, int tj7, int tj8, int tj9);
+/* symmetry coefficients (U1.hpp, SU2.hpp, S1xS2.hpp); q arguments are 2-int arrays */
+double xt_coeff(int kind0, int kind1, int which, const int* q1, const int* q2, const int* q3, const int* q4, const int* q5, const int* q6);
+
+/* ---- tensors ------------------------------------------------------------------------- */
+/* Tensor(basis_domain, basis_codomain) followed by setZero() */
+int xt_tensor_create(xt_rt* rt, int rank, int corank, const xt_basis* const* legs, xt_tensor** out);
+int xt_tensor_destroy(xt_tensor* t);
+int xt_tensor_copy(const xt_tensor* t, xt_tensor** out); /* new handle sharing the (immutable) data */
+int xt_tensor_shape(const xt_tensor* t, int* rank, int* corank, int* nsec, size_t* size);
+int xt_tensor_sector(const xt_tensor* t, int i, int* q, size_t* offset, size_t* rows, size_t* cols);
+int xt_tensor_leg(const xt_tensor* t, int leg, xt_basis** out);       /* uncoupled leg basis */
+int xt_tensor_coupled(const xt_tensor* t, int side, xt_basis** out);  /* side 0: coupledDomain, 1: coupledCodomain */
+int xt_tensor_upload(xt_tensor* t, const double* host, size_t n);
+int xt_tensor_download(const xt_tensor* t, double* host, size_t n);
+const double* xt_tensor_device_ptr(const xt_tensor* t);
+/* Tensor::setRandom(engine) with std::mt19937(seed): dist 0 uniform[-1,1), 1 normal(0,1) */
+int xt_tensor_set_random(xt_tensor* t, uint32_t seed, int dist);
+
+/* ---- operations ------------------------------------------------------------------------ */
+int xt_permute(const xt_tensor* t, int shift, const int* p, int np, xt_tensor** out);   /* Tensor.cpp:335-355 */
+/* structure-level description of the permute plan: number of output subblocks, sources, and the
+ * element count moved; works on a structure-only runtime */
+int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, int64_t* ntasks, int64_t* nsources, int64_t* elems);
+/* dump the plan: per task (dst_off, rows, cols, nsrc) and per source (src_off, coeff) */
+int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, int64_t* task_dst, int64_t* task_rows, int64_t* task_cols, int32_t* task_nsrc,
+                         int64_t* src_off, double* src_coeff);
+int xt_mul(const xt_tensor* l, const xt_tensor* r, int precision, xt_tensor** out);   /* TensorBase.cpp:237-288 */
+int xt_contract(const xt_tensor* a, const int* a1, int n1, const xt_tensor* b, const int* a2, int n2, int rankres, int precision,
+                xt_tensor** out); /* Tensor.hpp:328-351 */
+int xt_get_permutations(const int* a1, int n1, int rank1, const int* a2, int n2, int rank2, int rankres, int* p1, int* shift1, int* p2, int* shift2,
+                        int* pres, int* shiftres); /* Constfct.hpp:55-109 */
+/* Tensor::tSVD (Tensor.cpp:478-633).  *trunc_weight and *entropy are accumulated into (+=). */
+int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_multiplets, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
+            double* trunc_weight, double* entropy);
+int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R);   /* Tensor.cpp:757-780 */
+int xt_teigh(const xt_tensor* t, xt_tensor** D, xt_tensor** V); /* Tensor.cpp:782-803 */
+int xt_adjoint(const xt_tensor* t, xt_tensor** out);            /* AdjointOp.hpp:25-69 */
+int xt_twist(const xt_tensor* t, int leg, xt_tensor** out);     /* Tensor.cpp:357-377 */
+int xt_scale(const xt_tensor* t, double alpha, xt_tensor** out);
+int xt_axpby(const xt_tensor* x, const xt_tensor* y, double a, double b, xt_tensor** out);
+int xt_diag_map(const xt_tensor* t, int mode, xt_tensor** out); /* diag_sqrt (0), diag_inv (1), inverse sqrt (2) */
+int xt_trace(const xt_tensor* t, double* out);                  /* TensorBase.cpp:26-37 */
+int xt_squared_norm(const xt_tensor* t, double* out);           /* TensorBase.cpp:58-64 */
+int xt_max_norm(const xt_tensor* t, double* out);               /* TensorBase.cpp:48-56 */
+/* the host truncation rule of cutoff_matrices on explicit (q, sigma) pairs; returns per kept
+ * sector the q and the kept count (ascending q) */
+int xt_truncation_rule(int kind0, int kind1, const int* qs, const double* svs, int64_t n, size_t max_keep, double eps_svd, int preserve_multiplets,
+                       int* nsec, int* kept_q, int* kept_n, double* trunc_weight, double* entropy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPED_TENSOR_H_ */

@@ -1,0 +1,239 @@
+"""HIP path vs the CPU oracle through the C ABI (needs an MI355X).
+
+Tolerances: permute of abelian tensors is pure data movement (negation for fermionic signs) and
+must be bit-exact; SU(2) permutes use recoupling coefficients and agree to 1e-13 absolute on
+O(1) data; FP64 GEMM-based results agree to 1e-12 relative to the largest entry; FP32 mode to
+1e-5 and BF16x3 mode to 5e-5 relative (stated per test).
+"""
+import ctypes as C
+import itertools
+import math
+import random
+
+import numpy as np
+import pytest
+
+import xped_amd.tensor as xt
+from oracle import tensor as oten
+from oracle.structure import inverse
+from xped_amd import _lib, configs
+from xped_amd._lib import lib
+
+from .helpers import both_basis, both_tensor, oracle_structure, rand_entries, structure_of
+
+pytestmark = pytest.mark.gpu
+
+PCODES = [xt.U1, xt.FU1xU1, xt.FU1, xt.SU2]
+IDS = ["U1", "FU1xU1", "FU1", "SU2"]
+
+
+def test_set_random_matches_reference_generator(rt):
+    r = random.Random(1)
+    for dist in (0, 1):
+        legs = [rand_entries(xt.U1, r, 4, 5) for _ in range(3)]
+        o, t = both_tensor(rt, xt.U1, legs, 2, 77, dist)
+        t.setRandom(77, dist)
+        assert np.array_equal(t.download(), o.data)
+
+
+@pytest.mark.parametrize("pcode", PCODES, ids=IDS)
+def test_permute_all_permutations(rt, pcode):
+    r = random.Random(2)
+    for rank, corank in [(2, 2), (2, 1), (1, 2), (3, 1)]:
+        legs = [rand_entries(pcode, r, 3, 4) for _ in range(rank + corank)]
+        o, t = both_tensor(rt, pcode, legs, rank, 10 + rank)
+        for p in itertools.permutations(range(rank + corank)):
+            for shift in range(-corank, rank + 1):
+                op = oten.permute(o, shift, p)
+                tp = t.permute(shift, p)
+                assert structure_of(tp) == oracle_structure(op)
+                got = tp.download()
+                if pcode == xt.SU2:
+                    np.testing.assert_allclose(got, op.data, rtol=0, atol=1e-13)
+                else:
+                    assert np.array_equal(got, op.data), (p, shift)
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.FU1xU1], ids=["U1", "FU1xU1"])
+def test_permute_large_subblocks_transpose_path(rt, pcode):
+    # degeneracies >= 16 route subblocks through the LDS transpose kernel
+    r = random.Random(3)
+    legs = [rand_entries(pcode, r, 3, 70, mindim=16) for _ in range(4)]
+    o, t = both_tensor(rt, pcode, legs, 2, 5)
+    for p, shift in [((1, 0, 2, 3), 0), ((2, 3, 0, 1), 0), ((3, 1, 2, 0), 1), ((1, 2, 3, 0), -1), ((0, 3, 2, 1), 2)]:
+        op = oten.permute(o, shift, p)
+        tp = t.permute(shift, p)
+        assert structure_of(tp) == oracle_structure(op)
+        assert np.array_equal(tp.download(), op.data), (p, shift)
+
+
+def test_permute_round_trip_su2(rt):
+    r = random.Random(4)
+    legs = [rand_entries(xt.SU2, r, 3, 5) for _ in range(4)]
+    o, t = both_tensor(rt, xt.SU2, legs, 2, 6)
+    d0 = t.download()
+    for p in itertools.permutations(range(4)):
+        for shift in (-2, -1, 0, 1, 2):
+            back = t.permute(shift, p).permute(-shift, inverse(p))
+            np.testing.assert_allclose(back.download(), d0, rtol=0, atol=1e-13)
+
+
+def _gemm_ref(A, B, items, ta, tb):
+    C = {}
+    for it in items:
+        m, n, k = it["m"], it["n"], it["k"]
+        a = A[it["a"]:it["a"] + m * k].reshape((m, k) if ta else (k, m)).T if True else None
+        a = A[it["a"]:it["a"] + m * k].reshape(m, k) if ta else A[it["a"]:it["a"] + m * k].reshape(k, m).T
+        b = B[it["b"]:it["b"] + k * n].reshape(k, n) if tb else B[it["b"]:it["b"] + k * n].reshape(n, k).T
+        C[it["c"]] = (a @ b, m, n)
+    return C
+
+
+@pytest.mark.parametrize("precision,tol", [(_lib.XH_FP64, 1e-12), (_lib.XH_FP32, 1e-5), (_lib.XH_BF16X3, 5e-5)], ids=["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
+    L = lib()
+    ctx = rt.ctx()
+    r = np.random.default_rng(5)
+    shapes = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (65, 63, 17), (130, 7, 200), (8, 300, 33), (200, 150, 129), (3, 3, 0)]
+    items, ao, bo, co = [], 0, 0, 0
+    for m, n, k in shapes:
+        items.append({"a": ao, "b": bo, "c": co, "m": m, "n": n, "k": k})
+        ao += m * k
+        bo += k * n
+        co += m * n
+    A = r.uniform(-1, 1, ao)
+    B = r.uniform(-1, 1, bo)
+    dA, dB, dC = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, n in ((dA, ao), (dB, bo), (dC, co)):
+        assert L.xh_alloc(ctx, 8 * max(n, 1), C.byref(d)) == 0
+    L.xh_upload(ctx, dA, A.ctypes.data, 8 * ao)
+    L.xh_upload(ctx, dB, B.ctypes.data, 8 * bo)
+    L.xh_fill(ctx, dC, co, 0.0)
+    arr = (_lib.xh_gemm_item * len(items))(*[_lib.xh_gemm_item(it["a"], it["b"], it["c"], it["m"], it["n"], it["k"]) for it in items])
+    plan = C.c_void_p()
+    assert L.xh_gemm_plan_create(ctx, arr, len(items), ta, tb, C.byref(plan)) == 0
+    assert L.xh_gemm(ctx, plan, dA, dB, dC, 1.0, 0.0, precision) == 0
+    out = np.empty(co)
+    L.xh_download(ctx, out.ctypes.data, dC, 8 * co)
+    ref = _gemm_ref(A, B, items, ta, tb)
+    for it in items:
+        c, m, n = ref[it["c"]]
+        got = out[it["c"]:it["c"] + m * n].reshape(n, m).T
+        scale = max(1.0, np.abs(c).max() if c.size else 1.0) * math.sqrt(max(it["k"], 1))
+        assert np.abs(got - c).max(initial=0.0) <= tol * scale, (it, np.abs(got - c).max())
+    # alpha/beta accumulate
+    assert L.xh_gemm(ctx, plan, dA, dB, dC, 2.0, -1.0, precision) == 0
+    out2 = np.empty(co)
+    L.xh_download(ctx, out2.ctypes.data, dC, 8 * co)
+    for it in items:
+        c, m, n = ref[it["c"]]
+        got = out2[it["c"]:it["c"] + m * n].reshape(n, m).T
+        scale = max(1.0, np.abs(c).max() if c.size else 1.0) * math.sqrt(max(it["k"], 1))
+        assert np.abs(got - c).max(initial=0.0) <= 4 * tol * scale
+    L.xh_gemm_plan_destroy(ctx, plan)
+    for d in (dA, dB, dC):
+        L.xh_free(ctx, d)
+
+
+@pytest.mark.parametrize("pcode", PCODES, ids=IDS)
+def test_mul_and_contract(rt, pcode):
+    r = random.Random(6)
+    legs = [rand_entries(pcode, r, 3, 5) for _ in range(3)]
+    legs2 = [rand_entries(pcode, r, 3, 5) for _ in range(2)]
+    oA, A = both_tensor(rt, pcode, legs, 2, 1)
+    oB, B = both_tensor(rt, pcode, [legs[2]] + legs2, 1, 2)
+    oC = oten.mul(oA, oB)
+    Cm = A * B
+    assert structure_of(Cm) == oracle_structure(oC)
+    np.testing.assert_allclose(Cm.download(), oC.data, rtol=0, atol=1e-12 * max(1, np.abs(oC.data).max()))
+    # contract over the shared leg with a reordered result
+    oD = oten.contract(oA, [-2, -1, 1], oB, [1, -3, -4], 2)
+    D = A.contract(B, [-2, -1, 1], [1, -3, -4], 2)
+    assert structure_of(D) == oracle_structure(oD)
+    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max()))
+
+
+def _c1_pair(rt):
+    v = configs.c1_virtual()
+    ph = configs.c1_physical()
+    oA, A = both_tensor(rt, xt.U1, [v, ph, v], 2, 11)
+    oB, B = both_tensor(rt, xt.U1, [v, ph, v], 2, 12)
+    return oA, A, oB, B
+
+
+def test_c1_two_site_contraction_and_truncation(rt):
+    # configuration 1: contract over r, permute to (l,p|p',r'), tSVD truncated to chi = 64
+    oA, A, oB, B = _c1_pair(rt)
+    oC = oten.contract(oA, [-1, -2, 1], oB, [1, -3, -4], 2)
+    Cm = A.contract(B, [-1, -2, 1], [1, -3, -4], 2)
+    assert structure_of(Cm) == oracle_structure(oC)
+    assert [s[2] for s in Cm.sectors()] == [1, 2, 8, 16, 25, 28, 25, 16, 8, 2, 1]
+    np.testing.assert_allclose(Cm.download(), oC.data, rtol=0, atol=1e-12 * np.abs(oC.data).max())
+    U, S, V, tw, ent = Cm.tSVD(64, 0.0, PRESERVE_MULTIPLETS=False)
+    oU, oS, oV, otw, oent, kept = oten.tsvd(oC, 64, 0.0, False)
+    assert [s[0] for s in S.sectors()] == list(kept.keys())
+    assert [s[2] for s in S.sectors()] == [len(v) for v in kept.values()]
+    smax = oS.data.max()
+    np.testing.assert_allclose(S.download(), oS.data, rtol=0, atol=1e-12 * smax)
+    assert abs(tw - otw) <= 1e-12 * smax ** 2
+    assert abs(ent - oent) <= 1e-11 * max(1, abs(oent))
+    # gauge-invariant check of the kept subspace: U S Vdag equals the oracle's
+    rec = (U * S) * V
+    orec = oten.mul(oten.mul(oU, oS), oV)
+    np.testing.assert_allclose(rec.download(), orec.data, rtol=0, atol=1e-12 * smax)
+    # orthonormal columns of U
+    UtU = U.adjoint() * U
+    ident = oten.mul(oten.adjoint(oU), oU)
+    np.testing.assert_allclose(UtU.download(), ident.data, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.SU2, xt.FU1xU1], ids=["U1", "SU2", "FU1xU1"])
+def test_full_svd_invariants(rt, pcode):
+    r = random.Random(8)
+    legs = [rand_entries(pcode, r, 4, 9) for _ in range(4)]
+    o, t = both_tensor(rt, pcode, legs, 2, 3)
+    U, S, V, tw, ent = t.tSVD(10 ** 6, 0.0, PRESERVE_MULTIPLETS=False)
+    assert tw == 0.0
+    rec = (U * S) * V
+    np.testing.assert_allclose(rec.download(), o.data, rtol=0, atol=1e-12 * max(1, np.abs(o.data).max()) * 10)
+    _, oS, _, _, _, kept = oten.tsvd(o, 10 ** 6, 0.0, False)
+    np.testing.assert_allclose(np.sort(S.download()), np.sort(oS.data), rtol=0, atol=1e-12 * oS.data.max())
+
+
+def test_tqr_and_teigh(rt):
+    r = random.Random(9)
+    legs = [rand_entries(xt.U1, r, 3, 6) for _ in range(3)]
+    # tall domain so that the middle basis is the codomain
+    o, t = both_tensor(rt, xt.U1, [legs[0], legs[1], legs[2]], 2, 4)
+    Q, R = t.tQR()
+    oQ, oR = oten.tqr(o)
+    assert structure_of(Q) == oracle_structure(oQ) and structure_of(R) == oracle_structure(oR)
+    np.testing.assert_allclose(Q.download(), oQ.data, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(R.download(), oR.data, rtol=0, atol=1e-12 * max(1, np.abs(oR.data).max()))
+    # symmetric matrix for eigh: T^dagger T
+    H = t.adjoint() * t
+    oH = oten.mul(oten.adjoint(o), o)
+    D, V = H.teigh()
+    oD, oV = oten.teigh(oH)
+    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max()))
+    rec = (V * D) * V.adjoint()
+    np.testing.assert_allclose(rec.download(), oH.data, rtol=0, atol=1e-11 * max(1, np.abs(oH.data).max()))
+
+
+@pytest.mark.parametrize("pcode", PCODES, ids=IDS)
+def test_elementwise_ops(rt, pcode):
+    r = random.Random(10)
+    legs = [rand_entries(pcode, r, 3, 4) for _ in range(4)]
+    o, t = both_tensor(rt, pcode, legs, 2, 8)
+    for leg in range(4):
+        assert np.array_equal(t.twist(leg).download(), oten.twist(o, leg).data)
+    ad = t.adjoint()
+    oad = oten.adjoint(o)
+    assert structure_of(ad) == oracle_structure(oad)
+    assert np.array_equal(ad.download(), oad.data)
+    assert abs(t.squaredNorm() - oten.squared_norm(o)) <= 1e-12 * oten.squared_norm(o)
+    assert t.maxNorm() == oten.max_norm(o)
+    sq = t * t.adjoint()
+    assert abs(sq.trace() - oten.trace(oten.mul(o, oad))) <= 1e-12 * abs(oten.trace(oten.mul(o, oad)))
+    assert np.array_equal(t.scale(-2.5).download(), -2.5 * o.data)

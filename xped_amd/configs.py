@@ -1,0 +1,57 @@
+"""Synthetic workloads of the benchmark and the parity tests (leg bases and seeds).
+
+Each builder returns plain (q, degeneracy) lists so that the product tensors and the oracle's
+tensors can be made from identical legs.
+
+C1  U(1) MPS two-site: A(l,p|r) B(r,p'|r'), virtual charges -4..4 with
+    deg = max(1, round(64 exp(-q^2/4) / Z)) (sums to 66 as written), physical {-1:1, +1:1}.
+C3  U(1)xU(1) fermionic grouped contraction: three virtual legs of total dimension 512 over
+    sectors (N, 2Sz), |N|, |2Sz| <= 6, N = 2Sz mod 2, degeneracies from a seeded power law.
+C4  U(1) iPEPS: D = 8 over charges -3..3 ({+-3:1, +-2:1, +-1:1, 0:2}), physical spin-1/2.
+"""
+from __future__ import annotations
+
+import math
+import random
+from typing import List, Tuple
+
+Entries = List[Tuple[tuple, int]]
+
+
+def c1_virtual(chi: int = 64) -> Entries:
+    qs = list(range(-4, 5))
+    w = [math.exp(-q * q / 4.0) for q in qs]
+    Z = sum(w)
+    return [((q,), max(1, int(round(chi * x / Z)))) for q, x in zip(qs, w)]
+
+
+def c1_physical() -> Entries:
+    return [((-1,), 1), ((1,), 1)]
+
+
+def c3_leg(seed: int, total: int = 512, exponent: float = 1.5) -> Entries:
+    r = random.Random(seed)
+    sectors = [(n, sz) for n in range(-6, 7) for sz in range(-6, 7) if (n - sz) % 2 == 0]
+    w = [(1.0 + r.random() * 9.0) ** (-exponent) for _ in sectors]
+    Z = sum(w)
+    degs = [max(1, int(round(total * x / Z))) for x in w]
+    # renormalise to the exact total by adjusting the largest sectors
+    diff = total - sum(degs)
+    order = sorted(range(len(degs)), key=lambda i: -degs[i])
+    k = 0
+    while diff != 0:
+        i = order[k % len(order)]
+        step = 1 if diff > 0 else -1
+        if degs[i] + step >= 1:
+            degs[i] += step
+            diff -= step
+        k += 1
+    return [((n, sz), d) for (n, sz), d in zip(sectors, degs)]
+
+
+def c4_virtual() -> Entries:
+    return [((-3,), 1), ((-2,), 1), ((-1,), 1), ((0,), 2), ((1,), 1), ((2,), 1), ((3,), 1)]
+
+
+def c4_physical() -> Entries:
+    return [((-1,), 1), ((1,), 1)]

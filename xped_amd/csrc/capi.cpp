@@ -1,0 +1,535 @@
+// C ABI of the tensor layer (include/xped_tensor.h) over the C++ structure engine and device layer.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/xped_tensor.h"
+#include "core/tensor.hpp"
+
+struct xt_rt {
+    xa::Runtime rt;
+    std::string err;
+};
+struct xt_basis {
+    xa::Qbasis b;
+};
+struct xt_tensor {
+    xa::DTensor t;
+    xt_rt* owner = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int run(xt_rt* rt, F&& f)
+{
+    try {
+        f();
+        return XH_OK;
+    } catch(const xa::Error& e) {
+        g_err = e.what();
+        if(rt) rt->err = g_err;
+        return e.code;
+    } catch(const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        if(rt) rt->err = g_err;
+        return XH_ENOMEM;
+    } catch(const std::exception& e) {
+        g_err = e.what();
+        if(rt) rt->err = g_err;
+        return XH_EINVAL;
+    }
+}
+
+xa::Q q_from(const int* q)
+{
+    xa::Q out;
+    if(q) {
+        out.v[0] = q[0];
+        out.v[1] = q[1];
+    }
+    return out;
+}
+void q_to(const xa::Q& q, int* out)
+{
+    out[0] = q.v[0];
+    out[1] = q.v[1];
+}
+
+xt_tensor* wrap(xt_rt* owner, xa::DTensor t)
+{
+    auto* h = new xt_tensor();
+    h->t = std::move(t);
+    h->owner = owner;
+    return h;
+}
+
+void dump_tree(const xa::FTree& t, int* unc, int* inter, int* coupled)
+{
+    if(unc)
+        for(int i = 0; i < t.rank; ++i) q_to(t.unc[i], unc + 2 * i);
+    if(inter)
+        for(int i = 0; i < t.n_inter(); ++i) q_to(t.inter[i], inter + 2 * i);
+    if(coupled) q_to(t.coupled, coupled);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xt_rt_create(int device, xt_rt** out)
+{
+    *out = nullptr;
+    auto* r = new xt_rt();
+    if(device >= 0) {
+        xh_ctx* ctx = nullptr;
+        int rc = xh_create(device, &ctx);
+        if(rc != XH_OK) {
+            g_err = "no GPU available for the tensor runtime";
+            delete r;
+            return rc;
+        }
+        r->rt.holder = std::make_shared<xa::CtxHolder>();
+        r->rt.holder->ctx = ctx;
+        r->rt.ctx = ctx;
+    }
+    *out = r;
+    return XH_OK;
+}
+
+int xt_rt_destroy(xt_rt* rt)
+{
+    if(!rt) return XH_OK;
+    if(rt->rt.ctx) xh_synchronize(rt->rt.ctx);
+    // the runtime destructor releases the cached plans; the context itself lives on until the
+    // last tensor buffer that uses it is released
+    delete rt;
+    return XH_OK;
+}
+
+const char* xt_last_error(const xt_rt* rt) { return rt ? rt->err.c_str() : g_err.c_str(); }
+
+xh_ctx* xt_rt_ctx(xt_rt* rt) { return rt ? rt->rt.ctx : nullptr; }
+
+int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, long* n_gemm, long* n_permute)
+{
+    if(!rt) return XH_EINVAL;
+    if(gemm_flops) *gemm_flops = rt->rt.gemm_flops;
+    if(permute_bytes) *permute_bytes = rt->rt.permute_bytes;
+    if(n_gemm) *n_gemm = rt->rt.n_gemm;
+    if(n_permute) *n_permute = rt->rt.n_permute;
+    return XH_OK;
+}
+
+int xt_rt_clear_caches(xt_rt* rt)
+{
+    return run(rt, [&] {
+        if(rt->rt.ctx) xh_synchronize(rt->rt.ctx);
+        for(auto& [k, v] : rt->rt.permute_cache)
+            if(v->dev) xh_permute_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
+        for(auto& [k, v] : rt->rt.gemm_cache)
+            if(v->dev) xh_gemm_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
+        rt->rt.permute_cache.clear();
+        rt->rt.gemm_cache.clear();
+        rt->rt.struct_cache.clear();
+    });
+}
+
+// ---------------------------------------------------------------- bases
+
+int xt_basis_create(int kind0, int kind1, xt_basis** out)
+{
+    return run(nullptr, [&] {
+        auto* h = new xt_basis();
+        h->b = xa::Qbasis(xa::Sym::make(kind0, kind1), 1);
+        *out = h;
+    });
+}
+
+int xt_basis_destroy(xt_basis* b)
+{
+    delete b;
+    return XH_OK;
+}
+
+int xt_basis_copy(const xt_basis* b, xt_basis** out)
+{
+    return run(nullptr, [&] { *out = new xt_basis(*b); });
+}
+
+int xt_basis_push_back(xt_basis* b, const int* q, size_t dim)
+{
+    return run(nullptr, [&] {
+        if(b->b.depth != 1) throw xa::Error(XH_EINVAL, "push_back needs a depth-1 basis");
+        b->b.push_back(q_from(q), dim);
+    });
+}
+
+int xt_basis_conj(const xt_basis* b, xt_basis** out)
+{
+    return run(nullptr, [&] {
+        if(b->b.depth != 1) throw xa::Error(XH_EINVAL, "conj needs a depth-1 basis");
+        *out = new xt_basis{b->b.conj()};
+    });
+}
+
+int xt_basis_combine(const xt_basis* a, const xt_basis* b, xt_basis** out)
+{
+    return run(nullptr, [&] {
+        if(b->b.depth != 1) throw xa::Error(XH_EINVAL, "combine: the second basis must have depth 1");
+        if(!(a->b.sym == b->b.sym)) throw xa::Error(XH_EINVAL, "combine: symmetry mismatch");
+        *out = new xt_basis{a->b.combine(b->b)};
+    });
+}
+
+int xt_basis_forget_history(const xt_basis* b, xt_basis** out)
+{
+    return run(nullptr, [&] { *out = new xt_basis{b->b.forget_history()}; });
+}
+
+int xt_basis_sort(xt_basis* b)
+{
+    return run(nullptr, [&] { b->b.sort(); });
+}
+
+int xt_basis_sym(const xt_basis* b, int* kind0, int* kind1)
+{
+    if(!b) return XH_EINVAL;
+    *kind0 = b->b.sym.kind[0];
+    *kind1 = b->b.sym.kind[1];
+    return XH_OK;
+}
+
+int xt_basis_info(const xt_basis* b, int* depth, int* nsec, size_t* dim, size_t* full_dim, int* conj)
+{
+    if(!b) return XH_EINVAL;
+    if(depth) *depth = b->b.depth;
+    if(nsec) *nsec = static_cast<int>(b->b.data.size());
+    if(dim) *dim = b->b.dim();
+    if(full_dim) *full_dim = b->b.full_dim();
+    if(conj) *conj = b->b.conj_flag ? 1 : 0;
+    return XH_OK;
+}
+
+int xt_basis_entry(const xt_basis* b, int i, int* q, size_t* num, size_t* dim, int* ntrees)
+{
+    if(!b || i < 0 || i >= static_cast<int>(b->b.data.size())) return XH_EINVAL;
+    const auto& e = b->b.data[i];
+    if(q) q_to(e.q, q);
+    if(num) *num = e.num;
+    if(dim) *dim = e.dim;
+    if(ntrees) *ntrees = static_cast<int>(e.trees.size());
+    return XH_OK;
+}
+
+int xt_basis_tree(const xt_basis* b, int i, int t, int* rank, int* unc, int* inter, int* coupled, uint32_t* dims, int* dual, uint64_t* dim)
+{
+    if(!b || i < 0 || i >= static_cast<int>(b->b.data.size())) return XH_EINVAL;
+    const auto& e = b->b.data[i];
+    if(t < 0 || t >= static_cast<int>(e.trees.size())) return XH_EINVAL;
+    const xa::FTree& tr = e.trees[t];
+    if(rank) *rank = tr.rank;
+    dump_tree(tr, unc, inter, coupled);
+    for(int k = 0; k < tr.rank; ++k) {
+        if(dims) dims[k] = tr.dims[k];
+        if(dual) dual[k] = tr.dual[k] ? 1 : 0;
+    }
+    if(dim) *dim = tr.dim;
+    return XH_OK;
+}
+
+static int tree_results(const xa::TreeMap& m, int cap, int* n, int* unc, int* inter, int* coupled, double* coeff)
+{
+    *n = static_cast<int>(m.size());
+    if(static_cast<int>(m.size()) > cap) return XH_EINVAL;
+    int k = 0;
+    for(const auto& [tr, c] : m) {
+        dump_tree(tr, unc ? unc + 2 * xa::MAXR * k : nullptr, inter ? inter + 2 * xa::MAXR * k : nullptr, coupled ? coupled + 2 * k : nullptr);
+        if(coeff) coeff[k] = c;
+        ++k;
+    }
+    return XH_OK;
+}
+
+int xt_tree_permute(const xt_basis* b, int i, int t, const int* p, int np, int cap, int* n, int* unc, int* inter, int* coupled, double* coeff)
+{
+    int rc = XH_OK;
+    int r = run(nullptr, [&] {
+        const xa::FTree& tr = b->b.data.at(i).trees.at(t);
+        if(np != tr.rank) throw xa::Error(XH_EINVAL, "tree permute: permutation length differs from the tree rank");
+        xa::Permutation P(std::vector<int>(p, p + np));
+        rc = tree_results(xa::tree_permute(b->b.sym, tr, P), cap, n, unc, inter, coupled, coeff);
+    });
+    return r != XH_OK ? r : rc;
+}
+
+int xt_tree_swap(const xt_basis* b, int i, int t, int pos, int cap, int* n, int* unc, int* inter, int* coupled, double* coeff)
+{
+    int rc = XH_OK;
+    int r = run(nullptr, [&] {
+        const xa::FTree& tr = b->b.data.at(i).trees.at(t);
+        if(pos < 0 || pos >= tr.rank - 1) throw xa::Error(XH_EINVAL, "tree swap: invalid position");
+        rc = tree_results(xa::tree_swap(b->b.sym, tr, pos), cap, n, unc, inter, coupled, coeff);
+    });
+    return r != XH_OK ? r : rc;
+}
+
+double xt_wigner_3j(int a, int b, int c, int d, int e, int f) { return xa::wigner_3j(a, b, c, d, e, f); }
+double xt_wigner_6j(int a, int b, int c, int d, int e, int f) { return xa::wigner_6j(a, b, c, d, e, f); }
+double xt_wigner_9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) { return xa::wigner_9j(a, b, c, d, e, f, g, h, i); }
+
+double xt_coeff(int kind0, int kind1, int which, const int* q1, const int* q2, const int* q3, const int* q4, const int* q5, const int* q6)
+{
+    xa::Sym s = xa::Sym::make(kind0, kind1);
+    switch(which) {
+    case 0: return s.coeff_swap(q_from(q1), q_from(q2), q_from(q3));
+    case 1: return s.coeff_recouple(q_from(q1), q_from(q2), q_from(q3), q_from(q4), q_from(q5), q_from(q6));
+    case 2: return s.coeff_turn(q_from(q1), q_from(q2), q_from(q3));
+    case 3: return s.coeff_FS(q_from(q1));
+    case 4: return s.coeff_twist(q_from(q1));
+    case 5: return s.triangle(q_from(q1), q_from(q2), q_from(q3)) ? 1.0 : 0.0;
+    case 6: return static_cast<double>(s.degeneracy(q_from(q1)));
+    default: return 0.0;
+    }
+}
+
+// ---------------------------------------------------------------- tensors
+
+int xt_tensor_create(xt_rt* rt, int rank, int corank, const xt_basis* const* legs, xt_tensor** out)
+{
+    return run(rt, [&] {
+        if(rank < 0 || corank < 0 || rank > xa::MAXR || corank > xa::MAXR) throw xa::Error(XH_EINVAL, "invalid rank");
+        std::vector<xa::Qbasis> d, c;
+        xa::Sym s;
+        bool have = false;
+        for(int i = 0; i < rank + corank; ++i) {
+            if(legs[i]->b.depth != 1) throw xa::Error(XH_EINVAL, "tensor legs must be depth-1 bases");
+            if(have && !(legs[i]->b.sym == s)) throw xa::Error(XH_EINVAL, "legs with different symmetries");
+            s = legs[i]->b.sym;
+            have = true;
+            (i < rank ? d : c).push_back(legs[i]->b);
+        }
+        if(!have) throw xa::Error(XH_EINVAL, "a tensor needs at least one leg");
+        xa::StructPtr st = xa::cached_structure(&rt->rt, s, d, c);
+        *out = wrap(rt, xa::DTensor::zeros(&rt->rt, st));
+    });
+}
+
+int xt_tensor_destroy(xt_tensor* t)
+{
+    delete t;
+    return XH_OK;
+}
+
+int xt_tensor_copy(const xt_tensor* t, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, t->t); });
+}
+
+int xt_tensor_shape(const xt_tensor* t, int* rank, int* corank, int* nsec, size_t* size)
+{
+    if(!t) return XH_EINVAL;
+    if(rank) *rank = t->t.st->rank;
+    if(corank) *corank = t->t.st->corank;
+    if(nsec) *nsec = static_cast<int>(t->t.st->sector.size());
+    if(size) *size = t->t.st->size;
+    return XH_OK;
+}
+
+int xt_tensor_sector(const xt_tensor* t, int i, int* q, size_t* offset, size_t* rows, size_t* cols)
+{
+    if(!t || i < 0 || i >= static_cast<int>(t->t.st->sector.size())) return XH_EINVAL;
+    if(q) q_to(t->t.st->sector[i], q);
+    if(offset) *offset = t->t.st->offsets[i];
+    if(rows) *rows = t->t.st->rows[i];
+    if(cols) *cols = t->t.st->cols[i];
+    return XH_OK;
+}
+
+int xt_tensor_leg(const xt_tensor* t, int leg, xt_basis** out)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        const auto& s = *t->t.st;
+        if(leg < 0 || leg >= s.rank + s.corank) throw xa::Error(XH_EINVAL, "leg out of range");
+        *out = new xt_basis{leg < s.rank ? s.unc_dom[leg] : s.unc_cod[leg - s.rank]};
+    });
+}
+
+int xt_tensor_coupled(const xt_tensor* t, int side, xt_basis** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = new xt_basis{side == 0 ? t->t.st->dom : t->t.st->cod}; });
+}
+
+int xt_tensor_upload(xt_tensor* t, const double* host, size_t n)
+{
+    return run(t ? t->owner : nullptr, [&] { t->t.upload(host, n); });
+}
+
+int xt_tensor_download(const xt_tensor* t, double* host, size_t n)
+{
+    return run(t ? t->owner : nullptr, [&] { t->t.download(host, n); });
+}
+
+const double* xt_tensor_device_ptr(const xt_tensor* t) { return t ? t->t.data() : nullptr; }
+
+int xt_tensor_set_random(xt_tensor* t, uint32_t seed, int dist)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        std::mt19937 engine(seed);
+        t->t.set_random(engine, dist);
+    });
+}
+
+int xt_permute(const xt_tensor* t, int shift, const int* p, int np, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::permute(t->t, shift, std::vector<int>(p, p + np))); });
+}
+
+int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, int64_t* ntasks, int64_t* nsources, int64_t* elems)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        xa::PermutePlan plan = xa::make_permute_plan(*t->t.st, shift, std::vector<int>(p, p + np));
+        if(ntasks) *ntasks = static_cast<int64_t>(plan.tasks.size());
+        if(nsources) *nsources = static_cast<int64_t>(plan.sources.size());
+        if(elems) *elems = static_cast<int64_t>(plan.moved_elems);
+    });
+}
+
+int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, int64_t* task_dst, int64_t* task_rows, int64_t* task_cols, int32_t* task_nsrc,
+                         int64_t* src_off, double* src_coeff)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        xa::PermutePlan plan = xa::make_permute_plan(*t->t.st, shift, std::vector<int>(p, p + np));
+        for(std::size_t i = 0; i < plan.tasks.size(); ++i) {
+            task_dst[i] = plan.tasks[i].dst_off;
+            task_rows[i] = plan.tasks[i].rows;
+            task_cols[i] = plan.tasks[i].cols;
+            task_nsrc[i] = plan.tasks[i].nsrc;
+        }
+        for(std::size_t i = 0; i < plan.sources.size(); ++i) {
+            src_off[i] = plan.sources[i].src_off;
+            src_coeff[i] = plan.sources[i].coeff;
+        }
+    });
+}
+
+int xt_mul(const xt_tensor* l, const xt_tensor* r, int precision, xt_tensor** out)
+{
+    return run(l ? l->owner : nullptr, [&] { *out = wrap(l->owner, xa::mul(l->t, r->t, precision)); });
+}
+
+int xt_contract(const xt_tensor* a, const int* a1, int n1, const xt_tensor* b, const int* a2, int n2, int rankres, int precision, xt_tensor** out)
+{
+    return run(a ? a->owner : nullptr, [&] {
+        *out = wrap(a->owner, xa::contract(a->t, std::vector<int>(a1, a1 + n1), b->t, std::vector<int>(a2, a2 + n2), rankres, precision));
+    });
+}
+
+int xt_get_permutations(const int* a1, int n1, int rank1, const int* a2, int n2, int rank2, int rankres, int* p1, int* shift1, int* p2, int* shift2,
+                        int* pres, int* shiftres)
+{
+    return run(nullptr, [&] {
+        xa::ContractPerms cp = xa::get_permutations(std::vector<int>(a1, a1 + n1), rank1, std::vector<int>(a2, a2 + n2), rank2, rankres);
+        std::copy(cp.p1.begin(), cp.p1.end(), p1);
+        std::copy(cp.p2.begin(), cp.p2.end(), p2);
+        std::copy(cp.pres.begin(), cp.pres.end(), pres);
+        *shift1 = cp.shift1;
+        *shift2 = cp.shift2;
+        *shiftres = cp.shiftres;
+    });
+}
+
+int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_multiplets, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
+            double* trunc_weight, double* entropy)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        xa::SVDResult r = xa::tsvd(t->t, max_keep, eps_svd, preserve_multiplets != 0, trunc_weight ? *trunc_weight : 0.0, entropy ? *entropy : 0.0);
+        *U = wrap(t->owner, r.U);
+        *S = wrap(t->owner, r.S);
+        *Vdag = wrap(t->owner, r.Vdag);
+        if(trunc_weight) *trunc_weight = r.trunc_weight;
+        if(entropy) *entropy = r.entropy;
+    });
+}
+
+int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        auto [q, r] = xa::tqr(t->t);
+        *Q = wrap(t->owner, q);
+        *R = wrap(t->owner, r);
+    });
+}
+
+int xt_teigh(const xt_tensor* t, xt_tensor** D, xt_tensor** V)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        auto [d, v] = xa::teigh(t->t);
+        *D = wrap(t->owner, d);
+        *V = wrap(t->owner, v);
+    });
+}
+
+int xt_adjoint(const xt_tensor* t, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::adjoint(t->t)); });
+}
+
+int xt_twist(const xt_tensor* t, int leg, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::twist(t->t, leg)); });
+}
+
+int xt_scale(const xt_tensor* t, double alpha, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::scaled(t->t, alpha)); });
+}
+
+int xt_axpby(const xt_tensor* x, const xt_tensor* y, double a, double b, xt_tensor** out)
+{
+    return run(x ? x->owner : nullptr, [&] { *out = wrap(x->owner, xa::axpby(x->t, y->t, a, b)); });
+}
+
+int xt_diag_map(const xt_tensor* t, int mode, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::diag_map(t->t, mode)); });
+}
+
+int xt_trace(const xt_tensor* t, double* out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = xa::trace(t->t); });
+}
+
+int xt_squared_norm(const xt_tensor* t, double* out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = xa::squared_norm(t->t); });
+}
+
+int xt_max_norm(const xt_tensor* t, double* out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = xa::max_norm(t->t); });
+}
+
+int xt_truncation_rule(int kind0, int kind1, const int* qs, const double* svs, int64_t n, size_t max_keep, double eps_svd, int preserve_multiplets,
+                       int* nsec, int* kept_q, int* kept_n, double* trunc_weight, double* entropy)
+{
+    return run(nullptr, [&] {
+        xa::Sym s = xa::Sym::make(kind0, kind1);
+        std::vector<std::pair<xa::Q, double>> all;
+        for(int64_t i = 0; i < n; ++i) all.emplace_back(q_from(qs + 2 * i), svs[i]);
+        xa::TruncRule r = xa::truncation_rule(s, all, max_keep, eps_svd, preserve_multiplets != 0, *trunc_weight, *entropy);
+        int k = 0;
+        for(const auto& [q, v] : r.kept) {
+            q_to(q, kept_q + 2 * k);
+            kept_n[k] = static_cast<int>(v.size());
+            ++k;
+        }
+        *nsec = k;
+        *trunc_weight = r.trunc_weight;
+        *entropy = r.entropy;
+    });
+}
+
+}  // extern "C"

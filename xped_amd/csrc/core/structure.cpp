@@ -1,0 +1,738 @@
+// Host-side integer structure engine (fusion trees, bases, tree-pair moves, plans).
+#include "structure.hpp"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <numeric>
+#include <sstream>
+
+namespace xa {
+
+// ---------------------------------------------------------------- FTree
+
+FTree FTree::enlarge(const FTree& other) const
+{
+    FTree out;
+    out.rank = rank + 1;
+    for(int i = 0; i < rank; ++i) {
+        out.dims[i] = dims[i];
+        out.unc[i] = unc[i];
+        out.dual[i] = dual[i];
+    }
+    out.dims[rank] = other.dims[0];
+    out.unc[rank] = other.unc[0];
+    out.dual[rank] = other.dual[0];
+    out.compute_dim();
+    for(int i = 0; i < n_inter(); ++i) out.inter[i] = inter[i];
+    if(out.n_inter() > 0) out.inter[out.n_inter() - 1] = coupled;
+    return out;
+}
+
+void FTree::compute_intermediates(const Sym& s)
+{
+    if(rank == 0) return;
+    for(int i = 0; i < n_inter(); ++i)
+        inter[i] = (i == 0) ? s.combine(unc[0], unc[1]).front() : s.combine(inter[i - 1], unc[i + 1]).front();
+    if(rank < 2)
+        coupled = unc[0];
+    else if(rank == 2)
+        coupled = s.combine(unc[0], unc[1]).front();
+    else
+        coupled = s.combine(inter[n_inter() - 1], unc[rank - 1]).front();
+}
+
+bool FTreeKeyLess::operator()(const FTree& a, const FTree& b) const
+{
+    if(a.rank != b.rank) return a.rank < b.rank;
+    for(int i = 0; i < a.rank; ++i) {
+        if(a.unc[i] != b.unc[i]) return a.unc[i] < b.unc[i];
+    }
+    for(int i = 0; i < a.n_inter(); ++i) {
+        if(a.inter[i] != b.inter[i]) return a.inter[i] < b.inter[i];
+    }
+    if(a.coupled != b.coupled) return a.coupled < b.coupled;
+    for(int i = 0; i < a.rank; ++i) {
+        if(a.dims[i] != b.dims[i]) return a.dims[i] < b.dims[i];
+        if(a.dual[i] != b.dual[i]) return a.dual[i] < b.dual[i];
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- Permutation
+
+Permutation::Permutation(std::vector<int> p) : pi(std::move(p))
+{
+    pi_inv.assign(pi.size(), 0);
+    std::vector<int> seen(pi.size(), 0);
+    for(std::size_t i = 0; i < pi.size(); ++i) {
+        if(pi[i] < 0 || pi[i] >= static_cast<int>(pi.size()) || seen[pi[i]]) throw std::invalid_argument("invalid permutation");
+        seen[pi[i]] = 1;
+        pi_inv[pi[i]] = static_cast<int>(i);
+    }
+}
+
+bool Permutation::is_identity() const
+{
+    for(std::size_t i = 0; i < pi.size(); ++i)
+        if(pi[i] != static_cast<int>(i)) return false;
+    return true;
+}
+
+std::vector<int> Permutation::decompose() const
+{
+    // Bubble the target sequence into place with adjacent swaps; the swap positions are recorded
+    // from the rightmost one down to the target slot, as Permutations.hpp:195-225 does.
+    std::vector<int> a(pi.size());
+    std::iota(a.begin(), a.end(), 0);
+    std::vector<int> out;
+    if(a == pi) return out;
+    for(int i = 0; i < static_cast<int>(pi.size()); ++i) {
+        int index = static_cast<int>(std::find(a.begin(), a.end(), pi[i]) - a.begin());
+        if(index == 0) continue;
+        for(int j = index - 1; j >= i; --j) {
+            std::swap(a[j], a[j + 1]);
+            out.push_back(j);
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- Qbasis
+
+void Qbasis::push_back(const Q& q, std::size_t inner)
+{
+    if(inner == 0) return;
+    int i = find(q);
+    if(i < 0) {
+        QEntry e;
+        e.q = q;
+        e.num = curr_dim;
+        e.dim = inner;
+        FTree t;
+        if(depth == 1) {
+            t.rank = 1;
+            t.unc[0] = q;
+            t.coupled = q;
+            t.dual[0] = conj_flag;
+            t.dims[0] = static_cast<std::uint32_t>(inner);
+            t.dim = inner;
+        } else {
+            t.rank = 0;
+            t.dim = 1;
+            t.coupled = q;
+        }
+        e.trees.push_back(t);
+        data.push_back(e);
+        sorted = false;
+    } else {
+        data[i].dim += inner;
+        for(std::size_t j = i + 1; j < data.size(); ++j) data[j].num += inner;
+        if(depth == 1) {
+            data[i].trees[0].dim += inner;
+            data[i].trees[0].dims[0] += static_cast<std::uint32_t>(inner);
+        }
+    }
+    curr_dim += inner;
+}
+
+void Qbasis::push_back_trees(const Q& q, std::size_t inner, const std::vector<FTree>& t)
+{
+    if(find(q) >= 0) throw std::logic_error("Qbasis::push_back with trees: quantum number already present");
+    QEntry e;
+    e.q = q;
+    e.num = curr_dim;
+    e.dim = inner;
+    e.trees = t;
+    data.push_back(e);
+    sorted = false;
+    curr_dim += inner;
+}
+
+std::vector<std::size_t> Qbasis::sort()
+{
+    if(data.size() < 2) return {0};
+    std::vector<std::size_t> idx(data.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](std::size_t a, std::size_t b) {
+        if(conj_flag) return sym.compare(sym.conj(data[a].q), sym.conj(data[b].q));
+        return sym.compare(data[a].q, data[b].q);
+    });
+    std::vector<QEntry> nd(data.size());
+    for(std::size_t i = 0; i < data.size(); ++i) nd[i] = data[idx[i]];
+    std::size_t acc = 0;
+    for(auto& e : nd) {
+        e.num = acc;
+        acc += e.dim;
+    }
+    data = std::move(nd);
+    sorted = true;
+    return idx;
+}
+
+Qbasis Qbasis::conj() const
+{
+    Qbasis out(sym, 1);
+    out.conj_flag = !conj_flag;
+    for(const auto& e : data) {
+        const FTree& old = e.trees.at(0);
+        FTree t;
+        t.rank = 1;
+        t.unc[0] = sym.conj(e.q);
+        t.coupled = sym.conj(e.q);
+        t.dual[0] = !old.dual[0];
+        t.dims[0] = old.dims[0];
+        t.dim = old.dim;
+        out.push_back_trees(sym.conj(e.q), e.dim, {t});
+    }
+    out.sort();
+    return out;
+}
+
+Qbasis Qbasis::combine(const Qbasis& other) const
+{
+    Qbasis out(sym, depth + 1);
+    // coupled q -> trees in discovery order; the final order is fixed by the sorts below
+    std::vector<Q> qs;
+    std::vector<std::vector<FTree>> trees;
+    auto slot = [&](const Q& q) -> std::vector<FTree>& {
+        for(std::size_t i = 0; i < qs.size(); ++i)
+            if(qs[i] == q) return trees[i];
+        qs.push_back(q);
+        trees.emplace_back();
+        return trees.back();
+    };
+    for(const auto& e1 : data) {
+        for(const auto& e2 : other.data) {
+            for(const Q& q : sym.combine(e1.q, e2.q)) {
+                for(const auto& t1 : e1.trees)
+                    for(const auto& t2 : e2.trees) {
+                        // Qbasis.cpp:463-468 skips trees already present; distinct (q1, q2, t1, t2, q)
+                        // always give distinct enlarged trees, so the linear search is omitted.
+                        FTree tt = t1.enlarge(t2);
+                        tt.coupled = q;
+                        slot(q).push_back(tt);
+                    }
+            }
+        }
+    }
+    for(auto& v : trees) std::sort(v.begin(), v.end(), [](const FTree& a, const FTree& b) { return a.less(b); });
+    for(std::size_t i = 0; i < qs.size(); ++i) {
+        std::size_t inner = 0;
+        for(const auto& t : trees[i]) inner += t.dim;
+        out.push_back_trees(qs[i], inner, trees[i]);
+    }
+    out.sort();
+    return out;
+}
+
+Qbasis Qbasis::forget_history() const
+{
+    Qbasis out(sym, 1);
+    for(const auto& e : data) out.push_back(e.q, e.dim);
+    return out;
+}
+
+std::size_t Qbasis::left_offset(const FTree& t) const
+{
+    int i = find(t.coupled);
+    if(i < 0) throw std::runtime_error("Qbasis::leftOffset: coupled sector not present");
+    std::size_t out = 0;
+    for(const auto& x : data[i].trees) {
+        if(x.same(t)) return out;
+        out += x.dim;
+    }
+    throw std::runtime_error("Qbasis::leftOffset: tree not present");
+}
+
+bool Qbasis::equals(const Qbasis& o) const
+{
+    if(data.size() != o.data.size()) return false;
+    for(std::size_t i = 0; i < data.size(); ++i) {
+        const auto& a = data[i];
+        const auto& b = o.data[i];
+        if(a.q != b.q || a.num != b.num || a.dim != b.dim || a.trees.size() != b.trees.size()) return false;
+        for(std::size_t t = 0; t < a.trees.size(); ++t)
+            if(!a.trees[t].same(b.trees[t])) return false;
+    }
+    return true;
+}
+
+Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled)
+{
+    if(uncoupled.empty()) {
+        Qbasis b(s, 0);
+        b.push_back(s.vacuum(), 1);
+        return b;
+    }
+    Qbasis b = uncoupled[0];
+    for(std::size_t i = 1; i < uncoupled.size(); ++i) b = b.combine(uncoupled[i]);
+    return b;
+}
+
+// ---------------------------------------------------------------- tree moves
+
+TreeMap tree_swap(const Sym& s, const FTree& t, int pos)
+{
+    TreeMap out;
+    const int R = t.rank;
+    FTree tree = t;
+    std::swap(tree.unc[pos], tree.unc[pos + 1]);
+    std::swap(tree.dual[pos], tree.dual[pos + 1]);
+    std::swap(tree.dims[pos], tree.dims[pos + 1]);
+    if(pos == 0) {
+        const Q& ql = t.unc[0];
+        const Q& qr = t.unc[1];
+        const Q& qf = (R == 2) ? t.coupled : t.inter[0];
+        double c = s.coeff_swap(ql, qr, qf);
+        if(std::abs(c) < TREE_EPS) return out;
+        out.emplace(tree, c);
+        return out;
+    }
+    const Q q1 = (pos == 1) ? t.unc[0] : t.inter[pos - 2];
+    const Q q2 = t.unc[pos];
+    const Q q3 = t.unc[pos + 1];
+    const Q Qt = (pos == R - 2) ? t.coupled : t.inter[pos];
+    const Q Q12 = t.inter[pos - 1];
+    for(const Q& Q31 : s.combine(q3, q1)) {
+        tree.inter[pos - 1] = Q31;
+        double cgc = s.coeff_swap(Q12, q3, Qt) * s.coeff_recouple(q3, q1, q2, Qt, Q31, Q12) * s.coeff_swap(q3, q1, Q31);
+        if(std::abs(cgc) < TREE_EPS) continue;
+        out.emplace(tree, cgc);
+    }
+    return out;
+}
+
+TreeMap tree_permute(const Sym& s, const FTree& t, const Permutation& p)
+{
+    TreeMap out;
+    out.emplace(t, 1.0);
+    for(int pos : p.decompose()) {
+        TreeMap tmp;
+        for(const auto& [tree, c] : out) {
+            for(const auto& [tree2, c2] : tree_swap(s, tree, pos)) {
+                auto it = tmp.find(tree2);
+                if(it == tmp.end()) {
+                    tmp.emplace(tree2, c * c2);
+                } else if(std::abs(it->second + c * c2) < TREE_EPS) {
+                    tmp.erase(it);
+                } else {
+                    it->second += c * c2;
+                }
+            }
+        }
+        out.swap(tmp);
+    }
+    return out;
+}
+
+PairMap pair_turn_right(const Sym& s, const FTree& t1, const FTree& t2)
+{
+    const int R = t1.rank, C = t2.rank;
+    if(C < 1) throw std::logic_error("turn_right needs a codomain leg");
+    FTree t1p;
+    t1p.rank = R + 1;
+    for(int i = 0; i < R; ++i) {
+        t1p.unc[i] = t1.unc[i];
+        t1p.dual[i] = t1.dual[i];
+        t1p.dims[i] = t1.dims[i];
+    }
+    t1p.unc[R] = s.conj(t2.unc[C - 1]);
+    for(int i = 0; i < t1.n_inter(); ++i) t1p.inter[i] = t1.inter[i];
+    if(R + 1 > 2) t1p.inter[t1p.n_inter() - 1] = t1.coupled;
+    t1p.dual[R] = !t2.dual[C - 1];
+    Q a;
+    if(C == 1)
+        a = s.vacuum();
+    else if(C == 2)
+        a = t2.unc[0];
+    else
+        a = t2.inter[t2.n_inter() - 1];
+    t1p.coupled = a;
+    t1p.dims[R] = t2.dims[C - 1];
+    t1p.compute_dim();
+    FTree t2p;
+    t2p.rank = C - 1;
+    for(int i = 0; i < C - 1; ++i) {
+        t2p.unc[i] = t2.unc[i];
+        t2p.dual[i] = t2.dual[i];
+        t2p.dims[i] = t2.dims[i];
+    }
+    for(int i = 0; i < t2p.n_inter(); ++i) t2p.inter[i] = t2.inter[i];
+    t2p.coupled = a;
+    t2p.compute_dim();
+    const Q b = t2.unc[C - 1];
+    const Q c = t2.coupled;
+    double coeff = s.coeff_turn(a, b, c);
+    if(t2.dual[C - 1]) coeff *= s.coeff_FS(s.conj(b));
+    PairMap out;
+    if(std::abs(coeff) < TREE_EPS) return out;
+    out.emplace(TreePair{t1p, t2p}, coeff);
+    return out;
+}
+
+PairMap pair_turn_left(const Sym& s, const FTree& t1, const FTree& t2)
+{
+    PairMap out;
+    for(const auto& [pr, c] : pair_turn_right(s, t2, t1)) out.emplace(TreePair{pr.second, pr.first}, c);
+    return out;
+}
+
+PairMap pair_turn(const Sym& s, int shift, const FTree& t1, const FTree& t2)
+{
+    PairMap out;
+    if(shift == 0) {
+        out.emplace(TreePair{t1, t2}, 1.0);
+        return out;
+    }
+    if(shift > 0) {
+        for(const auto& [pr, c1] : pair_turn_left(s, t1, t2))
+            for(const auto& [pr2, c2] : pair_turn(s, shift - 1, pr.first, pr.second)) out.emplace(pr2, c1 * c2);
+        return out;
+    }
+    for(const auto& [pr, c1] : pair_turn_right(s, t1, t2))
+        for(const auto& [pr2, c2] : pair_turn(s, shift + 1, pr.first, pr.second)) out.emplace(pr2, c1 * c2);
+    return out;
+}
+
+PairMap pair_permute(const Sym& s, int shift, const FTree& t1, const FTree& t2, const Permutation& p)
+{
+    const int R = t1.rank, C = t2.rank;
+    const int newR = R - shift, newC = C + shift;
+    if(!s.any_non_abelian() && !s.any_fermionic()) {
+        // abelian bosonic fast path (treepair.cpp:147-168): one output pair, coefficient 1
+        FTree od, oc;
+        od.rank = newR;
+        oc.rank = newC;
+        for(int r = 0; r < newR; ++r) {
+            int src = p.pi[r];
+            od.unc[r] = src < R ? t1.unc[src] : s.conj(t2.unc[src - R]);
+            od.dims[r] = src < R ? t1.dims[src] : t2.dims[src - R];
+            od.dual[r] = src < R ? t1.dual[src] : !t2.dual[src - R];
+        }
+        for(int c = 0; c < newC; ++c) {
+            int src = p.pi[c + newR];
+            oc.unc[c] = src < R ? s.conj(t1.unc[src]) : t2.unc[src - R];
+            oc.dims[c] = src < R ? t1.dims[src] : t2.dims[src - R];
+            oc.dual[c] = src < R ? !t1.dual[src] : t2.dual[src - R];
+        }
+        od.compute_dim();
+        od.compute_intermediates(s);
+        oc.compute_dim();
+        oc.compute_intermediates(s);
+        PairMap out;
+        out.emplace(TreePair{od, oc}, 1.0);
+        return out;
+    }
+    // turn<-CoRank>() reverses the order of the codomain legs, so the permutation is corrected first
+    const int N = R + C;
+    std::vector<int> pi_id(N);
+    std::iota(pi_id.begin(), pi_id.end(), 0);
+    for(int i = 0; i < C; ++i) pi_id[i + R] = (C - 1) - i + R;
+    std::vector<int> pi_tmp(N);
+    for(int i = 0; i < N; ++i) pi_tmp[i] = pi_id[p.pi[i]];
+    std::vector<int> pi_corr(N);
+    for(int i = 0; i < newR; ++i) pi_corr[i] = pi_tmp[i];
+    for(int i = 0; i < newC; ++i) pi_corr[i + newR] = pi_tmp[(newC - 1) - i + newR];
+    Permutation pc(pi_corr);
+    PairMap out;
+    for(const auto& [pr, c1] : pair_turn(s, -C, t1, t2)) {
+        for(const auto& [t1pp, c2] : tree_permute(s, pr.first, pc)) {
+            for(const auto& [fin, c3] : pair_turn(s, newC, t1pp, pr.second)) {
+                auto it = out.find(fin);
+                if(it == out.end())
+                    out.emplace(fin, c1 * c2 * c3);
+                else
+                    it->second += c1 * c2 * c3;
+            }
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- tensor structure
+
+static void finish_layout(TensorStructure& t)
+{
+    std::size_t curr = 0;
+    t.offsets.clear();
+    t.rows.clear();
+    t.cols.clear();
+    for(const Q& q : t.sector) {
+        t.offsets.push_back(curr);
+        std::size_t r = t.dom.inner_dim(q), c = t.cod.inner_dim(q);
+        t.rows.push_back(r);
+        t.cols.push_back(c);
+        curr += r * c;
+    }
+    t.size = curr;
+}
+
+StructPtr TensorStructure::create(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c)
+{
+    auto t = std::make_shared<TensorStructure>();
+    t->sym = s;
+    t->rank = static_cast<int>(d.size());
+    t->corank = static_cast<int>(c.size());
+    if(t->rank > MAXR || t->corank > MAXR) throw std::invalid_argument("too many legs");
+    t->unc_dom = d;
+    t->unc_cod = c;
+    t->dom = build_fusion_tree(s, d);
+    t->cod = build_fusion_tree(s, c);
+    for(const auto& e : t->dom.data)
+        if(t->cod.present(e.q)) t->sector.push_back(e.q);
+    finish_layout(*t);
+    return t;
+}
+
+StructPtr TensorStructure::create_pushed(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c, const std::vector<Q>& sectors)
+{
+    auto t = std::make_shared<TensorStructure>();
+    t->sym = s;
+    t->rank = static_cast<int>(d.size());
+    t->corank = static_cast<int>(c.size());
+    t->unc_dom = d;
+    t->unc_cod = c;
+    t->dom = build_fusion_tree(s, d);
+    t->cod = build_fusion_tree(s, c);
+    t->sector = sectors;
+    finish_layout(*t);
+    return t;
+}
+
+static void permuted_bases(const TensorStructure& in, int shift, const std::vector<int>& p, bool split, std::vector<Qbasis>& nd,
+                           std::vector<Qbasis>& nc)
+{
+    const int R = in.rank;
+    const int newR = in.rank - shift, newC = in.corank + shift;
+    nd.clear();
+    nc.clear();
+    if(split) {
+        for(int i = 0; i < newR; ++i) nd.push_back(in.unc_dom[p[i]]);
+        for(int i = 0; i < newC; ++i) nc.push_back(in.unc_cod[p[i + newR] - R]);
+        return;
+    }
+    for(int i = 0; i < newR; ++i) nd.push_back(p[i] >= R ? in.unc_cod[p[i] - R].conj() : in.unc_dom[p[i]]);
+    for(int i = 0; i < newC; ++i) nc.push_back(p[i + newR] >= R ? in.unc_cod[p[i + newR] - R] : in.unc_dom[p[i + newR]].conj());
+}
+
+static void check_perm(const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    const int N = in.rank + in.corank;
+    if(static_cast<int>(p.size()) != N) throw std::invalid_argument("permutation length does not match the tensor rank");
+    if(shift > in.rank || -shift > in.corank) throw std::invalid_argument("invalid shift for permute");
+    Permutation chk(p);  // validates
+    (void)chk;
+}
+
+static bool is_split_case(const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    // Tensor.cpp:335-355: shift 0 and no leg crosses between domain and codomain
+    if(shift != 0) return false;
+    for(int i = 0; i < in.rank; ++i)
+        if(p[i] >= in.rank) return false;
+    for(int i = in.rank; i < in.rank + in.corank; ++i)
+        if(p[i] < in.rank) return false;
+    return true;
+}
+
+StructPtr permuted_structure(const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    check_perm(in, shift, p);
+    std::vector<Qbasis> nd, nc;
+    permuted_bases(in, shift, p, is_split_case(in, shift, p), nd, nc);
+    return TensorStructure::create(in.sym, nd, nc);
+}
+
+PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    check_perm(in, shift, p);
+    const int R = in.rank, C = in.corank, N = R + C;
+    const int newR = R - shift;
+    const bool split = is_split_case(in, shift, p);
+    std::vector<Qbasis> nd, nc;
+    permuted_bases(in, shift, p, split, nd, nc);
+    PermutePlan plan;
+    plan.nlegs = N;
+    plan.new_rank = newR;
+    plan.out = TensorStructure::create(in.sym, nd, nc);
+    const TensorStructure& out = *plan.out;
+
+    const Permutation Pfull(p);
+    // enumerate every output subblock so each output element is written exactly once
+    std::vector<std::map<std::pair<std::size_t, std::size_t>, int>> lookup(out.sector.size());
+    for(std::size_t s = 0; s < out.sector.size(); ++s) {
+        const Q& q = out.sector[s];
+        std::size_t roff = 0;
+        for(const auto& t1 : out.dom.tree(q)) {
+            std::size_t coff = 0;
+            for(const auto& t2 : out.cod.tree(q)) {
+                PermuteTask task;
+                task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
+                task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
+                task.rows = static_cast<std::int64_t>(t1.dim);
+                task.cols = static_cast<std::int64_t>(t2.dim);
+                for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
+                // strides of the corresponding source legs (source leg m = p[k])
+                for(int k = 0; k < N; ++k) {
+                    int m = p[k];
+                    std::int64_t st = 1;
+                    if(m < R) {
+                        for(int mm = 0; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
+                        task.is_col[k] = 0;
+                    } else {
+                        for(int mm = R; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
+                        task.is_col[k] = 1;
+                    }
+                    task.sstride[k] = st;
+                }
+                lookup[s][{roff, coff}] = static_cast<int>(plan.tasks.size());
+                plan.tasks.push_back(task);
+                plan.moved_elems += t1.dim * t2.dim;
+                coff += t2.dim;
+            }
+            roff += t1.dim;
+        }
+    }
+
+    std::vector<std::vector<PermuteSource>> contrib(plan.tasks.size());
+    std::vector<int> pdv(p.begin(), p.begin() + R), pcv;
+    for(int i = R; i < N; ++i) pcv.push_back(p[i] - R);
+    Permutation P(p);
+    std::unique_ptr<Permutation> Pd, Pc;
+    if(split) {
+        Pd = std::make_unique<Permutation>(pdv);
+        Pc = std::make_unique<Permutation>(pcv);
+    }
+    for(std::size_t i = 0; i < in.sector.size(); ++i) {
+        const Q& q = in.sector[i];
+        std::size_t roff_in = 0;
+        for(const auto& t1 : in.dom.tree(q)) {
+            std::size_t coff_in = 0;
+            for(const auto& t2 : in.cod.tree(q)) {
+                const std::int64_t src_off = static_cast<std::int64_t>(in.offsets[i] + roff_in + coff_in * in.rows[i]);
+                const std::int64_t src_ld = static_cast<std::int64_t>(in.rows[i]);
+                auto add = [&](const FTree& a, const FTree& b, double c) {
+                    if(std::abs(c) < PERMUTE_SKIP) return;
+                    int so = out.find_sector(a.coupled);
+                    if(so < 0) throw std::logic_error("permute: output sector missing");
+                    std::size_t ro = out.dom.left_offset(a), co = out.cod.left_offset(b);
+                    auto it = lookup[so].find({ro, co});
+                    if(it == lookup[so].end()) throw std::logic_error("permute: output subblock missing");
+                    contrib[it->second].push_back(PermuteSource{src_off, src_ld, c});
+                };
+                if(split) {
+                    TreeMap pd = tree_permute(in.sym, t1, *Pd);
+                    TreeMap pc = tree_permute(in.sym, t2, *Pc);
+                    for(const auto& [a, ca] : pd)
+                        for(const auto& [b, cb] : pc) add(a, b, ca * cb);
+                } else {
+                    for(const auto& [pr, c] : pair_permute(in.sym, shift, t1, t2, P)) add(pr.first, pr.second, c);
+                }
+                coff_in += t2.dim;
+            }
+            roff_in += t1.dim;
+        }
+    }
+    for(std::size_t t = 0; t < plan.tasks.size(); ++t) {
+        plan.tasks[t].src_begin = static_cast<int>(plan.sources.size());
+        plan.tasks[t].nsrc = static_cast<int>(contrib[t].size());
+        plan.sources.insert(plan.sources.end(), contrib[t].begin(), contrib[t].end());
+    }
+    return plan;
+}
+
+ContractPerms get_permutations(const std::vector<int>& a1, int rank1, const std::vector<int>& a2, int rank2, int rankres)
+{
+    const int r1 = static_cast<int>(a1.size()), r2 = static_cast<int>(a2.size());
+    int max1 = 0, max2 = 0;
+    for(int x : a1) max1 = std::max(max1, x);
+    for(int x : a2) max2 = std::max(max2, x);
+    if(max1 != max2) throw std::invalid_argument("contract: the two index lists contract a different number of legs");
+    ContractPerms out;
+    out.p2.assign(r2, 0);
+    std::vector<int> found2(r2, -1);
+    for(int ind = 0; ind < max2; ++ind) {
+        int val = static_cast<int>(std::find(a2.begin(), a2.end(), ind + 1) - a2.begin());
+        found2[ind] = val;
+        out.p2[ind] = val;
+    }
+    for(int ind = max2; ind < r2; ++ind) {
+        int val = 0;
+        for(int i = 0; i < r2; ++i)
+            if(std::find(found2.begin(), found2.end(), i) == found2.end()) {
+                val = i;
+                break;
+            }
+        found2[ind] = val;
+        out.p2[ind] = val;
+    }
+    out.p1.assign(r1, 0);
+    std::vector<int> found1(r1, -1);
+    for(int ind = 0; ind < max1; ++ind) {
+        int val = static_cast<int>(std::find(a1.begin(), a1.end(), ind + 1) - a1.begin());
+        found1[ind] = val;
+        out.p1[r1 - max1 + ind] = val;
+    }
+    for(int ind = max1; ind < r1; ++ind) {
+        int val = 0;
+        for(int i = 0; i < r1; ++i)
+            if(std::find(found1.begin(), found1.end(), i) == found1.end()) {
+                val = i;
+                break;
+            }
+        found1[ind] = val;
+        out.p1[ind - max1] = val;
+    }
+    std::vector<int> tmp;
+    for(int x : a1)
+        if(x < 0) tmp.push_back(x);
+    for(int x : a2)
+        if(x < 0) tmp.push_back(x);
+    out.pres.resize(tmp.size());
+    std::iota(out.pres.begin(), out.pres.end(), 0);
+    std::sort(out.pres.begin(), out.pres.end(), [&](int i, int j) { return tmp[i] > tmp[j]; });
+    out.shift1 = rank1 - r1 + max1;
+    out.shift2 = rank2 - max2;
+    out.shiftres = r1 - max1 - rankres;
+    return out;
+}
+
+GemmPlan make_gemm_plan(const TensorStructure& l, const TensorStructure& r)
+{
+    if(!(l.sym == r.sym)) throw std::invalid_argument("operator*: symmetry mismatch");
+    if(l.corank != r.rank) throw std::invalid_argument("operator*: CoRank of the left tensor must equal Rank of the right tensor");
+    GemmPlan plan;
+    plan.out = TensorStructure::create(l.sym, l.unc_dom, r.unc_cod);
+    const TensorStructure& o = *plan.out;
+    for(std::size_t i = 0; i < l.sector.size(); ++i) {
+        int j = r.find_sector(l.sector[i]);
+        if(j < 0) continue;
+        int so = o.find_sector(l.sector[i]);
+        if(so < 0) throw std::logic_error("operator*: output sector missing");
+        if(l.cols[i] != r.rows[j]) throw std::invalid_argument("operator*: inner block dimensions differ (codomain of left != domain of right)");
+        GemmItem it;
+        it.a_off = static_cast<std::int64_t>(l.offsets[i]);
+        it.b_off = static_cast<std::int64_t>(r.offsets[j]);
+        it.c_off = static_cast<std::int64_t>(o.offsets[so]);
+        it.m = static_cast<int>(l.rows[i]);
+        it.n = static_cast<int>(r.cols[j]);
+        it.k = static_cast<int>(l.cols[i]);
+        if(it.m > 0 && it.n > 0) {
+            plan.items.push_back(it);
+            plan.flops += 2.0 * it.m * it.n * static_cast<double>(it.k);
+        }
+    }
+    return plan;
+}
+
+std::string q_str(const Sym& s, const Q& q)
+{
+    std::ostringstream os;
+    os << "(";
+    for(int c = 0; c < s.nq; ++c) os << (c ? "," : "") << q.v[c];
+    os << ")";
+    return os.str();
+}
+
+}  // namespace xa

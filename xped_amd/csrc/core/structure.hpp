@@ -1,0 +1,245 @@
+// Host-side integer structure of block-sparse symmetric tensors: fusion trees, bases with
+// fusion history, tree-pair recoupling, and the storage layout of a tensor.
+//
+// Semantics follow the reference's Core layer line by line where ordering is observable:
+//   FusionTree          include/Xped/Core/FusionTree.hpp:22-115, src/Core/FusionTree.cpp
+//   Qbasis              include/Xped/Core/Qbasis.hpp:37-268, src/Core/Qbasis.cpp
+//   build_FusionTree    src/Core/TensorHelper.cpp:14-44
+//   treepair            src/Core/treepair.cpp:35-217
+//   StorageType layout  include/Xped/Core/storage/StorageType_contiguous.hpp:79-97,161-171
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "sym.hpp"
+
+namespace xa {
+
+constexpr int MAXR = 10;                 // maximal number of legs on one side of a tensor
+constexpr double TREE_EPS = 1.e-12;      // ScalarTraits<double>::epsilon() (ScalarTraits.hpp:17)
+constexpr double PERMUTE_SKIP = 1.e-10;  // |coeff| threshold in permute_impl (Tensor.cpp:127,193)
+
+struct FTree {
+    int rank = 0;
+    std::array<Q, MAXR> unc{};
+    std::array<Q, MAXR> inter{};  // rank-2 entries are meaningful (util::inter_dim)
+    Q coupled{};
+    std::array<std::uint32_t, MAXR> dims{};
+    std::uint64_t dim = 1;
+    std::array<bool, MAXR> dual{};
+
+    int n_inter() const { return rank >= 2 ? rank - 2 : 0; }
+    void compute_dim()
+    {
+        dim = 1;
+        for(int i = 0; i < rank; ++i) dim *= dims[i];
+    }
+    // FusionTree::operator== without XPED_PEDANTIC_ASSERTS (FusionTree.cpp:66-83): IS_DUAL is ignored
+    bool same(const FTree& o) const
+    {
+        if(rank != o.rank || coupled != o.coupled) return false;
+        for(int i = 0; i < rank; ++i)
+            if(unc[i] != o.unc[i] || dims[i] != o.dims[i]) return false;
+        for(int i = 0; i < n_inter(); ++i)
+            if(inter[i] != o.inter[i]) return false;
+        return true;
+    }
+    // FusionTree::operator< (FusionTree.cpp:32-64): uncoupled, intermediates, coupled (multiplicities are all 0)
+    bool less(const FTree& o) const
+    {
+        for(int i = 0; i < rank; ++i) {
+            if(unc[i] < o.unc[i]) return true;
+            if(unc[i] > o.unc[i]) return false;
+        }
+        for(int i = 0; i < n_inter(); ++i) {
+            if(inter[i] < o.inter[i]) return true;
+            if(inter[i] > o.inter[i]) return false;
+        }
+        return coupled < o.coupled;
+    }
+    FTree enlarge(const FTree& other) const;  // FusionTree.cpp:309-324
+    void compute_intermediates(const Sym& s);  // FusionTree.cpp:136-152 (abelian only)
+};
+
+// Total order over every field (used as a map key inside recoupling; no reference counterpart
+// for the order itself, which only influences floating-point accumulation order of SU(2) coefficients).
+struct FTreeKeyLess {
+    bool operator()(const FTree& a, const FTree& b) const;
+};
+using TreeMap = std::map<FTree, double, FTreeKeyLess>;
+using TreePair = std::pair<FTree, FTree>;
+struct TreePairLess {
+    bool operator()(const TreePair& a, const TreePair& b) const
+    {
+        FTreeKeyLess l;
+        if(l(a.first, b.first)) return true;
+        if(l(b.first, a.first)) return false;
+        return l(a.second, b.second);
+    }
+};
+using PairMap = std::map<TreePair, double, TreePairLess>;
+
+struct Permutation {
+    std::vector<int> pi, pi_inv;
+    explicit Permutation(std::vector<int> p);
+    std::vector<int> decompose() const;  // Permutations.hpp:195-225
+    bool is_identity() const;
+};
+
+// One basis entry (q, num, plain dim) plus its fusion trees.
+struct QEntry {
+    Q q;
+    std::size_t num = 0;
+    std::size_t dim = 0;
+    std::vector<FTree> trees;
+};
+
+struct Qbasis {
+    Sym sym;
+    int depth = 1;
+    bool conj_flag = false;
+    bool sorted = false;
+    std::size_t curr_dim = 0;
+    std::vector<QEntry> data;
+
+    Qbasis() = default;
+    Qbasis(const Sym& s, int d) : sym(s), depth(d) {}
+
+    int find(const Q& q) const
+    {
+        for(std::size_t i = 0; i < data.size(); ++i)
+            if(data[i].q == q) return static_cast<int>(i);
+        return -1;
+    }
+    bool present(const Q& q) const { return find(q) >= 0; }
+    std::size_t inner_dim(const Q& q) const
+    {
+        int i = find(q);
+        return i < 0 ? 0 : data[i].dim;
+    }
+    std::size_t dim() const
+    {
+        std::size_t d = 0;
+        for(auto& e : data) d += e.dim;
+        return d;
+    }
+    std::size_t full_dim() const
+    {
+        std::size_t d = 0;
+        for(auto& e : data) d += e.dim * sym.degeneracy(e.q);
+        return d;
+    }
+    const std::vector<FTree>& tree(const Q& q) const
+    {
+        int i = find(q);
+        if(i < 0) throw std::runtime_error("Qbasis::tree: quantum number not present");
+        return data[i].trees;
+    }
+
+    void push_back(const Q& q, std::size_t inner);                                    // Qbasis.cpp:34-73
+    void push_back_trees(const Q& q, std::size_t inner, const std::vector<FTree>& t);  // Qbasis.cpp:75-91
+    std::vector<std::size_t> sort();                                                    // Qbasis.cpp:306-331
+    Qbasis conj() const;                                                                // Qbasis.cpp:418-439
+    Qbasis combine(const Qbasis& other) const;                                          // Qbasis.cpp:441-495
+    Qbasis forget_history() const;                                                      // Qbasis.cpp:410-416
+    std::size_t left_offset(const FTree& t) const;                                      // Qbasis.cpp:208-225
+    bool equals(const Qbasis& o) const;                                                 // Qbasis.cpp:333-349
+};
+
+Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled);  // TensorHelper.cpp:14-44
+
+// Fusion tree moves.
+TreeMap tree_swap(const Sym& s, const FTree& t, int pos);                       // FusionTree.cpp:353-412
+TreeMap tree_permute(const Sym& s, const FTree& t, const Permutation& p);       // FusionTree.cpp:326-351
+PairMap pair_turn_right(const Sym& s, const FTree& t1, const FTree& t2);        // treepair.cpp:35-88
+PairMap pair_turn_left(const Sym& s, const FTree& t1, const FTree& t2);         // treepair.cpp:90-101
+PairMap pair_turn(const Sym& s, int shift, const FTree& t1, const FTree& t2);   // treepair.cpp:103-141
+PairMap pair_permute(const Sym& s, int shift, const FTree& t1, const FTree& t2, const Permutation& p);  // treepair.cpp:143-217
+
+// Structure (no data) of a Tensor<Scalar, Rank, CoRank, Symmetry>.
+struct TensorStructure {
+    Sym sym;
+    int rank = 0, corank = 0;
+    std::vector<Qbasis> unc_dom, unc_cod;  // depth-1 legs
+    Qbasis dom, cod;                       // coupled (fusion-tree) bases
+    std::vector<Q> sector;
+    std::vector<std::size_t> offsets, rows, cols;
+    std::size_t size = 0;
+
+    // Tensor(basis_domain, basis_codomain) + StorageType::resize (StorageType_contiguous.hpp:34-44,79-97)
+    static std::shared_ptr<TensorStructure> create(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c);
+    // Layout built by repeated StorageType::push_back (insertion order, StorageType_contiguous.hpp:161-171)
+    static std::shared_ptr<TensorStructure> create_pushed(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c,
+                                                          const std::vector<Q>& sectors);
+    int find_sector(const Q& q) const
+    {
+        for(std::size_t i = 0; i < sector.size(); ++i)
+            if(sector[i] == q) return static_cast<int>(i);
+        return -1;
+    }
+    std::size_t leg_dim(int leg, const Q& q) const
+    {
+        return leg < rank ? unc_dom[leg].inner_dim(q) : unc_cod[leg - rank].inner_dim(q);
+    }
+};
+using StructPtr = std::shared_ptr<TensorStructure>;
+
+// Gather plan of one block-sparse permute: every output subblock lists its (source subblock, coeff)
+// contributions in the order the reference accumulates them (Tensor.cpp:179-212).
+struct PermuteSource {
+    std::int64_t src_off;  // element offset of the source subblock's (0,0) in the input buffer
+    std::int64_t src_ld;   // leading dimension of the source block
+    double coeff;
+};
+struct PermuteTask {
+    std::int64_t dst_off;  // element offset of the output subblock's (0,0)
+    std::int64_t dst_ld;
+    std::int64_t rows, cols;  // output subblock extent
+    int nsrc = 0;
+    int src_begin = 0;
+    std::array<std::uint32_t, MAXR * 2> ext{};      // output leg extents
+    std::array<std::int64_t, MAXR * 2> sstride{};  // per output leg: stride in source row (is_col=0) or col (is_col=1) index
+    std::array<std::uint8_t, MAXR * 2> is_col{};
+};
+struct PermutePlan {
+    int nlegs = 0, new_rank = 0;
+    std::vector<PermuteTask> tasks;
+    std::vector<PermuteSource> sources;
+    StructPtr out;
+    std::size_t moved_elems = 0;  // sum over tasks of rows*cols
+};
+
+// Build the output structure and gather plan of T.permute<shift, p...>() (Tensor.cpp:83-214,335-355).
+PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p);
+// Output structure of permute without the plan.
+StructPtr permuted_structure(const TensorStructure& in, int shift, const std::vector<int>& p);
+
+// get_permutations (Constfct.hpp:55-109) evaluated at run time.
+struct ContractPerms {
+    std::vector<int> p1, p2, pres;
+    int shift1 = 0, shift2 = 0, shiftres = 0;
+};
+ContractPerms get_permutations(const std::vector<int>& a1, int rank1, const std::vector<int>& a2, int rank2, int rankres);
+
+// Matched-sector GEMM list of operator* (TensorBase.cpp:237-288).
+struct GemmItem {
+    std::int64_t a_off, b_off, c_off;
+    int m, n, k;
+};
+struct GemmPlan {
+    std::vector<GemmItem> items;
+    StructPtr out;
+    double flops = 0;
+};
+GemmPlan make_gemm_plan(const TensorStructure& l, const TensorStructure& r);
+
+std::string q_str(const Sym& s, const Q& q);
+
+}  // namespace xa

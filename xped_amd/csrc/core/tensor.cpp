@@ -1,0 +1,588 @@
+// Device-resident tensor operations: structure on the host, data movement and arithmetic on the GPU.
+#include "tensor.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+namespace xa {
+
+namespace {
+
+void dcheck(xh_ctx* ctx, int rc, const char* what)
+{
+    if(rc != XH_OK) throw Error(rc, std::string(what) + ": " + xh_last_error(ctx));
+}
+
+void put_i(std::string& s, long long v) { s.append(reinterpret_cast<const char*>(&v), sizeof(v)); }
+
+void put_basis(std::string& s, const Qbasis& b)
+{
+    put_i(s, b.conj_flag ? 1 : 0);
+    put_i(s, static_cast<long long>(b.data.size()));
+    for(const auto& e : b.data) {
+        put_i(s, e.q.v[0]);
+        put_i(s, e.q.v[1]);
+        put_i(s, static_cast<long long>(e.dim));
+        // a depth-1 tree records the duality of the leg
+        put_i(s, e.trees.empty() ? 0 : (e.trees[0].dual[0] ? 1 : 0));
+    }
+}
+
+}  // namespace
+
+Runtime::~Runtime()
+{
+    if(ctx) {
+        for(auto& [k, v] : permute_cache)
+            if(v->dev) xh_permute_plan_destroy(ctx, v->dev);
+        for(auto& [k, v] : gemm_cache)
+            if(v->dev) xh_gemm_plan_destroy(ctx, v->dev);
+    }
+}
+
+void Runtime::require_device() const
+{
+    if(!ctx) throw Error(XH_ENODEV, "this runtime has no GPU (structure-only)");
+}
+
+std::string legs_key(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c)
+{
+    std::string k;
+    put_i(k, s.kind[0]);
+    put_i(k, s.kind[1]);
+    put_i(k, static_cast<long long>(d.size()));
+    put_i(k, static_cast<long long>(c.size()));
+    for(const auto& b : d) put_basis(k, b);
+    for(const auto& b : c) put_basis(k, b);
+    return k;
+}
+
+std::string structure_key(const TensorStructure& t)
+{
+    std::string k = legs_key(t.sym, t.unc_dom, t.unc_cod);
+    put_i(k, static_cast<long long>(t.sector.size()));
+    for(const Q& q : t.sector) {
+        put_i(k, q.v[0]);
+        put_i(k, q.v[1]);
+    }
+    return k;
+}
+
+StructPtr cached_structure(Runtime* rt, const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c)
+{
+    std::string k = legs_key(s, d, c);
+    auto it = rt->struct_cache.find(k);
+    if(it != rt->struct_cache.end()) return it->second;
+    StructPtr st = TensorStructure::create(s, d, c);
+    rt->struct_cache.emplace(k, st);
+    return st;
+}
+
+// ------------------------------------------------------------------ DTensor basics
+
+DTensor DTensor::empty(Runtime* rt, StructPtr st)
+{
+    DTensor t;
+    t.rt = rt;
+    t.st = std::move(st);
+    t.buf = std::make_shared<DeviceBuffer>();
+    if(rt->ctx && t.st->size > 0) {
+        t.buf->holder = rt->holder;
+        t.buf->n = t.st->size;
+        void* p = nullptr;
+        dcheck(rt->ctx, xh_alloc(rt->ctx, t.st->size * sizeof(double), &p), "allocation of tensor data");
+        t.buf->ptr = static_cast<double*>(p);
+    }
+    return t;
+}
+
+DTensor DTensor::zeros(Runtime* rt, StructPtr st)
+{
+    DTensor t = empty(rt, std::move(st));
+    if(rt->ctx && t.size() > 0) dcheck(rt->ctx, xh_fill(rt->ctx, t.data(), t.size(), 0.0), "setZero");
+    return t;
+}
+
+void DTensor::upload(const double* host, std::size_t n)
+{
+    rt->require_device();
+    if(n != size()) throw Error(XH_EINVAL, "upload: size mismatch");
+    dcheck(rt->ctx, xh_upload(rt->ctx, data(), host, n * sizeof(double)), "upload");
+}
+
+void DTensor::download(double* host, std::size_t n) const
+{
+    rt->require_device();
+    if(n != size()) throw Error(XH_EINVAL, "download: size mismatch");
+    dcheck(rt->ctx, xh_download(rt->ctx, host, data(), n * sizeof(double)), "download");
+}
+
+void DTensor::set_random(std::mt19937& engine, int dist)
+{
+    std::vector<double> h(size());
+    if(dist == 0) {
+        std::uniform_real_distribution<double> d(-1., 1.);
+        for(auto& x : h) x = d(engine);
+    } else {
+        std::normal_distribution<double> d(0., 1.);
+        for(auto& x : h) x = d(engine);
+    }
+    upload(h.data(), h.size());
+}
+
+// ------------------------------------------------------------------ permute
+
+std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    std::string k = structure_key(in);
+    put_i(k, shift);
+    for(int x : p) put_i(k, x);
+    auto it = rt->permute_cache.find(k);
+    if(it != rt->permute_cache.end()) return it->second;
+    auto cp = std::make_shared<CachedPermute>();
+    cp->plan = make_permute_plan(in, shift, p);
+    // every access of the plan must stay inside the input and output buffers
+    {
+        const PermutePlan& pl = cp->plan;
+        const std::int64_t in_size = static_cast<std::int64_t>(in.size), out_size = static_cast<std::int64_t>(pl.out->size);
+        for(const PermuteTask& tk : pl.tasks) {
+            if(tk.rows <= 0 || tk.cols <= 0) throw Error(XH_EINVAL, "permute plan: empty subblock");
+            const std::int64_t dmax = tk.dst_off + (tk.rows - 1) + (tk.cols - 1) * tk.dst_ld;
+            if(tk.dst_off < 0 || dmax >= out_size) throw Error(XH_EINVAL, "permute plan: output subblock out of bounds");
+            std::int64_t rmax = 0, cmax = 0;
+            for(int k2 = 0; k2 < pl.nlegs; ++k2) (tk.is_col[k2] ? cmax : rmax) += (static_cast<std::int64_t>(tk.ext[k2]) - 1) * tk.sstride[k2];
+            for(int s2 = 0; s2 < tk.nsrc; ++s2) {
+                const PermuteSource& so = pl.sources[tk.src_begin + s2];
+                if(so.src_off < 0 || so.src_off + rmax + cmax * so.src_ld >= in_size || rmax >= so.src_ld)
+                    throw Error(XH_EINVAL, "permute plan: source subblock out of bounds");
+            }
+        }
+    }
+    // share output structures between identical leg sets
+    {
+        std::string sk = legs_key(cp->plan.out->sym, cp->plan.out->unc_dom, cp->plan.out->unc_cod);
+        auto si = rt->struct_cache.find(sk);
+        if(si == rt->struct_cache.end())
+            rt->struct_cache.emplace(sk, cp->plan.out);
+        else if(si->second->sector == cp->plan.out->sector)
+            cp->plan.out = si->second;
+    }
+    if(rt->ctx) {
+        std::vector<xh_permute_task> tasks(cp->plan.tasks.size());
+        for(std::size_t i = 0; i < tasks.size(); ++i) {
+            const PermuteTask& s = cp->plan.tasks[i];
+            xh_permute_task& d = tasks[i];
+            std::memset(&d, 0, sizeof(d));
+            d.dst_off = s.dst_off;
+            d.dst_ld = s.dst_ld;
+            d.rows = s.rows;
+            d.cols = s.cols;
+            d.nsrc = s.nsrc;
+            d.src_begin = s.src_begin;
+            for(int k2 = 0; k2 < cp->plan.nlegs; ++k2) {
+                d.ext[k2] = s.ext[k2];
+                d.sstride[k2] = s.sstride[k2];
+                d.is_col[k2] = s.is_col[k2];
+            }
+        }
+        std::vector<xh_permute_source> srcs(cp->plan.sources.size());
+        for(std::size_t i = 0; i < srcs.size(); ++i)
+            srcs[i] = xh_permute_source{cp->plan.sources[i].src_off, cp->plan.sources[i].src_ld, cp->plan.sources[i].coeff};
+        xh_permute_desc desc{cp->plan.nlegs, cp->plan.new_rank, static_cast<int64_t>(tasks.size()), static_cast<int64_t>(srcs.size()), tasks.data(),
+                             srcs.data()};
+        dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
+    }
+    rt->permute_cache.emplace(k, cp);
+    return cp;
+}
+
+DTensor permute(const DTensor& t, int shift, const std::vector<int>& p)
+{
+    // identity permutation without shift returns the tensor itself (Tensor.cpp:344)
+    bool ident = (shift == 0);
+    for(std::size_t i = 0; ident && i < p.size(); ++i) ident = (p[i] == static_cast<int>(i));
+    if(ident && static_cast<int>(p.size()) == t.st->rank + t.st->corank) return t;
+    auto cp = get_permute_plan(t.rt, *t.st, shift, p);
+    DTensor out = DTensor::empty(t.rt, cp->plan.out);
+    if(t.rt->ctx && out.size() > 0) {
+        dcheck(t.rt->ctx, xh_permute(t.rt->ctx, cp->dev, t.data(), out.data()), "permute");
+        double b = 0;
+        xh_permute_plan_bytes(cp->dev, &b);
+        t.rt->permute_bytes += b;
+        t.rt->n_permute++;
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ operator* and contract
+
+static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStructure& l, const TensorStructure& r)
+{
+    std::string k = structure_key(l);
+    k += "|";
+    k += structure_key(r);
+    auto it = rt->gemm_cache.find(k);
+    if(it != rt->gemm_cache.end()) return it->second;
+    auto cg = std::make_shared<CachedGemm>();
+    cg->plan = make_gemm_plan(l, r);
+    for(const auto& g : cg->plan.items) {
+        const std::int64_t mk = static_cast<std::int64_t>(g.m) * g.k, kn = static_cast<std::int64_t>(g.k) * g.n, mn = static_cast<std::int64_t>(g.m) * g.n;
+        if(g.a_off < 0 || g.a_off + mk > static_cast<std::int64_t>(l.size) || g.b_off < 0 || g.b_off + kn > static_cast<std::int64_t>(r.size) || g.c_off < 0 ||
+           g.c_off + mn > static_cast<std::int64_t>(cg->plan.out->size))
+            throw Error(XH_EINVAL, "gemm plan: block out of bounds");
+    }
+    {
+        std::string sk = legs_key(cg->plan.out->sym, cg->plan.out->unc_dom, cg->plan.out->unc_cod);
+        auto si = rt->struct_cache.find(sk);
+        if(si == rt->struct_cache.end())
+            rt->struct_cache.emplace(sk, cg->plan.out);
+        else
+            cg->plan.out = si->second;
+    }
+    const TensorStructure& o = *cg->plan.out;
+    std::vector<char> written(o.sector.size(), 0);
+    for(const auto& g : cg->plan.items) {
+        for(std::size_t s = 0; s < o.sector.size(); ++s)
+            if(static_cast<std::int64_t>(o.offsets[s]) == g.c_off) written[s] = 1;
+    }
+    for(std::size_t s = 0; s < o.sector.size(); ++s)
+        if(!written[s] && o.rows[s] * o.cols[s] > 0) {
+            xh_copy_rect r{0, 0, static_cast<int64_t>(o.offsets[s]), static_cast<int64_t>(o.rows[s]), static_cast<int32_t>(o.rows[s]),
+                           static_cast<int32_t>(o.cols[s]), 0.0, 0};
+            cg->zero_rects.push_back(r);
+        }
+    if(rt->ctx) {
+        std::vector<xh_gemm_item> items;
+        for(const auto& g : cg->plan.items) items.push_back(xh_gemm_item{g.a_off, g.b_off, g.c_off, g.m, g.n, g.k});
+        dcheck(rt->ctx, xh_gemm_plan_create(rt->ctx, items.data(), static_cast<int64_t>(items.size()), 0, 0, &cg->dev), "gemm plan");
+    }
+    rt->gemm_cache.emplace(k, cg);
+    return cg;
+}
+
+DTensor mul(const DTensor& l, const DTensor& r, int precision)
+{
+    auto cg = get_gemm_plan(l.rt, *l.st, *r.st);
+    DTensor out = DTensor::empty(l.rt, cg->plan.out);
+    if(l.rt->ctx && out.size() > 0) {
+        if(!cg->zero_rects.empty()) {
+            // sectors without a matching pair stay zero (Tout.setZero(), TensorBase.cpp:256-258)
+            for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
+        }
+        dcheck(l.rt->ctx, xh_gemm(l.rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision), "grouped gemm");
+        l.rt->gemm_flops += cg->plan.flops;
+        l.rt->n_gemm++;
+    }
+    return out;
+}
+
+DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision)
+{
+    // Tensor::contract (Tensor.hpp:328-351)
+    if(static_cast<int>(a1.size()) != a.st->rank + a.st->corank || static_cast<int>(a2.size()) != b.st->rank + b.st->corank)
+        throw Error(XH_EINVAL, "contract: index list length does not match the tensor rank");
+    ContractPerms cp = get_permutations(a1, a.st->rank, a2, b.st->rank, rankres);
+    DTensor pa = permute(a, cp.shift1, cp.p1);
+    DTensor pb = permute(b, cp.shift2, cp.p2);
+    DTensor prod = mul(pa, pb, precision);
+    return permute(prod, cp.shiftres, cp.pres);
+}
+
+// ------------------------------------------------------------------ elementwise
+
+DTensor adjoint(const DTensor& t)
+{
+    // AdjointOp + Tensor(TensorBase) (AdjointOp.hpp:25-69, Tensor.hpp:446-452): bases swapped
+    // without conjugation, sectors in the original order, every block transposed.
+    auto st = TensorStructure::create_pushed(t.st->sym, t.st->unc_cod, t.st->unc_dom, t.st->sector);
+    DTensor out = DTensor::empty(t.rt, st);
+    if(t.rt->ctx && out.size() > 0) {
+        std::vector<xh_copy_rect> rects;
+        for(std::size_t s = 0; s < st->sector.size(); ++s) {
+            // out block s is cols_in x rows_in
+            rects.push_back(xh_copy_rect{static_cast<int64_t>(t.st->offsets[s]), static_cast<int64_t>(t.st->rows[s]), static_cast<int64_t>(st->offsets[s]),
+                                         static_cast<int64_t>(st->rows[s]), static_cast<int32_t>(st->rows[s]), static_cast<int32_t>(st->cols[s]), 1.0, 1});
+        }
+        dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, t.data(), out.data(), rects.data(), static_cast<int64_t>(rects.size())), "adjoint");
+    }
+    return out;
+}
+
+static DTensor copy_of(const DTensor& t)
+{
+    DTensor out = DTensor::empty(t.rt, t.st);
+    if(t.rt->ctx && out.size() > 0) dcheck(t.rt->ctx, xh_axpby(t.rt->ctx, out.data(), t.data(), t.size(), 1.0, 0.0), "copy");
+    return out;
+}
+
+DTensor twist(const DTensor& t, int leg)
+{
+    // Tensor::twist (Tensor.cpp:357-377): identity for bosonic symmetries
+    if(!t.st->sym.any_fermionic()) return t;
+    const TensorStructure& s = *t.st;
+    if(leg < 0 || leg >= s.rank + s.corank) throw Error(XH_EINVAL, "twist: leg out of range");
+    DTensor out = copy_of(t);
+    std::vector<xh_rect> rects;
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        std::size_t ro = 0;
+        for(const auto& t1 : s.dom.tree(s.sector[i])) {
+            std::size_t co = 0;
+            for(const auto& t2 : s.cod.tree(s.sector[i])) {
+                double c = leg < s.rank ? s.sym.coeff_twist(t1.unc[leg]) : s.sym.coeff_twist(t2.unc[leg - s.rank]);
+                if(c != 1.0)
+                    rects.push_back(xh_rect{static_cast<int64_t>(s.offsets[i] + ro + co * s.rows[i]), static_cast<int64_t>(s.rows[i]),
+                                            static_cast<int32_t>(t1.dim), static_cast<int32_t>(t2.dim), c});
+                co += t2.dim;
+            }
+            ro += t1.dim;
+        }
+    }
+    if(t.rt->ctx && !rects.empty()) dcheck(t.rt->ctx, xh_scale_rects(t.rt->ctx, out.data(), rects.data(), static_cast<int64_t>(rects.size())), "twist");
+    return out;
+}
+
+DTensor scaled(const DTensor& t, double alpha)
+{
+    DTensor out = DTensor::empty(t.rt, t.st);
+    if(t.rt->ctx && out.size() > 0) dcheck(t.rt->ctx, xh_axpby(t.rt->ctx, out.data(), t.data(), t.size(), alpha, 0.0), "scale");
+    return out;
+}
+
+DTensor axpby(const DTensor& x, const DTensor& y, double a, double b)
+{
+    if(structure_key(*x.st) != structure_key(*y.st)) throw Error(XH_EINVAL, "binary op: tensors have different structure");
+    DTensor out = copy_of(y);
+    if(x.rt->ctx && out.size() > 0) dcheck(x.rt->ctx, xh_axpby(x.rt->ctx, out.data(), x.data(), x.size(), a, b), "axpby");
+    return out;
+}
+
+DTensor diag_map(const DTensor& t, int mode)
+{
+    DTensor out = copy_of(t);
+    std::vector<int64_t> off;
+    std::vector<int32_t> m;
+    for(std::size_t i = 0; i < t.st->sector.size(); ++i) {
+        if(t.st->rows[i] != t.st->cols[i]) throw Error(XH_EINVAL, "diagonal map needs square blocks");
+        off.push_back(static_cast<int64_t>(t.st->offsets[i]));
+        m.push_back(static_cast<int32_t>(t.st->rows[i]));
+    }
+    if(t.rt->ctx && !off.empty()) dcheck(t.rt->ctx, xh_block_diag_map(t.rt->ctx, off.data(), m.data(), static_cast<int64_t>(off.size()), out.data(), mode), "diag map");
+    return out;
+}
+
+double trace(const DTensor& t)
+{
+    t.rt->require_device();
+    std::vector<int64_t> off;
+    std::vector<int32_t> m;
+    for(std::size_t i = 0; i < t.st->sector.size(); ++i) {
+        if(t.st->rows[i] != t.st->cols[i]) throw Error(XH_EINVAL, "trace needs square blocks");
+        off.push_back(static_cast<int64_t>(t.st->offsets[i]));
+        m.push_back(static_cast<int32_t>(t.st->rows[i]));
+    }
+    std::vector<double> tr(off.size());
+    if(!off.empty()) dcheck(t.rt->ctx, xh_block_trace(t.rt->ctx, off.data(), m.data(), static_cast<int64_t>(off.size()), t.data(), tr.data()), "trace");
+    double out = 0.0;
+    for(std::size_t i = 0; i < tr.size(); ++i) out += tr[i] * static_cast<double>(t.st->sym.degeneracy(t.st->sector[i]));
+    return out;
+}
+
+double squared_norm(const DTensor& t)
+{
+    // (T * T^dagger).trace(): sum over blocks of degeneracy * |block|_F^2
+    t.rt->require_device();
+    double out = 0.0;
+    for(std::size_t i = 0; i < t.st->sector.size(); ++i) {
+        double d = 0;
+        const std::size_t n = t.st->rows[i] * t.st->cols[i];
+        if(n) dcheck(t.rt->ctx, xh_dot(t.rt->ctx, t.data() + t.st->offsets[i], t.data() + t.st->offsets[i], n, &d), "squaredNorm");
+        out += d * static_cast<double>(t.st->sym.degeneracy(t.st->sector[i]));
+    }
+    return out;
+}
+
+double max_norm(const DTensor& t)
+{
+    t.rt->require_device();
+    double out = 0.0;
+    if(t.size()) dcheck(t.rt->ctx, xh_max_abs(t.rt->ctx, t.data(), t.size(), &out), "maxNorm");
+    return out;
+}
+
+// ------------------------------------------------------------------ tSVD
+
+TruncRule truncation_rule(const Sym& s, std::vector<std::pair<Q, double>> allSV, std::size_t maxKeep, double eps_svd, bool preserve_multiplets,
+                          double truncWeight, double entropy)
+{
+    TruncRule out;
+    const std::size_t numberOfStates = allSV.size();
+    if(numberOfStates == 0) throw Error(XH_EINVAL, "tSVD of a tensor without sectors");
+    const auto first_entry = allSV[0];
+    std::sort(allSV.begin(), allSV.end(), [](const std::pair<Q, double>& a, const std::pair<Q, double>& b) { return a.second > b.second; });
+    for(std::size_t i = maxKeep; i < allSV.size(); ++i) truncWeight += s.degeneracy(allSV[i].first) * std::pow(std::abs(allSV[i].second), 2.);
+    allSV.resize(std::min(maxKeep, numberOfStates));
+    if(eps_svd > 0.) {
+        allSV.erase(std::remove_if(allSV.begin(), allSV.end(), [eps_svd](const std::pair<Q, double>& sv) { return sv.second < eps_svd; }),
+                    allSV.end());
+    }
+    if(preserve_multiplets) {
+        int endOfMultiplet = -1;
+        for(int i = static_cast<int>(allSV.size()) - 1; i > 0; i--) {
+            double rel_diff = 2 * (allSV[i - 1].second - allSV[i].second) / (allSV[i - 1].second + allSV[i].second);
+            if(rel_diff > 0.1) {
+                endOfMultiplet = i;
+                break;
+            }
+        }
+        if(endOfMultiplet != -1) allSV.resize(endOfMultiplet);
+    }
+    if(allSV.empty()) allSV.push_back(first_entry);
+    for(const auto& [q, sv] : allSV) {
+        out.kept[q].push_back(sv);
+        entropy += -s.degeneracy(q) * sv * sv * std::log(sv * sv);
+    }
+    out.trunc_weight = truncWeight;
+    out.entropy = entropy;
+    return out;
+}
+
+FullSVD full_svd(const DTensor& t)
+{
+    t.rt->require_device();
+    const TensorStructure& s = *t.st;
+    Qbasis middle(s.sym, 1);
+    for(std::size_t i = 0; i < s.sector.size(); ++i) middle.push_back(s.sector[i], std::min(s.rows[i], s.cols[i]));
+    FullSVD f;
+    f.sectors = s.sector;
+    // U(uncoupledDomain, {middle}), Sigma({middle},{middle}), Vdag({middle}, uncoupledCodomain), filled by push_back
+    f.U = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {middle}, s.sector));
+    f.S = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {middle}, {middle}, s.sector));
+    f.Vdag = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {middle}, s.unc_cod, s.sector));
+    std::vector<xh_factor_item> items;
+    std::size_t nsv = 0;
+    std::vector<std::size_t> sv_off;
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        sv_off.push_back(nsv);
+        nsv += std::min(s.rows[i], s.cols[i]);
+    }
+    double* d_sv = nullptr;
+    if(nsv) dcheck(t.rt->ctx, xh_alloc(t.rt->ctx, nsv * sizeof(double), reinterpret_cast<void**>(&d_sv)), "alloc");
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        items.push_back(xh_factor_item{static_cast<int64_t>(s.offsets[i]), static_cast<int64_t>(f.U.st->offsets[i]), static_cast<int64_t>(sv_off[i]),
+                                       static_cast<int64_t>(f.Vdag.st->offsets[i]), static_cast<int32_t>(s.rows[i]), static_cast<int32_t>(s.cols[i])});
+    }
+    if(!items.empty()) dcheck(t.rt->ctx, xh_block_svd(t.rt->ctx, items.data(), static_cast<int64_t>(items.size()), t.data(), f.U.data(), d_sv, f.Vdag.data()), "block svd");
+    std::vector<double> sv(nsv);
+    if(nsv) dcheck(t.rt->ctx, xh_download(t.rt->ctx, sv.data(), d_sv, nsv * sizeof(double)), "download singular values");
+    if(d_sv) xh_free(t.rt->ctx, d_sv);
+    // Sigma blocks: diagonal matrices
+    std::vector<double> hs(f.S.size(), 0.0);
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        const std::size_t r = std::min(s.rows[i], s.cols[i]);
+        for(std::size_t k = 0; k < r; ++k) {
+            hs[f.S.st->offsets[i] + k + k * r] = sv[sv_off[i] + k];
+            f.all_sv.emplace_back(s.sector[i], sv[sv_off[i] + k]);
+        }
+    }
+    if(f.S.size()) f.S.upload(hs.data(), hs.size());
+    return f;
+}
+
+SVDResult cutoff(const DTensor& t, const FullSVD& f, std::size_t maxKeep, double eps_svd, bool preserve_multiplets, double trunc_weight_in,
+                 double entropy_in)
+{
+    const TensorStructure& s = *t.st;
+    TruncRule rule = truncation_rule(s.sym, f.all_sv, maxKeep, eps_svd, preserve_multiplets, trunc_weight_in, entropy_in);
+    Qbasis truncBasis(s.sym, 1);
+    std::vector<Q> order;
+    for(const auto& [q, v] : rule.kept) {
+        truncBasis.push_back(q, v.size());
+        order.push_back(q);
+    }
+    SVDResult res;
+    res.trunc_weight = rule.trunc_weight;
+    res.entropy = rule.entropy;
+    res.spec = rule.kept;
+    res.n_states = f.all_sv.size();
+    res.U = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {truncBasis}, order));
+    res.S = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {truncBasis}, {truncBasis}, order));
+    res.Vdag = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {truncBasis}, s.unc_cod, order));
+    std::vector<xh_copy_rect> ru, rv;
+    std::vector<double> hs(res.S.size(), 0.0);
+    for(std::size_t o = 0; o < order.size(); ++o) {
+        const Q& q = order[o];
+        const int i = s.find_sector(q);
+        const std::size_t nret = rule.kept[q].size();
+        const std::size_t r = std::min(s.rows[i], s.cols[i]);
+        ru.push_back(xh_copy_rect{static_cast<int64_t>(f.U.st->offsets[i]), static_cast<int64_t>(s.rows[i]), static_cast<int64_t>(res.U.st->offsets[o]),
+                                  static_cast<int64_t>(s.rows[i]), static_cast<int32_t>(s.rows[i]), static_cast<int32_t>(nret), 1.0, 0});
+        rv.push_back(xh_copy_rect{static_cast<int64_t>(f.Vdag.st->offsets[i]), static_cast<int64_t>(r), static_cast<int64_t>(res.Vdag.st->offsets[o]),
+                                  static_cast<int64_t>(nret), static_cast<int32_t>(nret), static_cast<int32_t>(s.cols[i]), 1.0, 0});
+        for(std::size_t k = 0; k < nret; ++k) hs[res.S.st->offsets[o] + k + k * nret] = rule.kept[q][k];
+    }
+    if(!ru.empty()) {
+        dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, f.U.data(), res.U.data(), ru.data(), static_cast<int64_t>(ru.size())), "truncate U");
+        dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, f.Vdag.data(), res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
+    }
+    if(res.S.size()) res.S.upload(hs.data(), hs.size());
+    return res;
+}
+
+SVDResult tsvd(const DTensor& t, std::size_t maxKeep, double eps_svd, bool preserve_multiplets, double trunc_weight_in, double entropy_in)
+{
+    FullSVD f = full_svd(t);
+    return cutoff(t, f, maxKeep, eps_svd, preserve_multiplets, trunc_weight_in, entropy_in);
+}
+
+std::pair<DTensor, DTensor> tqr(const DTensor& t)
+{
+    t.rt->require_device();
+    const TensorStructure& s = *t.st;
+    Qbasis middle = s.dom.dim() > s.cod.dim() ? s.cod.forget_history() : s.dom.forget_history();
+    DTensor Qt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {middle}, s.sector));
+    DTensor Rt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {middle}, s.unc_cod, s.sector));
+    std::vector<xh_factor_item> items;
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        const std::size_t mid = middle.inner_dim(s.sector[i]);
+        items.push_back(xh_factor_item{static_cast<int64_t>(s.offsets[i]), static_cast<int64_t>(Qt.st->offsets[i]), static_cast<int64_t>(mid),
+                                       static_cast<int64_t>(Rt.st->offsets[i]), static_cast<int32_t>(s.rows[i]), static_cast<int32_t>(s.cols[i])});
+    }
+    if(!items.empty()) dcheck(t.rt->ctx, xh_block_qr(t.rt->ctx, items.data(), static_cast<int64_t>(items.size()), t.data(), Qt.data(), Rt.data()), "block qr");
+    return {Qt, Rt};
+}
+
+std::pair<DTensor, DTensor> teigh(const DTensor& t)
+{
+    t.rt->require_device();
+    const TensorStructure& s = *t.st;
+    if(s.rank != s.corank) throw Error(XH_EINVAL, "teigh needs Rank == CoRank");
+    Qbasis dh = s.dom.forget_history(), ch = s.cod.forget_history();
+    DTensor D = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {dh}, {ch}, s.sector));
+    DTensor V = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {ch}, s.sector));
+    std::vector<xh_factor_item> items;
+    std::size_t nev = 0;
+    std::vector<std::size_t> ev_off;
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        if(s.rows[i] != s.cols[i]) throw Error(XH_EINVAL, "teigh: non-square block");
+        ev_off.push_back(nev);
+        nev += s.rows[i];
+        items.push_back(xh_factor_item{static_cast<int64_t>(s.offsets[i]), static_cast<int64_t>(V.st->offsets[i]), static_cast<int64_t>(ev_off.back()), 0,
+                                       static_cast<int32_t>(s.rows[i]), static_cast<int32_t>(s.cols[i])});
+    }
+    double* d_ev = nullptr;
+    if(nev) dcheck(t.rt->ctx, xh_alloc(t.rt->ctx, nev * sizeof(double), reinterpret_cast<void**>(&d_ev)), "alloc");
+    if(!items.empty()) dcheck(t.rt->ctx, xh_block_eigh(t.rt->ctx, items.data(), static_cast<int64_t>(items.size()), t.data(), d_ev, V.data()), "block eigh");
+    std::vector<double> ev(nev);
+    if(nev) dcheck(t.rt->ctx, xh_download(t.rt->ctx, ev.data(), d_ev, nev * sizeof(double)), "download eigenvalues");
+    if(d_ev) xh_free(t.rt->ctx, d_ev);
+    std::vector<double> hd(D.size(), 0.0);
+    for(std::size_t i = 0; i < s.sector.size(); ++i)
+        for(std::size_t k = 0; k < s.rows[i]; ++k) hd[D.st->offsets[i] + k + k * s.rows[i]] = ev[ev_off[i] + k];
+    if(D.size()) D.upload(hd.data(), hd.size());
+    return {D, V};
+}
+
+}  // namespace xa

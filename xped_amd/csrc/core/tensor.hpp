@@ -1,0 +1,136 @@
+// Device-resident block-sparse tensor and the operations of the reference's Xped::Tensor
+// (include/Xped/Core/Tensor.hpp:59-444, TensorBase.hpp:47-135) on top of the HIP device layer.
+#pragma once
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/xped_hip.h"
+#include "structure.hpp"
+
+namespace xa {
+
+// Error carrying an xh status code (XH_EINVAL, XH_ENOMEM, XH_EHIP, XH_ENOTCONV, XH_ENODEV).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Owns a device context; buffers and the runtime keep it alive until the last one is gone.
+struct CtxHolder {
+    xh_ctx* ctx = nullptr;
+    ~CtxHolder()
+    {
+        if(ctx) xh_destroy(ctx);
+    }
+};
+
+struct DeviceBuffer {
+    std::shared_ptr<CtxHolder> holder;
+    double* ptr = nullptr;
+    std::size_t n = 0;
+    ~DeviceBuffer()
+    {
+        if(holder && holder->ctx && ptr) xh_free(holder->ctx, ptr);
+    }
+};
+
+struct CachedPermute {
+    PermutePlan plan;
+    xh_permute_plan* dev = nullptr;
+};
+struct CachedGemm {
+    GemmPlan plan;
+    xh_gemm_plan* dev = nullptr;
+    std::vector<xh_copy_rect> zero_rects;  // output sectors not written by any GEMM
+};
+
+// Device context, plan caches and counters.  ctx == nullptr is a structure-only runtime: every
+// integer/structure operation works, data operations fail with XH_ENODEV.
+struct Runtime {
+    std::shared_ptr<CtxHolder> holder;
+    xh_ctx* ctx = nullptr;
+    std::unordered_map<std::string, std::shared_ptr<CachedPermute>> permute_cache;
+    std::unordered_map<std::string, std::shared_ptr<CachedGemm>> gemm_cache;
+    std::unordered_map<std::string, StructPtr> struct_cache;
+    // counters for reporting
+    double gemm_flops = 0.0, permute_bytes = 0.0;
+    long n_gemm = 0, n_permute = 0;
+    ~Runtime();
+    void require_device() const;
+};
+
+std::string structure_key(const TensorStructure& t);
+std::string legs_key(const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c);
+StructPtr cached_structure(Runtime* rt, const Sym& s, const std::vector<Qbasis>& d, const std::vector<Qbasis>& c);
+
+class DTensor {
+public:
+    StructPtr st;
+    std::shared_ptr<DeviceBuffer> buf;
+    Runtime* rt = nullptr;
+
+    DTensor() = default;
+    double* data() const { return buf ? buf->ptr : nullptr; }
+    std::size_t size() const { return st ? st->size : 0; }
+
+    static DTensor empty(Runtime* rt, StructPtr st);  // uninitialised data
+    static DTensor zeros(Runtime* rt, StructPtr st);  // setZero
+    void upload(const double* host, std::size_t n);
+    void download(double* host, std::size_t n) const;
+    // setRandom (Tensor.cpp:32-43, MatrixInterface_Eigen_impl.cpp:51-64): one engine, blocks in
+    // storage order, column-major within a block.  dist 0: uniform_real_distribution(-1,1),
+    // dist 1: normal_distribution(0,1).
+    void set_random(std::mt19937& engine, int dist);
+};
+
+DTensor permute(const DTensor& t, int shift, const std::vector<int>& p);
+std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p);
+DTensor mul(const DTensor& l, const DTensor& r, int precision);
+DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision);
+DTensor adjoint(const DTensor& t);
+DTensor twist(const DTensor& t, int leg);
+DTensor scaled(const DTensor& t, double alpha);
+DTensor axpby(const DTensor& x, const DTensor& y, double a, double b);  // a*x + b*y, same structure
+DTensor diag_map(const DTensor& t, int mode);                          // 0 sqrt, 1 inv, 2 inv sqrt of diagonal blocks
+double trace(const DTensor& t);                                         // degeneracy-weighted (TensorBase.cpp:26-37)
+double squared_norm(const DTensor& t);                                  // (T * T^dagger).trace()
+double max_norm(const DTensor& t);                                      // TensorBase.cpp:48-56
+
+struct SVDResult {
+    DTensor U, S, Vdag;
+    double trunc_weight = 0.0, entropy = 0.0;
+    std::map<Q, std::vector<double>> spec;  // kept singular values per sector (RETURN_SPEC)
+    std::size_t n_states = 0;               // number of singular values before truncation
+};
+// Tensor::tSVD (Tensor.cpp:478-633).  trunc_weight and entropy are accumulated into the values
+// passed in, as the reference does.
+SVDResult tsvd(const DTensor& t, std::size_t max_keep, double eps_svd, bool preserve_multiplets, double trunc_weight_in = 0.0,
+               double entropy_in = 0.0);
+// full_svd_impl alone: untruncated U, Sigma, Vdag plus all singular values in sector order
+struct FullSVD {
+    DTensor U, S, Vdag;
+    std::vector<std::pair<Q, double>> all_sv;
+    std::vector<Q> sectors;
+};
+FullSVD full_svd(const DTensor& t);
+SVDResult cutoff(const DTensor& t, const FullSVD& f, std::size_t max_keep, double eps_svd, bool preserve_multiplets, double trunc_weight_in,
+                 double entropy_in);
+std::pair<DTensor, DTensor> tqr(const DTensor& t);    // Tensor.cpp:757-780 (QR, RETURN_LQ = false)
+std::pair<DTensor, DTensor> teigh(const DTensor& t);  // Tensor.cpp:782-803
+
+// Host-side truncation rule of cutoff_matrices (Tensor.cpp:523-633) on an explicit list of
+// (sector, singular value) pairs.  Returns the kept count per sector (ascending q).
+struct TruncRule {
+    std::map<Q, std::vector<double>> kept;
+    double trunc_weight = 0.0, entropy = 0.0;
+};
+TruncRule truncation_rule(const Sym& s, std::vector<std::pair<Q, double>> all_sv, std::size_t max_keep, double eps_svd, bool preserve_multiplets,
+                          double trunc_weight_in, double entropy_in);
+
+}  // namespace xa

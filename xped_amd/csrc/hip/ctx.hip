@@ -1,0 +1,385 @@
+// Context, buffers, events and the elementwise / reduction kernels of the device layer.
+#include <algorithm>
+#include <cstring>
+
+#include "device.hpp"
+
+using xh::check;
+using xh::guard;
+
+extern "C" int xh_device_count(int* count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if(e != hipSuccess) n = 0;
+    *count = n;
+    return n > 0 ? XH_OK : XH_ENODEV;
+}
+
+extern "C" int xh_create(int device, xh_ctx** out)
+{
+    *out = nullptr;
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess || n <= device) return XH_ENODEV;
+    auto* ctx = new xh_ctx();
+    int rc = guard(ctx, [&] {
+        ctx->device = device;
+        check(hipSetDevice(device), "hipSetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        ctx->num_cu = prop.multiProcessorCount;
+        // default priority stream, not synchronising with the null stream
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for(auto& ev : ctx->events) check(hipEventCreate(&ev), "hipEventCreate");
+        hipMemPool_t pool;
+        check(hipDeviceGetDefaultMemPool(&pool, device), "hipDeviceGetDefaultMemPool");
+        uint64_t thr = UINT64_MAX;
+        check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr), "hipMemPoolSetAttribute");
+        check(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scalar), 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        check(hipMalloc(reinterpret_cast<void**>(&ctx->dev_scalar), 64 * sizeof(double)), "hipMalloc");
+    });
+    if(rc != XH_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return XH_OK;
+}
+
+extern "C" int xh_destroy(xh_ctx* ctx)
+{
+    if(!ctx) return XH_OK;
+    hipSetDevice(ctx->device);
+    if(ctx->stream) hipStreamSynchronize(ctx->stream);
+    for(auto& ev : ctx->events)
+        if(ev) hipEventDestroy(ev);
+    if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
+    if(ctx->dev_scalar) hipFree(ctx->dev_scalar);
+    if(ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return XH_OK;
+}
+
+extern "C" const char* xh_last_error(const xh_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int xh_alloc(xh_ctx* ctx, size_t bytes, void** dptr)
+{
+    return guard(ctx, [&] {
+        *dptr = nullptr;
+        if(bytes == 0) return;
+        check(hipMallocAsync(dptr, bytes, ctx->stream), "hipMallocAsync");
+    });
+}
+
+extern "C" int xh_free(xh_ctx* ctx, void* dptr)
+{
+    return guard(ctx, [&] {
+        if(dptr) check(hipFreeAsync(dptr, ctx->stream), "hipFreeAsync");
+    });
+}
+
+extern "C" int xh_upload(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return guard(ctx, [&] {
+        if(bytes == 0) return;
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync H2D");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+extern "C" int xh_download(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return guard(ctx, [&] {
+        if(bytes == 0) return;
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync D2H");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+extern "C" int xh_synchronize(xh_ctx* ctx)
+{
+    return guard(ctx, [&] { check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+}
+
+extern "C" void* xh_stream(xh_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
+
+extern "C" int xh_event_record(xh_ctx* ctx, int slot)
+{
+    return guard(ctx, [&] {
+        if(slot < 0 || slot >= 16) throw xh::Error(XH_EINVAL, "event slot out of range");
+        check(hipEventRecord(ctx->events[slot], ctx->stream), "hipEventRecord");
+    });
+}
+
+extern "C" int xh_event_elapsed_ms(xh_ctx* ctx, int s0, int s1, float* ms)
+{
+    return guard(ctx, [&] {
+        if(s0 < 0 || s0 >= 16 || s1 < 0 || s1 >= 16) throw xh::Error(XH_EINVAL, "event slot out of range");
+        check(hipEventSynchronize(ctx->events[s1]), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, ctx->events[s0], ctx->events[s1]), "hipEventElapsedTime");
+    });
+}
+
+// ------------------------------------------------------------------ elementwise kernels
+
+namespace {
+
+constexpr int EW_THREADS = 256;
+
+inline unsigned ew_grid(size_t n, int per_thread = 4)
+{
+    size_t g = (n + static_cast<size_t>(EW_THREADS) * per_thread - 1) / (static_cast<size_t>(EW_THREADS) * per_thread);
+    return static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(g, 65535u * 16u)));
+}
+
+__global__ void k_fill(double* __restrict__ x, size_t n, double v)
+{
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = v;
+}
+
+__global__ void k_scale(double* __restrict__ x, size_t n, double a)
+{
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = a * x[i];
+}
+
+__global__ void k_axpby(double* __restrict__ y, const double* __restrict__ x, size_t n, double a, double b)
+{
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        y[i] = __dadd_rn(__dmul_rn(a, x[i]), __dmul_rn(b, y[i]));
+}
+
+// Two-stage reductions: per-block partials into scratch, then a single block folds them.
+__device__ inline double wave_max(double v)
+{
+    for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline double wave_sum(double v)
+{
+    for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ void k_maxabs(const double* __restrict__ x, size_t n, double* __restrict__ part)
+{
+    double m = 0.0;
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = fmax(m, fabs(x[i]));
+    m = wave_max(m);
+    __shared__ double s[EW_THREADS / 64];
+    if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double r = s[0];
+        for(int w = 1; w < EW_THREADS / 64; ++w) r = fmax(r, s[w]);
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ void k_dot(const double* __restrict__ x, const double* __restrict__ y, size_t n, double* __restrict__ part)
+{
+    double acc = 0.0;
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) acc = fma(x[i], y[i], acc);
+    acc = wave_sum(acc);
+    __shared__ double s[EW_THREADS / 64];
+    if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double r = 0;
+        for(int w = 0; w < EW_THREADS / 64; ++w) r += s[w];
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ void k_fold(const double* __restrict__ part, int n, double* __restrict__ out, int is_max)
+{
+    double acc = 0.0;
+    for(int i = threadIdx.x; i < n; i += blockDim.x) acc = is_max ? fmax(acc, part[i]) : acc + part[i];
+    acc = is_max ? wave_max(acc) : wave_sum(acc);
+    __shared__ double s[EW_THREADS / 64];
+    if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double r = s[0];
+        for(int w = 1; w < EW_THREADS / 64; ++w) r = is_max ? fmax(r, s[w]) : r + s[w];
+        *out = r;
+    }
+}
+
+__global__ void k_block_trace(const int64_t* __restrict__ off, const int32_t* __restrict__ m, const double* __restrict__ x, double* __restrict__ out)
+{
+    const int g = blockIdx.x;
+    const int64_t o = off[g];
+    const int mm = m[g];
+    double acc = 0.0;
+    for(int i = threadIdx.x; i < mm; i += blockDim.x) acc += x[o + int64_t(i) * mm + i];
+    acc = wave_sum(acc);
+    __shared__ double s[EW_THREADS / 64];
+    if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double r = 0;
+        for(int w = 0; w < EW_THREADS / 64; ++w) r += s[w];
+        out[g] = r;
+    }
+}
+
+__global__ void k_scale_rects(double* __restrict__ x, const xh_rect* __restrict__ rects)
+{
+    const xh_rect r = rects[blockIdx.x];
+    const int64_t n = int64_t(r.rows) * r.cols;
+    for(int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        int64_t i = e % r.rows, j = e / r.rows;
+        double* p = x + r.off + i + j * r.ld;
+        *p = r.factor * *p;
+    }
+}
+
+__global__ void k_diag_map(const int64_t* __restrict__ off, const int32_t* __restrict__ m, double* __restrict__ x, int mode)
+{
+    const int g = blockIdx.x;
+    const int64_t o = off[g];
+    const int mm = m[g];
+    for(int i = threadIdx.x; i < mm; i += blockDim.x) {
+        double* p = x + o + int64_t(i) * mm + i;
+        double v = *p;
+        if(mode == 0)
+            v = sqrt(v);
+        else if(mode == 1)
+            v = 1.0 / v;
+        else
+            v = 1.0 / sqrt(v);
+        *p = v;
+    }
+}
+
+__global__ void k_copy_rects(const double* __restrict__ src, double* __restrict__ dst, const xh_copy_rect* __restrict__ rects)
+{
+    const xh_copy_rect r = rects[blockIdx.x];
+    const int64_t n = int64_t(r.rows) * r.cols;
+    for(int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        int64_t i = e % r.rows, j = e / r.rows;
+        double v = r.transpose ? src[r.src_off + j + i * r.src_ld] : src[r.src_off + i + j * r.src_ld];
+        dst[r.dst_off + i + j * r.dst_ld] = r.factor == 1.0 ? v : r.factor * v;
+    }
+}
+
+}  // namespace
+
+extern "C" int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value)
+{
+    return guard(ctx, [&] {
+        if(n == 0) return;
+        hipLaunchKernelGGL(k_fill, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, dst, n, value);
+        xh::launch_check("k_fill");
+    });
+}
+
+extern "C" int xh_scale(xh_ctx* ctx, double* x, size_t n, double a)
+{
+    return guard(ctx, [&] {
+        if(n == 0) return;
+        hipLaunchKernelGGL(k_scale, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, x, n, a);
+        xh::launch_check("k_scale");
+    });
+}
+
+extern "C" int xh_axpby(xh_ctx* ctx, double* y, const double* x, size_t n, double a, double b)
+{
+    return guard(ctx, [&] {
+        if(n == 0) return;
+        hipLaunchKernelGGL(k_axpby, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, y, x, n, a, b);
+        xh::launch_check("k_axpby");
+    });
+}
+
+static void reduce(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out, bool is_max)
+{
+    if(n == 0) {
+        *out = 0.0;
+        return;
+    }
+    const int nb = static_cast<int>(std::min<size_t>(32, (n + EW_THREADS - 1) / EW_THREADS));
+    if(is_max)
+        hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, n, ctx->dev_scalar);
+    else
+        hipLaunchKernelGGL(k_dot, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, y, n, ctx->dev_scalar);
+    xh::launch_check("reduce");
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(EW_THREADS), 0, ctx->stream, ctx->dev_scalar, nb, ctx->dev_scalar + 32, is_max ? 1 : 0);
+    xh::launch_check("fold");
+    check(hipMemcpyAsync(ctx->host_scalar, ctx->dev_scalar + 32, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    *out = ctx->host_scalar[0];
+}
+
+extern "C" int xh_max_abs(xh_ctx* ctx, const double* x, size_t n, double* out)
+{
+    return guard(ctx, [&] { reduce(ctx, x, nullptr, n, out, true); });
+}
+
+extern "C" int xh_dot(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out)
+{
+    return guard(ctx, [&] { reduce(ctx, x, y, n, out, false); });
+}
+
+extern "C" int xh_block_trace(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* x, double* out)
+{
+    return guard(ctx, [&] {
+        if(nblocks <= 0) return;
+        std::vector<int64_t> o(off, off + nblocks);
+        std::vector<int32_t> mm(m, m + nblocks);
+        int64_t* d_off = xh::dev_upload(ctx, o);
+        int32_t* d_m = xh::dev_upload(ctx, mm);
+        double* d_out = nullptr;
+        check(hipMallocAsync(reinterpret_cast<void**>(&d_out), nblocks * sizeof(double), ctx->stream), "hipMallocAsync");
+        hipLaunchKernelGGL(k_block_trace, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, d_off, d_m, x, d_out);
+        xh::launch_check("k_block_trace");
+        check(hipMemcpyAsync(out, d_out, nblocks * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d_off);
+        xh::dev_free(ctx, d_m);
+        xh::dev_free(ctx, d_out);
+    });
+}
+
+extern "C" int xh_scale_rects(xh_ctx* ctx, double* x, const xh_rect* rects, int64_t nrects)
+{
+    return guard(ctx, [&] {
+        if(nrects <= 0) return;
+        std::vector<xh_rect> r(rects, rects + nrects);
+        xh_rect* d = xh::dev_upload(ctx, r);
+        hipLaunchKernelGGL(k_scale_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, x, d);
+        xh::launch_check("k_scale_rects");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d);
+    });
+}
+
+extern "C" int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, double* x, int mode)
+{
+    return guard(ctx, [&] {
+        if(nblocks <= 0) return;
+        if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "diag map mode must be 0, 1 or 2");
+        std::vector<int64_t> o(off, off + nblocks);
+        std::vector<int32_t> mm(m, m + nblocks);
+        int64_t* d_off = xh::dev_upload(ctx, o);
+        int32_t* d_m = xh::dev_upload(ctx, mm);
+        hipLaunchKernelGGL(k_diag_map, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, d_off, d_m, x, mode);
+        xh::launch_check("k_diag_map");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d_off);
+        xh::dev_free(ctx, d_m);
+    });
+}
+
+extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects)
+{
+    return guard(ctx, [&] {
+        if(nrects <= 0) return;
+        std::vector<xh_copy_rect> r(rects, rects + nrects);
+        xh_copy_rect* d = xh::dev_upload(ctx, r);
+        hipLaunchKernelGGL(k_copy_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, d);
+        xh::launch_check("k_copy_rects");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d);
+    });
+}

@@ -1,0 +1,93 @@
+// Internal definitions shared by the HIP translation units of the device layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/xped_hip.h"
+
+struct xh_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t events[16] = {};
+    // small pinned host staging buffer for scalar reductions
+    double* host_scalar = nullptr;
+    double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
+};
+
+namespace xh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what)
+{
+    if(e != hipSuccess) {
+        int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? XH_ENOMEM : XH_EHIP;
+        throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// Runs f and converts exceptions into status codes stored on the context.
+template <typename F>
+int guard(xh_ctx* ctx, F&& f)
+{
+    try {
+        f();
+        return XH_OK;
+    } catch(const Error& e) {
+        if(ctx) ctx->err = e.what();
+        return e.code;
+    } catch(const std::bad_alloc& e) {
+        if(ctx) ctx->err = "host allocation failed";
+        return XH_ENOMEM;
+    } catch(const std::exception& e) {
+        if(ctx) ctx->err = e.what();
+        return XH_EINVAL;
+    }
+}
+
+inline void launch_check(const char* name) { check(hipGetLastError(), name); }
+
+// Round-up magic-number division for 0 <= x < 2^31 and 1 <= d < 2^31
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994).
+struct FastDiv {
+    uint32_t d, magic, shift;
+};
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f{d, 0, 0};
+    uint32_t l = 0;
+    while((uint64_t(1) << l) < d) ++l;
+    f.shift = l;
+    f.magic = static_cast<uint32_t>(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    return f;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t x, uint32_t magic, uint32_t shift)
+{
+    return (__umulhi(x, magic) + x) >> shift;
+}
+
+template <typename T>
+T* dev_upload(xh_ctx* ctx, const std::vector<T>& v)
+{
+    if(v.empty()) return nullptr;
+    T* p = nullptr;
+    check(hipMallocAsync(reinterpret_cast<void**>(&p), v.size() * sizeof(T), ctx->stream), "hipMallocAsync(plan)");
+    check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
+    return p;
+}
+
+inline void dev_free(xh_ctx* ctx, void* p)
+{
+    if(p) hipFreeAsync(p, ctx->stream);
+}
+
+}  // namespace xh

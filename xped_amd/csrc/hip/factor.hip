@@ -1,0 +1,487 @@
+// Batched per-sector factorisations in FP64: SVD (one-sided Jacobi), QR (Householder, with the
+// reflector convention of Eigen::HouseholderQR) and symmetric eigen (parallel cyclic Jacobi).
+//
+// One workgroup owns one sector block.  Blocks that fit the LDS budget are factorised entirely in
+// LDS; larger blocks use a global-memory working copy with the same algorithm.
+//
+// The reference calls Eigen per sector (PlainInterface::svd, PlainInterface_Eigen_impl.cpp:116-127;
+// MatrixInterface::qr, MatrixInterface_Eigen_impl.cpp:249-255; MatrixInterface::eigh,
+// MatrixInterface_Eigen_impl.hpp:242-247), serially over sectors (Tensor.cpp:510-519, 762-777,
+// 782-803).
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+
+#include "device.hpp"
+
+using xh::check;
+using xh::guard;
+
+namespace {
+
+constexpr int F_THREADS = 256;
+constexpr int MAX_SWEEPS = 60;
+
+__device__ __forceinline__ double group_sum(double v, int g)
+{
+    for(int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ int rr_player(int pos, int step, int n2)
+{
+    // circle method: position 0 is fixed, the others rotate by one each step
+    if(pos == 0) return 0;
+    return ((pos - 1 + step) % (n2 - 1)) + 1;
+}
+
+struct FactorArgs {
+    const xh_factor_item* items;
+    const double* A;
+    double* U;
+    double* S;
+    double* Vt;
+    double* scratch;
+    int64_t scratch_stride;
+    int use_lds;
+    int* status;
+};
+
+// Working set of the Jacobi SVD for a block of mm x nn (mm >= nn):
+//   W (mm*nn), V (nn*nn), sigma (nn), order (nn ints)
+__host__ __device__ inline int64_t svd_work_doubles(int64_t mm, int64_t nn) { return mm * nn + nn * nn + nn + (nn + 1) / 2 + 2; }
+
+__global__ __launch_bounds__(F_THREADS) void k_svd_jacobi(FactorArgs fa)
+{
+    extern __shared__ double smem[];
+    __shared__ int rotated;
+    const xh_factor_item it = fa.items[blockIdx.x];
+    const int m = it.m, n = it.n;
+    if(m == 0 || n == 0) return;
+    const bool tr = m < n;
+    const int mm = tr ? n : m, nn = tr ? m : n;
+    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* V = W + int64_t(mm) * nn;
+    double* sig = V + int64_t(nn) * nn;
+    int* ord = reinterpret_cast<int*>(sig + nn);
+    const int tid = threadIdx.x;
+    for(int64_t e = tid; e < int64_t(mm) * nn; e += F_THREADS) {
+        const int i = static_cast<int>(e % mm), j = static_cast<int>(e / mm);
+        W[e] = tr ? fa.A[it.a_off + j + int64_t(i) * m] : fa.A[it.a_off + i + int64_t(j) * m];
+    }
+    for(int64_t e = tid; e < int64_t(nn) * nn; e += F_THREADS) V[e] = ((e % nn) == (e / nn)) ? 1.0 : 0.0;
+    __syncthreads();
+
+    const int n2 = nn + (nn & 1);
+    const int npairs = n2 / 2;
+    int g = 64;
+    while(g > 8 && g * npairs > F_THREADS) g >>= 1;
+    const int ngroups = F_THREADS / g;
+    const int grp = tid / g, gl = tid % g;
+    const double tol = fmax(1.0, sqrt(static_cast<double>(mm))) * DBL_EPSILON;
+    int sweep = 0;
+    if(nn > 1) {
+        for(sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+            if(tid == 0) rotated = 0;
+            __syncthreads();
+            for(int step = 0; step < n2 - 1; ++step) {
+                for(int pbase = 0; pbase < npairs; pbase += ngroups) {
+                    const int p = pbase + grp;
+                    int i = -1, j = -1;
+                    if(p < npairs) {
+                        i = rr_player(p, step, n2);
+                        j = rr_player(n2 - 1 - p, step, n2);
+                        if(i > j) {
+                            int t = i;
+                            i = j;
+                            j = t;
+                        }
+                        if(j >= nn) i = -1;
+                    }
+                    double al = 0, be = 0, ga = 0;
+                    if(i >= 0) {
+                        const double* wi = W + int64_t(i) * mm;
+                        const double* wj = W + int64_t(j) * mm;
+                        for(int r = gl; r < mm; r += g) {
+                            const double x = wi[r], y = wj[r];
+                            al = fma(x, x, al);
+                            be = fma(y, y, be);
+                            ga = fma(x, y, ga);
+                        }
+                    }
+                    al = group_sum(al, g);
+                    be = group_sum(be, g);
+                    ga = group_sum(ga, g);
+                    if(i >= 0 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+                        const double zeta = (be - al) / (2.0 * ga);
+                        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                        const double c = 1.0 / sqrt(1.0 + t * t);
+                        const double s = c * t;
+                        double* wi = W + int64_t(i) * mm;
+                        double* wj = W + int64_t(j) * mm;
+                        for(int r = gl; r < mm; r += g) {
+                            const double x = wi[r], y = wj[r];
+                            wi[r] = c * x - s * y;
+                            wj[r] = s * x + c * y;
+                        }
+                        double* vi = V + int64_t(i) * nn;
+                        double* vj = V + int64_t(j) * nn;
+                        for(int r = gl; r < nn; r += g) {
+                            const double x = vi[r], y = vj[r];
+                            vi[r] = c * x - s * y;
+                            vj[r] = s * x + c * y;
+                        }
+                        if(gl == 0) rotated = 1;
+                    }
+                }
+                __syncthreads();
+            }
+            const int any = rotated;
+            __syncthreads();
+            if(!any) break;
+        }
+    }
+    if(sweep >= MAX_SWEEPS && tid == 0) atomicExch(fa.status, XH_ENOTCONV);
+    // column norms
+    for(int j = tid; j < nn; j += F_THREADS) {
+        const double* wj = W + int64_t(j) * mm;
+        double s = 0;
+        for(int r = 0; r < mm; ++r) s = fma(wj[r], wj[r], s);
+        sig[j] = sqrt(s);
+    }
+    __syncthreads();
+    // descending order, ties broken by column index
+    for(int j = tid; j < nn; j += F_THREADS) {
+        const double sj = sig[j];
+        int rank = 0;
+        for(int k = 0; k < nn; ++k) {
+            const double sk = sig[k];
+            rank += (sk > sj) || (sk == sj && k < j);
+        }
+        ord[rank] = j;
+    }
+    __syncthreads();
+    for(int r = tid; r < nn; r += F_THREADS) fa.S[it.s_off + r] = sig[ord[r]];
+    // U (m x nn) and Vt (nn x n)
+    for(int64_t e = tid; e < int64_t(m) * nn; e += F_THREADS) {
+        const int i = static_cast<int>(e % m), r = static_cast<int>(e / m);
+        const int c = ord[r];
+        double v;
+        if(!tr) {
+            const double s = sig[c];
+            v = s > 0.0 ? W[i + int64_t(c) * mm] / s : 0.0;
+        } else {
+            v = V[i + int64_t(c) * nn];
+        }
+        fa.U[it.u_off + e] = v;
+    }
+    for(int64_t e = tid; e < int64_t(nn) * n; e += F_THREADS) {
+        const int r = static_cast<int>(e % nn), col = static_cast<int>(e / nn);
+        const int c = ord[r];
+        double v;
+        if(!tr) {
+            v = V[col + int64_t(c) * nn];
+        } else {
+            const double s = sig[c];
+            v = s > 0.0 ? W[col + int64_t(c) * mm] / s : 0.0;
+        }
+        fa.Vt[it.vt_off + e] = v;
+    }
+}
+
+// Householder QR with Eigen's reflector convention (Eigen/src/Householder/Householder.h
+// makeHouseholder: beta = -sign(c0) * ||x||, tau = (beta - c0) / beta, v = [1, x_tail / (c0 - beta)]).
+// Q has `q_cols` columns (the tensor's middle dimension; columns beyond m are zero) and R has
+// `q_cols` rows (rows beyond min(m, n) are zero).  Item fields: u_off -> Q, vt_off -> R,
+// s_off -> q_cols (reused as an integer).
+__global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
+{
+    extern __shared__ double smem[];
+    __shared__ double red[F_THREADS / 64];
+    __shared__ double s_beta, s_tau, s_c0;
+    const xh_factor_item it = fa.items[blockIdx.x];
+    const int m = it.m, n = it.n;
+    const int qc = static_cast<int>(it.s_off);
+    if(m == 0 || n == 0) return;
+    const int kmax = min(m, n);
+    double* Wk = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;  // m x n working copy
+    double* tau = Wk + int64_t(m) * n;                                                      // kmax
+    const int tid = threadIdx.x;
+    for(int64_t e = tid; e < int64_t(m) * n; e += F_THREADS) Wk[e] = fa.A[it.a_off + e];
+    __syncthreads();
+    for(int k = 0; k < kmax; ++k) {
+        double* col = Wk + int64_t(k) * m;
+        double part = 0.0;
+        for(int r = k + 1 + tid; r < m; r += F_THREADS) part = fma(col[r], col[r], part);
+        for(int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        if((tid & 63) == 0) red[tid >> 6] = part;
+        __syncthreads();
+        if(tid == 0) {
+            double tail = 0;
+            for(int w = 0; w < F_THREADS / 64; ++w) tail += red[w];
+            const double c0 = col[k];
+            double beta, t;
+            if(tail <= DBL_MIN) {
+                t = 0.0;
+                beta = c0;
+            } else {
+                beta = sqrt(c0 * c0 + tail);
+                if(c0 >= 0.0) beta = -beta;
+                t = (beta - c0) / beta;
+            }
+            s_beta = beta;
+            s_tau = t;
+            s_c0 = c0;
+        }
+        __syncthreads();
+        const double beta = s_beta, t = s_tau, c0 = s_c0;
+        if(t != 0.0) {
+            const double inv = 1.0 / (c0 - beta);
+            for(int r = k + 1 + tid; r < m; r += F_THREADS) col[r] *= inv;
+        }
+        __syncthreads();
+        if(tid == 0) {
+            col[k] = beta;
+            tau[k] = t;
+        }
+        // apply H = I - tau v v^T to the trailing columns, v = [1; col[k+1:]]
+        if(t != 0.0) {
+            for(int j = k + 1 + tid; j < n; j += F_THREADS) {
+                double* cj = Wk + int64_t(j) * m;
+                double w = cj[k];
+                for(int r = k + 1; r < m; ++r) w = fma(col[r], cj[r], w);
+                w *= t;
+                cj[k] -= w;
+                for(int r = k + 1; r < m; ++r) cj[r] = fma(-w, col[r], cj[r]);
+            }
+        }
+        __syncthreads();
+    }
+    // R: qc x n, upper triangle of the working copy (rows beyond min(m, n) are zero)
+    for(int64_t e = tid; e < int64_t(qc) * n; e += F_THREADS) {
+        const int r = static_cast<int>(e % qc), c = static_cast<int>(e / qc);
+        fa.Vt[it.vt_off + e] = (r < m && r <= c) ? Wk[r + int64_t(c) * m] : 0.0;
+    }
+    // Q columns 0..qc-1: Q e_c = H_0 H_1 ... H_{kmax-1} e_c (zero for c >= m)
+    for(int c = tid; c < qc; c += F_THREADS) {
+        double* qcol = fa.U + it.u_off + int64_t(c) * m;
+        for(int r = 0; r < m; ++r) qcol[r] = (r == c) ? 1.0 : 0.0;
+        if(c >= m) continue;
+        for(int k = kmax - 1; k >= 0; --k) {
+            const double t = tau[k];
+            if(t == 0.0) continue;
+            const double* v = Wk + int64_t(k) * m;
+            double w = qcol[k];
+            for(int r = k + 1; r < m; ++r) w = fma(v[r], qcol[r], w);
+            w *= t;
+            qcol[k] -= w;
+            for(int r = k + 1; r < m; ++r) qcol[r] = fma(-w, v[r], qcol[r]);
+        }
+    }
+}
+
+// Symmetric eigen-decomposition by parallel cyclic Jacobi.  Item: a_off (n x n), s_off -> eigenvalues,
+// u_off -> eigenvectors (columns), ascending order.
+__global__ __launch_bounds__(F_THREADS) void k_eigh(FactorArgs fa)
+{
+    extern __shared__ double smem[];
+    __shared__ int rotated;
+    const xh_factor_item it = fa.items[blockIdx.x];
+    const int n = it.m;
+    if(n == 0) return;
+    double* Am = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* V = Am + int64_t(n) * n;
+    double* cs = V + int64_t(n) * n;  // per pair: c, s  (n entries)
+    int* pr = reinterpret_cast<int*>(cs + n + 1);  // per pair: p, q (n ints)
+    double* ev = cs;  // reused after the sweeps
+    int* ord = pr;
+    const int tid = threadIdx.x;
+    for(int64_t e = tid; e < int64_t(n) * n; e += F_THREADS) {
+        // symmetrise from the lower triangle as SelfAdjointEigenSolver reads it
+        const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+        Am[e] = (i >= j) ? fa.A[it.a_off + e] : fa.A[it.a_off + j + int64_t(i) * n];
+        V[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int n2 = n + (n & 1), npairs = n2 / 2;
+    int sweep = 0;
+    if(n > 1) {
+        for(sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+            if(tid == 0) rotated = 0;
+            __syncthreads();
+            for(int step = 0; step < n2 - 1; ++step) {
+                for(int p = tid; p < npairs; p += F_THREADS) {
+                    int i = rr_player(p, step, n2), j = rr_player(n2 - 1 - p, step, n2);
+                    if(i > j) {
+                        int t = i;
+                        i = j;
+                        j = t;
+                    }
+                    double c = 1.0, s = 0.0;
+                    if(j < n) {
+                        const double apq = Am[i + int64_t(j) * n];
+                        const double app = Am[i + int64_t(i) * n], aqq = Am[j + int64_t(j) * n];
+                        if(fabs(apq) > DBL_EPSILON * sqrt(fabs(app) * fabs(aqq)) && fabs(apq) > 1e-300) {
+                            const double theta = (aqq - app) / (2.0 * apq);
+                            const double t = copysign(1.0, theta) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                            c = 1.0 / sqrt(t * t + 1.0);
+                            s = t * c;
+                            rotated = 1;
+                        }
+                    } else {
+                        i = -1;
+                    }
+                    cs[2 * p] = c;
+                    cs[2 * p + 1] = s;
+                    pr[2 * p] = i;
+                    pr[2 * p + 1] = j;
+                }
+                __syncthreads();
+                // columns: A <- A J, V <- V J
+                for(int64_t e = tid; e < int64_t(npairs) * n; e += F_THREADS) {
+                    const int p = static_cast<int>(e / n), r = static_cast<int>(e % n);
+                    const int i = pr[2 * p], j = pr[2 * p + 1];
+                    if(i < 0) continue;
+                    const double c = cs[2 * p], s = cs[2 * p + 1];
+                    if(s == 0.0) continue;
+                    double x = Am[r + int64_t(i) * n], y = Am[r + int64_t(j) * n];
+                    Am[r + int64_t(i) * n] = c * x - s * y;
+                    Am[r + int64_t(j) * n] = s * x + c * y;
+                    x = V[r + int64_t(i) * n];
+                    y = V[r + int64_t(j) * n];
+                    V[r + int64_t(i) * n] = c * x - s * y;
+                    V[r + int64_t(j) * n] = s * x + c * y;
+                }
+                __syncthreads();
+                // rows: A <- J^T A
+                for(int64_t e = tid; e < int64_t(npairs) * n; e += F_THREADS) {
+                    const int p = static_cast<int>(e / n), col = static_cast<int>(e % n);
+                    const int i = pr[2 * p], j = pr[2 * p + 1];
+                    if(i < 0) continue;
+                    const double c = cs[2 * p], s = cs[2 * p + 1];
+                    if(s == 0.0) continue;
+                    const double x = Am[i + int64_t(col) * n], y = Am[j + int64_t(col) * n];
+                    Am[i + int64_t(col) * n] = c * x - s * y;
+                    Am[j + int64_t(col) * n] = s * x + c * y;
+                }
+                __syncthreads();
+            }
+            const int any = rotated;
+            __syncthreads();
+            if(!any) break;
+        }
+    }
+    if(sweep >= MAX_SWEEPS && tid == 0) atomicExch(fa.status, XH_ENOTCONV);
+    for(int j = tid; j < n; j += F_THREADS) ev[j] = Am[j + int64_t(j) * n];
+    __syncthreads();
+    for(int j = tid; j < n; j += F_THREADS) {
+        const double vj = ev[j];
+        int rank = 0;
+        for(int k = 0; k < n; ++k) rank += (ev[k] < vj) || (ev[k] == vj && k < j);
+        ord[rank] = j;
+    }
+    __syncthreads();
+    for(int r = tid; r < n; r += F_THREADS) fa.S[it.s_off + r] = ev[ord[r]];
+    for(int64_t e = tid; e < int64_t(n) * n; e += F_THREADS) {
+        const int i = static_cast<int>(e % n), r = static_cast<int>(e / n);
+        fa.U[it.u_off + e] = V[i + int64_t(ord[r]) * n];
+    }
+}
+
+struct Launch {
+    std::vector<xh_factor_item> lds_items, glb_items;
+    int64_t lds_bytes = 0, glb_stride = 0;
+};
+
+size_t lds_budget(xh_ctx* ctx)
+{
+    static size_t cached = 0;
+    if(cached == 0) {
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
+        cached = prop.sharedMemPerBlock;
+        if(cached > 160 * 1024) cached = 160 * 1024;
+    }
+    return cached - 1024;  // leave room for the static shared variables
+}
+
+template <typename Need>
+void run_factor(xh_ctx* ctx, void (*kern)(FactorArgs), const std::vector<xh_factor_item>& items, Need need_doubles, const double* A, double* U,
+                double* S, double* Vt)
+{
+    const size_t budget = lds_budget(ctx);
+    std::vector<xh_factor_item> small, big;
+    int64_t small_max = 0, big_max = 0;
+    for(const auto& it : items) {
+        if(it.m < 0 || it.n < 0) throw xh::Error(XH_EINVAL, "factor: negative extent");
+        if(it.m == 0 || it.n == 0) continue;
+        const int64_t nd = need_doubles(it);
+        if(size_t(nd) * sizeof(double) <= budget) {
+            small.push_back(it);
+            small_max = std::max(small_max, nd);
+        } else {
+            big.push_back(it);
+            big_max = std::max(big_max, nd);
+        }
+    }
+    int* d_status = nullptr;
+    check(hipMallocAsync(reinterpret_cast<void**>(&d_status), sizeof(int), ctx->stream), "hipMallocAsync");
+    check(hipMemsetAsync(d_status, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
+    auto go = [&](const std::vector<xh_factor_item>& v, bool lds, int64_t maxd) {
+        if(v.empty()) return;
+        xh_factor_item* d_items = xh::dev_upload(ctx, v);
+        double* scratch = nullptr;
+        if(!lds) check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(maxd) * v.size() * sizeof(double), ctx->stream), "hipMallocAsync");
+        FactorArgs fa{d_items, A, U, S, Vt, scratch, maxd, lds ? 1 : 0, d_status};
+        const size_t shmem = lds ? size_t(maxd) * sizeof(double) : 0;
+        if(lds && shmem > 64 * 1024) check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(v.size())), dim3(F_THREADS), shmem, ctx->stream, fa);
+        xh::launch_check("factor kernel");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d_items);
+        xh::dev_free(ctx, scratch);
+    };
+    go(small, true, small_max);
+    go(big, false, big_max);
+    int status = 0;
+    check(hipMemcpyAsync(&status, d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    xh::dev_free(ctx, d_status);
+    if(status != 0) throw xh::Error(XH_ENOTCONV, "Jacobi iteration did not converge within the sweep limit");
+}
+
+}  // namespace
+
+extern "C" int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt)
+{
+    return guard(ctx, [&] {
+        std::vector<xh_factor_item> v(items, items + nitems);
+        run_factor(
+            ctx, k_svd_jacobi, v,
+            [](const xh_factor_item& it) {
+                const int64_t mm = std::max(it.m, it.n), nn = std::min(it.m, it.n);
+                return svd_work_doubles(mm, nn);
+            },
+            A, U, S, Vt);
+    });
+}
+
+extern "C" int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* Q, double* R)
+{
+    return guard(ctx, [&] {
+        std::vector<xh_factor_item> v(items, items + nitems);
+        run_factor(
+            ctx, k_qr, v, [](const xh_factor_item& it) { return int64_t(it.m) * it.n + std::min(it.m, it.n) + 1; }, A, Q, nullptr, R);
+    });
+}
+
+extern "C" int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* W, double* V)
+{
+    return guard(ctx, [&] {
+        std::vector<xh_factor_item> v(items, items + nitems);
+        for(auto& it : v)
+            if(it.m != it.n) throw xh::Error(XH_EINVAL, "eigh: blocks must be square");
+        run_factor(
+            ctx, k_eigh, v, [](const xh_factor_item& it) { return 2 * int64_t(it.m) * it.m + 2 * int64_t(it.m) + 4; }, A, V, W, nullptr);
+    });
+}

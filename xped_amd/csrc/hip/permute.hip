@@ -1,0 +1,387 @@
+// Grouped block-sparse permute: one launch moves every subblock of a tensor.
+//
+// The reference walks sectors x domain trees x codomain trees on the host and, per subblock,
+// copies it out of its sector block, shuffles it with Eigen and adds it (scaled by the
+// recoupling coefficient) into the output block (src/Core/Tensor.cpp:179-212, with
+// PlainInterface::tensor_from_matrix_block / shuffle / add_to_block_from_tensor).
+//
+// Here the host builds a gather plan once: every output subblock ("task") knows its sources
+// and coefficients.  The kernel walks the concatenated output element space in tiles; each
+// output element is written exactly once (accumulated over its sources in the reference's
+// order without fused multiply-add), so abelian permutes are bit-exact copies/negations.
+//
+// Tasks whose first output leg maps to a unit-stride source leg are copied with coalesced
+// reads and writes.  Tasks where the leading output leg is strided in the source (a real
+// transpose) go through a 64x64 LDS tile so that both the global loads and the global stores
+// are coalesced.
+#include <algorithm>
+#include <cstring>
+
+#include "device.hpp"
+
+using xh::check;
+using xh::guard;
+
+namespace {
+
+constexpr int PT_THREADS = 256;
+constexpr int PT_TILE = 2048;  // output elements per workgroup in the gather path
+constexpr int TR_DIM = 64;     // transpose tile edge
+
+struct TaskMeta {
+    int64_t dst_off, dst_ld;
+    uint32_t rows, cols;
+    int32_t nsrc, src_begin;
+};
+
+struct LegInfo {
+    uint32_t ext, magic, shift, pad;
+    int64_t rstride, cstride;
+};
+
+struct PermDev {
+    int nlegs, new_rank;
+    int64_t ntasks, total;
+    const TaskMeta* meta;
+    const LegInfo* legs;  // ntasks * nlegs
+    const int64_t* prefix;  // ntasks + 1 (element prefix of gather tasks)
+    const int32_t* tile_first;  // gather tiles: first task of each tile (+1 sentinel)
+    const xh_permute_source* src;
+    // transpose path
+    int64_t ntr_tiles;
+    const int4* tr_tiles;  // (task, tile index along the contiguous-source leg, tile index along rows, rest index)
+};
+
+__device__ __forceinline__ void decode_src(const LegInfo* __restrict__ L, int nlegs, int new_rank, uint32_t row, uint32_t col, int64_t& srow,
+                                           int64_t& scol)
+{
+    srow = 0;
+    scol = 0;
+    for(int k = 0; k < new_rank; ++k) {
+        const LegInfo li = L[k];
+        uint32_t q = xh::fdiv(row, li.magic, li.shift);
+        uint32_t j = row - q * li.ext;
+        row = q;
+        srow += int64_t(j) * li.rstride;
+        scol += int64_t(j) * li.cstride;
+    }
+    for(int k = new_rank; k < nlegs; ++k) {
+        const LegInfo li = L[k];
+        uint32_t q = xh::fdiv(col, li.magic, li.shift);
+        uint32_t j = col - q * li.ext;
+        col = q;
+        srow += int64_t(j) * li.rstride;
+        scol += int64_t(j) * li.cstride;
+    }
+}
+
+__device__ __forceinline__ double gather_one(const PermDev& p, const TaskMeta& tm, const double* __restrict__ src, int64_t srow, int64_t scol)
+{
+    double acc = 0.0;
+    for(int s = 0; s < tm.nsrc; ++s) {
+        const xh_permute_source so = p.src[tm.src_begin + s];
+        const double v = src[so.src_off + srow + scol * so.src_ld];
+        acc = __dadd_rn(acc, __dmul_rn(so.coeff, v));
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int64_t e0 = int64_t(blockIdx.x) * PT_TILE;
+    const int64_t e1 = min(e0 + PT_TILE, p.total);
+    const int t_lo = p.tile_first[blockIdx.x];
+    const int t_hi = p.tile_first[blockIdx.x + 1];
+    for(int64_t e = e0 + threadIdx.x; e < e1; e += PT_THREADS) {
+        // last task t in [t_lo, t_hi] with prefix[t] <= e
+        int lo = t_lo, hi = t_hi;
+        while(lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if(p.prefix[mid] <= e)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int t = lo;
+        const TaskMeta tm = p.meta[t];
+        const uint32_t el = static_cast<uint32_t>(e - p.prefix[t]);
+        const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
+        // the row index runs over the output domain legs, the column index over the codomain legs
+        uint32_t row = el % tm.rows;
+        uint32_t col = el / tm.rows;
+        int64_t srow, scol;
+        decode_src(L, p.nlegs, p.new_rank, row, col, srow, scol);
+        dst[tm.dst_off + row + int64_t(col) * tm.dst_ld] = gather_one(p, tm, src, srow, scol);
+    }
+}
+
+// Transpose path.  For a task whose output leg 0 is not the unit-stride source leg, let leg u be
+// the output leg that reads source stride 1.  A tile covers TR_DIM values of output leg 0 (the
+// contiguous direction of the output) times TR_DIM values of leg u (the contiguous direction of
+// the source) at one fixed value of every other leg.  Loads run along leg u, stores along leg 0.
+struct TrTaskInfo {
+    int32_t task, u;  // task id and the output leg with unit source stride
+    uint32_t ext0, extu;
+    int64_t rest;  // number of combinations of the remaining legs
+};
+
+__global__ __launch_bounds__(PT_THREADS) void k_permute_transpose(PermDev p, const TrTaskInfo* __restrict__ info, const double* __restrict__ src,
+                                                                  double* __restrict__ dst)
+{
+    __shared__ double tile[TR_DIM][TR_DIM + 1];
+    const int4 tt = p.tr_tiles[blockIdx.x];
+    const TrTaskInfo ti = info[tt.x];
+    const int t = ti.task;
+    const TaskMeta tm = p.meta[t];
+    const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
+    const int u = ti.u;
+    const uint32_t i0 = uint32_t(tt.z) * TR_DIM;  // along output leg 0
+    const uint32_t iu = uint32_t(tt.y) * TR_DIM;  // along output leg u
+    // Decode the fixed remaining-leg index (every leg except 0 and u) into base offsets.
+    uint32_t rest = uint32_t(tt.w);
+    int64_t base_srow = 0, base_scol = 0;
+    // output position contributions of the remaining legs: row (domain legs) and col (codomain legs)
+    int64_t base_row = 0, base_col = 0;
+    int64_t rmul = 1, cmul = 1;
+    for(int k = 0; k < p.nlegs; ++k) {
+        const LegInfo li = L[k];
+        const bool is_row = k < p.new_rank;
+        int64_t mult = is_row ? rmul : cmul;
+        if(k != 0 && k != u) {
+            uint32_t q = xh::fdiv(rest, li.magic, li.shift);
+            uint32_t j = rest - q * li.ext;
+            rest = q;
+            base_srow += int64_t(j) * li.rstride;
+            base_scol += int64_t(j) * li.cstride;
+            if(is_row)
+                base_row += int64_t(j) * mult;
+            else
+                base_col += int64_t(j) * mult;
+        }
+        if(is_row)
+            rmul *= li.ext;
+        else
+            cmul *= li.ext;
+    }
+    // output multiplier of leg u
+    int64_t umul = 1;
+    {
+        const bool urow = u < p.new_rank;
+        int64_t m = 1;
+        for(int k = urow ? 0 : p.new_rank; k < u; ++k) m *= L[k].ext;
+        umul = m;
+    }
+    const bool urow = u < p.new_rank;
+    const LegInfo l0 = L[0];
+    const LegInfo lu = L[u];
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;  // 0..3
+    // load: rows of the tile indexed by leg-0 value a, columns by leg-u value b; b runs along lanes
+    for(int a = ty; a < TR_DIM; a += 4) {
+        const uint32_t j0 = i0 + a, ju = iu + tx;
+        double v = 0.0;
+        if(j0 < ti.ext0 && ju < ti.extu) {
+            const int64_t srow = base_srow + int64_t(j0) * l0.rstride + int64_t(ju) * lu.rstride;
+            const int64_t scol = base_scol + int64_t(j0) * l0.cstride + int64_t(ju) * lu.cstride;
+            v = gather_one(p, tm, src, srow, scol);
+        }
+        tile[a][tx] = v;
+    }
+    __syncthreads();
+    // store: a runs along lanes (output leg 0 is the contiguous output direction)
+    for(int b = ty; b < TR_DIM; b += 4) {
+        const uint32_t j0 = i0 + tx, ju = iu + b;
+        if(j0 < ti.ext0 && ju < ti.extu) {
+            int64_t row = base_row + j0;  // leg 0 is a domain leg with multiplier 1 (new_rank >= 1 here)
+            int64_t col = base_col;
+            if(urow)
+                row += int64_t(ju) * umul;
+            else
+                col += int64_t(ju) * umul;
+            dst[tm.dst_off + row + col * tm.dst_ld] = tile[tx][b];
+        }
+    }
+}
+
+}  // namespace
+
+struct xh_permute_plan {
+    PermDev dev{};
+    void* mem_meta = nullptr;
+    void* mem_legs = nullptr;
+    void* mem_prefix = nullptr;
+    void* mem_tiles = nullptr;
+    void* mem_src = nullptr;
+    void* mem_tr = nullptr;
+    void* mem_trinfo = nullptr;
+    int64_t ngather_tiles = 0;
+    double bytes = 0.0;
+    const TrTaskInfo* trinfo = nullptr;
+};
+
+extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, xh_permute_plan** out)
+{
+    *out = nullptr;
+    auto* plan = new xh_permute_plan();
+    int rc = guard(ctx, [&] {
+        const int NL = desc->nlegs;
+        if(NL < 0 || NL > XH_MAXLEGS) throw xh::Error(XH_EINVAL, "permute plan: too many legs");
+        if(desc->new_rank < 0 || desc->new_rank > NL) throw xh::Error(XH_EINVAL, "permute plan: invalid new_rank");
+        std::vector<TaskMeta> meta;
+        std::vector<LegInfo> legs;
+        std::vector<int64_t> prefix;
+        std::vector<TrTaskInfo> trinfo;
+        std::vector<int4> trtiles;
+        std::vector<int32_t> gather_ids;  // tasks handled by the gather kernel
+        double moved = 0.0, read = 0.0;
+        for(int64_t t = 0; t < desc->ntasks; ++t) {
+            const xh_permute_task& tk = desc->tasks[t];
+            if(tk.rows < 0 || tk.cols < 0 || tk.rows * tk.cols >= (int64_t(1) << 31))
+                throw xh::Error(XH_EINVAL, "permute plan: subblock too large for 32-bit indexing");
+            if(tk.nsrc < 0 || tk.src_begin < 0 || tk.src_begin + tk.nsrc > desc->nsources)
+                throw xh::Error(XH_EINVAL, "permute plan: source range out of bounds");
+            TaskMeta m{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
+            int64_t rprod = 1, cprod = 1;
+            for(int k = 0; k < NL; ++k) {
+                if(tk.ext[k] == 0) throw xh::Error(XH_EINVAL, "permute plan: zero extent");
+                xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
+                LegInfo li{tk.ext[k], f.magic, f.shift, 0, tk.is_col[k] ? 0 : tk.sstride[k], tk.is_col[k] ? tk.sstride[k] : 0};
+                legs.push_back(li);
+                if(k < desc->new_rank)
+                    rprod *= tk.ext[k];
+                else
+                    cprod *= tk.ext[k];
+            }
+            if(rprod != tk.rows || cprod != tk.cols) throw xh::Error(XH_EINVAL, "permute plan: leg extents do not match the subblock shape");
+            meta.push_back(m);
+            const double elems = double(tk.rows) * double(tk.cols);
+            moved += elems;
+            read += elems * tk.nsrc;
+            // route: transpose path when output leg 0 is a real leg (ext >= 16) whose source stride is
+            // not 1 and some other leg with ext >= 16 has unit source stride
+            int u = -1;
+            if(desc->new_rank >= 1 && NL >= 2 && tk.nsrc >= 1 && tk.sstride[0] != 1 && !tk.is_col[0] && tk.ext[0] >= 16) {
+                for(int k = 1; k < NL; ++k)
+                    if(tk.sstride[k] == 1 && !tk.is_col[k] && tk.ext[k] >= 16) {
+                        u = k;
+                        break;
+                    }
+            }
+            if(u > 0) {
+                TrTaskInfo ti{static_cast<int32_t>(t), u, tk.ext[0], tk.ext[u], 0};
+                int64_t rest = 1;
+                for(int k = 0; k < NL; ++k)
+                    if(k != 0 && k != u) rest *= tk.ext[k];
+                ti.rest = rest;
+                const int info_id = static_cast<int>(trinfo.size());
+                trinfo.push_back(ti);
+                const int n0 = (tk.ext[0] + TR_DIM - 1) / TR_DIM, nu = (tk.ext[u] + TR_DIM - 1) / TR_DIM;
+                for(int64_t r = 0; r < rest; ++r)
+                    for(int a = 0; a < n0; ++a)
+                        for(int b = 0; b < nu; ++b) trtiles.push_back(int4{info_id, b, a, static_cast<int>(r)});
+            } else {
+                gather_ids.push_back(static_cast<int32_t>(t));
+            }
+        }
+        // gather tasks: compact metadata in gather order
+        std::vector<TaskMeta> gmeta;
+        std::vector<LegInfo> glegs;
+        prefix.push_back(0);
+        for(int32_t t : gather_ids) {
+            gmeta.push_back(meta[t]);
+            for(int k = 0; k < NL; ++k) glegs.push_back(legs[int64_t(t) * NL + k]);
+            prefix.push_back(prefix.back() + int64_t(meta[t].rows) * meta[t].cols);
+        }
+        const int64_t total = prefix.back();
+        std::vector<int32_t> tile_first;
+        const int64_t ntiles = (total + PT_TILE - 1) / PT_TILE;
+        {
+            int t = 0;
+            const int ng = static_cast<int>(gather_ids.size());
+            for(int64_t i = 0; i < ntiles; ++i) {
+                const int64_t e = i * PT_TILE;
+                while(t + 1 < ng && prefix[t + 1] <= e) ++t;
+                tile_first.push_back(t);
+            }
+            // sentinel: last task touching the final tile
+            tile_first.push_back(ng > 0 ? ng - 1 : 0);
+            // per tile upper bound: the first task of the next tile is a valid upper bound for binary search
+        }
+        // for the transpose kernel the full metadata table is needed; append it after the gather one
+        const int64_t gcount = static_cast<int64_t>(gmeta.size());
+        std::vector<TaskMeta> allmeta = gmeta;
+        std::vector<LegInfo> alllegs = glegs;
+        for(auto& ti : trinfo) {
+            const int32_t t = ti.task;
+            allmeta.push_back(meta[t]);
+            for(int k = 0; k < NL; ++k) alllegs.push_back(legs[int64_t(t) * NL + k]);
+            ti.task = static_cast<int32_t>(allmeta.size() - 1);
+        }
+        (void)gcount;
+        std::vector<xh_permute_source> srcs(desc->sources, desc->sources + desc->nsources);
+        plan->dev.nlegs = NL;
+        plan->dev.new_rank = desc->new_rank;
+        plan->dev.ntasks = static_cast<int64_t>(allmeta.size());
+        plan->dev.total = total;
+        plan->mem_meta = xh::dev_upload(ctx, allmeta);
+        plan->mem_legs = xh::dev_upload(ctx, alllegs);
+        plan->mem_prefix = xh::dev_upload(ctx, prefix);
+        plan->mem_tiles = xh::dev_upload(ctx, tile_first);
+        plan->mem_src = xh::dev_upload(ctx, srcs);
+        plan->mem_tr = xh::dev_upload(ctx, trtiles);
+        plan->mem_trinfo = xh::dev_upload(ctx, trinfo);
+        plan->dev.meta = static_cast<const TaskMeta*>(plan->mem_meta);
+        plan->dev.legs = static_cast<const LegInfo*>(plan->mem_legs);
+        plan->dev.prefix = static_cast<const int64_t*>(plan->mem_prefix);
+        plan->dev.tile_first = static_cast<const int32_t*>(plan->mem_tiles);
+        plan->dev.src = static_cast<const xh_permute_source*>(plan->mem_src);
+        plan->dev.ntr_tiles = static_cast<int64_t>(trtiles.size());
+        plan->dev.tr_tiles = static_cast<const int4*>(plan->mem_tr);
+        plan->trinfo = static_cast<const TrTaskInfo*>(plan->mem_trinfo);
+        plan->ngather_tiles = ntiles;
+        plan->bytes = 8.0 * (moved + read);
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+    if(rc != XH_OK) {
+        xh_permute_plan_destroy(ctx, plan);
+        return rc;
+    }
+    *out = plan;
+    return XH_OK;
+}
+
+extern "C" int xh_permute_plan_destroy(xh_ctx* ctx, xh_permute_plan* plan)
+{
+    if(!plan) return XH_OK;
+    return guard(ctx, [&] {
+        for(void* p : {plan->mem_meta, plan->mem_legs, plan->mem_prefix, plan->mem_tiles, plan->mem_src, plan->mem_tr, plan->mem_trinfo})
+            xh::dev_free(ctx, p);
+        delete plan;
+    });
+}
+
+extern "C" int xh_permute_plan_bytes(const xh_permute_plan* plan, double* bytes)
+{
+    if(!plan || !bytes) return XH_EINVAL;
+    *bytes = plan->bytes;
+    return XH_OK;
+}
+
+extern "C" int xh_permute(xh_ctx* ctx, const xh_permute_plan* plan, const double* src, double* dst)
+{
+    return guard(ctx, [&] {
+        if(!plan) throw xh::Error(XH_EINVAL, "null permute plan");
+        if(plan->ngather_tiles > 0) {
+            if(plan->ngather_tiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
+            hipLaunchKernelGGL(k_permute_gather, dim3(static_cast<unsigned>(plan->ngather_tiles)), dim3(PT_THREADS), 0, ctx->stream, plan->dev, src,
+                               dst);
+            xh::launch_check("k_permute_gather");
+        }
+        if(plan->dev.ntr_tiles > 0) {
+            if(plan->dev.ntr_tiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
+            hipLaunchKernelGGL(k_permute_transpose, dim3(static_cast<unsigned>(plan->dev.ntr_tiles)), dim3(PT_THREADS), 0, ctx->stream, plan->dev,
+                               plan->trinfo, src, dst);
+            xh::launch_check("k_permute_transpose");
+        }
+    });
+}

@@ -1,0 +1,391 @@
+"""Python mirror of the reference's Xped::Tensor / Qbasis interface over the C ABI.
+
+Names, argument meaning and error behaviour follow include/Xped/Core/Tensor.hpp:59-444,
+TensorBase.hpp:47-135 and Qbasis.hpp:37-268.  All data lives on the GPU; structural queries
+also work on a structure-only runtime (``Runtime(device=-1)``), which has no data.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import XH_BF16X3, XH_FP32, XH_FP64, check, lib
+
+# symmetry kinds (component codes of the C ABI)
+U1 = (1, 0)          # U1<SpinU1> / U1<ChargeU1>
+FU1 = (2, 0)         # U1<FChargeU1>
+SU2 = (3, 0)         # SU2<SpinSU2>
+FU1xU1 = (2, 1)      # Combined<U1<FChargeU1>, U1<SpinU1>>
+U1xU1 = (1, 1)
+
+PRECISION = {"fp64": XH_FP64, "fp32": XH_FP32, "bf16x3": XH_BF16X3}
+
+
+def _q2(q) -> C.Array:
+    q = tuple(q) if isinstance(q, (tuple, list)) else (q,)
+    a = (C.c_int * 2)(0, 0)
+    for i, x in enumerate(q):
+        a[i] = int(x)
+    return a
+
+
+def _ints(v) -> C.Array:
+    v = list(v)
+    return (C.c_int * max(1, len(v)))(*v)
+
+
+class Runtime:
+    """Device context plus plan caches (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().xt_rt_create(device, C.byref(h))
+        check(rc, None, "xt_rt_create")
+        self.h = h
+        self.device = device
+
+    @property
+    def has_device(self) -> bool:
+        return self.device >= 0
+
+    def ctx(self):
+        return lib().xt_rt_ctx(self.h)
+
+    def counters(self):
+        f, b = C.c_double(), C.c_double()
+        ng, npm = C.c_long(), C.c_long()
+        lib().xt_rt_counters(self.h, C.byref(f), C.byref(b), C.byref(ng), C.byref(npm))
+        return {"gemm_flops": f.value, "permute_bytes": b.value, "n_gemm": ng.value, "n_permute": npm.value}
+
+    def synchronize(self):
+        if self.has_device:
+            lib().xh_synchronize(self.ctx())
+
+    def clear_caches(self):
+        check(lib().xt_rt_clear_caches(self.h), self.h, "clear_caches")
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_rt_destroy(h)
+            self.h = None
+
+
+class Qbasis:
+    """Qbasis<Symmetry, depth> handle."""
+
+    def __init__(self, sym=U1, handle=None):
+        self.sym = tuple(sym)
+        if handle is None:
+            h = C.c_void_p()
+            check(lib().xt_basis_create(self.sym[0], self.sym[1], C.byref(h)), None, "xt_basis_create")
+            handle = h
+        self.h = handle
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_basis_destroy(h)
+            self.h = None
+
+    def push_back(self, q, dim: int):
+        check(lib().xt_basis_push_back(self.h, _q2(q), int(dim)), None, "push_back")
+        return self
+
+    def _new(self, fn, *args):
+        h = C.c_void_p()
+        check(fn(self.h, *args, C.byref(h)), None, fn.__name__)
+        return Qbasis(self.sym, h)
+
+    def conj(self) -> "Qbasis":
+        return self._new(lib().xt_basis_conj)
+
+    def combine(self, other: "Qbasis") -> "Qbasis":
+        return self._new(lib().xt_basis_combine, other.h)
+
+    def forgetHistory(self) -> "Qbasis":
+        return self._new(lib().xt_basis_forget_history)
+
+    def copy(self) -> "Qbasis":
+        return self._new(lib().xt_basis_copy)
+
+    def sort(self):
+        check(lib().xt_basis_sort(self.h), None, "sort")
+        return self
+
+    def info(self):
+        d, n, conj = C.c_int(), C.c_int(), C.c_int()
+        dim, fd = C.c_size_t(), C.c_size_t()
+        lib().xt_basis_info(self.h, C.byref(d), C.byref(n), C.byref(dim), C.byref(fd), C.byref(conj))
+        return {"depth": d.value, "nsec": n.value, "dim": dim.value, "full_dim": fd.value, "conj": bool(conj.value)}
+
+    def dim(self):
+        return self.info()["dim"]
+
+    def fullDim(self):
+        return self.info()["full_dim"]
+
+    def nq(self):
+        return 2 if self.sym[1] else 1
+
+    def entries(self):
+        """[(q, num, dim, ntrees)] in basis order."""
+        out = []
+        n = self.info()["nsec"]
+        q = (C.c_int * 2)()
+        num, dim = C.c_size_t(), C.c_size_t()
+        nt = C.c_int()
+        for i in range(n):
+            lib().xt_basis_entry(self.h, i, q, C.byref(num), C.byref(dim), C.byref(nt))
+            out.append((tuple(q[: self.nq()]), num.value, dim.value, nt.value))
+        return out
+
+    def trees(self, i: int):
+        """Fusion trees of sector i: list of dicts (unc, inter, coupled, dims, dual, dim)."""
+        nq = self.nq()
+        ents = self.entries()
+        res = []
+        for t in range(ents[i][3]):
+            rank = C.c_int()
+            unc = (C.c_int * 40)()
+            inter = (C.c_int * 40)()
+            cp = (C.c_int * 2)()
+            dims = (C.c_uint32 * 20)()
+            dual = (C.c_int * 20)()
+            dim = C.c_uint64()
+            check(lib().xt_basis_tree(self.h, i, t, C.byref(rank), unc, inter, cp, dims, dual, C.byref(dim)), None, "tree")
+            r = rank.value
+            res.append({
+                "unc": tuple(tuple(unc[2 * k: 2 * k + nq]) for k in range(r)),
+                "inter": tuple(tuple(inter[2 * k: 2 * k + nq]) for k in range(max(r - 2, 0))),
+                "coupled": tuple(cp[:nq]),
+                "dims": tuple(dims[:r]),
+                "dual": tuple(bool(x) for x in dual[:r]),
+                "dim": dim.value,
+            })
+        return res
+
+
+def _basis_sym(h):
+    k0, k1 = C.c_int(), C.c_int()
+    lib().xt_basis_sym(h, C.byref(k0), C.byref(k1))
+    return (k0.value, k1.value)
+
+
+class Tensor:
+    """Tensor<double, Rank, CoRank, Symmetry> with GPU-resident data."""
+
+    def __init__(self, rt: Runtime, domain: Sequence[Qbasis] = (), codomain: Sequence[Qbasis] = (), handle=None):
+        self.rt = rt
+        if handle is None:
+            legs = list(domain) + list(codomain)
+            arr = (C.c_void_p * max(1, len(legs)))(*[b.h.value if isinstance(b.h, C.c_void_p) else b.h for b in legs])
+            h = C.c_void_p()
+            check(lib().xt_tensor_create(rt.h, len(domain), len(codomain), arr, C.byref(h)), rt.h, "Tensor()")
+            handle = h
+        self.h = handle
+        r, c, n = C.c_int(), C.c_int(), C.c_int()
+        s = C.c_size_t()
+        lib().xt_tensor_shape(self.h, C.byref(r), C.byref(c), C.byref(n), C.byref(s))
+        self.rank, self.corank, self.nsec, self.size = r.value, c.value, n.value, s.value
+        leg = self.leg(0) if self.rank + self.corank else None
+        self.sym = leg.sym if leg is not None else U1
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_tensor_destroy(h)
+            self.h = None
+
+    def _wrap(self, h):
+        return Tensor(self.rt, handle=h)
+
+    # ---- structure
+    def leg(self, i: int) -> Qbasis:
+        h = C.c_void_p()
+        check(lib().xt_tensor_leg(self.h, i, C.byref(h)), self.rt.h, "leg")
+        return Qbasis(_basis_sym(h), h)
+
+    def uncoupledDomain(self):
+        return [self.leg(i) for i in range(self.rank)]
+
+    def uncoupledCodomain(self):
+        return [self.leg(self.rank + i) for i in range(self.corank)]
+
+    def coupledDomain(self) -> Qbasis:
+        h = C.c_void_p()
+        check(lib().xt_tensor_coupled(self.h, 0, C.byref(h)), self.rt.h, "coupledDomain")
+        return Qbasis(self.sym, h)
+
+    def coupledCodomain(self) -> Qbasis:
+        h = C.c_void_p()
+        check(lib().xt_tensor_coupled(self.h, 1, C.byref(h)), self.rt.h, "coupledCodomain")
+        return Qbasis(self.sym, h)
+
+    def sectors(self):
+        """[(q, offset, rows, cols)] in storage order."""
+        out = []
+        q = (C.c_int * 2)()
+        o, r, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        for i in range(self.nsec):
+            lib().xt_tensor_sector(self.h, i, q, C.byref(o), C.byref(r), C.byref(c))
+            out.append((tuple(q[: 2 if self.sym[1] else 1]), o.value, r.value, c.value))
+        return out
+
+    # ---- data
+    def upload(self, data: np.ndarray):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        check(lib().xt_tensor_upload(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), a.size), self.rt.h, "upload")
+        return self
+
+    def download(self) -> np.ndarray:
+        a = np.empty(self.size, dtype=np.float64)
+        check(lib().xt_tensor_download(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), a.size), self.rt.h, "download")
+        return a
+
+    def setRandom(self, seed: int, dist: int = 0):
+        check(lib().xt_tensor_set_random(self.h, seed & 0xFFFFFFFF, dist), self.rt.h, "setRandom")
+        return self
+
+    def device_ptr(self) -> int:
+        return lib().xt_tensor_device_ptr(self.h) or 0
+
+    # ---- operations
+    def permute(self, shift: int, p: Sequence[int]) -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_permute(self.h, shift, _ints(p), len(p), C.byref(h)), self.rt.h, "permute")
+        return self._wrap(h)
+
+    def permute_plan(self, shift: int, p: Sequence[int]):
+        nt, ns, ne = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().xt_permute_plan_stats(self.h, shift, _ints(p), len(p), C.byref(nt), C.byref(ns), C.byref(ne)), self.rt.h, "plan")
+        n_t, n_s = nt.value, ns.value
+        td = np.zeros(max(n_t, 1), np.int64)
+        tr = np.zeros(max(n_t, 1), np.int64)
+        tc = np.zeros(max(n_t, 1), np.int64)
+        tn = np.zeros(max(n_t, 1), np.int32)
+        so = np.zeros(max(n_s, 1), np.int64)
+        sc = np.zeros(max(n_s, 1), np.float64)
+        I64 = C.POINTER(C.c_int64)
+        check(lib().xt_permute_plan_dump(self.h, shift, _ints(p), len(p), td.ctypes.data_as(I64), tr.ctypes.data_as(I64), tc.ctypes.data_as(I64),
+                                         tn.ctypes.data_as(C.POINTER(C.c_int32)), so.ctypes.data_as(I64), sc.ctypes.data_as(C.POINTER(C.c_double))),
+              self.rt.h, "plan dump")
+        return {"task_dst": td[:n_t], "task_rows": tr[:n_t], "task_cols": tc[:n_t], "task_nsrc": tn[:n_t], "src_off": so[:n_s], "src_coeff": sc[:n_s],
+                "elems": ne.value}
+
+    def __mul__(self, other):
+        if isinstance(other, Tensor):
+            return self.mul(other)
+        return self.scale(float(other))
+
+    __rmul__ = lambda self, other: self.scale(float(other))
+
+    def mul(self, other: "Tensor", precision: str = "fp64") -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_mul(self.h, other.h, PRECISION[precision], C.byref(h)), self.rt.h, "operator*")
+        return self._wrap(h)
+
+    def contract(self, other: "Tensor", a1: Sequence[int], a2: Sequence[int], rankres: int, precision: str = "fp64") -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_contract(self.h, _ints(a1), len(a1), other.h, _ints(a2), len(a2), rankres, PRECISION[precision], C.byref(h)), self.rt.h,
+              "contract")
+        return self._wrap(h)
+
+    def tSVD(self, maxKeep: int, eps_svd: float, truncWeight: float = 0.0, entropy: float = 0.0, PRESERVE_MULTIPLETS: bool = True):
+        U, S, V = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        tw, en = C.c_double(truncWeight), C.c_double(entropy)
+        check(lib().xt_tsvd(self.h, int(maxKeep), float(eps_svd), int(bool(PRESERVE_MULTIPLETS)), C.byref(U), C.byref(S), C.byref(V), C.byref(tw),
+                            C.byref(en)), self.rt.h, "tSVD")
+        return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, en.value
+
+    def tQR(self):
+        Q, R = C.c_void_p(), C.c_void_p()
+        check(lib().xt_tqr(self.h, C.byref(Q), C.byref(R)), self.rt.h, "tQR")
+        return self._wrap(Q), self._wrap(R)
+
+    def teigh(self):
+        D, V = C.c_void_p(), C.c_void_p()
+        check(lib().xt_teigh(self.h, C.byref(D), C.byref(V)), self.rt.h, "teigh")
+        return self._wrap(D), self._wrap(V)
+
+    def _unary(self, fn, *args):
+        h = C.c_void_p()
+        check(fn(self.h, *args, C.byref(h)), self.rt.h, fn.__name__)
+        return self._wrap(h)
+
+    def adjoint(self) -> "Tensor":
+        return self._unary(lib().xt_adjoint)
+
+    def twist(self, leg: int) -> "Tensor":
+        return self._unary(lib().xt_twist, leg)
+
+    def scale(self, alpha: float) -> "Tensor":
+        return self._unary(lib().xt_scale, alpha)
+
+    def axpby(self, other: "Tensor", a: float, b: float) -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_axpby(self.h, other.h, a, b, C.byref(h)), self.rt.h, "axpby")
+        return self._wrap(h)
+
+    def __add__(self, other):
+        return self.axpby(other, 1.0, 1.0)
+
+    def __sub__(self, other):
+        return other.axpby(self, -1.0, 1.0)
+
+    def diag_sqrt(self):
+        return self._unary(lib().xt_diag_map, 0)
+
+    def diag_inv(self):
+        return self._unary(lib().xt_diag_map, 1)
+
+    def _scalar(self, fn):
+        v = C.c_double()
+        check(fn(self.h, C.byref(v)), self.rt.h, fn.__name__)
+        return v.value
+
+    def trace(self) -> float:
+        return self._scalar(lib().xt_trace)
+
+    def squaredNorm(self) -> float:
+        return self._scalar(lib().xt_squared_norm)
+
+    def norm(self) -> float:
+        return float(np.sqrt(self.squaredNorm()))
+
+    def maxNorm(self) -> float:
+        return self._scalar(lib().xt_max_norm)
+
+
+def get_permutations(a1, rank1, a2, rank2, rankres):
+    """util::constFct::get_permutations through the C ABI."""
+    n1, n2 = len(a1), len(a2)
+    p1, p2 = (C.c_int * n1)(), (C.c_int * n2)()
+    pres = (C.c_int * (n1 + n2))()
+    s1, s2, sr = C.c_int(), C.c_int(), C.c_int()
+    check(lib().xt_get_permutations(_ints(a1), n1, rank1, _ints(a2), n2, rank2, rankres, p1, C.byref(s1), p2, C.byref(s2), pres, C.byref(sr)),
+          None, "get_permutations")
+    nfree = sum(1 for x in a1 if x < 0) + sum(1 for x in a2 if x < 0)
+    return list(p1), s1.value, list(p2), s2.value, list(pres)[:nfree], sr.value
+
+
+def truncation_rule(sym, allSV, maxKeep, eps_svd, preserve_multiplets, truncWeight=0.0, entropy=0.0):
+    n = len(allSV)
+    qs = (C.c_int * (2 * max(n, 1)))()
+    sv = (C.c_double * max(n, 1))()
+    for i, (q, s) in enumerate(allSV):
+        q = tuple(q) + (0,) * (2 - len(tuple(q)))
+        qs[2 * i], qs[2 * i + 1] = q[0], q[1]
+        sv[i] = s
+    nsec = C.c_int()
+    kq = (C.c_int * (2 * max(n, 1)))()
+    kn = (C.c_int * max(n, 1))()
+    tw, en = C.c_double(truncWeight), C.c_double(entropy)
+    check(lib().xt_truncation_rule(sym[0], sym[1], qs, sv, n, maxKeep, eps_svd, int(preserve_multiplets), C.byref(nsec), kq, kn, C.byref(tw),
+                                   C.byref(en)), None, "truncation_rule")
+    nq = 2 if sym[1] else 1
+    kept = {tuple(kq[2 * i: 2 * i + nq]): kn[i] for i in range(nsec.value)}
+    return kept, tw.value, en.value
