@@ -146,12 +146,12 @@ def test_mul_and_contract(rt, pcode):
     oC = oten.mul(oA, oB)
     Cm = A * B
     assert structure_of(Cm) == oracle_structure(oC)
-    np.testing.assert_allclose(Cm.download(), oC.data, rtol=0, atol=1e-12 * max(1, np.abs(oC.data).max()))
+    np.testing.assert_allclose(Cm.download(), oC.data, rtol=0, atol=1e-12 * max(1, np.abs(oC.data).max(initial=0.0)))
     # contract over the shared leg with a reordered result
     oD = oten.contract(oA, [-2, -1, 1], oB, [1, -3, -4], 2)
     D = A.contract(B, [-2, -1, 1], [1, -3, -4], 2)
     assert structure_of(D) == oracle_structure(oD)
-    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max()))
+    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max(initial=0.0)))
 
 
 def _c1_pair(rt):
@@ -216,7 +216,7 @@ def test_tqr_and_teigh(rt):
     oH = oten.mul(oten.adjoint(o), o)
     D, V = H.teigh()
     oD, oV = oten.teigh(oH)
-    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max()))
+    np.testing.assert_allclose(D.download(), oD.data, rtol=0, atol=1e-12 * max(1, np.abs(oD.data).max(initial=0.0)))
     rec = (V * D) * V.adjoint()
     np.testing.assert_allclose(rec.download(), oH.data, rtol=0, atol=1e-11 * max(1, np.abs(oH.data).max()))
 

@@ -34,11 +34,14 @@ def c3_leg(seed: int, total: int = 512, exponent: float = 1.5) -> Entries:
     sectors = [(n, sz) for n in range(-6, 7) for sz in range(-6, 7) if (n - sz) % 2 == 0]
     w = [(1.0 + r.random() * 9.0) ** (-exponent) for _ in sectors]
     Z = sum(w)
+    if total < len(sectors):
+        raise ValueError(f"a leg of dimension {total} cannot hold {len(sectors)} sectors of degeneracy >= 1")
     degs = [max(1, int(round(total * x / Z))) for x in w]
-    # renormalise to the exact total by adjusting the largest sectors
+    # renormalise to the exact total: add to / take from the largest sectors (never below 1)
     diff = total - sum(degs)
     order = sorted(range(len(degs)), key=lambda i: -degs[i])
     k = 0
+    guard = 0
     while diff != 0:
         i = order[k % len(order)]
         step = 1 if diff > 0 else -1
@@ -46,6 +49,9 @@ def c3_leg(seed: int, total: int = 512, exponent: float = 1.5) -> Entries:
             degs[i] += step
             diff -= step
         k += 1
+        guard += 1
+        if guard > 100 * total:
+            raise RuntimeError("degeneracy renormalisation did not terminate")
     return [((n, sz), d) for (n, sz), d in zip(sectors, degs)]
 
 

@@ -55,13 +55,16 @@ __device__ __forceinline__ double load_b(const GemmArgs& g, const xh_gemm_item& 
     return g.tb ? g.B[it.b_off + int64_t(kk) * it.n + j] : g.B[it.b_off + int64_t(j) * it.k + kk];
 }
 
-template <typename Acc>
+// Output layout of the 16x16 accumulators: for the FP32-accumulating MFMAs register r of lane l
+// holds row 4*(l/16) + r; for v_mfma_f64_16x16x4_f64 it holds row (l/16) + 4*r.  Column = l % 16.
+template <bool F64LAYOUT, typename Acc>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, const xh_gemm_item& it, int m0, int n0, int wm, int wn, int lane, Acc acc[2][2])
 {
     for(int mi = 0; mi < 2; ++mi)
         for(int ni = 0; ni < 2; ++ni)
             for(int r = 0; r < 4; ++r) {
-                const int row = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + r;
+                const int rr = F64LAYOUT ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+                const int row = m0 + wm * 32 + mi * 16 + rr;
                 const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
                 if(row < it.m && col < it.n) {
                     double* p = g.C + it.c_off + int64_t(col) * it.m + row;
@@ -104,7 +107,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<true>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
     }
 }
 
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f32(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<false>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
     }
 }
 
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<false>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
     }
 }
 
