@@ -116,6 +116,28 @@ int xt_max_norm(const xt_tensor* t, double* out);               /* TensorBase.cp
 int xt_truncation_rule(int kind0, int kind1, const int* qs, const double* svs, int64_t n, size_t max_keep, double eps_svd, int preserve_multiplets,
                        int* nsec, int* kept_q, int* kept_n, double* trunc_weight, double* entropy);
 
+/* ---- CTMRG on a 2x2 unit cell (src/PEPS/CTM.cpp, TRank = 2, CORNER projection) ---------- */
+typedef struct xt_ctm xt_ctm;
+/* A: four site tensors Tensor<2,3>(left, top | right, bottom, phys) for (x,y) = (0,0),(0,1),(1,0),(1,1) */
+int xt_ctm_create(xt_rt* rt, const xt_tensor* const* A, size_t chi, xt_ctm** out);
+int xt_ctm_destroy(xt_ctm* c);
+/* precision: XH_FP64/XH_FP32/XH_BF16X3 for the contractions; svd_mode 0 = full FP64 block SVD,
+ * 1 = randomized block SVD with per-sector target ranks */
+int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int record_spectra);
+int xt_ctm_init(xt_ctm* c);                         /* CTM::init, FROM_A (CTM.cpp:137-240) */
+int xt_ctm_grow_all(xt_ctm* c);                     /* CTM::grow_all (CTM.cpp:386-420) */
+int xt_ctm_compute_rdm(xt_ctm* c);                  /* CTM::computeRDM (CTM.cpp:422-430) */
+int xt_ctm_energy(xt_ctm* c, double Jxy, double Jz, double* E); /* avg + CTMSolver.cpp:60-62 */
+/* which: 0..7 = C1..C4, T1..T4; 8 = rho_h, 9 = rho_v; site = x*2 + y */
+int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out);
+int xt_ctm_svd_ms(const xt_ctm* c, double* ms);
+/* recorded full spectra: number of records, then per record the sector count and flat arrays */
+int xt_ctm_num_spectra(const xt_ctm* c, int* n);
+int xt_ctm_spectrum(const xt_ctm* c, int rec, int* site, int* dir, int* nsec, int* qs, int* rows, int* cols, int* kept, int* nsv, double* sv, int cap);
+int xt_ctm_clear_spectra(xt_ctm* c);
+/* target ranks for the next randomized SVD of (site, dir): nsec pairs (q[2], rank) */
+int xt_ctm_set_ranks(xt_ctm* c, int site, int dir, const int* qs, const int* ranks, int nsec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -141,6 +141,19 @@ _SIGS = [
     ("xt_trace", C.c_int, [_P, _DP]),
     ("xt_squared_norm", C.c_int, [_P, _DP]),
     ("xt_max_norm", C.c_int, [_P, _DP]),
+    ("xt_ctm_create", C.c_int, [_P, C.POINTER(C.c_void_p), _SZ, _PP]),
+    ("xt_ctm_destroy", C.c_int, [_P]),
+    ("xt_ctm_set_options", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("xt_ctm_init", C.c_int, [_P]),
+    ("xt_ctm_grow_all", C.c_int, [_P]),
+    ("xt_ctm_compute_rdm", C.c_int, [_P]),
+    ("xt_ctm_energy", C.c_int, [_P, C.c_double, C.c_double, _DP]),
+    ("xt_ctm_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
+    ("xt_ctm_svd_ms", C.c_int, [_P, _DP]),
+    ("xt_ctm_num_spectra", C.c_int, [_P, _IP]),
+    ("xt_ctm_spectrum", C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, C.c_int]),
+    ("xt_ctm_clear_spectra", C.c_int, [_P]),
+    ("xt_ctm_set_ranks", C.c_int, [_P, C.c_int, C.c_int, _IP, _IP, C.c_int]),
     ("xt_truncation_rule", C.c_int, [C.c_int, C.c_int, _IP, _DP, C.c_int64, _SZ, C.c_double, C.c_int, _IP, _IP, _IP, _DP, _DP]),
 ]
 

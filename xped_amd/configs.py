@@ -61,3 +61,14 @@ def c4_virtual() -> Entries:
 
 def c4_physical() -> Entries:
     return [((-1,), 1), ((1,), 1)]
+
+
+def c4_small_virtual() -> Entries:
+    """D = 3 over charges -1..1 (a reduced C4 shape for parity tests)."""
+    return [((-1,), 1), ((0,), 1), ((1,), 1)]
+
+
+def ipeps_site_legs(virtual: Entries, physical: Entries):
+    """Leg entries of the four Tensor<2,3>(left, top | right, bottom, phys) site tensors of a 2x2
+    cell (order (0,0), (0,1), (1,0), (1,1)) with every virtual bond carrying `virtual`."""
+    return [[virtual, virtual, virtual, virtual, physical] for _ in range(4)]

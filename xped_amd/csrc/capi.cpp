@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/xped_tensor.h"
+#include "core/ctm.hpp"
 #include "core/tensor.hpp"
 
 struct xt_rt {
@@ -529,6 +530,130 @@ int xt_truncation_rule(int kind0, int kind1, const int* qs, const double* svs, i
         *nsec = k;
         *trunc_weight = r.trunc_weight;
         *entropy = r.entropy;
+    });
+}
+
+// ---------------------------------------------------------------- CTMRG
+
+}  // extern "C"
+
+struct xt_ctm {
+    std::unique_ptr<xa::CTM> ctm;
+    xt_rt* owner = nullptr;
+};
+
+extern "C" {
+
+int xt_ctm_create(xt_rt* rt, const xt_tensor* const* A, size_t chi, xt_ctm** out)
+{
+    return run(rt, [&] {
+        std::vector<xa::DTensor> a;
+        for(int i = 0; i < 4; ++i) a.push_back(A[i]->t);
+        auto* c = new xt_ctm();
+        c->ctm = std::make_unique<xa::CTM>(&rt->rt, a, chi);
+        c->owner = rt;
+        *out = c;
+    });
+}
+
+int xt_ctm_destroy(xt_ctm* c)
+{
+    delete c;
+    return XH_OK;
+}
+
+int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int record_spectra)
+{
+    return run(c->owner, [&] {
+        xa::CTMOptions o = c->ctm->opt;
+        o.precision = precision;
+        o.svd = svd_mode == 1 ? xa::SvdMode::Randomized : xa::SvdMode::Full;
+        o.oversample = oversample;
+        o.power_iters = power_iters;
+        o.record_spectra = record_spectra != 0;
+        c->ctm->set_options(o);
+    });
+}
+
+int xt_ctm_init(xt_ctm* c)
+{
+    return run(c->owner, [&] { c->ctm->init_from_A(); });
+}
+
+int xt_ctm_grow_all(xt_ctm* c)
+{
+    return run(c->owner, [&] { c->ctm->grow_all(); });
+}
+
+int xt_ctm_compute_rdm(xt_ctm* c)
+{
+    return run(c->owner, [&] { c->ctm->compute_rdm(); });
+}
+
+int xt_ctm_energy(xt_ctm* c, double Jxy, double Jz, double* E)
+{
+    return run(c->owner, [&] { *E = c->ctm->energy(Jxy, Jz); });
+}
+
+int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out)
+{
+    return run(c->owner, [&] {
+        if(site < 0 || site >= 4 || which < 0 || which > 9) throw xa::Error(XH_EINVAL, "ctm tensor index out of range");
+        xa::CTM& m = *c->ctm;
+        std::vector<xa::DTensor>* tabs[10] = {&m.C1, &m.C2, &m.C3, &m.C4, &m.T1, &m.T2, &m.T3, &m.T4, &m.rho_h, &m.rho_v};
+        const xa::DTensor& t = (*tabs[which])[site];
+        if(!t.st) throw xa::Error(XH_EINVAL, "ctm tensor not computed yet");
+        *out = wrap(c->owner, t);
+    });
+}
+
+int xt_ctm_svd_ms(const xt_ctm* c, double* ms)
+{
+    *ms = c->ctm->svd_ms;
+    return XH_OK;
+}
+
+int xt_ctm_num_spectra(const xt_ctm* c, int* n)
+{
+    *n = static_cast<int>(c->ctm->spectra.size());
+    return XH_OK;
+}
+
+int xt_ctm_spectrum(const xt_ctm* c, int rec, int* site, int* dir, int* nsec, int* qs, int* rows, int* cols, int* kept, int* nsv, double* sv, int cap)
+{
+    return run(c->owner, [&] {
+        const auto& r = c->ctm->spectra.at(rec);
+        *site = r.site;
+        *dir = r.dir;
+        int k = 0, off = 0;
+        for(const auto& [q, v] : r.sv) {
+            q_to(q, qs + 2 * k);
+            auto sh = r.shape.at(q);
+            rows[k] = sh.first;
+            cols[k] = sh.second;
+            auto it = r.kept.find(q);
+            kept[k] = it == r.kept.end() ? 0 : it->second;
+            nsv[k] = static_cast<int>(v.size());
+            if(off + static_cast<int>(v.size()) > cap) throw xa::Error(XH_EINVAL, "spectrum buffer too small");
+            for(double x : v) sv[off++] = x;
+            ++k;
+        }
+        *nsec = k;
+    });
+}
+
+int xt_ctm_clear_spectra(xt_ctm* c)
+{
+    c->ctm->spectra.clear();
+    return XH_OK;
+}
+
+int xt_ctm_set_ranks(xt_ctm* c, int site, int dir, const int* qs, const int* ranks, int nsec)
+{
+    return run(c->owner, [&] {
+        std::map<xa::Q, int> m;
+        for(int i = 0; i < nsec; ++i) m[q_from(qs + 2 * i)] = ranks[i];
+        c->ctm->last_kept[{site, dir}] = m;
     });
 }
 

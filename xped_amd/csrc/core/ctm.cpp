@@ -1,0 +1,434 @@
+// CTMRG driver on the GPU tensor layer (see ctm.hpp for the reference functions it follows).
+//
+// The contraction index lists below restate the reference's network (CTM.cpp, TRank = 2,
+// ALL_OUT_LEGS = false).  Twists are kept where the reference applies them; they are the
+// identity for bosonic symmetries.
+#include "ctm.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+
+namespace xa {
+
+namespace {
+enum Leg { L_LEFT = 0, L_TOP = 1, L_RIGHT = 2, L_BOTTOM = 3, L_PHYS = 4 };
+enum Dir { D_LEFT = 0, D_RIGHT = 1, D_TOP = 2, D_BOTTOM = 3 };
+enum Corner { UPPER_LEFT = 0, LOWER_LEFT = 1, UPPER_RIGHT = 2, LOWER_RIGHT = 3 };
+
+DTensor tw(const DTensor& t, std::initializer_list<int> legs)
+{
+    DTensor out = t;
+    for(int l : legs) out = twist(out, l);
+    return out;
+}
+}  // namespace
+
+CTM::CTM(Runtime* r, const std::vector<DTensor>& A, std::size_t chi) : rt(r)
+{
+    if(A.size() != Lx * Ly) throw Error(XH_EINVAL, "CTM: expected 4 site tensors for the 2x2 unit cell");
+    opt.chi = chi;
+    As = A;
+    // iPEPS::updateAdags (iPEPS.cpp:200-214): Adag = A^dagger permuted to (left, top, phys | right, bottom)
+    for(const auto& a : As) Adags.push_back(permute(adjoint(a), 0, {3, 4, 2, 0, 1}));
+    const int n = Lx * Ly;
+    C1.resize(n);
+    C2.resize(n);
+    C3.resize(n);
+    C4.resize(n);
+    T1.resize(n);
+    T2.resize(n);
+    T3.resize(n);
+    T4.resize(n);
+    rho_h.resize(n);
+    rho_v.resize(n);
+}
+
+DTensor CTM::ct(const DTensor& a, std::vector<int> a1, const DTensor& b, std::vector<int> a2, int rank) const
+{
+    return contract(a, a1, b, a2, rank, opt.precision);
+}
+
+DTensor CTM::identity(const std::vector<Qbasis>& dom, const std::vector<Qbasis>& cod) const
+{
+    // Tensor::Identity / setIdentity (Tensor.hpp:229-235, Tensor.cpp:58-68): every block gets ones
+    // on its main diagonal
+    StructPtr st = cached_structure(rt, dom.empty() ? cod[0].sym : dom[0].sym, dom, cod);
+    DTensor t = DTensor::empty(rt, st);
+    std::vector<double> h(st->size, 0.0);
+    for(std::size_t i = 0; i < st->sector.size(); ++i)
+        for(std::size_t k = 0; k < std::min(st->rows[i], st->cols[i]); ++k) h[st->offsets[i] + k + k * st->rows[i]] = 1.0;
+    if(st->size) t.upload(h.data(), h.size());
+    return t;
+}
+
+Qbasis CTM::ket(int x, int y, int leg) const
+{
+    const TensorStructure& s = *A(x, y).st;
+    switch(leg) {
+    case L_LEFT: return s.unc_dom[0];
+    case L_TOP: return s.unc_dom[1];
+    case L_RIGHT: return s.unc_cod[0];
+    case L_BOTTOM: return s.unc_cod[1];
+    default: return s.unc_cod[2];
+    }
+}
+
+Qbasis CTM::bra(int x, int y, int leg) const
+{
+    const TensorStructure& s = *Adag(x, y).st;
+    switch(leg) {
+    case L_LEFT: return s.unc_dom[0];
+    case L_TOP: return s.unc_dom[1];
+    case L_RIGHT: return s.unc_cod[0];
+    case L_BOTTOM: return s.unc_cod[1];
+    default: return s.unc_dom[2];
+    }
+}
+
+DTensor CTM::normalized(const DTensor& t) const { return scaled(t, 1.0 / max_norm(t)); }
+
+void CTM::init_from_A()
+{
+    for(int x = 0; x < Lx; ++x)
+        for(int y = 0; y < Ly; ++y) {
+            const int p = idx(x, y);
+            const DTensor& a = A(x, y);
+            const DTensor& ad = Adag(x, y);
+            auto fused = [&](int leg) { return ket(x, y, leg).combine(bra(x, y, leg)).forget_history(); };
+            DTensor fuse_ll = identity({fused(L_LEFT)}, {ket(x, y, L_LEFT), bra(x, y, L_LEFT)});
+            DTensor fuse_uu = identity({fused(L_TOP)}, {ket(x, y, L_TOP), bra(x, y, L_TOP)});
+            DTensor fuse_rr = identity({ket(x, y, L_RIGHT), bra(x, y, L_RIGHT)}, {fused(L_RIGHT)});
+            DTensor fuse_dd = identity({ket(x, y, L_BOTTOM), bra(x, y, L_BOTTOM)}, {fused(L_BOTTOM)});
+            C1[p] = ct(ct(ct(a, {1, 2, -1, -2, 3}, tw(ad, {3, 4}), {1, 2, 3, -3, -4}, 2), {1, -1, 2, -2}, fuse_rr, {1, 2, -3}, 2), {1, 2, -2}, fuse_dd,
+                       {1, 2, -1}, 0);
+            C2[p] = ct(ct(ct(a, {-1, 1, 2, -2, 3}, tw(ad, {4}), {-3, 1, 3, 2, -4}, 2), {1, -1, 2, -2}, tw(fuse_ll, {1, 2}), {-3, 1, 2}, 2), {1, 2, -1},
+                       fuse_dd, {1, 2, -2}, 1);
+            C3[p] = ct(ct(ct(a, {-1, -2, 1, 2, 3}, ad, {-3, -4, 3, 1, 2}, 2), {1, -1, 2, -2}, tw(fuse_ll, {1, 2}), {-3, 1, 2}, 2), {1, 2, -2},
+                       tw(fuse_uu, {1, 2}), {-1, 1, 2}, 2);
+            C4[p] = ct(ct(ct(a, {1, -1, -2, 2, 3}, tw(ad, {3}), {1, -3, 3, -4, 2}, 2), {1, -1, 2, -2}, tw(fuse_uu, {1, 2}), {-3, 1, 2}, 2), {1, 2, -1},
+                       fuse_rr, {1, 2, -2}, 1);
+            T1[p] = ct(ct(ct(a, {-1, 1, -2, -3, 2}, tw(ad, {3, 4}), {-4, 1, 2, -5, -6}, 3), {1, -1, -2, 2, -3, -4}, tw(fuse_ll, {1, 2}), {-5, 1, 2}, 4),
+                       {1, -3, 2, -4, -1}, fuse_rr, {1, 2, -2}, 1);
+            T2[p] = ct(ct(ct(a, {-1, -2, 1, -3, 2}, tw(ad, {4}), {-4, -5, 2, 1, -6}, 3), {-1, 1, -2, -3, 2, -4}, tw(fuse_uu, {1, 2}), {-5, 1, 2}, 4),
+                       {-1, 1, -2, 2, -3}, fuse_dd, {1, 2, -4}, 3);
+            T3[p] = ct(ct(ct(a, {-1, -2, -3, 1, 2}, tw(ad, {3}), {-4, -5, 2, -6, 1}, 3), {1, -1, -2, 2, -3, -4}, tw(fuse_ll, {1, 2}), {-5, 1, 2}, 4),
+                       {-1, 1, -2, 2, -3}, fuse_rr, {1, 2, -4}, 3);
+            T4[p] = ct(ct(ct(a, {1, -1, -2, -3, 2}, tw(ad, {3, 4}), {1, -4, 2, -5, -6}, 3), {1, -1, -2, 2, -3, -4}, tw(fuse_uu, {1, 2}), {-5, 1, 2}, 4),
+                       {-3, 1, -4, 2, -1}, fuse_dd, {1, 2, -2}, 1);
+        }
+}
+
+DTensor CTM::contract_corner(int x, int y, int corner) const
+{
+    switch(corner) {
+    case UPPER_LEFT: {
+        DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -2, -3, -4}, 1);
+        DTensor T4C1T1 = ct(T4[idx(x - 1, y)], {1, -1, -2, -3}, tw(C1T1, {0}), {1, -4, -5, -6}, 3);
+        DTensor T4C1T1A = ct(T4C1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
+        return ct(T4C1T1A, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3);
+    }
+    case LOWER_LEFT: {
+        DTensor C4T3 = ct(C4[idx(x - 1, y + 1)], {-1, 1}, T3[idx(x, y + 1)], {-2, -3, 1, -4}, 1);
+        DTensor T4C4T3 = ct(T4[idx(x - 1, y)], {-1, 1, -2, -3}, C4T3, {1, -4, -5, -6}, 3);
+        DTensor T4C4T3A = ct(T4C4T3, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {3}), {1, -5, -6, 2, -7}, 4);
+        return ct(T4C4T3A, {-4, 1, 2, -1, -5, -2, 3}, tw(Adag(x, y), {3}), {1, -6, 3, -3, 2}, 3);
+    }
+    case UPPER_RIGHT: {
+        DTensor T1C2 = ct(T1[idx(x, y - 1)], {-1, 1, -2, -3}, C2[idx(x + 1, y - 1)], {1, -4}, 3);
+        DTensor T1C2T2 = ct(T1C2, {-1, -2, -3, 1}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3);
+        DTensor T1C2T2A = ct(T1C2T2, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4);
+        return ct(T1C2T2A, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3);
+    }
+    default: {
+        DTensor C3T3 = ct(tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 1);
+        DTensor T2C3T3 = ct(T2[idx(x + 1, y)], {-1, -2, -3, 1}, C3T3, {1, -4, -5, -6}, 3);
+        DTensor T2C3T3A = ct(T2C3T3, {1, -1, -2, 2, -3, -4}, tw(A(x, y), {2, 3}), {-5, -6, 1, 2, -7}, 4);
+        return ct(T2C3T3A, {1, -1, 2, -4, -5, -2, 3}, Adag(x, y), {-6, -3, 3, 1, 2}, 3);
+    }
+    }
+}
+
+std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb, int site, int dir)
+{
+    // PEPSContractions.cpp:16-43: prod = T1 * T2, tSVD(chi, 1e-10, ., PRESERVE_MULTIPLETS = false),
+    // P1 = S^-1/2 U^dagger T1, P2 = T2 V S^-1/2
+    DTensor prod = mul(Ta, Tb, opt.precision);
+    auto t0 = std::chrono::steady_clock::now();
+    SVDResult r;
+    if(opt.svd == SvdMode::Full) {
+        FullSVD f = full_svd(prod);
+        r = cutoff(prod, f, opt.chi, 1.e-10, false, 0.0, 0.0);
+        std::map<Q, int> kept;
+        for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
+        last_kept[{site, dir}] = kept;
+        if(opt.record_spectra) {
+            SpectrumRecord rec;
+            rec.site = site;
+            rec.dir = dir;
+            for(const auto& [q, s] : f.all_sv) rec.sv[q].push_back(s);
+            rec.kept = kept;
+            for(std::size_t i = 0; i < prod.st->sector.size(); ++i)
+                rec.shape[prod.st->sector[i]] = {static_cast<int>(prod.st->rows[i]), static_cast<int>(prod.st->cols[i])};
+            spectra.push_back(std::move(rec));
+        }
+    } else {
+        std::map<Q, int> ranks;
+        if(opt.rank_policy)
+            ranks = opt.rank_policy(prod, site, dir);
+        else {
+            auto it = last_kept.find({site, dir});
+            if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
+            ranks = it->second;
+        }
+        r = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.oversample, opt.power_iters, opt.precision, 1234u + 17u * site + 5u * dir);
+        std::map<Q, int> kept;
+        for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
+        last_kept[{site, dir}] = kept;
+    }
+    if(rt->ctx) xh_synchronize(rt->ctx);
+    svd_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    DTensor isqrtS = diag_map(r.S, 2);  // S.diag_sqrt().diag_inv()
+    DTensor P1 = mul(mul(isqrtS, adjoint(r.U), opt.precision), Ta, opt.precision);
+    DTensor P2 = mul(mul(Tb, adjoint(r.Vdag), opt.precision), isqrtS, opt.precision);
+    return {P1, P2};
+}
+
+std::pair<DTensor, DTensor> CTM::get_projectors(int x, int y, int dir)
+{
+    const int site = idx(x, y);
+    switch(dir) {
+    case D_LEFT: {
+        DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
+        DTensor Q4 = tw(contract_corner(x, y + 1, LOWER_LEFT), {3, 4, 5});
+        return decompose(Q4, Q1, site, dir);
+    }
+    case D_RIGHT: {
+        DTensor Q2 = contract_corner(x, y, UPPER_RIGHT);
+        DTensor Q3 = contract_corner(x, y + 1, LOWER_RIGHT);
+        return decompose(Q2, Q3, site, dir);
+    }
+    case D_TOP: {
+        DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
+        DTensor Q2 = contract_corner(x + 1, y, UPPER_RIGHT);
+        return decompose(Q1, Q2, site, dir);
+    }
+    default: {
+        DTensor Q4 = tw(contract_corner(x, y, LOWER_LEFT), {0, 1, 2});
+        DTensor Q3 = contract_corner(x + 1, y, LOWER_RIGHT);
+        return decompose(Q3, Q4, site, dir);
+    }
+    }
+}
+
+void CTM::left_move()
+{
+    // CTM.cpp:924-988; projector and new-tensor tables are indexed like the unit cell
+    for(int x = 0; x < Lx; ++x) {
+        std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C1n(Lx * Ly), T4n(Lx * Ly), C4n(Lx * Ly);
+        for(int y = 0; y < Ly; ++y) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_LEFT);
+        for(int y = 0; y < Ly; ++y) {
+            // renormalize_left (CTM.cpp:1339-1416)
+            DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -4, -2, -3}, 3);
+            C1n[idx(x, y - 1)] = normalized(ct(tw(P1[idx(x, y - 1)], {1, 2, 3}), {-1, 1, 2, 3}, C1T1, {1, 2, 3, -2}, 0));
+            DTensor C4T3 = ct(C4[idx(x - 1, y + 1)], {-2, 1}, T3[idx(x, y + 1)], {-3, -4, 1, -1}, 1);
+            C4n[idx(x, y + 1)] = normalized(ct(C4T3, {-2, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -1}, 1));
+            DTensor P1T4 = ct(tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, T4[idx(x - 1, y)], {-4, 1, -5, -6}, 3);
+            DTensor P1T4A = ct(P1T4, {-1, 2, -2, -3, 1, -4}, A(x, y), {1, -5, -6, 2, -7}, 4);
+            DTensor P1T4AAdag = ct(P1T4A, {-1, 2, -4, 1, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, -6, 3, -3, 2}, 3);
+            T4n[idx(x, y)] = normalized(ct(P1T4AAdag, {-2, -3, -4, 1, 2, 3}, tw(P2[idx(x, y - 1)], {0, 1, 2}), {1, 2, 3, -1}, 1));
+        }
+        for(int y = 0; y < Ly; ++y) {
+            C1[idx(x, y)] = C1n[idx(x, y)];
+            T4[idx(x, y)] = T4n[idx(x, y)];
+            C4[idx(x, y)] = C4n[idx(x, y)];
+        }
+    }
+}
+
+void CTM::right_move()
+{
+    // CTM.cpp:990-1051: columns are visited from x = Lx (== 0) downwards; every column once
+    for(int xs = Lx; xs > 0; --xs) {
+        const int x = xs % Lx;
+        std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C2n(Lx * Ly), T2n(Lx * Ly), C3n(Lx * Ly);
+        for(int y = 0; y < Ly; ++y) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_RIGHT);
+        for(int y = 0; y < Ly; ++y) {
+            // renormalize_right (CTM.cpp:1418-1488)
+            DTensor T1C2 = ct(T1[idx(x, y - 1)], {-1, 1, -3, -4}, C2[idx(x + 1, y - 1)], {1, -2}, 1);
+            C2n[idx(x, y - 1)] = normalized(ct(T1C2, {-1, 1, 2, 3}, P2[idx(x, y - 1)], {1, 2, 3, -2}, 1));
+            DTensor C3T3 = ct(tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 3);
+            C3n[idx(x, y + 1)] = normalized(ct(P1[idx(x, y)], {-1, 1, 2, 3}, C3T3, {1, 2, 3, -2}, 2));
+            DTensor P1T2 = ct(P1[idx(x, y - 1)], {-1, 1, -2, -3}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3);
+            DTensor P1T2A = ct(P1T2, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4);
+            DTensor P1T2AAdag = ct(P1T2A, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3);
+            T2n[idx(x, y)] = normalized(ct(P1T2AAdag, {-3, -1, -2, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -4}, 3));
+        }
+        for(int y = 0; y < Ly; ++y) {
+            C2[idx(x, y)] = C2n[idx(x, y)];
+            T2[idx(x, y)] = T2n[idx(x, y)];
+            C3[idx(x, y)] = C3n[idx(x, y)];
+        }
+    }
+}
+
+void CTM::top_move()
+{
+    // CTM.cpp:1053-1117
+    for(int y = 0; y < Ly; ++y) {
+        std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C1n(Lx * Ly), T1n(Lx * Ly), C2n(Lx * Ly);
+        for(int x = 0; x < Lx; ++x) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_TOP);
+        for(int x = 0; x < Lx; ++x) {
+            // renormalize_top (CTM.cpp:1490-1562)
+            DTensor T4C1 = ct(T4[idx(x - 1, y)], {1, -1, -3, -4}, tw(C1[idx(x - 1, y - 1)], {0}), {1, -2}, 1);
+            C1n[idx(x - 1, y)] = normalized(ct(T4C1, {-1, 1, 2, 3}, P2[idx(x - 1, y)], {1, 2, 3, -2}, 0));
+            DTensor C2T2 = ct(C2[idx(x + 1, y - 1)], {-1, 1}, T2[idx(x + 1, y)], {-2, -3, 1, -4}, 3);
+            C2n[idx(x + 1, y)] = normalized(ct(P1[idx(x, y)], {-1, 1, 2, 3}, C2T2, {1, 2, 3, -2}, 1));
+            DTensor P1T1 = ct(P1[idx(x - 1, y)], {-1, 1, -2, -3}, T1[idx(x, y - 1)], {1, -4, -5, -6}, 3);
+            DTensor P1T1A = ct(P1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
+            DTensor P1T1AAdag = ct(P1T1A, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3);
+            T1n[idx(x, y)] = normalized(ct(P1T1AAdag, {-1, -3, -4, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -2}, 1));
+        }
+        for(int x = 0; x < Lx; ++x) {
+            C1[idx(x, y)] = C1n[idx(x, y)];
+            T1[idx(x, y)] = T1n[idx(x, y)];
+            C2[idx(x, y)] = C2n[idx(x, y)];
+        }
+    }
+}
+
+void CTM::bottom_move()
+{
+    // CTM.cpp:1119-1183: rows are visited from y = Ly (== 0) downwards; every row once
+    for(int ys = Ly; ys > 0; --ys) {
+        const int y = ys % Ly;
+        std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C4n(Lx * Ly), T3n(Lx * Ly), C3n(Lx * Ly);
+        for(int x = 0; x < Lx; ++x) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_BOTTOM);
+        for(int x = 0; x < Lx; ++x) {
+            // renormalize_bottom (CTM.cpp:1564-1641)
+            DTensor T4C4 = ct(T4[idx(x - 1, y)], {-4, 1, -2, -3}, C4[idx(x - 1, y + 1)], {1, -1}, 3);
+            C4n[idx(x - 1, y)] = normalized(ct(tw(P1[idx(x - 1, y)], {1, 2, 3}), {-2, 1, 2, 3}, T4C4, {1, 2, 3, -1}, 1));
+            DTensor T2C3 = ct(T2[idx(x + 1, y)], {-3, -4, -1, 1}, C3[idx(x + 1, y + 1)], {1, -2}, 1);
+            C3n[idx(x + 1, y)] = normalized(ct(T2C3, {-1, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -2}, 2));
+            DTensor P1T3 = ct(tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, tw(T3[idx(x, y + 1)], {0}), {-4, -5, -6, 1}, 3);
+            DTensor P1T3A = ct(P1T3, {-1, 1, -2, 2, -3, -4}, A(x, y), {-5, -6, 1, 2, -7}, 4);
+            DTensor P1T3AAdag = ct(P1T3A, {-1, 1, 2, -4, -5, -2, 3}, tw(Adag(x, y), {3}), {-6, -3, 3, 1, 2}, 3);
+            T3n[idx(x, y)] = normalized(ct(P1T3AAdag, {-4, -1, -2, 1, 2, 3}, tw(P2[idx(x - 1, y)], {0, 1, 2}), {1, 2, 3, -3}, 3));
+        }
+        for(int x = 0; x < Lx; ++x) {
+            C4[idx(x, y)] = C4n[idx(x, y)];
+            T3[idx(x, y)] = T3n[idx(x, y)];
+            C3[idx(x, y)] = C3n[idx(x, y)];
+        }
+    }
+}
+
+void CTM::grow_all()
+{
+    left_move();
+    right_move();
+    top_move();
+    bottom_move();
+}
+
+void CTM::compute_rdm()
+{
+    auto normalize_rho = [&](DTensor rho) {
+        DTensor Id2 = identity(rho.st->unc_cod, rho.st->unc_dom);
+        const double norm = trace(ct(rho, {1, 2, 3, 4}, tw(Id2, {0, 1}), {3, 4, 1, 2}, 0));
+        return scaled(rho, 1.0 / norm);
+    };
+    for(int x = 0; x < Lx; ++x)
+        for(int y = 0; y < Ly; ++y) {
+            // computeRDM_h (CTM.cpp:528-640, TRank = 2)
+            DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -2, -3, -4}, 1);
+            DTensor T4C1T1 = ct(T4[idx(x - 1, y)], {1, -1, -2, -3}, tw(C1T1, {0}), {1, -4, -5, -6}, 3);
+            DTensor T4C1T1A = ct(T4C1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
+            DTensor C4T3 = ct(C4[idx(x - 1, y + 1)], {-1, 1}, T3[idx(x, y + 1)], {-2, -3, 1, -4}, 1);
+            DTensor C4T3Ad = ct(C4T3, {-1, -2, 1, -3}, tw(Adag(x, y), {3}), {-4, -5, -6, -7, 1}, 3);
+            DTensor left_half = ct(T4C1T1A, {1, 2, -1, 3, -2, 4, -3}, C4T3Ad, {1, 4, -4, 2, 3, -5, -6}, 3);
+            DTensor T1C2 = ct(T1[idx(x + 1, y - 1)], {-1, 1, -2, -3}, C2[idx(x + 2, y - 1)], {1, -4}, 3);
+            DTensor T1C2T2 = ct(T1C2, {-1, -2, -3, 1}, T2[idx(x + 2, y)], {-4, -5, 1, -6}, 3);
+            DTensor AT1C2T2 = ct(tw(A(x + 1, y), {1}), {-1, 1, 2, -2, -3}, T1C2T2, {-4, 1, -5, 2, -6, -7}, 3);
+            DTensor T3C3 = ct(T3[idx(x + 1, y + 1)], {-1, -2, -3, 1}, C3[idx(x + 2, y + 1)], {-4, 1}, 3);
+            DTensor T3C3Ad = ct(T3C3, {-1, 1, -2, -3}, Adag(x + 1, y), {-4, -5, -6, -7, 1}, 3);
+            DTensor right_half = ct(AT1C2T2, {-1, 1, -2, -3, 2, 3, 4}, T3C3Ad, {1, -4, 4, -5, 2, -6, 3}, 3);
+            rho_h[idx(x, y)] = normalize_rho(ct(left_half, {1, 2, -3, 3, -1, 4}, right_half, {2, -4, 1, 3, 4, -2}, 2));
+        }
+    for(int x = 0; x < Lx; ++x)
+        for(int y = 0; y < Ly; ++y) {
+            // computeRDM_v (CTM.cpp:706-800, TRank = 2)
+            DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -2, -3, -4}, 1);
+            DTensor T4C1T1 = ct(T4[idx(x - 1, y)], {1, -1, -2, -3}, tw(C1T1, {0}), {1, -4, -5, -6}, 3);
+            DTensor T4C1T1A = ct(T4C1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
+            DTensor C2T2 = ct(C2[idx(x + 1, y - 1)], {-1, 1}, T2[idx(x + 1, y)], {-2, -3, 1, -4}, 1);
+            DTensor AdC2T2 = ct(tw(Adag(x, y), {3, 4}), {-1, -2, -3, 1, -4}, C2T2, {-5, -6, 1, -7}, 4);
+            DTensor upper_half = ct(T4C1T1A, {-1, 1, 2, 3, 4, -2, -3}, AdC2T2, {1, 3, -4, -5, 2, 4, -6}, 3);
+            DTensor C4T3 = ct(C4[idx(x - 1, y + 2)], {-1, 1}, T3[idx(x, y + 2)], {-2, -3, 1, -4}, 1);
+            DTensor T4C4T3 = ct(T4[idx(x - 1, y + 1)], {-1, 1, -2, -3}, C4T3, {1, -4, -5, -6}, 3);
+            DTensor AT4C4T3 = ct(tw(A(x, y + 1), {0}), {1, -1, -2, 2, -3}, T4C4T3, {-4, 1, -5, 2, -6, -7}, 3);
+            DTensor T2C3 = ct(T2[idx(x + 1, y + 1)], {-1, -2, -3, 1}, C3[idx(x + 1, y + 2)], {1, -4}, 3);
+            DTensor T2C3Ad = ct(T2C3, {-1, 1, -2, -3}, Adag(x, y + 1), {-4, -5, -6, 1, -7}, 3);
+            DTensor lower_half = ct(AT4C4T3, {-1, 1, -2, -3, 2, 3, 4}, T2C3Ad, {1, -4, 4, 2, -5, -6, 3}, 3);
+            rho_v[idx(x, y)] = normalize_rho(ct(upper_half, {1, 2, -3, -1, 3, 4}, lower_half, {2, -4, 1, 4, 3, -2}, 2));
+        }
+}
+
+double CTM::energy(double Jxy, double Jz) const
+{
+    // Heisenberg gate Jz Sz.Sz + Jxy/2 (S+S- + S-S+) for spin 1/2 (Heisenberg.hpp:47-53), built on
+    // the bases of the reduced density matrix; labels of a U(1) leg are 2 Sz (negated on a
+    // conjugated leg).  <H_bond> = rho.contract<{1,2,3,4},{3,4,1,2},0>(gate).trace()
+    // (LinearAlgebra.cpp:218-246); E = (sum_h + sum_v) / number of sites (CTMSolver.cpp:60-62).
+    auto bond = [&](const DTensor& rho) {
+        const TensorStructure& rs = *rho.st;
+        StructPtr gs = cached_structure(rt, rs.sym, rs.unc_cod, rs.unc_dom);
+        std::vector<double> h(gs->size, 0.0);
+        for(std::size_t i = 0; i < gs->sector.size(); ++i) {
+            const Q& q = gs->sector[i];
+            const auto& dtrees = gs->dom.tree(q);
+            const auto& ctrees = gs->cod.tree(q);
+            std::size_t r0 = 0;
+            for(const auto& td : dtrees) {
+                std::size_t c0 = 0;
+                for(const auto& tc : ctrees) {
+                    if(td.dim == 1 && tc.dim == 1) {
+                        auto sz = [&](const Qbasis& b, const Q& lab) { return 0.5 * (b.conj_flag ? -lab.v[0] : lab.v[0]); };
+                        const double c1 = sz(gs->unc_dom[0], td.unc[0]), c2 = sz(gs->unc_dom[1], td.unc[1]);
+                        const double a1 = sz(gs->unc_cod[0], tc.unc[0]), a2 = sz(gs->unc_cod[1], tc.unc[1]);
+                        double v = 0.0;
+                        if(c1 == a1 && c2 == a2) v = Jz * a1 * a2;
+                        if(c1 == a1 + 1 && c2 == a2 - 1) v = 0.5 * Jxy;  // S+ S-
+                        if(c1 == a1 - 1 && c2 == a2 + 1) v = 0.5 * Jxy;  // S- S+
+                        h[gs->offsets[i] + r0 + c0 * gs->rows[i]] = v;
+                    }
+                    c0 += tc.dim;
+                }
+                r0 += td.dim;
+            }
+        }
+        DTensor gate = DTensor::empty(rt, gs);
+        if(gs->size) gate.upload(h.data(), h.size());
+        return trace(ct(tw(rho, {0, 1}), {1, 2, 3, 4}, gate, {3, 4, 1, 2}, 0));
+    };
+    double e = 0.0;
+    for(int i = 0; i < Lx * Ly; ++i) e += bond(rho_h[i]);
+    for(int i = 0; i < Lx * Ly; ++i) e += bond(rho_v[i]);
+    return e / (Lx * Ly);
+}
+
+double CTM::step()
+{
+    grow_all();
+    compute_rdm();
+    return energy();
+}
+
+}  // namespace xa
+
+namespace xa {
+SVDResult tsvd_randomized(const DTensor&, const std::map<Q, int>&, std::size_t, double, int, int, int, std::uint32_t)
+{
+    throw Error(XH_EINVAL, "randomized SVD is not available yet");
+}
+}  // namespace xa

@@ -1,0 +1,95 @@
+// Corner-transfer-matrix renormalisation (CTMRG) of an iPEPS on a 2x2 unit cell, driven on the GPU.
+//
+// Follows the reference's CTM<Scalar, Symmetry, TRank = 2, ALL_OUT_LEGS = false> with CORNER
+// projection (src/PEPS/CTM.cpp): FROM_A initialisation (:137-240), grow_all (:386-420) with the
+// left/right/top/bottom moves (:924-1183), get_projectors (:1185-1337) + decompose
+// (PEPSContractions.cpp:16-43), renormalize_* (:1339-1641), contractCorner (:1643-1760),
+// computeRDM_h/_v (:528-874) and the nearest-neighbour energy of avg (LinearAlgebra.cpp:198-260,
+// CTMSolver.cpp:60-62).  Every tensor operation runs through the block-sparse device layer.
+#pragma once
+#include <functional>
+#include <map>
+#include <vector>
+
+#include "tensor.hpp"
+
+namespace xa {
+
+enum class SvdMode : int {
+    Full = 0,        // full FP64 block SVD of every sector, then the reference's truncation rule
+    Randomized = 1,  // randomized block SVD with per-sector target ranks (see RankPolicy)
+};
+
+struct CTMOptions {
+    std::size_t chi = 16;
+    int precision = XH_FP64;  // GEMM precision of the contractions
+    SvdMode svd = SvdMode::Full;
+    // randomized SVD: oversampling, power iterations and the rank source
+    int oversample = 8;
+    int power_iters = 2;
+    // target rank per sector for randomized SVD; when empty the kept counts of the previous full
+    // SVD of the same (site, direction) are used
+    std::function<std::map<Q, int>(const DTensor& prod, int site, int dir)> rank_policy;
+    bool record_spectra = false;
+};
+
+struct SpectrumRecord {
+    int site = 0, dir = 0;
+    std::map<Q, std::vector<double>> sv;      // full spectra per sector (Full mode only)
+    std::map<Q, int> kept;                    // kept count per sector
+    std::map<Q, std::pair<int, int>> shape;   // block shape per sector
+};
+
+class CTM {
+public:
+    CTM(Runtime* rt, const std::vector<DTensor>& A, std::size_t chi);
+    void set_options(const CTMOptions& o) { opt = o; }
+    CTMOptions opt;
+
+    void init_from_A();
+    void grow_all();
+    void compute_rdm();
+    double energy(double Jxy = 1.0, double Jz = 1.0) const;  // (sum E_h + sum E_v) / 4
+    double step();  // grow_all + computeRDM + energy
+
+    const DTensor& A(int x, int y) const { return As[idx(x, y)]; }
+    const DTensor& Adag(int x, int y) const { return Adags[idx(x, y)]; }
+    std::vector<DTensor> C1, C2, C3, C4, T1, T2, T3, T4, rho_h, rho_v;
+    std::vector<SpectrumRecord> spectra;
+    std::map<std::pair<int, int>, std::map<Q, int>> last_kept;  // (site, dir) -> kept counts of the last full SVD
+    double svd_ms = 0.0;  // host-timed SVD phase (diagnostics)
+
+    static constexpr int Lx = 2, Ly = 2;
+    static int idx(int x, int y)
+    {
+        x = ((x % Lx) + Lx) % Lx;
+        y = ((y % Ly) + Ly) % Ly;
+        return x * Ly + y;
+    }
+
+private:
+    Runtime* rt;
+    std::vector<DTensor> As, Adags;
+
+    DTensor ct(const DTensor& a, std::vector<int> a1, const DTensor& b, std::vector<int> a2, int rank) const;
+    DTensor identity(const std::vector<Qbasis>& dom, const std::vector<Qbasis>& cod) const;
+    Qbasis ket(int x, int y, int leg) const;
+    Qbasis bra(int x, int y, int leg) const;
+    DTensor contract_corner(int x, int y, int corner) const;
+    std::pair<DTensor, DTensor> decompose(const DTensor& T1, const DTensor& T2, int site, int dir);
+    std::pair<DTensor, DTensor> get_projectors(int x, int y, int dir);
+    void left_move();
+    void right_move();
+    void top_move();
+    void bottom_move();
+    DTensor normalized(const DTensor& t) const;
+};
+
+// Randomized truncated block SVD: per sector q a rank k_q + oversampling sketch, power
+// iterations with CholeskyQR2 re-orthonormalisation, and an exact SVD of the small projected
+// block; sectors with k_q = 0 are dropped.  Then the reference's truncation rule over the
+// union of the computed singular values.
+SVDResult tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, int oversample, int power_iters,
+                          int precision, std::uint32_t seed);
+
+}  // namespace xa

@@ -389,3 +389,87 @@ def truncation_rule(sym, allSV, maxKeep, eps_svd, preserve_multiplets, truncWeig
     nq = 2 if sym[1] else 1
     kept = {tuple(kq[2 * i: 2 * i + nq]): kn[i] for i in range(nsec.value)}
     return kept, tw.value, en.value
+
+
+class CTM:
+    """CTMRG environment of a 2x2-cell iPEPS (the reference's CTM<double, Symmetry, 2>)."""
+
+    def __init__(self, rt: Runtime, A: Sequence[Tensor], chi: int):
+        self.rt = rt
+        self._A = list(A)  # keep the site tensors alive
+        arr = (C.c_void_p * 4)(*[a.h.value if isinstance(a.h, C.c_void_p) else a.h for a in A])
+        h = C.c_void_p()
+        check(lib().xt_ctm_create(rt.h, arr, int(chi), C.byref(h)), rt.h, "CTM()")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.xt_ctm_destroy(h)
+            self.h = None
+
+    def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=2, record_spectra=False):
+        check(lib().xt_ctm_set_options(self.h, PRECISION[precision], 1 if svd == "randomized" else 0, oversample, power_iters, int(record_spectra)),
+              self.rt.h, "set_options")
+        return self
+
+    def init(self):
+        check(lib().xt_ctm_init(self.h), self.rt.h, "CTM::init")
+        return self
+
+    def grow_all(self):
+        check(lib().xt_ctm_grow_all(self.h), self.rt.h, "grow_all")
+
+    def computeRDM(self):
+        check(lib().xt_ctm_compute_rdm(self.h), self.rt.h, "computeRDM")
+
+    def energy(self, Jxy=1.0, Jz=1.0) -> float:
+        e = C.c_double()
+        check(lib().xt_ctm_energy(self.h, Jxy, Jz, C.byref(e)), self.rt.h, "energy")
+        return e.value
+
+    def step(self) -> float:
+        self.grow_all()
+        self.computeRDM()
+        return self.energy()
+
+    def tensor(self, which: str, site: int) -> Tensor:
+        names = ["C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4", "rho_h", "rho_v"]
+        h = C.c_void_p()
+        check(lib().xt_ctm_tensor(self.h, names.index(which), site, C.byref(h)), self.rt.h, "ctm tensor")
+        return Tensor(self.rt, handle=h)
+
+    def svd_ms(self) -> float:
+        v = C.c_double()
+        lib().xt_ctm_svd_ms(self.h, C.byref(v))
+        return v.value
+
+    def spectra(self):
+        n = C.c_int()
+        lib().xt_ctm_num_spectra(self.h, C.byref(n))
+        out = []
+        for r in range(n.value):
+            cap = 1 << 20
+            site, d, nsec = C.c_int(), C.c_int(), C.c_int()
+            qs = (C.c_int * 4096)()
+            rows, cols, kept, nsv = (C.c_int * 2048)(), (C.c_int * 2048)(), (C.c_int * 2048)(), (C.c_int * 2048)()
+            sv = (C.c_double * cap)()
+            check(lib().xt_ctm_spectrum(self.h, r, C.byref(site), C.byref(d), C.byref(nsec), qs, rows, cols, kept, nsv, sv, cap), self.rt.h, "spectrum")
+            secs, off = [], 0
+            for k in range(nsec.value):
+                secs.append({"q": qs[2 * k], "rows": rows[k], "cols": cols[k], "kept": kept[k], "sv": np.array(sv[off:off + nsv[k]])})
+                off += nsv[k]
+            out.append({"site": site.value, "dir": d.value, "sectors": secs})
+        return out
+
+    def clear_spectra(self):
+        lib().xt_ctm_clear_spectra(self.h)
+
+    def set_ranks(self, site, direction, ranks: Dict):
+        n = len(ranks)
+        qs = (C.c_int * (2 * max(n, 1)))()
+        rk = (C.c_int * max(n, 1))()
+        for i, (q, r) in enumerate(ranks.items()):
+            qs[2 * i] = q if isinstance(q, int) else q[0]
+            rk[i] = int(r)
+        check(lib().xt_ctm_set_ranks(self.h, site, direction, qs, rk, n), self.rt.h, "set_ranks")
