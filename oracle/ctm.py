@@ -145,7 +145,7 @@ class CTM:
     def decompose(self, Ta, Tb, site, direction):
         prod = ot.mul(Ta, Tb)
         U, S, V, tw_, ent, kept = ot.tsvd(prod, self.chi, 1.0e-10, False)
-        self.spectra.append((site, direction, {q: len(v) for q, v in kept.items()}))
+        self.spectra.append((site, direction, {q: len(v) for q, v in kept.items()}, sorted(x for v in kept.values() for x in v)))
         isqrtS = diag_map(S, lambda d: 1.0 / np.sqrt(d))
         P1 = ot.mul(ot.mul(isqrtS, ot.adjoint(U)), Ta)
         P2 = ot.mul(ot.mul(Tb, ot.adjoint(V)), isqrtS)

@@ -222,7 +222,8 @@ def truncation_rule(sym: Sym, allSV, maxKeep, eps_svd, preserve_multiplets, trun
     kept: Dict[tuple, list] = {}
     for q, s in allSV:
         kept.setdefault(q, []).append(s)
-        entropy += -sym.degeneracy(q) * s * s * math.log(s * s)
+        # std::log(0) is -inf in C++ (the reference asserts against all-zero spectra, Tensor.cpp:587)
+        entropy += -sym.degeneracy(q) * s * s * (math.log(s * s) if s * s > 0 else float("-inf"))
     kept = dict(sorted(kept.items()))
     return kept, truncWeight, entropy
 

@@ -23,9 +23,11 @@ def test_neel_product_state_energy(rt):
     assert abs(E - (-0.5)) < 1e-14
 
 
-def test_random_ipeps_step_matches_oracle(rt):
-    oA, pA = make_ipeps(rt, configs.c4_small_virtual(), configs.c4_physical(), 7)
-    chi = 9
+def test_random_ipeps_steps_match_oracle(rt):
+    # D = 2 and chi = 400: chi is not reached in two steps, so only the eps = 1e-10 rule truncates and
+    # the kept states are unambiguous; energies, structures and density matrices must agree
+    oA, pA = make_ipeps(rt, [((0,), 1), ((1,), 1)], configs.c4_physical(), 7)
+    chi = 400
     oc = octm.CTM(oA, chi)
     oc.init_from_A()
     c = xt.CTM(rt, pA, chi).init()
@@ -33,7 +35,7 @@ def test_random_ipeps_step_matches_oracle(rt):
         Eo = oc.step()
         E = c.step()
         assert abs(E - Eo) <= 1e-10 * abs(Eo), (it, E, Eo)
-        for name, otab in (("C1", oc.C1), ("T1", oc.T1), ("C3", oc.C3), ("T4", oc.T4)):
+        for name, otab in (("C1", oc.C1), ("T1", oc.T1), ("C3", oc.C3), ("T4", oc.T4), ("T2", oc.T2)):
             for site in range(4):
                 assert structure_of(c.tensor(name, site)) == oracle_structure(otab[site]), (it, name, site)
         for site in range(4):
@@ -41,3 +43,28 @@ def test_random_ipeps_step_matches_oracle(rt):
                 r = c.tensor(name, site)
                 assert structure_of(r) == oracle_structure(otab[site])
                 np.testing.assert_allclose(r.download(), otab[site].data, rtol=0, atol=1e-9)
+
+
+def test_truncated_projectors_keep_the_same_singular_values(rt):
+    # D = 3, chi = 9 truncates inside the exactly degenerate +q/-q pairs that the double-layer
+    # environment produces; which partner survives depends on last-bit rounding (as it does in the
+    # reference's LAPACK path), so the kept singular VALUES are compared as a multiset up to and
+    # including the first call where the per-sector choice differs
+    oA, pA = make_ipeps(rt, configs.c4_small_virtual(), configs.c4_physical(), 7)
+    chi = 9
+    oc = octm.CTM(oA, chi)
+    oc.init_from_A()
+    c = xt.CTM(rt, pA, chi).set_options(record_spectra=True).init()
+    oc.grow_all()
+    c.grow_all()
+    sp = c.spectra()
+    assert len(sp) == len(oc.spectra) == 16
+    for g, o in zip(sp, oc.spectra):
+        assert (g["site"], g["dir"]) == o[:2]
+        kept_g = {s["q"]: s["kept"] for s in g["sectors"] if s["kept"] > 0}
+        vals_g = sorted(x for s in g["sectors"] for x in s["sv"][: s["kept"]])
+        assert sum(kept_g.values()) == len(o[3])
+        smax = max(vals_g)
+        np.testing.assert_allclose(vals_g, o[3], rtol=0, atol=1e-12 * smax)
+        if kept_g != {q[0]: n for q, n in o[2].items()}:
+            break

@@ -68,7 +68,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
                 if out.strip():
                     print(out)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for _, o in steps], "-o", str(tmp)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for _, o in steps], "-o", str(tmp),
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -181,7 +181,16 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
             if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
             ranks = it->second;
         }
-        r = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.oversample, opt.power_iters, opt.precision, 1234u + 17u * site + 5u * dir);
+        RandomizedSVD rs = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.oversample, opt.power_iters, opt.sketch_precision, 1234u + 17u * site + 5u * dir);
+        ++n_rsvd;
+        if(rs.safe) {
+            r = std::move(rs.res);
+        } else {
+            // the full FP64 tSVD is the fallback and the judge
+            ++n_fallback;
+            FullSVD f = full_svd(prod);
+            r = cutoff(prod, f, opt.chi, 1.e-10, false, 0.0, 0.0);
+        }
         std::map<Q, int> kept;
         for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
         last_kept[{site, dir}] = kept;
@@ -426,9 +435,3 @@ double CTM::step()
 
 }  // namespace xa
 
-namespace xa {
-SVDResult tsvd_randomized(const DTensor&, const std::map<Q, int>&, std::size_t, double, int, int, int, std::uint32_t)
-{
-    throw Error(XH_EINVAL, "randomized SVD is not available yet");
-}
-}  // namespace xa

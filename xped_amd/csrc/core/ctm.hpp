@@ -27,6 +27,7 @@ struct CTMOptions {
     // randomized SVD: oversampling, power iterations and the rank source
     int oversample = 8;
     int power_iters = 2;
+    int sketch_precision = XH_FP32;  // GEMM precision of the randomized range finder
     // target rank per sector for randomized SVD; when empty the kept counts of the previous full
     // SVD of the same (site, direction) are used
     std::function<std::map<Q, int>(const DTensor& prod, int site, int dir)> rank_policy;
@@ -58,6 +59,7 @@ public:
     std::vector<SpectrumRecord> spectra;
     std::map<std::pair<int, int>, std::map<Q, int>> last_kept;  // (site, dir) -> kept counts of the last full SVD
     double svd_ms = 0.0;  // host-timed SVD phase (diagnostics)
+    long n_rsvd = 0, n_fallback = 0;
 
     static constexpr int Lx = 2, Ly = 2;
     static int idx(int x, int y)
@@ -85,11 +87,19 @@ private:
     DTensor normalized(const DTensor& t) const;
 };
 
-// Randomized truncated block SVD: per sector q a rank k_q + oversampling sketch, power
-// iterations with CholeskyQR2 re-orthonormalisation, and an exact SVD of the small projected
-// block; sectors with k_q = 0 are dropped.  Then the reference's truncation rule over the
-// union of the computed singular values.
-SVDResult tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, int oversample, int power_iters,
-                          int precision, std::uint32_t seed);
+// Randomized truncated block SVD (Halko, Martinsson & Tropp, SIAM Review 53 (2011), Alg. 4.4 with
+// re-orthonormalised power iterations): per sector q a Gaussian sketch of width
+// l_q = min(k_q + oversample, min(m, n)), range finding in `sketch_precision`, Householder QR and
+// the projection B = Q^T A plus the small SVD of B in FP64; sectors with k_q = 0 are not
+// sketched.  Then the reference's truncation rule over the computed singular values.
+// `safe` is false when the uncaptured Frobenius weight of some sector could hide a singular value
+// above the kept cut (the caller then falls back to the full SVD).
+struct RandomizedSVD {
+    SVDResult res;
+    bool safe = true;
+    double max_residual_ratio = 0.0;  // max_q residual_q / sigma_cut^2
+};
+RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, int oversample, int power_iters,
+                              int sketch_precision, std::uint32_t seed);
 
 }  // namespace xa

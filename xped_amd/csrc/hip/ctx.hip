@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <rocblas/rocblas.h>
+
 #include "device.hpp"
 
 using xh::check;
@@ -55,6 +57,7 @@ extern "C" int xh_destroy(xh_ctx* ctx)
         if(ev) hipEventDestroy(ev);
     if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
     if(ctx->dev_scalar) hipFree(ctx->dev_scalar);
+    if(ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
     if(ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return XH_OK;

@@ -18,6 +18,7 @@ struct xh_ctx {
     // small pinned host staging buffer for scalar reductions
     double* host_scalar = nullptr;
     double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
+    void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
 };
 
 namespace xh {
