@@ -44,9 +44,16 @@ int xh_alloc(xh_ctx* ctx, size_t bytes, void** dptr);
 int xh_free(xh_ctx* ctx, void* dptr);
 int xh_upload(xh_ctx* ctx, void* dst, const void* src, size_t bytes);   /* host -> device, synchronous */
 int xh_download(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device -> host, synchronous */
+int xh_copy(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device to device, async on the stream */
 int xh_synchronize(xh_ctx* ctx);
 void* xh_stream(xh_ctx* ctx); /* the context's hipStream_t */
 int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value);
+/* Standard normal entries from a counter-based generator (splitmix64 + Box-Muller), used for the
+ * test matrices of the randomized block SVD; no counterpart in the reference. */
+int xh_fill_gaussian(xh_ctx* ctx, double* dst, size_t n, uint64_t seed);
+/* out[g] (host) = sum of squares of x[off[g] .. off[g] + len[g]): the per-sector squared
+ * Frobenius norms that squaredNorm (TensorBase.cpp:58-64) sums with degeneracy weights. */
+int xh_range_sumsq(xh_ctx* ctx, const int64_t* off, const int64_t* len, int64_t nranges, const double* x, double* out);
 
 /* ---- grouped permute ------------------------------------------------------------------
  * Replaces the subblock loop of Tensor::permute_impl (src/Core/Tensor.cpp:83-146, 148-214):
@@ -107,6 +114,11 @@ int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const
  * Eigen::HouseholderQR (diagonal may be negative). */
 int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* Q, double* R);
 int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* W, double* V);
+/* Inverse X = R^-1 of the upper Cholesky factor of G + shift_rel[g] * trace(G) * I for square
+ * Gram blocks (a_off -> G, u_off -> X, m == n); the step of (shifted) CholeskyQR used by the
+ * randomized block SVD.  Numerically dependent columns get a zero row and column in X.  No
+ * counterpart in the reference (its SVD is always the full one, Tensor.cpp:494-521). */
+int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* G, double* X, const double* shift_rel);
 
 /* ---- elementwise / reductions on a whole tensor buffer or its blocks ---------------------- */
 int xh_scale(xh_ctx* ctx, double* x, size_t n, double alpha);                          /* TensorBase.hpp:197-209 */

@@ -101,6 +101,16 @@ int xt_get_permutations(const int* a1, int n1, int rank1, const int* a2, int n2,
 /* Tensor::tSVD (Tensor.cpp:478-633).  *trunc_weight and *entropy are accumulated into (+=). */
 int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_multiplets, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
             double* trunc_weight, double* entropy);
+/* Randomized truncated block SVD with the tSVD interface and truncation rule (Tensor.cpp:478-633,
+ * PRESERVE_MULTIPLETS = false): sector q is sketched with min(ceil(rank_factor * k_q) + oversample,
+ * m_q, n_q) columns for the target ranks k_q given as nsec pairs (qs[2 i], qs[2 i + 1]) -> ranks[i]
+ * (0 for absent sectors), power_iters subspace iterations in sketch_precision.  *safe is 0 when
+ * the weight missed by a sketch could hide a singular value at or above the smallest kept one;
+ * *residual_ratio is the largest missed weight over the smallest kept sigma squared.  The full
+ * tSVD is the reference's only SVD; this entry point has no counterpart there. */
+int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, const int* qs, const int* ranks, int nsec, int oversample,
+                       double rank_factor, int power_iters, int sketch_precision, uint64_t seed, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
+                       double* trunc_weight, int* safe, double* residual_ratio);
 int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R);   /* Tensor.cpp:757-780 */
 int xt_teigh(const xt_tensor* t, xt_tensor** D, xt_tensor** V); /* Tensor.cpp:782-803 */
 int xt_adjoint(const xt_tensor* t, xt_tensor** out);            /* AdjointOp.hpp:25-69 */
@@ -131,6 +141,10 @@ int xt_ctm_energy(xt_ctm* c, double Jxy, double Jz, double* E); /* avg + CTMSolv
 /* which: 0..7 = C1..C4, T1..T4; 8 = rho_h, 9 = rho_v; site = x*2 + y */
 int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out);
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms);
+/* randomized SVD parameters of the CTM decompositions (see xt_tsvd_randomized) */
+int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_iters, int sketch_precision);
+/* counters: randomized SVDs run, full-SVD fallbacks taken, largest residual ratio seen */
+int xt_ctm_rsvd_stats(const xt_ctm* c, long* n_rsvd, long* n_fallback, double* max_residual_ratio);
 /* recorded full spectra: number of records, then per record the sector count and flat arrays */
 int xt_ctm_num_spectra(const xt_ctm* c, int* n);
 int xt_ctm_spectrum(const xt_ctm* c, int rec, int* site, int* dir, int* nsec, int* qs, int* rows, int* cols, int* kept, int* nsv, double* sv, int cap);

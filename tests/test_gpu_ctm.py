@@ -68,3 +68,20 @@ def test_truncated_projectors_keep_the_same_singular_values(rt):
         np.testing.assert_allclose(vals_g, o[3], rtol=0, atol=1e-12 * smax)
         if kept_g != {q[0]: n for q, n in o[2].items()}:
             break
+
+
+def test_randomized_svd_step_matches_full_step(rt):
+    # the second step with randomized projector SVDs (target ranks = kept counts of the first,
+    # full step; unsafe sketches fall back to the full SVD) against the same step with full SVDs
+    _, A = make_ipeps(rt, configs.c4_small_virtual(), configs.c4_physical(), 7)
+    chi = 24
+    full = xt.CTM(rt, A, chi).init()
+    rand = xt.CTM(rt, A, chi).init()
+    full.grow_all()
+    rand.grow_all()
+    rand.set_options(svd="randomized", oversample=8, power_iters=2)
+    Ef = full.step()
+    Er = rand.step()
+    n, nfb, ratio = rand.rsvd_stats()
+    assert n == 16
+    assert abs(Er - Ef) <= 1e-10 * abs(Ef), (Er, Ef, nfb, ratio)

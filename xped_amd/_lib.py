@@ -68,9 +68,12 @@ _SIGS = [
     ("xh_free", C.c_int, [_P, _P]),
     ("xh_upload", C.c_int, [_P, _P, _P, _SZ]),
     ("xh_download", C.c_int, [_P, _P, _P, _SZ]),
+    ("xh_copy", C.c_int, [_P, _P, _P, _SZ]),
     ("xh_synchronize", C.c_int, [_P]),
     ("xh_stream", _P, [_P]),
     ("xh_fill", C.c_int, [_P, _P, _SZ, C.c_double]),
+    ("xh_fill_gaussian", C.c_int, [_P, _P, _SZ, C.c_uint64]),
+    ("xh_range_sumsq", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64, _P, _DP]),
     ("xh_permute_plan_create", C.c_int, [_P, C.POINTER(xh_permute_desc), _PP]),
     ("xh_permute_plan_destroy", C.c_int, [_P, _P]),
     ("xh_permute", C.c_int, [_P, _P, _P, _P]),
@@ -82,6 +85,7 @@ _SIGS = [
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_eigh", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
+    ("xh_block_chol_inv", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _DP]),
     ("xh_scale", C.c_int, [_P, _P, _SZ, C.c_double]),
     ("xh_axpby", C.c_int, [_P, _P, _P, _SZ, C.c_double, C.c_double]),
     ("xh_max_abs", C.c_int, [_P, _P, _SZ, _DP]),
@@ -131,6 +135,8 @@ _SIGS = [
     ("xt_contract", C.c_int, [_P, _IP, C.c_int, _P, _IP, C.c_int, C.c_int, C.c_int, _PP]),
     ("xt_get_permutations", C.c_int, [_IP, C.c_int, C.c_int, _IP, C.c_int, C.c_int, C.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
     ("xt_tsvd", C.c_int, [_P, _SZ, C.c_double, C.c_int, _PP, _PP, _PP, _DP, _DP]),
+    ("xt_tsvd_randomized", C.c_int, [_P, _SZ, C.c_double, _IP, _IP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint64, _PP, _PP, _PP,
+                                     _DP, _IP, _DP]),
     ("xt_tqr", C.c_int, [_P, _PP, _PP]),
     ("xt_teigh", C.c_int, [_P, _PP, _PP]),
     ("xt_adjoint", C.c_int, [_P, _PP]),
@@ -150,6 +156,8 @@ _SIGS = [
     ("xt_ctm_energy", C.c_int, [_P, C.c_double, C.c_double, _DP]),
     ("xt_ctm_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
     ("xt_ctm_svd_ms", C.c_int, [_P, _DP]),
+    ("xt_ctm_set_rsvd", C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.c_int]),
+    ("xt_ctm_rsvd_stats", C.c_int, [_P, C.POINTER(C.c_long), C.POINTER(C.c_long), _DP]),
     ("xt_ctm_num_spectra", C.c_int, [_P, _IP]),
     ("xt_ctm_spectrum", C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, C.c_int]),
     ("xt_ctm_clear_spectra", C.c_int, [_P]),

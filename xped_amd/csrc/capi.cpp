@@ -455,6 +455,29 @@ int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_mu
     });
 }
 
+int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, const int* qs, const int* ranks, int nsec, int oversample,
+                       double rank_factor, int power_iters, int sketch_precision, uint64_t seed, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
+                       double* trunc_weight, int* safe, double* residual_ratio)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        std::map<xa::Q, int> m;
+        for(int i = 0; i < nsec; ++i) m[q_from(qs + 2 * i)] = ranks[i];
+        xa::RsvdOptions o;
+        o.oversample = oversample;
+        o.rank_factor = rank_factor;
+        o.power_iters = power_iters;
+        o.sketch_precision = sketch_precision;
+        xa::RandomizedSVD r = xa::tsvd_randomized(t->t, m, max_keep, eps_svd, o, seed);
+        if(r.res.U.st == nullptr) throw xa::Error(XH_EINVAL, "randomized SVD of an empty tensor");
+        *U = wrap(t->owner, r.res.U);
+        *S = wrap(t->owner, r.res.S);
+        *Vdag = wrap(t->owner, r.res.Vdag);
+        if(trunc_weight) *trunc_weight = r.res.trunc_weight;
+        if(safe) *safe = r.safe ? 1 : 0;
+        if(residual_ratio) *residual_ratio = r.max_residual_ratio;
+    });
+}
+
 int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R)
 {
     return run(t ? t->owner : nullptr, [&] {
@@ -568,8 +591,8 @@ int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, i
         xa::CTMOptions o = c->ctm->opt;
         o.precision = precision;
         o.svd = svd_mode == 1 ? xa::SvdMode::Randomized : xa::SvdMode::Full;
-        o.oversample = oversample;
-        o.power_iters = power_iters;
+        o.rsvd.oversample = oversample;
+        o.rsvd.power_iters = power_iters;
         o.record_spectra = record_spectra != 0;
         c->ctm->set_options(o);
     });
@@ -610,6 +633,27 @@ int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out)
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms)
 {
     *ms = c->ctm->svd_ms;
+    return XH_OK;
+}
+
+int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_iters, int sketch_precision)
+{
+    return run(c->owner, [&] {
+        if(oversample < 0 || power_iters < 0 || !(rank_factor >= 0.0)) throw xa::Error(XH_EINVAL, "invalid randomized SVD parameters");
+        xa::CTMOptions o = c->ctm->opt;
+        o.rsvd.oversample = oversample;
+        o.rsvd.rank_factor = rank_factor;
+        o.rsvd.power_iters = power_iters;
+        o.rsvd.sketch_precision = sketch_precision;
+        c->ctm->set_options(o);
+    });
+}
+
+int xt_ctm_rsvd_stats(const xt_ctm* c, long* n_rsvd, long* n_fallback, double* max_residual_ratio)
+{
+    *n_rsvd = c->ctm->n_rsvd;
+    *n_fallback = c->ctm->n_fallback;
+    *max_residual_ratio = c->ctm->max_residual_ratio;
     return XH_OK;
 }
 

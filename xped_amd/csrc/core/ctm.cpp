@@ -86,7 +86,14 @@ Qbasis CTM::bra(int x, int y, int leg) const
     }
 }
 
-DTensor CTM::normalized(const DTensor& t) const { return scaled(t, 1.0 / max_norm(t)); }
+DTensor CTM::normalized(const DTensor& t) const
+{
+    // the reference divides by maxNorm unchecked (CTM.cpp:1372, 1383, ...); a vanished or non-finite
+    // environment tensor is reported instead of being propagated as inf/NaN
+    const double m = max_norm(t);
+    if(!(m > 0.0) || !std::isfinite(m)) throw Error(XH_EINVAL, "CTM: environment tensor with max norm " + std::to_string(m) + " cannot be normalized");
+    return scaled(t, 1.0 / m);
+}
 
 void CTM::init_from_A()
 {
@@ -181,8 +188,9 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
             if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
             ranks = it->second;
         }
-        RandomizedSVD rs = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.oversample, opt.power_iters, opt.sketch_precision, 1234u + 17u * site + 5u * dir);
+        RandomizedSVD rs = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.rsvd, 0x5eedull + 64ull * static_cast<std::uint64_t>(n_rsvd) + 4ull * site + dir);
         ++n_rsvd;
+        max_residual_ratio = std::max(max_residual_ratio, rs.max_residual_ratio);
         if(rs.safe) {
             r = std::move(rs.res);
         } else {

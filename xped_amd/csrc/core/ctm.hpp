@@ -20,14 +20,20 @@ enum class SvdMode : int {
     Randomized = 1,  // randomized block SVD with per-sector target ranks (see RankPolicy)
 };
 
+// Randomized block SVD parameters: sector q is sketched with
+// l_q = min(ceil(rank_factor * k_q) + oversample, m_q, n_q) columns for a target rank k_q.
+struct RsvdOptions {
+    int oversample = 8;
+    double rank_factor = 2.0;
+    int power_iters = 1;
+    int sketch_precision = XH_FP64;  // GEMM precision of the range-finder products
+};
+
 struct CTMOptions {
     std::size_t chi = 16;
     int precision = XH_FP64;  // GEMM precision of the contractions
     SvdMode svd = SvdMode::Full;
-    // randomized SVD: oversampling, power iterations and the rank source
-    int oversample = 8;
-    int power_iters = 2;
-    int sketch_precision = XH_FP32;  // GEMM precision of the randomized range finder
+    RsvdOptions rsvd;  // randomized SVD: panel width, power iterations, sketch precision
     // target rank per sector for randomized SVD; when empty the kept counts of the previous full
     // SVD of the same (site, direction) are used
     std::function<std::map<Q, int>(const DTensor& prod, int site, int dir)> rank_policy;
@@ -60,6 +66,7 @@ public:
     std::map<std::pair<int, int>, std::map<Q, int>> last_kept;  // (site, dir) -> kept counts of the last full SVD
     double svd_ms = 0.0;  // host-timed SVD phase (diagnostics)
     long n_rsvd = 0, n_fallback = 0;
+    double max_residual_ratio = 0.0;
 
     static constexpr int Lx = 2, Ly = 2;
     static int idx(int x, int y)
@@ -87,19 +94,17 @@ private:
     DTensor normalized(const DTensor& t) const;
 };
 
-// Randomized truncated block SVD (Halko, Martinsson & Tropp, SIAM Review 53 (2011), Alg. 4.4 with
-// re-orthonormalised power iterations): per sector q a Gaussian sketch of width
-// l_q = min(k_q + oversample, min(m, n)), range finding in `sketch_precision`, Householder QR and
-// the projection B = Q^T A plus the small SVD of B in FP64; sectors with k_q = 0 are not
-// sketched.  Then the reference's truncation rule over the computed singular values.
-// `safe` is false when the uncaptured Frobenius weight of some sector could hide a singular value
-// above the kept cut (the caller then falls back to the full SVD).
+// Randomized truncated block SVD (Halko, Martinsson & Tropp, SIAM Review 53 (2011), Alg. 4.4,
+// subspace iteration with re-orthonormalisation): every sector is sketched with the panel width
+// of RsvdOptions for its target rank in `ranks` (0 when absent); then the reference's truncation
+// rule over the computed singular values.  `safe` is false when the Frobenius weight a sketch
+// missed could hide a singular value at or above the smallest kept one.
 struct RandomizedSVD {
     SVDResult res;
     bool safe = true;
-    double max_residual_ratio = 0.0;  // max_q residual_q / sigma_cut^2
+    double max_residual_ratio = 0.0;  // max_q missed weight / sigma_cut^2
 };
-RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, int oversample, int power_iters,
-                              int sketch_precision, std::uint32_t seed);
+RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
+                              std::uint64_t seed);
 
 }  // namespace xa

@@ -1,7 +1,25 @@
-// Randomized truncated block SVD on the GPU (see ctm.hpp).
+// Randomized truncated block SVD on the GPU.
+//
+// Per sector block A_q (m x n) with target rank k_q the range finder works on l_q columns,
+// l_q = min(ceil(rank_factor * k_q) + oversample, m, n):
+//   Y = A Omega (Omega Gaussian, n x l)            Q = orth(Y)
+//   repeat power_iters times: Z = A^T Q, Q_z = orth(Z), Y = A Q_z, Q = orth(Y)
+//   B^T = A^T Q (n x l)                            Q_b = orth(B^T),  R = Q_b^T B^T (l x l)
+//   R = U_r S V_r^T (Jacobi)                       A ~ (Q V_r) S (Q_b U_r)^T
+// orth is shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 42 (2020) A477): three passes
+// of Gram GEMM, Cholesky-inverse of the l x l Gram block and a GEMM that applies it.  Every step
+// is one grouped launch over all sectors.  The range-finder GEMMs (Y, Z) run in the sketch
+// precision; B^T, R, the orthonormalisation and the final products run in FP64.
+//
+// The reference's truncation rule (Tensor.cpp:523-633) then runs over the computed singular
+// values.  The residual E_q = A_q - Q Q^T A_q is formed explicitly (one GEMM; the difference of
+// squared norms would cancel catastrophically): by Weyl's inequality every singular value of A_q
+// beyond the l_q captured ones is at most ||E_q||_2 <= ||E_q||_F.  The result is flagged unsafe
+// when ||E_q||_F^2 reaches sigma_cut^2 for the smallest kept sigma_cut, i.e. when the exact tSVD
+// could keep a singular value the sketch did not see.
 #include <algorithm>
 #include <cmath>
-#include <random>
+#include <limits>
 
 #include "ctm.hpp"
 
@@ -17,9 +35,9 @@ void dchk(xh_ctx* ctx, int rc, const char* what)
 struct DevBuf {
     xh_ctx* ctx = nullptr;
     double* p = nullptr;
-    DevBuf(xh_ctx* c, std::size_t n) : ctx(c)
+    DevBuf(xh_ctx* c, int64_t n) : ctx(c)
     {
-        if(n) dchk(ctx, xh_alloc(ctx, n * sizeof(double), reinterpret_cast<void**>(&p)), "alloc");
+        if(n > 0) dchk(ctx, xh_alloc(ctx, static_cast<size_t>(n) * sizeof(double), reinterpret_cast<void**>(&p)), "alloc");
     }
     ~DevBuf()
     {
@@ -29,153 +47,151 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
-void grouped_gemm(xh_ctx* ctx, const std::vector<xh_gemm_item>& items, int ta, int tb, const double* A, const double* B, double* C, int precision)
-{
-    if(items.empty()) return;
-    xh_gemm_plan* plan = nullptr;
-    dchk(ctx, xh_gemm_plan_create(ctx, items.data(), static_cast<int64_t>(items.size()), ta, tb, &plan), "gemm plan");
-    int rc = xh_gemm(ctx, plan, A, B, C, 1.0, 0.0, precision);
-    xh_gemm_plan_destroy(ctx, plan);
-    dchk(ctx, rc, "grouped gemm");
-}
+struct Plan {
+    xh_ctx* ctx = nullptr;
+    xh_gemm_plan* p = nullptr;
+    Plan(xh_ctx* c, const std::vector<xh_gemm_item>& items, int ta, int tb) : ctx(c)
+    {
+        if(!items.empty()) dchk(ctx, xh_gemm_plan_create(ctx, items.data(), static_cast<int64_t>(items.size()), ta, tb, &p), "gemm plan");
+    }
+    ~Plan()
+    {
+        if(p) xh_gemm_plan_destroy(ctx, p);
+    }
+    Plan(const Plan&) = delete;
+    Plan& operator=(const Plan&) = delete;
+    void run(const double* A, const double* B, double* C, int precision, double alpha = 1.0, double beta = 0.0) const
+    {
+        if(p) dchk(ctx, xh_gemm(ctx, p, A, B, C, alpha, beta, precision), "grouped gemm");
+    }
+};
+
+struct Sec {
+    int i;                   // sector index in the input tensor
+    int64_t a, m, n, l;      // block offset and extents
+    int64_t ym, zn, g, s;    // offsets: m x l panels, n x l panels, l x l blocks, l vectors
+};
 
 }  // namespace
 
-RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, int oversample, int power_iters,
-                              int sketch_precision, std::uint32_t seed)
+RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
+                              std::uint64_t seed)
 {
     t.rt->require_device();
     xh_ctx* ctx = t.rt->ctx;
     const TensorStructure& s = *t.st;
-    struct Sec {
-        int i;  // sector index in t
-        int64_t m, n, l;
-        int64_t om, y, z, b, ub, sb, u;  // offsets into the work buffers
-    };
     std::vector<Sec> secs;
-    int64_t nOm = 0, nY = 0, nZ = 0, nB = 0, nUb = 0, nSb = 0;
+    int64_t nY = 0, nZ = 0, nG = 0, nS = 0;
     for(std::size_t i = 0; i < s.sector.size(); ++i) {
-        auto it = ranks.find(s.sector[i]);
-        const int k = it == ranks.end() ? 0 : it->second;
-        if(k <= 0) continue;
         const int64_t m = static_cast<int64_t>(s.rows[i]), n = static_cast<int64_t>(s.cols[i]);
-        const int64_t l = std::min<int64_t>(k + oversample, std::min(m, n));
-        if(l <= 0) continue;
-        Sec c{static_cast<int>(i), m, n, l, nOm, nY, nZ, nB, nUb, nSb, 0};
-        nOm += n * l;
+        if(m == 0 || n == 0) continue;
+        auto it = ranks.find(s.sector[i]);
+        const int k = it == ranks.end() ? 0 : std::max(0, it->second);
+        const int64_t want = static_cast<int64_t>(std::ceil(o.rank_factor * k)) + o.oversample;
+        const int64_t l = std::min<int64_t>(std::max<int64_t>(want, 1), std::min(m, n));
+        secs.push_back(Sec{static_cast<int>(i), static_cast<int64_t>(s.offsets[i]), m, n, l, nY, nZ, nG, nS});
         nY += m * l;
         nZ += n * l;
-        nB += l * n;
-        nUb += l * l;
-        nSb += l;
-        secs.push_back(c);
+        nG += l * l;
+        nS += l;
     }
-    // Gaussian test matrices from one engine, sector by sector
-    std::vector<double> hom(nOm);
-    {
-        std::mt19937 eng(seed);
-        std::normal_distribution<double> nd(0.0, 1.0);
-        for(auto& x : hom) x = nd(eng);
-    }
-    DevBuf Om(ctx, nOm), Y(ctx, nY), Qy(ctx, nY), Z(ctx, nZ), Qz(ctx, nZ), Rw(ctx, std::max(nUb, int64_t(1))), Bm(ctx, nB), Ub(ctx, nUb), Sb(ctx, nSb),
-        Vtb(ctx, nB), Uf(ctx, nY);
-    if(nOm) dchk(ctx, xh_upload(ctx, Om.p, hom.data(), nOm * sizeof(double)), "upload sketch");
-    const double* A = t.data();
-    auto items = [&](auto f) {
-        std::vector<xh_gemm_item> v;
-        for(const auto& c : secs) v.push_back(f(c));
-        return v;
-    };
-    auto qr = [&](double* src, double* dst, bool rows_m) {
-        std::vector<xh_factor_item> it;
-        for(const auto& c : secs) {
-            const int64_t rr = rows_m ? c.m : c.n;
-            const int64_t off = rows_m ? c.y : c.z;
-            it.push_back(xh_factor_item{off, off, c.l, c.ub, static_cast<int32_t>(rr), static_cast<int32_t>(c.l)});
-        }
-        if(!it.empty()) dchk(ctx, xh_block_qr(ctx, it.data(), static_cast<int64_t>(it.size()), src, dst, Rw.p), "block qr");
-    };
-    const int64_t a0 = 0;
-    (void)a0;
-    // Y = A Omega
-    grouped_gemm(ctx, items([&](const Sec& c) {
-                     return xh_gemm_item{static_cast<int64_t>(s.offsets[c.i]), c.om, c.y, static_cast<int32_t>(c.m), static_cast<int32_t>(c.l),
-                                         static_cast<int32_t>(c.n)};
-                 }),
-                 0, 0, A, Om.p, Y.p, sketch_precision);
-    for(int p = 0; p < power_iters; ++p) {
-        qr(Y.p, Qy.p, true);
-        // Z = A^T Q
-        grouped_gemm(ctx, items([&](const Sec& c) {
-                         return xh_gemm_item{static_cast<int64_t>(s.offsets[c.i]), c.y, c.z, static_cast<int32_t>(c.n), static_cast<int32_t>(c.l),
-                                             static_cast<int32_t>(c.m)};
-                     }),
-                     1, 0, A, Qy.p, Z.p, sketch_precision);
-        qr(Z.p, Qz.p, false);
-        // Y = A Qz
-        grouped_gemm(ctx, items([&](const Sec& c) {
-                         return xh_gemm_item{static_cast<int64_t>(s.offsets[c.i]), c.z, c.y, static_cast<int32_t>(c.m), static_cast<int32_t>(c.l),
-                                             static_cast<int32_t>(c.n)};
-                     }),
-                     0, 0, A, Qz.p, Y.p, sketch_precision);
-    }
-    qr(Y.p, Qy.p, true);
-    // B = Q^T A in FP64
-    grouped_gemm(ctx, items([&](const Sec& c) {
-                     return xh_gemm_item{c.y, static_cast<int64_t>(s.offsets[c.i]), c.b, static_cast<int32_t>(c.l), static_cast<int32_t>(c.n),
-                                         static_cast<int32_t>(c.m)};
-                 }),
-                 1, 0, Qy.p, A, Bm.p, XH_FP64);
-    // small SVDs of B (l x n)
-    {
-        std::vector<xh_factor_item> it;
-        for(const auto& c : secs) it.push_back(xh_factor_item{c.b, c.ub, c.sb, c.b, static_cast<int32_t>(c.l), static_cast<int32_t>(c.n)});
-        if(!it.empty()) dchk(ctx, xh_block_svd(ctx, it.data(), static_cast<int64_t>(it.size()), Bm.p, Ub.p, Sb.p, Vtb.p), "block svd of the projection");
-    }
-    // U = Q Ub in FP64
-    grouped_gemm(ctx, items([&](const Sec& c) {
-                     return xh_gemm_item{c.y, c.ub, c.y, static_cast<int32_t>(c.m), static_cast<int32_t>(c.l), static_cast<int32_t>(c.l)};
-                 }),
-                 0, 0, Qy.p, Ub.p, Uf.p, XH_FP64);
-    // singular values and captured weights
-    std::vector<double> sv(nSb);
-    if(nSb) dchk(ctx, xh_download(ctx, sv.data(), Sb.p, nSb * sizeof(double)), "download singular values");
-    std::vector<double> normA(s.sector.size(), 0.0), normB(secs.size(), 0.0);
-    for(std::size_t i = 0; i < s.sector.size(); ++i) {
-        const std::size_t n = s.rows[i] * s.cols[i];
-        if(n) dchk(ctx, xh_dot(ctx, A + s.offsets[i], A + s.offsets[i], n, &normA[i]), "norm");
-    }
-    for(std::size_t k = 0; k < secs.size(); ++k) {
-        double acc = 0.0;
-        for(int64_t j = 0; j < secs[k].l; ++j) acc += sv[secs[k].sb + j] * sv[secs[k].sb + j];
-        normB[k] = acc;
-    }
-    std::vector<std::pair<Q, double>> all_sv;
-    for(const auto& c : secs)
-        for(int64_t j = 0; j < c.l; ++j) all_sv.emplace_back(s.sector[c.i], sv[c.sb + j]);
     RandomizedSVD out;
-    if(all_sv.empty()) {
+    if(secs.empty()) {
         out.safe = false;
         return out;
     }
+    const double* A = t.data();
+    auto items = [&](auto f) {
+        std::vector<xh_gemm_item> v;
+        v.reserve(secs.size());
+        for(const auto& c : secs) v.push_back(f(c));
+        return v;
+    };
+    auto I32 = [](int64_t x) { return static_cast<int32_t>(x); };
+    // grouped GEMM plans (offsets are shared by all panels of one kind)
+    const Plan p_sketch(ctx, items([&](const Sec& c) { return xh_gemm_item{c.a, c.zn, c.ym, I32(c.m), I32(c.l), I32(c.n)}; }), 0, 0);  // Y = A Om
+    const Plan p_atq(ctx, items([&](const Sec& c) { return xh_gemm_item{c.a, c.ym, c.zn, I32(c.n), I32(c.l), I32(c.m)}; }), 1, 0);    // Z = A^T Q
+    const Plan p_aqz(ctx, items([&](const Sec& c) { return xh_gemm_item{c.a, c.zn, c.ym, I32(c.m), I32(c.l), I32(c.n)}; }), 0, 0);    // Y = A Qz
+    const Plan p_gram_m(ctx, items([&](const Sec& c) { return xh_gemm_item{c.ym, c.ym, c.g, I32(c.l), I32(c.l), I32(c.m)}; }), 1, 0);
+    const Plan p_apply_m(ctx, items([&](const Sec& c) { return xh_gemm_item{c.ym, c.g, c.ym, I32(c.m), I32(c.l), I32(c.l)}; }), 0, 0);
+    const Plan p_gram_n(ctx, items([&](const Sec& c) { return xh_gemm_item{c.zn, c.zn, c.g, I32(c.l), I32(c.l), I32(c.n)}; }), 1, 0);
+    const Plan p_apply_n(ctx, items([&](const Sec& c) { return xh_gemm_item{c.zn, c.g, c.zn, I32(c.n), I32(c.l), I32(c.l)}; }), 0, 0);
+    const Plan p_u(ctx, items([&](const Sec& c) { return xh_gemm_item{c.ym, c.g, c.ym, I32(c.m), I32(c.l), I32(c.l)}; }), 0, 1);      // U = Q V_r
+    std::vector<xh_factor_item> chol_m, chol_n, svd_r;
+    std::vector<double> shift_m, shift_n, zero(secs.size(), 0.0);
+    for(const auto& c : secs) {
+        chol_m.push_back(xh_factor_item{c.g, c.g, 0, 0, I32(c.l), I32(c.l)});
+        svd_r.push_back(xh_factor_item{c.g, c.g, c.s, c.g, I32(c.l), I32(c.l)});
+        const double eps = std::numeric_limits<double>::epsilon();
+        const double ll = static_cast<double>(c.l);
+        shift_m.push_back(11.0 * (static_cast<double>(c.m) * ll + ll * (ll + 1.0)) * eps);
+        shift_n.push_back(11.0 * (static_cast<double>(c.n) * ll + ll * (ll + 1.0)) * eps);
+    }
+    chol_n = chol_m;
+    DevBuf Y0(ctx, nY), Y1(ctx, nY), Y2(ctx, nY), Z0(ctx, nZ), Z1(ctx, nZ), Z2(ctx, nZ), G(ctx, nG), X(ctx, nG), Ur(ctx, nG), Vtr(ctx, nG),
+        Sr(ctx, nS);
+    // shifted CholeskyQR3 of the panels in src; the result lands in b1 (src is left untouched)
+    auto orth = [&](const double* src, double* b1, double* b2, const Plan& gram, const Plan& apply, const std::vector<xh_factor_item>& chol,
+                    const std::vector<double>& shift) {
+        const double* in = src;
+        double* outs[3] = {b1, b2, b1};
+        for(int pass = 0; pass < 3; ++pass) {
+            gram.run(in, in, G.p, XH_FP64);
+            dchk(ctx, xh_block_chol_inv(ctx, chol.data(), static_cast<int64_t>(chol.size()), G.p, X.p, pass == 0 ? shift.data() : zero.data()),
+                 "cholesky inverse");
+            apply.run(in, X.p, outs[pass], XH_FP64);
+            in = outs[pass];
+        }
+        return b1;
+    };
+    // range finder
+    dchk(ctx, xh_fill_gaussian(ctx, Z0.p, static_cast<size_t>(nZ), seed), "gaussian test matrix");
+    p_sketch.run(A, Z0.p, Y0.p, o.sketch_precision);
+    double* Qm = orth(Y0.p, Y1.p, Y2.p, p_gram_m, p_apply_m, chol_m, shift_m);
+    for(int p = 0; p < o.power_iters; ++p) {
+        p_atq.run(A, Qm, Z0.p, o.sketch_precision);
+        double* Qn = orth(Z0.p, Z1.p, Z2.p, p_gram_n, p_apply_n, chol_n, shift_n);
+        p_aqz.run(A, Qn, Y0.p, o.sketch_precision);
+        Qm = orth(Y0.p, Y1.p, Y2.p, p_gram_m, p_apply_m, chol_m, shift_m);
+    }
+    // B^T = A^T Q in FP64 and the explicit residual E = A - Q B (its Frobenius norm bounds every
+    // singular value the sketch missed)
+    p_atq.run(A, Qm, Z0.p, XH_FP64);
+    std::vector<double> normE(s.sector.size(), 0.0);
+    {
+        DevBuf E(ctx, static_cast<int64_t>(t.size()));
+        dchk(ctx, xh_copy(ctx, E.p, A, t.size() * sizeof(double)), "copy");
+        const Plan p_res(ctx, items([&](const Sec& c) { return xh_gemm_item{c.ym, c.zn, c.a, I32(c.m), I32(c.n), I32(c.l)}; }), 0, 1);
+        p_res.run(Qm, Z0.p, E.p, XH_FP64, -1.0, 1.0);
+        std::vector<int64_t> off, len;
+        for(std::size_t i = 0; i < s.sector.size(); ++i) {
+            off.push_back(static_cast<int64_t>(s.offsets[i]));
+            len.push_back(static_cast<int64_t>(s.rows[i] * s.cols[i]));
+        }
+        dchk(ctx, xh_range_sumsq(ctx, off.data(), len.data(), static_cast<int64_t>(off.size()), E.p, normE.data()), "residual norms");
+    }
+    // QR of B^T and the small SVD
+    double* Qb = orth(Z0.p, Z1.p, Z2.p, p_gram_n, p_apply_n, chol_n, shift_n);
+    p_gram_n.run(Qb, Z0.p, G.p, XH_FP64);  // R = Q_b^T B^T
+    dchk(ctx, xh_block_svd(ctx, svd_r.data(), static_cast<int64_t>(svd_r.size()), G.p, Ur.p, Sr.p, Vtr.p), "block svd of R");
+    p_u.run(Qm, Vtr.p, Y0.p, XH_FP64);     // U = Q V_r   (m x l)
+    p_apply_n.run(Qb, Ur.p, Z0.p, XH_FP64);  // V = Q_b U_r (n x l)
+    // singular values and the captured weights
+    std::vector<double> sv(nS);
+    dchk(ctx, xh_download(ctx, sv.data(), Sr.p, static_cast<size_t>(nS) * sizeof(double)), "download singular values");
+    std::vector<std::pair<Q, double>> all_sv;
+    for(const auto& c : secs)
+        for(int64_t j = 0; j < c.l; ++j) all_sv.emplace_back(s.sector[c.i], sv[c.s + j]);
     TruncRule rule = truncation_rule(s.sym, all_sv, max_keep, eps_svd, false, 0.0, 0.0);
     double sigma_cut = std::numeric_limits<double>::infinity();
     for(const auto& [q, v] : rule.kept)
         for(double x : v) sigma_cut = std::min(sigma_cut, x);
-    // an uncaptured singular value of sector q is at most sqrt(residual_q)
-    std::vector<char> sketched(s.sector.size(), 0);
-    for(std::size_t k = 0; k < secs.size(); ++k) {
-        sketched[secs[k].i] = 1;
-        const double res = std::max(0.0, normA[secs[k].i] - normB[k]);
-        out.max_residual_ratio = std::max(out.max_residual_ratio, res / (sigma_cut * sigma_cut));
-        if(res >= sigma_cut * sigma_cut) out.safe = false;
+    const double cut2 = sigma_cut * sigma_cut;
+    for(std::size_t i = 0; i < s.sector.size(); ++i) {
+        out.max_residual_ratio = std::max(out.max_residual_ratio, normE[i] / cut2);
+        if(!(normE[i] < cut2)) out.safe = false;
     }
-    for(std::size_t i = 0; i < s.sector.size(); ++i)
-        if(!sketched[i]) {
-            out.max_residual_ratio = std::max(out.max_residual_ratio, normA[i] / (sigma_cut * sigma_cut));
-            if(normA[i] >= sigma_cut * sigma_cut) out.safe = false;
-        }
-    // assemble U, Sigma, Vdag over the kept sectors (ascending q, as cutoff_matrices does)
+    // U, Sigma, Vdag over the kept sectors (ascending q, as cutoff_matrices builds them)
     Qbasis truncBasis(s.sym, 1);
     std::vector<Q> order;
     for(const auto& [q, v] : rule.kept) {
@@ -192,19 +208,19 @@ RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, s
     res.Vdag = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {truncBasis}, s.unc_cod, order));
     std::vector<xh_copy_rect> ru, rv;
     std::vector<double> hs(res.S.size(), 0.0);
-    for(std::size_t o = 0; o < order.size(); ++o) {
-        const Q& q = order[o];
+    for(std::size_t o2 = 0; o2 < order.size(); ++o2) {
+        const Q& q = order[o2];
         const Sec* c = nullptr;
         for(const auto& x : secs)
             if(s.sector[x.i] == q) c = &x;
-        const std::size_t nret = rule.kept[q].size();
-        ru.push_back(xh_copy_rect{c->y, c->m, static_cast<int64_t>(res.U.st->offsets[o]), c->m, static_cast<int32_t>(c->m), static_cast<int32_t>(nret), 1.0, 0});
-        rv.push_back(xh_copy_rect{c->b, c->l, static_cast<int64_t>(res.Vdag.st->offsets[o]), static_cast<int64_t>(nret), static_cast<int32_t>(nret),
-                                  static_cast<int32_t>(c->n), 1.0, 0});
-        for(std::size_t k = 0; k < nret; ++k) hs[res.S.st->offsets[o] + k + k * nret] = rule.kept[q][k];
+        if(!c) throw Error(XH_EINVAL, "randomized SVD: kept sector without a sketch");
+        const int64_t nret = static_cast<int64_t>(rule.kept[q].size());
+        ru.push_back(xh_copy_rect{c->ym, c->m, static_cast<int64_t>(res.U.st->offsets[o2]), c->m, I32(c->m), I32(nret), 1.0, 0});
+        rv.push_back(xh_copy_rect{c->zn, c->n, static_cast<int64_t>(res.Vdag.st->offsets[o2]), nret, I32(nret), I32(c->n), 1.0, 1});
+        for(int64_t k = 0; k < nret; ++k) hs[res.S.st->offsets[o2] + k + k * nret] = rule.kept[q][k];
     }
-    dchk(ctx, xh_copy_rects(ctx, Uf.p, res.U.data(), ru.data(), static_cast<int64_t>(ru.size())), "truncate U");
-    dchk(ctx, xh_copy_rects(ctx, Vtb.p, res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
+    dchk(ctx, xh_copy_rects(ctx, Y0.p, res.U.data(), ru.data(), static_cast<int64_t>(ru.size())), "truncate U");
+    dchk(ctx, xh_copy_rects(ctx, Z0.p, res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
     if(res.S.size()) res.S.upload(hs.data(), hs.size());
     return out;
 }

@@ -420,6 +420,9 @@ TruncRule truncation_rule(const Sym& s, std::vector<std::pair<Q, double>> allSV,
     TruncRule out;
     const std::size_t numberOfStates = allSV.size();
     if(numberOfStates == 0) throw Error(XH_EINVAL, "tSVD of a tensor without sectors");
+    // std::sort needs a strict weak order: NaN (from non-finite input data) would break it
+    for(const auto& e : allSV)
+        if(!std::isfinite(e.second)) throw Error(XH_EINVAL, "tSVD: non-finite singular value (non-finite tensor data)");
     const auto first_entry = allSV[0];
     std::sort(allSV.begin(), allSV.end(), [](const std::pair<Q, double>& a, const std::pair<Q, double>& b) { return a.second > b.second; });
     for(std::size_t i = maxKeep; i < allSV.size(); ++i) truncWeight += s.degeneracy(allSV[i].first) * std::pow(std::abs(allSV[i].second), 2.);

@@ -99,6 +99,14 @@ extern "C" int xh_download(xh_ctx* ctx, void* dst, const void* src, size_t bytes
     });
 }
 
+extern "C" int xh_copy(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    return guard(ctx, [&] {
+        if(bytes == 0) return;
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync D2D");
+    });
+}
+
 extern "C" int xh_synchronize(xh_ctx* ctx)
 {
     return guard(ctx, [&] { check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
@@ -266,6 +274,47 @@ __global__ void k_copy_rects(const double* __restrict__ src, double* __restrict_
     }
 }
 
+// Counter-based Gaussian fill: element i takes two 53-bit uniforms from splitmix64 (Steele, Lea &
+// Flood, OOPSLA 2014) of (seed, i) and one Box-Muller normal (Box & Muller, Ann. Math. Stat. 29
+// (1958) 610).  Deterministic for a given seed and independent of the launch shape.
+__device__ inline uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_fill_gaussian(double* __restrict__ x, size_t n, uint64_t seed)
+{
+    const uint64_t key = splitmix64(seed);
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const uint64_t a = splitmix64(key ^ (2 * uint64_t(i))), b = splitmix64(key ^ (2 * uint64_t(i) + 1));
+        const double u1 = (double((a >> 11) + 1)) * 0x1.0p-53;  // (0, 1]
+        const double u2 = double(b >> 11) * 0x1.0p-53;          // [0, 1)
+        x[i] = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    }
+}
+
+// Per range g: out[g] = sum of squares of x[off[g] .. off[g] + len[g]).
+__global__ void k_range_sumsq(const int64_t* __restrict__ off, const int64_t* __restrict__ len, const double* __restrict__ x, double* __restrict__ out)
+{
+    const int g = blockIdx.x;
+    const double* p = x + off[g];
+    const int64_t n = len[g];
+    double acc = 0.0;
+    for(int64_t i = threadIdx.x; i < n; i += blockDim.x) acc = fma(p[i], p[i], acc);
+    acc = wave_sum(acc);
+    __shared__ double s[EW_THREADS / 64];
+    if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double r = 0;
+        for(int w = 0; w < EW_THREADS / 64; ++w) r += s[w];
+        out[g] = r;
+    }
+}
+
 }  // namespace
 
 extern "C" int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value)
@@ -384,5 +433,32 @@ extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const 
         xh::launch_check("k_copy_rects");
         check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         xh::dev_free(ctx, d);
+    });
+}
+
+extern "C" int xh_fill_gaussian(xh_ctx* ctx, double* dst, size_t n, uint64_t seed)
+{
+    return guard(ctx, [&] {
+        if(n == 0) return;
+        hipLaunchKernelGGL(k_fill_gaussian, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, dst, n, seed);
+        xh::launch_check("k_fill_gaussian");
+    });
+}
+
+extern "C" int xh_range_sumsq(xh_ctx* ctx, const int64_t* off, const int64_t* len, int64_t nranges, const double* x, double* out)
+{
+    return guard(ctx, [&] {
+        if(nranges <= 0) return;
+        std::vector<int64_t> v(off, off + nranges);
+        v.insert(v.end(), len, len + nranges);
+        int64_t* d = xh::dev_upload(ctx, v);
+        double* d_out = nullptr;
+        check(hipMallocAsync(reinterpret_cast<void**>(&d_out), size_t(nranges) * sizeof(double), ctx->stream), "hipMallocAsync");
+        hipLaunchKernelGGL(k_range_sumsq, dim3(static_cast<unsigned>(nranges)), dim3(EW_THREADS), 0, ctx->stream, d, d + nranges, x, d_out);
+        xh::launch_check("k_range_sumsq");
+        check(hipMemcpyAsync(out, d_out, size_t(nranges) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, d);
+        xh::dev_free(ctx, d_out);
     });
 }

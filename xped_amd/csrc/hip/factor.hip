@@ -38,6 +38,10 @@ __device__ __forceinline__ int rr_player(int pos, int step, int n2)
     return ((pos - 1 + step) % (n2 - 1)) + 1;
 }
 
+// total orders over doubles for rank-based sorting: NaN below (above) every number
+__device__ __forceinline__ double nan_low(double x) { return x == x ? x : -DBL_MAX * 2.0; }
+__device__ __forceinline__ double nan_high(double x) { return x == x ? x : DBL_MAX * 2.0; }
+
 struct FactorArgs {
     const xh_factor_item* items;
     const double* A;
@@ -153,12 +157,13 @@ __global__ __launch_bounds__(F_THREADS) void k_svd_jacobi(FactorArgs fa)
         sig[j] = sqrt(s);
     }
     __syncthreads();
-    // descending order, ties broken by column index
+    // descending order, ties broken by column index; NaN (from non-finite input) sorts last so
+    // that the ranks stay a permutation of 0..nn-1 for any input
     for(int j = tid; j < nn; j += F_THREADS) {
-        const double sj = sig[j];
+        const double sj = nan_low(sig[j]);
         int rank = 0;
         for(int k = 0; k < nn; ++k) {
-            const double sk = sig[k];
+            const double sk = nan_low(sig[k]);
             rank += (sk > sj) || (sk == sj && k < j);
         }
         ord[rank] = j;
@@ -283,6 +288,77 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
     }
 }
 
+// Inverse of the upper Cholesky factor of a Gram block G + shift * trace(G) * I (the building
+// block of shifted CholeskyQR, Fukaya, Kannan, Nakatsukasa, Yamamoto & Yanagisawa, SIAM J. Sci.
+// Comput. 42 (2020) A477).  Item: a_off -> G (n x n), u_off -> X = R^-1 (n x n, upper), s_off ->
+// index of the item's relative shift in fa.S.  A pivot that does not exceed 64 eps times its
+// column's original diagonal marks a numerically dependent column: its row and column of X are
+// zero, so Y X has a zero column there.
+__global__ __launch_bounds__(F_THREADS) void k_chol_inv(FactorArgs fa)
+{
+    extern __shared__ double smem[];
+    const xh_factor_item it = fa.items[blockIdx.x];
+    const int n = it.n;
+    if(n == 0) return;
+    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;  // n x n
+    double* rd = W + int64_t(n) * n;                                                           // n
+    const int tid = threadIdx.x;
+    const int64_t nn = int64_t(n) * n;
+    for(int64_t e = tid; e < nn; e += F_THREADS) W[e] = fa.A[it.a_off + e];
+    __syncthreads();
+    if(tid == 0) {
+        double tr = 0.0;
+        for(int k = 0; k < n; ++k) tr += W[k + int64_t(k) * n];
+        const double shift = fa.S[it.s_off] * tr;
+        for(int k = 0; k < n; ++k) {
+            W[k + int64_t(k) * n] += shift;
+            rd[k] = W[k + int64_t(k) * n];
+        }
+    }
+    __syncthreads();
+    for(int k = 0; k < n; ++k) {
+        if(tid == 0) {
+            const double d = W[k + int64_t(k) * n];
+            rd[k] = (d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX) ? sqrt(d) : 0.0;
+        }
+        __syncthreads();
+        const double r = rd[k];
+        for(int j = k + 1 + tid; j < n; j += F_THREADS) W[k + int64_t(j) * n] = r > 0.0 ? W[k + int64_t(j) * n] / r : 0.0;
+        __syncthreads();
+        const int t = n - k - 1;
+        if(r > 0.0 && t > 0) {
+            for(int e = tid; e < t * t; e += F_THREADS) {
+                const int i = k + 1 + e % t, j = k + 1 + e / t;
+                if(i <= j) W[i + int64_t(j) * n] -= W[k + int64_t(i) * n] * W[k + int64_t(j) * n];
+            }
+        }
+        __syncthreads();
+    }
+    // back substitution, one column of X per thread; X[i][j] (i < j) is kept at W[j + i n]
+    for(int j = tid; j < n; j += F_THREADS) {
+        const double xjj = rd[j] > 0.0 ? 1.0 / rd[j] : 0.0;
+        for(int i = j - 1; i >= 0; --i) {
+            double x = 0.0;
+            if(rd[i] > 0.0 && xjj != 0.0) {
+                double acc = W[i + int64_t(j) * n] * xjj;
+                for(int k = i + 1; k < j; ++k) acc = fma(W[i + int64_t(k) * n], W[j + int64_t(k) * n], acc);
+                x = -acc / rd[i];
+            }
+            W[j + int64_t(i) * n] = x;
+        }
+    }
+    __syncthreads();
+    for(int64_t e = tid; e < nn; e += F_THREADS) {
+        const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+        double x = 0.0;
+        if(i < j)
+            x = W[j + int64_t(i) * n];
+        else if(i == j)
+            x = rd[j] > 0.0 ? 1.0 / rd[j] : 0.0;
+        fa.U[it.u_off + e] = x;
+    }
+}
+
 // Symmetric eigen-decomposition by parallel cyclic Jacobi.  Item: a_off (n x n), s_off -> eigenvalues,
 // u_off -> eigenvectors (columns), ascending order.
 __global__ __launch_bounds__(F_THREADS) void k_eigh(FactorArgs fa)
@@ -378,9 +454,9 @@ __global__ __launch_bounds__(F_THREADS) void k_eigh(FactorArgs fa)
     for(int j = tid; j < n; j += F_THREADS) ev[j] = Am[j + int64_t(j) * n];
     __syncthreads();
     for(int j = tid; j < n; j += F_THREADS) {
-        const double vj = ev[j];
+        const double vj = nan_high(ev[j]);
         int rank = 0;
-        for(int k = 0; k < n; ++k) rank += (ev[k] < vj) || (ev[k] == vj && k < j);
+        for(int k = 0; k < n; ++k) rank += (nan_high(ev[k]) < vj) || (nan_high(ev[k]) == vj && k < j);
         ord[rank] = j;
     }
     __syncthreads();
@@ -536,5 +612,27 @@ extern "C" int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t n
             if(it.m != it.n) throw xh::Error(XH_EINVAL, "eigh: blocks must be square");
         run_factor(
             ctx, k_eigh, v, [](const xh_factor_item& it) { return 2 * int64_t(it.m) * it.m + 2 * int64_t(it.m) + 4; }, A, V, W, nullptr);
+    });
+}
+
+extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* G, double* X, const double* shift_rel)
+{
+    return guard(ctx, [&] {
+        std::vector<xh_factor_item> v(items, items + nitems);
+        std::vector<double> sh(std::max<int64_t>(nitems, 1), 0.0);
+        for(int64_t i = 0; i < nitems; ++i) {
+            if(v[i].m != v[i].n) throw xh::Error(XH_EINVAL, "chol_inv: blocks must be square");
+            v[i].s_off = i;
+            if(shift_rel) sh[i] = shift_rel[i];
+        }
+        double* d_sh = xh::dev_upload(ctx, sh);
+        try {
+            run_factor(
+                ctx, k_chol_inv, v, [](const xh_factor_item& it) { return int64_t(it.n) * it.n + it.n; }, G, X, d_sh, nullptr);
+        } catch(...) {
+            xh::dev_free(ctx, d_sh);
+            throw;
+        }
+        xh::dev_free(ctx, d_sh);
     });
 }

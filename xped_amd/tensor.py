@@ -225,6 +225,13 @@ class Tensor:
         check(lib().xt_tensor_coupled(self.h, 1, C.byref(h)), self.rt.h, "coupledCodomain")
         return Qbasis(self.sym, h)
 
+    def plain_size(self) -> int:
+        """Number of entries of the dense (symmetry-free) tensor: the product of the legs' full dims."""
+        n = 1
+        for i in range(self.rank + self.corank):
+            n *= self.leg(i).fullDim()
+        return n
+
     def sectors(self):
         """[(q, offset, rows, cols)] in storage order."""
         out = []
@@ -300,6 +307,24 @@ class Tensor:
         check(lib().xt_tsvd(self.h, int(maxKeep), float(eps_svd), int(bool(PRESERVE_MULTIPLETS)), C.byref(U), C.byref(S), C.byref(V), C.byref(tw),
                             C.byref(en)), self.rt.h, "tSVD")
         return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, en.value
+
+    def tSVD_randomized(self, maxKeep: int, eps_svd: float, ranks: Dict, oversample: int = 8, rank_factor: float = 2.0, power_iters: int = 1,
+                        sketch_precision: str = "fp64", seed: int = 1):
+        """Randomized truncated block SVD; returns (U, S, Vdag, truncWeight, safe, residual_ratio)."""
+        n = len(ranks)
+        qs = (C.c_int * (2 * max(n, 1)))()
+        rk = (C.c_int * max(n, 1))()
+        for i, (q, r) in enumerate(ranks.items()):
+            qq = (q,) if isinstance(q, int) else tuple(q)
+            qs[2 * i] = qq[0]
+            qs[2 * i + 1] = qq[1] if len(qq) > 1 else 0
+            rk[i] = int(r)
+        U, S, V = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        tw, safe, ratio = C.c_double(0.0), C.c_int(), C.c_double()
+        check(lib().xt_tsvd_randomized(self.h, int(maxKeep), float(eps_svd), qs, rk, n, int(oversample), float(rank_factor), int(power_iters),
+                                       PRECISION[sketch_precision], int(seed), C.byref(U), C.byref(S), C.byref(V), C.byref(tw), C.byref(safe),
+                                       C.byref(ratio)), self.rt.h, "tSVD_randomized")
+        return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, bool(safe.value), ratio.value
 
     def tQR(self):
         Q, R = C.c_void_p(), C.c_void_p()
@@ -408,10 +433,20 @@ class CTM:
             _lib._lib.xt_ctm_destroy(h)
             self.h = None
 
-    def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=2, record_spectra=False):
+    def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=1, record_spectra=False):
         check(lib().xt_ctm_set_options(self.h, PRECISION[precision], 1 if svd == "randomized" else 0, oversample, power_iters, int(record_spectra)),
               self.rt.h, "set_options")
         return self
+
+    def set_rsvd(self, oversample=8, rank_factor=2.0, power_iters=1, sketch_precision="fp64"):
+        check(lib().xt_ctm_set_rsvd(self.h, int(oversample), float(rank_factor), int(power_iters), PRECISION[sketch_precision]), self.rt.h, "set_rsvd")
+        return self
+
+    def rsvd_stats(self):
+        """(randomized SVDs run, full-SVD fallbacks, largest missed-weight ratio)"""
+        a, b, r = C.c_long(), C.c_long(), C.c_double()
+        lib().xt_ctm_rsvd_stats(self.h, C.byref(a), C.byref(b), C.byref(r))
+        return a.value, b.value, r.value
 
     def init(self):
         check(lib().xt_ctm_init(self.h), self.rt.h, "CTM::init")
