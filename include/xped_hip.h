@@ -110,6 +110,8 @@ typedef struct {
     int32_t m, n;
 } xh_factor_item;
 int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt);
+/* SVD backend for large blocks: 0 = own one-sided (block) Jacobi (default), 1 = rocSOLVER gesvd */
+int xh_set_svd_backend(xh_ctx* ctx, int backend);
 /* QR: Q (m x r) at u_off, R (r x n) at vt_off, Householder, R with the sign convention of
  * Eigen::HouseholderQR (diagonal may be negative). */
 int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* Q, double* R);

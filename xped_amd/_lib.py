@@ -83,6 +83,7 @@ _SIGS = [
     ("xh_gemm", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]),
     ("xh_gemm_plan_flops", C.c_int, [_P, _DP]),
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
+    ("xh_set_svd_backend", C.c_int, [_P, C.c_int]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_eigh", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_chol_inv", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _DP]),

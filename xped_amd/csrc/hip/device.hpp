@@ -19,7 +19,12 @@ struct xh_ctx {
     double* host_scalar = nullptr;
     double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
+    int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
 };
+
+// FP64 SVD of large sector blocks by one-sided block Jacobi (bjsvd.hip); same item layout as
+// xh_block_svd.
+void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, const double* A, double* U, double* S, double* Vt);
 
 namespace xh {
 

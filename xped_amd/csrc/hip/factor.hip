@@ -531,10 +531,12 @@ void run_factor(xh_ctx* ctx, void (*kern)(FactorArgs), const std::vector<xh_fact
 
 }  // namespace
 
-// Blocks whose smaller side exceeds this use rocSOLVER's FP64 gesvd (Householder
-// bidiagonalisation + bidiagonal QR, the LAPACK algorithm the reference reaches through Eigen and
-// LAPACKE); smaller ones the batched one-sided Jacobi kernel above.
+// With the vendor backend selected (xh_set_svd_backend), blocks whose smaller side exceeds this use
+// rocSOLVER's FP64 gesvd (Householder bidiagonalisation + bidiagonal QR, the LAPACK algorithm the
+// reference reaches through Eigen and LAPACKE).
 constexpr int VENDOR_SVD_MIN = 384;
+// blocks with more columns than this (after transposition) use the block Jacobi path
+constexpr int BJ_MIN_COLS = 64;
 
 static void vendor_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& big, const double* A, double* U, double* S, double* Vt)
 {
@@ -577,14 +579,23 @@ static void vendor_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& big, cons
 extern "C" int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt)
 {
     return guard(ctx, [&] {
-        std::vector<xh_factor_item> v, big;
+        // small blocks: one workgroup each, in LDS; larger ones: block Jacobi over grouped GEMMs
+        // (or the vendor LAPACK-style path when selected)
+        std::vector<xh_factor_item> v, big, vendor;
+        const size_t budget = lds_budget(ctx);
         for(int64_t i = 0; i < nitems; ++i) {
-            if(std::min(items[i].m, items[i].n) > VENDOR_SVD_MIN)
-                big.push_back(items[i]);
+            const xh_factor_item& it = items[i];
+            if(it.m <= 0 || it.n <= 0) continue;
+            const int mn = std::min(it.m, it.n), mx = std::max(it.m, it.n);
+            if(ctx->svd_backend == 1 && mn > VENDOR_SVD_MIN)
+                vendor.push_back(it);
+            else if(mn > BJ_MIN_COLS || size_t(svd_work_doubles(mx, mn)) * sizeof(double) > budget)
+                big.push_back(it);
             else
-                v.push_back(items[i]);
+                v.push_back(it);
         }
-        vendor_svd(ctx, big, A, U, S, Vt);
+        vendor_svd(ctx, vendor, A, U, S, Vt);
+        block_jacobi_svd(ctx, big, A, U, S, Vt);
         run_factor(
             ctx, k_svd_jacobi, v,
             [](const xh_factor_item& it) {
@@ -634,5 +645,13 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
             throw;
         }
         xh::dev_free(ctx, d_sh);
+    });
+}
+
+extern "C" int xh_set_svd_backend(xh_ctx* ctx, int backend)
+{
+    return guard(ctx, [&] {
+        if(backend != 0 && backend != 1) throw xh::Error(XH_EINVAL, "svd backend must be 0 (own Jacobi) or 1 (rocSOLVER gesvd)");
+        ctx->svd_backend = backend;
     });
 }
