@@ -124,7 +124,7 @@ int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, 
 
 /* ---- elementwise / reductions on a whole tensor buffer or its blocks ---------------------- */
 int xh_scale(xh_ctx* ctx, double* x, size_t n, double alpha);                          /* TensorBase.hpp:197-209 */
-int xh_axpby(xh_ctx* ctx, double* y, const double* x, size_t n, double a, double b);    /* y = a*x + b*y (CoeffBinaryOp) */
+int xh_axpby(xh_ctx* ctx, double* y, const double* x, size_t n, double a, double b);    /* y = a*x + b*y (CoeffBinaryOp); b == 0: y not read */
 int xh_max_abs(xh_ctx* ctx, const double* x, size_t n, double* out);                   /* maxNorm, TensorBase.cpp:48-56 */
 int xh_dot(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out);       /* squaredNorm building block */
 /* Per block g (m_g x n_g at off_g): out[g] = trace of the block (TensorBase.cpp:26-46). */

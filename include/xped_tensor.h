@@ -132,8 +132,10 @@ typedef struct xt_ctm xt_ctm;
 int xt_ctm_create(xt_rt* rt, const xt_tensor* const* A, size_t chi, xt_ctm** out);
 int xt_ctm_destroy(xt_ctm* c);
 /* precision: XH_FP64/XH_FP32/XH_BF16X3 for the contractions; svd_mode 0 = full FP64 block SVD,
- * 1 = randomized block SVD with per-sector target ranks */
-int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int record_spectra);
+ * 1 = randomized block SVD with per-sector target ranks; flags: bit 0 records the full spectra,
+ * bit 1 checks every projector factor for non-finite entries, bit 2 prints one line per
+ * projector decomposition to stderr */
+int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int flags);
 int xt_ctm_init(xt_ctm* c);                         /* CTM::init, FROM_A (CTM.cpp:137-240) */
 int xt_ctm_grow_all(xt_ctm* c);                     /* CTM::grow_all (CTM.cpp:386-420) */
 int xt_ctm_compute_rdm(xt_ctm* c);                  /* CTM::computeRDM (CTM.cpp:422-430) */

@@ -237,3 +237,22 @@ def test_elementwise_ops(rt, pcode):
     sq = t * t.adjoint()
     assert abs(sq.trace() - oten.trace(oten.mul(o, oad))) <= 1e-12 * abs(oten.trace(oten.mul(o, oad)))
     assert np.array_equal(t.scale(-2.5).download(), -2.5 * o.data)
+
+
+def test_axpby_with_zero_b_does_not_read_the_output(rt):
+    # scaling into a fresh buffer must not depend on its previous contents (0 * NaN = NaN)
+    L = lib()
+    ctx = rt.ctx()
+    n = 1000
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    x[7] = -0.0
+    dx, dy = C.c_void_p(), C.c_void_p()
+    assert L.xh_alloc(ctx, 8 * n, C.byref(dx)) == 0 and L.xh_alloc(ctx, 8 * n, C.byref(dy)) == 0
+    L.xh_upload(ctx, dx, x.ctypes.data, 8 * n)
+    L.xh_fill(ctx, dy, n, float("nan"))
+    assert L.xh_axpby(ctx, dy, dx, n, 2.5, 0.0) == 0
+    y = np.empty(n)
+    L.xh_download(ctx, y.ctypes.data, dy, 8 * n)
+    L.xh_free(ctx, dx)
+    L.xh_free(ctx, dy)
+    assert np.array_equal(y, 2.5 * x) and np.signbit(y[7])

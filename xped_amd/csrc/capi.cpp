@@ -585,7 +585,7 @@ int xt_ctm_destroy(xt_ctm* c)
     return XH_OK;
 }
 
-int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int record_spectra)
+int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int flags)
 {
     return run(c->owner, [&] {
         xa::CTMOptions o = c->ctm->opt;
@@ -593,7 +593,9 @@ int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, i
         o.svd = svd_mode == 1 ? xa::SvdMode::Randomized : xa::SvdMode::Full;
         o.rsvd.oversample = oversample;
         o.rsvd.power_iters = power_iters;
-        o.record_spectra = record_spectra != 0;
+        o.record_spectra = (flags & 1) != 0;
+        o.check_finite = (flags & 2) != 0;
+        o.verbose = (flags & 4) != 0;
         c->ctm->set_options(o);
     });
 }

@@ -7,6 +7,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <stdexcept>
 
 namespace xa {
@@ -161,6 +162,11 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
     // PEPSContractions.cpp:16-43: prod = T1 * T2, tSVD(chi, 1e-10, ., PRESERVE_MULTIPLETS = false),
     // P1 = S^-1/2 U^dagger T1, P2 = T2 V S^-1/2
     DTensor prod = mul(Ta, Tb, opt.precision);
+    {
+        const double m = max_norm(prod);
+        if(!std::isfinite(m))
+            throw Error(XH_EINVAL, "CTM: non-finite projector product at site " + std::to_string(site) + ", direction " + std::to_string(dir));
+    }
     auto t0 = std::chrono::steady_clock::now();
     SVDResult r;
     if(opt.svd == SvdMode::Full) {
@@ -204,10 +210,23 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
         last_kept[{site, dir}] = kept;
     }
     if(rt->ctx) xh_synchronize(rt->ctx);
-    svd_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    svd_ms += dt;
+    if(opt.verbose) {
+        std::size_t kept = 0;
+        for(const auto& [q, v] : r.spec) kept += v.size();
+        std::fprintf(stderr, "[ctm] site %d dir %d: %zu sectors, %zu kept, svd %.1f ms\n", site, dir, prod.st->sector.size(), kept, dt);
+        std::fflush(stderr);
+    }
     DTensor isqrtS = diag_map(r.S, 2);  // S.diag_sqrt().diag_inv()
     DTensor P1 = mul(mul(isqrtS, adjoint(r.U), opt.precision), Ta, opt.precision);
     DTensor P2 = mul(mul(Tb, adjoint(r.Vdag), opt.precision), isqrtS, opt.precision);
+    if(opt.check_finite) {
+        const std::pair<const char*, const DTensor*> parts[] = {{"U", &r.U}, {"S", &r.S}, {"Vdag", &r.Vdag}, {"P1", &P1}, {"P2", &P2}};
+        for(const auto& [name, t] : parts)
+            if(!std::isfinite(max_norm(*t)))
+                throw Error(XH_EINVAL, std::string("CTM: non-finite ") + name + " at site " + std::to_string(site) + ", direction " + std::to_string(dir));
+    }
     return {P1, P2};
 }
 

@@ -38,6 +38,8 @@ struct CTMOptions {
     // SVD of the same (site, direction) are used
     std::function<std::map<Q, int>(const DTensor& prod, int site, int dir)> rank_policy;
     bool record_spectra = false;
+    bool check_finite = false;  // check every projector factor for non-finite entries (diagnostics)
+    bool verbose = false;       // one line per projector decomposition on stderr
 };
 
 struct SpectrumRecord {

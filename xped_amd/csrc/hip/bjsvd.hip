@@ -29,8 +29,8 @@ namespace {
 
 constexpr int BJ_W = 32;          // column block width; pairs are 64 columns wide
 constexpr int BJ_THREADS = 256;
-constexpr int BJ_MAX_SWEEPS = 40;
-constexpr int BJ_INNER_SWEEPS = 30;
+constexpr int BJ_MAX_SWEEPS = 60;
+constexpr int BJ_INNER_SWEEPS = 2;  // inexact pair solves: the outer sweeps finish the job
 
 struct BjPair {
     int64_t g;       // offset of the pair's n2 x n2 Gram matrix (and of its J)
@@ -358,6 +358,7 @@ void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, con
             const int rc0 = xh_range_sumsq(ctx, off.data(), len.data(), static_cast<int64_t>(off.size()), A, nrm.data());
             if(rc0 != XH_OK) throw xh::Error(rc0, "block Jacobi: block norms");
             for(std::size_t q = 0; q < secs.size(); ++q) {
+                if(!std::isfinite(nrm[q])) throw xh::Error(XH_EINVAL, "block SVD: non-finite input block");
                 const double f = double(secs[q].rows) * DBL_EPSILON;
                 secs[q].z = f * f * nrm[q];
             }
@@ -445,7 +446,10 @@ void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, con
             std::vector<double> nrm(roff[b].size());
             rc = xh_range_sumsq(ctx, roff[b].data(), rlen[b].data(), static_cast<int64_t>(roff[b].size()), buf[b], nrm.data());
             if(rc != XH_OK) throw xh::Error(rc, "block Jacobi: column norms");
-            for(std::size_t k = 0; k < nrm.size(); ++k) sig[where[b][k].first][where[b][k].second] = std::sqrt(nrm[k]);
+            for(std::size_t k = 0; k < nrm.size(); ++k) {
+                if(!std::isfinite(nrm[k])) throw xh::Error(XH_EHIP, "block Jacobi SVD: non-finite column norm");
+                sig[where[b][k].first][where[b][k].second] = std::sqrt(nrm[k]);
+            }
         }
         std::vector<xh_copy_rect> out[2];
         for(std::size_t q = 0; q < secs.size(); ++q) {

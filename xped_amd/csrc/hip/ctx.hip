@@ -153,16 +153,24 @@ __global__ void k_scale(double* __restrict__ x, size_t n, double a)
     for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = a * x[i];
 }
 
+// y = a x + b y; with b == 0, y is not read (it may be uninitialised, and 0 * NaN is NaN), and
+// the result is the plain product a x, as the reference's scalar multiplication computes it
 __global__ void k_axpby(double* __restrict__ y, const double* __restrict__ x, size_t n, double a, double b)
 {
+    if(b == 0.0) {
+        for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = __dmul_rn(a, x[i]);
+        return;
+    }
     for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
         y[i] = __dadd_rn(__dmul_rn(a, x[i]), __dmul_rn(b, y[i]));
 }
 
 // Two-stage reductions: per-block partials into scratch, then a single block folds them.
+// max that propagates NaN (fmax would drop it), so that maxNorm of non-finite data is non-finite
+__device__ inline double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
 __device__ inline double wave_max(double v)
 {
-    for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    for(int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
     return v;
 }
 __device__ inline double wave_sum(double v)
@@ -174,14 +182,14 @@ __device__ inline double wave_sum(double v)
 __global__ void k_maxabs(const double* __restrict__ x, size_t n, double* __restrict__ part)
 {
     double m = 0.0;
-    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = fmax(m, fabs(x[i]));
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = nmax(m, fabs(x[i]));
     m = wave_max(m);
     __shared__ double s[EW_THREADS / 64];
     if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
     __syncthreads();
     if(threadIdx.x == 0) {
         double r = s[0];
-        for(int w = 1; w < EW_THREADS / 64; ++w) r = fmax(r, s[w]);
+        for(int w = 1; w < EW_THREADS / 64; ++w) r = nmax(r, s[w]);
         part[blockIdx.x] = r;
     }
 }
@@ -204,14 +212,14 @@ __global__ void k_dot(const double* __restrict__ x, const double* __restrict__ y
 __global__ void k_fold(const double* __restrict__ part, int n, double* __restrict__ out, int is_max)
 {
     double acc = 0.0;
-    for(int i = threadIdx.x; i < n; i += blockDim.x) acc = is_max ? fmax(acc, part[i]) : acc + part[i];
+    for(int i = threadIdx.x; i < n; i += blockDim.x) acc = is_max ? nmax(acc, part[i]) : acc + part[i];
     acc = is_max ? wave_max(acc) : wave_sum(acc);
     __shared__ double s[EW_THREADS / 64];
     if((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
     __syncthreads();
     if(threadIdx.x == 0) {
         double r = s[0];
-        for(int w = 1; w < EW_THREADS / 64; ++w) r = is_max ? fmax(r, s[w]) : r + s[w];
+        for(int w = 1; w < EW_THREADS / 64; ++w) r = is_max ? nmax(r, s[w]) : r + s[w];
         *out = r;
     }
 }
