@@ -1,19 +1,29 @@
 """Benchmark of the block-sparse symmetric tensor hot path on one MI355X.
 
-Default workload (one "step"): the grouped U(1)xU(1) fermionic contraction
-A(l1,l2|l3,l4) . B(l3,l4|l5,l6) over two legs, every leg of dimension 192 spread over the 85
-charge sectors (N, 2Sz), |N|, |2Sz| <= 6, N = 2Sz mod 2, with seeded power-law degeneracies
-(exponent 1.5): 313 matched sector GEMMs of sizes 1..468, issued as ONE persistent grouped
-launch.  Inputs are resident in HBM before the timed region.
+Default workload (--workload c4): one CTMRG step (grow_all + computeRDM + energy, CORNER
+projection) of a U(1) iPEPS on a 2x2 unit cell, virtual D = 8 over charges -3..3
+({+-3:1, +-2:1, +-1:1, 0:2}), physical spin 1/2, environment chi = 256, nearest-neighbour
+Heisenberg energy.  Two modes run on fresh runtimes (no shared plan caches):
+  fp64 - plain FP64 GPU path: FP64 grouped GEMMs, full FP64 block SVDs (one-sided block Jacobi)
+  fast - the accelerated path: FP64 grouped GEMMs, randomized block SVDs of the projector
+         products with target ranks from the previous step (full FP64 SVD as fallback)
+Both start from the same warm-up (init from the double layer + one full-SVD grow_all reaching
+chi = 256).  `value` is the contraction throughput of the fast step: the summed algorithmic flops
+of its grouped contraction GEMMs over their HIP-event-timed kernel time; `ms_per_step` is the
+fast step's wall time; the fp64 step time, the speedup and the energy difference are reported
+beside them.
 
-Prints one JSON line (metric / value / roofline / cpu_baseline ...).  --gpus N > 1 is refused:
-this build runs on one GPU.
+--workload c3 times the grouped U(1)xU(1) fermionic contraction A(l1,l2|l3,l4).B(l3,l4|l5,l6)
+(leg dim 192, 313 matched sector GEMMs) instead.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -24,20 +34,152 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "contraction TFLOP/s + CTMRG step ms, mixed/learned vs FP64 GPU at ≤1e-10 err"
 PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}  # TFLOP/s, dense; BF16x3 counted as 3 BF16 products
+SCALE = 10.0  # site tensors N(0,1) * SCALE / sqrt(plain size), see run_c4
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--leg-dim", type=int, default=192)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c3"])
+    ap.add_argument("--steps", type=int, default=3, help="timed fast-mode steps (c4) or contractions (c3)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed fast-mode steps after the common warm-up (c4)")
+    ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps of the plain FP64 mode (c4)")
+    ap.add_argument("--chi", type=int, default=256)
+    ap.add_argument("--power-iters", type=int, default=6)
+    ap.add_argument("--oversample", type=int, default=16)
+    ap.add_argument("--leg-dim", type=int, default=192, help="c3 leg dimension")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the fast mode / c3")
+    ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
     return ap.parse_args()
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------------------------------------ C4
+def run_mode(mode, args, xt, configs, random_ipeps):
+    """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
+    rt = xt.Runtime(0)
+    A = [a.scale(SCALE) for a in random_ipeps(rt, configs.c4_virtual(), configs.c4_physical(), args.seed)]
+    c = xt.CTM(rt, A, args.chi).init()
+    t0 = time.perf_counter()
+    c.grow_all()  # common warm-up: full FP64 SVDs, chi grows from D^2 = 64 to chi
+    rt.synchronize()
+    log(f"[bench] {mode}: warm-up grow_all {time.perf_counter() - t0:.1f} s")
+    if mode == "fast":
+        c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters)
+    nsteps = args.steps if mode == "fast" else args.fp64_steps
+    nwarm = args.warmup if mode == "fast" else 0
+    energies = []
+    for _ in range(nwarm):
+        energies.append(c.step())
+    rt.synchronize()
+    svd0 = c.svd_ms()
+    c0 = rt.counters()
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        energies.append(c.step())
+        log(f"[bench] {mode}: step {k} E = {energies[-1]:.15f}")
+    rt.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = rt.counters()
+    out = {"ms_per_step": 1e3 * wall / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
+           "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
+           "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
+    if mode == "fast":
+        # one more step with every contraction GEMM timed by HIP events on the context stream
+        rt.gemm_timing(True, reset=True)
+        c.step()
+        ms, fl, nlog = rt.gemm_timing(False)
+        out["gemm_kernel_ms"] = ms
+        out["gemm_timed_flops"] = fl
+        out["gemm_launches_blocks"] = nlog
+        out["gemm_shapes"] = rt.gemm_log()
+    return out
+
+
+def cpu_gemm_baseline(shapes, seconds):
+    """The reference's CPU hot loop of operator* (TensorBase.cpp:262-279) restated by the oracle:
+    one FP64 BLAS GEMM per matched sector block, BLAS threads on all host cores, over the block
+    shapes of the fast step's contraction GEMMs (random FP64 data of those shapes)."""
+    import numpy as np
+
+    from oracle import tensor as oten
+
+    rng = np.random.default_rng(0)
+    cache = {}
+    work = []
+    flops = 0.0
+    for m, n, k in shapes:
+        key = (int(m), int(n), int(k))
+        if key not in cache:
+            cache[key] = (np.asfortranarray(rng.uniform(-1, 1, (key[0], key[2]))), np.asfortranarray(rng.uniform(-1, 1, (key[2], key[1]))),
+                          np.empty((key[0], key[1]), order="F"))
+        work.append(cache[key])
+        flops += 2.0 * key[0] * key[1] * key[2]
+    done, t0, n = 0.0, time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds and n < len(work) * 50:
+        a, b, c = work[n % len(work)]
+        oten.block_gemm(a, b, c)
+        done += 2.0 * a.shape[0] * a.shape[1] * b.shape[1]
+        n += 1
+    el = time.perf_counter() - t0
+    threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+    return {"value": round(done / el / 1e12, 6), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
+            "sample": f"{n} sector-block GEMMs ({done / 1e12:.3f} TFLOP of the {flops / 1e12:.3f} TFLOP of one step's {len(work)} blocks, "
+                      f"in step order) in {el:.1f} s; oracle per-block FP64 matmul (numpy/OpenBLAS)"}
+
+
+def run_c4(args):
+    import xped_amd.tensor as xt
+    from xped_amd import configs
+    from xped_amd.ipeps import random_ipeps
+
+    fast = run_mode("fast", args, xt, configs, random_ipeps)
+    ref = None if args.no_fp64 else run_mode("fp64", args, xt, configs, random_ipeps)
+    value = fast["gemm_timed_flops"] / (fast["gemm_kernel_ms"] * 1e-3) / 1e12 if fast["gemm_kernel_ms"] > 0 else 0.0
+    peak = PEAKS[args.precision]
+    res = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fast["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[args.precision],
+        "data": f"synthetic (site tensors N(0,1)*{SCALE:g}/sqrt(plain size), std::mt19937(seed + site))",
+        "config": {"workload": f"U(1) iPEPS CTMRG step, 2x2 cell, D=8, chi={args.chi}, Heisenberg", "chi": args.chi, "D": 8,
+                   "svd": f"randomized, l = 2 k + {args.oversample} for the previous step's kept k, {args.power_iters} power iterations, FP64 fallback",
+                   "gemm_precision": args.precision, "seed": args.seed},
+        "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
+                 "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
+                 "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
+                 "host_plan_s": round(fast["host"]["permute_plan_s"] + fast["host"]["gemm_plan_s"], 3)},
+        "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5), "traffic": None,
+                     "kernel": f"k_gemm_{'f64' if args.precision == 'fp64' else args.precision}",
+                     "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},
+    }
+    if ref is not None:
+        nb = min(len(ref["energies"]), len(fast["energies"]))
+        errs = [abs(fast["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nb)]
+        res["fp64"] = {"ms_per_step": round(ref["ms_per_step"], 3), "svd_ms_per_step": round(ref["svd_ms_per_step"], 3), "energies": ref["energies"],
+                       "host_plan_s": round(ref["host"]["permute_plan_s"] + ref["host"]["gemm_plan_s"], 3)}
+        res["speedup_vs_fp64"] = round(ref["ms_per_step"] / fast["ms_per_step"], 3)
+        res["energy_rel_err_vs_fp64"] = max(errs) if errs else None
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_gemm_baseline(fast["gemm_shapes"], args.cpu_seconds)
+    return res
+
+
+# ------------------------------------------------------------------------------------------ C3
 def build_c3(rt, leg_dim, xt, configs):
     def basis(entries):
         b = xt.Qbasis(xt.FU1xU1)
@@ -51,10 +193,9 @@ def build_c3(rt, leg_dim, xt, configs):
     return A, B
 
 
-def cpu_baseline(leg_dim, seconds, configs):
-    """The reference's CPU hot loop of operator* (TensorBase.cpp:262-279) restated by the oracle:
-    one FP64 BLAS GEMM per matched sector, serial over sectors, BLAS threads on all host cores.
-    Block structures are built once outside the timed loop, as plans are on the GPU side."""
+def cpu_baseline_c3(leg_dim, seconds, configs):
+    """operator* (TensorBase.cpp:262-279) restated by the oracle: one FP64 BLAS GEMM per matched
+    sector, serial over sectors; block structures are built once outside the timed loop."""
     import numpy as np
 
     from oracle import tensor as oten
@@ -84,9 +225,9 @@ def cpu_baseline(leg_dim, seconds, configs):
 
     def step():
         for a, b, c in pairs:
-            np.matmul(a, b, out=c)
+            oten.block_gemm(a, b, c)
 
-    step()  # warm-up
+    step()
     n, t0 = 0, time.perf_counter()
     while True:
         step()
@@ -100,11 +241,7 @@ def cpu_baseline(leg_dim, seconds, configs):
                       f"(oracle: numpy/OpenBLAS matmul per sector, structures prebuilt)"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1"}))
-        sys.exit(2)
+def run_c3(args):
     import xped_amd.tensor as xt
     from xped_amd import _lib, configs
 
@@ -113,15 +250,15 @@ def main():
     A, B = build_c3(rt, args.leg_dim, xt, configs)
     prec = args.precision
     a1, a2 = [-1, -2, 1, 2], [1, 2, -3, -4]
-    # plans are built on the first call (host structure work, cached afterwards)
-    for _ in range(args.warmup):
+    steps = max(args.steps, 10)
+    for _ in range(max(args.warmup, 2)):
         Cm = A.contract(B, a1, a2, 2, precision=prec)
     rt.synchronize()
     c0 = rt.counters()
     ctx = rt.ctx()
     L.xh_event_record(ctx, 0)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         Cm = A.contract(B, a1, a2, 2, precision=prec)
     L.xh_event_record(ctx, 1)
     rt.synchronize()
@@ -129,58 +266,38 @@ def main():
     ms = C.c_float()
     L.xh_event_elapsed_ms(ctx, 0, 1, C.byref(ms))
     c1 = rt.counters()
-    flops_step = (c1["gemm_flops"] - c0["gemm_flops"]) / args.steps
-    step_ms = ms.value / args.steps
+    flops_step = (c1["gemm_flops"] - c0["gemm_flops"]) / steps
+    step_ms = ms.value / steps
     value = flops_step / (step_ms * 1e-3) / 1e12
-
-    # dominant kernel: the grouped GEMM launch alone (plan of the same matched sectors, output
-    # preallocated), timed with HIP events on the context stream it is launched on
-    Cm = A.mul(B, precision=prec)
-    sa = {q: (o, r, c) for q, o, r, c in A.sectors()}
-    sb = {q: (o, r, c) for q, o, r, c in B.sectors()}
-    sc = {q: (o, r, c) for q, o, r, c in Cm.sectors()}
-    items = [_lib.xh_gemm_item(sa[q][0], sb[q][0], sc[q][0], sa[q][1], sb[q][2], sa[q][2]) for q in sa if q in sb]
-    arr = (_lib.xh_gemm_item * len(items))(*items)
-    plan = C.c_void_p()
-    _lib.check(L.xh_gemm_plan_create(ctx, arr, len(items), 0, 0, C.byref(plan)), None, "gemm plan")
-    pa, pb, pc = A.device_ptr(), B.device_ptr(), Cm.device_ptr()
-    code = _lib.XH_FP64 if prec == "fp64" else (_lib.XH_FP32 if prec == "fp32" else _lib.XH_BF16X3)
-    L.xh_gemm(ctx, plan, pa, pb, pc, 1.0, 0.0, code)
-    rt.synchronize()
-    reps = max(20, args.steps)
-    L.xh_event_record(ctx, 2)
-    for _ in range(reps):
-        L.xh_gemm(ctx, plan, pa, pb, pc, 1.0, 0.0, code)
-    L.xh_event_record(ctx, 3)
-    rt.synchronize()
-    L.xh_event_elapsed_ms(ctx, 2, 3, C.byref(ms))
-    gemm_ms = ms.value / reps
-    L.xh_gemm_plan_destroy(ctx, plan)
-    achieved = flops_step / (gemm_ms * 1e-3) / 1e12
+    # dominant kernel: the grouped GEMM launch alone, timed with HIP events on the context stream
+    rt.gemm_timing(True, reset=True)
+    for _ in range(20):
+        Cm = A.mul(B, precision=prec)
+    gemm_ms, gemm_fl, nblocks = rt.gemm_timing(False)
+    achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
     peak = PEAKS[prec]
-    nblocks = len(items)
-    out = {
-        "metric": METRIC,
-        "value": round(value, 4),
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(step_ms, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[prec],
-        "data": "synthetic (seeded mt19937 uniform[-1,1) blocks)",
+    res = {
+        "metric": METRIC, "value": round(value, 4), "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[prec], "data": "synthetic (seeded mt19937 uniform[-1,1) blocks)",
         "config": {"workload": f"U(1)xU(1) fermionic grouped contraction A(l1,l2|l3,l4).B(l3,l4|l5,l6), leg dim {args.leg_dim}",
-                   "matched_blocks": nblocks, "gflop_per_step": round(flops_step / 1e9, 4), "precision": prec},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5),
-                     "traffic": None, "kernel": f"k_gemm_{'f64' if prec == 'fp64' else prec}", "kernel_ms": round(gemm_ms, 5)},
+                   "matched_blocks": nblocks // 20, "gflop_per_step": round(flops_step / 1e9, 4), "precision": prec},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
+                     "kernel": f"k_gemm_{'f64' if prec == 'fp64' else prec}", "kernel_ms": round(gemm_ms / 20, 5)},
         "wall_s_timed": round(wall, 4),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.leg_dim, args.cpu_seconds, configs)
-    print(json.dumps(out))
+        res["cpu_baseline"] = cpu_baseline_c3(args.leg_dim, args.cpu_seconds, configs)
+    return res
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (the multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    res = run_c4(args) if args.workload == "c4" else run_c3(args)
+    print(json.dumps(res))
 
 
 if __name__ == "__main__":

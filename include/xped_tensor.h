@@ -39,6 +39,14 @@ xh_ctx* xt_rt_ctx(xt_rt* rt);
 /* counters: total GEMM flops and permute bytes issued, number of launches */
 int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, long* n_gemm, long* n_permute);
 int xt_rt_clear_caches(xt_rt* rt);
+/* host time (seconds) spent building permute and GEMM plans, and how many were built */
+int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans);
+/* contraction GEMM timing: enable = 1 times every operator* launch with HIP events (and logs its
+ * block shapes), 0 stops, -1 leaves the setting; returns the totals since the last reset
+ * (reset = 1 clears them first) */
+int xt_rt_gemm_timing(xt_rt* rt, int enable, int reset, double* kernel_ms, double* flops, long* nlog);
+/* the logged block shapes: 3 ints (m, n, k) per block, up to cap blocks */
+int xt_rt_gemm_log(const xt_rt* rt, int32_t* mnk, long cap);
 
 /* ---- bases: Qbasis<Symmetry, depth> (include/Xped/Core/Qbasis.hpp) -------------------- */
 int xt_basis_create(int kind0, int kind1, xt_basis** out);                 /* empty depth-1 basis */
@@ -105,9 +113,10 @@ int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_mu
  * PRESERVE_MULTIPLETS = false): sector q is sketched with min(ceil(rank_factor * k_q) + oversample,
  * m_q, n_q) columns for the target ranks k_q given as nsec pairs (qs[2 i], qs[2 i + 1]) -> ranks[i]
  * (0 for absent sectors), power_iters subspace iterations in sketch_precision.  *safe is 0 when
- * the weight missed by a sketch could hide a singular value at or above the smallest kept one;
- * *residual_ratio is the largest missed weight over the smallest kept sigma squared.  The full
- * tSVD is the reference's only SVD; this entry point has no counterpart there. */
+ * some sketched sector's smallest Ritz value is not 2x below the smallest kept singular value
+ * (a singular value the sketch missed could then belong to the kept set); *residual_ratio is the
+ * largest such ratio.  The full tSVD is the reference's only SVD; this entry point has no
+ * counterpart there. */
 int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, const int* qs, const int* ranks, int nsec, int oversample,
                        double rank_factor, int power_iters, int sketch_precision, uint64_t seed, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
                        double* trunc_weight, int* safe, double* residual_ratio);

@@ -139,6 +139,12 @@ def permute(t: Tensor, shift: int, p) -> Tensor:
     return out
 
 
+def block_gemm(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """The per-sector product of operator* (MatrixInterface::prod, MatrixInterface_Eigen_impl.cpp:155-160):
+    one FP64 BLAS GEMM, c = a @ b."""
+    return np.matmul(a, b, out=c)
+
+
 def mul(l: Tensor, r: Tensor) -> Tensor:
     """TensorBase::operator* (TensorBase.cpp:237-288)."""
     out = Tensor(l.sym, l.unc_dom, r.unc_cod)
@@ -147,7 +153,8 @@ def mul(l: Tensor, r: Tensor) -> Tensor:
         if j < 0:
             continue
         o = out.find(q)
-        out.set_block(o, out.block(o) + l.block(i) @ r.block(j))
+        a, b = l.block(i), r.block(j)
+        out.set_block(o, out.block(o) + block_gemm(a, b, np.empty((a.shape[0], b.shape[1]))))
     return out
 
 

@@ -100,6 +100,9 @@ _SIGS = [
     ("xt_rt_ctx", _P, [_P]),
     ("xt_rt_counters", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_clear_caches", C.c_int, [_P]),
+    ("xt_rt_host_stats", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("xt_rt_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_long)]),
+    ("xt_rt_gemm_log", C.c_int, [_P, C.POINTER(C.c_int32), C.c_long]),
     ("xt_basis_create", C.c_int, [C.c_int, C.c_int, _PP]),
     ("xt_basis_destroy", C.c_int, [_P]),
     ("xt_basis_copy", C.c_int, [_P, _PP]),

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-result", "-Wno-sign-compare"]
+CXX_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-pthread", "-Wall", "-Wno-unused-result", "-Wno-sign-compare"]
 
 
 def sources():
@@ -69,7 +69,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
                     print(out)
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for _, o in steps], "-o", str(tmp),
-          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,4 +1,5 @@
 // C ABI of the tensor layer (include/xped_tensor.h) over the C++ structure engine and device layer.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -121,6 +122,40 @@ int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, l
     if(permute_bytes) *permute_bytes = rt->rt.permute_bytes;
     if(n_gemm) *n_gemm = rt->rt.n_gemm;
     if(n_permute) *n_permute = rt->rt.n_permute;
+    return XH_OK;
+}
+
+int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans)
+{
+    if(!rt) return XH_EINVAL;
+    *permute_plan_s = rt->rt.permute_plan_s;
+    *gemm_plan_s = rt->rt.gemm_plan_s;
+    *n_permute_plans = rt->rt.n_permute_plans;
+    *n_gemm_plans = rt->rt.n_gemm_plans;
+    return XH_OK;
+}
+
+int xt_rt_gemm_timing(xt_rt* rt, int enable, int reset, double* kernel_ms, double* flops, long* nlog)
+{
+    if(!rt) return XH_EINVAL;
+    if(reset) {
+        rt->rt.gemm_kernel_ms = 0.0;
+        rt->rt.gemm_timed_flops = 0.0;
+        rt->rt.gemm_log.clear();
+    }
+    if(enable >= 0) rt->rt.time_gemm = enable != 0;
+    if(kernel_ms) *kernel_ms = rt->rt.gemm_kernel_ms;
+    if(flops) *flops = rt->rt.gemm_timed_flops;
+    if(nlog) *nlog = static_cast<long>(rt->rt.gemm_log.size());
+    return XH_OK;
+}
+
+int xt_rt_gemm_log(const xt_rt* rt, int32_t* mnk, long cap)
+{
+    if(!rt) return XH_EINVAL;
+    const long n = std::min<long>(cap, static_cast<long>(rt->rt.gemm_log.size()));
+    for(long i = 0; i < n; ++i)
+        for(int j = 0; j < 3; ++j) mnk[3 * i + j] = rt->rt.gemm_log[i][j];
     return XH_OK;
 }
 

@@ -27,6 +27,7 @@ struct RsvdOptions {
     double rank_factor = 2.0;
     int power_iters = 1;
     int sketch_precision = XH_FP64;  // GEMM precision of the range-finder products
+    double safety = 2.0;             // trust a sketch when safety * sigma_l(B) < sigma_cut
 };
 
 struct CTMOptions {
@@ -99,12 +100,13 @@ private:
 // Randomized truncated block SVD (Halko, Martinsson & Tropp, SIAM Review 53 (2011), Alg. 4.4,
 // subspace iteration with re-orthonormalisation): every sector is sketched with the panel width
 // of RsvdOptions for its target rank in `ranks` (0 when absent); then the reference's truncation
-// rule over the computed singular values.  `safe` is false when the Frobenius weight a sketch
-// missed could hide a singular value at or above the smallest kept one.
+// rule over the computed singular values.  `safe` is false when a sector's smallest Ritz value
+// is not a safety factor below the smallest kept singular value, i.e. when a singular value the
+// sketch missed could belong to the kept set.
 struct RandomizedSVD {
     SVDResult res;
     bool safe = true;
-    double max_residual_ratio = 0.0;  // max_q missed weight / sigma_cut^2
+    double max_residual_ratio = 0.0;  // max_q sigma_l(B_q) / sigma_cut over the sketched sectors
 };
 RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
                               std::uint64_t seed);

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <numeric>
 #include <sstream>
+#include <thread>
+#include <unordered_map>
 
 namespace xa {
 
@@ -125,7 +127,9 @@ void Qbasis::push_back(const Q& q, std::size_t inner)
         e.trees.push_back(t);
         data.push_back(e);
         sorted = false;
+        tree_index.reset();
     } else {
+        tree_index.reset();
         data[i].dim += inner;
         for(std::size_t j = i + 1; j < data.size(); ++j) data[j].num += inner;
         if(depth == 1) {
@@ -146,6 +150,7 @@ void Qbasis::push_back_trees(const Q& q, std::size_t inner, const std::vector<FT
     e.trees = t;
     data.push_back(e);
     sorted = false;
+    tree_index.reset();
     curr_dim += inner;
 }
 
@@ -167,6 +172,7 @@ std::vector<std::size_t> Qbasis::sort()
     }
     data = std::move(nd);
     sorted = true;
+    tree_index.reset();
     return idx;
 }
 
@@ -233,15 +239,42 @@ Qbasis Qbasis::forget_history() const
     return out;
 }
 
+std::uint64_t FTree::hash() const
+{
+    // hashes exactly the fields that same() compares
+    auto mix = [](std::uint64_t h, std::uint64_t v) {
+        h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        return h;
+    };
+    auto qv = [](const Q& q) { return (static_cast<std::uint64_t>(static_cast<std::uint32_t>(q.v[0])) << 32) | static_cast<std::uint32_t>(q.v[1]); };
+    std::uint64_t h = mix(static_cast<std::uint64_t>(rank), qv(coupled));
+    for(int i = 0; i < rank; ++i) h = mix(mix(h, qv(unc[i])), dims[i]);
+    for(int i = 0; i < n_inter(); ++i) h = mix(h, qv(inter[i]));
+    return h;
+}
+
 std::size_t Qbasis::left_offset(const FTree& t) const
 {
+    // Qbasis::leftOffset (Qbasis.cpp:208-225): sum of the dims of the trees before t in its
+    // sector, found through a hash index instead of a linear scan
     int i = find(t.coupled);
     if(i < 0) throw std::runtime_error("Qbasis::leftOffset: coupled sector not present");
-    std::size_t out = 0;
-    for(const auto& x : data[i].trees) {
-        if(x.same(t)) return out;
-        out += x.dim;
+    if(!tree_index) {
+        auto idx = std::make_shared<TreeIndex>(data.size());
+        for(std::size_t s = 0; s < data.size(); ++s) {
+            std::size_t off = 0;
+            (*idx)[s].reserve(data[s].trees.size());
+            for(std::size_t k = 0; k < data[s].trees.size(); ++k) {
+                (*idx)[s].emplace(data[s].trees[k].hash(), std::make_pair(k, off));
+                off += data[s].trees[k].dim;
+            }
+        }
+        tree_index = idx;
     }
+    const auto& m = (*tree_index)[i];
+    auto range = m.equal_range(t.hash());
+    for(auto it = range.first; it != range.second; ++it)
+        if(data[i].trees[it->second.first].same(t)) return it->second.second;
     throw std::runtime_error("Qbasis::leftOffset: tree not present");
 }
 
@@ -559,14 +592,19 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
     const TensorStructure& out = *plan.out;
 
     const Permutation Pfull(p);
-    // enumerate every output subblock so each output element is written exactly once
-    std::vector<std::map<std::pair<std::size_t, std::size_t>, int>> lookup(out.sector.size());
+    // enumerate every output subblock so each output element is written exactly once; subblock
+    // (row offset, column offset) -> task, per output sector
+    std::vector<std::unordered_map<std::uint64_t, int>> lookup(out.sector.size());
+    auto sub_key = [](std::size_t ro, std::size_t co) { return (static_cast<std::uint64_t>(ro) << 32) | static_cast<std::uint64_t>(co); };
     for(std::size_t s = 0; s < out.sector.size(); ++s) {
         const Q& q = out.sector[s];
+        const auto& dtrees = out.dom.tree(q);
+        const auto& ctrees = out.cod.tree(q);
+        lookup[s].reserve(dtrees.size() * ctrees.size());
         std::size_t roff = 0;
-        for(const auto& t1 : out.dom.tree(q)) {
+        for(const auto& t1 : dtrees) {
             std::size_t coff = 0;
-            for(const auto& t2 : out.cod.tree(q)) {
+            for(const auto& t2 : ctrees) {
                 PermuteTask task;
                 task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
                 task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
@@ -586,7 +624,7 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
                     }
                     task.sstride[k] = st;
                 }
-                lookup[s][{roff, coff}] = static_cast<int>(plan.tasks.size());
+                lookup[s].emplace(sub_key(roff, coff), static_cast<int>(plan.tasks.size()));
                 plan.tasks.push_back(task);
                 plan.moved_elems += t1.dim * t2.dim;
                 coff += t2.dim;
@@ -595,7 +633,6 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
         }
     }
 
-    std::vector<std::vector<PermuteSource>> contrib(plan.tasks.size());
     std::vector<int> pdv(p.begin(), p.begin() + R), pcv;
     for(int i = R; i < N; ++i) pcv.push_back(p[i] - R);
     Permutation P(p);
@@ -604,22 +641,43 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
         Pd = std::make_unique<Permutation>(pdv);
         Pc = std::make_unique<Permutation>(pcv);
     }
+    // the source subblocks, in the reference's order (input sector, then row tree, then column
+    // tree), are cut into contiguous ranges that host threads map to their tasks; the ranges are
+    // merged in order, so every task keeps its sources in that order
+    struct Work {
+        std::size_t sector, row_tree;
+        std::size_t roff;
+    };
+    std::vector<Work> work;
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
-        const Q& q = in.sector[i];
         std::size_t roff_in = 0;
-        for(const auto& t1 : in.dom.tree(q)) {
+        const auto& dtrees = in.dom.tree(in.sector[i]);
+        for(std::size_t a = 0; a < dtrees.size(); ++a) {
+            work.push_back(Work{i, a, roff_in});
+            roff_in += dtrees[a].dim;
+        }
+    }
+    // build the lazily created tree indices before threads share the bases
+    for(const Qbasis* b : {&out.dom, &out.cod})
+        if(!b->data.empty() && !b->data[0].trees.empty()) (void)b->left_offset(b->data[0].trees[0]);
+    using Contrib = std::vector<std::pair<int, PermuteSource>>;
+    auto run_range = [&](std::size_t w0, std::size_t w1, Contrib& res) {
+        for(std::size_t w = w0; w < w1; ++w) {
+            const std::size_t i = work[w].sector;
+            const Q& q = in.sector[i];
+            const FTree& t1 = in.dom.tree(q)[work[w].row_tree];
             std::size_t coff_in = 0;
             for(const auto& t2 : in.cod.tree(q)) {
-                const std::int64_t src_off = static_cast<std::int64_t>(in.offsets[i] + roff_in + coff_in * in.rows[i]);
+                const std::int64_t src_off = static_cast<std::int64_t>(in.offsets[i] + work[w].roff + coff_in * in.rows[i]);
                 const std::int64_t src_ld = static_cast<std::int64_t>(in.rows[i]);
                 auto add = [&](const FTree& a, const FTree& b, double c) {
                     if(std::abs(c) < PERMUTE_SKIP) return;
                     int so = out.find_sector(a.coupled);
                     if(so < 0) throw std::logic_error("permute: output sector missing");
                     std::size_t ro = out.dom.left_offset(a), co = out.cod.left_offset(b);
-                    auto it = lookup[so].find({ro, co});
+                    auto it = lookup[so].find(sub_key(ro, co));
                     if(it == lookup[so].end()) throw std::logic_error("permute: output subblock missing");
-                    contrib[it->second].push_back(PermuteSource{src_off, src_ld, c});
+                    res.emplace_back(it->second, PermuteSource{src_off, src_ld, c});
                 };
                 if(split) {
                     TreeMap pd = tree_permute(in.sym, t1, *Pd);
@@ -631,14 +689,42 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
                 }
                 coff_in += t2.dim;
             }
-            roff_in += t1.dim;
         }
+    };
+    const std::size_t nw = work.size();
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const std::size_t nthreads = plan.tasks.size() < 4096 ? 1 : std::min<std::size_t>(hw, std::max<std::size_t>(1, nw / 4));
+    std::vector<Contrib> parts(nthreads);
+    if(nthreads == 1) {
+        run_range(0, nw, parts[0]);
+    } else {
+        std::vector<std::thread> th;
+        std::vector<std::exception_ptr> errs(nthreads);
+        for(std::size_t k = 0; k < nthreads; ++k)
+            th.emplace_back([&, k] {
+                try {
+                    run_range(nw * k / nthreads, nw * (k + 1) / nthreads, parts[k]);
+                } catch(...) {
+                    errs[k] = std::current_exception();
+                }
+            });
+        for(auto& t : th) t.join();
+        for(auto& e : errs)
+            if(e) std::rethrow_exception(e);
     }
+    std::vector<int> count(plan.tasks.size(), 0);
+    for(const auto& part : parts)
+        for(const auto& [t, src] : part) ++count[t];
+    std::size_t nsrc = 0;
     for(std::size_t t = 0; t < plan.tasks.size(); ++t) {
-        plan.tasks[t].src_begin = static_cast<int>(plan.sources.size());
-        plan.tasks[t].nsrc = static_cast<int>(contrib[t].size());
-        plan.sources.insert(plan.sources.end(), contrib[t].begin(), contrib[t].end());
+        plan.tasks[t].src_begin = static_cast<int>(nsrc);
+        plan.tasks[t].nsrc = count[t];
+        nsrc += static_cast<std::size_t>(count[t]);
     }
+    plan.sources.resize(nsrc);
+    std::vector<int> fill(plan.tasks.size(), 0);
+    for(const auto& part : parts)
+        for(const auto& [t, src] : part) plan.sources[static_cast<std::size_t>(plan.tasks[t].src_begin + fill[t]++)] = src;
     return plan;
 }
 

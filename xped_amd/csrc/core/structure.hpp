@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -64,6 +65,7 @@ struct FTree {
         }
         return coupled < o.coupled;
     }
+    std::uint64_t hash() const;               // consistent with same()
     FTree enlarge(const FTree& other) const;  // FusionTree.cpp:309-324
     void compute_intermediates(const Sym& s);  // FusionTree.cpp:136-152 (abelian only)
 };
@@ -108,6 +110,10 @@ struct Qbasis {
     bool sorted = false;
     std::size_t curr_dim = 0;
     std::vector<QEntry> data;
+    // per sector: tree hash -> (tree index, left offset); built on the first left_offset call and
+    // dropped by every mutation (push_back, push_back_trees, sort)
+    using TreeIndex = std::vector<std::unordered_multimap<std::uint64_t, std::pair<std::size_t, std::size_t>>>;
+    mutable std::shared_ptr<const TreeIndex> tree_index;
 
     Qbasis() = default;
     Qbasis(const Sym& s, int d) : sym(s), depth(d) {}

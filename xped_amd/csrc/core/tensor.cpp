@@ -2,6 +2,7 @@
 #include "tensor.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <sstream>
@@ -142,6 +143,7 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     for(int x : p) put_i(k, x);
     auto it = rt->permute_cache.find(k);
     if(it != rt->permute_cache.end()) return it->second;
+    const auto t0 = std::chrono::steady_clock::now();
     auto cp = std::make_shared<CachedPermute>();
     cp->plan = make_permute_plan(in, shift, p);
     // every access of the plan must stay inside the input and output buffers
@@ -196,6 +198,8 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
         dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
     }
     rt->permute_cache.emplace(k, cp);
+    rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->n_permute_plans++;
     return cp;
 }
 
@@ -226,6 +230,7 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
     k += structure_key(r);
     auto it = rt->gemm_cache.find(k);
     if(it != rt->gemm_cache.end()) return it->second;
+    const auto t0 = std::chrono::steady_clock::now();
     auto cg = std::make_shared<CachedGemm>();
     cg->plan = make_gemm_plan(l, r);
     for(const auto& g : cg->plan.items) {
@@ -260,6 +265,8 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
         dcheck(rt->ctx, xh_gemm_plan_create(rt->ctx, items.data(), static_cast<int64_t>(items.size()), 0, 0, &cg->dev), "gemm plan");
     }
     rt->gemm_cache.emplace(k, cg);
+    rt->gemm_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->n_gemm_plans++;
     return cg;
 }
 
@@ -272,9 +279,19 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision)
             // sectors without a matching pair stay zero (Tout.setZero(), TensorBase.cpp:256-258)
             for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
         }
-        dcheck(l.rt->ctx, xh_gemm(l.rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision), "grouped gemm");
-        l.rt->gemm_flops += cg->plan.flops;
-        l.rt->n_gemm++;
+        Runtime* rt = l.rt;
+        if(rt->time_gemm) dcheck(rt->ctx, xh_event_record(rt->ctx, 14), "event");
+        dcheck(rt->ctx, xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision), "grouped gemm");
+        if(rt->time_gemm) {
+            float ms = 0.f;
+            dcheck(rt->ctx, xh_event_record(rt->ctx, 15), "event");
+            dcheck(rt->ctx, xh_event_elapsed_ms(rt->ctx, 14, 15, &ms), "event time");
+            rt->gemm_kernel_ms += ms;
+            rt->gemm_timed_flops += cg->plan.flops;
+            for(const auto& g : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(g.m), static_cast<int32_t>(g.n), static_cast<int32_t>(g.k)});
+        }
+        rt->gemm_flops += cg->plan.flops;
+        rt->n_gemm++;
     }
     return out;
 }

@@ -1,6 +1,7 @@
 // Device-resident block-sparse tensor and the operations of the reference's Xped::Tensor
 // (include/Xped/Core/Tensor.hpp:59-444, TensorBase.hpp:47-135) on top of the HIP device layer.
 #pragma once
+#include <array>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -61,6 +62,14 @@ struct Runtime {
     // counters for reporting
     double gemm_flops = 0.0, permute_bytes = 0.0;
     long n_gemm = 0, n_permute = 0;
+    // host time spent building plans (seconds) and the number built
+    double permute_plan_s = 0.0, gemm_plan_s = 0.0;
+    long n_permute_plans = 0, n_gemm_plans = 0;
+    // optional per-launch timing of the contraction GEMMs (operator*) with HIP events, and a log
+    // of their block shapes (m, n, k); off by default (it synchronises after every launch)
+    bool time_gemm = false;
+    double gemm_kernel_ms = 0.0, gemm_timed_flops = 0.0;
+    std::vector<std::array<int32_t, 3>> gemm_log;
     ~Runtime();
     void require_device() const;
 };

@@ -53,6 +53,7 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     if(!ctx) return XH_OK;
     hipSetDevice(ctx->device);
     if(ctx->stream) hipStreamSynchronize(ctx->stream);
+    xh::collect_staging(ctx, true);
     for(auto& ev : ctx->events)
         if(ev) hipEventDestroy(ev);
     if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
@@ -109,7 +110,10 @@ extern "C" int xh_copy(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
 
 extern "C" int xh_synchronize(xh_ctx* ctx)
 {
-    return guard(ctx, [&] { check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+    return guard(ctx, [&] {
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::collect_staging(ctx, true);
+    });
 }
 
 extern "C" void* xh_stream(xh_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
@@ -406,11 +410,12 @@ extern "C" int xh_scale_rects(xh_ctx* ctx, double* x, const xh_rect* rects, int6
     return guard(ctx, [&] {
         if(nrects <= 0) return;
         std::vector<xh_rect> r(rects, rects + nrects);
-        xh_rect* d = xh::dev_upload(ctx, r);
-        hipLaunchKernelGGL(k_scale_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, x, d);
+        xh::PackedUpload up;
+        const size_t o = up.add(r);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_scale_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, x, up.ptr<const xh_rect>(o));
         xh::launch_check("k_scale_rects");
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        xh::dev_free(ctx, d);
+        up.release_deferred(ctx);
     });
 }
 
@@ -421,13 +426,13 @@ extern "C" int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t*
         if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "diag map mode must be 0, 1 or 2");
         std::vector<int64_t> o(off, off + nblocks);
         std::vector<int32_t> mm(m, m + nblocks);
-        int64_t* d_off = xh::dev_upload(ctx, o);
-        int32_t* d_m = xh::dev_upload(ctx, mm);
-        hipLaunchKernelGGL(k_diag_map, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, d_off, d_m, x, mode);
+        xh::PackedUpload up;
+        const size_t oo = up.add(o), om = up.add(mm);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_diag_map, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, up.ptr<const int64_t>(oo),
+                           up.ptr<const int32_t>(om), x, mode);
         xh::launch_check("k_diag_map");
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        xh::dev_free(ctx, d_off);
-        xh::dev_free(ctx, d_m);
+        up.release_deferred(ctx);
     });
 }
 
@@ -436,11 +441,12 @@ extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const 
     return guard(ctx, [&] {
         if(nrects <= 0) return;
         std::vector<xh_copy_rect> r(rects, rects + nrects);
-        xh_copy_rect* d = xh::dev_upload(ctx, r);
-        hipLaunchKernelGGL(k_copy_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, d);
+        xh::PackedUpload up;
+        const size_t o = up.add(r);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_copy_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, up.ptr<const xh_copy_rect>(o));
         xh::launch_check("k_copy_rects");
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        xh::dev_free(ctx, d);
+        up.release_deferred(ctx);
     });
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -20,6 +21,8 @@ struct xh_ctx {
     double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
+    // host staging buffers of asynchronous uploads, kept until the event after their copy fires
+    std::vector<std::pair<hipEvent_t, std::vector<char>>> pending_host;
 };
 
 // FP64 SVD of large sector blocks by one-sided block Jacobi (bjsvd.hip); same item layout as
@@ -94,6 +97,90 @@ T* dev_upload(xh_ctx* ctx, const std::vector<T>& v)
 inline void dev_free(xh_ctx* ctx, void* p)
 {
     if(p) hipFreeAsync(p, ctx->stream);
+}
+
+// Host arrays packed into one device allocation and sent with one asynchronous copy on the
+// context stream, without a host synchronisation.  The host copy is kept until release(), which
+// first waits for the event recorded after the copy.
+struct PackedUpload {
+    std::vector<char> host;
+    void* dev = nullptr;
+    hipEvent_t done = nullptr;
+    xh_ctx* owner = nullptr;
+    PackedUpload() = default;
+    PackedUpload(const PackedUpload&) = delete;
+    PackedUpload& operator=(const PackedUpload&) = delete;
+    ~PackedUpload()
+    {
+        if(owner && (dev || done)) release(owner);  // error paths: wait and free
+    }
+    template <typename T>
+    size_t add(const std::vector<T>& v)
+    {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + v.size() * sizeof(T));
+        if(!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+    void upload(xh_ctx* ctx)
+    {
+        if(host.empty()) return;
+        owner = ctx;
+        check(hipMallocAsync(&dev, host.size(), ctx->stream), "hipMallocAsync(plan)");
+        check(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
+        check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventRecord(done, ctx->stream), "hipEventRecord");
+    }
+    template <typename T>
+    T* ptr(size_t off) const
+    {
+        return dev ? reinterpret_cast<T*>(static_cast<char*>(dev) + off) : nullptr;
+    }
+    void release(xh_ctx* ctx)
+    {
+        if(done) {
+            hipEventSynchronize(done);
+            hipEventDestroy(done);
+            done = nullptr;
+        }
+        if(dev) hipFreeAsync(dev, ctx->stream);
+        dev = nullptr;
+        host.clear();
+    }
+    // after the kernels that read the device copy have been enqueued: the device memory is
+    // freed in stream order and the host copy handed to the context until its copy has finished
+    void release_deferred(xh_ctx* ctx);
+};
+
+// drop the staging buffers whose copies have completed (wait = true: all of them, after a sync)
+inline void collect_staging(xh_ctx* ctx, bool wait)
+{
+    auto& v = ctx->pending_host;
+    std::size_t keep = 0;
+    for(std::size_t i = 0; i < v.size(); ++i) {
+        bool ready = true;
+        if(wait)
+            hipEventSynchronize(v[i].first);
+        else
+            ready = hipEventQuery(v[i].first) == hipSuccess;
+        if(ready) {
+            hipEventDestroy(v[i].first);
+        } else {
+            if(keep != i) v[keep] = std::move(v[i]);
+            ++keep;
+        }
+    }
+    v.resize(keep);
+}
+
+inline void PackedUpload::release_deferred(xh_ctx* ctx)
+{
+    if(dev) hipFreeAsync(dev, ctx->stream);
+    dev = nullptr;
+    if(done) ctx->pending_host.emplace_back(done, std::move(host));
+    done = nullptr;
+    host.clear();
+    if(ctx->pending_host.size() > 64) collect_staging(ctx, false);
 }
 
 }  // namespace xh

@@ -628,23 +628,49 @@ extern "C" int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t n
 
 extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* G, double* X, const double* shift_rel)
 {
+    // asynchronous: no status to read back, so no host synchronisation
     return guard(ctx, [&] {
-        std::vector<xh_factor_item> v(items, items + nitems);
-        std::vector<double> sh(std::max<int64_t>(nitems, 1), 0.0);
+        const size_t budget = lds_budget(ctx);
+        std::vector<xh_factor_item> small, big;
+        std::vector<double> sh;
+        int64_t small_max = 0, big_max = 0;
         for(int64_t i = 0; i < nitems; ++i) {
-            if(v[i].m != v[i].n) throw xh::Error(XH_EINVAL, "chol_inv: blocks must be square");
-            v[i].s_off = i;
-            if(shift_rel) sh[i] = shift_rel[i];
+            xh_factor_item it = items[i];
+            if(it.m != it.n) throw xh::Error(XH_EINVAL, "chol_inv: blocks must be square");
+            if(it.n <= 0) continue;
+            it.s_off = static_cast<int64_t>(sh.size());
+            sh.push_back(shift_rel ? shift_rel[i] : 0.0);
+            const int64_t need = int64_t(it.n) * it.n + it.n;
+            if(size_t(need) * sizeof(double) <= budget) {
+                small.push_back(it);
+                small_max = std::max(small_max, need);
+            } else {
+                big.push_back(it);
+                big_max = std::max(big_max, need);
+            }
         }
-        double* d_sh = xh::dev_upload(ctx, sh);
-        try {
-            run_factor(
-                ctx, k_chol_inv, v, [](const xh_factor_item& it) { return int64_t(it.n) * it.n + it.n; }, G, X, d_sh, nullptr);
-        } catch(...) {
-            xh::dev_free(ctx, d_sh);
-            throw;
+        if(sh.empty()) return;
+        xh::PackedUpload up;
+        const size_t o_small = up.add(small), o_big = up.add(big), o_sh = up.add(sh);
+        up.upload(ctx);
+        double* d_sh = up.ptr<double>(o_sh);
+        if(!small.empty()) {
+            const size_t shmem = size_t(small_max) * sizeof(double);
+            if(shmem > 64 * 1024)
+                check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
+            FactorArgs fa{up.ptr<xh_factor_item>(o_small), G, X, d_sh, nullptr, nullptr, 0, 1, nullptr};
+            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(small.size())), dim3(F_THREADS), shmem, ctx->stream, fa);
+            xh::launch_check("k_chol_inv");
         }
-        xh::dev_free(ctx, d_sh);
+        if(!big.empty()) {
+            double* scratch = nullptr;
+            check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(big_max) * big.size() * sizeof(double), ctx->stream), "hipMallocAsync");
+            FactorArgs fa{up.ptr<xh_factor_item>(o_big), G, X, d_sh, nullptr, scratch, big_max, 0, nullptr};
+            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(big.size())), dim3(F_THREADS), 0, ctx->stream, fa);
+            xh::launch_check("k_chol_inv");
+            xh::dev_free(ctx, scratch);
+        }
+        up.release_deferred(ctx);
     });
 }
 

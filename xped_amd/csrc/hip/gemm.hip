@@ -221,6 +221,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
 }  // namespace
 
 struct xh_gemm_plan {
+    xh::PackedUpload up;
     xh_gemm_item* d_items = nullptr;
     Tile* d_tiles = nullptr;
     int32_t ntiles = 0;
@@ -251,9 +252,10 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         plan->ntiles = static_cast<int32_t>(tiles.size());
         plan->ta = trans_a ? 1 : 0;
         plan->tb = trans_b ? 1 : 0;
-        plan->d_items = xh::dev_upload(ctx, v);
-        plan->d_tiles = xh::dev_upload(ctx, tiles);
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        const size_t oi = plan->up.add(v), ot = plan->up.add(tiles);
+        plan->up.upload(ctx);
+        plan->d_items = plan->up.ptr<xh_gemm_item>(oi);
+        plan->d_tiles = plan->up.ptr<Tile>(ot);
     });
     if(rc != XH_OK) {
         xh_gemm_plan_destroy(ctx, plan);
@@ -267,8 +269,7 @@ extern "C" int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan)
 {
     if(!plan) return XH_OK;
     return guard(ctx, [&] {
-        xh::dev_free(ctx, plan->d_items);
-        xh::dev_free(ctx, plan->d_tiles);
+        plan->up.release(ctx);
         delete plan;
     });
 }

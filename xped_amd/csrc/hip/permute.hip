@@ -207,13 +207,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_transpose(PermDev p, con
 
 struct xh_permute_plan {
     PermDev dev{};
-    void* mem_meta = nullptr;
-    void* mem_legs = nullptr;
-    void* mem_prefix = nullptr;
-    void* mem_tiles = nullptr;
-    void* mem_src = nullptr;
-    void* mem_tr = nullptr;
-    void* mem_trinfo = nullptr;
+    xh::PackedUpload up;
     int64_t ngather_tiles = 0;
     double bytes = 0.0;
     const TrTaskInfo* trinfo = nullptr;
@@ -323,24 +317,19 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         plan->dev.new_rank = desc->new_rank;
         plan->dev.ntasks = static_cast<int64_t>(allmeta.size());
         plan->dev.total = total;
-        plan->mem_meta = xh::dev_upload(ctx, allmeta);
-        plan->mem_legs = xh::dev_upload(ctx, alllegs);
-        plan->mem_prefix = xh::dev_upload(ctx, prefix);
-        plan->mem_tiles = xh::dev_upload(ctx, tile_first);
-        plan->mem_src = xh::dev_upload(ctx, srcs);
-        plan->mem_tr = xh::dev_upload(ctx, trtiles);
-        plan->mem_trinfo = xh::dev_upload(ctx, trinfo);
-        plan->dev.meta = static_cast<const TaskMeta*>(plan->mem_meta);
-        plan->dev.legs = static_cast<const LegInfo*>(plan->mem_legs);
-        plan->dev.prefix = static_cast<const int64_t*>(plan->mem_prefix);
-        plan->dev.tile_first = static_cast<const int32_t*>(plan->mem_tiles);
-        plan->dev.src = static_cast<const xh_permute_source*>(plan->mem_src);
+        const size_t o_meta = plan->up.add(allmeta), o_legs = plan->up.add(alllegs), o_prefix = plan->up.add(prefix),
+                     o_tiles = plan->up.add(tile_first), o_src = plan->up.add(srcs), o_tr = plan->up.add(trtiles), o_trinfo = plan->up.add(trinfo);
+        plan->up.upload(ctx);
+        plan->dev.meta = plan->up.ptr<const TaskMeta>(o_meta);
+        plan->dev.legs = plan->up.ptr<const LegInfo>(o_legs);
+        plan->dev.prefix = plan->up.ptr<const int64_t>(o_prefix);
+        plan->dev.tile_first = plan->up.ptr<const int32_t>(o_tiles);
+        plan->dev.src = plan->up.ptr<const xh_permute_source>(o_src);
         plan->dev.ntr_tiles = static_cast<int64_t>(trtiles.size());
-        plan->dev.tr_tiles = static_cast<const int4*>(plan->mem_tr);
-        plan->trinfo = static_cast<const TrTaskInfo*>(plan->mem_trinfo);
+        plan->dev.tr_tiles = plan->up.ptr<const int4>(o_tr);
+        plan->trinfo = plan->up.ptr<const TrTaskInfo>(o_trinfo);
         plan->ngather_tiles = ntiles;
         plan->bytes = 8.0 * (moved + read);
-        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
     if(rc != XH_OK) {
         xh_permute_plan_destroy(ctx, plan);
@@ -354,8 +343,7 @@ extern "C" int xh_permute_plan_destroy(xh_ctx* ctx, xh_permute_plan* plan)
 {
     if(!plan) return XH_OK;
     return guard(ctx, [&] {
-        for(void* p : {plan->mem_meta, plan->mem_legs, plan->mem_prefix, plan->mem_tiles, plan->mem_src, plan->mem_tr, plan->mem_trinfo})
-            xh::dev_free(ctx, p);
+        plan->up.release(ctx);
         delete plan;
     });
 }

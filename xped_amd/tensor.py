@@ -60,6 +60,26 @@ class Runtime:
         lib().xt_rt_counters(self.h, C.byref(f), C.byref(b), C.byref(ng), C.byref(npm))
         return {"gemm_flops": f.value, "permute_bytes": b.value, "n_gemm": ng.value, "n_permute": npm.value}
 
+    def host_stats(self):
+        a, b = C.c_double(), C.c_double()
+        na, nb = C.c_long(), C.c_long()
+        lib().xt_rt_host_stats(self.h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb))
+        return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value}
+
+    def gemm_timing(self, enable: bool, reset: bool = False):
+        """Per-launch HIP-event timing of the contraction GEMMs; returns (kernel ms, flops, logged blocks)."""
+        ms, fl, n = C.c_double(), C.c_double(), C.c_long()
+        lib().xt_rt_gemm_timing(self.h, int(enable), int(reset), C.byref(ms), C.byref(fl), C.byref(n))
+        return ms.value, fl.value, n.value
+
+    def gemm_log(self) -> np.ndarray:
+        """Block shapes (m, n, k) of the contraction GEMMs logged while timing was on."""
+        ms, fl, nn = C.c_double(), C.c_double(), C.c_long()
+        lib().xt_rt_gemm_timing(self.h, -1, 0, C.byref(ms), C.byref(fl), C.byref(nn))
+        out = np.zeros((max(nn.value, 1), 3), np.int32)
+        lib().xt_rt_gemm_log(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), nn.value)
+        return out[:nn.value]
+
     def synchronize(self):
         if self.has_device:
             lib().xh_synchronize(self.ctx())
