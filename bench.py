@@ -6,7 +6,8 @@ projection) of a U(1) iPEPS on a 2x2 unit cell, virtual D = 8 over charges -3..3
 Heisenberg energy.  Two modes run on fresh runtimes (no shared plan caches):
   fp64 - plain FP64 GPU path: FP64 grouped GEMMs, full FP64 block SVDs (one-sided block Jacobi)
   fast - the accelerated path: FP64 grouped GEMMs, randomized block SVDs of the projector
-         products with target ranks from the previous step (full FP64 SVD as fallback)
+         products with per-sector target ranks from the learned predictor (or, with
+         --ranks previous, the previous step's kept counts); full FP64 SVD as fallback
 Both start from the same warm-up (init from the double layer + one full-SVD grow_all reaching
 chi = 256).  `value` is the contraction throughput of the fast step: the summed algorithmic flops
 of its grouped contraction GEMMs over their HIP-event-timed kernel time; `ms_per_step` is the
@@ -47,6 +48,8 @@ def parse():
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--power-iters", type=int, default=6)
     ap.add_argument("--oversample", type=int, default=16)
+    ap.add_argument("--ranks", default="learned", choices=["learned", "previous"],
+                    help="target ranks of the randomized SVDs: the trained predictor or the previous step's kept counts")
     ap.add_argument("--leg-dim", type=int, default=192, help="c3 leg dimension")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the fast mode / c3")
     ap.add_argument("--seed", type=int, default=1)
@@ -72,6 +75,11 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     log(f"[bench] {mode}: warm-up grow_all {time.perf_counter() - t0:.1f} s")
     if mode == "fast":
         c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters)
+        if args.ranks == "learned":
+            from xped_amd import rank_predictor
+
+            params, width = rank_predictor.load_weights()
+            c.set_predictor(params, width)
     nsteps = args.steps if mode == "fast" else args.fp64_steps
     nwarm = args.warmup if mode == "fast" else 0
     energies = []
@@ -157,12 +165,13 @@ def run_c4(args):
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[args.precision],
         "data": f"synthetic (site tensors N(0,1)*{SCALE:g}/sqrt(plain size), std::mt19937(seed + site))",
         "config": {"workload": f"U(1) iPEPS CTMRG step, 2x2 cell, D=8, chi={args.chi}, Heisenberg", "chi": args.chi, "D": 8,
-                   "svd": f"randomized, l = 2 k + {args.oversample} for the previous step's kept k, {args.power_iters} power iterations, FP64 fallback",
+                   "svd": f"randomized, l = 2 k + {args.oversample}, k from the {'learned predictor' if args.ranks == 'learned' else 'previous step'}, "
+                          f"{args.power_iters} power iterations, FP64 fallback",
                    "gemm_precision": args.precision, "seed": args.seed},
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
-                 "host_plan_s": round(fast["host"]["permute_plan_s"] + fast["host"]["gemm_plan_s"], 3)},
+                 "host_plan_s": round(fast["host"]["permute_plan_s"] + fast["host"]["gemm_plan_s"], 3), "host": fast["host"]},
         "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5), "traffic": None,
                      "kernel": f"k_gemm_{'f64' if args.precision == 'fp64' else args.precision}",
                      "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},

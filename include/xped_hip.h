@@ -148,6 +148,22 @@ typedef struct {
 } xh_copy_rect;
 int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects);
 
+/* ---- learned rank predictor: a small FP32 network nin -> width -> width -> 1 (ReLU), MSE loss,
+ * Adam; hand-written forward/backward kernels.  No counterpart in the reference (its SVD is always
+ * the full one); used to choose the per-sector ranks of the randomized block SVD.  X is B x nin
+ * row-major, Y, T and grad are device FP32 arrays; parameters: W1 (nin x width), b1, W2
+ * (width x width), b2, w3 (width), b3. */
+typedef struct xh_mlp xh_mlp;
+int xh_mlp_create(xh_ctx* ctx, int nin, int width, uint64_t seed, xh_mlp** out);
+int xh_mlp_destroy(xh_ctx* ctx, xh_mlp* net);
+int xh_mlp_num_params(const xh_mlp* net, int64_t* n);
+int xh_mlp_get_params(xh_ctx* ctx, const xh_mlp* net, float* host);
+int xh_mlp_set_params(xh_ctx* ctx, xh_mlp* net, const float* host); /* also resets the Adam moments */
+int xh_mlp_forward(xh_ctx* ctx, xh_mlp* net, const float* X, int64_t B, float* Y);
+/* gradient of mean((y - T)^2) over the batch into grad (device, num_params floats); *loss (host) if non-null */
+int xh_mlp_grad(xh_ctx* ctx, xh_mlp* net, const float* X, const float* T, int64_t B, float* grad, float* loss);
+int xh_mlp_adam_step(xh_ctx* ctx, xh_mlp* net, const float* grad, float lr, int64_t t);
+
 /* ---- timing helpers (events on the context stream) --------------------------------------- */
 int xh_event_record(xh_ctx* ctx, int slot);
 int xh_event_elapsed_ms(xh_ctx* ctx, int slot0, int slot1, float* ms);

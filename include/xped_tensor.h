@@ -41,6 +41,9 @@ int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, l
 int xt_rt_clear_caches(xt_rt* rt);
 /* host time (seconds) spent building permute and GEMM plans, and how many were built */
 int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans);
+/* breakdown of that host time (seconds): [0] permute plans on the host, [1] their checks and
+ * packing, [2] device permute plans, [3] GEMM plans on the host, [4] device GEMM plans */
+int xt_rt_host_timers(const xt_rt* rt, double* t, int n);
 /* contraction GEMM timing: enable = 1 times every operator* launch with HIP events (and logs its
  * block shapes), 0 stops, -1 leaves the setting; returns the totals since the last reset
  * (reset = 1 clears them first) */
@@ -154,6 +157,14 @@ int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out);
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms);
 /* randomized SVD parameters of the CTM decompositions (see xt_tsvd_randomized) */
 int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_iters, int sketch_precision);
+/* learned target ranks for the randomized SVDs: network parameters (FP32, the xh_mlp layout with
+ * nin = 19 features), hidden width and the rank margin; nparams = 0 removes the predictor.  Without
+ * a predictor the ranks are the previous step's kept counts.  No counterpart in the reference. */
+int xt_ctm_set_predictor(xt_ctm* c, const float* params, int64_t nparams, int width, double margin);
+/* the predictor's 19 input features of one sector (see the network description in the sources):
+ * the sector's leading singular values top[0..ntop), min(m, n), the largest sketched value of the
+ * tensor and the candidate cut in decades below it */
+int xt_rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* out);
 /* counters: randomized SVDs run, full-SVD fallbacks taken, largest residual ratio seen */
 int xt_ctm_rsvd_stats(const xt_ctm* c, long* n_rsvd, long* n_fallback, double* max_residual_ratio);
 /* recorded full spectra: number of records, then per record the sector count and flat arrays */

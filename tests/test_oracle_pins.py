@@ -372,3 +372,40 @@ def test_oracle_tsvd_reconstructs():
     U, S, V, allSV = tensor.full_svd(t)
     rec = tensor.mul(tensor.mul(U, S), V)
     np.testing.assert_allclose(rec.data, t.data, atol=1e-13)
+
+
+# ---------------------------------------------------------------- rank-predictor network oracle
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp as om
+
+    r = np.random.default_rng(4)
+    nin, w = 5, 7
+    n = om.layout(nin, w)["b3"][1]
+    p = r.standard_normal(n) * 0.5
+    X = r.standard_normal((11, nin))
+    T = r.standard_normal(11)
+    _, g = om.loss_and_grad(p, X, T, nin, w)
+    h = 1e-6
+    fd = np.array([(om.loss_and_grad(p + h * e, X, T, nin, w)[0] - om.loss_and_grad(p - h * e, X, T, nin, w)[0]) / (2 * h) for e in np.eye(n)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-8)
+
+
+def test_mlp_oracle_forward_by_hand():
+    from oracle import mlp as om
+
+    # nin = 1, width = 2: h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), y = h2 . w3 + b3
+    W1, b1 = np.array([[1.0, -2.0]]), np.array([0.5, 0.25])
+    W2, b2 = np.array([[1.0, 0.0], [3.0, -1.0]]), np.array([0.0, 1.0])
+    w3, b3 = np.array([2.0, -1.0]), 0.125
+    p = np.concatenate([W1.ravel(), b1, W2.ravel(), b2, w3, [b3]])
+    # x = 1: pre1 = (1.5, -1.75) -> h1 = (1.5, 0); pre2 = (1.5, 1.0) -> h2 = (1.5, 1.0); y = 3 - 1 + 0.125
+    y, _ = om.forward(p, np.array([[1.0]]), 1, 2)
+    assert y[0] == 2.125
+
+
+def test_mlp_oracle_adam_first_step_moves_by_lr():
+    from oracle import mlp as om
+
+    g = np.array([0.3, -2.0, 1e-3])
+    p1, _, _ = om.adam_step(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, 0.01)
+    np.testing.assert_allclose(p1, -0.01 * np.sign(g), rtol=1e-4)

@@ -5,6 +5,7 @@ is missing: there is no CPU fallback for any data operation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 from pathlib import Path
@@ -91,6 +92,14 @@ _SIGS = [
     ("xh_axpby", C.c_int, [_P, _P, _P, _SZ, C.c_double, C.c_double]),
     ("xh_max_abs", C.c_int, [_P, _P, _SZ, _DP]),
     ("xh_dot", C.c_int, [_P, _P, _P, _SZ, _DP]),
+    ("xh_mlp_create", C.c_int, [_P, C.c_int, C.c_int, C.c_uint64, _PP]),
+    ("xh_mlp_destroy", C.c_int, [_P, _P]),
+    ("xh_mlp_num_params", C.c_int, [_P, C.POINTER(C.c_int64)]),
+    ("xh_mlp_get_params", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+    ("xh_mlp_set_params", C.c_int, [_P, _P, C.POINTER(C.c_float)]),
+    ("xh_mlp_forward", C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    ("xh_mlp_grad", C.c_int, [_P, _P, _P, _P, C.c_int64, _P, C.POINTER(C.c_float)]),
+    ("xh_mlp_adam_step", C.c_int, [_P, _P, _P, C.c_float, C.c_int64]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),
     ("xh_event_elapsed_ms", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     # tensor layer
@@ -101,6 +110,7 @@ _SIGS = [
     ("xt_rt_counters", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_clear_caches", C.c_int, [_P]),
     ("xt_rt_host_stats", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("xt_rt_host_timers", C.c_int, [_P, _DP, C.c_int]),
     ("xt_rt_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_long)]),
     ("xt_rt_gemm_log", C.c_int, [_P, C.POINTER(C.c_int32), C.c_long]),
     ("xt_basis_create", C.c_int, [C.c_int, C.c_int, _PP]),
@@ -161,6 +171,8 @@ _SIGS = [
     ("xt_ctm_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
     ("xt_ctm_svd_ms", C.c_int, [_P, _DP]),
     ("xt_ctm_set_rsvd", C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.c_int]),
+    ("xt_ctm_set_predictor", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64, C.c_int, C.c_double]),
+    ("xt_rank_features", C.c_int, [_DP, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_float)]),
     ("xt_ctm_rsvd_stats", C.c_int, [_P, C.POINTER(C.c_long), C.POINTER(C.c_long), _DP]),
     ("xt_ctm_num_spectra", C.c_int, [_P, _IP]),
     ("xt_ctm_spectrum", C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, C.c_int]),
@@ -170,6 +182,26 @@ _SIGS = [
 ]
 
 _lib = None
+_finalizing = False
+
+
+def _mark_finalizing():
+    global _finalizing
+    _finalizing = True
+
+
+# Objects still alive when the interpreter exits are not torn down natively: their finalizers
+# would run in an unspecified order, and destroying the device context (and its stream) while
+# stream-ordered allocations made on it are still live leaves the HIP runtime's own exit
+# handling with a dangling stream.  The process exit releases the device memory.
+atexit.register(_mark_finalizing)
+
+
+def native(name):
+    """The library function `name` for a finalizer, or None once the interpreter is exiting."""
+    if _finalizing or _lib is None:
+        return None
+    return getattr(_lib, name)
 
 
 def lib():

@@ -135,6 +135,13 @@ int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_
     return XH_OK;
 }
 
+int xt_rt_host_timers(const xt_rt* rt, double* t, int n)
+{
+    if(!rt || n < 0) return XH_EINVAL;
+    for(int i = 0; i < n && i < 8; ++i) t[i] = rt->rt.host_t[i];
+    return XH_OK;
+}
+
 int xt_rt_gemm_timing(xt_rt* rt, int enable, int reset, double* kernel_ms, double* flops, long* nlog)
 {
     if(!rt) return XH_EINVAL;
@@ -684,6 +691,25 @@ int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_ite
         o.rsvd.sketch_precision = sketch_precision;
         c->ctm->set_options(o);
     });
+}
+
+int xt_ctm_set_predictor(xt_ctm* c, const float* params, int64_t nparams, int width, double margin)
+{
+    return run(c->owner, [&] {
+        if(nparams <= 0) {
+            c->ctm->predictor.reset();
+            return;
+        }
+        c->owner->rt.require_device();
+        c->ctm->predictor = std::make_shared<xa::RankPredictor>(c->owner->rt.ctx, params, nparams, width, margin);
+    });
+}
+
+int xt_rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* out)
+{
+    if(!out || ntop < 0 || (ntop > 0 && !top)) return XH_EINVAL;
+    xa::rank_features(top, ntop, dim, s_glob, log_cut, out);
+    return XH_OK;
 }
 
 int xt_ctm_rsvd_stats(const xt_ctm* c, long* n_rsvd, long* n_fallback, double* max_residual_ratio)

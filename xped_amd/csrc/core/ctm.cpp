@@ -189,6 +189,8 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
         std::map<Q, int> ranks;
         if(opt.rank_policy)
             ranks = opt.rank_policy(prod, site, dir);
+        else if(predictor)
+            ranks = predictor->predict(prod, opt.chi, 0xfeedull + 64ull * static_cast<std::uint64_t>(n_rsvd) + 4ull * site + dir);
         else {
             auto it = last_kept.find({site, dir});
             if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");

@@ -9,6 +9,7 @@
 #pragma once
 #include <functional>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "tensor.hpp"
@@ -50,6 +51,39 @@ struct SpectrumRecord {
     std::map<Q, std::pair<int, int>> shape;   // block shape per sector
 };
 
+// Learned rank prediction for the randomized block SVD.  A cheap sketch (RANK_TOP leading
+// singular values per sector) feeds a small network that predicts, for a candidate cut c, how
+// many singular values of each sector lie above c; the cut where the predicted counts sum to chi
+// (found on two batched grids) gives the target ranks.  Features per sector (rank_features), all decades
+// scaled by 1/4:
+//   f[0]      log10(s_1 / s_glob)          (s_glob = largest sketched value of the tensor)
+//   f[1..15]  log10(s_j / s_1), j = 2..16  (-16 where the sector has fewer values)
+//   f[16]     log10(min(m, n)) / 4
+//   f[17]     log10(c / s_glob)
+//   f[18]     log10(c / s_last)            (s_last = smallest of the sector's sketched values)
+// The network output y is log(1 + count).
+constexpr int RANK_TOP = 16;
+constexpr int RANK_NFEAT = 19;
+void rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* f);
+
+struct RankPredictor {
+    xh_ctx* ctx = nullptr;
+    xh_mlp* net = nullptr;
+    int width = 64;
+    double margin = 1.25;  // ranks = ceil(margin * predicted count)
+    RankPredictor(xh_ctx* c, const float* params, int64_t nparams, int width, double margin);
+    ~RankPredictor();
+    RankPredictor(const RankPredictor&) = delete;
+    RankPredictor& operator=(const RankPredictor&) = delete;
+    // target ranks for prod at bond dimension chi (sketch + network + cut search)
+    std::map<Q, int> predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const;
+    // predicted counts above the cut 10^log_cut * s_glob for explicit sketches (row-major nsec x ntop)
+    std::vector<double> counts(const std::vector<std::vector<double>>& top, const std::vector<int>& dims, double s_glob, double log_cut) const;
+    // the same for several cuts at once (cut-major rows)
+    std::vector<double> counts_grid(const std::vector<std::vector<double>>& top, const std::vector<int>& dims, double s_glob,
+                                    const std::vector<double>& log_cuts) const;
+};
+
 class CTM {
 public:
     CTM(Runtime* rt, const std::vector<DTensor>& A, std::size_t chi);
@@ -67,6 +101,7 @@ public:
     std::vector<DTensor> C1, C2, C3, C4, T1, T2, T3, T4, rho_h, rho_v;
     std::vector<SpectrumRecord> spectra;
     std::map<std::pair<int, int>, std::map<Q, int>> last_kept;  // (site, dir) -> kept counts of the last full SVD
+    std::shared_ptr<RankPredictor> predictor;  // learned target ranks (randomized mode) when set
     double svd_ms = 0.0;  // host-timed SVD phase (diagnostics)
     long n_rsvd = 0, n_fallback = 0;
     double max_residual_ratio = 0.0;

@@ -144,8 +144,14 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     auto it = rt->permute_cache.find(k);
     if(it != rt->permute_cache.end()) return it->second;
     const auto t0 = std::chrono::steady_clock::now();
+    auto lap = [rt, t = t0](int slot) mutable {
+        const auto now = std::chrono::steady_clock::now();
+        rt->host_t[slot] += std::chrono::duration<double>(now - t).count();
+        t = now;
+    };
     auto cp = std::make_shared<CachedPermute>();
     cp->plan = make_permute_plan(in, shift, p);
+    lap(0);
     // every access of the plan must stay inside the input and output buffers
     {
         const PermutePlan& pl = cp->plan;
@@ -195,7 +201,9 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
             srcs[i] = xh_permute_source{cp->plan.sources[i].src_off, cp->plan.sources[i].src_ld, cp->plan.sources[i].coeff};
         xh_permute_desc desc{cp->plan.nlegs, cp->plan.new_rank, static_cast<int64_t>(tasks.size()), static_cast<int64_t>(srcs.size()), tasks.data(),
                              srcs.data()};
+        lap(1);
         dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
+        lap(2);
     }
     rt->permute_cache.emplace(k, cp);
     rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -231,6 +239,11 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
     auto it = rt->gemm_cache.find(k);
     if(it != rt->gemm_cache.end()) return it->second;
     const auto t0 = std::chrono::steady_clock::now();
+    auto lap = [rt, t = t0](int slot) mutable {
+        const auto now = std::chrono::steady_clock::now();
+        rt->host_t[slot] += std::chrono::duration<double>(now - t).count();
+        t = now;
+    };
     auto cg = std::make_shared<CachedGemm>();
     cg->plan = make_gemm_plan(l, r);
     for(const auto& g : cg->plan.items) {
@@ -259,11 +272,13 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
                            static_cast<int32_t>(o.cols[s]), 0.0, 0};
             cg->zero_rects.push_back(r);
         }
+    lap(3);
     if(rt->ctx) {
         std::vector<xh_gemm_item> items;
         for(const auto& g : cg->plan.items) items.push_back(xh_gemm_item{g.a_off, g.b_off, g.c_off, g.m, g.n, g.k});
         dcheck(rt->ctx, xh_gemm_plan_create(rt->ctx, items.data(), static_cast<int64_t>(items.size()), 0, 0, &cg->dev), "gemm plan");
     }
+    lap(4);
     rt->gemm_cache.emplace(k, cg);
     rt->gemm_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rt->n_gemm_plans++;

@@ -65,6 +65,9 @@ struct Runtime {
     // host time spent building plans (seconds) and the number built
     double permute_plan_s = 0.0, gemm_plan_s = 0.0;
     long n_permute_plans = 0, n_gemm_plans = 0;
+    // breakdown (seconds): 0 host permute plan, 1 its checks and packing, 2 device permute plan,
+    // 3 host GEMM plan, 4 device GEMM plan, 5 output structures of mul
+    double host_t[8] = {};
     // optional per-launch timing of the contraction GEMMs (operator*) with HIP events, and a log
     // of their block shapes (m, n, k); off by default (it synchronises after every launch)
     bool time_gemm = false;

@@ -64,7 +64,10 @@ class Runtime:
         a, b = C.c_double(), C.c_double()
         na, nb = C.c_long(), C.c_long()
         lib().xt_rt_host_stats(self.h, C.byref(a), C.byref(b), C.byref(na), C.byref(nb))
-        return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value}
+        t = (C.c_double * 8)()
+        lib().xt_rt_host_timers(self.h, t, 8)
+        return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value,
+                "breakdown_s": [round(x, 4) for x in t[:5]]}
 
     def gemm_timing(self, enable: bool, reset: bool = False):
         """Per-launch HIP-event timing of the contraction GEMMs; returns (kernel ms, flops, logged blocks)."""
@@ -89,8 +92,9 @@ class Runtime:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.xt_rt_destroy(h)
+        f = _lib.native("xt_rt_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
             self.h = None
 
 
@@ -107,8 +111,9 @@ class Qbasis:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.xt_basis_destroy(h)
+        f = _lib.native("xt_basis_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
             self.h = None
 
     def push_back(self, q, dim: int):
@@ -216,8 +221,9 @@ class Tensor:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.xt_tensor_destroy(h)
+        f = _lib.native("xt_tensor_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
             self.h = None
 
     def _wrap(self, h):
@@ -449,8 +455,9 @@ class CTM:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and _lib._lib is not None:
-            _lib._lib.xt_ctm_destroy(h)
+        f = _lib.native("xt_ctm_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
             self.h = None
 
     def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=1, record_spectra=False, check_finite=False, verbose=False):
@@ -461,6 +468,15 @@ class CTM:
 
     def set_rsvd(self, oversample=8, rank_factor=2.0, power_iters=1, sketch_precision="fp64"):
         check(lib().xt_ctm_set_rsvd(self.h, int(oversample), float(rank_factor), int(power_iters), PRECISION[sketch_precision]), self.rt.h, "set_rsvd")
+        return self
+
+    def set_predictor(self, params=None, width=64, margin=1.25):
+        """Learned target ranks for the randomized SVDs (see xped_amd.rank_predictor); None removes them."""
+        if params is None:
+            check(lib().xt_ctm_set_predictor(self.h, None, 0, int(width), float(margin)), self.rt.h, "set_predictor")
+            return self
+        p = np.ascontiguousarray(params, np.float32)
+        check(lib().xt_ctm_set_predictor(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size, int(width), float(margin)), self.rt.h, "set_predictor")
         return self
 
     def rsvd_stats(self):
