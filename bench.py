@@ -9,8 +9,9 @@ Heisenberg energy.  Two modes run on fresh runtimes (no shared plan caches):
          products with per-sector target ranks from the learned predictor (or, with
          --ranks previous, the previous step's kept counts); full FP64 SVD as fallback
 Both start from the same warm-up (init from the double layer + one full-SVD grow_all reaching
-chi = 256).  `value` is the contraction throughput of the fast step: the summed algorithmic flops
-of its grouped contraction GEMMs over their HIP-event-timed kernel time; `ms_per_step` is the
+chi = 256).  `value` is the GEMM throughput of the fast step: the summed algorithmic flops of
+all its grouped GEMM launches (contractions, sketch and Gram products) over their HIP-event-timed
+kernel time (k_gemm_f64 plus the split-K reduction); `ms_per_step` is the
 fast step's wall time; the fp64 step time, the speedup and the energy difference are reported
 beside them.
 
@@ -88,6 +89,7 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     rt.synchronize()
     svd0 = c.svd_ms()
     c0 = rt.counters()
+    h0 = rt.host_stats()
     t0 = time.perf_counter()
     for k in range(nsteps):
         energies.append(c.step())
@@ -95,13 +97,20 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     rt.synchronize()
     wall = time.perf_counter() - t0
     c1 = rt.counters()
-    out = {"ms_per_step": 1e3 * wall / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
+    h1 = rt.host_stats()
+    plan_ms = 1e3 * ((h1["permute_plan_s"] - h0["permute_plan_s"]) + (h1["gemm_plan_s"] - h0["gemm_plan_s"])) / nsteps
+    out = {"ms_per_step": 1e3 * wall / nsteps, "host_plan_ms_per_step": plan_ms,
+           "plans_per_step": ((h1["n_permute_plans"] - h0["n_permute_plans"]) + (h1["n_gemm_plans"] - h0["n_gemm_plans"])) / nsteps,
+           "host_breakdown_ms_per_step": [round(1e3 * (b - a) / nsteps, 1) for a, b in zip(h0["breakdown_s"], h1["breakdown_s"])], "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
     if mode == "fast":
-        # one more step with every contraction GEMM timed by HIP events on the context stream
+        # one more step with every GEMM launch timed by HIP events on the context stream; marker
+        # kernels of 1 and 2 workgroups delimit it in a kernel trace
         rt.gemm_timing(True, reset=True)
+        rt.marker(1)
         c.step()
+        rt.marker(2)
         ms, fl, nlog = rt.gemm_timing(False)
         out["gemm_kernel_ms"] = ms
         out["gemm_timed_flops"] = fl
@@ -138,8 +147,8 @@ def cpu_gemm_baseline(shapes, seconds):
     el = time.perf_counter() - t0
     threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
     return {"value": round(done / el / 1e12, 6), "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n} sector-block GEMMs ({done / 1e12:.3f} TFLOP of the {flops / 1e12:.3f} TFLOP of one step's {len(work)} blocks, "
-                      f"in step order) in {el:.1f} s; oracle per-block FP64 matmul (numpy/OpenBLAS)"}
+            "sample": f"{n} sector-block GEMMs, {done / 1e12:.3f} TFLOP in {el:.1f} s: the {len(work)} contraction blocks of one fast step "
+                      f"({flops / 1e12:.3f} TFLOP) cycled in step order; oracle per-block FP64 matmul (numpy/OpenBLAS)"}
 
 
 def run_c4(args):
@@ -171,7 +180,8 @@ def run_c4(args):
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
-                 "host_plan_s": round(fast["host"]["permute_plan_s"] + fast["host"]["gemm_plan_s"], 3), "host": fast["host"]},
+                 "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
+                 "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"]},
         "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5), "traffic": None,
                      "kernel": f"k_gemm_{'f64' if args.precision == 'fp64' else args.precision}",
                      "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},
@@ -180,7 +190,7 @@ def run_c4(args):
         nb = min(len(ref["energies"]), len(fast["energies"]))
         errs = [abs(fast["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nb)]
         res["fp64"] = {"ms_per_step": round(ref["ms_per_step"], 3), "svd_ms_per_step": round(ref["svd_ms_per_step"], 3), "energies": ref["energies"],
-                       "host_plan_s": round(ref["host"]["permute_plan_s"] + ref["host"]["gemm_plan_s"], 3)}
+                       "host_plan_ms_per_step": round(ref["host_plan_ms_per_step"], 1), "plans_per_step": ref["plans_per_step"]}
         res["speedup_vs_fp64"] = round(ref["ms_per_step"] / fast["ms_per_step"], 3)
         res["energy_rel_err_vs_fp64"] = max(errs) if errs else None
     if not args.no_cpu:

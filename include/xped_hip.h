@@ -97,6 +97,13 @@ int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, 
 int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan);
 int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
 int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops);
+/* HIP-event timing of every xh_gemm launch on the context stream (measurement; no counterpart in
+ * the reference): enable (1) / disable (0) / keep (-1); returns the summed kernel time, the flops of
+ * the timed launches and their count since the last reset (reset != 0 clears after reading). */
+int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* flops, int64_t* launches);
+/* an empty kernel of `id` workgroups (1..4096) on the context stream: a mark in a kernel trace
+ * that delimits a region of interest (measurement; no counterpart in the reference) */
+int xh_trace_marker(xh_ctx* ctx, int id);
 
 /* ---- batched factorisations -------------------------------------------------------------
  * Replace the per-sector loops of Tensor::full_svd_impl (src/Core/Tensor.cpp:510-519) over
@@ -137,7 +144,9 @@ typedef struct {
     double factor;
 } xh_rect;
 int xh_scale_rects(xh_ctx* ctx, double* x, const xh_rect* rects, int64_t nrects);
-/* diagonal maps of square blocks (DiagCoeffUnaryOp, TensorBase.cpp:191-201): mode 0 = sqrt, 1 = inverse, 2 = inverse sqrt */
+/* diagonal maps of square blocks (DiagCoeffUnaryOp, TensorBase.cpp:191-201): mode 0 = sqrt, 1 = inverse, 2 = inverse sqrt;
+ * + 4: the off-diagonal entries are set to zero, as the reference's diagonal expression is a
+ * diagonal matrix (PlainInterface diagUnaryFunc: M.diagonal().unaryExpr(f).asDiagonal()) */
 int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, double* x, int mode);
 /* copy rectangles: dst block (rows x cols at dst_off, ld dst_ld) = src block (src_off, src_ld) */
 typedef struct {
@@ -147,6 +156,35 @@ typedef struct {
     int32_t transpose; /* 1: dst = factor * src^T (src is cols x rows) */
 } xh_copy_rect;
 int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects);
+/* dst block += factor * op(src block): the scatter of the truncated SVD adjoints into the full ones
+ * (cutoff_matrices reverse pass, include/Xped/AD/ADTensor.hpp:357-370).  Rectangles must not overlap. */
+int xh_add_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects);
+
+/* ---- reverse mode ----------------------------------------------------------------------
+ * The adjoint of the per-sector SVD A = U S V (full_svd_impl reverse pass,
+ * include/Xped/AD/ADTensor.hpp:262-320, with the 1e-12 regularised F of :279-282), accumulated
+ * into dA for every item.  U and dU blocks are m x k (ld m) at u_off, S and dS k x k (ld k, S
+ * diagonal) at s_off, V and dV k x n (ld k) at v_off, dA m x n (ld m) at a_off; k <= min(m, n).
+ * The permute adjoint (Tensor::permute_adj, src/Core/Tensor.cpp:258-333) is a permute plan built
+ * on the host, and the operator* adjoints (ADTensor.hpp:745-746) are xh_gemm plans with
+ * trans_a / trans_b set. */
+/* Diagonal-map reverse passes of square blocks (ADTensor.hpp:560-590; DiagCoeffBinaryOp acts on
+ * the diagonal only): mode 0 for y = diag_sqrt(x): dx_ii += dy_ii * 0.5 / y_ii; mode 1 for
+ * y = diag_inv(x): dx_ii += -dy_ii * y_ii^2. */
+int xh_block_diag_bwd(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* y, const double* dy, double* dx, int mode);
+/* x_g += w_g * identity (rows_g x cols_g blocks): the trace reverse pass with WeightedIdentity
+ * (ADTensor.hpp:533-545, Tensor.hpp:238-246). */
+int xh_block_add_identity(xh_ctx* ctx, const int64_t* off, const int32_t* rows, const int32_t* cols, const double* w, int64_t nblocks, double* x);
+/* max |x_i| and the first index attaining it (maxNorm with maxCoeff's location, ADTensor.hpp:450-476) */
+int xh_max_abs_index(xh_ctx* ctx, const double* x, size_t n, double* value, int64_t* index);
+/* x[index] += v (the maxNorm reverse pass, ADTensor.hpp:462-471) */
+int xh_add_at(xh_ctx* ctx, double* x, int64_t index, double v);
+typedef struct {
+    int64_t u_off, s_off, v_off, a_off;
+    int32_t m, n, k;
+} xh_svd_bwd_item;
+int xh_svd_bwd(xh_ctx* ctx, const xh_svd_bwd_item* items, int64_t nitems, const double* U, const double* S, const double* V, const double* dU,
+               const double* dS, const double* dV, double* dA);
 
 /* ---- learned rank predictor: a small FP32 network nin -> width -> width -> 1 (ReLU), MSE loss,
  * Adam; hand-written forward/backward kernels.  No counterpart in the reference (its SVD is always

@@ -42,12 +42,17 @@ int xt_rt_clear_caches(xt_rt* rt);
 /* host time (seconds) spent building permute and GEMM plans, and how many were built */
 int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans);
 /* breakdown of that host time (seconds): [0] permute plans on the host, [1] their checks and
- * packing, [2] device permute plans, [3] GEMM plans on the host, [4] device GEMM plans */
+ * packing, [2] device permute plans, [3] GEMM plans on the host, [4] device GEMM plans; [5] the
+ * number of permute plans whose dimension-free part came from the cache (same leg charges) */
 int xt_rt_host_timers(const xt_rt* rt, double* t, int n);
-/* contraction GEMM timing: enable = 1 times every operator* launch with HIP events (and logs its
- * block shapes), 0 stops, -1 leaves the setting; returns the totals since the last reset
- * (reset = 1 clears them first) */
+/* GEMM timing: enable = 1 times every grouped GEMM launch of the runtime's context with HIP events
+ * (contractions, sketch and Gram products, adjoints) and logs the block shapes of the operator*
+ * contractions; 0 stops, -1 leaves the setting; returns the kernel time and flops since the last
+ * reset (reset = 1 clears them first) and the number of logged contraction blocks.  Measurement
+ * only; no counterpart in the reference. */
 int xt_rt_gemm_timing(xt_rt* rt, int enable, int reset, double* kernel_ms, double* flops, long* nlog);
+/* an empty marker kernel of `id` workgroups (xh_trace_marker) on the runtime's stream */
+int xt_rt_marker(xt_rt* rt, int id);
 /* the logged block shapes: 3 ints (m, n, k) per block, up to cap blocks */
 int xt_rt_gemm_log(const xt_rt* rt, int32_t* mnk, long cap);
 
@@ -123,6 +128,40 @@ int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_mu
 int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, const int* qs, const int* ranks, int nsec, int oversample,
                        double rank_factor, int power_iters, int sketch_precision, uint64_t seed, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
                        double* trunc_weight, int* safe, double* residual_ratio);
+/* The two halves of tSVD, as the AD tensor records them (include/Xped/AD/ADTensor.hpp:221-233):
+ * full_svd_impl (Tensor.cpp:494-521): untruncated U (uncoupled domain | middle), Sigma, Vdag
+ * with middle dimension min(rows, cols) per sector; and cutoff_matrices (Tensor.cpp:523-633)
+ * applied to those factors of t (Sigma's diagonal is the singular-value list). */
+int xt_svd_full(const xt_tensor* t, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag);
+int xt_cutoff(const xt_tensor* t, const xt_tensor* U, const xt_tensor* S, const xt_tensor* Vdag, size_t max_keep, double eps_svd,
+              int preserve_multiplets, xt_tensor** Ut, xt_tensor** St, xt_tensor** Vdagt, double* trunc_weight, double* entropy);
+
+/* ---- reverse mode (include/Xped/AD/ADTensor.hpp) ----------------------------------------
+ * Each backward accumulates (+=) into adjoint tensors that have the structures of the values,
+ * as the reference's reverse callbacks do. */
+/* a zero tensor with the structure of t (an adjoint's initial value) */
+int xt_tensor_zeros_like(const xt_tensor* t, xt_tensor** out);
+/* Y.permute_adj<shift, p...>() (Tensor.cpp:258-333): for Y = X.permute<shift, p...>(), returns the
+ * transpose map applied to Y, with the legs of X; the permute reverse pass (ADTensor.hpp:171-185). */
+int xt_permute_adj(const xt_tensor* y, int shift, const int* p, int np, xt_tensor** out);
+/* operator* reverse pass (ADTensor.hpp:745-746): dA += dC * B^dagger, dB += A^dagger * dC for
+ * C = A * B.  dA or dB may be NULL. */
+int xt_mul_bwd(const xt_tensor* A, const xt_tensor* B, const xt_tensor* dC, xt_tensor* dA, xt_tensor* dB, int precision);
+/* full_svd_impl reverse pass (ADTensor.hpp:262-320): dA += the SVD adjoint per sector, F with the
+ * 1e-12 regularisation of :279-282. */
+int xt_svd_bwd(const xt_tensor* U, const xt_tensor* S, const xt_tensor* Vdag, const xt_tensor* dU, const xt_tensor* dS, const xt_tensor* dVdag,
+               xt_tensor* dA);
+/* cutoff_matrices reverse pass (ADTensor.hpp:357-370): adds the truncated adjoints into the
+ * leading columns / corner / rows of the full ones. */
+int xt_cutoff_bwd(const xt_tensor* dUt, const xt_tensor* dSt, const xt_tensor* dVdagt, xt_tensor* dU, xt_tensor* dS, xt_tensor* dVdag);
+/* diag_sqrt (mode 0) / diag_inv (mode 1) reverse passes (ADTensor.hpp:560-590): dx += ... with
+ * y the forward result and dy its adjoint; diagonal entries only. */
+int xt_diag_bwd(const xt_tensor* y, const xt_tensor* dy, xt_tensor* dx, int mode);
+/* trace reverse pass (ADTensor.hpp:533-545): dx += g * WeightedIdentity (Tensor.hpp:238-246). */
+int xt_trace_bwd(xt_tensor* dx, double g);
+/* maxNorm reverse pass (ADTensor.hpp:450-476): the first entry of largest modulus of x gets
+ * dx += g * sign(x). */
+int xt_max_norm_bwd(const xt_tensor* x, xt_tensor* dx, double g);
 int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R);   /* Tensor.cpp:757-780 */
 int xt_teigh(const xt_tensor* t, xt_tensor** D, xt_tensor** V); /* Tensor.cpp:782-803 */
 int xt_adjoint(const xt_tensor* t, xt_tensor** out);            /* AdjointOp.hpp:25-69 */

@@ -52,12 +52,12 @@ def normalized(t):
 
 
 def diag_map(t, f):
+    """diag_sqrt / diag_inv (TensorBase.cpp:191-201): the result block is the diagonal matrix of the
+    mapped diagonal (PlainInterface_Eigen_impl.hpp:141-144, M.diagonal().unaryExpr(f).asDiagonal())."""
     out = t.copy()
     for i in range(len(t.sector)):
-        b = out.block(i).copy()
-        d = np.diag(b).copy()
-        b[range(len(d)), range(len(d))] = f(d)
-        out.set_block(i, b)
+        d = np.diag(t.block(i)).copy()
+        out.set_block(i, np.diag(f(d)))
     return out
 
 

@@ -95,7 +95,9 @@ def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
     L = lib()
     ctx = rt.ctx()
     r = np.random.default_rng(5)
-    shapes = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (65, 63, 17), (130, 7, 200), (8, 300, 33), (200, 150, 129), (3, 3, 0)]
+    # the last three have K >= 512 and few output tiles: the plan splits their K into chunks
+    shapes = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (65, 63, 17), (130, 7, 200), (8, 300, 33), (200, 150, 129), (3, 3, 0),
+              (40, 30, 1500), (130, 70, 2048), (5, 3, 777)]
     items, ao, bo, co = [], 0, 0, 0
     for m, n, k in shapes:
         items.append({"a": ao, "b": bo, "c": co, "m": m, "n": n, "k": k})
@@ -131,6 +133,38 @@ def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
         got = out2[it["c"]:it["c"] + m * n].reshape(n, m).T
         scale = max(1.0, np.abs(c).max() if c.size else 1.0) * math.sqrt(max(it["k"], 1))
         assert np.abs(got - c).max(initial=0.0) <= 4 * tol * scale
+    L.xh_gemm_plan_destroy(ctx, plan)
+    for d in (dA, dB, dC):
+        L.xh_free(ctx, d)
+
+
+def test_gemm_timing_and_marker(rt):
+    L = lib()
+    ctx = rt.ctx()
+    m, n, k = 96, 80, 1200
+    r = np.random.default_rng(6)
+    A, B = r.uniform(-1, 1, m * k), r.uniform(-1, 1, k * n)
+    dA, dB, dC = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, sz in ((dA, m * k), (dB, k * n), (dC, m * n)):
+        assert L.xh_alloc(ctx, 8 * sz, C.byref(d)) == 0
+    L.xh_upload(ctx, dA, A.ctypes.data, 8 * m * k)
+    L.xh_upload(ctx, dB, B.ctypes.data, 8 * k * n)
+    arr = (_lib.xh_gemm_item * 1)(_lib.xh_gemm_item(0, 0, 0, m, n, k))
+    plan = C.c_void_p()
+    assert L.xh_gemm_plan_create(ctx, arr, 1, 0, 0, C.byref(plan)) == 0
+    ms, fl, nl = C.c_double(), C.c_double(), C.c_int64()
+    assert L.xh_gemm_timing(ctx, 1, 1, C.byref(ms), C.byref(fl), C.byref(nl)) == 0
+    assert L.xh_trace_marker(ctx, 1) == 0
+    for _ in range(3):
+        assert L.xh_gemm(ctx, plan, dA, dB, dC, 1.0, 0.0, _lib.XH_FP64) == 0
+    assert L.xh_trace_marker(ctx, 2) == 0
+    assert L.xh_gemm_timing(ctx, 0, 1, C.byref(ms), C.byref(fl), C.byref(nl)) == 0
+    assert nl.value == 3 and fl.value == 3 * 2.0 * m * n * k and ms.value > 0.0
+    assert L.xh_trace_marker(ctx, 0) != 0
+    out = np.empty(m * n)
+    L.xh_download(ctx, out.ctypes.data, dC, 8 * m * n)
+    ref = A.reshape(k, m).T @ B.reshape(n, k).T
+    np.testing.assert_allclose(out.reshape(n, m).T, ref, rtol=0, atol=1e-12 * np.abs(ref).max() * math.sqrt(k))
     L.xh_gemm_plan_destroy(ctx, plan)
     for d in (dA, dB, dC):
         L.xh_free(ctx, d)

@@ -111,7 +111,8 @@ def test_cholesky_inverse_kernel(rt):
     ctx = rt.ctx()
     r = np.random.default_rng(7)
     blocks = []
-    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6)]:
+    # n = 141 is the largest block held in LDS; n = 180 and 300 run from global memory
+    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6), (160, 141), (260, 180), (400, 300)]:
         Y = r.standard_normal((m, n))
         if n > 2:
             Y[:, 2] = Y[:, 0] - 2.0 * Y[:, 1]  # an exactly dependent column

@@ -256,3 +256,35 @@ def test_truncation_rule_matches_oracle(xlib):
         assert kept_p == {q: len(v) for q, v in kept_o.items()}
         assert tw_p == tw_o or abs(tw_p - tw_o) <= 1e-15 * abs(tw_o)
         assert en_p == en_o or abs(en_p - en_o) <= 1e-15 * max(1.0, abs(en_o))
+
+
+@pytest.mark.parametrize("kinds,pcode", SYMCASES, ids=["U1", "SU2", "FU1xU1", "FU1"])
+def test_permute_plans_reused_across_dimension_changes(xlib, kinds, pcode):
+    """A runtime keeps the dimension-free part of each permute plan by the charges of the input legs;
+    a tensor with the same charges and other dimensions (a truncation changed the sector sizes)
+    gets its plan from that part.  The result must equal the oracle's plan for the new dimensions."""
+    rt = xt.Runtime(-1)
+    exact = kinds != (SU2,)
+    rank, corank = 2, 2
+    legs = rand_legs(kinds, 7, rank + corank, 3, 3)
+    for scale in (1, 2, 3):
+        # same charges in the same order, dimensions scaled
+        scaled = [[(q, d * scale + (i % 2)) for i, (q, d) in enumerate(e)] for e in legs]
+        ob, pb = zip(*[both_bases(kinds, pcode, e) for e in scaled])
+        ot = oten.Tensor(Sym(*kinds), list(ob[:rank]), list(ob[rank:]))
+        pt = xt.Tensor(rt, list(pb[:rank]), list(pb[rank:]))
+        for p, shift in [((1, 0, 2, 3), 0), ((3, 1, 2, 0), 1), ((2, 3, 0, 1), -1), ((0, 2, 1, 3), 0)]:
+            out, tasks = oracle_plan(ot, shift, p)
+            plan = pt.permute_plan(shift, p)
+            assert list(plan["task_dst"]) == [t[0] for t in tasks]
+            assert list(plan["task_rows"]) == [t[1] for t in tasks]
+            assert list(plan["task_cols"]) == [t[2] for t in tasks]
+            assert list(plan["task_nsrc"]) == [len(t[3]) for t in tasks]
+            srcs = [s for t in tasks for s in t[3]]
+            assert list(plan["src_off"]) == [s[0] for s in srcs]
+            if exact:
+                assert list(plan["src_coeff"]) == [s[1] for s in srcs]
+            else:
+                np.testing.assert_allclose(plan["src_coeff"], [s[1] for s in srcs], rtol=0, atol=1e-13)
+    st = rt.host_stats()
+    assert st["n_permute_plans"] == 12 and st["symbolic_hits"] == 8

@@ -100,6 +100,9 @@ _SIGS = [
     ("xh_mlp_forward", C.c_int, [_P, _P, _P, C.c_int64, _P]),
     ("xh_mlp_grad", C.c_int, [_P, _P, _P, _P, C.c_int64, _P, C.POINTER(C.c_float)]),
     ("xh_mlp_adam_step", C.c_int, [_P, _P, _P, C.c_float, C.c_int64]),
+    ("xh_add_at", C.c_int, [_P, _P, C.c_int64, C.c_double]),
+    ("xh_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
+    ("xh_trace_marker", C.c_int, [_P, C.c_int]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),
     ("xh_event_elapsed_ms", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     # tensor layer
@@ -113,6 +116,7 @@ _SIGS = [
     ("xt_rt_host_timers", C.c_int, [_P, _DP, C.c_int]),
     ("xt_rt_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_long)]),
     ("xt_rt_gemm_log", C.c_int, [_P, C.POINTER(C.c_int32), C.c_long]),
+    ("xt_rt_marker", C.c_int, [_P, C.c_int]),
     ("xt_basis_create", C.c_int, [C.c_int, C.c_int, _PP]),
     ("xt_basis_destroy", C.c_int, [_P]),
     ("xt_basis_copy", C.c_int, [_P, _PP]),
@@ -151,6 +155,16 @@ _SIGS = [
     ("xt_tsvd", C.c_int, [_P, _SZ, C.c_double, C.c_int, _PP, _PP, _PP, _DP, _DP]),
     ("xt_tsvd_randomized", C.c_int, [_P, _SZ, C.c_double, _IP, _IP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint64, _PP, _PP, _PP,
                                      _DP, _IP, _DP]),
+    ("xt_svd_full", C.c_int, [_P, _PP, _PP, _PP]),
+    ("xt_cutoff", C.c_int, [_P, _P, _P, _P, _SZ, C.c_double, C.c_int, _PP, _PP, _PP, _DP, _DP]),
+    ("xt_tensor_zeros_like", C.c_int, [_P, _PP]),
+    ("xt_permute_adj", C.c_int, [_P, C.c_int, _IP, C.c_int, _PP]),
+    ("xt_mul_bwd", C.c_int, [_P, _P, _P, _P, _P, C.c_int]),
+    ("xt_svd_bwd", C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    ("xt_cutoff_bwd", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("xt_diag_bwd", C.c_int, [_P, _P, _P, C.c_int]),
+    ("xt_trace_bwd", C.c_int, [_P, C.c_double]),
+    ("xt_max_norm_bwd", C.c_int, [_P, _P, C.c_double]),
     ("xt_tqr", C.c_int, [_P, _PP, _PP]),
     ("xt_teigh", C.c_int, [_P, _PP, _PP]),
     ("xt_adjoint", C.c_int, [_P, _PP]),

@@ -6,6 +6,7 @@
 
 #include "../../include/xped_tensor.h"
 #include "core/ctm.hpp"
+#include "core/ad.hpp"
 #include "core/tensor.hpp"
 
 struct xt_rt {
@@ -138,23 +139,35 @@ int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_
 int xt_rt_host_timers(const xt_rt* rt, double* t, int n)
 {
     if(!rt || n < 0) return XH_EINVAL;
-    for(int i = 0; i < n && i < 8; ++i) t[i] = rt->rt.host_t[i];
+    for(int i = 0; i < n && i < 5; ++i) t[i] = rt->rt.host_t[i];
+    if(n > 5) t[5] = static_cast<double>(rt->rt.n_symbolic_hits);
+    for(int i = 6; i < n && i < 8; ++i) t[i] = 0.0;
     return XH_OK;
 }
 
 int xt_rt_gemm_timing(xt_rt* rt, int enable, int reset, double* kernel_ms, double* flops, long* nlog)
 {
-    if(!rt) return XH_EINVAL;
-    if(reset) {
-        rt->rt.gemm_kernel_ms = 0.0;
-        rt->rt.gemm_timed_flops = 0.0;
-        rt->rt.gemm_log.clear();
-    }
-    if(enable >= 0) rt->rt.time_gemm = enable != 0;
-    if(kernel_ms) *kernel_ms = rt->rt.gemm_kernel_ms;
-    if(flops) *flops = rt->rt.gemm_timed_flops;
-    if(nlog) *nlog = static_cast<long>(rt->rt.gemm_log.size());
-    return XH_OK;
+    return run(rt, [&] {
+        xh_ctx* ctx = rt->rt.ctx;
+        if(reset) {
+            rt->rt.gemm_log.clear();
+            if(ctx && xh_gemm_timing(ctx, -1, 1, nullptr, nullptr, nullptr) != XH_OK) throw xa::Error(XH_EHIP, xh_last_error(ctx));
+        }
+        if(enable >= 0) rt->rt.time_gemm = enable != 0;  // shape log of the contraction blocks
+        double ms = 0.0, fl = 0.0;
+        int64_t n = 0;
+        if(ctx && xh_gemm_timing(ctx, enable, 0, &ms, &fl, &n) != XH_OK) throw xa::Error(XH_EHIP, xh_last_error(ctx));
+        if(kernel_ms) *kernel_ms = ms;
+        if(flops) *flops = fl;
+        if(nlog) *nlog = static_cast<long>(rt->rt.gemm_log.size());
+    });
+}
+
+int xt_rt_marker(xt_rt* rt, int id)
+{
+    return run(rt, [&] {
+        if(rt->rt.ctx && xh_trace_marker(rt->rt.ctx, id) != XH_OK) throw xa::Error(XH_EINVAL, xh_last_error(rt->rt.ctx));
+    });
 }
 
 int xt_rt_gemm_log(const xt_rt* rt, int32_t* mnk, long cap)
@@ -175,6 +188,7 @@ int xt_rt_clear_caches(xt_rt* rt)
         for(auto& [k, v] : rt->rt.gemm_cache)
             if(v->dev) xh_gemm_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
         rt->rt.permute_cache.clear();
+        rt->rt.sym_permute_cache.clear();
         rt->rt.gemm_cache.clear();
         rt->rt.struct_cache.clear();
     });
@@ -433,7 +447,7 @@ int xt_permute(const xt_tensor* t, int shift, const int* p, int np, xt_tensor** 
 int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, int64_t* ntasks, int64_t* nsources, int64_t* elems)
 {
     return run(t ? t->owner : nullptr, [&] {
-        xa::PermutePlan plan = xa::make_permute_plan(*t->t.st, shift, std::vector<int>(p, p + np));
+        const xa::PermutePlan& plan = xa::get_permute_plan(t->t.rt, *t->t.st, shift, std::vector<int>(p, p + np))->plan;
         if(ntasks) *ntasks = static_cast<int64_t>(plan.tasks.size());
         if(nsources) *nsources = static_cast<int64_t>(plan.sources.size());
         if(elems) *elems = static_cast<int64_t>(plan.moved_elems);
@@ -444,7 +458,8 @@ int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, in
                          int64_t* src_off, double* src_coeff)
 {
     return run(t ? t->owner : nullptr, [&] {
-        xa::PermutePlan plan = xa::make_permute_plan(*t->t.st, shift, std::vector<int>(p, p + np));
+        // the runtime's cached path (symbolic plans reused across dimension changes)
+        const xa::PermutePlan& plan = xa::get_permute_plan(t->t.rt, *t->t.st, shift, std::vector<int>(p, p + np))->plan;
         for(std::size_t i = 0; i < plan.tasks.size(); ++i) {
             task_dst[i] = plan.tasks[i].dst_off;
             task_rows[i] = plan.tasks[i].rows;
@@ -517,6 +532,101 @@ int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, cons
         if(trunc_weight) *trunc_weight = r.res.trunc_weight;
         if(safe) *safe = r.safe ? 1 : 0;
         if(residual_ratio) *residual_ratio = r.max_residual_ratio;
+    });
+}
+
+int xt_svd_full(const xt_tensor* t, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        xa::FullSVD f = xa::full_svd(t->t);
+        *U = wrap(t->owner, f.U);
+        *S = wrap(t->owner, f.S);
+        *Vdag = wrap(t->owner, f.Vdag);
+    });
+}
+
+int xt_cutoff(const xt_tensor* t, const xt_tensor* U, const xt_tensor* S, const xt_tensor* Vdag, size_t max_keep, double eps_svd,
+              int preserve_multiplets, xt_tensor** Ut, xt_tensor** St, xt_tensor** Vdagt, double* trunc_weight, double* entropy)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        const xa::TensorStructure& ts = *t->t.st;
+        const xa::TensorStructure& ss = *S->t.st;
+        if(U->t.st->sector != ts.sector || ss.sector != ts.sector || Vdag->t.st->sector != ts.sector)
+            throw xa::Error(XH_EINVAL, "cutoff: the factors do not have the sectors of the tensor");
+        xa::FullSVD f;
+        f.U = U->t;
+        f.S = S->t;
+        f.Vdag = Vdag->t;
+        f.sectors = ts.sector;
+        std::vector<double> hs(ss.size);
+        if(ss.size) S->t.download(hs.data(), hs.size());
+        for(std::size_t i = 0; i < ts.sector.size(); ++i) {
+            const std::size_t r = std::min(ts.rows[i], ts.cols[i]);
+            if(ss.rows[i] != r || ss.cols[i] != r || U->t.st->cols[i] != r || Vdag->t.st->rows[i] != r)
+                throw xa::Error(XH_EINVAL, "cutoff: the factors are not the full SVD of the tensor");
+            for(std::size_t k = 0; k < r; ++k) f.all_sv.emplace_back(ts.sector[i], hs[ss.offsets[i] + k + k * r]);
+        }
+        xa::SVDResult res = xa::cutoff(t->t, f, max_keep, eps_svd, preserve_multiplets != 0, trunc_weight ? *trunc_weight : 0.0, entropy ? *entropy : 0.0);
+        *Ut = wrap(t->owner, res.U);
+        *St = wrap(t->owner, res.S);
+        *Vdagt = wrap(t->owner, res.Vdag);
+        if(trunc_weight) *trunc_weight = res.trunc_weight;
+        if(entropy) *entropy = res.entropy;
+    });
+}
+
+int xt_tensor_zeros_like(const xt_tensor* t, xt_tensor** out)
+{
+    return run(t ? t->owner : nullptr, [&] { *out = wrap(t->owner, xa::DTensor::zeros(t->t.rt, t->t.st)); });
+}
+
+int xt_permute_adj(const xt_tensor* y, int shift, const int* p, int np, xt_tensor** out)
+{
+    return run(y ? y->owner : nullptr, [&] { *out = wrap(y->owner, xa::permute_adj(y->t, shift, std::vector<int>(p, p + np))); });
+}
+
+int xt_mul_bwd(const xt_tensor* A, const xt_tensor* B, const xt_tensor* dC, xt_tensor* dA, xt_tensor* dB, int precision)
+{
+    return run(A ? A->owner : nullptr, [&] {
+        if(!B || !dC) throw xa::Error(XH_EINVAL, "mul adjoint: null operand");
+        xa::mul_bwd(A->t, B->t, dC->t, dA ? &dA->t : nullptr, dB ? &dB->t : nullptr, precision);
+    });
+}
+
+int xt_svd_bwd(const xt_tensor* U, const xt_tensor* S, const xt_tensor* Vdag, const xt_tensor* dU, const xt_tensor* dS, const xt_tensor* dVdag, xt_tensor* dA)
+{
+    return run(U ? U->owner : nullptr, [&] {
+        if(!S || !Vdag || !dU || !dS || !dVdag || !dA) throw xa::Error(XH_EINVAL, "svd adjoint: null operand");
+        xa::svd_bwd(U->t, S->t, Vdag->t, dU->t, dS->t, dVdag->t, dA->t);
+    });
+}
+
+int xt_cutoff_bwd(const xt_tensor* dUt, const xt_tensor* dSt, const xt_tensor* dVdagt, xt_tensor* dU, xt_tensor* dS, xt_tensor* dVdag)
+{
+    return run(dUt ? dUt->owner : nullptr, [&] {
+        if(!dSt || !dVdagt || !dU || !dS || !dVdag) throw xa::Error(XH_EINVAL, "cutoff adjoint: null operand");
+        xa::cutoff_bwd(dUt->t, dSt->t, dVdagt->t, dU->t, dS->t, dVdag->t);
+    });
+}
+
+int xt_diag_bwd(const xt_tensor* y, const xt_tensor* dy, xt_tensor* dx, int mode)
+{
+    return run(y ? y->owner : nullptr, [&] {
+        if(!dy || !dx) throw xa::Error(XH_EINVAL, "diag adjoint: null operand");
+        xa::diag_bwd(y->t, dy->t, dx->t, mode);
+    });
+}
+
+int xt_trace_bwd(xt_tensor* dx, double g)
+{
+    return run(dx ? dx->owner : nullptr, [&] { xa::trace_bwd(dx->t, g); });
+}
+
+int xt_max_norm_bwd(const xt_tensor* x, xt_tensor* dx, double g)
+{
+    return run(x ? x->owner : nullptr, [&] {
+        if(!dx) throw xa::Error(XH_EINVAL, "maxNorm adjoint: null operand");
+        xa::max_norm_bwd(x->t, dx->t, g);
     });
 }
 

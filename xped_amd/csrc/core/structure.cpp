@@ -278,6 +278,17 @@ std::size_t Qbasis::left_offset(const FTree& t) const
     throw std::runtime_error("Qbasis::leftOffset: tree not present");
 }
 
+std::pair<std::size_t, std::size_t> Qbasis::tree_pos(const FTree& t) const
+{
+    (void)left_offset(t);  // builds the index and checks presence
+    const int i = find(t.coupled);
+    const auto& m = (*tree_index)[i];
+    auto range = m.equal_range(t.hash());
+    for(auto it = range.first; it != range.second; ++it)
+        if(data[i].trees[it->second.first].same(t)) return it->second;
+    throw std::runtime_error("Qbasis::tree_pos: tree not present");
+}
+
 bool Qbasis::equals(const Qbasis& o) const
 {
     if(data.size() != o.data.size()) return false;
@@ -577,61 +588,23 @@ StructPtr permuted_structure(const TensorStructure& in, int shift, const std::ve
     return TensorStructure::create(in.sym, nd, nc);
 }
 
-PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p)
+SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStructure& out, int shift, const std::vector<int>& p)
 {
     check_perm(in, shift, p);
     const int R = in.rank, C = in.corank, N = R + C;
-    const int newR = R - shift;
     const bool split = is_split_case(in, shift, p);
-    std::vector<Qbasis> nd, nc;
-    permuted_bases(in, shift, p, split, nd, nc);
-    PermutePlan plan;
-    plan.nlegs = N;
-    plan.new_rank = newR;
-    plan.out = TensorStructure::create(in.sym, nd, nc);
-    const TensorStructure& out = *plan.out;
-
-    const Permutation Pfull(p);
-    // enumerate every output subblock so each output element is written exactly once; subblock
-    // (row offset, column offset) -> task, per output sector
-    std::vector<std::unordered_map<std::uint64_t, int>> lookup(out.sector.size());
-    auto sub_key = [](std::size_t ro, std::size_t co) { return (static_cast<std::uint64_t>(ro) << 32) | static_cast<std::uint64_t>(co); };
+    SymbolicPermute sp;
+    sp.nlegs = N;
+    sp.new_rank = R - shift;
+    // every output subblock is one task, so each output element is written exactly once
+    int64_t ntasks = 0;
     for(std::size_t s = 0; s < out.sector.size(); ++s) {
         const Q& q = out.sector[s];
-        const auto& dtrees = out.dom.tree(q);
-        const auto& ctrees = out.cod.tree(q);
-        lookup[s].reserve(dtrees.size() * ctrees.size());
-        std::size_t roff = 0;
-        for(const auto& t1 : dtrees) {
-            std::size_t coff = 0;
-            for(const auto& t2 : ctrees) {
-                PermuteTask task;
-                task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
-                task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
-                task.rows = static_cast<std::int64_t>(t1.dim);
-                task.cols = static_cast<std::int64_t>(t2.dim);
-                for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
-                // strides of the corresponding source legs (source leg m = p[k])
-                for(int k = 0; k < N; ++k) {
-                    int m = p[k];
-                    std::int64_t st = 1;
-                    if(m < R) {
-                        for(int mm = 0; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
-                        task.is_col[k] = 0;
-                    } else {
-                        for(int mm = R; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
-                        task.is_col[k] = 1;
-                    }
-                    task.sstride[k] = st;
-                }
-                lookup[s].emplace(sub_key(roff, coff), static_cast<int>(plan.tasks.size()));
-                plan.tasks.push_back(task);
-                plan.moved_elems += t1.dim * t2.dim;
-                coff += t2.dim;
-            }
-            roff += t1.dim;
-        }
+        sp.task_base.push_back(static_cast<std::int32_t>(ntasks));
+        sp.ncod.push_back(static_cast<std::int32_t>(out.cod.tree(q).size()));
+        ntasks += static_cast<int64_t>(out.dom.tree(q).size()) * out.cod.tree(q).size();
     }
+    if(ntasks > 0x7fffffff) throw std::length_error("permute: too many subblocks");
 
     std::vector<int> pdv(p.begin(), p.begin() + R), pcv;
     for(int i = R; i < N; ++i) pcv.push_back(p[i] - R);
@@ -645,39 +618,32 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
     // tree), are cut into contiguous ranges that host threads map to their tasks; the ranges are
     // merged in order, so every task keeps its sources in that order
     struct Work {
-        std::size_t sector, row_tree;
-        std::size_t roff;
+        std::int32_t sector, row_tree;
     };
     std::vector<Work> work;
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
-        std::size_t roff_in = 0;
         const auto& dtrees = in.dom.tree(in.sector[i]);
-        for(std::size_t a = 0; a < dtrees.size(); ++a) {
-            work.push_back(Work{i, a, roff_in});
-            roff_in += dtrees[a].dim;
-        }
+        for(std::size_t a = 0; a < dtrees.size(); ++a) work.push_back(Work{static_cast<std::int32_t>(i), static_cast<std::int32_t>(a)});
     }
     // build the lazily created tree indices before threads share the bases
     for(const Qbasis* b : {&out.dom, &out.cod})
         if(!b->data.empty() && !b->data[0].trees.empty()) (void)b->left_offset(b->data[0].trees[0]);
-    using Contrib = std::vector<std::pair<int, PermuteSource>>;
+    using Contrib = std::vector<std::pair<std::int32_t, SymbolicPermute::Src>>;
     auto run_range = [&](std::size_t w0, std::size_t w1, Contrib& res) {
         for(std::size_t w = w0; w < w1; ++w) {
-            const std::size_t i = work[w].sector;
+            const std::int32_t i = work[w].sector;
             const Q& q = in.sector[i];
             const FTree& t1 = in.dom.tree(q)[work[w].row_tree];
-            std::size_t coff_in = 0;
-            for(const auto& t2 : in.cod.tree(q)) {
-                const std::int64_t src_off = static_cast<std::int64_t>(in.offsets[i] + work[w].roff + coff_in * in.rows[i]);
-                const std::int64_t src_ld = static_cast<std::int64_t>(in.rows[i]);
+            const auto& ctrees = in.cod.tree(q);
+            for(std::size_t b2 = 0; b2 < ctrees.size(); ++b2) {
+                const FTree& t2 = ctrees[b2];
                 auto add = [&](const FTree& a, const FTree& b, double c) {
                     if(std::abs(c) < PERMUTE_SKIP) return;
-                    int so = out.find_sector(a.coupled);
+                    const int so = out.find_sector(a.coupled);
                     if(so < 0) throw std::logic_error("permute: output sector missing");
-                    std::size_t ro = out.dom.left_offset(a), co = out.cod.left_offset(b);
-                    auto it = lookup[so].find(sub_key(ro, co));
-                    if(it == lookup[so].end()) throw std::logic_error("permute: output subblock missing");
-                    res.emplace_back(it->second, PermuteSource{src_off, src_ld, c});
+                    const std::size_t ia = out.dom.tree_pos(a).first, ib = out.cod.tree_pos(b).first;
+                    const std::int32_t task = sp.task_base[so] + static_cast<std::int32_t>(ia) * sp.ncod[so] + static_cast<std::int32_t>(ib);
+                    res.emplace_back(task, SymbolicPermute::Src{i, work[w].row_tree, static_cast<std::int32_t>(b2), c});
                 };
                 if(split) {
                     TreeMap pd = tree_permute(in.sym, t1, *Pd);
@@ -687,13 +653,12 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
                 } else {
                     for(const auto& [pr, c] : pair_permute(in.sym, shift, t1, t2, P)) add(pr.first, pr.second, c);
                 }
-                coff_in += t2.dim;
             }
         }
     };
     const std::size_t nw = work.size();
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const std::size_t nthreads = plan.tasks.size() < 4096 ? 1 : std::min<std::size_t>(hw, std::max<std::size_t>(1, nw / 4));
+    const std::size_t nthreads = ntasks < 4096 ? 1 : std::min<std::size_t>(hw, std::max<std::size_t>(1, nw / 4));
     std::vector<Contrib> parts(nthreads);
     if(nthreads == 1) {
         run_range(0, nw, parts[0]);
@@ -712,19 +677,207 @@ PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::v
         for(auto& e : errs)
             if(e) std::rethrow_exception(e);
     }
-    std::vector<int> count(plan.tasks.size(), 0);
+    sp.src_begin.assign(static_cast<std::size_t>(ntasks), 0);
+    sp.nsrc.assign(static_cast<std::size_t>(ntasks), 0);
     for(const auto& part : parts)
-        for(const auto& [t, src] : part) ++count[t];
-    std::size_t nsrc = 0;
-    for(std::size_t t = 0; t < plan.tasks.size(); ++t) {
-        plan.tasks[t].src_begin = static_cast<int>(nsrc);
-        plan.tasks[t].nsrc = count[t];
-        nsrc += static_cast<std::size_t>(count[t]);
+        for(const auto& [t, src] : part) ++sp.nsrc[static_cast<std::size_t>(t)];
+    std::size_t total = 0;
+    for(std::size_t t = 0; t < sp.nsrc.size(); ++t) {
+        sp.src_begin[t] = static_cast<std::int32_t>(total);
+        total += static_cast<std::size_t>(sp.nsrc[t]);
     }
-    plan.sources.resize(nsrc);
-    std::vector<int> fill(plan.tasks.size(), 0);
+    if(total > 0x7fffffff) throw std::length_error("permute: too many sources");
+    sp.sources.resize(total);
+    std::vector<std::int32_t> fill(sp.nsrc.size(), 0);
     for(const auto& part : parts)
-        for(const auto& [t, src] : part) plan.sources[static_cast<std::size_t>(plan.tasks[t].src_begin + fill[t]++)] = src;
+        for(const auto& [t, src] : part) sp.sources[static_cast<std::size_t>(sp.src_begin[t] + fill[t]++)] = src;
+    return sp;
+}
+
+namespace {
+// left offsets of the trees of sector q in a coupled basis (Qbasis::leftOffset for every tree)
+std::vector<std::size_t> tree_offsets(const Qbasis& b, const Q& q)
+{
+    const auto& trees = b.tree(q);
+    std::vector<std::size_t> off(trees.size());
+    std::size_t o = 0;
+    for(std::size_t k = 0; k < trees.size(); ++k) {
+        off[k] = o;
+        o += trees[k].dim;
+    }
+    return off;
+}
+}  // namespace
+
+PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructure& in, const StructPtr& outp, const std::vector<int>& p)
+{
+    const TensorStructure& out = *outp;
+    const int R = in.rank, N = sp.nlegs, newR = sp.new_rank;
+    if(N != in.rank + in.corank || sp.task_base.size() != out.sector.size()) throw std::logic_error("permute: symbolic plan does not match the structure");
+    const Permutation Pfull(p);
+    PermutePlan plan;
+    plan.nlegs = N;
+    plan.new_rank = newR;
+    plan.out = outp;
+    plan.tasks.resize(sp.src_begin.size());
+    for(std::size_t s = 0; s < out.sector.size(); ++s) {
+        const Q& q = out.sector[s];
+        const auto& dtrees = out.dom.tree(q);
+        const auto& ctrees = out.cod.tree(q);
+        if(static_cast<std::int32_t>(ctrees.size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
+        std::size_t roff = 0;
+        for(std::size_t a = 0; a < dtrees.size(); ++a) {
+            const FTree& t1 = dtrees[a];
+            std::size_t coff = 0;
+            for(std::size_t b = 0; b < ctrees.size(); ++b) {
+                const FTree& t2 = ctrees[b];
+                const std::size_t id = static_cast<std::size_t>(sp.task_base[s]) + a * ctrees.size() + b;
+                PermuteTask& task = plan.tasks[id];
+                task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
+                task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
+                task.rows = static_cast<std::int64_t>(t1.dim);
+                task.cols = static_cast<std::int64_t>(t2.dim);
+                for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
+                // strides of the corresponding source legs (source leg m = p[k])
+                for(int k = 0; k < N; ++k) {
+                    const int m = p[k];
+                    std::int64_t st = 1;
+                    if(m < R) {
+                        for(int mm = 0; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
+                        task.is_col[k] = 0;
+                    } else {
+                        for(int mm = R; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
+                        task.is_col[k] = 1;
+                    }
+                    task.sstride[k] = st;
+                }
+                task.src_begin = sp.src_begin[id];
+                task.nsrc = sp.nsrc[id];
+                plan.moved_elems += t1.dim * t2.dim;
+                coff += t2.dim;
+            }
+            roff += t1.dim;
+        }
+    }
+    std::vector<std::vector<std::size_t>> in_roff(in.sector.size()), in_coff(in.sector.size());
+    for(std::size_t i = 0; i < in.sector.size(); ++i) {
+        in_roff[i] = tree_offsets(in.dom, in.sector[i]);
+        in_coff[i] = tree_offsets(in.cod, in.sector[i]);
+    }
+    plan.sources.resize(sp.sources.size());
+    for(std::size_t k = 0; k < sp.sources.size(); ++k) {
+        const auto& src = sp.sources[k];
+        const std::size_t i = static_cast<std::size_t>(src.sector);
+        if(i >= in.sector.size() || static_cast<std::size_t>(src.t1) >= in_roff[i].size() || static_cast<std::size_t>(src.t2) >= in_coff[i].size())
+            throw std::logic_error("permute: symbolic plan does not match the structure");
+        plan.sources[k] = PermuteSource{static_cast<std::int64_t>(in.offsets[i] + in_roff[i][src.t1] + in_coff[i][src.t2] * in.rows[i]),
+                                        static_cast<std::int64_t>(in.rows[i]), src.coeff};
+    }
+    return plan;
+}
+
+PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p)
+{
+    StructPtr out = permuted_structure(in, shift, p);
+    return numeric_permute_plan(make_symbolic_permute(in, *out, shift, p), in, out, p);
+}
+
+StructPtr permute_adj_structure(const TensorStructure& y, int shift, const std::vector<int>& p)
+{
+    const int N = y.rank + y.corank;
+    if(static_cast<int>(p.size()) != N) throw std::invalid_argument("permute_adj: permutation length does not match the tensor rank");
+    const int R = y.rank + shift;  // rank of X
+    if(R < 0 || R > N) throw std::invalid_argument("permute_adj: invalid shift");
+    const Permutation P(p);
+    std::vector<Qbasis> nd, nc;
+    for(int m = 0; m < N; ++m) {
+        const int k = P.pi_inv[m];  // Y leg that reads X leg m
+        const bool y_cod = k >= y.rank;
+        const Qbasis& leg = y_cod ? y.unc_cod[k - y.rank] : y.unc_dom[k];
+        if(m < R)
+            nd.push_back(y_cod ? leg.conj() : leg);
+        else
+            nc.push_back(y_cod ? leg : leg.conj());
+    }
+    return TensorStructure::create(y.sym, nd, nc);
+}
+
+PermutePlan make_permute_adj_plan(const StructPtr& xs, const TensorStructure& y, int shift, const std::vector<int>& p)
+{
+    const TensorStructure& in = *xs;
+    check_perm(in, shift, p);
+    const int R = in.rank, C = in.corank, N = R + C;
+    const int newR = R - shift;
+    const bool split = is_split_case(in, shift, p);
+    {
+        std::vector<Qbasis> nd, nc;
+        permuted_bases(in, shift, p, split, nd, nc);
+        StructPtr fwd = TensorStructure::create(in.sym, nd, nc);
+        if(fwd->sector != y.sector || fwd->size != y.size) throw std::invalid_argument("permute_adj: the tensor does not have the permuted structure");
+    }
+    const Permutation P(p);
+    std::unique_ptr<Permutation> Pd, Pc;
+    if(split) {
+        std::vector<int> pdv(p.begin(), p.begin() + R), pcv;
+        for(int i = R; i < N; ++i) pcv.push_back(p[i] - R);
+        Pd = std::make_unique<Permutation>(pdv);
+        Pc = std::make_unique<Permutation>(pcv);
+    }
+    for(const Qbasis* b : {&y.dom, &y.cod})
+        if(!b->data.empty() && !b->data[0].trees.empty()) (void)b->left_offset(b->data[0].trees[0]);
+    PermutePlan plan;
+    plan.nlegs = N;
+    plan.new_rank = R;
+    plan.out = xs;
+    for(std::size_t i = 0; i < in.sector.size(); ++i) {
+        const Q& q = in.sector[i];
+        std::size_t roff = 0;
+        for(const auto& t1 : in.dom.tree(q)) {
+            std::size_t coff = 0;
+            for(const auto& t2 : in.cod.tree(q)) {
+                PermuteTask task;
+                task.dst_off = static_cast<std::int64_t>(in.offsets[i] + roff + coff * in.rows[i]);
+                task.dst_ld = static_cast<std::int64_t>(in.rows[i]);
+                task.rows = static_cast<std::int64_t>(t1.dim);
+                task.cols = static_cast<std::int64_t>(t2.dim);
+                for(int m = 0; m < N; ++m) task.ext[m] = m < R ? t1.dims[m] : t2.dims[m - R];
+                // X leg m is Y leg k = pi_inv[m]; its stride inside a Y subblock (rows: Y domain legs, cols: Y codomain legs)
+                for(int m = 0; m < N; ++m) {
+                    const int k = P.pi_inv[m];
+                    std::int64_t st = 1;
+                    if(k < newR) {
+                        for(int j = 0; j < k; ++j) st *= task.ext[p[j]];
+                        task.is_col[m] = 0;
+                    } else {
+                        for(int j = newR; j < k; ++j) st *= task.ext[p[j]];
+                        task.is_col[m] = 1;
+                    }
+                    task.sstride[m] = st;
+                }
+                task.src_begin = static_cast<int>(plan.sources.size());
+                auto add = [&](const FTree& a, const FTree& b, double c) {
+                    if(std::abs(c) < PERMUTE_SKIP) return;
+                    const int so = y.find_sector(a.coupled);
+                    if(so < 0) throw std::logic_error("permute_adj: image sector missing");
+                    const std::size_t ro = y.dom.left_offset(a), co = y.cod.left_offset(b);
+                    plan.sources.push_back(PermuteSource{static_cast<std::int64_t>(y.offsets[so] + ro + co * y.rows[so]), static_cast<std::int64_t>(y.rows[so]), c});
+                };
+                if(split) {
+                    TreeMap pd = tree_permute(in.sym, t1, *Pd);
+                    TreeMap pc = tree_permute(in.sym, t2, *Pc);
+                    for(const auto& [a, ca] : pd)
+                        for(const auto& [b, cb] : pc) add(a, b, ca * cb);
+                } else {
+                    for(const auto& [pr, c] : pair_permute(in.sym, shift, t1, t2, P)) add(pr.first, pr.second, c);
+                }
+                task.nsrc = static_cast<int>(plan.sources.size()) - task.src_begin;
+                plan.tasks.push_back(task);
+                plan.moved_elems += t1.dim * t2.dim;
+                coff += t2.dim;
+            }
+            roff += t1.dim;
+        }
+    }
     return plan;
 }
 

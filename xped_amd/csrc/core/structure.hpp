@@ -156,6 +156,7 @@ struct Qbasis {
     Qbasis combine(const Qbasis& other) const;                                          // Qbasis.cpp:441-495
     Qbasis forget_history() const;                                                      // Qbasis.cpp:410-416
     std::size_t left_offset(const FTree& t) const;                                      // Qbasis.cpp:208-225
+    std::pair<std::size_t, std::size_t> tree_pos(const FTree& t) const;                 // (index in its sector's tree list, left offset)
     bool equals(const Qbasis& o) const;                                                 // Qbasis.cpp:333-349
 };
 
@@ -222,10 +223,34 @@ struct PermutePlan {
     std::size_t moved_elems = 0;  // sum over tasks of rows*cols
 };
 
+// Dimension-free part of a permute plan: which input subblocks (sector, row tree, column tree),
+// with which coefficients, feed every output subblock.  Tree lists are enumerated from charges
+// alone, so this part serves every input structure with the same charges on every leg; only the
+// offsets and extents (numeric_permute_plan) depend on the dimensions.
+struct SymbolicPermute {
+    struct Src {
+        std::int32_t sector, t1, t2;
+        double coeff;
+    };
+    int nlegs = 0, new_rank = 0;
+    std::vector<std::int32_t> task_base, ncod;  // per output sector: first task, codomain trees
+    std::vector<std::int32_t> src_begin, nsrc;  // per task (sector-major, then row tree, then column tree)
+    std::vector<Src> sources;                   // grouped by task, each group in the reference's order
+};
+SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStructure& out, int shift, const std::vector<int>& p);
+PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructure& in, const StructPtr& out, const std::vector<int>& p);
+
 // Build the output structure and gather plan of T.permute<shift, p...>() (Tensor.cpp:83-214,335-355).
 PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p);
 // Output structure of permute without the plan.
 StructPtr permuted_structure(const TensorStructure& in, int shift, const std::vector<int>& p);
+// Structure of X for a given Y = X.permute<shift, p...>() (the legs of Tensor::permute_adj,
+// Tensor.cpp:270-288).
+StructPtr permute_adj_structure(const TensorStructure& y, int shift, const std::vector<int>& p);
+// Gather plan of Y.permute_adj<shift, p...>() (Tensor.cpp:258-333): the transpose of the permute
+// map X -> Y.  Output = the structure `x` (which must permute to y), one task per subblock of x
+// with the forward images of its tree pair as sources.
+PermutePlan make_permute_adj_plan(const StructPtr& x, const TensorStructure& y, int shift, const std::vector<int>& p);
 
 // get_permutations (Constfct.hpp:55-109) evaluated at run time.
 struct ContractPerms {

@@ -32,6 +32,30 @@ void put_basis(std::string& s, const Qbasis& b)
     }
 }
 
+// the charges (not the dimensions) of every leg: equal keys give equal tree enumerations
+void put_basis_charges(std::string& s, const Qbasis& b)
+{
+    put_i(s, b.conj_flag ? 1 : 0);
+    put_i(s, static_cast<long long>(b.data.size()));
+    for(const auto& e : b.data) {
+        put_i(s, e.q.v[0]);
+        put_i(s, e.q.v[1]);
+        put_i(s, e.trees.empty() ? 0 : (e.trees[0].dual[0] ? 1 : 0));
+    }
+}
+
+std::string charges_key(const TensorStructure& t)
+{
+    std::string k;
+    put_i(k, t.sym.kind[0]);
+    put_i(k, t.sym.kind[1]);
+    put_i(k, static_cast<long long>(t.unc_dom.size()));
+    put_i(k, static_cast<long long>(t.unc_cod.size()));
+    for(const auto& b : t.unc_dom) put_basis_charges(k, b);
+    for(const auto& b : t.unc_cod) put_basis_charges(k, b);
+    return k;
+}
+
 }  // namespace
 
 Runtime::~Runtime()
@@ -136,6 +160,9 @@ void DTensor::set_random(std::mt19937& engine, int dist)
 
 // ------------------------------------------------------------------ permute
 
+template <class Lap>
+static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStructure& in, Lap& lap);
+
 std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p)
 {
     std::string k = structure_key(in);
@@ -150,8 +177,35 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
         t = now;
     };
     auto cp = std::make_shared<CachedPermute>();
-    cp->plan = make_permute_plan(in, shift, p);
+    // the dimension-free part is shared by all structures with the same charges (tensors whose
+    // sector dimensions changed after a truncation); only offsets and extents are recomputed
+    StructPtr out = permuted_structure(in, shift, p);
+    std::string sk = charges_key(in);
+    put_i(sk, shift);
+    for(int x : p) put_i(sk, x);
+    std::shared_ptr<const SymbolicPermute> sp;
+    auto si = rt->sym_permute_cache.find(sk);
+    if(si != rt->sym_permute_cache.end()) {
+        sp = si->second;
+        rt->n_symbolic_hits++;
+    } else {
+        sp = std::make_shared<const SymbolicPermute>(make_symbolic_permute(in, *out, shift, p));
+        rt->sym_permute_cache.emplace(sk, sp);
+    }
+    cp->plan = numeric_permute_plan(*sp, in, out, p);
     lap(0);
+    finish_permute_plan(rt, *cp, in, lap);
+    rt->permute_cache.emplace(k, cp);
+    rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->n_permute_plans++;
+    return cp;
+}
+
+// Bounds checks, output-structure sharing and the device upload of a host permute plan.
+template <class Lap>
+static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStructure& in, Lap& lap)
+{
+    CachedPermute* cp = &cpr;
     // every access of the plan must stay inside the input and output buffers
     {
         const PermutePlan& pl = cp->plan;
@@ -205,10 +259,48 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
         dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
         lap(2);
     }
+}
+
+std::shared_ptr<CachedPermute> get_permute_adj_plan(Runtime* rt, const TensorStructure& y, int shift, const std::vector<int>& p)
+{
+    std::string k = "adj|" + structure_key(y);
+    put_i(k, shift);
+    for(int x : p) put_i(k, x);
+    auto it = rt->permute_cache.find(k);
+    if(it != rt->permute_cache.end()) return it->second;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto lap = [rt, t = t0](int slot) mutable {
+        const auto now = std::chrono::steady_clock::now();
+        rt->host_t[slot] += std::chrono::duration<double>(now - t).count();
+        t = now;
+    };
+    auto cp = std::make_shared<CachedPermute>();
+    StructPtr xs = permute_adj_structure(y, shift, p);
+    cp->plan = make_permute_adj_plan(xs, y, shift, p);
+    lap(0);
+    finish_permute_plan(rt, *cp, y, lap);
     rt->permute_cache.emplace(k, cp);
     rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rt->n_permute_plans++;
     return cp;
+}
+
+DTensor permute_adj(const DTensor& y, int shift, const std::vector<int>& p)
+{
+    // Tensor.cpp:264: the identity returns the tensor itself
+    bool ident = (shift == 0);
+    for(std::size_t i = 0; ident && i < p.size(); ++i) ident = (p[i] == static_cast<int>(i));
+    if(ident && static_cast<int>(p.size()) == y.st->rank + y.st->corank) return y;
+    auto cp = get_permute_adj_plan(y.rt, *y.st, shift, p);
+    DTensor out = DTensor::empty(y.rt, cp->plan.out);
+    if(y.rt->ctx && out.size() > 0) {
+        dcheck(y.rt->ctx, xh_permute(y.rt->ctx, cp->dev, y.data(), out.data()), "permute_adj");
+        double b = 0;
+        xh_permute_plan_bytes(cp->dev, &b);
+        y.rt->permute_bytes += b;
+        y.rt->n_permute++;
+    }
+    return out;
 }
 
 DTensor permute(const DTensor& t, int shift, const std::vector<int>& p)
@@ -295,16 +387,9 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision)
             for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
         }
         Runtime* rt = l.rt;
-        if(rt->time_gemm) dcheck(rt->ctx, xh_event_record(rt->ctx, 14), "event");
         dcheck(rt->ctx, xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision), "grouped gemm");
-        if(rt->time_gemm) {
-            float ms = 0.f;
-            dcheck(rt->ctx, xh_event_record(rt->ctx, 15), "event");
-            dcheck(rt->ctx, xh_event_elapsed_ms(rt->ctx, 14, 15, &ms), "event time");
-            rt->gemm_kernel_ms += ms;
-            rt->gemm_timed_flops += cg->plan.flops;
+        if(rt->time_gemm)
             for(const auto& g : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(g.m), static_cast<int32_t>(g.n), static_cast<int32_t>(g.k)});
-        }
         rt->gemm_flops += cg->plan.flops;
         rt->n_gemm++;
     }
@@ -401,7 +486,9 @@ DTensor diag_map(const DTensor& t, int mode)
         off.push_back(static_cast<int64_t>(t.st->offsets[i]));
         m.push_back(static_cast<int32_t>(t.st->rows[i]));
     }
-    if(t.rt->ctx && !off.empty()) dcheck(t.rt->ctx, xh_block_diag_map(t.rt->ctx, off.data(), m.data(), static_cast<int64_t>(off.size()), out.data(), mode), "diag map");
+    // the reference's result is the diagonal matrix of the mapped diagonal (diagUnaryFunc)
+    if(t.rt->ctx && !off.empty())
+        dcheck(t.rt->ctx, xh_block_diag_map(t.rt->ctx, off.data(), m.data(), static_cast<int64_t>(off.size()), out.data(), mode | 4), "diag map");
     return out;
 }
 

@@ -57,6 +57,9 @@ struct Runtime {
     std::shared_ptr<CtxHolder> holder;
     xh_ctx* ctx = nullptr;
     std::unordered_map<std::string, std::shared_ptr<CachedPermute>> permute_cache;
+    // dimension-free permute plans keyed by the charges of the input legs, shift and p
+    std::unordered_map<std::string, std::shared_ptr<const SymbolicPermute>> sym_permute_cache;
+    long n_symbolic_hits = 0;
     std::unordered_map<std::string, std::shared_ptr<CachedGemm>> gemm_cache;
     std::unordered_map<std::string, StructPtr> struct_cache;
     // counters for reporting
@@ -71,7 +74,6 @@ struct Runtime {
     // optional per-launch timing of the contraction GEMMs (operator*) with HIP events, and a log
     // of their block shapes (m, n, k); off by default (it synchronises after every launch)
     bool time_gemm = false;
-    double gemm_kernel_ms = 0.0, gemm_timed_flops = 0.0;
     std::vector<std::array<int32_t, 3>> gemm_log;
     ~Runtime();
     void require_device() const;
@@ -102,6 +104,8 @@ public:
 };
 
 DTensor permute(const DTensor& t, int shift, const std::vector<int>& p);
+// Y.permute_adj<shift, p...>() (Tensor.cpp:258-333): the transpose of X -> X.permute<shift, p...>().
+DTensor permute_adj(const DTensor& y, int shift, const std::vector<int>& p);
 std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p);
 DTensor mul(const DTensor& l, const DTensor& r, int precision);
 DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision);

@@ -56,6 +56,7 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     xh::collect_staging(ctx, true);
     for(auto& ev : ctx->events)
         if(ev) hipEventDestroy(ev);
+    for(auto& ev : ctx->gemm_events) hipEventDestroy(ev);
     if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
     if(ctx->dev_scalar) hipFree(ctx->dev_scalar);
     if(ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
@@ -123,6 +124,42 @@ extern "C" int xh_event_record(xh_ctx* ctx, int slot)
     return guard(ctx, [&] {
         if(slot < 0 || slot >= 16) throw xh::Error(XH_EINVAL, "event slot out of range");
         check(hipEventRecord(ctx->events[slot], ctx->stream), "hipEventRecord");
+    });
+}
+
+extern "C" int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* flops, int64_t* launches)
+{
+    return guard(ctx, [&] {
+        double ms = 0.0, fl = 0.0;
+        const size_t n = ctx->gemm_events_used / 2;
+        if(n > 0) check(hipEventSynchronize(ctx->gemm_events[ctx->gemm_events_used - 1]), "hipEventSynchronize");
+        for(size_t i = 0; i < n; ++i) {
+            float t = 0.f;
+            check(hipEventElapsedTime(&t, ctx->gemm_events[2 * i], ctx->gemm_events[2 * i + 1]), "hipEventElapsedTime");
+            ms += t;
+            fl += ctx->gemm_event_flops[i];
+        }
+        if(kernel_ms) *kernel_ms = ms;
+        if(flops) *flops = fl;
+        if(launches) *launches = static_cast<int64_t>(n);
+        if(reset) {
+            ctx->gemm_events_used = 0;
+            ctx->gemm_event_flops.clear();
+        }
+        if(enable >= 0) ctx->gemm_timing = enable != 0;
+    });
+}
+
+namespace {
+__global__ void k_trace_marker() {}
+}  // namespace
+
+extern "C" int xh_trace_marker(xh_ctx* ctx, int id)
+{
+    return guard(ctx, [&] {
+        if(id < 1 || id > 4096) throw xh::Error(XH_EINVAL, "marker id must be in 1..4096");
+        hipLaunchKernelGGL(k_trace_marker, dim3(static_cast<unsigned>(id)), dim3(64), 0, ctx->stream);
+        xh::launch_check("k_trace_marker");
     });
 }
 
@@ -262,19 +299,96 @@ __global__ void k_diag_map(const int64_t* __restrict__ off, const int32_t* __res
     const int g = blockIdx.x;
     const int64_t o = off[g];
     const int mm = m[g];
+    const int f = mode & 3;
     for(int i = threadIdx.x; i < mm; i += blockDim.x) {
         double* p = x + o + int64_t(i) * mm + i;
         double v = *p;
-        if(mode == 0)
+        if(f == 0)
             v = sqrt(v);
-        else if(mode == 1)
+        else if(f == 1)
             v = 1.0 / v;
         else
             v = 1.0 / sqrt(v);
         *p = v;
     }
+    if(mode & 4) {  // diagonal matrix result: off-diagonal entries become zero
+        const int64_t n2 = int64_t(mm) * mm;
+        for(int64_t e = threadIdx.x; e < n2; e += blockDim.x)
+            if(e % mm != e / mm) x[o + e] = 0.0;
+    }
 }
 
+// Reverse passes of the diagonal maps (ADTensor.hpp:560-590), diagonal entries only, as the
+// reference's diagonal expressions are diagonal matrices: mode 0 (diag_sqrt, y = sqrt(x)):
+// dx_ii += dy_ii * 0.5 / y_ii;  mode 1 (diag_inv, y = 1/x): dx_ii += -dy_ii * y_ii^2.
+__global__ void k_diag_bwd(const int64_t* __restrict__ off, const int32_t* __restrict__ m, const double* __restrict__ y, const double* __restrict__ dy,
+                           double* __restrict__ dx, int mode)
+{
+    const int g = blockIdx.x;
+    const int64_t o = off[g];
+    const int mm = m[g];
+    for(int i = threadIdx.x; i < mm; i += blockDim.x) {
+        const int64_t e = o + int64_t(i) * mm + i;
+        const double yv = y[e], g1 = dy[e];
+        dx[e] += mode == 0 ? g1 * (1.0 / yv) * 0.5 : g1 * (-(yv * yv));
+    }
+}
+
+// x_g += w_g * identity for every block g (rows x cols, ld rows): the trace reverse pass with
+// WeightedIdentity (ADTensor.hpp:533-545, Tensor.hpp:238-246).
+__global__ void k_add_identity(const int64_t* __restrict__ off, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+                               const double* __restrict__ w, double* __restrict__ x)
+{
+    const int g = blockIdx.x;
+    const int r = rows[g], n = min(r, cols[g]);
+    for(int i = threadIdx.x; i < n; i += blockDim.x) x[off[g] + int64_t(i) * r + i] += w[g];
+}
+
+// Largest |x_i| and the first index attaining it (NaN counts as largest), two passes.
+struct MaxIdx {
+    double v;
+    int64_t i;
+};
+__device__ inline MaxIdx better(MaxIdx a, MaxIdx b)
+{
+    const bool an = isnan(a.v), bn = isnan(b.v);
+    if(an != bn) return an ? a : b;
+    if(a.v != b.v && !an) return a.v > b.v ? a : b;
+    return a.i <= b.i ? a : b;
+}
+__global__ void k_maxabs_idx(const double* __restrict__ x, size_t n, MaxIdx* __restrict__ part)
+{
+    MaxIdx m{-1.0, INT64_MAX};
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = better(m, MaxIdx{fabs(x[i]), int64_t(i)});
+    __shared__ MaxIdx s[EW_THREADS];
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for(int h = EW_THREADS / 2; h > 0; h >>= 1) {
+        if(int(threadIdx.x) < h) s[threadIdx.x] = better(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ void k_fold_idx(MaxIdx* __restrict__ part, int nb)
+{
+    __shared__ MaxIdx s[EW_THREADS];
+    MaxIdx m{-1.0, INT64_MAX};
+    for(int i = threadIdx.x; i < nb; i += blockDim.x) m = better(m, part[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for(int h = EW_THREADS / 2; h > 0; h >>= 1) {
+        if(int(threadIdx.x) < h) s[threadIdx.x] = better(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) part[nb] = s[0];
+}
+
+__global__ void k_add_at(double* __restrict__ x, int64_t i, double v)
+{
+    if(threadIdx.x == 0) x[i] += v;
+}
+
+template <bool ACC>
 __global__ void k_copy_rects(const double* __restrict__ src, double* __restrict__ dst, const xh_copy_rect* __restrict__ rects)
 {
     const xh_copy_rect r = rects[blockIdx.x];
@@ -282,7 +396,11 @@ __global__ void k_copy_rects(const double* __restrict__ src, double* __restrict_
     for(int64_t e = threadIdx.x; e < n; e += blockDim.x) {
         int64_t i = e % r.rows, j = e / r.rows;
         double v = r.transpose ? src[r.src_off + j + i * r.src_ld] : src[r.src_off + i + j * r.src_ld];
-        dst[r.dst_off + i + j * r.dst_ld] = r.factor == 1.0 ? v : r.factor * v;
+        v = r.factor == 1.0 ? v : r.factor * v;
+        if constexpr(ACC)
+            dst[r.dst_off + i + j * r.dst_ld] += v;
+        else
+            dst[r.dst_off + i + j * r.dst_ld] = v;
     }
 }
 
@@ -423,7 +541,7 @@ extern "C" int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t*
 {
     return guard(ctx, [&] {
         if(nblocks <= 0) return;
-        if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "diag map mode must be 0, 1 or 2");
+        if((mode & 3) > 2 || mode < 0 || mode > 6) throw xh::Error(XH_EINVAL, "diag map mode must be 0, 1 or 2, optionally + 4");
         std::vector<int64_t> o(off, off + nblocks);
         std::vector<int32_t> mm(m, m + nblocks);
         xh::PackedUpload up;
@@ -436,7 +554,72 @@ extern "C" int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t*
     });
 }
 
-extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects)
+extern "C" int xh_block_diag_bwd(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* y, const double* dy, double* dx, int mode)
+{
+    return guard(ctx, [&] {
+        if(nblocks <= 0) return;
+        if(mode != 0 && mode != 1) throw xh::Error(XH_EINVAL, "diag adjoint mode must be 0 (sqrt) or 1 (inverse)");
+        std::vector<int64_t> o(off, off + nblocks);
+        std::vector<int32_t> mm(m, m + nblocks);
+        xh::PackedUpload up;
+        const size_t oo = up.add(o), om = up.add(mm);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_diag_bwd, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, up.ptr<const int64_t>(oo),
+                           up.ptr<const int32_t>(om), y, dy, dx, mode);
+        xh::launch_check("k_diag_bwd");
+        up.release_deferred(ctx);
+    });
+}
+
+extern "C" int xh_block_add_identity(xh_ctx* ctx, const int64_t* off, const int32_t* rows, const int32_t* cols, const double* w, int64_t nblocks, double* x)
+{
+    return guard(ctx, [&] {
+        if(nblocks <= 0) return;
+        std::vector<int64_t> o(off, off + nblocks);
+        std::vector<int32_t> r(rows, rows + nblocks), c(cols, cols + nblocks);
+        std::vector<double> ww(w, w + nblocks);
+        xh::PackedUpload up;
+        const size_t oo = up.add(o), orr = up.add(r), oc = up.add(c), ow = up.add(ww);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_add_identity, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, up.ptr<const int64_t>(oo),
+                           up.ptr<const int32_t>(orr), up.ptr<const int32_t>(oc), up.ptr<const double>(ow), x);
+        xh::launch_check("k_add_identity");
+        up.release_deferred(ctx);
+    });
+}
+
+extern "C" int xh_max_abs_index(xh_ctx* ctx, const double* x, size_t n, double* value, int64_t* index)
+{
+    return guard(ctx, [&] {
+        *value = 0.0;
+        *index = -1;
+        if(n == 0) return;
+        const unsigned nb = std::min(ew_grid(n), 1024u);
+        MaxIdx* part = nullptr;
+        check(hipMallocAsync(reinterpret_cast<void**>(&part), (nb + 1) * sizeof(MaxIdx), ctx->stream), "hipMallocAsync");
+        hipLaunchKernelGGL(k_maxabs_idx, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, n, part);
+        xh::launch_check("k_maxabs_idx");
+        hipLaunchKernelGGL(k_fold_idx, dim3(1), dim3(EW_THREADS), 0, ctx->stream, part, static_cast<int>(nb));
+        xh::launch_check("k_fold_idx");
+        MaxIdx r;
+        check(hipMemcpyAsync(&r, part + nb, sizeof(MaxIdx), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        xh::dev_free(ctx, part);
+        *value = r.v;
+        *index = r.i;
+    });
+}
+
+extern "C" int xh_add_at(xh_ctx* ctx, double* x, int64_t index, double v)
+{
+    return guard(ctx, [&] {
+        if(index < 0) throw xh::Error(XH_EINVAL, "add_at: negative index");
+        hipLaunchKernelGGL(k_add_at, dim3(1), dim3(64), 0, ctx->stream, x, index, v);
+        xh::launch_check("k_add_at");
+    });
+}
+
+static int copy_rects_impl(bool acc, xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects)
 {
     return guard(ctx, [&] {
         if(nrects <= 0) return;
@@ -444,10 +627,20 @@ extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const 
         xh::PackedUpload up;
         const size_t o = up.add(r);
         up.upload(ctx);
-        hipLaunchKernelGGL(k_copy_rects, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, up.ptr<const xh_copy_rect>(o));
+        hipLaunchKernelGGL(acc ? k_copy_rects<true> : k_copy_rects<false>, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, up.ptr<const xh_copy_rect>(o));
         xh::launch_check("k_copy_rects");
         up.release_deferred(ctx);
     });
+}
+
+extern "C" int xh_copy_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects)
+{
+    return copy_rects_impl(false, ctx, src, dst, rects, nrects);
+}
+
+extern "C" int xh_add_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect* rects, int64_t nrects)
+{
+    return copy_rects_impl(true, ctx, src, dst, rects, nrects);
 }
 
 extern "C" int xh_fill_gaussian(xh_ctx* ctx, double* dst, size_t n, uint64_t seed)

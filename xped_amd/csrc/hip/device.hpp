@@ -23,6 +23,11 @@ struct xh_ctx {
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
     // host staging buffers of asynchronous uploads, kept until the event after their copy fires
     std::vector<std::pair<hipEvent_t, std::vector<char>>> pending_host;
+    // HIP-event timing of every grouped GEMM launch (xh_gemm_timing): event pairs in launch order
+    bool gemm_timing = false;
+    std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
+    size_t gemm_events_used = 0;
+    std::vector<double> gemm_event_flops;
 };
 
 // FP64 SVD of large sector blocks by one-sided block Jacobi (bjsvd.hip); same item layout as
@@ -64,6 +69,23 @@ int guard(xh_ctx* ctx, F&& f)
 }
 
 inline void launch_check(const char* name) { check(hipGetLastError(), name); }
+
+// event pair around one timed GEMM launch on the context stream
+inline void gemm_timer_begin(xh_ctx* ctx)
+{
+    while(ctx->gemm_events.size() < ctx->gemm_events_used + 2) {
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        ctx->gemm_events.push_back(e);
+    }
+    check(hipEventRecord(ctx->gemm_events[ctx->gemm_events_used], ctx->stream), "hipEventRecord");
+}
+inline void gemm_timer_end(xh_ctx* ctx, double flops)
+{
+    check(hipEventRecord(ctx->gemm_events[ctx->gemm_events_used + 1], ctx->stream), "hipEventRecord");
+    ctx->gemm_events_used += 2;
+    ctx->gemm_event_flops.push_back(flops);
+}
 
 // Round-up magic-number division for 0 <= x < 2^31 and 1 <= d < 2^31
 // (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994).

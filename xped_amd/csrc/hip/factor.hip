@@ -294,67 +294,117 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
 // index of the item's relative shift in fa.S.  A pivot that does not exceed 64 eps times its
 // column's original diagonal marks a numerically dependent column: its row and column of X are
 // zero, so Y X has a zero column there.
-__global__ __launch_bounds__(F_THREADS) void k_chol_inv(FactorArgs fa)
+//
+// One workgroup per block, one barrier per pivot.  The working matrix W is row-major (W[i n + j]);
+// its upper triangle holds G, then R, its strict lower triangle X^T, and sv[k] = 1 / r_kk (0 for
+// a dependent pivot) is the diagonal of X.  Pivot step k runs three disjoint phases:
+//   (a) row k-1 of R is scaled by sv[k-1] (the division by r_{k-1,k-1}, deferred one step);
+//   (b) right-looking update with the unscaled row k: W_ij -= W_ki W_kj / d_k for k < i <= j;
+//   (c) column k-1 of X from the finished rows/columns < k-1:
+//       X_{i,k-1} = -sv[k-1] * sum_{l=i}^{k-2} X_il R_{l,k-1}  (X R = I by columns).
+// (a) writes row k-1 (columns >= k), (b) reads row k and writes rows > k of the upper triangle,
+// (c) reads rows < k-1 and writes row k-1 of the lower triangle, so one barrier per step suffices.
+constexpr int CI_THREADS = 1024;
+
+__host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * n + 2 * n; }
+
+__global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
 {
     extern __shared__ double smem[];
+    __shared__ double s_part[CI_THREADS / 64];
     const xh_factor_item it = fa.items[blockIdx.x];
     const int n = it.n;
     if(n == 0) return;
-    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;  // n x n
-    double* rd = W + int64_t(n) * n;                                                           // n
+    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* rd = W + int64_t(n) * n;  // shifted original diagonal (dependence threshold)
+    double* sv = rd + n;              // 1 / r_kk, or 0
     const int tid = threadIdx.x;
+    const double* G = fa.A + it.a_off;
     const int64_t nn = int64_t(n) * n;
-    for(int64_t e = tid; e < nn; e += F_THREADS) W[e] = fa.A[it.a_off + e];
+    // upper triangle of G, read coalesced (column-major e = i + j n) into row-major W
+    double tr = 0.0;
+    for(int64_t e = tid; e < nn; e += CI_THREADS) {
+        const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+        const double g = G[e];
+        if(i <= j) W[int64_t(i) * n + j] = g;
+        if(i == j) tr += g;
+    }
+    for(int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
+    if((tid & 63) == 0) s_part[tid >> 6] = tr;
     __syncthreads();
-    if(tid == 0) {
-        double tr = 0.0;
-        for(int k = 0; k < n; ++k) tr += W[k + int64_t(k) * n];
-        const double shift = fa.S[it.s_off] * tr;
-        for(int k = 0; k < n; ++k) {
-            W[k + int64_t(k) * n] += shift;
-            rd[k] = W[k + int64_t(k) * n];
+    {
+        double t = 0.0;
+        for(int w = 0; w < CI_THREADS / 64; ++w) t += s_part[w];
+        const double shift = fa.S[it.s_off] * t;
+        for(int k = tid; k < n; k += CI_THREADS) {
+            const double d = W[int64_t(k) * n + k] + shift;
+            W[int64_t(k) * n + k] = d;
+            rd[k] = d;
         }
     }
-    __syncthreads();
-    for(int k = 0; k < n; ++k) {
-        if(tid == 0) {
-            const double d = W[k + int64_t(k) * n];
-            rd[k] = (d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX) ? sqrt(d) : 0.0;
+    for(int k = 0; k <= n; ++k) {
+        __syncthreads();
+        double inv_d = 0.0;
+        if(k < n) {
+            const double d = W[int64_t(k) * n + k];
+            const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
+            inv_d = ok ? 1.0 / d : 0.0;
+            if(tid == 0) sv[k] = ok ? 1.0 / sqrt(d) : 0.0;  // read from step k + 1 on
         }
-        __syncthreads();
-        const double r = rd[k];
-        for(int j = k + 1 + tid; j < n; j += F_THREADS) W[k + int64_t(j) * n] = r > 0.0 ? W[k + int64_t(j) * n] / r : 0.0;
-        __syncthreads();
-        const int t = n - k - 1;
-        if(r > 0.0 && t > 0) {
-            for(int e = tid; e < t * t; e += F_THREADS) {
-                const int i = k + 1 + e % t, j = k + 1 + e / t;
-                if(i <= j) W[i + int64_t(j) * n] -= W[k + int64_t(i) * n] * W[k + int64_t(j) * n];
+        if(k >= 1) {
+            // (a) deferred scaling of row k - 1
+            const double s = sv[k - 1];
+            double* row = W + int64_t(k - 1) * n;
+            for(int j = k + tid; j < n; j += CI_THREADS) row[j] *= s;
+        }
+        if(k + 1 < n && inv_d != 0.0) {
+            // (b) thread (s, jj): column j = k + 1 + jj, rows k + 1 + s, + c, ... up to j
+            const double* rowk = W + int64_t(k) * n;
+            const int t = n - k - 1;
+            if(t <= CI_THREADS) {
+                const int c = CI_THREADS / t;
+                const int sidx = tid / t, jj = tid - sidx * t;
+                if(sidx < c) {
+                    const int j = k + 1 + jj;
+                    const double wkj = rowk[j] * inv_d;
+                    for(int i = k + 1 + sidx; i <= j; i += c) W[int64_t(i) * n + j] -= rowk[i] * wkj;
+                }
+            } else {
+                for(int jj = tid; jj < t; jj += CI_THREADS) {
+                    const int j = k + 1 + jj;
+                    const double wkj = rowk[j] * inv_d;
+                    for(int i = k + 1; i <= j; ++i) W[int64_t(i) * n + j] -= rowk[i] * wkj;
+                }
             }
         }
-        __syncthreads();
-    }
-    // back substitution, one column of X per thread; X[i][j] (i < j) is kept at W[j + i n]
-    for(int j = tid; j < n; j += F_THREADS) {
-        const double xjj = rd[j] > 0.0 ? 1.0 / rd[j] : 0.0;
-        for(int i = j - 1; i >= 0; --i) {
-            double x = 0.0;
-            if(rd[i] > 0.0 && xjj != 0.0) {
-                double acc = W[i + int64_t(j) * n] * xjj;
-                for(int k = i + 1; k < j; ++k) acc = fma(W[i + int64_t(k) * n], W[j + int64_t(k) * n], acc);
-                x = -acc / rd[i];
+        if(k >= 2) {
+            // (c) column j = k - 1 of X, rows i < j; P lanes share one dot product
+            const int j = k - 1;
+            int P = 1;
+            while(P < 64 && P * 2 * j <= CI_THREADS) P *= 2;
+            const int per = CI_THREADS / P;
+            for(int base = 0; base < j; base += per) {
+                const int i = base + tid / P, part = tid & (P - 1);
+                double acc = 0.0;
+                if(i < j) {
+                    for(int l = i + part; l < j; l += P) {
+                        const double x = (l == i) ? sv[i] : W[int64_t(l) * n + i];
+                        acc = fma(x, W[int64_t(l) * n + j], acc);
+                    }
+                }
+                for(int o = P >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                if(i < j && part == 0) W[int64_t(j) * n + i] = -acc * sv[j];
             }
-            W[j + int64_t(i) * n] = x;
         }
     }
     __syncthreads();
-    for(int64_t e = tid; e < nn; e += F_THREADS) {
+    for(int64_t e = tid; e < nn; e += CI_THREADS) {
         const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
         double x = 0.0;
         if(i < j)
-            x = W[j + int64_t(i) * n];
+            x = W[int64_t(j) * n + i];
         else if(i == j)
-            x = rd[j] > 0.0 ? 1.0 / rd[j] : 0.0;
+            x = sv[j];
         fa.U[it.u_off + e] = x;
     }
 }
@@ -637,10 +687,11 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
         for(int64_t i = 0; i < nitems; ++i) {
             xh_factor_item it = items[i];
             if(it.m != it.n) throw xh::Error(XH_EINVAL, "chol_inv: blocks must be square");
-            if(it.n <= 0) continue;
+            if(it.n < 0 || it.a_off < 0 || it.u_off < 0) throw xh::Error(XH_EINVAL, "chol_inv: negative extent or offset");
+            if(it.n == 0) continue;
             it.s_off = static_cast<int64_t>(sh.size());
             sh.push_back(shift_rel ? shift_rel[i] : 0.0);
-            const int64_t need = int64_t(it.n) * it.n + it.n;
+            const int64_t need = chol_work_doubles(it.n);
             if(size_t(need) * sizeof(double) <= budget) {
                 small.push_back(it);
                 small_max = std::max(small_max, need);
@@ -659,14 +710,14 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
             if(shmem > 64 * 1024)
                 check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
             FactorArgs fa{up.ptr<xh_factor_item>(o_small), G, X, d_sh, nullptr, nullptr, 0, 1, nullptr};
-            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(small.size())), dim3(F_THREADS), shmem, ctx->stream, fa);
+            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(small.size())), dim3(CI_THREADS), shmem, ctx->stream, fa);
             xh::launch_check("k_chol_inv");
         }
         if(!big.empty()) {
             double* scratch = nullptr;
             check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(big_max) * big.size() * sizeof(double), ctx->stream), "hipMallocAsync");
             FactorArgs fa{up.ptr<xh_factor_item>(o_big), G, X, d_sh, nullptr, scratch, big_max, 0, nullptr};
-            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(big.size())), dim3(F_THREADS), 0, ctx->stream, fa);
+            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(big.size())), dim3(CI_THREADS), 0, ctx->stream, fa);
             xh::launch_check("k_chol_inv");
             xh::dev_free(ctx, scratch);
         }

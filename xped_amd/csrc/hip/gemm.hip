@@ -2,10 +2,13 @@
 //
 // The reference multiplies matched sectors one at a time with Eigen/BLAS
 // (src/Core/TensorBase.cpp:262-279 -> MatrixInterface::prod, MatrixInterface_Eigen_impl.cpp:155-160).
-// Here every (sector, 64x64 output tile) pair of one contraction is an entry of a single work
-// list, processed by one persistent launch: workgroups stride through the list, so uneven
-// sector sizes are balanced across the 256 CUs.  Tiles of one sector are spread over workgroups
-// whose blockIdx are congruent mod 8, i.e. they run on the same XCD and share its L2.
+// Here every (sector, 64x64 output tile, K chunk) triple of one contraction is an entry of a
+// single work list, processed by one persistent launch.  The list is cut into 8 contiguous parts
+// of equal work (sum of K over the tiles), one per XCD: workgroup b serves part b % 8, so the
+// tiles of one sector run on one XCD and share its L2, and uneven sector sizes are balanced.
+// Launches with few output tiles (a sketch's Gram matrix: a small output with a long K) split K
+// into chunks that write FP64 partial products to a workspace; a second kernel sums the chunks
+// in a fixed order (deterministic) and applies alpha / beta.
 //
 // Precision modes (operands are always FP64 in memory, the output is FP64):
 //   XH_FP64   v_mfma_f64_16x16x4_f64, FP64 accumulation
@@ -29,49 +32,92 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64;
 constexpr int G_THREADS = 256;
 
+constexpr int NPART = 8;  // XCDs of an MI355X
+
 struct Tile {
-    int32_t item, m0, n0, pad;
+    int32_t item, m0, n0, ks;  // ks: K chunk
+};
+
+// K split of an item: chunks of kc (the last one shorter); split > 1 writes partials to ws.
+struct ItemExt {
+    int32_t split, kc;
+    int64_t ws_off;
 };
 
 struct GemmArgs {
     const xh_gemm_item* items;
+    const ItemExt* ext;
     const Tile* tiles;
-    int32_t ntiles;
+    int32_t part[NPART + 1];  // tile range of each XCD's part
     int32_t ta, tb;
     const double* A;
     const double* B;
     double* C;
+    double* ws;
     double alpha, beta;
 };
 
-__device__ __forceinline__ double load_a(const GemmArgs& g, const xh_gemm_item& it, int i, int kk)
+__device__ __forceinline__ double load_a(const GemmArgs& g, const xh_gemm_item& it, int i, int kk, int kend)
 {
-    if(i >= it.m || kk >= it.k) return 0.0;
+    if(i >= it.m || kk >= kend) return 0.0;
     return g.ta ? g.A[it.a_off + int64_t(i) * it.k + kk] : g.A[it.a_off + int64_t(kk) * it.m + i];
 }
-__device__ __forceinline__ double load_b(const GemmArgs& g, const xh_gemm_item& it, int kk, int j)
+__device__ __forceinline__ double load_b(const GemmArgs& g, const xh_gemm_item& it, int kk, int j, int kend)
 {
-    if(kk >= it.k || j >= it.n) return 0.0;
+    if(kk >= kend || j >= it.n) return 0.0;
     return g.tb ? g.B[it.b_off + int64_t(kk) * it.n + j] : g.B[it.b_off + int64_t(j) * it.k + kk];
+}
+
+// The tile loop of workgroup blockIdx.x: its XCD's part, strided by the workgroups of that XCD.
+struct TileRange {
+    int32_t begin, end, stride;
+};
+__device__ __forceinline__ TileRange my_tiles(const GemmArgs& g)
+{
+    const int x = blockIdx.x % NPART;
+    const int per = (gridDim.x + NPART - 1 - x) / NPART;  // workgroups on this XCD
+    return TileRange{g.part[x] + int32_t(blockIdx.x / NPART), g.part[x + 1], per};
 }
 
 // Output layout of the 16x16 accumulators: for the FP32-accumulating MFMAs register r of lane l
 // holds row 4*(l/16) + r; for v_mfma_f64_16x16x4_f64 it holds row (l/16) + 4*r.  Column = l % 16.
 template <bool F64LAYOUT, typename Acc>
-__device__ __forceinline__ void store_tile(const GemmArgs& g, const xh_gemm_item& it, int m0, int n0, int wm, int wn, int lane, Acc acc[2][2])
+__device__ __forceinline__ void store_tile(const GemmArgs& g, const xh_gemm_item& it, const ItemExt& ex, const Tile& tl, int wm, int wn, int lane,
+                                           Acc acc[2][2])
 {
+    // a K chunk of a split item stores its raw partial product into the workspace
+    const bool part = ex.split > 1;
+    double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
+    const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
     for(int mi = 0; mi < 2; ++mi)
         for(int ni = 0; ni < 2; ++ni)
             for(int r = 0; r < 4; ++r) {
                 const int rr = F64LAYOUT ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                const int row = m0 + wm * 32 + mi * 16 + rr;
-                const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+                const int row = tl.m0 + wm * 32 + mi * 16 + rr;
+                const int col = tl.n0 + wn * 32 + ni * 16 + (lane & 15);
                 if(row < it.m && col < it.n) {
-                    double* p = g.C + it.c_off + int64_t(col) * it.m + row;
+                    double* p = base + int64_t(col) * it.m + row;
                     const double v = static_cast<double>(acc[mi][ni][r]);
-                    *p = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *p;
+                    *p = (beta == 0.0) ? alpha * v : alpha * v + beta * *p;
                 }
             }
+}
+
+// C = alpha * sum of the K chunks + beta * C for the split items (blockIdx.y: split item).
+struct RedItem {
+    int64_t c_off, ws_off, mn;
+    int32_t split, pad;
+};
+__global__ __launch_bounds__(256) void k_splitk_reduce(const RedItem* __restrict__ red, const double* __restrict__ ws, double* __restrict__ C, double alpha,
+                                                       double beta)
+{
+    const RedItem r = red[blockIdx.y];
+    for(int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < r.mn; e += int64_t(gridDim.x) * blockDim.x) {
+        double acc = 0.0;
+        for(int s = 0; s < r.split; ++s) acc += ws[r.ws_off + s * r.mn + e];
+        double* p = C + r.c_off + e;
+        *p = (beta == 0.0) ? alpha * acc : alpha * acc + beta * *p;
+    }
 }
 
 // ---------------------------------------------------------------- FP64
@@ -82,19 +128,22 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
     __shared__ double sB[BK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    for(int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
         const Tile tl = g.tiles[t];
         const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
         d4 acc[2][2];
         for(int a = 0; a < 2; ++a)
             for(int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
-        for(int k0 = 0; k0 < it.k; k0 += BK) {
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 4; ++r) {
                 const int idx = tid + G_THREADS * r;
                 const int i = idx & 63, kk = idx >> 6;
-                sA[kk][i] = load_a(g, it, tl.m0 + i, k0 + kk);
+                sA[kk][i] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
                 const int kb = idx & 15, j = idx >> 4;
-                sB[kb][j] = load_b(g, it, k0 + kb, tl.n0 + j);
+                sB[kb][j] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
             }
             __syncthreads();
             for(int kq = 0; kq < BK; kq += 4) {
@@ -107,7 +156,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile<true>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<true>(g, it, ex, tl, wm, wn, lane, acc);
     }
 }
 
@@ -119,19 +168,22 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f32(GemmArgs g)
     __shared__ float sB[BK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    for(int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
         const Tile tl = g.tiles[t];
         const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
         f4 acc[2][2];
         for(int a = 0; a < 2; ++a)
             for(int b = 0; b < 2; ++b) acc[a][b] = f4{0, 0, 0, 0};
-        for(int k0 = 0; k0 < it.k; k0 += BK) {
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 8; ++r) {
                 const int idx = tid + G_THREADS * r;
                 const int i = idx & 63, kk = idx >> 6;
-                sA[kk][i] = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk));
+                sA[kk][i] = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk, kend));
                 const int kb = idx & 31, j = idx >> 5;
-                sB[kb][j] = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j));
+                sB[kb][j] = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j, kend));
             }
             __syncthreads();
             for(int kq = 0; kq < BK; kq += 4) {
@@ -144,7 +196,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f32(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile<false>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<false>(g, it, ex, tl, wm, wn, lane, acc);
     }
 }
 
@@ -167,22 +219,25 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
     __shared__ unsigned short sBh[BK][BN + 4], sBl[BK][BN + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    for(int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
         const Tile tl = g.tiles[t];
         const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
         f4 acc[2][2];
         for(int a = 0; a < 2; ++a)
             for(int b = 0; b < 2; ++b) acc[a][b] = f4{0, 0, 0, 0};
-        for(int k0 = 0; k0 < it.k; k0 += BK) {
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 8; ++r) {
                 const int idx = tid + G_THREADS * r;
                 const int i = idx & 63, kk = idx >> 6;
-                float a = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk));
+                float a = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk, kend));
                 unsigned short ah = bf16_rn(a);
                 sAh[kk][i] = ah;
                 sAl[kk][i] = bf16_rn(a - bf16_to_f(ah));
                 const int kb = idx & 31, j = idx >> 5;
-                float b = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j));
+                float b = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j, kend));
                 unsigned short bh = bf16_rn(b);
                 sBh[kb][j] = bh;
                 sBl[kb][j] = bf16_rn(b - bf16_to_f(bh));
@@ -214,7 +269,7 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
             }
             __syncthreads();
         }
-        store_tile<false>(g, it, tl.m0, tl.n0, wm, wn, lane, acc);
+        store_tile<false>(g, it, ex, tl, wm, wn, lane, acc);
     }
 }
 
@@ -223,8 +278,12 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
 struct xh_gemm_plan {
     xh::PackedUpload up;
     xh_gemm_item* d_items = nullptr;
+    ItemExt* d_ext = nullptr;
     Tile* d_tiles = nullptr;
-    int32_t ntiles = 0;
+    RedItem* d_red = nullptr;
+    int32_t ntiles = 0, nred = 0;
+    int32_t part[NPART + 1] = {};
+    int64_t ws_doubles = 0, max_red_mn = 0;
     int ta = 0, tb = 0;
     double flops = 0.0;
 };
@@ -235,27 +294,76 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
     auto* plan = new xh_gemm_plan();
     int rc = guard(ctx, [&] {
         std::vector<xh_gemm_item> v(items, items + nitems);
-        // largest K*tiles first so the persistent loop ends with small work
+        // largest K*tiles first so each part's loop ends with small work
         std::vector<int> order(v.size());
         for(size_t i = 0; i < v.size(); ++i) order[i] = static_cast<int>(i);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return double(v[a].m) * v[a].n * v[a].k > double(v[b].m) * v[b].n * v[b].k; });
+        int64_t base_tiles = 0;
+        for(const auto& it : v) {
+            if(it.m < 0 || it.n < 0 || it.k < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative extent");
+            if(it.a_off < 0 || it.b_off < 0 || it.c_off < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative offset");
+            if(it.m > 0 && it.n > 0) base_tiles += int64_t((it.m + BM - 1) / BM) * ((it.n + BN - 1) / BN);
+        }
+        // K split: aim at 4 tiles per CU, chunks of at least 256
+        const int64_t target = 4 * int64_t(std::max(ctx->num_cu, 1));
+        const int64_t S = base_tiles > 0 && base_tiles < target ? (target + base_tiles - 1) / base_tiles : 1;
+        std::vector<ItemExt> ext(v.size(), ItemExt{1, 0, 0});
+        std::vector<RedItem> red;
         std::vector<Tile> tiles;
+        std::vector<double> work;  // K length of each tile
         for(int id : order) {
             const xh_gemm_item& it = v[id];
-            if(it.m < 0 || it.n < 0 || it.k < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative extent");
             if(it.m == 0 || it.n == 0) continue;
             plan->flops += 2.0 * it.m * double(it.n) * it.k;
-            for(int n0 = 0; n0 < it.n; n0 += BN)
-                for(int m0 = 0; m0 < it.m; m0 += BM) tiles.push_back(Tile{id, m0, n0, 0});
+            ItemExt& e = ext[id];
+            e.kc = std::max(it.k, 1);
+            if(S > 1 && it.k >= 512) {
+                int kc = static_cast<int>((it.k + S - 1) / S);
+                kc = std::max(256, (kc + 15) / 16 * 16);
+                const int split = (it.k + kc - 1) / kc;
+                if(split > 1) {
+                    e.split = split;
+                    e.kc = kc;
+                    e.ws_off = plan->ws_doubles;
+                    const int64_t mn = int64_t(it.m) * it.n;
+                    red.push_back(RedItem{it.c_off, e.ws_off, mn, split, 0});
+                    plan->ws_doubles += mn * split;
+                    plan->max_red_mn = std::max(plan->max_red_mn, mn);
+                }
+            }
+            for(int ks = 0; ks < e.split; ++ks) {
+                const double kw = std::min(it.k - ks * e.kc, e.kc) + 8.0;  // + per-tile overhead
+                for(int n0 = 0; n0 < it.n; n0 += BN)
+                    for(int m0 = 0; m0 < it.m; m0 += BM) {
+                        tiles.push_back(Tile{id, m0, n0, ks});
+                        work.push_back(kw);
+                    }
+            }
         }
         if(tiles.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
+        // contiguous parts of equal work, one per XCD
+        double total = 0.0;
+        for(double w : work) total += w;
+        plan->part[0] = 0;
+        {
+            double acc = 0.0;
+            int x = 1;
+            for(size_t t = 0; t < tiles.size() && x < NPART; ++t) {
+                acc += work[t];
+                while(x < NPART && acc >= total * x / NPART) plan->part[x++] = static_cast<int32_t>(t + 1);
+            }
+            for(; x <= NPART; ++x) plan->part[x] = static_cast<int32_t>(tiles.size());
+        }
         plan->ntiles = static_cast<int32_t>(tiles.size());
+        plan->nred = static_cast<int32_t>(red.size());
         plan->ta = trans_a ? 1 : 0;
         plan->tb = trans_b ? 1 : 0;
-        const size_t oi = plan->up.add(v), ot = plan->up.add(tiles);
+        const size_t oi = plan->up.add(v), oe = plan->up.add(ext), ot = plan->up.add(tiles), orr = plan->up.add(red);
         plan->up.upload(ctx);
         plan->d_items = plan->up.ptr<xh_gemm_item>(oi);
+        plan->d_ext = plan->up.ptr<ItemExt>(oe);
         plan->d_tiles = plan->up.ptr<Tile>(ot);
+        plan->d_red = red.empty() ? nullptr : plan->up.ptr<RedItem>(orr);
     });
     if(rc != XH_OK) {
         xh_gemm_plan_destroy(ctx, plan);
@@ -286,8 +394,15 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
     return guard(ctx, [&] {
         if(!plan) throw xh::Error(XH_EINVAL, "null gemm plan");
         if(plan->ntiles == 0) return;
-        GemmArgs g{plan->d_items, plan->d_tiles, plan->ntiles, plan->ta, plan->tb, A, B, C, alpha, beta};
-        const int grid = std::min<int>(plan->ntiles, ctx->num_cu * 8);
+        double* ws = nullptr;
+        if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
+        GemmArgs g{plan->d_items, plan->d_ext, plan->d_tiles, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
+        for(int x = 0; x <= NPART; ++x) g.part[x] = plan->part[x];
+        // a multiple of NPART workgroups, at most 8 per CU
+        int grid = std::min<int>(plan->ntiles, ctx->num_cu * 8);
+        grid = (grid + NPART - 1) / NPART * NPART;
+        const bool timed = ctx->gemm_timing;
+        if(timed) xh::gemm_timer_begin(ctx);
         switch(precision) {
         case XH_FP64: hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(G_THREADS), 0, ctx->stream, g); break;
         case XH_FP32: hipLaunchKernelGGL(k_gemm_f32, dim3(grid), dim3(G_THREADS), 0, ctx->stream, g); break;
@@ -295,5 +410,13 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
         default: throw xh::Error(XH_EINVAL, "unknown precision mode");
         }
         xh::launch_check("k_gemm");
+        if(plan->nred > 0) {
+            const unsigned gx = static_cast<unsigned>(std::min<int64_t>((plan->max_red_mn + 1023) / 1024, 256));
+            hipLaunchKernelGGL(k_splitk_reduce, dim3(std::max(gx, 1u), static_cast<unsigned>(plan->nred)), dim3(256), 0, ctx->stream, plan->d_red, ws, C, alpha,
+                               beta);
+            xh::launch_check("k_splitk_reduce");
+        }
+        if(timed) xh::gemm_timer_end(ctx, plan->flops);
+        if(ws) xh::dev_free(ctx, ws);
     });
 }

@@ -67,10 +67,14 @@ class Runtime:
         t = (C.c_double * 8)()
         lib().xt_rt_host_timers(self.h, t, 8)
         return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value,
-                "breakdown_s": [round(x, 4) for x in t[:5]]}
+                "breakdown_s": [round(x, 4) for x in t[:5]], "symbolic_hits": int(t[5])}
+
+    def marker(self, id: int):
+        """An empty kernel of `id` workgroups on the runtime's stream: a mark in a kernel trace."""
+        check(lib().xt_rt_marker(self.h, int(id)), self.h, "marker")
 
     def gemm_timing(self, enable: bool, reset: bool = False):
-        """Per-launch HIP-event timing of the contraction GEMMs; returns (kernel ms, flops, logged blocks)."""
+        """HIP-event timing of every grouped GEMM launch; returns (kernel ms, flops, logged contraction blocks)."""
         ms, fl, n = C.c_double(), C.c_double(), C.c_long()
         lib().xt_rt_gemm_timing(self.h, int(enable), int(reset), C.byref(ms), C.byref(fl), C.byref(n))
         return ms.value, fl.value, n.value
@@ -352,6 +356,47 @@ class Tensor:
                                        C.byref(ratio)), self.rt.h, "tSVD_randomized")
         return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, bool(safe.value), ratio.value
 
+    # ---- the halves of tSVD and the reverse-mode primitives (include/Xped/AD/ADTensor.hpp)
+    def zeros_like(self) -> "Tensor":
+        return self._unary(lib().xt_tensor_zeros_like)
+
+    def permute_adj(self, shift: int, p: Sequence[int]) -> "Tensor":
+        """Tensor::permute_adj (Tensor.cpp:258-333): the transpose of X -> X.permute(shift, p)."""
+        h = C.c_void_p()
+        check(lib().xt_permute_adj(self.h, shift, _ints(p), len(p), C.byref(h)), self.rt.h, "permute_adj")
+        return self._wrap(h)
+
+    def full_svd_impl(self):
+        """Untruncated (U, Sigma, Vdag) per sector (Tensor.cpp:494-521)."""
+        U, S, V = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib().xt_svd_full(self.h, C.byref(U), C.byref(S), C.byref(V)), self.rt.h, "full_svd_impl")
+        return self._wrap(U), self._wrap(S), self._wrap(V)
+
+    def cutoff_matrices(self, U, S, V, maxKeep: int, eps_svd: float, truncWeight: float = 0.0, entropy: float = 0.0, PRESERVE_MULTIPLETS: bool = True):
+        """cutoff_matrices (Tensor.cpp:523-633) applied to the full factors of this tensor."""
+        Ut, St, Vt = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        tw, en = C.c_double(truncWeight), C.c_double(entropy)
+        check(lib().xt_cutoff(self.h, U.h, S.h, V.h, int(maxKeep), float(eps_svd), int(bool(PRESERVE_MULTIPLETS)), C.byref(Ut), C.byref(St), C.byref(Vt),
+                              C.byref(tw), C.byref(en)), self.rt.h, "cutoff_matrices")
+        return self._wrap(Ut), self._wrap(St), self._wrap(Vt), tw.value, en.value
+
+    def flat_index(self, block: int, row: int, col: int) -> int:
+        """Storage index of block(block)(row, col) (blocks column-major)."""
+        q, off, r, c = self.sectors()[block]
+        if not (0 <= row < r and 0 <= col < c):
+            raise IndexError("coefficient outside the block")
+        return off + row + col * r
+
+    def coeff(self, block: int, row: int, col: int) -> float:
+        return float(self.download()[self.flat_index(block, row, col)])
+
+    def add_at(self, index: int, value: float):
+        """In place: entry `index` of the storage += value (on the device)."""
+        if not 0 <= index < self.size:
+            raise IndexError("index outside the tensor")
+        check(lib().xh_add_at(self.rt.ctx(), C.c_void_p(self.device_ptr()), int(index), float(value)), self.rt.h, "add_at")
+        return self
+
     def tQR(self):
         Q, R = C.c_void_p(), C.c_void_p()
         check(lib().xt_tqr(self.h, C.byref(Q), C.byref(R)), self.rt.h, "tQR")
@@ -409,6 +454,37 @@ class Tensor:
 
     def maxNorm(self) -> float:
         return self._scalar(lib().xt_max_norm)
+
+
+def mul_bwd(A: Tensor, B: Tensor, dC: Tensor, dA: Tensor = None, dB: Tensor = None, precision: str = "fp64"):
+    """operator* reverse pass (ADTensor.hpp:745-746), in place: dA += dC B^dagger, dB += A^dagger dC."""
+    check(lib().xt_mul_bwd(A.h, B.h, dC.h, dA.h if dA is not None else None, dB.h if dB is not None else None, PRECISION[precision]), A.rt.h,
+          "operator* adjoint")
+
+
+def svd_bwd(U: Tensor, S: Tensor, V: Tensor, dU: Tensor, dS: Tensor, dV: Tensor, dA: Tensor):
+    """full_svd_impl reverse pass (ADTensor.hpp:262-320), in place into dA."""
+    check(lib().xt_svd_bwd(U.h, S.h, V.h, dU.h, dS.h, dV.h, dA.h), U.rt.h, "svd adjoint")
+
+
+def cutoff_bwd(dUt: Tensor, dSt: Tensor, dVt: Tensor, dU: Tensor, dS: Tensor, dV: Tensor):
+    """cutoff_matrices reverse pass (ADTensor.hpp:357-370), in place into dU, dS, dV."""
+    check(lib().xt_cutoff_bwd(dUt.h, dSt.h, dVt.h, dU.h, dS.h, dV.h), dUt.rt.h, "cutoff adjoint")
+
+
+def diag_bwd(y: Tensor, dy: Tensor, dx: Tensor, mode: str):
+    """diag_sqrt / diag_inv reverse passes (ADTensor.hpp:560-590), in place into dx."""
+    check(lib().xt_diag_bwd(y.h, dy.h, dx.h, {"sqrt": 0, "inv": 1}[mode]), y.rt.h, "diag adjoint")
+
+
+def trace_bwd(dx: Tensor, g: float):
+    """trace reverse pass (ADTensor.hpp:533-545): dx += g * WeightedIdentity."""
+    check(lib().xt_trace_bwd(dx.h, float(g)), dx.rt.h, "trace adjoint")
+
+
+def max_norm_bwd(x: Tensor, dx: Tensor, g: float):
+    """maxNorm reverse pass (ADTensor.hpp:450-476)."""
+    check(lib().xt_max_norm_bwd(x.h, dx.h, float(g)), x.rt.h, "maxNorm adjoint")
 
 
 def get_permutations(a1, rank1, a2, rank2, rankres):
