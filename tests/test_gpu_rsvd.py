@@ -111,8 +111,9 @@ def test_cholesky_inverse_kernel(rt):
     ctx = rt.ctx()
     r = np.random.default_rng(7)
     blocks = []
-    # n = 141 is the largest block held in LDS; n = 180 and 300 run from global memory
-    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6), (160, 141), (260, 180), (400, 300)]:
+    # n = 199 is the largest block held in LDS (packed upper triangle); n = 200 and 300 run from
+    # a global working copy
+    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6), (160, 141), (260, 180), (230, 199), (240, 200), (400, 300)]:
         Y = r.standard_normal((m, n))
         if n > 2:
             Y[:, 2] = Y[:, 0] - 2.0 * Y[:, 1]  # an exactly dependent column

@@ -1,6 +1,8 @@
 // Device-resident tensor operations: structure on the host, data movement and arithmetic on the GPU.
 #include "tensor.hpp"
 
+#include <cstddef>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -233,28 +235,15 @@ static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStr
             cp->plan.out = si->second;
     }
     if(rt->ctx) {
-        std::vector<xh_permute_task> tasks(cp->plan.tasks.size());
-        for(std::size_t i = 0; i < tasks.size(); ++i) {
-            const PermuteTask& s = cp->plan.tasks[i];
-            xh_permute_task& d = tasks[i];
-            std::memset(&d, 0, sizeof(d));
-            d.dst_off = s.dst_off;
-            d.dst_ld = s.dst_ld;
-            d.rows = s.rows;
-            d.cols = s.cols;
-            d.nsrc = s.nsrc;
-            d.src_begin = s.src_begin;
-            for(int k2 = 0; k2 < cp->plan.nlegs; ++k2) {
-                d.ext[k2] = s.ext[k2];
-                d.sstride[k2] = s.sstride[k2];
-                d.is_col[k2] = s.is_col[k2];
-            }
-        }
-        std::vector<xh_permute_source> srcs(cp->plan.sources.size());
-        for(std::size_t i = 0; i < srcs.size(); ++i)
-            srcs[i] = xh_permute_source{cp->plan.sources[i].src_off, cp->plan.sources[i].src_ld, cp->plan.sources[i].coeff};
-        xh_permute_desc desc{cp->plan.nlegs, cp->plan.new_rank, static_cast<int64_t>(tasks.size()), static_cast<int64_t>(srcs.size()), tasks.data(),
-                             srcs.data()};
+        // the host plan's task and source records have the layout of the C ABI's descriptors
+        static_assert(sizeof(PermuteTask) == sizeof(xh_permute_task) && offsetof(PermuteTask, ext) == offsetof(xh_permute_task, ext) &&
+                          offsetof(PermuteTask, sstride) == offsetof(xh_permute_task, sstride) &&
+                          offsetof(PermuteTask, is_col) == offsetof(xh_permute_task, is_col) && offsetof(PermuteTask, nsrc) == offsetof(xh_permute_task, nsrc),
+                      "PermuteTask must match xh_permute_task");
+        static_assert(sizeof(PermuteSource) == sizeof(xh_permute_source) && offsetof(PermuteSource, coeff) == offsetof(xh_permute_source, coeff),
+                      "PermuteSource must match xh_permute_source");
+        xh_permute_desc desc{cp->plan.nlegs, cp->plan.new_rank, static_cast<int64_t>(cp->plan.tasks.size()), static_cast<int64_t>(cp->plan.sources.size()),
+                             reinterpret_cast<xh_permute_task*>(cp->plan.tasks.data()), reinterpret_cast<xh_permute_source*>(cp->plan.sources.data())};
         lap(1);
         dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
         lap(2);

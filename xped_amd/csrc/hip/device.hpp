@@ -144,6 +144,19 @@ struct PackedUpload {
         if(!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
         return off;
     }
+    // an aligned region of n elements of T, filled through host_ptr before upload
+    template <typename T>
+    size_t reserve(int64_t n)
+    {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + size_t(std::max<int64_t>(n, 0)) * sizeof(T));
+        return off;
+    }
+    template <typename T>
+    T* host_ptr(size_t off)
+    {
+        return reinterpret_cast<T*>(host.data() + off);
+    }
     void upload(xh_ctx* ctx)
     {
         if(host.empty()) return;

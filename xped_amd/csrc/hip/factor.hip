@@ -295,18 +295,23 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
 // column's original diagonal marks a numerically dependent column: its row and column of X are
 // zero, so Y X has a zero column there.
 //
-// One workgroup per block, one barrier per pivot.  The working matrix W is row-major (W[i n + j]);
-// its upper triangle holds G, then R, its strict lower triangle X^T, and sv[k] = 1 / r_kk (0 for
-// a dependent pivot) is the diagonal of X.  Pivot step k runs three disjoint phases:
+// One workgroup per block, one barrier per pivot.  The upper triangle of G (then R) is kept
+// packed column by column, (i, j) at j (j + 1) / 2 + i, in LDS when it fits (n <= 199) and in a
+// global working copy otherwise; X^T is written straight into the output's strict lower triangle
+// (X_ij at U[j + i n]) and reflected at the end; sv[k] = 1 / r_kk (0 for a dependent pivot) is the
+// diagonal of X.  Pivot step k runs three disjoint phases:
 //   (a) row k-1 of R is scaled by sv[k-1] (the division by r_{k-1,k-1}, deferred one step);
-//   (b) right-looking update with the unscaled row k: W_ij -= W_ki W_kj / d_k for k < i <= j;
-//   (c) column k-1 of X from the finished rows/columns < k-1:
+//   (b) right-looking update with the unscaled row k: W_ij -= W_ki W_kj / d_k for k < i <= j,
+//       columns paired (j, t-1-j) so that every thread gets the same number of entries;
+//   (c) column k-1 of X from the finished rows/columns < k-1, 16 lanes per entry:
 //       X_{i,k-1} = -sv[k-1] * sum_{l=i}^{k-2} X_il R_{l,k-1}  (X R = I by columns).
-// (a) writes row k-1 (columns >= k), (b) reads row k and writes rows > k of the upper triangle,
-// (c) reads rows < k-1 and writes row k-1 of the lower triangle, so one barrier per step suffices.
+// (a) writes row k-1 (columns >= k), (b) reads row k and writes rows > k, (c) reads rows < k-1 of
+// R and columns < k-1 of X and writes column k-1 of X, so one barrier per step suffices.
 constexpr int CI_THREADS = 1024;
+constexpr int CI_LANES = 16;  // lanes per dot product in (c)
 
-__host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * n + 2 * n; }
+__host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * (n + 1) / 2 + 2 * n; }
+__device__ __forceinline__ int64_t pk(int i, int j) { return int64_t(j) * (j + 1) / 2 + i; }
 
 __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
 {
@@ -316,18 +321,21 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
     const int n = it.n;
     if(n == 0) return;
     double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
-    double* rd = W + int64_t(n) * n;  // shifted original diagonal (dependence threshold)
-    double* sv = rd + n;              // 1 / r_kk, or 0
+    const int64_t np = int64_t(n) * (n + 1) / 2;
+    double* rd = W + np;  // shifted original diagonal (dependence threshold)
+    double* sv = rd + n;  // 1 / r_kk, or 0
+    double* X = fa.U + it.u_off;
     const int tid = threadIdx.x;
     const double* G = fa.A + it.a_off;
     const int64_t nn = int64_t(n) * n;
-    // upper triangle of G, read coalesced (column-major e = i + j n) into row-major W
     double tr = 0.0;
     for(int64_t e = tid; e < nn; e += CI_THREADS) {
         const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
-        const double g = G[e];
-        if(i <= j) W[int64_t(i) * n + j] = g;
-        if(i == j) tr += g;
+        if(i <= j) {
+            const double g = G[e];
+            W[pk(i, j)] = g;
+            if(i == j) tr += g;
+        }
     }
     for(int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
     if((tid & 63) == 0) s_part[tid >> 6] = tr;
@@ -337,8 +345,8 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
         for(int w = 0; w < CI_THREADS / 64; ++w) t += s_part[w];
         const double shift = fa.S[it.s_off] * t;
         for(int k = tid; k < n; k += CI_THREADS) {
-            const double d = W[int64_t(k) * n + k] + shift;
-            W[int64_t(k) * n + k] = d;
+            const double d = W[pk(k, k)] + shift;
+            W[pk(k, k)] = d;
             rd[k] = d;
         }
     }
@@ -346,7 +354,7 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
         __syncthreads();
         double inv_d = 0.0;
         if(k < n) {
-            const double d = W[int64_t(k) * n + k];
+            const double d = W[pk(k, k)];
             const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
             inv_d = ok ? 1.0 / d : 0.0;
             if(tid == 0) sv[k] = ok ? 1.0 / sqrt(d) : 0.0;  // read from step k + 1 on
@@ -354,58 +362,68 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
         if(k >= 1) {
             // (a) deferred scaling of row k - 1
             const double s = sv[k - 1];
-            double* row = W + int64_t(k - 1) * n;
-            for(int j = k + tid; j < n; j += CI_THREADS) row[j] *= s;
+            for(int j = k + tid; j < n; j += CI_THREADS) W[pk(k - 1, j)] *= s;
         }
         if(k + 1 < n && inv_d != 0.0) {
-            // (b) thread (s, jj): column j = k + 1 + jj, rows k + 1 + s, + c, ... up to j
-            const double* rowk = W + int64_t(k) * n;
+            // (b) columns a = k+1+pp (rows k+1 .. a) and b = n-1-pp (rows k+1 .. b) as one list
             const int t = n - k - 1;
-            if(t <= CI_THREADS) {
-                const int c = CI_THREADS / t;
-                const int sidx = tid / t, jj = tid - sidx * t;
-                if(sidx < c) {
-                    const int j = k + 1 + jj;
-                    const double wkj = rowk[j] * inv_d;
-                    for(int i = k + 1 + sidx; i <= j; i += c) W[int64_t(i) * n + j] -= rowk[i] * wkj;
-                }
-            } else {
-                for(int jj = tid; jj < t; jj += CI_THREADS) {
-                    const int j = k + 1 + jj;
-                    const double wkj = rowk[j] * inv_d;
-                    for(int i = k + 1; i <= j; ++i) W[int64_t(i) * n + j] -= rowk[i] * wkj;
+            const int npairs = (t + 1) / 2;
+            const int c2 = npairs <= CI_THREADS ? CI_THREADS / npairs : 1;
+            const int p0 = npairs <= CI_THREADS ? tid % npairs : tid;
+            const int s0 = npairs <= CI_THREADS ? tid / npairs : 0;
+            if(s0 < c2) {
+                for(int pp = p0; pp < npairs; pp += (npairs <= CI_THREADS ? npairs : CI_THREADS)) {
+                    const int a = pp, b = t - 1 - pp;
+                    const int la = a + 1;
+                    const int len = (a == b) ? la : la + b + 1;
+                    const double wka = W[pk(k, k + 1 + a)] * inv_d, wkb = W[pk(k, k + 1 + b)] * inv_d;
+                    for(int e = s0; e < len; e += c2) {
+                        const bool first = e < la;
+                        const int i = k + 1 + (first ? e : e - la);
+                        const int j = k + 1 + (first ? a : b);
+                        W[pk(i, j)] -= W[pk(k, i)] * (first ? wka : wkb);
+                    }
                 }
             }
         }
         if(k >= 2) {
-            // (c) column j = k - 1 of X, rows i < j; P lanes share one dot product
+            // (c) column j = k - 1 of X, rows i < j; X_il is read from X^T at X[l + i n]
             const int j = k - 1;
-            int P = 1;
-            while(P < 64 && P * 2 * j <= CI_THREADS) P *= 2;
-            const int per = CI_THREADS / P;
+            constexpr int per = CI_THREADS / CI_LANES;
+            const double sj = sv[j];
             for(int base = 0; base < j; base += per) {
-                const int i = base + tid / P, part = tid & (P - 1);
-                double acc = 0.0;
+                const int i = base + tid / CI_LANES, part = tid & (CI_LANES - 1);
+                double acc0 = 0.0, acc1 = 0.0;
                 if(i < j) {
-                    for(int l = i + part; l < j; l += P) {
-                        const double x = (l == i) ? sv[i] : W[int64_t(l) * n + i];
-                        acc = fma(x, W[int64_t(l) * n + j], acc);
+                    int l = i + part;
+                    for(; l + CI_LANES < j; l += 2 * CI_LANES) {
+                        const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
+                        const double x1 = X[l + CI_LANES + int64_t(i) * n];
+                        acc0 = fma(x0, W[pk(l, j)], acc0);
+                        acc1 = fma(x1, W[pk(l + CI_LANES, j)], acc1);
+                    }
+                    if(l < j) {
+                        const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
+                        acc0 = fma(x0, W[pk(l, j)], acc0);
                     }
                 }
-                for(int o = P >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                if(i < j && part == 0) W[int64_t(j) * n + i] = -acc * sv[j];
+                double acc = acc0 + acc1;
+                for(int o = CI_LANES >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                if(i < j && part == 0) X[j + int64_t(i) * n] = -acc * sj;
             }
         }
     }
     __syncthreads();
+    // reflect X^T into the upper triangle, zero the lower one, set the diagonal
     for(int64_t e = tid; e < nn; e += CI_THREADS) {
         const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
-        double x = 0.0;
-        if(i < j)
-            x = W[int64_t(j) * n + i];
-        else if(i == j)
-            x = sv[j];
-        fa.U[it.u_off + e] = x;
+        if(i < j) {
+            const int64_t lo = j + int64_t(i) * n;
+            X[e] = X[lo];
+            X[lo] = 0.0;
+        } else if(i == j) {
+            X[e] = sv[j];
+        }
     }
 }
 

@@ -34,10 +34,14 @@ struct TaskMeta {
     int32_t nsrc, src_begin;
 };
 
+// one output leg of one task: extent, magic division, and the stride of its source leg in the
+// source subblock's row (bit 8 of flags clear) or column (set) index; bits 0-7: division shift
 struct LegInfo {
-    uint32_t ext, magic, shift, pad;
-    int64_t rstride, cstride;
+    uint32_t ext, magic, flags, stride;
 };
+__host__ __device__ __forceinline__ uint32_t leg_shift(const LegInfo& l) { return l.flags & 255u; }
+__host__ __device__ __forceinline__ int64_t leg_rstride(const LegInfo& l) { return (l.flags & 256u) ? 0 : int64_t(l.stride); }
+__host__ __device__ __forceinline__ int64_t leg_cstride(const LegInfo& l) { return (l.flags & 256u) ? int64_t(l.stride) : 0; }
 
 struct PermDev {
     int nlegs, new_rank;
@@ -59,19 +63,19 @@ __device__ __forceinline__ void decode_src(const LegInfo* __restrict__ L, int nl
     scol = 0;
     for(int k = 0; k < new_rank; ++k) {
         const LegInfo li = L[k];
-        uint32_t q = xh::fdiv(row, li.magic, li.shift);
+        uint32_t q = xh::fdiv(row, li.magic, leg_shift(li));
         uint32_t j = row - q * li.ext;
         row = q;
-        srow += int64_t(j) * li.rstride;
-        scol += int64_t(j) * li.cstride;
+        srow += int64_t(j) * leg_rstride(li);
+        scol += int64_t(j) * leg_cstride(li);
     }
     for(int k = new_rank; k < nlegs; ++k) {
         const LegInfo li = L[k];
-        uint32_t q = xh::fdiv(col, li.magic, li.shift);
+        uint32_t q = xh::fdiv(col, li.magic, leg_shift(li));
         uint32_t j = col - q * li.ext;
         col = q;
-        srow += int64_t(j) * li.rstride;
-        scol += int64_t(j) * li.cstride;
+        srow += int64_t(j) * leg_rstride(li);
+        scol += int64_t(j) * leg_cstride(li);
     }
 }
 
@@ -148,11 +152,11 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_transpose(PermDev p, con
         const bool is_row = k < p.new_rank;
         int64_t mult = is_row ? rmul : cmul;
         if(k != 0 && k != u) {
-            uint32_t q = xh::fdiv(rest, li.magic, li.shift);
+            uint32_t q = xh::fdiv(rest, li.magic, leg_shift(li));
             uint32_t j = rest - q * li.ext;
             rest = q;
-            base_srow += int64_t(j) * li.rstride;
-            base_scol += int64_t(j) * li.cstride;
+            base_srow += int64_t(j) * leg_rstride(li);
+            base_scol += int64_t(j) * leg_cstride(li);
             if(is_row)
                 base_row += int64_t(j) * mult;
             else
@@ -181,8 +185,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_transpose(PermDev p, con
         const uint32_t j0 = i0 + a, ju = iu + tx;
         double v = 0.0;
         if(j0 < ti.ext0 && ju < ti.extu) {
-            const int64_t srow = base_srow + int64_t(j0) * l0.rstride + int64_t(ju) * lu.rstride;
-            const int64_t scol = base_scol + int64_t(j0) * l0.cstride + int64_t(ju) * lu.cstride;
+            const int64_t srow = base_srow + int64_t(j0) * leg_rstride(l0) + int64_t(ju) * leg_rstride(lu);
+            const int64_t scol = base_scol + int64_t(j0) * leg_cstride(l0) + int64_t(ju) * leg_cstride(lu);
             v = gather_one(p, tm, src, srow, scol);
         }
         tile[a][tx] = v;
@@ -221,38 +225,30 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         const int NL = desc->nlegs;
         if(NL < 0 || NL > XH_MAXLEGS) throw xh::Error(XH_EINVAL, "permute plan: too many legs");
         if(desc->new_rank < 0 || desc->new_rank > NL) throw xh::Error(XH_EINVAL, "permute plan: invalid new_rank");
-        std::vector<TaskMeta> meta;
-        std::vector<LegInfo> legs;
-        std::vector<int64_t> prefix;
-        std::vector<TrTaskInfo> trinfo;
-        std::vector<int4> trtiles;
-        std::vector<int32_t> gather_ids;  // tasks handled by the gather kernel
+        const int64_t NT = desc->ntasks;
+        if(NT < 0 || NT > 0x7fffffff || desc->nsources < 0) throw xh::Error(XH_EINVAL, "permute plan: invalid task or source count");
+        // pass 1: validate every task and pick its kernel (-1: gather, u > 0: transpose via leg u)
+        std::vector<int32_t> route(static_cast<size_t>(NT));
+        int64_t ngather = 0, ntr = 0, ntr_tiles = 0, total = 0;
         double moved = 0.0, read = 0.0;
-        for(int64_t t = 0; t < desc->ntasks; ++t) {
+        for(int64_t t = 0; t < NT; ++t) {
             const xh_permute_task& tk = desc->tasks[t];
             if(tk.rows < 0 || tk.cols < 0 || tk.rows * tk.cols >= (int64_t(1) << 31))
                 throw xh::Error(XH_EINVAL, "permute plan: subblock too large for 32-bit indexing");
             if(tk.nsrc < 0 || tk.src_begin < 0 || tk.src_begin + tk.nsrc > desc->nsources)
                 throw xh::Error(XH_EINVAL, "permute plan: source range out of bounds");
-            TaskMeta m{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
             int64_t rprod = 1, cprod = 1;
             for(int k = 0; k < NL; ++k) {
                 if(tk.ext[k] == 0) throw xh::Error(XH_EINVAL, "permute plan: zero extent");
-                xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
-                LegInfo li{tk.ext[k], f.magic, f.shift, 0, tk.is_col[k] ? 0 : tk.sstride[k], tk.is_col[k] ? tk.sstride[k] : 0};
-                legs.push_back(li);
-                if(k < desc->new_rank)
-                    rprod *= tk.ext[k];
-                else
-                    cprod *= tk.ext[k];
+                if(tk.sstride[k] < 0 || tk.sstride[k] >= (int64_t(1) << 31)) throw xh::Error(XH_EINVAL, "permute plan: source stride out of range");
+                (k < desc->new_rank ? rprod : cprod) *= tk.ext[k];
             }
             if(rprod != tk.rows || cprod != tk.cols) throw xh::Error(XH_EINVAL, "permute plan: leg extents do not match the subblock shape");
-            meta.push_back(m);
             const double elems = double(tk.rows) * double(tk.cols);
             moved += elems;
             read += elems * tk.nsrc;
-            // route: transpose path when output leg 0 is a real leg (ext >= 16) whose source stride is
-            // not 1 and some other leg with ext >= 16 has unit source stride
+            // transpose path when output leg 0 is a real leg (ext >= 16) whose source stride is not 1
+            // and some other leg with ext >= 16 has unit source stride
             int u = -1;
             if(desc->new_rank >= 1 && NL >= 2 && tk.nsrc >= 1 && tk.sstride[0] != 1 && !tk.is_col[0] && tk.ext[0] >= 16) {
                 for(int k = 1; k < NL; ++k)
@@ -261,73 +257,86 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                         break;
                     }
             }
+            route[t] = u;
             if(u > 0) {
-                TrTaskInfo ti{static_cast<int32_t>(t), u, tk.ext[0], tk.ext[u], 0};
                 int64_t rest = 1;
                 for(int k = 0; k < NL; ++k)
                     if(k != 0 && k != u) rest *= tk.ext[k];
-                ti.rest = rest;
-                const int info_id = static_cast<int>(trinfo.size());
-                trinfo.push_back(ti);
+                ntr_tiles += rest * ((tk.ext[0] + TR_DIM - 1) / TR_DIM) * ((tk.ext[u] + TR_DIM - 1) / TR_DIM);
+                ++ntr;
+            } else {
+                ++ngather;
+                total += tk.rows * tk.cols;
+            }
+        }
+        if(ntr_tiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
+        const int64_t ntiles = (total + PT_TILE - 1) / PT_TILE;
+        if(ntiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
+        // pass 2: write the device tables straight into the upload buffer; gather tasks first,
+        // then the transpose tasks (their metadata follows the gather metadata)
+        const int64_t nmeta = ngather + ntr;
+        xh::PackedUpload& up = plan->up;
+        const size_t o_meta = up.reserve<TaskMeta>(nmeta), o_legs = up.reserve<LegInfo>(nmeta * NL), o_prefix = up.reserve<int64_t>(ngather + 1),
+                     o_tiles = up.reserve<int32_t>(ntiles + 1), o_src = up.reserve<xh_permute_source>(desc->nsources),
+                     o_tr = up.reserve<int4>(ntr_tiles), o_trinfo = up.reserve<TrTaskInfo>(ntr);
+        TaskMeta* meta = up.host_ptr<TaskMeta>(o_meta);
+        LegInfo* legs = up.host_ptr<LegInfo>(o_legs);
+        int64_t* prefix = up.host_ptr<int64_t>(o_prefix);
+        int32_t* tile_first = up.host_ptr<int32_t>(o_tiles);
+        int4* trtiles = up.host_ptr<int4>(o_tr);
+        TrTaskInfo* trinfo = up.host_ptr<TrTaskInfo>(o_trinfo);
+        int64_t gi = 0, ti = 0, tt = 0;
+        if(ngather >= 0) prefix[0] = 0;
+        for(int64_t t = 0; t < NT; ++t) {
+            const xh_permute_task& tk = desc->tasks[t];
+            const int u = route[t];
+            const int64_t idx = u > 0 ? ngather + ti : gi;
+            meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
+            LegInfo* L = legs + idx * NL;
+            for(int k = 0; k < NL; ++k) {
+                const xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
+                L[k] = LegInfo{tk.ext[k], f.magic, f.shift | (tk.is_col[k] ? 256u : 0u), static_cast<uint32_t>(tk.sstride[k])};
+            }
+            if(u > 0) {
+                int64_t rest = 1;
+                for(int k = 0; k < NL; ++k)
+                    if(k != 0 && k != u) rest *= tk.ext[k];
+                trinfo[ti] = TrTaskInfo{static_cast<int32_t>(idx), u, tk.ext[0], tk.ext[u], rest};
                 const int n0 = (tk.ext[0] + TR_DIM - 1) / TR_DIM, nu = (tk.ext[u] + TR_DIM - 1) / TR_DIM;
                 for(int64_t r = 0; r < rest; ++r)
                     for(int a = 0; a < n0; ++a)
-                        for(int b = 0; b < nu; ++b) trtiles.push_back(int4{info_id, b, a, static_cast<int>(r)});
+                        for(int b = 0; b < nu; ++b) trtiles[tt++] = int4{static_cast<int>(ti), b, a, static_cast<int>(r)};
+                ++ti;
             } else {
-                gather_ids.push_back(static_cast<int32_t>(t));
+                prefix[gi + 1] = prefix[gi] + tk.rows * tk.cols;
+                ++gi;
             }
         }
-        // gather tasks: compact metadata in gather order
-        std::vector<TaskMeta> gmeta;
-        std::vector<LegInfo> glegs;
-        prefix.push_back(0);
-        for(int32_t t : gather_ids) {
-            gmeta.push_back(meta[t]);
-            for(int k = 0; k < NL; ++k) glegs.push_back(legs[int64_t(t) * NL + k]);
-            prefix.push_back(prefix.back() + int64_t(meta[t].rows) * meta[t].cols);
-        }
-        const int64_t total = prefix.back();
-        std::vector<int32_t> tile_first;
-        const int64_t ntiles = (total + PT_TILE - 1) / PT_TILE;
+        // gather tiles: first task of each tile, plus a sentinel (last gather task)
         {
-            int t = 0;
-            const int ng = static_cast<int>(gather_ids.size());
+            int64_t t = 0;
             for(int64_t i = 0; i < ntiles; ++i) {
                 const int64_t e = i * PT_TILE;
-                while(t + 1 < ng && prefix[t + 1] <= e) ++t;
-                tile_first.push_back(t);
+                while(t + 1 < ngather && prefix[t + 1] <= e) ++t;
+                tile_first[i] = static_cast<int32_t>(t);
             }
-            // sentinel: last task touching the final tile
-            tile_first.push_back(ng > 0 ? ng - 1 : 0);
-            // per tile upper bound: the first task of the next tile is a valid upper bound for binary search
+            tile_first[ntiles] = static_cast<int32_t>(ngather > 0 ? ngather - 1 : 0);
         }
-        // for the transpose kernel the full metadata table is needed; append it after the gather one
-        const int64_t gcount = static_cast<int64_t>(gmeta.size());
-        std::vector<TaskMeta> allmeta = gmeta;
-        std::vector<LegInfo> alllegs = glegs;
-        for(auto& ti : trinfo) {
-            const int32_t t = ti.task;
-            allmeta.push_back(meta[t]);
-            for(int k = 0; k < NL; ++k) alllegs.push_back(legs[int64_t(t) * NL + k]);
-            ti.task = static_cast<int32_t>(allmeta.size() - 1);
-        }
-        (void)gcount;
-        std::vector<xh_permute_source> srcs(desc->sources, desc->sources + desc->nsources);
+        if(desc->nsources > 0)
+            std::memcpy(up.host_ptr<xh_permute_source>(o_src), desc->sources, size_t(desc->nsources) * sizeof(xh_permute_source));
         plan->dev.nlegs = NL;
         plan->dev.new_rank = desc->new_rank;
-        plan->dev.ntasks = static_cast<int64_t>(allmeta.size());
+        plan->dev.ntasks = nmeta;
         plan->dev.total = total;
-        const size_t o_meta = plan->up.add(allmeta), o_legs = plan->up.add(alllegs), o_prefix = plan->up.add(prefix),
-                     o_tiles = plan->up.add(tile_first), o_src = plan->up.add(srcs), o_tr = plan->up.add(trtiles), o_trinfo = plan->up.add(trinfo);
-        plan->up.upload(ctx);
-        plan->dev.meta = plan->up.ptr<const TaskMeta>(o_meta);
-        plan->dev.legs = plan->up.ptr<const LegInfo>(o_legs);
-        plan->dev.prefix = plan->up.ptr<const int64_t>(o_prefix);
-        plan->dev.tile_first = plan->up.ptr<const int32_t>(o_tiles);
-        plan->dev.src = plan->up.ptr<const xh_permute_source>(o_src);
-        plan->dev.ntr_tiles = static_cast<int64_t>(trtiles.size());
-        plan->dev.tr_tiles = plan->up.ptr<const int4>(o_tr);
-        plan->trinfo = plan->up.ptr<const TrTaskInfo>(o_trinfo);
+        up.upload(ctx);
+        plan->dev.meta = up.ptr<const TaskMeta>(o_meta);
+        plan->dev.legs = up.ptr<const LegInfo>(o_legs);
+        plan->dev.prefix = up.ptr<const int64_t>(o_prefix);
+        plan->dev.tile_first = up.ptr<const int32_t>(o_tiles);
+        plan->dev.src = up.ptr<const xh_permute_source>(o_src);
+        plan->dev.ntr_tiles = ntr_tiles;
+        plan->dev.tr_tiles = up.ptr<const int4>(o_tr);
+        plan->trinfo = up.ptr<const TrTaskInfo>(o_trinfo);
         plan->ngather_tiles = ntiles;
         plan->bytes = 8.0 * (moved + read);
     });
