@@ -18,6 +18,13 @@ beside them.
 --workload c3 times the grouped U(1)xU(1) fermionic contraction A(l1,l2|l3,l4).B(l3,l4|l5,l6)
 (leg dim 192, 313 matched sector GEMMs) instead.
 
+--workload copy is the HBM calibration for counter-based traffic figures: k_axpby with b = 0
+copies 1 GiB (reads 1 GiB, writes 1 GiB) three times; it prints the copy bandwidth.
+
+`roofline.traffic` (HBM bytes per launch of the dominant kernel) is read from the rocprofv3
+counter summary in profiles/ when one is present for the workload (FETCH_SIZE + WRITE_SIZE,
+scaled by the copy calibration), else null.
+
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
@@ -42,7 +49,7 @@ SCALE = 10.0  # site tensors N(0,1) * SCALE / sqrt(plain size), see run_c4
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c4", choices=["c4", "c3"])
+    ap.add_argument("--workload", default="c4", choices=["c4", "c3", "copy"])
     ap.add_argument("--steps", type=int, default=3, help="timed fast-mode steps (c4) or contractions (c3)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed fast-mode steps after the common warm-up (c4)")
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps of the plain FP64 mode (c4)")
@@ -101,7 +108,8 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     plan_ms = 1e3 * ((h1["permute_plan_s"] - h0["permute_plan_s"]) + (h1["gemm_plan_s"] - h0["gemm_plan_s"])) / nsteps
     out = {"ms_per_step": 1e3 * wall / nsteps, "host_plan_ms_per_step": plan_ms,
            "plans_per_step": ((h1["n_permute_plans"] - h0["n_permute_plans"]) + (h1["n_gemm_plans"] - h0["n_gemm_plans"])) / nsteps,
-           "host_breakdown_ms_per_step": [round(1e3 * (b - a) / nsteps, 1) for a, b in zip(h0["breakdown_s"], h1["breakdown_s"])], "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
+           "host_breakdown_ms_per_step": [round(1e3 * (b - a) / nsteps, 1) for a, b in zip(h0["breakdown_s"], h1["breakdown_s"])],
+           "symbolic_plan_hits_per_step": (h1["symbolic_hits"] - h0["symbolic_hits"]) / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
     if mode == "fast":
@@ -112,6 +120,7 @@ def run_mode(mode, args, xt, configs, random_ipeps):
         c.step()
         rt.marker(2)
         ms, fl, nlog = rt.gemm_timing(False)
+        out["gemm_launches"] = rt.gemm_launches()
         out["gemm_kernel_ms"] = ms
         out["gemm_timed_flops"] = fl
         out["gemm_launches_blocks"] = nlog
@@ -181,9 +190,11 @@ def run_c4(args):
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
-                 "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"]},
-        "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5), "traffic": None,
-                     "kernel": f"k_gemm_{'f64' if args.precision == 'fp64' else args.precision}",
+                 "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
+        "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5),
+                     "traffic": measured_traffic("k_gemm_f64_ctm_step_counters.md") if args.precision == "fp64" and args.chi == 256 else None,
+                     "kernel": "k_gemm_f64 + k_gemm_f64_128 (+ k_splitk_reduce)" if args.precision == "fp64" else f"k_gemm_{args.precision}",
+                     "launches_per_step": fast["gemm_launches"], "flops_per_step": fast["gemm_timed_flops"],
                      "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},
     }
     if ref is not None:
@@ -310,12 +321,57 @@ def run_c3(args):
     return res
 
 
+def run_copy(args):
+    """HBM copy calibration: y = 2 x over 1 GiB with k_axpby (b = 0 does not read y)."""
+    import ctypes as C
+
+    import xped_amd.tensor as xt
+    from xped_amd._lib import lib
+
+    rt = xt.Runtime(0)
+    L, ctx = lib(), rt.ctx()
+    n = 1 << 27
+    x, y = C.c_void_p(), C.c_void_p()
+    for d in (x, y):
+        if L.xh_alloc(ctx, 8 * n, C.byref(d)) != 0:
+            raise RuntimeError("allocation failed")
+    L.xh_fill(ctx, x, n, 1.0)
+    L.xh_axpby(ctx, y, x, n, 2.0, 0.0)  # warm-up
+    rt.synchronize()
+    steps = 3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.xh_axpby(ctx, y, x, n, 2.0, 0.0)
+    rt.synchronize()
+    el = time.perf_counter() - t0
+    gbs = 2 * 8 * n * steps / el / 1e9
+    L.xh_free(ctx, x)
+    L.xh_free(ctx, y)
+    return {"metric": "HBM copy bandwidth (k_axpby, b = 0)", "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+            "ms_per_step": round(1e3 * el / steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (constant fill)", "config": {"workload": "copy of 2^27 doubles (1 GiB read + 1 GiB written per launch)"},
+            "bytes_per_launch": 2 * 8 * n}
+
+
+def measured_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary (a line
+    `traffic_bytes_per_launch: <value>` in profiles/<name>), or None."""
+    path = ROOT / "profiles" / name
+    try:
+        for line in path.read_text().splitlines():
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].strip())
+    except OSError:
+        return None
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (the multi-GPU sharding is a design note only)"}))
         sys.exit(2)
-    res = run_c4(args) if args.workload == "c4" else run_c3(args)
+    res = {"c4": run_c4, "c3": run_c3, "copy": run_copy}[args.workload](args)
     print(json.dumps(res))
 
 

@@ -95,9 +95,10 @@ def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
     L = lib()
     ctx = rt.ctx()
     r = np.random.default_rng(5)
-    # the last three have K >= 512 and few output tiles: the plan splits their K into chunks
+    # (40, 30, 1500), (130, 70, 2048), (5, 3, 777) have a long K and few output tiles: the plan
+    # splits their K into chunks; (260, 300, 70) and (193, 450, 33) take the 128 x 128 FP64 tiles
     shapes = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (65, 63, 17), (130, 7, 200), (8, 300, 33), (200, 150, 129), (3, 3, 0),
-              (40, 30, 1500), (130, 70, 2048), (5, 3, 777)]
+              (40, 30, 1500), (130, 70, 2048), (5, 3, 777), (260, 300, 70), (193, 450, 33)]
     items, ao, bo, co = [], 0, 0, 0
     for m, n, k in shapes:
         items.append({"a": ao, "b": bo, "c": co, "m": m, "n": n, "k": k})

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <numeric>
 #include <sstream>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 
@@ -302,6 +304,57 @@ bool Qbasis::equals(const Qbasis& o) const
     return true;
 }
 
+namespace {
+
+// The coupled basis of a list of legs depends on their dimensions only through the tree and
+// entry dimensions: the trees, their order and the entry order follow from the charges (combine
+// sorts trees by charges and entries by q).  Coupled bases are therefore kept per charge pattern
+// and re-dimensioned for legs with the same charges and other dimensions.
+std::string charge_pattern(const Sym& s, const std::vector<Qbasis>& legs)
+{
+    std::string k;
+    auto put = [&k](long long v) { k.append(reinterpret_cast<const char*>(&v), sizeof(v)); };
+    put(s.kind[0]);
+    put(s.kind[1]);
+    put(static_cast<long long>(legs.size()));
+    for(const auto& b : legs) {
+        put(b.conj_flag ? 1 : 0);
+        put(b.depth);
+        put(static_cast<long long>(b.data.size()));
+        for(const auto& e : b.data) {
+            put(e.q.v[0]);
+            put(e.q.v[1]);
+            put(e.trees.empty() ? 0 : (e.trees[0].dual[0] ? 1 : 0));
+        }
+    }
+    return k;
+}
+
+std::mutex pattern_mutex;
+std::unordered_map<std::string, std::shared_ptr<const Qbasis>> pattern_cache;
+
+Qbasis redimension(const Qbasis& tmpl, const std::vector<Qbasis>& legs)
+{
+    Qbasis b = tmpl;
+    std::size_t acc = 0;
+    for(auto& e : b.data) {
+        std::size_t d = 0;
+        for(auto& t : e.trees) {
+            for(int i = 0; i < t.rank; ++i) t.dims[i] = static_cast<std::uint32_t>(legs[static_cast<std::size_t>(i)].inner_dim(t.unc[i]));
+            t.compute_dim();
+            d += t.dim;
+        }
+        e.dim = d;
+        e.num = acc;
+        acc += d;
+    }
+    b.curr_dim = acc;
+    b.tree_index.reset();
+    return b;
+}
+
+}  // namespace
+
 Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled)
 {
     if(uncoupled.empty()) {
@@ -309,8 +362,25 @@ Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled)
         b.push_back(s.vacuum(), 1);
         return b;
     }
+    if(uncoupled.size() == 1) return uncoupled[0];
+    bool depth1 = true;
+    for(const auto& b : uncoupled) depth1 = depth1 && b.depth == 1;
+    std::string key;
+    if(depth1) {
+        key = charge_pattern(s, uncoupled);
+        std::lock_guard<std::mutex> lk(pattern_mutex);
+        auto it = pattern_cache.find(key);
+        if(it != pattern_cache.end()) return redimension(*it->second, uncoupled);
+    }
     Qbasis b = uncoupled[0];
     for(std::size_t i = 1; i < uncoupled.size(); ++i) b = b.combine(uncoupled[i]);
+    if(depth1) {
+        std::lock_guard<std::mutex> lk(pattern_mutex);
+        if(pattern_cache.size() > 8192) pattern_cache.clear();
+        auto tmpl = std::make_shared<Qbasis>(b);
+        tmpl->tree_index.reset();
+        pattern_cache.emplace(key, std::move(tmpl));
+    }
     return b;
 }
 
@@ -709,6 +779,33 @@ std::vector<std::size_t> tree_offsets(const Qbasis& b, const Q& q)
 }
 }  // namespace
 
+namespace {
+// run f(begin, end) over [0, n) on up to 16 host threads when n is large
+template <class F>
+void parallel_ranges(std::size_t n, std::size_t min_per_thread, F&& f)
+{
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const std::size_t nt = std::min<std::size_t>(hw, std::max<std::size_t>(1, n / std::max<std::size_t>(min_per_thread, 1)));
+    if(nt <= 1) {
+        f(std::size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> errs(nt);
+    for(std::size_t k = 0; k < nt; ++k)
+        th.emplace_back([&, k] {
+            try {
+                f(n * k / nt, n * (k + 1) / nt);
+            } catch(...) {
+                errs[k] = std::current_exception();
+            }
+        });
+    for(auto& t : th) t.join();
+    for(auto& e : errs)
+        if(e) std::rethrow_exception(e);
+}
+}  // namespace
+
 PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructure& in, const StructPtr& outp, const std::vector<int>& p)
 {
     const TensorStructure& out = *outp;
@@ -720,59 +817,67 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
     plan.new_rank = newR;
     plan.out = outp;
     plan.tasks.resize(sp.src_begin.size());
-    for(std::size_t s = 0; s < out.sector.size(); ++s) {
-        const Q& q = out.sector[s];
-        const auto& dtrees = out.dom.tree(q);
-        const auto& ctrees = out.cod.tree(q);
-        if(static_cast<std::int32_t>(ctrees.size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
-        std::size_t roff = 0;
-        for(std::size_t a = 0; a < dtrees.size(); ++a) {
-            const FTree& t1 = dtrees[a];
-            std::size_t coff = 0;
-            for(std::size_t b = 0; b < ctrees.size(); ++b) {
-                const FTree& t2 = ctrees[b];
-                const std::size_t id = static_cast<std::size_t>(sp.task_base[s]) + a * ctrees.size() + b;
-                PermuteTask& task = plan.tasks[id];
-                task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
-                task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
-                task.rows = static_cast<std::int64_t>(t1.dim);
-                task.cols = static_cast<std::int64_t>(t2.dim);
-                for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
-                // strides of the corresponding source legs (source leg m = p[k])
-                for(int k = 0; k < N; ++k) {
-                    const int m = p[k];
-                    std::int64_t st = 1;
-                    if(m < R) {
-                        for(int mm = 0; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
-                        task.is_col[k] = 0;
-                    } else {
-                        for(int mm = R; mm < m; ++mm) st *= task.ext[Pfull.pi_inv[mm]];
-                        task.is_col[k] = 1;
-                    }
-                    task.sstride[k] = st;
-                }
-                task.src_begin = sp.src_begin[id];
-                task.nsrc = sp.nsrc[id];
-                plan.moved_elems += t1.dim * t2.dim;
-                coff += t2.dim;
-            }
-            roff += t1.dim;
-        }
+    for(std::size_t s = 0; s < out.sector.size(); ++s)
+        if(static_cast<std::int32_t>(out.cod.tree(out.sector[s]).size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
+    // source stride of output leg k = product of the output extents of the source legs before p[k]
+    std::array<std::array<int, 2 * MAXR>, 2 * MAXR> before{};
+    std::array<int, 2 * MAXR> nbefore{};
+    for(int k = 0; k < N; ++k) {
+        const int m = p[k];
+        const int lo = m < R ? 0 : R;
+        for(int mm = lo; mm < m; ++mm) before[k][nbefore[k]++] = Pfull.pi_inv[mm];
     }
+    std::vector<std::size_t> moved(out.sector.size(), 0);
+    parallel_ranges(out.sector.size(), 2, [&](std::size_t s0, std::size_t s1) {
+        for(std::size_t s = s0; s < s1; ++s) {
+            const Q& q = out.sector[s];
+            const auto& dtrees = out.dom.tree(q);
+            const auto& ctrees = out.cod.tree(q);
+            std::size_t roff = 0;
+            for(std::size_t a = 0; a < dtrees.size(); ++a) {
+                const FTree& t1 = dtrees[a];
+                std::size_t coff = 0;
+                for(std::size_t b = 0; b < ctrees.size(); ++b) {
+                    const FTree& t2 = ctrees[b];
+                    const std::size_t id = static_cast<std::size_t>(sp.task_base[s]) + a * ctrees.size() + b;
+                    PermuteTask& task = plan.tasks[id];
+                    task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
+                    task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
+                    task.rows = static_cast<std::int64_t>(t1.dim);
+                    task.cols = static_cast<std::int64_t>(t2.dim);
+                    for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
+                    for(int k = 0; k < N; ++k) {
+                        std::int64_t st = 1;
+                        for(int u = 0; u < nbefore[k]; ++u) st *= task.ext[before[k][u]];
+                        task.sstride[k] = st;
+                        task.is_col[k] = p[k] < R ? 0 : 1;
+                    }
+                    task.src_begin = sp.src_begin[id];
+                    task.nsrc = sp.nsrc[id];
+                    moved[s] += t1.dim * t2.dim;
+                    coff += t2.dim;
+                }
+                roff += t1.dim;
+            }
+        }
+    });
+    for(std::size_t m : moved) plan.moved_elems += m;
     std::vector<std::vector<std::size_t>> in_roff(in.sector.size()), in_coff(in.sector.size());
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
         in_roff[i] = tree_offsets(in.dom, in.sector[i]);
         in_coff[i] = tree_offsets(in.cod, in.sector[i]);
     }
     plan.sources.resize(sp.sources.size());
-    for(std::size_t k = 0; k < sp.sources.size(); ++k) {
-        const auto& src = sp.sources[k];
-        const std::size_t i = static_cast<std::size_t>(src.sector);
-        if(i >= in.sector.size() || static_cast<std::size_t>(src.t1) >= in_roff[i].size() || static_cast<std::size_t>(src.t2) >= in_coff[i].size())
-            throw std::logic_error("permute: symbolic plan does not match the structure");
-        plan.sources[k] = PermuteSource{static_cast<std::int64_t>(in.offsets[i] + in_roff[i][src.t1] + in_coff[i][src.t2] * in.rows[i]),
-                                        static_cast<std::int64_t>(in.rows[i]), src.coeff};
-    }
+    parallel_ranges(sp.sources.size(), 16384, [&](std::size_t k0, std::size_t k1) {
+        for(std::size_t k = k0; k < k1; ++k) {
+            const auto& src = sp.sources[k];
+            const std::size_t i = static_cast<std::size_t>(src.sector);
+            if(i >= in.sector.size() || static_cast<std::size_t>(src.t1) >= in_roff[i].size() || static_cast<std::size_t>(src.t2) >= in_coff[i].size())
+                throw std::logic_error("permute: symbolic plan does not match the structure");
+            plan.sources[k] = PermuteSource{static_cast<std::int64_t>(in.offsets[i] + in_roff[i][src.t1] + in_coff[i][src.t2] * in.rows[i]),
+                                            static_cast<std::int64_t>(in.rows[i]), src.coeff};
+        }
+    });
     return plan;
 }
 

@@ -54,6 +54,7 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     hipSetDevice(ctx->device);
     if(ctx->stream) hipStreamSynchronize(ctx->stream);
     xh::collect_staging(ctx, true);
+    xh::pinned_destroy_all(ctx);
     for(auto& ev : ctx->events)
         if(ev) hipEventDestroy(ev);
     for(auto& ev : ctx->gemm_events) hipEventDestroy(ev);

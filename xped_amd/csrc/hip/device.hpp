@@ -21,8 +21,16 @@ struct xh_ctx {
     double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
+    int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 (4 waves) for large sectors, 1 64x64 only, 2 128x128 (8 waves)
     // host staging buffers of asynchronous uploads, kept until the event after their copy fires
     std::vector<std::pair<hipEvent_t, std::vector<char>>> pending_host;
+    // pinned host staging buffers for large plan uploads: free ones, and ones whose copy may still run
+    struct PinnedBuf {
+        void* p = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+    };
+    std::vector<PinnedBuf> pinned_free, pinned_busy;
     // HIP-event timing of every grouped GEMM launch (xh_gemm_timing): event pairs in launch order
     bool gemm_timing = false;
     std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
@@ -121,20 +129,87 @@ inline void dev_free(xh_ctx* ctx, void* p)
     if(p) hipFreeAsync(p, ctx->stream);
 }
 
+// Pinned staging buffers (hipHostMalloc), reused once the copy that read them has completed.
+inline void pinned_reclaim(xh_ctx* ctx)
+{
+    auto& busy = ctx->pinned_busy;
+    std::size_t keep = 0;
+    for(std::size_t i = 0; i < busy.size(); ++i) {
+        if(hipEventQuery(busy[i].ev) == hipSuccess)
+            ctx->pinned_free.push_back(busy[i]);
+        else
+            busy[keep++] = busy[i];
+    }
+    busy.resize(keep);
+}
+inline xh_ctx::PinnedBuf pinned_get(xh_ctx* ctx, size_t bytes)
+{
+    pinned_reclaim(ctx);
+    size_t cap = size_t(1) << 20;
+    while(cap < bytes) cap <<= 1;
+    auto& fr = ctx->pinned_free;
+    for(std::size_t i = 0; i < fr.size(); ++i)
+        if(fr[i].cap == cap) {
+            xh_ctx::PinnedBuf b = fr[i];
+            fr[i] = fr.back();
+            fr.pop_back();
+            return b;
+        }
+    // keep at most 16 idle buffers
+    while(fr.size() >= 16) {
+        hipHostFree(fr.back().p);
+        hipEventDestroy(fr.back().ev);
+        fr.pop_back();
+    }
+    xh_ctx::PinnedBuf b;
+    b.cap = cap;
+    check(hipHostMalloc(&b.p, cap, hipHostMallocDefault), "hipHostMalloc(staging)");
+    check(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming), "hipEventCreate");
+    return b;
+}
+// the buffer becomes reusable after everything enqueued so far on the stream (its copy) is done
+inline void pinned_put(xh_ctx* ctx, const xh_ctx::PinnedBuf& b)
+{
+    check(hipEventRecord(b.ev, ctx->stream), "hipEventRecord");
+    ctx->pinned_busy.push_back(b);
+}
+inline void pinned_destroy_all(xh_ctx* ctx)
+{
+    for(auto* v : {&ctx->pinned_free, &ctx->pinned_busy})
+        for(auto& b : *v) {
+            hipHostFree(b.p);
+            hipEventDestroy(b.ev);
+        }
+    ctx->pinned_free.clear();
+    ctx->pinned_busy.clear();
+}
+
 // Host arrays packed into one device allocation and sent with one asynchronous copy on the
 // context stream, without a host synchronisation.  The host copy is kept until release(), which
-// first waits for the event recorded after the copy.
+// first waits for the event recorded after the copy.  Large uploads whose size is known up front
+// (begin_pinned) are written straight into a pinned staging buffer that returns to the context's
+// pool when the copy has completed.
 struct PackedUpload {
     std::vector<char> host;
     void* dev = nullptr;
     hipEvent_t done = nullptr;
     xh_ctx* owner = nullptr;
+    xh_ctx::PinnedBuf pin{};
+    size_t pin_used = 0;
     PackedUpload() = default;
     PackedUpload(const PackedUpload&) = delete;
     PackedUpload& operator=(const PackedUpload&) = delete;
     ~PackedUpload()
     {
+        if(owner && pin.p) pinned_put(owner, pin);  // error path before the upload
+        pin.p = nullptr;
         if(owner && (dev || done)) release(owner);  // error paths: wait and free
+    }
+    void begin_pinned(xh_ctx* ctx, size_t bytes)
+    {
+        owner = ctx;
+        pin = pinned_get(ctx, bytes);
+        pin_used = 0;
     }
     template <typename T>
     size_t add(const std::vector<T>& v)
@@ -148,17 +223,41 @@ struct PackedUpload {
     template <typename T>
     size_t reserve(int64_t n)
     {
+        const size_t bytes = size_t(std::max<int64_t>(n, 0)) * sizeof(T);
+        if(pin.p) {
+            const size_t off = (pin_used + 255) & ~size_t(255);
+            if(off + bytes > pin.cap) throw Error(XH_EINVAL, "staging buffer too small");
+            pin_used = off + bytes;
+            return off;
+        }
         const size_t off = (host.size() + 255) & ~size_t(255);
-        host.resize(off + size_t(std::max<int64_t>(n, 0)) * sizeof(T));
+        host.resize(off + bytes);
         return off;
+    }
+    // bytes that reserve() regions of the given sizes take, with their alignment
+    static size_t packed_bytes(std::initializer_list<size_t> sizes)
+    {
+        size_t t = 0;
+        for(size_t b : sizes) t = ((t + 255) & ~size_t(255)) + b;
+        return t;
     }
     template <typename T>
     T* host_ptr(size_t off)
     {
-        return reinterpret_cast<T*>(host.data() + off);
+        return reinterpret_cast<T*>((pin.p ? static_cast<char*>(pin.p) : host.data()) + off);
     }
     void upload(xh_ctx* ctx)
     {
+        if(pin.p) {
+            owner = ctx;
+            if(pin_used > 0) {
+                check(hipMallocAsync(&dev, pin_used, ctx->stream), "hipMallocAsync(plan)");
+                check(hipMemcpyAsync(dev, pin.p, pin_used, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
+            }
+            pinned_put(ctx, pin);
+            pin.p = nullptr;
+            return;
+        }
         if(host.empty()) return;
         owner = ctx;
         check(hipMallocAsync(&dev, host.size(), ctx->stream), "hipMallocAsync(plan)");

@@ -2,8 +2,9 @@
 //
 // The reference multiplies matched sectors one at a time with Eigen/BLAS
 // (src/Core/TensorBase.cpp:262-279 -> MatrixInterface::prod, MatrixInterface_Eigen_impl.cpp:155-160).
-// Here every (sector, 64x64 output tile, K chunk) triple of one contraction is an entry of a
-// single work list, processed by one persistent launch.  The list is cut into 8 contiguous parts
+// Here every (sector, output tile, K chunk) triple of one contraction is an entry of a work list
+// processed by one persistent launch (FP64: sectors with both output sides >= 192 in 128 x 128
+// tiles by one launch, the others in 64 x 64 tiles by a second).  The list is cut into 8 contiguous parts
 // of equal work (sum of K over the tiles), one per XCD: workgroup b serves part b % 8, so the
 // tiles of one sector run on one XCD and share its L2, and uneven sector sizes are balanced.
 // Launches with few output tiles (a sketch's Gram matrix: a small output with a long K) split K
@@ -17,6 +18,7 @@
 //             A*B ~ hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x16_bf16, FP32 accumulation
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "device.hpp"
 
@@ -139,10 +141,11 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
             for(int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
         for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 4; ++r) {
+                // lanes run along the contiguous direction of each operand in memory
                 const int idx = tid + G_THREADS * r;
-                const int i = idx & 63, kk = idx >> 6;
+                const int i = g.ta ? idx >> 4 : idx & 63, kk = g.ta ? idx & 15 : idx >> 6;
                 sA[kk][i] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
-                const int kb = idx & 15, j = idx >> 4;
+                const int kb = g.tb ? idx >> 6 : idx & 15, j = g.tb ? idx & 63 : idx >> 4;
                 sB[kb][j] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
             }
             __syncthreads();
@@ -157,6 +160,144 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
             __syncthreads();
         }
         store_tile<true>(g, it, ex, tl, wm, wn, lane, acc);
+    }
+}
+
+// ---------------------------------------------------------------- FP64, 128 x 128 tiles
+// For items with both output sides >= 192: 4 waves in 2 x 2, each owning 64 x 64 (4 x 4 MFMA
+// blocks, 16 accumulators), K steps of 16 staged in LDS, and the next K step's operands fetched
+// into registers while the matrix cores work on the current one.
+constexpr int BM2 = 128, BN2 = 128;
+constexpr int BIG_MIN = 192;
+
+__global__ __launch_bounds__(G_THREADS) void k_gemm_f64_128(GemmArgs g)
+{
+    constexpr int BK = 16;
+    __shared__ double sA[BK][BM2 + 1];
+    __shared__ double sB[BK][BN2 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
+        const Tile tl = g.tiles[t];
+        const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
+        d4 acc[4][4];
+        for(int a = 0; a < 4; ++a)
+            for(int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
+        double ra[8], rb[8];
+        auto fetch = [&](int k0) {
+            for(int r = 0; r < 8; ++r) {
+                const int idx = tid + G_THREADS * r;
+                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
+                ra[r] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
+                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
+                rb[r] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
+            }
+        };
+        if(kbeg < kend) fetch(kbeg);
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
+            __syncthreads();  // every wave is done with the previous K step (or tile)
+            for(int r = 0; r < 8; ++r) {
+                const int idx = tid + G_THREADS * r;
+                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
+                sA[kk][i] = ra[r];
+                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
+                sB[kb][j] = rb[r];
+            }
+            __syncthreads();
+            if(k0 + BK < kend) fetch(k0 + BK);
+            for(int kq = 0; kq < BK; kq += 4) {
+                const int kr = kq + (lane >> 4);
+                double av[4], bv[4];
+                for(int mi = 0; mi < 4; ++mi) av[mi] = sA[kr][wm * 64 + mi * 16 + (lane & 15)];
+                for(int ni = 0; ni < 4; ++ni) bv[ni] = sB[kr][wn * 64 + ni * 16 + (lane & 15)];
+                for(int mi = 0; mi < 4; ++mi)
+                    for(int ni = 0; ni < 4; ++ni) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+            }
+        }
+        const bool part = ex.split > 1;
+        double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
+        const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
+        for(int mi = 0; mi < 4; ++mi)
+            for(int ni = 0; ni < 4; ++ni)
+                for(int r = 0; r < 4; ++r) {
+                    const int row = tl.m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * r;
+                    const int col = tl.n0 + wn * 64 + ni * 16 + (lane & 15);
+                    if(row < it.m && col < it.n) {
+                        double* p = base + int64_t(col) * it.m + row;
+                        const double v = acc[mi][ni][r];
+                        *p = (beta == 0.0) ? alpha * v : alpha * v + beta * *p;
+                    }
+                }
+    }
+}
+
+// 128 x 128 tiles with 8 waves (2 x 4), each owning 64 x 32 (4 x 2 MFMA blocks): half the
+// accumulators per wave of the 4-wave kernel, so two waves share each SIMD.
+constexpr int G8_THREADS = 512;
+__global__ __launch_bounds__(G8_THREADS) void k_gemm_f64_128w8(GemmArgs g)
+{
+    constexpr int BK = 16;
+    __shared__ double sA[BK][BM2 + 1];
+    __shared__ double sB[BK][BN2 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;  // wn in 0..3
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
+        const Tile tl = g.tiles[t];
+        const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
+        d4 acc[4][2];
+        for(int a = 0; a < 4; ++a)
+            for(int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
+        double ra[4], rb[4];
+        auto fetch = [&](int k0) {
+            for(int r = 0; r < 4; ++r) {
+                const int idx = tid + G8_THREADS * r;
+                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
+                ra[r] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
+                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
+                rb[r] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
+            }
+        };
+        if(kbeg < kend) fetch(kbeg);
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
+            __syncthreads();
+            for(int r = 0; r < 4; ++r) {
+                const int idx = tid + G8_THREADS * r;
+                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
+                sA[kk][i] = ra[r];
+                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
+                sB[kb][j] = rb[r];
+            }
+            __syncthreads();
+            if(k0 + BK < kend) fetch(k0 + BK);
+            for(int kq = 0; kq < BK; kq += 4) {
+                const int kr = kq + (lane >> 4);
+                double av[4], bv[2];
+                for(int mi = 0; mi < 4; ++mi) av[mi] = sA[kr][wm * 64 + mi * 16 + (lane & 15)];
+                for(int ni = 0; ni < 2; ++ni) bv[ni] = sB[kr][wn * 32 + ni * 16 + (lane & 15)];
+                for(int mi = 0; mi < 4; ++mi)
+                    for(int ni = 0; ni < 2; ++ni) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+            }
+        }
+        const bool part = ex.split > 1;
+        double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
+        const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
+        for(int mi = 0; mi < 4; ++mi)
+            for(int ni = 0; ni < 2; ++ni)
+                for(int r = 0; r < 4; ++r) {
+                    const int row = tl.m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * r;
+                    const int col = tl.n0 + wn * 32 + ni * 16 + (lane & 15);
+                    if(row < it.m && col < it.n) {
+                        double* p = base + int64_t(col) * it.m + row;
+                        const double v = acc[mi][ni][r];
+                        *p = (beta == 0.0) ? alpha * v : alpha * v + beta * *p;
+                    }
+                }
     }
 }
 
@@ -180,9 +321,9 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f32(GemmArgs g)
         for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 8; ++r) {
                 const int idx = tid + G_THREADS * r;
-                const int i = idx & 63, kk = idx >> 6;
+                const int i = g.ta ? idx >> 5 : idx & 63, kk = g.ta ? idx & 31 : idx >> 6;
                 sA[kk][i] = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk, kend));
-                const int kb = idx & 31, j = idx >> 5;
+                const int kb = g.tb ? idx >> 6 : idx & 31, j = g.tb ? idx & 63 : idx >> 5;
                 sB[kb][j] = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j, kend));
             }
             __syncthreads();
@@ -231,12 +372,12 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
         for(int k0 = kbeg; k0 < kend; k0 += BK) {
             for(int r = 0; r < 8; ++r) {
                 const int idx = tid + G_THREADS * r;
-                const int i = idx & 63, kk = idx >> 6;
+                const int i = g.ta ? idx >> 5 : idx & 63, kk = g.ta ? idx & 31 : idx >> 6;
                 float a = static_cast<float>(load_a(g, it, tl.m0 + i, k0 + kk, kend));
                 unsigned short ah = bf16_rn(a);
                 sAh[kk][i] = ah;
                 sAl[kk][i] = bf16_rn(a - bf16_to_f(ah));
-                const int kb = idx & 31, j = idx >> 5;
+                const int kb = g.tb ? idx >> 6 : idx & 31, j = g.tb ? idx & 63 : idx >> 5;
                 float b = static_cast<float>(load_b(g, it, k0 + kb, tl.n0 + j, kend));
                 unsigned short bh = bf16_rn(b);
                 sBh[kb][j] = bh;
@@ -275,18 +416,44 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_bf16x3(GemmArgs g)
 
 }  // namespace
 
+// A work list of tiles cut into per-XCD parts.
+struct TileList {
+    Tile* d_tiles = nullptr;
+    int32_t ntiles = 0;
+    int32_t part[NPART + 1] = {};
+};
+
 struct xh_gemm_plan {
     xh::PackedUpload up;
     xh_gemm_item* d_items = nullptr;
     ItemExt* d_ext = nullptr;
-    Tile* d_tiles = nullptr;
     RedItem* d_red = nullptr;
-    int32_t ntiles = 0, nred = 0;
-    int32_t part[NPART + 1] = {};
+    TileList all64;           // every item in 64 x 64 tiles (FP32 / BF16x3 modes)
+    TileList big128, rest64;  // FP64: items with both sides >= BIG_MIN in 128 x 128 tiles, the others in 64 x 64
+    int32_t nred = 0;
     int64_t ws_doubles = 0, max_red_mn = 0;
     int ta = 0, tb = 0;
     double flops = 0.0;
 };
+
+namespace {
+// contiguous parts of equal work (sum of K over the tiles), one per XCD
+void cut_parts(const std::vector<double>& work, int32_t part[NPART + 1])
+{
+    double total = 0.0;
+    for(double w : work) total += w;
+    part[0] = 0;
+    double acc = 0.0;
+    int x = 1;
+    for(size_t t = 0; t < work.size() && x < NPART; ++t) {
+        acc += work[t];
+        while(x < NPART && acc >= total * x / NPART) part[x++] = static_cast<int32_t>(t + 1);
+    }
+    for(; x <= NPART; ++x) part[x] = static_cast<int32_t>(work.size());
+}
+
+bool is_big(const xh_gemm_item& it) { return it.m >= BIG_MIN && it.n >= BIG_MIN; }
+}  // namespace
 
 extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, int trans_a, int trans_b, xh_gemm_plan** out)
 {
@@ -298,28 +465,30 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         std::vector<int> order(v.size());
         for(size_t i = 0; i < v.size(); ++i) order[i] = static_cast<int>(i);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return double(v[a].m) * v[a].n * v[a].k > double(v[b].m) * v[b].n * v[b].k; });
-        int64_t base_tiles = 0;
+        int64_t base_tiles = 0;  // output tiles of the FP64 launch
         for(const auto& it : v) {
             if(it.m < 0 || it.n < 0 || it.k < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative extent");
             if(it.a_off < 0 || it.b_off < 0 || it.c_off < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative offset");
-            if(it.m > 0 && it.n > 0) base_tiles += int64_t((it.m + BM - 1) / BM) * ((it.n + BN - 1) / BN);
+            if(it.m == 0 || it.n == 0) continue;
+            const int tm = is_big(it) ? BM2 : BM, tn = is_big(it) ? BN2 : BN;
+            base_tiles += int64_t((it.m + tm - 1) / tm) * ((it.n + tn - 1) / tn);
         }
-        // K split: aim at 4 tiles per CU, chunks of at least 256
+        // K split: aim at 4 tiles per CU, chunks of at least 128
         const int64_t target = 4 * int64_t(std::max(ctx->num_cu, 1));
         const int64_t S = base_tiles > 0 && base_tiles < target ? (target + base_tiles - 1) / base_tiles : 1;
         std::vector<ItemExt> ext(v.size(), ItemExt{1, 0, 0});
         std::vector<RedItem> red;
-        std::vector<Tile> tiles;
-        std::vector<double> work;  // K length of each tile
+        std::vector<Tile> t_all, t_big, t_rest;
+        std::vector<double> w_all, w_big, w_rest;
         for(int id : order) {
             const xh_gemm_item& it = v[id];
             if(it.m == 0 || it.n == 0) continue;
             plan->flops += 2.0 * it.m * double(it.n) * it.k;
             ItemExt& e = ext[id];
             e.kc = std::max(it.k, 1);
-            if(S > 1 && it.k >= 512) {
+            if(S > 1 && it.k >= 256) {
                 int kc = static_cast<int>((it.k + S - 1) / S);
-                kc = std::max(256, (kc + 15) / 16 * 16);
+                kc = std::max(128, (kc + 15) / 16 * 16);
                 const int split = (it.k + kc - 1) / kc;
                 if(split > 1) {
                     e.split = split;
@@ -331,38 +500,56 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
                     plan->max_red_mn = std::max(plan->max_red_mn, mn);
                 }
             }
+            const bool big = is_big(it);
             for(int ks = 0; ks < e.split; ++ks) {
                 const double kw = std::min(it.k - ks * e.kc, e.kc) + 8.0;  // + per-tile overhead
                 for(int n0 = 0; n0 < it.n; n0 += BN)
                     for(int m0 = 0; m0 < it.m; m0 += BM) {
-                        tiles.push_back(Tile{id, m0, n0, ks});
-                        work.push_back(kw);
+                        t_all.push_back(Tile{id, m0, n0, ks});
+                        w_all.push_back(kw);
                     }
+                if(big) {
+                    for(int n0 = 0; n0 < it.n; n0 += BN2)
+                        for(int m0 = 0; m0 < it.m; m0 += BM2) {
+                            t_big.push_back(Tile{id, m0, n0, ks});
+                            w_big.push_back(4.0 * kw);
+                        }
+                } else {
+                    for(int n0 = 0; n0 < it.n; n0 += BN)
+                        for(int m0 = 0; m0 < it.m; m0 += BM) {
+                            t_rest.push_back(Tile{id, m0, n0, ks});
+                            w_rest.push_back(kw);
+                        }
+                }
             }
         }
-        if(tiles.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
-        // contiguous parts of equal work, one per XCD
-        double total = 0.0;
-        for(double w : work) total += w;
-        plan->part[0] = 0;
-        {
-            double acc = 0.0;
-            int x = 1;
-            for(size_t t = 0; t < tiles.size() && x < NPART; ++t) {
-                acc += work[t];
-                while(x < NPART && acc >= total * x / NPART) plan->part[x++] = static_cast<int32_t>(t + 1);
-            }
-            for(; x <= NPART; ++x) plan->part[x] = static_cast<int32_t>(tiles.size());
-        }
-        plan->ntiles = static_cast<int32_t>(tiles.size());
+        if(t_all.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
+        cut_parts(w_all, plan->all64.part);
+        cut_parts(w_big, plan->big128.part);
+        cut_parts(w_rest, plan->rest64.part);
+        plan->all64.ntiles = static_cast<int32_t>(t_all.size());
+        plan->big128.ntiles = static_cast<int32_t>(t_big.size());
+        plan->rest64.ntiles = static_cast<int32_t>(t_rest.size());
         plan->nred = static_cast<int32_t>(red.size());
         plan->ta = trans_a ? 1 : 0;
         plan->tb = trans_b ? 1 : 0;
-        const size_t oi = plan->up.add(v), oe = plan->up.add(ext), ot = plan->up.add(tiles), orr = plan->up.add(red);
+        xh::PackedUpload& up = plan->up;
+        const size_t bytes = xh::PackedUpload::packed_bytes({v.size() * sizeof(xh_gemm_item), ext.size() * sizeof(ItemExt), t_all.size() * sizeof(Tile),
+                                                             t_big.size() * sizeof(Tile), t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem)});
+        if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);  // large plans: straight into pinned staging
+        auto put = [&up](const auto& vec) {
+            using T = typename std::decay_t<decltype(vec)>::value_type;
+            const size_t o = up.reserve<T>(static_cast<int64_t>(vec.size()));
+            if(!vec.empty()) std::memcpy(up.host_ptr<T>(o), vec.data(), vec.size() * sizeof(T));
+            return o;
+        };
+        const size_t oi = put(v), oe = put(ext), oa = put(t_all), ob = put(t_big), orr_t = put(t_rest), orr = put(red);
         plan->up.upload(ctx);
         plan->d_items = plan->up.ptr<xh_gemm_item>(oi);
         plan->d_ext = plan->up.ptr<ItemExt>(oe);
-        plan->d_tiles = plan->up.ptr<Tile>(ot);
+        plan->all64.d_tiles = plan->up.ptr<Tile>(oa);
+        plan->big128.d_tiles = plan->up.ptr<Tile>(ob);
+        plan->rest64.d_tiles = plan->up.ptr<Tile>(orr_t);
         plan->d_red = red.empty() ? nullptr : plan->up.ptr<RedItem>(orr);
     });
     if(rc != XH_OK) {
@@ -393,23 +580,34 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
 {
     return guard(ctx, [&] {
         if(!plan) throw xh::Error(XH_EINVAL, "null gemm plan");
-        if(plan->ntiles == 0) return;
+        if(plan->all64.ntiles == 0) return;
+        if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
         double* ws = nullptr;
         if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
-        GemmArgs g{plan->d_items, plan->d_ext, plan->d_tiles, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
-        for(int x = 0; x <= NPART; ++x) g.part[x] = plan->part[x];
-        // a multiple of NPART workgroups, at most 8 per CU
-        int grid = std::min<int>(plan->ntiles, ctx->num_cu * 8);
-        grid = (grid + NPART - 1) / NPART * NPART;
+        GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
+        // a multiple of NPART workgroups, at most per_cu per CU
+        auto launch = [&](const TileList& L, void (*kernel)(GemmArgs), int per_cu, int threads, const char* name) {
+            if(L.ntiles == 0) return;
+            g.tiles = L.d_tiles;
+            for(int x = 0; x <= NPART; ++x) g.part[x] = L.part[x];
+            int grid = std::min<int>(L.ntiles, ctx->num_cu * per_cu);
+            grid = (grid + NPART - 1) / NPART * NPART;
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, ctx->stream, g);
+            xh::launch_check(name);
+        };
         const bool timed = ctx->gemm_timing;
         if(timed) xh::gemm_timer_begin(ctx);
-        switch(precision) {
-        case XH_FP64: hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(G_THREADS), 0, ctx->stream, g); break;
-        case XH_FP32: hipLaunchKernelGGL(k_gemm_f32, dim3(grid), dim3(G_THREADS), 0, ctx->stream, g); break;
-        case XH_BF16X3: hipLaunchKernelGGL(k_gemm_bf16x3, dim3(grid), dim3(G_THREADS), 0, ctx->stream, g); break;
-        default: throw xh::Error(XH_EINVAL, "unknown precision mode");
+        if(precision == XH_FP64 && ctx->gemm_tiles != 1) {
+            if(ctx->gemm_tiles == 2)
+                launch(plan->big128, k_gemm_f64_128w8, 2, G8_THREADS, "k_gemm_f64_128w8");
+            else
+                launch(plan->big128, k_gemm_f64_128, 2, G_THREADS, "k_gemm_f64_128");
+            launch(plan->rest64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+        } else if(precision == XH_FP64) {
+            launch(plan->all64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+        } else {
+            launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");
         }
-        xh::launch_check("k_gemm");
         if(plan->nred > 0) {
             const unsigned gx = static_cast<unsigned>(std::min<int64_t>((plan->max_red_mn + 1023) / 1024, 256));
             hipLaunchKernelGGL(k_splitk_reduce, dim3(std::max(gx, 1u), static_cast<unsigned>(plan->nred)), dim3(256), 0, ctx->stream, plan->d_red, ws, C, alpha,
@@ -418,5 +616,13 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
         }
         if(timed) xh::gemm_timer_end(ctx, plan->flops);
         if(ws) xh::dev_free(ctx, ws);
+    });
+}
+
+extern "C" int xh_set_gemm_tiles(xh_ctx* ctx, int mode)
+{
+    return guard(ctx, [&] {
+        if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "gemm tile mode must be 0, 1 or 2");
+        ctx->gemm_tiles = mode;
     });
 }

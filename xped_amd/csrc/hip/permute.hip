@@ -276,6 +276,10 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         // then the transpose tasks (their metadata follows the gather metadata)
         const int64_t nmeta = ngather + ntr;
         xh::PackedUpload& up = plan->up;
+        const size_t bytes = xh::PackedUpload::packed_bytes(
+            {size_t(nmeta) * sizeof(TaskMeta), size_t(nmeta * NL) * sizeof(LegInfo), size_t(ngather + 1) * sizeof(int64_t), size_t(ntiles + 1) * sizeof(int32_t),
+             size_t(desc->nsources) * sizeof(xh_permute_source), size_t(ntr_tiles) * sizeof(int4), size_t(ntr) * sizeof(TrTaskInfo)});
+        if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);  // large plans: straight into pinned staging
         const size_t o_meta = up.reserve<TaskMeta>(nmeta), o_legs = up.reserve<LegInfo>(nmeta * NL), o_prefix = up.reserve<int64_t>(ngather + 1),
                      o_tiles = up.reserve<int32_t>(ntiles + 1), o_src = up.reserve<xh_permute_source>(desc->nsources),
                      o_tr = up.reserve<int4>(ntr_tiles), o_trinfo = up.reserve<TrTaskInfo>(ntr);

@@ -79,6 +79,12 @@ class Runtime:
         lib().xt_rt_gemm_timing(self.h, int(enable), int(reset), C.byref(ms), C.byref(fl), C.byref(n))
         return ms.value, fl.value, n.value
 
+    def gemm_launches(self) -> int:
+        """Number of xh_gemm calls timed since the last reset."""
+        ms, fl, n = C.c_double(), C.c_double(), C.c_int64()
+        lib().xh_gemm_timing(self.ctx(), -1, 0, C.byref(ms), C.byref(fl), C.byref(n))
+        return n.value
+
     def gemm_log(self) -> np.ndarray:
         """Block shapes (m, n, k) of the contraction GEMMs logged while timing was on."""
         ms, fl, nn = C.c_double(), C.c_double(), C.c_long()
