@@ -97,9 +97,8 @@ int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, 
 int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan);
 int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
 int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops);
-/* FP64 tiling of sectors with both output sides >= 192: 0 = 128 x 128 tiles with 4 waves
- * (default), 1 = 64 x 64 tiles like the other sectors, 2 = 128 x 128 tiles with 8 waves.  A tuning
- * switch; no counterpart in the reference. */
+/* FP64 tiling of sectors with both output sides >= 192: 0 = 128 x 128 tiles (default), 1 = 64 x 64
+ * tiles like the other sectors.  A tuning switch; no counterpart in the reference. */
 int xh_set_gemm_tiles(xh_ctx* ctx, int mode);
 /* HIP-event timing of every xh_gemm launch on the context stream (measurement; no counterpart in
  * the reference): enable (1) / disable (0) / keep (-1); returns the summed kernel time, the flops of

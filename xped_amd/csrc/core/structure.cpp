@@ -509,33 +509,40 @@ PairMap pair_turn(const Sym& s, int shift, const FTree& t1, const FTree& t2)
     return out;
 }
 
+// abelian bosonic fast path (treepair.cpp:147-168): one output pair, coefficient 1
+static TreePair abelian_pair_image(const Sym& s, int shift, const FTree& t1, const FTree& t2, const Permutation& p)
+{
+    const int R = t1.rank, C = t2.rank;
+    const int newR = R - shift, newC = C + shift;
+    FTree od, oc;
+    od.rank = newR;
+    oc.rank = newC;
+    for(int r = 0; r < newR; ++r) {
+        int src = p.pi[r];
+        od.unc[r] = src < R ? t1.unc[src] : s.conj(t2.unc[src - R]);
+        od.dims[r] = src < R ? t1.dims[src] : t2.dims[src - R];
+        od.dual[r] = src < R ? t1.dual[src] : !t2.dual[src - R];
+    }
+    for(int c = 0; c < newC; ++c) {
+        int src = p.pi[c + newR];
+        oc.unc[c] = src < R ? s.conj(t1.unc[src]) : t2.unc[src - R];
+        oc.dims[c] = src < R ? t1.dims[src] : t2.dims[src - R];
+        oc.dual[c] = src < R ? !t1.dual[src] : t2.dual[src - R];
+    }
+    od.compute_dim();
+    od.compute_intermediates(s);
+    oc.compute_dim();
+    oc.compute_intermediates(s);
+    return TreePair{od, oc};
+}
+
 PairMap pair_permute(const Sym& s, int shift, const FTree& t1, const FTree& t2, const Permutation& p)
 {
     const int R = t1.rank, C = t2.rank;
     const int newR = R - shift, newC = C + shift;
     if(!s.any_non_abelian() && !s.any_fermionic()) {
-        // abelian bosonic fast path (treepair.cpp:147-168): one output pair, coefficient 1
-        FTree od, oc;
-        od.rank = newR;
-        oc.rank = newC;
-        for(int r = 0; r < newR; ++r) {
-            int src = p.pi[r];
-            od.unc[r] = src < R ? t1.unc[src] : s.conj(t2.unc[src - R]);
-            od.dims[r] = src < R ? t1.dims[src] : t2.dims[src - R];
-            od.dual[r] = src < R ? t1.dual[src] : !t2.dual[src - R];
-        }
-        for(int c = 0; c < newC; ++c) {
-            int src = p.pi[c + newR];
-            oc.unc[c] = src < R ? s.conj(t1.unc[src]) : t2.unc[src - R];
-            oc.dims[c] = src < R ? t1.dims[src] : t2.dims[src - R];
-            oc.dual[c] = src < R ? !t1.dual[src] : t2.dual[src - R];
-        }
-        od.compute_dim();
-        od.compute_intermediates(s);
-        oc.compute_dim();
-        oc.compute_intermediates(s);
         PairMap out;
-        out.emplace(TreePair{od, oc}, 1.0);
+        out.emplace(abelian_pair_image(s, shift, t1, t2, p), 1.0);
         return out;
     }
     // turn<-CoRank>() reverses the order of the codomain legs, so the permutation is corrected first
@@ -663,6 +670,7 @@ SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStr
     check_perm(in, shift, p);
     const int R = in.rank, C = in.corank, N = R + C;
     const bool split = is_split_case(in, shift, p);
+    const bool abelian = !in.sym.any_non_abelian() && !in.sym.any_fermionic();
     SymbolicPermute sp;
     sp.nlegs = N;
     sp.new_rank = R - shift;
@@ -715,7 +723,11 @@ SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStr
                     const std::int32_t task = sp.task_base[so] + static_cast<std::int32_t>(ia) * sp.ncod[so] + static_cast<std::int32_t>(ib);
                     res.emplace_back(task, SymbolicPermute::Src{i, work[w].row_tree, static_cast<std::int32_t>(b2), c});
                 };
-                if(split) {
+                if(abelian) {
+                    // one image with coefficient 1, whether or not the permutation splits
+                    const TreePair im = abelian_pair_image(in.sym, shift, t1, t2, P);
+                    add(im.first, im.second, 1.0);
+                } else if(split) {
                     TreeMap pd = tree_permute(in.sym, t1, *Pd);
                     TreeMap pc = tree_permute(in.sym, t2, *Pc);
                     for(const auto& [a, ca] : pd)

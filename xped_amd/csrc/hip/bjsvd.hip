@@ -112,28 +112,37 @@ __global__ __launch_bounds__(BJ_THREADS) void k_bj_pair(const BjPair* __restrict
                     pi[2 * p + 1] = j;
                 }
                 __syncthreads();
-                for(int e = tid; e < npairs * n; e += BJ_THREADS) {  // columns: A <- A R, V <- V R
-                    const int p = e / n, r = e % n;
-                    const int i = pi[2 * p], j = pi[2 * p + 1];
-                    const double c = cs[2 * p], s = cs[2 * p + 1];
-                    if(i < 0 || s == 0.0) continue;
-                    double x = A[r + i * n], y = A[r + j * n];
-                    A[r + i * n] = c * x - s * y;
-                    A[r + j * n] = s * x + c * y;
-                    x = V[r + i * n];
-                    y = V[r + j * n];
-                    V[r + i * n] = c * x - s * y;
-                    V[r + j * n] = s * x + c * y;
+                // thread (p, q): pair p = tid / G with G = BJ_THREADS / npairs lanes per pair, rows
+                // (then columns) q, q + G, ...: no index division inside the loops
+                const int G = BJ_THREADS / npairs;
+                const int pp = tid / G, q0 = tid - pp * G;
+                int ri = -1, rj = 0;
+                double rc = 1.0, rs = 0.0;
+                if(pp < npairs) {
+                    ri = pi[2 * pp];
+                    rj = pi[2 * pp + 1];
+                    rc = cs[2 * pp];
+                    rs = cs[2 * pp + 1];
+                }
+                const bool act = ri >= 0 && rs != 0.0;
+                if(act) {  // columns: A <- A R, V <- V R
+                    for(int r = q0; r < n; r += G) {
+                        double x = A[r + ri * n], y = A[r + rj * n];
+                        A[r + ri * n] = rc * x - rs * y;
+                        A[r + rj * n] = rs * x + rc * y;
+                        x = V[r + ri * n];
+                        y = V[r + rj * n];
+                        V[r + ri * n] = rc * x - rs * y;
+                        V[r + rj * n] = rs * x + rc * y;
+                    }
                 }
                 __syncthreads();
-                for(int e = tid; e < npairs * n; e += BJ_THREADS) {  // rows: A <- R^T A
-                    const int p = e / n, col = e % n;
-                    const int i = pi[2 * p], j = pi[2 * p + 1];
-                    const double c = cs[2 * p], s = cs[2 * p + 1];
-                    if(i < 0 || s == 0.0) continue;
-                    const double x = A[i + col * n], y = A[j + col * n];
-                    A[i + col * n] = c * x - s * y;
-                    A[j + col * n] = s * x + c * y;
+                if(act) {  // rows: A <- R^T A
+                    for(int col = q0; col < n; col += G) {
+                        const double x = A[ri + col * n], y = A[rj + col * n];
+                        A[ri + col * n] = rc * x - rs * y;
+                        A[rj + col * n] = rs * x + rc * y;
+                    }
                 }
                 __syncthreads();
             }

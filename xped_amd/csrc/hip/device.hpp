@@ -21,7 +21,7 @@ struct xh_ctx {
     double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
-    int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 (4 waves) for large sectors, 1 64x64 only, 2 128x128 (8 waves)
+    int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 tiles for large sectors, 1 64x64 tiles only
     // host staging buffers of asynchronous uploads, kept until the event after their copy fires
     std::vector<std::pair<hipEvent_t, std::vector<char>>> pending_host;
     // pinned host staging buffers for large plan uploads: free ones, and ones whose copy may still run

@@ -164,80 +164,15 @@ __global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
 }
 
 // ---------------------------------------------------------------- FP64, 128 x 128 tiles
-// For items with both output sides >= 192: 4 waves in 2 x 2, each owning 64 x 64 (4 x 4 MFMA
-// blocks, 16 accumulators), K steps of 16 staged in LDS, and the next K step's operands fetched
-// into registers while the matrix cores work on the current one.
+// For items with both output sides >= 192: 8 waves in 2 x 4, each owning 64 x 32 (4 x 2 MFMA
+// blocks, 8 accumulators, so two or three waves share a SIMD), K steps of 16 staged in LDS, and
+// the next K step's operands fetched into registers while the matrix cores work on the current
+// one.  (A 4-wave variant with 64 x 64 per wave ran at one wave per SIMD and was 25-30 % slower.)
 constexpr int BM2 = 128, BN2 = 128;
 constexpr int BIG_MIN = 192;
 
-__global__ __launch_bounds__(G_THREADS) void k_gemm_f64_128(GemmArgs g)
-{
-    constexpr int BK = 16;
-    __shared__ double sA[BK][BM2 + 1];
-    __shared__ double sB[BK][BN2 + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const TileRange tr = my_tiles(g);
-    for(int t = tr.begin; t < tr.end; t += tr.stride) {
-        const Tile tl = g.tiles[t];
-        const xh_gemm_item it = g.items[tl.item];
-        const ItemExt ex = g.ext[tl.item];
-        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
-        d4 acc[4][4];
-        for(int a = 0; a < 4; ++a)
-            for(int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
-        double ra[8], rb[8];
-        auto fetch = [&](int k0) {
-            for(int r = 0; r < 8; ++r) {
-                const int idx = tid + G_THREADS * r;
-                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
-                ra[r] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
-                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
-                rb[r] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
-            }
-        };
-        if(kbeg < kend) fetch(kbeg);
-        for(int k0 = kbeg; k0 < kend; k0 += BK) {
-            __syncthreads();  // every wave is done with the previous K step (or tile)
-            for(int r = 0; r < 8; ++r) {
-                const int idx = tid + G_THREADS * r;
-                const int i = g.ta ? idx >> 4 : idx & 127, kk = g.ta ? idx & 15 : idx >> 7;
-                sA[kk][i] = ra[r];
-                const int kb = g.tb ? idx >> 7 : idx & 15, j = g.tb ? idx & 127 : idx >> 4;
-                sB[kb][j] = rb[r];
-            }
-            __syncthreads();
-            if(k0 + BK < kend) fetch(k0 + BK);
-            for(int kq = 0; kq < BK; kq += 4) {
-                const int kr = kq + (lane >> 4);
-                double av[4], bv[4];
-                for(int mi = 0; mi < 4; ++mi) av[mi] = sA[kr][wm * 64 + mi * 16 + (lane & 15)];
-                for(int ni = 0; ni < 4; ++ni) bv[ni] = sB[kr][wn * 64 + ni * 16 + (lane & 15)];
-                for(int mi = 0; mi < 4; ++mi)
-                    for(int ni = 0; ni < 4; ++ni) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
-            }
-        }
-        const bool part = ex.split > 1;
-        double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
-        const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
-        for(int mi = 0; mi < 4; ++mi)
-            for(int ni = 0; ni < 4; ++ni)
-                for(int r = 0; r < 4; ++r) {
-                    const int row = tl.m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * r;
-                    const int col = tl.n0 + wn * 64 + ni * 16 + (lane & 15);
-                    if(row < it.m && col < it.n) {
-                        double* p = base + int64_t(col) * it.m + row;
-                        const double v = acc[mi][ni][r];
-                        *p = (beta == 0.0) ? alpha * v : alpha * v + beta * *p;
-                    }
-                }
-    }
-}
-
-// 128 x 128 tiles with 8 waves (2 x 4), each owning 64 x 32 (4 x 2 MFMA blocks): half the
-// accumulators per wave of the 4-wave kernel, so two waves share each SIMD.
 constexpr int G8_THREADS = 512;
-__global__ __launch_bounds__(G8_THREADS) void k_gemm_f64_128w8(GemmArgs g)
+__global__ __launch_bounds__(G8_THREADS) void k_gemm_f64_128(GemmArgs g)
 {
     constexpr int BK = 16;
     __shared__ double sA[BK][BM2 + 1];
@@ -597,11 +532,8 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
         };
         const bool timed = ctx->gemm_timing;
         if(timed) xh::gemm_timer_begin(ctx);
-        if(precision == XH_FP64 && ctx->gemm_tiles != 1) {
-            if(ctx->gemm_tiles == 2)
-                launch(plan->big128, k_gemm_f64_128w8, 2, G8_THREADS, "k_gemm_f64_128w8");
-            else
-                launch(plan->big128, k_gemm_f64_128, 2, G_THREADS, "k_gemm_f64_128");
+        if(precision == XH_FP64 && ctx->gemm_tiles == 0) {
+            launch(plan->big128, k_gemm_f64_128, 2, G8_THREADS, "k_gemm_f64_128");
             launch(plan->rest64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
         } else if(precision == XH_FP64) {
             launch(plan->all64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
@@ -622,7 +554,7 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, c
 extern "C" int xh_set_gemm_tiles(xh_ctx* ctx, int mode)
 {
     return guard(ctx, [&] {
-        if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "gemm tile mode must be 0, 1 or 2");
+        if(mode < 0 || mode > 1) throw xh::Error(XH_EINVAL, "gemm tile mode must be 0 or 1");
         ctx->gemm_tiles = mode;
     });
 }
