@@ -41,9 +41,10 @@ int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, l
 int xt_rt_clear_caches(xt_rt* rt);
 /* host time (seconds) spent building permute and GEMM plans, and how many were built */
 int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans);
-/* breakdown of that host time (seconds): [0] permute plans on the host, [1] their checks and
- * packing, [2] device permute plans, [3] GEMM plans on the host, [4] device GEMM plans; [5] the
- * number of permute plans whose dimension-free part came from the cache (same leg charges) */
+/* breakdown of that host time (seconds): [0] numeric part of the permute plans on the host, [1]
+ * their checks and packing, [2] device permute plans, [3] GEMM plans on the host, [4] device GEMM
+ * plans; [5] the number of permute plans whose dimension-free part came from the cache (same leg
+ * charges); [6] output structures of the permute plans, [7] their dimension-free parts */
 int xt_rt_host_timers(const xt_rt* rt, double* t, int n);
 /* GEMM timing: enable = 1 times every grouped GEMM launch of the runtime's context with HIP events
  * (contractions, sketch and Gram products, adjoints) and logs the block shapes of the operator*

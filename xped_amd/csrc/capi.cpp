@@ -141,7 +141,8 @@ int xt_rt_host_timers(const xt_rt* rt, double* t, int n)
     if(!rt || n < 0) return XH_EINVAL;
     for(int i = 0; i < n && i < 5; ++i) t[i] = rt->rt.host_t[i];
     if(n > 5) t[5] = static_cast<double>(rt->rt.n_symbolic_hits);
-    for(int i = 6; i < n && i < 8; ++i) t[i] = 0.0;
+    if(n > 6) t[6] = rt->rt.host_t[5];
+    if(n > 7) t[7] = rt->rt.host_t[6];
     return XH_OK;
 }
 
@@ -447,7 +448,11 @@ int xt_permute(const xt_tensor* t, int shift, const int* p, int np, xt_tensor** 
 int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, int64_t* ntasks, int64_t* nsources, int64_t* elems)
 {
     return run(t ? t->owner : nullptr, [&] {
-        const xa::PermutePlan& plan = xa::get_permute_plan(t->t.rt, *t->t.st, shift, std::vector<int>(p, p + np))->plan;
+        // the runtime's cached path; on a GPU runtime the cached plan gave its host tables to
+        // the device plan, so they are rebuilt from the structure
+        const std::vector<int> pv(p, p + np);
+        const xa::PermutePlan& cached = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv)->plan;
+        const xa::PermutePlan plan = cached.tasks.empty() && cached.moved_elems > 0 ? xa::make_permute_plan(*t->t.st, shift, pv) : cached;
         if(ntasks) *ntasks = static_cast<int64_t>(plan.tasks.size());
         if(nsources) *nsources = static_cast<int64_t>(plan.sources.size());
         if(elems) *elems = static_cast<int64_t>(plan.moved_elems);
@@ -458,8 +463,11 @@ int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, in
                          int64_t* src_off, double* src_coeff)
 {
     return run(t ? t->owner : nullptr, [&] {
-        // the runtime's cached path (symbolic plans reused across dimension changes)
-        const xa::PermutePlan& plan = xa::get_permute_plan(t->t.rt, *t->t.st, shift, std::vector<int>(p, p + np))->plan;
+        // the runtime's cached path (symbolic plans reused across dimension changes); on a GPU
+        // runtime the host tables were handed to the device plan and are rebuilt here
+        const std::vector<int> pv(p, p + np);
+        const xa::PermutePlan& cached = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv)->plan;
+        const xa::PermutePlan plan = cached.tasks.empty() && cached.moved_elems > 0 ? xa::make_permute_plan(*t->t.st, shift, pv) : cached;
         for(std::size_t i = 0; i < plan.tasks.size(); ++i) {
             task_dst[i] = plan.tasks[i].dst_off;
             task_rows[i] = plan.tasks[i].rows;

@@ -6,12 +6,130 @@
 #include <cmath>
 #include <numeric>
 #include <sstream>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
 
 namespace xa {
+
+namespace {
+
+// A persistent pool of host worker threads for the plan builders: starting threads per plan
+// costs more than the work of a typical plan.  run(n, f) calls f(i) for i in [0, n) on the
+// workers and the calling thread, and returns when all calls are done; the first exception is
+// rethrown.
+class WorkerPool {
+public:
+    static WorkerPool& get()
+    {
+        static WorkerPool pool;
+        return pool;
+    }
+    unsigned size() const { return static_cast<unsigned>(threads_.size()) + 1; }
+    template <class F>
+    void run(std::size_t n, F&& f)
+    {
+        if(n == 0) return;
+        if(n == 1 || threads_.empty()) {
+            for(std::size_t i = 0; i < n; ++i) f(i);
+            return;
+        }
+        std::lock_guard<std::mutex> one(run_mutex_);  // one parallel region at a time
+        std::function<void(std::size_t)> job = [&f](std::size_t i) { f(i); };
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            job_ = &job;
+            n_ = n;
+            next_ = 0;
+            pending_ = n;
+            err_ = nullptr;
+            ++generation_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m_);
+        done_cv_.wait(lk, [this] { return pending_ == 0; });
+        job_ = nullptr;
+        if(err_) std::rethrow_exception(err_);
+    }
+    ~WorkerPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for(auto& t : threads_) t.join();
+    }
+
+private:
+    WorkerPool()
+    {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        for(unsigned i = 1; i < hw; ++i) threads_.emplace_back([this] { loop(); });
+    }
+    void work()
+    {
+        for(;;) {
+            std::size_t i;
+            const std::function<void(std::size_t)>* job;
+            {
+                std::lock_guard<std::mutex> lk(m_);
+                if(!job_ || next_ >= n_) return;
+                i = next_++;
+                job = job_;
+            }
+            try {
+                (*job)(i);
+            } catch(...) {
+                std::lock_guard<std::mutex> lk(m_);
+                if(!err_) err_ = std::current_exception();
+            }
+            std::lock_guard<std::mutex> lk(m_);
+            if(--pending_ == 0) done_cv_.notify_all();
+        }
+    }
+    void loop()
+    {
+        std::uint64_t seen = 0;
+        for(;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || generation_ != seen; });
+                if(stop_) return;
+                seen = generation_;
+            }
+            work();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex m_, run_mutex_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(std::size_t)>* job_ = nullptr;
+    std::size_t n_ = 0, next_ = 0, pending_ = 0;
+    std::uint64_t generation_ = 0;
+    std::exception_ptr err_;
+    bool stop_ = false;
+};
+
+// f(begin, end) over [0, n) in up to pool-size chunks when n / min_per_chunk > 1
+template <class F>
+void parallel_ranges(std::size_t n, std::size_t min_per_chunk, F&& f)
+{
+    WorkerPool& pool = WorkerPool::get();
+    const std::size_t nt = std::min<std::size_t>(pool.size(), std::max<std::size_t>(1, n / std::max<std::size_t>(min_per_chunk, 1)));
+    if(nt <= 1) {
+        f(std::size_t(0), n);
+        return;
+    }
+    pool.run(nt, [&](std::size_t k) { f(n * k / nt, n * (k + 1) / nt); });
+}
+
+}  // namespace
+
 
 // ---------------------------------------------------------------- FTree
 
@@ -739,26 +857,12 @@ SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStr
         }
     };
     const std::size_t nw = work.size();
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const std::size_t nthreads = ntasks < 4096 ? 1 : std::min<std::size_t>(hw, std::max<std::size_t>(1, nw / 4));
+    const std::size_t nthreads = ntasks < 4096 ? 1 : std::min<std::size_t>(WorkerPool::get().size(), std::max<std::size_t>(1, nw / 4));
     std::vector<Contrib> parts(nthreads);
-    if(nthreads == 1) {
+    if(nthreads == 1)
         run_range(0, nw, parts[0]);
-    } else {
-        std::vector<std::thread> th;
-        std::vector<std::exception_ptr> errs(nthreads);
-        for(std::size_t k = 0; k < nthreads; ++k)
-            th.emplace_back([&, k] {
-                try {
-                    run_range(nw * k / nthreads, nw * (k + 1) / nthreads, parts[k]);
-                } catch(...) {
-                    errs[k] = std::current_exception();
-                }
-            });
-        for(auto& t : th) t.join();
-        for(auto& e : errs)
-            if(e) std::rethrow_exception(e);
-    }
+    else
+        WorkerPool::get().run(nthreads, [&](std::size_t k) { run_range(nw * k / nthreads, nw * (k + 1) / nthreads, parts[k]); });
     sp.src_begin.assign(static_cast<std::size_t>(ntasks), 0);
     sp.nsrc.assign(static_cast<std::size_t>(ntasks), 0);
     for(const auto& part : parts)
@@ -791,32 +895,19 @@ std::vector<std::size_t> tree_offsets(const Qbasis& b, const Q& q)
 }
 }  // namespace
 
+
 namespace {
-// run f(begin, end) over [0, n) on up to 16 host threads when n is large
-template <class F>
-void parallel_ranges(std::size_t n, std::size_t min_per_thread, F&& f)
-{
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const std::size_t nt = std::min<std::size_t>(hw, std::max<std::size_t>(1, n / std::max<std::size_t>(min_per_thread, 1)));
-    if(nt <= 1) {
-        f(std::size_t(0), n);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::vector<std::exception_ptr> errs(nt);
-    for(std::size_t k = 0; k < nt; ++k)
-        th.emplace_back([&, k] {
-            try {
-                f(n * k / nt, n * (k + 1) / nt);
-            } catch(...) {
-                errs[k] = std::current_exception();
-            }
-        });
-    for(auto& t : th) t.join();
-    for(auto& e : errs)
-        if(e) std::rethrow_exception(e);
-}
+thread_local std::vector<PermuteTask> spare_tasks;
+thread_local std::vector<PermuteSource> spare_sources;
 }  // namespace
+
+void recycle_plan_tables(PermutePlan& plan)
+{
+    if(plan.tasks.capacity() > spare_tasks.capacity()) spare_tasks = std::move(plan.tasks);
+    if(plan.sources.capacity() > spare_sources.capacity()) spare_sources = std::move(plan.sources);
+    plan.tasks = std::vector<PermuteTask>();
+    plan.sources = std::vector<PermuteSource>();
+}
 
 PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructure& in, const StructPtr& outp, const std::vector<int>& p)
 {
@@ -828,6 +919,11 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
     plan.nlegs = N;
     plan.new_rank = newR;
     plan.out = outp;
+    // every field of every task and source is written below, into recycled memory when available
+    plan.tasks = std::move(spare_tasks);
+    plan.sources = std::move(spare_sources);
+    spare_tasks = std::vector<PermuteTask>();
+    spare_sources = std::vector<PermuteSource>();
     plan.tasks.resize(sp.src_begin.size());
     for(std::size_t s = 0; s < out.sector.size(); ++s)
         if(static_cast<std::int32_t>(out.cod.tree(out.sector[s]).size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
@@ -840,7 +936,7 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
         for(int mm = lo; mm < m; ++mm) before[k][nbefore[k]++] = Pfull.pi_inv[mm];
     }
     std::vector<std::size_t> moved(out.sector.size(), 0);
-    parallel_ranges(out.sector.size(), 2, [&](std::size_t s0, std::size_t s1) {
+    parallel_ranges(out.sector.size(), plan.tasks.size() >= 16384 ? 2 : out.sector.size() + 1, [&](std::size_t s0, std::size_t s1) {
         for(std::size_t s = s0; s < s1; ++s) {
             const Q& q = out.sector[s];
             const auto& dtrees = out.dom.tree(q);

@@ -205,15 +205,18 @@ struct PermuteSource {
     std::int64_t src_ld;   // leading dimension of the source block
     double coeff;
 };
+// The builders write every scalar field and the first nlegs entries of each array; the empty
+// constructor leaves the rest unwritten, so resizing a recycled table touches no memory.
 struct PermuteTask {
+    PermuteTask() {}
     std::int64_t dst_off;  // element offset of the output subblock's (0,0)
     std::int64_t dst_ld;
     std::int64_t rows, cols;  // output subblock extent
-    int nsrc = 0;
-    int src_begin = 0;
-    std::array<std::uint32_t, MAXR * 2> ext{};      // output leg extents
-    std::array<std::int64_t, MAXR * 2> sstride{};  // per output leg: stride in source row (is_col=0) or col (is_col=1) index
-    std::array<std::uint8_t, MAXR * 2> is_col{};
+    int nsrc;
+    int src_begin;
+    std::array<std::uint32_t, MAXR * 2> ext;      // output leg extents
+    std::array<std::int64_t, MAXR * 2> sstride;  // per output leg: stride in source row (is_col=0) or col (is_col=1) index
+    std::array<std::uint8_t, MAXR * 2> is_col;
 };
 struct PermutePlan {
     int nlegs = 0, new_rank = 0;
@@ -242,6 +245,10 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
 
 // Build the output structure and gather plan of T.permute<shift, p...>() (Tensor.cpp:83-214,335-355).
 PermutePlan make_permute_plan(const TensorStructure& in, int shift, const std::vector<int>& p);
+// Hand a plan's task and source tables back to this thread's buffers once the device copy exists:
+// the next plan reuses their memory instead of faulting in fresh pages.  The plan keeps its
+// output structure and counts; its tables are empty afterwards.
+void recycle_plan_tables(PermutePlan& plan);
 // Output structure of permute without the plan.
 StructPtr permuted_structure(const TensorStructure& in, int shift, const std::vector<int>& p);
 // Structure of X for a given Y = X.permute<shift, p...>() (the legs of Tensor::permute_adj,

@@ -182,6 +182,7 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     // the dimension-free part is shared by all structures with the same charges (tensors whose
     // sector dimensions changed after a truncation); only offsets and extents are recomputed
     StructPtr out = permuted_structure(in, shift, p);
+    lap(5);
     std::string sk = charges_key(in);
     put_i(sk, shift);
     for(int x : p) put_i(sk, x);
@@ -194,6 +195,7 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
         sp = std::make_shared<const SymbolicPermute>(make_symbolic_permute(in, *out, shift, p));
         rt->sym_permute_cache.emplace(sk, sp);
     }
+    lap(6);
     cp->plan = numeric_permute_plan(*sp, in, out, p);
     lap(0);
     finish_permute_plan(rt, *cp, in, lap);
@@ -246,6 +248,7 @@ static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStr
                              reinterpret_cast<xh_permute_task*>(cp->plan.tasks.data()), reinterpret_cast<xh_permute_source*>(cp->plan.sources.data())};
         lap(1);
         dcheck(rt->ctx, xh_permute_plan_create(rt->ctx, &desc, &cp->dev), "permute plan");
+        recycle_plan_tables(cp->plan);  // the device plan holds its own copy
         lap(2);
     }
 }

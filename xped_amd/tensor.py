@@ -67,7 +67,7 @@ class Runtime:
         t = (C.c_double * 8)()
         lib().xt_rt_host_timers(self.h, t, 8)
         return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value,
-                "breakdown_s": [round(x, 4) for x in t[:5]], "symbolic_hits": int(t[5])}
+                "breakdown_s": [round(x, 4) for x in list(t[:5]) + list(t[6:8])], "symbolic_hits": int(t[5])}
 
     def marker(self, id: int):
         """An empty kernel of `id` workgroups on the runtime's stream: a mark in a kernel trace."""
