@@ -1,6 +1,8 @@
 // Device-resident tensor operations: structure on the host, data movement and arithmetic on the GPU.
 #include "tensor.hpp"
 
+#include "../common/pool.hpp"
+
 #include <cstddef>
 
 #include <algorithm>
@@ -214,18 +216,23 @@ static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStr
     {
         const PermutePlan& pl = cp->plan;
         const std::int64_t in_size = static_cast<std::int64_t>(in.size), out_size = static_cast<std::int64_t>(pl.out->size);
-        for(const PermuteTask& tk : pl.tasks) {
-            if(tk.rows <= 0 || tk.cols <= 0) throw Error(XH_EINVAL, "permute plan: empty subblock");
-            const std::int64_t dmax = tk.dst_off + (tk.rows - 1) + (tk.cols - 1) * tk.dst_ld;
-            if(tk.dst_off < 0 || dmax >= out_size) throw Error(XH_EINVAL, "permute plan: output subblock out of bounds");
-            std::int64_t rmax = 0, cmax = 0;
-            for(int k2 = 0; k2 < pl.nlegs; ++k2) (tk.is_col[k2] ? cmax : rmax) += (static_cast<std::int64_t>(tk.ext[k2]) - 1) * tk.sstride[k2];
-            for(int s2 = 0; s2 < tk.nsrc; ++s2) {
-                const PermuteSource& so = pl.sources[tk.src_begin + s2];
-                if(so.src_off < 0 || so.src_off + rmax + cmax * so.src_ld >= in_size || rmax >= so.src_ld)
-                    throw Error(XH_EINVAL, "permute plan: source subblock out of bounds");
+        xp::parallel_ranges(pl.tasks.size(), 8192, [&](std::size_t t0, std::size_t t1) {
+            for(std::size_t t = t0; t < t1; ++t) {
+                const PermuteTask& tk = pl.tasks[t];
+                if(tk.rows <= 0 || tk.cols <= 0) throw Error(XH_EINVAL, "permute plan: empty subblock");
+                const std::int64_t dmax = tk.dst_off + (tk.rows - 1) + (tk.cols - 1) * tk.dst_ld;
+                if(tk.dst_off < 0 || dmax >= out_size) throw Error(XH_EINVAL, "permute plan: output subblock out of bounds");
+                std::int64_t rmax = 0, cmax = 0;
+                for(int k2 = 0; k2 < pl.nlegs; ++k2) (tk.is_col[k2] ? cmax : rmax) += (static_cast<std::int64_t>(tk.ext[k2]) - 1) * tk.sstride[k2];
+                if(tk.src_begin < 0 || tk.nsrc < 0 || static_cast<std::size_t>(tk.src_begin) + static_cast<std::size_t>(tk.nsrc) > pl.sources.size())
+                    throw Error(XH_EINVAL, "permute plan: source range out of bounds");
+                for(int s2 = 0; s2 < tk.nsrc; ++s2) {
+                    const PermuteSource& so = pl.sources[tk.src_begin + s2];
+                    if(so.src_off < 0 || so.src_off + rmax + cmax * so.src_ld >= in_size || rmax >= so.src_ld)
+                        throw Error(XH_EINVAL, "permute plan: source subblock out of bounds");
+                }
             }
-        }
+        });
     }
     // share output structures between identical leg sets
     {

@@ -363,8 +363,13 @@ struct xh_gemm_plan {
     xh_gemm_item* d_items = nullptr;
     ItemExt* d_ext = nullptr;
     RedItem* d_red = nullptr;
-    TileList all64;           // every item in 64 x 64 tiles (FP32 / BF16x3 modes)
+    TileList all64;           // every item in 64 x 64 tiles (FP32 / BF16x3 modes), built on first use
     TileList big128, rest64;  // FP64: items with both sides >= BIG_MIN in 128 x 128 tiles, the others in 64 x 64
+    bool all64_built = false, any_tiles = false;
+    xh::PackedUpload up_all;
+    std::vector<xh_gemm_item> h_items;  // host copies for the deferred list
+    std::vector<ItemExt> h_ext;
+    std::vector<int> h_order;
     int32_t nred = 0;
     int64_t ws_doubles = 0, max_red_mn = 0;
     int ta = 0, tb = 0;
@@ -413,8 +418,8 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         const int64_t S = base_tiles > 0 && base_tiles < target ? (target + base_tiles - 1) / base_tiles : 1;
         std::vector<ItemExt> ext(v.size(), ItemExt{1, 0, 0});
         std::vector<RedItem> red;
-        std::vector<Tile> t_all, t_big, t_rest;
-        std::vector<double> w_all, w_big, w_rest;
+        std::vector<Tile> t_big, t_rest;
+        std::vector<double> w_big, w_rest;
         for(int id : order) {
             const xh_gemm_item& it = v[id];
             if(it.m == 0 || it.n == 0) continue;
@@ -438,11 +443,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
             const bool big = is_big(it);
             for(int ks = 0; ks < e.split; ++ks) {
                 const double kw = std::min(it.k - ks * e.kc, e.kc) + 8.0;  // + per-tile overhead
-                for(int n0 = 0; n0 < it.n; n0 += BN)
-                    for(int m0 = 0; m0 < it.m; m0 += BM) {
-                        t_all.push_back(Tile{id, m0, n0, ks});
-                        w_all.push_back(kw);
-                    }
+                plan->any_tiles = true;
                 if(big) {
                     for(int n0 = 0; n0 < it.n; n0 += BN2)
                         for(int m0 = 0; m0 < it.m; m0 += BM2) {
@@ -458,19 +459,17 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
                 }
             }
         }
-        if(t_all.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
-        cut_parts(w_all, plan->all64.part);
+        if(t_big.size() + t_rest.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
         cut_parts(w_big, plan->big128.part);
         cut_parts(w_rest, plan->rest64.part);
-        plan->all64.ntiles = static_cast<int32_t>(t_all.size());
         plan->big128.ntiles = static_cast<int32_t>(t_big.size());
         plan->rest64.ntiles = static_cast<int32_t>(t_rest.size());
         plan->nred = static_cast<int32_t>(red.size());
         plan->ta = trans_a ? 1 : 0;
         plan->tb = trans_b ? 1 : 0;
         xh::PackedUpload& up = plan->up;
-        const size_t bytes = xh::PackedUpload::packed_bytes({v.size() * sizeof(xh_gemm_item), ext.size() * sizeof(ItemExt), t_all.size() * sizeof(Tile),
-                                                             t_big.size() * sizeof(Tile), t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem)});
+        const size_t bytes = xh::PackedUpload::packed_bytes({v.size() * sizeof(xh_gemm_item), ext.size() * sizeof(ItemExt), t_big.size() * sizeof(Tile),
+                                                             t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem)});
         if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);  // large plans: straight into pinned staging
         auto put = [&up](const auto& vec) {
             using T = typename std::decay_t<decltype(vec)>::value_type;
@@ -478,11 +477,13 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
             if(!vec.empty()) std::memcpy(up.host_ptr<T>(o), vec.data(), vec.size() * sizeof(T));
             return o;
         };
-        const size_t oi = put(v), oe = put(ext), oa = put(t_all), ob = put(t_big), orr_t = put(t_rest), orr = put(red);
+        const size_t oi = put(v), oe = put(ext), ob = put(t_big), orr_t = put(t_rest), orr = put(red);
         plan->up.upload(ctx);
         plan->d_items = plan->up.ptr<xh_gemm_item>(oi);
         plan->d_ext = plan->up.ptr<ItemExt>(oe);
-        plan->all64.d_tiles = plan->up.ptr<Tile>(oa);
+        plan->h_items = std::move(v);
+        plan->h_ext = std::move(ext);
+        plan->h_order = std::move(order);
         plan->big128.d_tiles = plan->up.ptr<Tile>(ob);
         plan->rest64.d_tiles = plan->up.ptr<Tile>(orr_t);
         plan->d_red = red.empty() ? nullptr : plan->up.ptr<RedItem>(orr);
@@ -500,6 +501,7 @@ extern "C" int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan)
     if(!plan) return XH_OK;
     return guard(ctx, [&] {
         plan->up.release(ctx);
+        plan->up_all.release(ctx);
         delete plan;
     });
 }
@@ -511,11 +513,43 @@ extern "C" int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops)
     return XH_OK;
 }
 
-extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision)
+namespace {
+// the every-item 64 x 64 list, needed only by the FP32 / BF16x3 modes and the 64-only FP64 tiling
+void build_all64(xh_ctx* ctx, xh_gemm_plan* plan)
+{
+    std::vector<Tile> t;
+    std::vector<double> w;
+    for(int id : plan->h_order) {
+        const xh_gemm_item& it = plan->h_items[id];
+        if(it.m == 0 || it.n == 0) continue;
+        const ItemExt& e = plan->h_ext[id];
+        for(int ks = 0; ks < e.split; ++ks) {
+            const double kw = std::min(it.k - ks * e.kc, e.kc) + 8.0;
+            for(int n0 = 0; n0 < it.n; n0 += BN)
+                for(int m0 = 0; m0 < it.m; m0 += BM) {
+                    t.push_back(Tile{id, m0, n0, ks});
+                    w.push_back(kw);
+                }
+        }
+    }
+    if(t.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
+    cut_parts(w, plan->all64.part);
+    plan->all64.ntiles = static_cast<int32_t>(t.size());
+    const size_t o = plan->up_all.add(t);
+    plan->up_all.upload(ctx);
+    plan->all64.d_tiles = plan->up_all.ptr<Tile>(o);
+    plan->all64_built = true;
+}
+}  // namespace
+
+extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const double* B, double* C, double alpha, double beta, int precision)
 {
     return guard(ctx, [&] {
-        if(!plan) throw xh::Error(XH_EINVAL, "null gemm plan");
-        if(plan->all64.ntiles == 0) return;
+        if(!cplan) throw xh::Error(XH_EINVAL, "null gemm plan");
+        xh_gemm_plan* plan = const_cast<xh_gemm_plan*>(cplan);  // the deferred tile list is filled in on first use
+        if(!plan->any_tiles) return;
+        const bool need_all = precision != XH_FP64 || ctx->gemm_tiles != 0;
+        if(need_all && !plan->all64_built) build_all64(ctx, plan);
         if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
         double* ws = nullptr;
         if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");

@@ -18,6 +18,7 @@
 #include <cstring>
 
 #include "device.hpp"
+#include "../common/pool.hpp"
 
 using xh::check;
 using xh::guard;
@@ -229,6 +230,7 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         if(NT < 0 || NT > 0x7fffffff || desc->nsources < 0) throw xh::Error(XH_EINVAL, "permute plan: invalid task or source count");
         // pass 1: validate every task and pick its kernel (-1: gather, u > 0: transpose via leg u)
         std::vector<int32_t> route(static_cast<size_t>(NT));
+        std::vector<int64_t> slot(static_cast<size_t>(NT)), tile0(static_cast<size_t>(NT), 0);  // table row; first transpose tile
         int64_t ngather = 0, ntr = 0, ntr_tiles = 0, total = 0;
         double moved = 0.0, read = 0.0;
         for(int64_t t = 0; t < NT; ++t) {
@@ -262,9 +264,12 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                 int64_t rest = 1;
                 for(int k = 0; k < NL; ++k)
                     if(k != 0 && k != u) rest *= tk.ext[k];
+                slot[t] = ntr;  // offset by ngather below
+                tile0[t] = ntr_tiles;
                 ntr_tiles += rest * ((tk.ext[0] + TR_DIM - 1) / TR_DIM) * ((tk.ext[u] + TR_DIM - 1) / TR_DIM);
                 ++ntr;
             } else {
+                slot[t] = ngather;
                 ++ngather;
                 total += tk.rows * tk.cols;
             }
@@ -289,33 +294,36 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         int32_t* tile_first = up.host_ptr<int32_t>(o_tiles);
         int4* trtiles = up.host_ptr<int4>(o_tr);
         TrTaskInfo* trinfo = up.host_ptr<TrTaskInfo>(o_trinfo);
-        int64_t gi = 0, ti = 0, tt = 0;
-        if(ngather >= 0) prefix[0] = 0;
-        for(int64_t t = 0; t < NT; ++t) {
-            const xh_permute_task& tk = desc->tasks[t];
-            const int u = route[t];
-            const int64_t idx = u > 0 ? ngather + ti : gi;
-            meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
-            LegInfo* L = legs + idx * NL;
-            for(int k = 0; k < NL; ++k) {
-                const xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
-                L[k] = LegInfo{tk.ext[k], f.magic, f.shift | (tk.is_col[k] ? 256u : 0u), static_cast<uint32_t>(tk.sstride[k])};
+        // element prefix of the gather tasks (in table order)
+        prefix[0] = 0;
+        for(int64_t t = 0; t < NT; ++t)
+            if(route[t] <= 0) prefix[slot[t] + 1] = prefix[slot[t]] + desc->tasks[t].rows * desc->tasks[t].cols;
+        // per-task tables, independent across tasks: host worker threads for large plans
+        xp::parallel_ranges(static_cast<size_t>(NT), 8192, [&](size_t t0, size_t t1) {
+            for(size_t t = t0; t < t1; ++t) {
+                const xh_permute_task& tk = desc->tasks[t];
+                const int u = route[t];
+                const int64_t idx = u > 0 ? ngather + slot[t] : slot[t];
+                meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
+                LegInfo* L = legs + idx * NL;
+                for(int k = 0; k < NL; ++k) {
+                    const xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
+                    L[k] = LegInfo{tk.ext[k], f.magic, f.shift | (tk.is_col[k] ? 256u : 0u), static_cast<uint32_t>(tk.sstride[k])};
+                }
+                if(u > 0) {
+                    int64_t rest = 1;
+                    for(int k = 0; k < NL; ++k)
+                        if(k != 0 && k != u) rest *= tk.ext[k];
+                    const int64_t ti = slot[t];
+                    trinfo[ti] = TrTaskInfo{static_cast<int32_t>(idx), u, tk.ext[0], tk.ext[u], rest};
+                    const int n0 = (tk.ext[0] + TR_DIM - 1) / TR_DIM, nu = (tk.ext[u] + TR_DIM - 1) / TR_DIM;
+                    int64_t tt = tile0[t];
+                    for(int64_t r = 0; r < rest; ++r)
+                        for(int a = 0; a < n0; ++a)
+                            for(int b = 0; b < nu; ++b) trtiles[tt++] = int4{static_cast<int>(ti), b, a, static_cast<int>(r)};
+                }
             }
-            if(u > 0) {
-                int64_t rest = 1;
-                for(int k = 0; k < NL; ++k)
-                    if(k != 0 && k != u) rest *= tk.ext[k];
-                trinfo[ti] = TrTaskInfo{static_cast<int32_t>(idx), u, tk.ext[0], tk.ext[u], rest};
-                const int n0 = (tk.ext[0] + TR_DIM - 1) / TR_DIM, nu = (tk.ext[u] + TR_DIM - 1) / TR_DIM;
-                for(int64_t r = 0; r < rest; ++r)
-                    for(int a = 0; a < n0; ++a)
-                        for(int b = 0; b < nu; ++b) trtiles[tt++] = int4{static_cast<int>(ti), b, a, static_cast<int>(r)};
-                ++ti;
-            } else {
-                prefix[gi + 1] = prefix[gi] + tk.rows * tk.cols;
-                ++gi;
-            }
-        }
+        });
         // gather tiles: first task of each tile, plus a sentinel (last gather task)
         {
             int64_t t = 0;
