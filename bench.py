@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--power-iters", type=int, default=6)
     ap.add_argument("--oversample", type=int, default=16)
+    ap.add_argument("--rank-factor", type=float, default=2.0, help="sketch width l = ceil(rank_factor * k) + oversample")
+    ap.add_argument("--sketch-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the range finder (c4)")
     ap.add_argument("--ranks", default="learned", choices=["learned", "previous"],
                     help="target ranks of the randomized SVDs: the trained predictor or the previous step's kept counts")
     ap.add_argument("--leg-dim", type=int, default=192, help="c3 leg dimension")
@@ -81,18 +83,30 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     c.grow_all()  # common warm-up: full FP64 SVDs, chi grows from D^2 = 64 to chi
     rt.synchronize()
     log(f"[bench] {mode}: warm-up grow_all {time.perf_counter() - t0:.1f} s")
-    if mode == "fast":
-        c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters)
-        if args.ranks == "learned":
-            from xped_amd import rank_predictor
+    def configure(record):
+        if mode == "fast":
+            c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters, record_spectra=record)
+            c.set_rsvd(oversample=args.oversample, rank_factor=args.rank_factor, power_iters=args.power_iters, sketch_precision=args.sketch_precision)
+        else:
+            c.set_options(precision="fp64", svd="full", record_spectra=record)
 
-            params, width = rank_predictor.load_weights()
-            c.set_predictor(params, width)
+    # the first step after the warm-up records its kept spectra (compared between the modes)
+    configure(True)
+    c.clear_spectra()
+    if mode == "fast" and args.ranks == "learned":
+        from xped_amd import rank_predictor
+
+        params, width = rank_predictor.load_weights()
+        c.set_predictor(params, width)
     nsteps = args.steps if mode == "fast" else args.fp64_steps
     nwarm = args.warmup if mode == "fast" else 0
     energies = []
+    spectra = None
     for _ in range(nwarm):
         energies.append(c.step())
+        if spectra is None:
+            spectra = c.spectra()
+            configure(False)
     rt.synchronize()
     svd0 = c.svd_ms()
     c0 = rt.counters()
@@ -105,8 +119,11 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     wall = time.perf_counter() - t0
     c1 = rt.counters()
     h1 = rt.host_stats()
+    if spectra is None:  # no warm-up step: the first timed step recorded (16 decompositions per step)
+        spectra = c.spectra()[:16]
+        configure(False)
     plan_ms = 1e3 * ((h1["permute_plan_s"] - h0["permute_plan_s"]) + (h1["gemm_plan_s"] - h0["gemm_plan_s"])) / nsteps
-    out = {"ms_per_step": 1e3 * wall / nsteps, "host_plan_ms_per_step": plan_ms,
+    out = {"ms_per_step": 1e3 * wall / nsteps, "host_plan_ms_per_step": plan_ms, "spectra": spectra,
            "plans_per_step": ((h1["n_permute_plans"] - h0["n_permute_plans"]) + (h1["n_gemm_plans"] - h0["n_gemm_plans"])) / nsteps,
            "host_breakdown_ms_per_step": [round(1e3 * (b - a) / nsteps, 1) for a, b in zip(h0["breakdown_s"], h1["breakdown_s"])],
            "symbolic_plan_hits_per_step": (h1["symbolic_hits"] - h0["symbolic_hits"]) / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
@@ -126,6 +143,33 @@ def run_mode(mode, args, xt, configs, random_ipeps):
         out["gemm_launches_blocks"] = nlog
         out["gemm_shapes"] = rt.gemm_log()
     return out
+
+
+def compare_spectra(fast, ref):
+    """Kept singular values of the first compared step, decomposition by decomposition (same site
+    and direction order in both modes): the largest difference relative to the decomposition's
+    largest singular value, and the number of sectors whose kept count differs."""
+    import numpy as np
+
+    if not fast or not ref:
+        return None
+    worst, mism, n = 0.0, 0, 0
+    for a, b in zip(fast, ref):
+        if (a["site"], a["dir"]) != (b["site"], b["dir"]):
+            return {"error": "decomposition order differs"}
+        kb = {s["q"]: np.sort(s["sv"])[::-1][: s["kept"]] for s in b["sectors"] if s["kept"] > 0}
+        ka = {s["q"]: np.sort(s["sv"])[::-1][: s["kept"]] for s in a["sectors"] if s["kept"] > 0}
+        smax = max((v[0] for v in kb.values() if len(v)), default=1.0)
+        for q in set(ka) | set(kb):
+            va, vb = ka.get(q, np.zeros(0)), kb.get(q, np.zeros(0))
+            if len(va) != len(vb):
+                mism += 1
+                m = min(len(va), len(vb))
+                va, vb = va[:m], vb[:m]
+            if len(va):
+                worst = max(worst, float(np.abs(va - vb).max()) / smax)
+        n += 1
+    return {"decompositions": n, "max_abs_diff_over_sigma_max": worst, "sectors_with_other_kept_count": mism}
 
 
 def cpu_gemm_baseline(shapes, seconds):
@@ -183,7 +227,7 @@ def run_c4(args):
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[args.precision],
         "data": f"synthetic (site tensors N(0,1)*{SCALE:g}/sqrt(plain size), std::mt19937(seed + site))",
         "config": {"workload": f"U(1) iPEPS CTMRG step, 2x2 cell, D=8, chi={args.chi}, Heisenberg", "chi": args.chi, "D": 8,
-                   "svd": f"randomized, l = 2 k + {args.oversample}, k from the {'learned predictor' if args.ranks == 'learned' else 'previous step'}, "
+                   "svd": f"randomized, l = {args.rank_factor:g} k + {args.oversample}, k from the {'learned predictor' if args.ranks == 'learned' else 'previous step'}, "
                           f"{args.power_iters} power iterations, FP64 fallback",
                    "gemm_precision": args.precision, "seed": args.seed},
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
@@ -204,6 +248,7 @@ def run_c4(args):
                        "host_plan_ms_per_step": round(ref["host_plan_ms_per_step"], 1), "plans_per_step": ref["plans_per_step"]}
         res["speedup_vs_fp64"] = round(ref["ms_per_step"] / fast["ms_per_step"], 3)
         res["energy_rel_err_vs_fp64"] = max(errs) if errs else None
+        res["spectra_vs_fp64"] = compare_spectra(fast["spectra"], ref["spectra"])
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_gemm_baseline(fast["gemm_shapes"], args.cpu_seconds)
     return res

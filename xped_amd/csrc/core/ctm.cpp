@@ -210,6 +210,17 @@ std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb,
         std::map<Q, int> kept;
         for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
         last_kept[{site, dir}] = kept;
+        if(opt.record_spectra) {
+            // the kept singular values (the randomized SVD computes no others exactly)
+            SpectrumRecord rec;
+            rec.site = site;
+            rec.dir = dir;
+            for(const auto& [q, v] : r.spec) rec.sv[q] = v;
+            rec.kept = kept;
+            for(std::size_t i = 0; i < prod.st->sector.size(); ++i)
+                rec.shape[prod.st->sector[i]] = {static_cast<int>(prod.st->rows[i]), static_cast<int>(prod.st->cols[i])};
+            spectra.push_back(std::move(rec));
+        }
     }
     if(rt->ctx) xh_synchronize(rt->ctx);
     const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
