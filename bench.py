@@ -130,14 +130,16 @@ def run_mode(mode, args, xt, configs, random_ipeps):
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
     if mode == "fast":
-        # one more step with every GEMM launch timed by HIP events on the context stream; marker
+        # one more step with every GEMM and permute launch timed by HIP events on the context stream; marker
         # kernels of 1 and 2 workgroups delimit it in a kernel trace
         rt.gemm_timing(True, reset=True)
+        rt.permute_timing(True, reset=True)
         rt.marker(1)
         c.step()
         rt.marker(2)
         ms, fl, nlog = rt.gemm_timing(False)
         out["gemm_launches"] = rt.gemm_launches()
+        out["permute_ms"], out["permute_bytes"], out["permute_launches"] = rt.permute_timing(False, reset=True)
         out["gemm_kernel_ms"] = ms
         out["gemm_timed_flops"] = fl
         out["gemm_launches_blocks"] = nlog
@@ -240,6 +242,12 @@ def run_c4(args):
                      "kernel": "k_gemm_f64 + k_gemm_f64_128 (+ k_splitk_reduce)" if args.precision == "fp64" else f"k_gemm_{args.precision}",
                      "launches_per_step": fast["gemm_launches"], "flops_per_step": fast["gemm_timed_flops"],
                      "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},
+        "permute_roofline": {"bound": "hbm", "kernel": "k_permute_gather + k_permute_transpose",
+                             "achieved": round(fast["permute_bytes"] / (fast["permute_ms"] * 1e-3) / 1e9, 1) if fast["permute_ms"] > 0 else 0.0,
+                             "peak": 8000.0, "unit": "GB/s",
+                             "frac": round(fast["permute_bytes"] / (fast["permute_ms"] * 1e-3) / 1e9 / 8000.0, 4) if fast["permute_ms"] > 0 else 0.0,
+                             "bytes_per_step": fast["permute_bytes"], "launches_per_step": fast["permute_launches"],
+                             "kernel_ms_per_step": round(fast["permute_ms"], 3)},
     }
     if ref is not None:
         nb = min(len(ref["energies"]), len(fast["energies"]))

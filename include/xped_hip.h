@@ -104,6 +104,10 @@ int xh_set_gemm_tiles(xh_ctx* ctx, int mode);
  * the reference): enable (1) / disable (0) / keep (-1); returns the summed kernel time, the flops of
  * the timed launches and their count since the last reset (reset != 0 clears after reading). */
 int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* flops, int64_t* launches);
+/* HIP-event timing of every xh_permute call, as xh_gemm_timing: summed kernel time, the bytes the
+ * timed calls move (8 (elements read + written)) and their count (measurement; no counterpart in
+ * the reference) */
+int xh_permute_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* bytes, int64_t* launches);
 /* an empty kernel of `id` workgroups (1..4096) on the context stream: a mark in a kernel trace
  * that delimits a region of interest (measurement; no counterpart in the reference) */
 int xh_trace_marker(xh_ctx* ctx, int id);

@@ -67,6 +67,20 @@ def test_permute_large_subblocks_transpose_path(rt, pcode):
         assert np.array_equal(tp.download(), op.data), (p, shift)
 
 
+def test_permute_timing_counts_launches_and_bytes(rt):
+    # an abelian permute reads and writes every element once: 16 bytes per FP64 element
+    r = random.Random(4)
+    legs = [rand_entries(xt.U1, r, 3, 40, mindim=16) for _ in range(4)]
+    o, t = both_tensor(rt, xt.U1, legs, 2, 6)
+    rt.permute_timing(True, reset=True)
+    outs = [t.permute(0, (2, 3, 0, 1)), t.permute(1, (3, 1, 2, 0))]
+    ms, nbytes, n = rt.permute_timing(False, reset=True)
+    assert n == 2 and ms > 0.0
+    assert nbytes == 16.0 * sum(x.size for x in outs)
+    assert rt.permute_timing(False)[2] == 0
+    assert np.array_equal(outs[0].download(), oten.permute(o, 0, (2, 3, 0, 1)).data)
+
+
 def test_permute_round_trip_su2(rt):
     r = random.Random(4)
     legs = [rand_entries(xt.SU2, r, 3, 5) for _ in range(4)]

@@ -113,7 +113,7 @@ def test_cholesky_inverse_kernel(rt):
     blocks = []
     # n = 199 is the largest block held in LDS (packed upper triangle); n = 200 and 300 run from
     # a global working copy
-    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6), (160, 141), (260, 180), (230, 199), (240, 200), (400, 300)]:
+    for m, n in [(50, 20), (200, 130), (9, 9), (30, 6), (2, 2), (3, 3), (160, 141), (260, 180), (230, 199), (240, 200), (400, 300)]:
         Y = r.standard_normal((m, n))
         if n > 2:
             Y[:, 2] = Y[:, 0] - 2.0 * Y[:, 1]  # an exactly dependent column

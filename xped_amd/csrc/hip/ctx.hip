@@ -58,6 +58,7 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     for(auto& ev : ctx->events)
         if(ev) hipEventDestroy(ev);
     for(auto& ev : ctx->gemm_events) hipEventDestroy(ev);
+    for(auto& ev : ctx->perm_events) hipEventDestroy(ev);
     if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
     if(ctx->dev_scalar) hipFree(ctx->dev_scalar);
     if(ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
@@ -148,6 +149,29 @@ extern "C" int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel
             ctx->gemm_event_flops.clear();
         }
         if(enable >= 0) ctx->gemm_timing = enable != 0;
+    });
+}
+
+extern "C" int xh_permute_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* bytes, int64_t* launches)
+{
+    return guard(ctx, [&] {
+        double ms = 0.0, b = 0.0;
+        const size_t n = ctx->perm_events_used / 2;
+        if(n > 0) check(hipEventSynchronize(ctx->perm_events[ctx->perm_events_used - 1]), "hipEventSynchronize");
+        for(size_t i = 0; i < n; ++i) {
+            float t = 0.f;
+            check(hipEventElapsedTime(&t, ctx->perm_events[2 * i], ctx->perm_events[2 * i + 1]), "hipEventElapsedTime");
+            ms += t;
+            b += ctx->perm_event_bytes[i];
+        }
+        if(kernel_ms) *kernel_ms = ms;
+        if(bytes) *bytes = b;
+        if(launches) *launches = static_cast<int64_t>(n);
+        if(reset) {
+            ctx->perm_events_used = 0;
+            ctx->perm_event_bytes.clear();
+        }
+        if(enable >= 0) ctx->perm_timing = enable != 0;
     });
 }
 

@@ -36,6 +36,11 @@ struct xh_ctx {
     std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
     size_t gemm_events_used = 0;
     std::vector<double> gemm_event_flops;
+    // the same for permute launches (xh_permute_timing), with the bytes each one moves
+    bool perm_timing = false;
+    std::vector<hipEvent_t> perm_events;
+    size_t perm_events_used = 0;
+    std::vector<double> perm_event_bytes;
 };
 
 // FP64 SVD of large sector blocks by one-sided block Jacobi (bjsvd.hip); same item layout as
@@ -93,6 +98,21 @@ inline void gemm_timer_end(xh_ctx* ctx, double flops)
     check(hipEventRecord(ctx->gemm_events[ctx->gemm_events_used + 1], ctx->stream), "hipEventRecord");
     ctx->gemm_events_used += 2;
     ctx->gemm_event_flops.push_back(flops);
+}
+inline void perm_timer_begin(xh_ctx* ctx)
+{
+    while(ctx->perm_events.size() < ctx->perm_events_used + 2) {
+        hipEvent_t e;
+        check(hipEventCreate(&e), "hipEventCreate");
+        ctx->perm_events.push_back(e);
+    }
+    check(hipEventRecord(ctx->perm_events[ctx->perm_events_used], ctx->stream), "hipEventRecord");
+}
+inline void perm_timer_end(xh_ctx* ctx, double bytes)
+{
+    check(hipEventRecord(ctx->perm_events[ctx->perm_events_used + 1], ctx->stream), "hipEventRecord");
+    ctx->perm_events_used += 2;
+    ctx->perm_event_bytes.push_back(bytes);
 }
 
 // Round-up magic-number division for 0 <= x < 2^31 and 1 <= d < 2^31

@@ -57,24 +57,29 @@ struct PermDev {
     const int4* tr_tiles;  // (task, tile index along the contiguous-source leg, tile index along rows, rest index)
 };
 
+// Source row / column offsets of the output element (row, col) of a task: every output leg's
+// index is peeled off the output row (domain legs) or column (codomain legs) and mapped through
+// its source stride.  NLT > 0 fixes the leg count at compile time (the loop unrolls and the
+// leg records load together); NLT = 0 reads it from nlegs.
+template <int NLT>
 __device__ __forceinline__ void decode_src(const LegInfo* __restrict__ L, int nlegs, int new_rank, uint32_t row, uint32_t col, int64_t& srow,
                                            int64_t& scol)
 {
     srow = 0;
     scol = 0;
-    for(int k = 0; k < new_rank; ++k) {
+    const int nl = NLT > 0 ? NLT : nlegs;
+#pragma unroll
+    for(int k = 0; k < (NLT > 0 ? NLT : XH_MAXLEGS); ++k) {
+        if(NLT == 0 && k >= nl) break;
         const LegInfo li = L[k];
-        uint32_t q = xh::fdiv(row, li.magic, leg_shift(li));
-        uint32_t j = row - q * li.ext;
-        row = q;
-        srow += int64_t(j) * leg_rstride(li);
-        scol += int64_t(j) * leg_cstride(li);
-    }
-    for(int k = new_rank; k < nlegs; ++k) {
-        const LegInfo li = L[k];
-        uint32_t q = xh::fdiv(col, li.magic, leg_shift(li));
-        uint32_t j = col - q * li.ext;
-        col = q;
+        const bool is_row = k < new_rank;
+        const uint32_t x = is_row ? row : col;
+        const uint32_t q = xh::fdiv(x, li.magic, leg_shift(li));
+        const uint32_t j = x - q * li.ext;
+        if(is_row)
+            row = q;
+        else
+            col = q;
         srow += int64_t(j) * leg_rstride(li);
         scol += int64_t(j) * leg_cstride(li);
     }
@@ -91,33 +96,105 @@ __device__ __forceinline__ double gather_one(const PermDev& p, const TaskMeta& t
     return acc;
 }
 
+// Gather path.  A workgroup covers PT_TILE consecutive output elements (in task order, each task
+// column-major in its output subblock); thread i handles elements i, i + PT_THREADS, ...  A tile
+// inside one task reads that task's metadata once (uniform across the workgroup); a tile that
+// spans several tasks stages their element offsets in LDS and finds each element's task by binary
+// search there.
+//
+// Single-source plans (abelian symmetries) run in three phases: every element's source offset,
+// then all PER source loads back to back (independent loads in flight per thread), then the
+// scaled stores.  0 + c v keeps the sign of zero of the general path's accumulation.  Plans with
+// several sources per task (recoupling sums) accumulate per element in the reference's order.
+// NLT: the plan's leg count when it is 2 .. 8, else 0 (read at run time).
+template <bool SINGLE_SRC, int NLT>
 __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const double* __restrict__ src, double* __restrict__ dst)
 {
+    constexpr int PER = PT_TILE / PT_THREADS;
+    __shared__ int32_t start[PT_TILE + 1];  // tile-relative first element of tasks t_lo .. t_hi (clamped at 0)
     const int64_t e0 = int64_t(blockIdx.x) * PT_TILE;
-    const int64_t e1 = min(e0 + PT_TILE, p.total);
+    const int n_here = static_cast<int>(min(int64_t(PT_TILE), p.total - e0));
     const int t_lo = p.tile_first[blockIdx.x];
     const int t_hi = p.tile_first[blockIdx.x + 1];
-    for(int64_t e = e0 + threadIdx.x; e < e1; e += PT_THREADS) {
-        // last task t in [t_lo, t_hi] with prefix[t] <= e
-        int lo = t_lo, hi = t_hi;
-        while(lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if(p.prefix[mid] <= e)
-                lo = mid;
-            else
-                hi = mid - 1;
+    int64_t soff[PER], dpos[PER];
+    double coef[PER];
+    double val[PER];
+    if(t_lo == t_hi) {
+        const TaskMeta tm = p.meta[t_lo];
+        const LegInfo* L = p.legs + int64_t(t_lo) * p.nlegs;
+        const int64_t base = e0 - p.prefix[t_lo];
+        xh_permute_source so{};
+        if constexpr(SINGLE_SRC) so = p.src[tm.src_begin];
+#pragma unroll
+        for(int i = 0; i < PER; ++i) {
+            const int r = threadIdx.x + i * PT_THREADS;
+            dpos[i] = -1;
+            soff[i] = 0;
+            coef[i] = 0.0;
+            val[i] = 0.0;
+            if(r < n_here) {
+                const uint32_t el = static_cast<uint32_t>(base + r);
+                const uint32_t row = el % tm.rows, col = el / tm.rows;
+                int64_t srow, scol;
+                decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
+                dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
+                if constexpr(SINGLE_SRC) {
+                    soff[i] = so.src_off + srow + scol * so.src_ld;
+                    coef[i] = so.coeff;
+                } else {
+                    val[i] = gather_one(p, tm, src, srow, scol);
+                }
+            }
         }
-        const int t = lo;
-        const TaskMeta tm = p.meta[t];
-        const uint32_t el = static_cast<uint32_t>(e - p.prefix[t]);
-        const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
-        // the row index runs over the output domain legs, the column index over the codomain legs
-        uint32_t row = el % tm.rows;
-        uint32_t col = el / tm.rows;
-        int64_t srow, scol;
-        decode_src(L, p.nlegs, p.new_rank, row, col, srow, scol);
-        dst[tm.dst_off + row + int64_t(col) * tm.dst_ld] = gather_one(p, tm, src, srow, scol);
+    } else {
+        // tasks t_lo .. t_hi intersect the tile; t_hi - t_lo <= PT_TILE since every task holds an element
+        const int nt = t_hi - t_lo + 1;
+        for(int j = threadIdx.x; j < nt; j += PT_THREADS) start[j] = static_cast<int32_t>(max(int64_t(0), p.prefix[t_lo + j] - e0));
+        __syncthreads();
+#pragma unroll
+        for(int i = 0; i < PER; ++i) {
+            const int r = threadIdx.x + i * PT_THREADS;
+            dpos[i] = -1;
+            soff[i] = 0;
+            coef[i] = 0.0;
+            val[i] = 0.0;
+            if(r < n_here) {
+                int lo = 0, hi = nt - 1;  // last j with start[j] <= r
+                while(lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if(start[mid] <= r)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int t = t_lo + lo;
+                const TaskMeta tm = p.meta[t];
+                const uint32_t el = static_cast<uint32_t>(e0 + r - p.prefix[t]);
+                const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
+                const uint32_t row = el % tm.rows, col = el / tm.rows;
+                int64_t srow, scol;
+                decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
+                dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
+                if constexpr(SINGLE_SRC) {
+                    const xh_permute_source so = p.src[tm.src_begin];
+                    soff[i] = so.src_off + srow + scol * so.src_ld;
+                    coef[i] = so.coeff;
+                } else {
+                    val[i] = gather_one(p, tm, src, srow, scol);
+                }
+            }
+        }
     }
+    if constexpr(SINGLE_SRC) {
+        // inactive elements load src[0]: a single-source plan reads at least one source element
+#pragma unroll
+        for(int i = 0; i < PER; ++i) val[i] = src[soff[i]];
+#pragma unroll
+        for(int i = 0; i < PER; ++i) val[i] = __dadd_rn(0.0, __dmul_rn(coef[i], val[i]));
+    }
+#pragma unroll
+    for(int i = 0; i < PER; ++i)
+        if(dpos[i] >= 0) dst[dpos[i]] = val[i];
 }
 
 // Transpose path.  For a task whose output leg 0 is not the unit-stride source leg, let leg u be
@@ -214,6 +291,7 @@ struct xh_permute_plan {
     PermDev dev{};
     xh::PackedUpload up;
     int64_t ngather_tiles = 0;
+    bool single_src = true;  // every gather task has exactly one source
     double bytes = 0.0;
     const TrTaskInfo* trinfo = nullptr;
 };
@@ -260,6 +338,7 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                     }
             }
             route[t] = u;
+            if(u <= 0 && tk.nsrc != 1) plan->single_src = false;
             if(u > 0) {
                 int64_t rest = 1;
                 for(int k = 0; k < NL; ++k)
@@ -376,14 +455,41 @@ extern "C" int xh_permute_plan_bytes(const xh_permute_plan* plan, double* bytes)
     return XH_OK;
 }
 
+namespace {
+template <bool SINGLE_SRC>
+void launch_gather_s(xh_ctx* ctx, const xh_permute_plan* plan, const double* src, double* dst)
+{
+    const dim3 grid(static_cast<unsigned>(plan->ngather_tiles)), block(PT_THREADS);
+    switch(plan->dev.nlegs) {
+    case 2: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 2>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 3: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 3>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 4: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 4>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 5: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 5>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 6: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 6>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 7: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 7>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    case 8: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 8>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    default: hipLaunchKernelGGL((k_permute_gather<SINGLE_SRC, 0>), grid, block, 0, ctx->stream, plan->dev, src, dst); break;
+    }
+}
+
+void launch_gather(xh_ctx* ctx, const xh_permute_plan* plan, const double* src, double* dst)
+{
+    if(plan->single_src)
+        launch_gather_s<true>(ctx, plan, src, dst);
+    else
+        launch_gather_s<false>(ctx, plan, src, dst);
+}
+}  // namespace
+
 extern "C" int xh_permute(xh_ctx* ctx, const xh_permute_plan* plan, const double* src, double* dst)
 {
     return guard(ctx, [&] {
         if(!plan) throw xh::Error(XH_EINVAL, "null permute plan");
+        const bool timed = ctx->perm_timing && (plan->ngather_tiles > 0 || plan->dev.ntr_tiles > 0);
+        if(timed) xh::perm_timer_begin(ctx);
         if(plan->ngather_tiles > 0) {
             if(plan->ngather_tiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
-            hipLaunchKernelGGL(k_permute_gather, dim3(static_cast<unsigned>(plan->ngather_tiles)), dim3(PT_THREADS), 0, ctx->stream, plan->dev, src,
-                               dst);
+            launch_gather(ctx, plan, src, dst);
             xh::launch_check("k_permute_gather");
         }
         if(plan->dev.ntr_tiles > 0) {
@@ -392,5 +498,6 @@ extern "C" int xh_permute(xh_ctx* ctx, const xh_permute_plan* plan, const double
                                plan->trinfo, src, dst);
             xh::launch_check("k_permute_transpose");
         }
+        if(timed) xh::perm_timer_end(ctx, plan->bytes);
     });
 }

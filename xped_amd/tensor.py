@@ -79,6 +79,12 @@ class Runtime:
         lib().xt_rt_gemm_timing(self.h, int(enable), int(reset), C.byref(ms), C.byref(fl), C.byref(n))
         return ms.value, fl.value, n.value
 
+    def permute_timing(self, enable: bool, reset: bool = False):
+        """HIP-event timing of every permute launch; returns (kernel ms, bytes moved, launches)."""
+        ms, b, n = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().xh_permute_timing(self.ctx(), int(enable), int(reset), C.byref(ms), C.byref(b), C.byref(n)), self.h, "permute timing")
+        return ms.value, b.value, n.value
+
     def gemm_launches(self) -> int:
         """Number of xh_gemm calls timed since the last reset."""
         ms, fl, n = C.c_double(), C.c_double(), C.c_int64()
