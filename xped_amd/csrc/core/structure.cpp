@@ -825,37 +825,49 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
         const int lo = m < R ? 0 : R;
         for(int mm = lo; mm < m; ++mm) before[k][nbefore[k]++] = Pfull.pi_inv[mm];
     }
-    std::vector<std::size_t> moved(out.sector.size(), 0);
-    parallel_ranges(out.sector.size(), plan.tasks.size() >= 16384 ? 2 : out.sector.size() + 1, [&](std::size_t s0, std::size_t s1) {
-        for(std::size_t s = s0; s < s1; ++s) {
+    // one work row per (output sector, domain tree): rows balance the host workers far better
+    // than sectors, whose task counts differ by orders of magnitude
+    struct Row {
+        std::uint32_t s, a;
+        std::size_t roff;
+    };
+    std::vector<Row> rows;
+    for(std::size_t s = 0; s < out.sector.size(); ++s) {
+        const auto& dtrees = out.dom.tree(out.sector[s]);
+        std::size_t roff = 0;
+        for(std::size_t a = 0; a < dtrees.size(); ++a) {
+            rows.push_back(Row{static_cast<std::uint32_t>(s), static_cast<std::uint32_t>(a), roff});
+            roff += dtrees[a].dim;
+        }
+    }
+    std::vector<std::size_t> moved(rows.size(), 0);
+    const std::size_t min_rows = plan.tasks.size() >= 16384 ? std::max<std::size_t>(1, rows.size() * 4096 / plan.tasks.size()) : rows.size() + 1;
+    parallel_ranges(rows.size(), min_rows, [&](std::size_t r0, std::size_t r1) {
+        for(std::size_t r = r0; r < r1; ++r) {
+            const std::size_t s = rows[r].s, a = rows[r].a, roff = rows[r].roff;
             const Q& q = out.sector[s];
-            const auto& dtrees = out.dom.tree(q);
+            const FTree& t1 = out.dom.tree(q)[a];
             const auto& ctrees = out.cod.tree(q);
-            std::size_t roff = 0;
-            for(std::size_t a = 0; a < dtrees.size(); ++a) {
-                const FTree& t1 = dtrees[a];
-                std::size_t coff = 0;
-                for(std::size_t b = 0; b < ctrees.size(); ++b) {
-                    const FTree& t2 = ctrees[b];
-                    const std::size_t id = static_cast<std::size_t>(sp.task_base[s]) + a * ctrees.size() + b;
-                    PermuteTask& task = plan.tasks[id];
-                    task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
-                    task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
-                    task.rows = static_cast<std::int64_t>(t1.dim);
-                    task.cols = static_cast<std::int64_t>(t2.dim);
-                    for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
-                    for(int k = 0; k < N; ++k) {
-                        std::int64_t st = 1;
-                        for(int u = 0; u < nbefore[k]; ++u) st *= task.ext[before[k][u]];
-                        task.sstride[k] = st;
-                        task.is_col[k] = p[k] < R ? 0 : 1;
-                    }
-                    task.src_begin = sp.src_begin[id];
-                    task.nsrc = sp.nsrc[id];
-                    moved[s] += t1.dim * t2.dim;
-                    coff += t2.dim;
+            std::size_t coff = 0;
+            for(std::size_t b = 0; b < ctrees.size(); ++b) {
+                const FTree& t2 = ctrees[b];
+                const std::size_t id = static_cast<std::size_t>(sp.task_base[s]) + a * ctrees.size() + b;
+                PermuteTask& task = plan.tasks[id];
+                task.dst_off = static_cast<std::int64_t>(out.offsets[s] + roff + coff * out.rows[s]);
+                task.dst_ld = static_cast<std::int64_t>(out.rows[s]);
+                task.rows = static_cast<std::int64_t>(t1.dim);
+                task.cols = static_cast<std::int64_t>(t2.dim);
+                for(int k = 0; k < N; ++k) task.ext[k] = k < newR ? t1.dims[k] : t2.dims[k - newR];
+                for(int k = 0; k < N; ++k) {
+                    std::int64_t st = 1;
+                    for(int u = 0; u < nbefore[k]; ++u) st *= task.ext[before[k][u]];
+                    task.sstride[k] = st;
+                    task.is_col[k] = p[k] < R ? 0 : 1;
                 }
-                roff += t1.dim;
+                task.src_begin = sp.src_begin[id];
+                task.nsrc = sp.nsrc[id];
+                moved[r] += t1.dim * t2.dim;
+                coff += t2.dim;
             }
         }
     });

@@ -306,52 +306,83 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         if(desc->new_rank < 0 || desc->new_rank > NL) throw xh::Error(XH_EINVAL, "permute plan: invalid new_rank");
         const int64_t NT = desc->ntasks;
         if(NT < 0 || NT > 0x7fffffff || desc->nsources < 0) throw xh::Error(XH_EINVAL, "permute plan: invalid task or source count");
-        // pass 1: validate every task and pick its kernel (-1: gather, u > 0: transpose via leg u)
+        // pass 1: validate every task and pick its kernel (-1: gather, u > 0: transpose via leg u),
+        // in chunks on the host worker pool for large plans; slot and tile0 are first numbered
+        // within the chunk and shifted by the chunk's base once all chunks are counted
         std::vector<int32_t> route(static_cast<size_t>(NT));
         std::vector<int64_t> slot(static_cast<size_t>(NT)), tile0(static_cast<size_t>(NT), 0);  // table row; first transpose tile
+        struct ChunkCount {
+            int64_t ngather = 0, ntr = 0, ntr_tiles = 0, total = 0;
+            double moved = 0.0, read = 0.0;
+            bool single = true;
+        };
+        const size_t nchunk = NT >= 16384 ? std::max<size_t>(1, std::min<size_t>(xp::WorkerPool::get().size(), size_t(NT) / 4096)) : 1;
+        std::vector<ChunkCount> cc(nchunk);
+        auto chunk_lo = [&](size_t c) { return int64_t(size_t(NT) * c / nchunk); };
+        auto classify = [&](size_t c) {
+            ChunkCount& k = cc[c];
+            for(int64_t t = chunk_lo(c); t < chunk_lo(c + 1); ++t) {
+                const xh_permute_task& tk = desc->tasks[t];
+                if(tk.rows < 0 || tk.cols < 0 || tk.rows * tk.cols >= (int64_t(1) << 31))
+                    throw xh::Error(XH_EINVAL, "permute plan: subblock too large for 32-bit indexing");
+                if(tk.nsrc < 0 || tk.src_begin < 0 || tk.src_begin + tk.nsrc > desc->nsources)
+                    throw xh::Error(XH_EINVAL, "permute plan: source range out of bounds");
+                int64_t rprod = 1, cprod = 1;
+                for(int q = 0; q < NL; ++q) {
+                    if(tk.ext[q] == 0) throw xh::Error(XH_EINVAL, "permute plan: zero extent");
+                    if(tk.sstride[q] < 0 || tk.sstride[q] >= (int64_t(1) << 31)) throw xh::Error(XH_EINVAL, "permute plan: source stride out of range");
+                    (q < desc->new_rank ? rprod : cprod) *= tk.ext[q];
+                }
+                if(rprod != tk.rows || cprod != tk.cols) throw xh::Error(XH_EINVAL, "permute plan: leg extents do not match the subblock shape");
+                const double elems = double(tk.rows) * double(tk.cols);
+                k.moved += elems;
+                k.read += elems * tk.nsrc;
+                // transpose path when output leg 0 is a real leg (ext >= 16) whose source stride is
+                // not 1 and some other leg with ext >= 16 has unit source stride
+                int u = -1;
+                if(desc->new_rank >= 1 && NL >= 2 && tk.nsrc >= 1 && tk.sstride[0] != 1 && !tk.is_col[0] && tk.ext[0] >= 16) {
+                    for(int q = 1; q < NL; ++q)
+                        if(tk.sstride[q] == 1 && !tk.is_col[q] && tk.ext[q] >= 16) {
+                            u = q;
+                            break;
+                        }
+                }
+                route[t] = u;
+                if(u <= 0 && tk.nsrc != 1) k.single = false;
+                if(u > 0) {
+                    int64_t rest = 1;
+                    for(int q = 0; q < NL; ++q)
+                        if(q != 0 && q != u) rest *= tk.ext[q];
+                    slot[t] = k.ntr;  // offset by ngather below
+                    tile0[t] = k.ntr_tiles;
+                    k.ntr_tiles += rest * ((tk.ext[0] + TR_DIM - 1) / TR_DIM) * ((tk.ext[u] + TR_DIM - 1) / TR_DIM);
+                    ++k.ntr;
+                } else {
+                    slot[t] = k.ngather;
+                    ++k.ngather;
+                    k.total += tk.rows * tk.cols;
+                }
+            }
+        };
+        if(nchunk > 1)
+            xp::WorkerPool::get().run(nchunk, classify);
+        else
+            classify(0);
+        std::vector<ChunkCount> base(nchunk);  // per chunk: counts of the chunks before it
         int64_t ngather = 0, ntr = 0, ntr_tiles = 0, total = 0;
         double moved = 0.0, read = 0.0;
-        for(int64_t t = 0; t < NT; ++t) {
-            const xh_permute_task& tk = desc->tasks[t];
-            if(tk.rows < 0 || tk.cols < 0 || tk.rows * tk.cols >= (int64_t(1) << 31))
-                throw xh::Error(XH_EINVAL, "permute plan: subblock too large for 32-bit indexing");
-            if(tk.nsrc < 0 || tk.src_begin < 0 || tk.src_begin + tk.nsrc > desc->nsources)
-                throw xh::Error(XH_EINVAL, "permute plan: source range out of bounds");
-            int64_t rprod = 1, cprod = 1;
-            for(int k = 0; k < NL; ++k) {
-                if(tk.ext[k] == 0) throw xh::Error(XH_EINVAL, "permute plan: zero extent");
-                if(tk.sstride[k] < 0 || tk.sstride[k] >= (int64_t(1) << 31)) throw xh::Error(XH_EINVAL, "permute plan: source stride out of range");
-                (k < desc->new_rank ? rprod : cprod) *= tk.ext[k];
-            }
-            if(rprod != tk.rows || cprod != tk.cols) throw xh::Error(XH_EINVAL, "permute plan: leg extents do not match the subblock shape");
-            const double elems = double(tk.rows) * double(tk.cols);
-            moved += elems;
-            read += elems * tk.nsrc;
-            // transpose path when output leg 0 is a real leg (ext >= 16) whose source stride is not 1
-            // and some other leg with ext >= 16 has unit source stride
-            int u = -1;
-            if(desc->new_rank >= 1 && NL >= 2 && tk.nsrc >= 1 && tk.sstride[0] != 1 && !tk.is_col[0] && tk.ext[0] >= 16) {
-                for(int k = 1; k < NL; ++k)
-                    if(tk.sstride[k] == 1 && !tk.is_col[k] && tk.ext[k] >= 16) {
-                        u = k;
-                        break;
-                    }
-            }
-            route[t] = u;
-            if(u <= 0 && tk.nsrc != 1) plan->single_src = false;
-            if(u > 0) {
-                int64_t rest = 1;
-                for(int k = 0; k < NL; ++k)
-                    if(k != 0 && k != u) rest *= tk.ext[k];
-                slot[t] = ntr;  // offset by ngather below
-                tile0[t] = ntr_tiles;
-                ntr_tiles += rest * ((tk.ext[0] + TR_DIM - 1) / TR_DIM) * ((tk.ext[u] + TR_DIM - 1) / TR_DIM);
-                ++ntr;
-            } else {
-                slot[t] = ngather;
-                ++ngather;
-                total += tk.rows * tk.cols;
-            }
+        for(size_t c = 0; c < nchunk; ++c) {
+            base[c].ngather = ngather;
+            base[c].ntr = ntr;
+            base[c].ntr_tiles = ntr_tiles;
+            base[c].total = total;
+            ngather += cc[c].ngather;
+            ntr += cc[c].ntr;
+            ntr_tiles += cc[c].ntr_tiles;
+            total += cc[c].total;
+            moved += cc[c].moved;
+            read += cc[c].read;
+            if(!cc[c].single) plan->single_src = false;
         }
         if(ntr_tiles > 0x7fffffff) throw xh::Error(XH_EINVAL, "permute: too many tiles");
         const int64_t ntiles = (total + PT_TILE - 1) / PT_TILE;
@@ -373,15 +404,24 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
         int32_t* tile_first = up.host_ptr<int32_t>(o_tiles);
         int4* trtiles = up.host_ptr<int4>(o_tr);
         TrTaskInfo* trinfo = up.host_ptr<TrTaskInfo>(o_trinfo);
-        // element prefix of the gather tasks (in table order)
-        prefix[0] = 0;
-        for(int64_t t = 0; t < NT; ++t)
-            if(route[t] <= 0) prefix[slot[t] + 1] = prefix[slot[t]] + desc->tasks[t].rows * desc->tasks[t].cols;
-        // per-task tables, independent across tasks: host worker threads for large plans
-        xp::parallel_ranges(static_cast<size_t>(NT), 8192, [&](size_t t0, size_t t1) {
-            for(size_t t = t0; t < t1; ++t) {
+        // second pass over the same chunks: final slots and first transpose tiles, the element
+        // prefix of the gather tasks (in table order), the first task of every gather tile (the
+        // tiles that start inside a task) and the per-task device tables
+        auto place = [&](size_t c) {
+            int64_t e = base[c].total;
+            for(int64_t t = chunk_lo(c); t < chunk_lo(c + 1); ++t) {
                 const xh_permute_task& tk = desc->tasks[t];
                 const int u = route[t];
+                if(u > 0) {
+                    slot[t] += base[c].ntr;
+                    tile0[t] += base[c].ntr_tiles;
+                } else {
+                    slot[t] += base[c].ngather;
+                    prefix[slot[t]] = e;
+                    const int64_t e_end = e + tk.rows * tk.cols;
+                    for(int64_t i = (e + PT_TILE - 1) / PT_TILE; i * PT_TILE < e_end; ++i) tile_first[i] = static_cast<int32_t>(slot[t]);
+                    e = e_end;
+                }
                 const int64_t idx = u > 0 ? ngather + slot[t] : slot[t];
                 meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
                 LegInfo* L = legs + idx * NL;
@@ -402,17 +442,13 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                             for(int b = 0; b < nu; ++b) trtiles[tt++] = int4{static_cast<int>(ti), b, a, static_cast<int>(r)};
                 }
             }
-        });
-        // gather tiles: first task of each tile, plus a sentinel (last gather task)
-        {
-            int64_t t = 0;
-            for(int64_t i = 0; i < ntiles; ++i) {
-                const int64_t e = i * PT_TILE;
-                while(t + 1 < ngather && prefix[t + 1] <= e) ++t;
-                tile_first[i] = static_cast<int32_t>(t);
-            }
-            tile_first[ntiles] = static_cast<int32_t>(ngather > 0 ? ngather - 1 : 0);
-        }
+        };
+        if(nchunk > 1)
+            xp::WorkerPool::get().run(nchunk, place);
+        else
+            place(0);
+        prefix[ngather] = total;
+        tile_first[ntiles] = static_cast<int32_t>(ngather > 0 ? ngather - 1 : 0);  // sentinel: the last gather task
         if(desc->nsources > 0)
             std::memcpy(up.host_ptr<xh_permute_source>(o_src), desc->sources, size_t(desc->nsources) * sizeof(xh_permute_source));
         plan->dev.nlegs = NL;
