@@ -417,7 +417,7 @@ int xt_tensor_leg(const xt_tensor* t, int leg, xt_basis** out)
 
 int xt_tensor_coupled(const xt_tensor* t, int side, xt_basis** out)
 {
-    return run(t ? t->owner : nullptr, [&] { *out = new xt_basis{side == 0 ? t->t.st->dom : t->t.st->cod}; });
+    return run(t ? t->owner : nullptr, [&] { *out = new xt_basis{side == 0 ? t->t.st->dom() : t->t.st->cod()}; });
 }
 
 int xt_tensor_upload(xt_tensor* t, const double* host, size_t n)

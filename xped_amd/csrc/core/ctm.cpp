@@ -435,8 +435,8 @@ double CTM::energy(double Jxy, double Jz) const
         std::vector<double> h(gs->size, 0.0);
         for(std::size_t i = 0; i < gs->sector.size(); ++i) {
             const Q& q = gs->sector[i];
-            const auto& dtrees = gs->dom.tree(q);
-            const auto& ctrees = gs->cod.tree(q);
+            const auto& dtrees = gs->dom().tree(q);
+            const auto& ctrees = gs->cod().tree(q);
             std::size_t r0 = 0;
             for(const auto& td : dtrees) {
                 std::size_t c0 = 0;

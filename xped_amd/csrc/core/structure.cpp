@@ -392,6 +392,39 @@ Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled)
     return b;
 }
 
+std::shared_ptr<const Qbasis> coupled_basis(const Sym& s, const std::vector<Qbasis>& uncoupled)
+{
+    bool depth1 = uncoupled.size() >= 2;
+    for(const auto& b : uncoupled) depth1 = depth1 && b.depth == 1;
+    if(!depth1) return std::make_shared<const Qbasis>(build_fusion_tree(s, uncoupled));
+    std::string key = charge_pattern(s, uncoupled);
+    for(const auto& b : uncoupled)
+        for(const auto& e : b.data) {
+            const long long d = static_cast<long long>(e.dim);
+            key.append(reinterpret_cast<const char*>(&d), sizeof(d));
+        }
+    // bounded by the trees held: a full cache is dropped (its bases stay alive in their structures)
+    static std::mutex basis_mutex;
+    static std::unordered_map<std::string, std::shared_ptr<const Qbasis>> basis_cache;
+    static std::size_t held_trees = 0;
+    constexpr std::size_t max_held_trees = std::size_t(1) << 21;
+    {
+        std::lock_guard<std::mutex> lk(basis_mutex);
+        auto it = basis_cache.find(key);
+        if(it != basis_cache.end()) return it->second;
+    }
+    auto b = std::make_shared<const Qbasis>(build_fusion_tree(s, uncoupled));
+    std::size_t nt = 0;
+    for(const auto& e : b->data) nt += e.trees.size();
+    std::lock_guard<std::mutex> lk(basis_mutex);
+    if(held_trees + nt > max_held_trees) {
+        basis_cache.clear();
+        held_trees = 0;
+    }
+    if(basis_cache.emplace(std::move(key), b).second) held_trees += nt;
+    return b;
+}
+
 // ---------------------------------------------------------------- tree moves
 
 TreeMap tree_swap(const Sym& s, const FTree& t, int pos)
@@ -589,7 +622,7 @@ static void finish_layout(TensorStructure& t)
     t.cols.clear();
     for(const Q& q : t.sector) {
         t.offsets.push_back(curr);
-        std::size_t r = t.dom.inner_dim(q), c = t.cod.inner_dim(q);
+        std::size_t r = t.dom().inner_dim(q), c = t.cod().inner_dim(q);
         t.rows.push_back(r);
         t.cols.push_back(c);
         curr += r * c;
@@ -606,10 +639,10 @@ StructPtr TensorStructure::create(const Sym& s, const std::vector<Qbasis>& d, co
     if(t->rank > MAXR || t->corank > MAXR) throw std::invalid_argument("too many legs");
     t->unc_dom = d;
     t->unc_cod = c;
-    t->dom = build_fusion_tree(s, d);
-    t->cod = build_fusion_tree(s, c);
-    for(const auto& e : t->dom.data)
-        if(t->cod.present(e.q)) t->sector.push_back(e.q);
+    t->dom_p = coupled_basis(s, d);
+    t->cod_p = coupled_basis(s, c);
+    for(const auto& e : t->dom().data)
+        if(t->cod().present(e.q)) t->sector.push_back(e.q);
     finish_layout(*t);
     return t;
 }
@@ -622,8 +655,8 @@ StructPtr TensorStructure::create_pushed(const Sym& s, const std::vector<Qbasis>
     t->corank = static_cast<int>(c.size());
     t->unc_dom = d;
     t->unc_cod = c;
-    t->dom = build_fusion_tree(s, d);
-    t->cod = build_fusion_tree(s, c);
+    t->dom_p = coupled_basis(s, d);
+    t->cod_p = coupled_basis(s, c);
     t->sector = sectors;
     finish_layout(*t);
     return t;
@@ -687,8 +720,8 @@ SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStr
     for(std::size_t s = 0; s < out.sector.size(); ++s) {
         const Q& q = out.sector[s];
         sp.task_base.push_back(static_cast<std::int32_t>(ntasks));
-        sp.ncod.push_back(static_cast<std::int32_t>(out.cod.tree(q).size()));
-        ntasks += static_cast<int64_t>(out.dom.tree(q).size()) * out.cod.tree(q).size();
+        sp.ncod.push_back(static_cast<std::int32_t>(out.cod().tree(q).size()));
+        ntasks += static_cast<int64_t>(out.dom().tree(q).size()) * out.cod().tree(q).size();
     }
     if(ntasks > 0x7fffffff) throw std::length_error("permute: too many subblocks");
 
@@ -708,26 +741,26 @@ SymbolicPermute make_symbolic_permute(const TensorStructure& in, const TensorStr
     };
     std::vector<Work> work;
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
-        const auto& dtrees = in.dom.tree(in.sector[i]);
+        const auto& dtrees = in.dom().tree(in.sector[i]);
         for(std::size_t a = 0; a < dtrees.size(); ++a) work.push_back(Work{static_cast<std::int32_t>(i), static_cast<std::int32_t>(a)});
     }
     // build the lazily created tree indices before threads share the bases
-    for(const Qbasis* b : {&out.dom, &out.cod})
+    for(const Qbasis* b : {&out.dom(), &out.cod()})
         if(!b->data.empty() && !b->data[0].trees.empty()) (void)b->left_offset(b->data[0].trees[0]);
     using Contrib = std::vector<std::pair<std::int32_t, SymbolicPermute::Src>>;
     auto run_range = [&](std::size_t w0, std::size_t w1, Contrib& res) {
         for(std::size_t w = w0; w < w1; ++w) {
             const std::int32_t i = work[w].sector;
             const Q& q = in.sector[i];
-            const FTree& t1 = in.dom.tree(q)[work[w].row_tree];
-            const auto& ctrees = in.cod.tree(q);
+            const FTree& t1 = in.dom().tree(q)[work[w].row_tree];
+            const auto& ctrees = in.cod().tree(q);
             for(std::size_t b2 = 0; b2 < ctrees.size(); ++b2) {
                 const FTree& t2 = ctrees[b2];
                 auto add = [&](const FTree& a, const FTree& b, double c) {
                     if(std::abs(c) < PERMUTE_SKIP) return;
                     const int so = out.find_sector(a.coupled);
                     if(so < 0) throw std::logic_error("permute: output sector missing");
-                    const std::size_t ia = out.dom.tree_pos(a).first, ib = out.cod.tree_pos(b).first;
+                    const std::size_t ia = out.dom().tree_pos(a).first, ib = out.cod().tree_pos(b).first;
                     const std::int32_t task = sp.task_base[so] + static_cast<std::int32_t>(ia) * sp.ncod[so] + static_cast<std::int32_t>(ib);
                     res.emplace_back(task, SymbolicPermute::Src{i, work[w].row_tree, static_cast<std::int32_t>(b2), c});
                 };
@@ -816,7 +849,7 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
     spare_sources = std::vector<PermuteSource>();
     plan.tasks.resize(sp.src_begin.size());
     for(std::size_t s = 0; s < out.sector.size(); ++s)
-        if(static_cast<std::int32_t>(out.cod.tree(out.sector[s]).size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
+        if(static_cast<std::int32_t>(out.cod().tree(out.sector[s]).size()) != sp.ncod[s]) throw std::logic_error("permute: symbolic plan does not match the structure");
     // source stride of output leg k = product of the output extents of the source legs before p[k]
     std::array<std::array<int, 2 * MAXR>, 2 * MAXR> before{};
     std::array<int, 2 * MAXR> nbefore{};
@@ -833,7 +866,7 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
     };
     std::vector<Row> rows;
     for(std::size_t s = 0; s < out.sector.size(); ++s) {
-        const auto& dtrees = out.dom.tree(out.sector[s]);
+        const auto& dtrees = out.dom().tree(out.sector[s]);
         std::size_t roff = 0;
         for(std::size_t a = 0; a < dtrees.size(); ++a) {
             rows.push_back(Row{static_cast<std::uint32_t>(s), static_cast<std::uint32_t>(a), roff});
@@ -846,8 +879,8 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
         for(std::size_t r = r0; r < r1; ++r) {
             const std::size_t s = rows[r].s, a = rows[r].a, roff = rows[r].roff;
             const Q& q = out.sector[s];
-            const FTree& t1 = out.dom.tree(q)[a];
-            const auto& ctrees = out.cod.tree(q);
+            const FTree& t1 = out.dom().tree(q)[a];
+            const auto& ctrees = out.cod().tree(q);
             std::size_t coff = 0;
             for(std::size_t b = 0; b < ctrees.size(); ++b) {
                 const FTree& t2 = ctrees[b];
@@ -874,8 +907,8 @@ PermutePlan numeric_permute_plan(const SymbolicPermute& sp, const TensorStructur
     for(std::size_t m : moved) plan.moved_elems += m;
     std::vector<std::vector<std::size_t>> in_roff(in.sector.size()), in_coff(in.sector.size());
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
-        in_roff[i] = tree_offsets(in.dom, in.sector[i]);
-        in_coff[i] = tree_offsets(in.cod, in.sector[i]);
+        in_roff[i] = tree_offsets(in.dom(), in.sector[i]);
+        in_coff[i] = tree_offsets(in.cod(), in.sector[i]);
     }
     plan.sources.resize(sp.sources.size());
     parallel_ranges(sp.sources.size(), 16384, [&](std::size_t k0, std::size_t k1) {
@@ -938,7 +971,7 @@ PermutePlan make_permute_adj_plan(const StructPtr& xs, const TensorStructure& y,
         Pd = std::make_unique<Permutation>(pdv);
         Pc = std::make_unique<Permutation>(pcv);
     }
-    for(const Qbasis* b : {&y.dom, &y.cod})
+    for(const Qbasis* b : {&y.dom(), &y.cod()})
         if(!b->data.empty() && !b->data[0].trees.empty()) (void)b->left_offset(b->data[0].trees[0]);
     PermutePlan plan;
     plan.nlegs = N;
@@ -947,9 +980,9 @@ PermutePlan make_permute_adj_plan(const StructPtr& xs, const TensorStructure& y,
     for(std::size_t i = 0; i < in.sector.size(); ++i) {
         const Q& q = in.sector[i];
         std::size_t roff = 0;
-        for(const auto& t1 : in.dom.tree(q)) {
+        for(const auto& t1 : in.dom().tree(q)) {
             std::size_t coff = 0;
-            for(const auto& t2 : in.cod.tree(q)) {
+            for(const auto& t2 : in.cod().tree(q)) {
                 PermuteTask task;
                 task.dst_off = static_cast<std::int64_t>(in.offsets[i] + roff + coff * in.rows[i]);
                 task.dst_ld = static_cast<std::int64_t>(in.rows[i]);
@@ -974,7 +1007,7 @@ PermutePlan make_permute_adj_plan(const StructPtr& xs, const TensorStructure& y,
                     if(std::abs(c) < PERMUTE_SKIP) return;
                     const int so = y.find_sector(a.coupled);
                     if(so < 0) throw std::logic_error("permute_adj: image sector missing");
-                    const std::size_t ro = y.dom.left_offset(a), co = y.cod.left_offset(b);
+                    const std::size_t ro = y.dom().left_offset(a), co = y.cod().left_offset(b);
                     plan.sources.push_back(PermuteSource{static_cast<std::int64_t>(y.offsets[so] + ro + co * y.rows[so]), static_cast<std::int64_t>(y.rows[so]), c});
                 };
                 if(split) {

@@ -161,6 +161,9 @@ struct Qbasis {
 };
 
 Qbasis build_fusion_tree(const Sym& s, const std::vector<Qbasis>& uncoupled);  // TensorHelper.cpp:14-44
+// build_fusion_tree through a process-wide cache keyed by the exact legs (charges, dimensions and
+// dual flags of depth-1 legs); the result is immutable and shared
+std::shared_ptr<const Qbasis> coupled_basis(const Sym& s, const std::vector<Qbasis>& uncoupled);
 
 // Fusion tree moves.
 TreeMap tree_swap(const Sym& s, const FTree& t, int pos);                       // FusionTree.cpp:353-412
@@ -175,7 +178,10 @@ struct TensorStructure {
     Sym sym;
     int rank = 0, corank = 0;
     std::vector<Qbasis> unc_dom, unc_cod;  // depth-1 legs
-    Qbasis dom, cod;                       // coupled (fusion-tree) bases
+    // coupled (fusion-tree) bases, shared by all structures built from the same legs
+    std::shared_ptr<const Qbasis> dom_p, cod_p;
+    const Qbasis& dom() const { return *dom_p; }
+    const Qbasis& cod() const { return *cod_p; }
     std::vector<Q> sector;
     std::vector<std::size_t> offsets, rows, cols;
     std::size_t size = 0;

@@ -444,9 +444,9 @@ DTensor twist(const DTensor& t, int leg)
     std::vector<xh_rect> rects;
     for(std::size_t i = 0; i < s.sector.size(); ++i) {
         std::size_t ro = 0;
-        for(const auto& t1 : s.dom.tree(s.sector[i])) {
+        for(const auto& t1 : s.dom().tree(s.sector[i])) {
             std::size_t co = 0;
-            for(const auto& t2 : s.cod.tree(s.sector[i])) {
+            for(const auto& t2 : s.cod().tree(s.sector[i])) {
                 double c = leg < s.rank ? s.sym.coeff_twist(t1.unc[leg]) : s.sym.coeff_twist(t2.unc[leg - s.rank]);
                 if(c != 1.0)
                     rects.push_back(xh_rect{static_cast<int64_t>(s.offsets[i] + ro + co * s.rows[i]), static_cast<int64_t>(s.rows[i]),
@@ -662,7 +662,7 @@ std::pair<DTensor, DTensor> tqr(const DTensor& t)
 {
     t.rt->require_device();
     const TensorStructure& s = *t.st;
-    Qbasis middle = s.dom.dim() > s.cod.dim() ? s.cod.forget_history() : s.dom.forget_history();
+    Qbasis middle = s.dom().dim() > s.cod().dim() ? s.cod().forget_history() : s.dom().forget_history();
     DTensor Qt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {middle}, s.sector));
     DTensor Rt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {middle}, s.unc_cod, s.sector));
     std::vector<xh_factor_item> items;
@@ -680,7 +680,7 @@ std::pair<DTensor, DTensor> teigh(const DTensor& t)
     t.rt->require_device();
     const TensorStructure& s = *t.st;
     if(s.rank != s.corank) throw Error(XH_EINVAL, "teigh needs Rank == CoRank");
-    Qbasis dh = s.dom.forget_history(), ch = s.cod.forget_history();
+    Qbasis dh = s.dom().forget_history(), ch = s.cod().forget_history();
     DTensor D = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {dh}, {ch}, s.sector));
     DTensor V = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {ch}, s.sector));
     std::vector<xh_factor_item> items;
