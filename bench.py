@@ -23,7 +23,9 @@ copies 1 GiB (reads 1 GiB, writes 1 GiB) three times; it prints the copy bandwid
 
 `roofline.traffic` (HBM bytes per launch of the dominant kernel) is read from the rocprofv3
 counter summary in profiles/ when one is present for the workload (FETCH_SIZE + WRITE_SIZE,
-scaled by the copy calibration), else null.
+scaled by the copy calibration), else null.  `permute_roofline` reports the permute kernels of
+the same step against the HBM peak: algorithmic bytes 8 (elements read + written) over their
+HIP-event kernel time, with their counter-based traffic per launch from profiles/ likewise.
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -246,6 +248,7 @@ def run_c4(args):
                              "achieved": round(fast["permute_bytes"] / (fast["permute_ms"] * 1e-3) / 1e9, 1) if fast["permute_ms"] > 0 else 0.0,
                              "peak": 8000.0, "unit": "GB/s",
                              "frac": round(fast["permute_bytes"] / (fast["permute_ms"] * 1e-3) / 1e9 / 8000.0, 4) if fast["permute_ms"] > 0 else 0.0,
+                             "traffic": measured_traffic("k_permute_ctm_step_counters.md") if args.chi == 256 else None,
                              "bytes_per_step": fast["permute_bytes"], "launches_per_step": fast["permute_launches"],
                              "kernel_ms_per_step": round(fast["permute_ms"], 3)},
     }

@@ -38,7 +38,7 @@ extern "C" int xh_create(int device, xh_ctx** out)
         uint64_t thr = UINT64_MAX;
         check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr), "hipMemPoolSetAttribute");
         check(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scalar), 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
-        check(hipMalloc(reinterpret_cast<void**>(&ctx->dev_scalar), 64 * sizeof(double)), "hipMalloc");
+        check(hipMalloc(reinterpret_cast<void**>(&ctx->dev_scalar), (xh::RED_PARTS + 64) * sizeof(double)), "hipMalloc");
     });
     if(rc != XH_OK) {
         delete ctx;
@@ -505,15 +505,19 @@ static void reduce(xh_ctx* ctx, const double* x, const double* y, size_t n, doub
         *out = 0.0;
         return;
     }
-    const int nb = static_cast<int>(std::min<size_t>(32, (n + EW_THREADS - 1) / EW_THREADS));
+    // the maximum does not depend on the order of its partials, so it spreads over up to
+    // RED_PARTS workgroups (the whole chip); the sum keeps 32 partials in a fixed order
+    const size_t want = (n + EW_THREADS - 1) / EW_THREADS;
+    const int nb = static_cast<int>(std::min<size_t>(is_max ? xh::RED_PARTS : 32, want));
+    double* res = ctx->dev_scalar + xh::RED_PARTS;
     if(is_max)
         hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, n, ctx->dev_scalar);
     else
         hipLaunchKernelGGL(k_dot, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, y, n, ctx->dev_scalar);
     xh::launch_check("reduce");
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(EW_THREADS), 0, ctx->stream, ctx->dev_scalar, nb, ctx->dev_scalar + 32, is_max ? 1 : 0);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(EW_THREADS), 0, ctx->stream, ctx->dev_scalar, nb, res, is_max ? 1 : 0);
     xh::launch_check("fold");
-    check(hipMemcpyAsync(ctx->host_scalar, ctx->dev_scalar + 32, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+    check(hipMemcpyAsync(ctx->host_scalar, res, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
     check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     *out = ctx->host_scalar[0];
 }

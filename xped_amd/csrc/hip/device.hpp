@@ -18,7 +18,7 @@ struct xh_ctx {
     hipEvent_t events[16] = {};
     // small pinned host staging buffer for scalar reductions
     double* host_scalar = nullptr;
-    double* dev_scalar = nullptr;  // 64 doubles of device scratch for reductions
+    double* dev_scalar = nullptr;  // RED_PARTS + 64 doubles of device scratch for reductions
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
     int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 tiles for large sectors, 1 64x64 tiles only
@@ -48,6 +48,9 @@ struct xh_ctx {
 void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, const double* A, double* U, double* S, double* Vt);
 
 namespace xh {
+
+// partial results of one reduction launch (k_maxabs / k_dot) in the context scratch
+constexpr int RED_PARTS = 1024;
 
 struct Error : std::runtime_error {
     int code;
