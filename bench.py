@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1, help="untimed fast-mode steps after the common warm-up (c4)")
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps of the plain FP64 mode (c4)")
     ap.add_argument("--chi", type=int, default=256)
+    ap.add_argument("--graph", action="store_true", help="c3: also time the product replayed from a captured HIP graph")
     ap.add_argument("--power-iters", type=int, default=6)
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--rank-factor", type=float, default=2.0, help="sketch width l = ceil(rank_factor * k) + oversample")
@@ -355,6 +356,26 @@ def run_c3(args):
     flops_step = (c1["gemm_flops"] - c0["gemm_flops"]) / steps
     step_ms = ms.value / steps
     value = flops_step / (step_ms * 1e-3) / 1e12
+    graph = None
+    if args.graph:
+        # the same product replayed from a HIP graph captured once (mul_into on the output of the
+        # last eager step: no allocation under capture); timed like the eager steps
+        out = Cm
+        rt.graph_begin()
+        A.mul_into(B, out, precision=prec)
+        g = rt.graph_end()
+        for _ in range(2):
+            g.launch()
+        rt.synchronize()
+        L.xh_event_record(ctx, 0)
+        for _ in range(steps):
+            g.launch()
+        L.xh_event_record(ctx, 1)
+        rt.synchronize()
+        gms = C.c_float()
+        L.xh_event_elapsed_ms(ctx, 0, 1, C.byref(gms))
+        graph = {"ms_per_step": round(gms.value / steps, 5), "tflops": round(flops_step / (gms.value / steps * 1e-3) / 1e12, 4)}
+        g.close()
     # dominant kernel: the grouped GEMM launch alone, timed with HIP events on the context stream
     rt.gemm_timing(True, reset=True)
     for _ in range(20):
@@ -372,6 +393,8 @@ def run_c3(args):
                      "kernel": f"k_gemm_{'f64' if prec == 'fp64' else prec}", "kernel_ms": round(gemm_ms / 20, 5)},
         "wall_s_timed": round(wall, 4),
     }
+    if graph is not None:
+        res["graph_replay"] = graph
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_c3(args.leg_dim, args.cpu_seconds, configs)
     return res

@@ -213,6 +213,22 @@ int xh_mlp_adam_step(xh_ctx* ctx, xh_mlp* net, const float* grad, float lr, int6
 int xh_event_record(xh_ctx* ctx, int slot);
 int xh_event_elapsed_ms(xh_ctx* ctx, int slot0, int slot1, float* ms);
 
+/* ---- HIP graphs of grouped launches -----------------------------------------------------
+ * Stream capture of the context stream: the grouped launches issued between begin and end
+ * (permutes, GEMMs, fills, elementwise kernels on existing buffers) become one graph that
+ * xh_graph_launch replays on the stream.  Work that allocates or synchronises cannot be
+ * captured: a split-K GEMM plan (it allocates its partials) fails with XH_EINVAL while capturing,
+ * as do plan creation and host copies; the capture is then abandoned by xh_graph_end.  Replays
+ * read and write the buffers named at capture time.  (No counterpart in the reference, whose
+ * CPU path has no launches to batch: the hipGraph-captured steps of a GPU port.) */
+typedef struct xh_graph xh_graph;
+int xh_graph_begin(xh_ctx* ctx);
+int xh_graph_end(xh_ctx* ctx, xh_graph** out);
+int xh_graph_launch(xh_ctx* ctx, const xh_graph* g);
+int xh_graph_destroy(xh_ctx* ctx, xh_graph* g);
+/* nonzero while the context stream is being captured */
+int xh_graph_capturing(xh_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

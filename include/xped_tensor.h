@@ -111,6 +111,9 @@ int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, i
 int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, int64_t* task_dst, int64_t* task_rows, int64_t* task_cols, int32_t* task_nsrc,
                          int64_t* src_off, double* src_coeff);
 int xt_mul(const xt_tensor* l, const xt_tensor* r, int precision, xt_tensor** out);   /* TensorBase.cpp:237-288 */
+/* operator* into an existing tensor that has the product's structure (no allocation, so it can be
+ * captured with xh_graph_begin/xh_graph_end once the product's plan exists); XH_EINVAL otherwise */
+int xt_mul_into(const xt_tensor* l, const xt_tensor* r, xt_tensor* out, int precision);
 int xt_contract(const xt_tensor* a, const int* a1, int n1, const xt_tensor* b, const int* a2, int n2, int rankres, int precision,
                 xt_tensor** out); /* Tensor.hpp:328-351 */
 int xt_get_permutations(const int* a1, int n1, int rank1, const int* a2, int n2, int rank2, int rankres, int* p1, int* shift1, int* p2, int* shift2,

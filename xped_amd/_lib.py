@@ -107,6 +107,11 @@ _SIGS = [
     ("xh_set_gemm_tiles", C.c_int, [_P, C.c_int]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),
     ("xh_event_elapsed_ms", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    ("xh_graph_begin", C.c_int, [_P]),
+    ("xh_graph_end", C.c_int, [_P, _PP]),
+    ("xh_graph_launch", C.c_int, [_P, _P]),
+    ("xh_graph_destroy", C.c_int, [_P, _P]),
+    ("xh_graph_capturing", C.c_int, [_P]),
     # tensor layer
     ("xt_rt_create", C.c_int, [C.c_int, _PP]),
     ("xt_rt_destroy", C.c_int, [_P]),
@@ -152,6 +157,7 @@ _SIGS = [
     ("xt_permute_plan_stats", C.c_int, [_P, C.c_int, _IP, C.c_int, _I64P, _I64P, _I64P]),
     ("xt_permute_plan_dump", C.c_int, [_P, C.c_int, _IP, C.c_int, _I64P, _I64P, _I64P, C.POINTER(C.c_int32), _I64P, _DP]),
     ("xt_mul", C.c_int, [_P, _P, C.c_int, _PP]),
+    ("xt_mul_into", C.c_int, [_P, _P, _P, C.c_int]),
     ("xt_contract", C.c_int, [_P, _IP, C.c_int, _P, _IP, C.c_int, C.c_int, C.c_int, _PP]),
     ("xt_get_permutations", C.c_int, [_IP, C.c_int, C.c_int, _IP, C.c_int, C.c_int, C.c_int, _IP, _IP, _IP, _IP, _IP, _IP]),
     ("xt_tsvd", C.c_int, [_P, _SZ, C.c_double, C.c_int, _PP, _PP, _PP, _DP, _DP]),

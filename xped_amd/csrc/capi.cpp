@@ -486,6 +486,14 @@ int xt_mul(const xt_tensor* l, const xt_tensor* r, int precision, xt_tensor** ou
     return run(l ? l->owner : nullptr, [&] { *out = wrap(l->owner, xa::mul(l->t, r->t, precision)); });
 }
 
+int xt_mul_into(const xt_tensor* l, const xt_tensor* r, xt_tensor* out, int precision)
+{
+    return run(l ? l->owner : nullptr, [&] {
+        if(!r || !out) throw xa::Error(XH_EINVAL, "xt_mul_into: null tensor");
+        xa::mul_into(l->t, r->t, out->t, precision);
+    });
+}
+
 int xt_contract(const xt_tensor* a, const int* a1, int n1, const xt_tensor* b, const int* a2, int n2, int rankres, int precision, xt_tensor** out)
 {
     return run(a ? a->owner : nullptr, [&] {

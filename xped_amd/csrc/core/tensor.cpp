@@ -376,10 +376,26 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
     return cg;
 }
 
+static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r, DTensor& out, int precision);
+
 DTensor mul(const DTensor& l, const DTensor& r, int precision)
 {
     auto cg = get_gemm_plan(l.rt, *l.st, *r.st);
     DTensor out = DTensor::empty(l.rt, cg->plan.out);
+    launch_mul(cg.get(), l, r, out, precision);
+    return out;
+}
+
+void mul_into(const DTensor& l, const DTensor& r, DTensor& out, int precision)
+{
+    auto cg = get_gemm_plan(l.rt, *l.st, *r.st);
+    if(out.rt != l.rt || !out.st || (out.st != cg->plan.out && structure_key(*out.st) != structure_key(*cg->plan.out)))
+        throw Error(XH_EINVAL, "mul_into: the output tensor does not have the structure of the product");
+    launch_mul(cg.get(), l, r, out, precision);
+}
+
+static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r, DTensor& out, int precision)
+{
     if(l.rt->ctx && out.size() > 0) {
         if(!cg->zero_rects.empty()) {
             // sectors without a matching pair stay zero (Tout.setZero(), TensorBase.cpp:256-258)
@@ -392,7 +408,6 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision)
         rt->gemm_flops += cg->plan.flops;
         rt->n_gemm++;
     }
-    return out;
 }
 
 DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision)

@@ -108,6 +108,9 @@ DTensor permute(const DTensor& t, int shift, const std::vector<int>& p);
 DTensor permute_adj(const DTensor& y, int shift, const std::vector<int>& p);
 std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p);
 DTensor mul(const DTensor& l, const DTensor& r, int precision);
+// operator* into an existing tensor with the product's structure: no allocation, so the launches
+// can be captured in a graph once the plan exists
+void mul_into(const DTensor& l, const DTensor& r, DTensor& out, int precision);
 DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision);
 DTensor adjoint(const DTensor& t);
 DTensor twist(const DTensor& t, int leg);

@@ -121,6 +121,86 @@ extern "C" int xh_synchronize(xh_ctx* ctx)
 
 extern "C" void* xh_stream(xh_ctx* ctx) { return ctx ? reinterpret_cast<void*>(ctx->stream) : nullptr; }
 
+struct xh_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int xh_graph_capturing(xh_ctx* ctx)
+{
+    if(!ctx) return 0;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if(hipStreamIsCapturing(ctx->stream, &st) != hipSuccess) return 0;
+    return st == hipStreamCaptureStatusActive ? 1 : 0;
+}
+
+void xh::refuse_under_capture(xh_ctx* ctx, const char* what)
+{
+    if(xh_graph_capturing(ctx)) {
+        ctx->capture_failed = true;
+        throw xh::Error(XH_EINVAL, std::string(what) + " cannot run under graph capture (build plans before capturing)");
+    }
+}
+
+extern "C" int xh_graph_begin(xh_ctx* ctx)
+{
+    return guard(ctx, [&] {
+        if(xh_graph_capturing(ctx)) throw xh::Error(XH_EINVAL, "graph capture already active on this context");
+        ctx->capture_failed = false;
+        check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    });
+}
+
+extern "C" int xh_graph_end(xh_ctx* ctx, xh_graph** out)
+{
+    if(out) *out = nullptr;
+    return guard(ctx, [&] {
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        const bool refused = ctx->capture_failed;
+        ctx->capture_failed = false;
+        if(e != hipSuccess || !g || refused) {
+            if(g) hipGraphDestroy(g);
+            (void)hipGetLastError();
+            if(refused) throw xh::Error(XH_EINVAL, "graph capture discarded: a captured call could not run under capture");
+            throw xh::Error(XH_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        }
+        auto* gr = new xh_graph();
+        gr->graph = g;
+        const hipError_t ei = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+        if(ei != hipSuccess) {
+            hipGraphDestroy(g);
+            delete gr;
+            throw xh::Error(XH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        if(!out) {
+            hipGraphExecDestroy(gr->exec);
+            hipGraphDestroy(gr->graph);
+            delete gr;
+            throw xh::Error(XH_EINVAL, "xh_graph_end: null output");
+        }
+        *out = gr;
+    });
+}
+
+extern "C" int xh_graph_launch(xh_ctx* ctx, const xh_graph* g)
+{
+    return guard(ctx, [&] {
+        if(!g || !g->exec) throw xh::Error(XH_EINVAL, "null graph");
+        check(hipGraphLaunch(g->exec, ctx->stream), "hipGraphLaunch");
+    });
+}
+
+extern "C" int xh_graph_destroy(xh_ctx* ctx, xh_graph* g)
+{
+    if(!g) return XH_OK;
+    return guard(ctx, [&] {
+        if(g->exec) check(hipGraphExecDestroy(g->exec), "hipGraphExecDestroy");
+        if(g->graph) check(hipGraphDestroy(g->graph), "hipGraphDestroy");
+        delete g;
+    });
+}
+
 extern "C" int xh_event_record(xh_ctx* ctx, int slot)
 {
     return guard(ctx, [&] {

@@ -37,6 +37,9 @@ struct xh_ctx {
     size_t gemm_events_used = 0;
     std::vector<double> gemm_event_flops;
     // the same for permute launches (xh_permute_timing), with the bytes each one moves
+    // set when a call refuses to run under graph capture (it would allocate or synchronise);
+    // xh_graph_end then discards the capture
+    bool capture_failed = false;
     bool perm_timing = false;
     std::vector<hipEvent_t> perm_events;
     size_t perm_events_used = 0;
@@ -51,6 +54,9 @@ namespace xh {
 
 // partial results of one reduction launch (k_maxabs / k_dot) in the context scratch
 constexpr int RED_PARTS = 1024;
+
+// refuse work that cannot be recorded into a graph while the context stream is being captured
+void refuse_under_capture(xh_ctx* ctx, const char* what);
 
 struct Error : std::runtime_error {
     int code;

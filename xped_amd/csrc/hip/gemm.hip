@@ -400,6 +400,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
     *out = nullptr;
     auto* plan = new xh_gemm_plan();
     int rc = guard(ctx, [&] {
+        xh::refuse_under_capture(ctx, "plan creation");
         std::vector<xh_gemm_item> v(items, items + nitems);
         // largest K*tiles first so each part's loop ends with small work
         std::vector<int> order(v.size());
@@ -552,6 +553,7 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, 
         if(need_all && !plan->all64_built) build_all64(ctx, plan);
         if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
         double* ws = nullptr;
+        if(plan->ws_doubles > 0) xh::refuse_under_capture(ctx, "a split-K GEMM plan (it allocates its partials)");
         if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
         GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
         // a multiple of NPART workgroups, at most per_cu per CU

@@ -301,6 +301,7 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
     *out = nullptr;
     auto* plan = new xh_permute_plan();
     int rc = guard(ctx, [&] {
+        xh::refuse_under_capture(ctx, "plan creation");
         const int NL = desc->nlegs;
         if(NL < 0 || NL > XH_MAXLEGS) throw xh::Error(XH_EINVAL, "permute plan: too many legs");
         if(desc->new_rank < 0 || desc->new_rank > NL) throw xh::Error(XH_EINVAL, "permute plan: invalid new_rank");

@@ -37,6 +37,34 @@ def _ints(v) -> C.Array:
     return (C.c_int * max(1, len(v)))(*v)
 
 
+def _check_xh(rc, ctx, what):
+    if rc != _lib.XH_OK:
+        msg = lib().xh_last_error(ctx)
+        raise _lib.XpedError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+class Graph:
+    """A captured sequence of grouped launches on a runtime's stream (xh_graph_*): launch()
+    replays it on the buffers named at capture time."""
+
+    def __init__(self, rt: "Runtime", h):
+        self.rt, self.h = rt, h
+
+    def launch(self):
+        _check_xh(lib().xh_graph_launch(self.rt.ctx(), self.h), self.rt.ctx(), "graph launch")
+
+    def close(self):
+        if self.h:
+            lib().xh_graph_destroy(self.rt.ctx(), self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Runtime:
     """Device context plus plan caches (one per host thread)."""
 
@@ -68,6 +96,17 @@ class Runtime:
         lib().xt_rt_host_timers(self.h, t, 8)
         return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value,
                 "breakdown_s": [round(x, 4) for x in list(t[:5]) + list(t[6:8])], "symbolic_hits": int(t[5])}
+
+    def graph_begin(self):
+        """Start capturing the runtime's stream into a HIP graph (see Graph)."""
+        _check_xh(lib().xh_graph_begin(self.ctx()), self.ctx(), "graph capture")
+
+    def graph_end(self) -> "Graph":
+        """End the capture and instantiate it; a capture that recorded work which allocates or
+        synchronises (a new plan, a split-K GEMM) raises instead."""
+        g = C.c_void_p()
+        _check_xh(lib().xh_graph_end(self.ctx(), C.byref(g)), self.ctx(), "graph capture")
+        return Graph(self, g)
 
     def marker(self, id: int):
         """An empty kernel of `id` workgroups on the runtime's stream: a mark in a kernel trace."""
@@ -336,6 +375,11 @@ class Tensor:
         h = C.c_void_p()
         check(lib().xt_mul(self.h, other.h, PRECISION[precision], C.byref(h)), self.rt.h, "operator*")
         return self._wrap(h)
+
+    def mul_into(self, other: "Tensor", out: "Tensor", precision: str = "fp64") -> "Tensor":
+        """operator* written into `out` (which must have the product's structure); no allocation."""
+        check(lib().xt_mul_into(self.h, other.h, out.h, PRECISION[precision]), self.rt.h, "operator* into")
+        return out
 
     def contract(self, other: "Tensor", a1: Sequence[int], a2: Sequence[int], rankres: int, precision: str = "fp64") -> "Tensor":
         h = C.c_void_p()
