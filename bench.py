@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps of the plain FP64 mode (c4)")
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--graph", action="store_true", help="c3: also time the product replayed from a captured HIP graph")
+    ap.add_argument("--autotune", action="store_true", help="pick each FP64 GEMM plan's tiling by timing both on its first call")
     ap.add_argument("--power-iters", type=int, default=6)
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--rank-factor", type=float, default=2.0, help="sketch width l = ceil(rank_factor * k) + oversample")
@@ -80,6 +81,8 @@ def log(msg):
 def run_mode(mode, args, xt, configs, random_ipeps):
     """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
     rt = xt.Runtime(0)
+    if args.autotune and mode == "fast":
+        xt.lib().xh_set_gemm_autotune(rt.ctx(), 1)
     A = [a.scale(SCALE) for a in random_ipeps(rt, configs.c4_virtual(), configs.c4_physical(), args.seed)]
     c = xt.CTM(rt, A, args.chi).init()
     t0 = time.perf_counter()
@@ -147,6 +150,8 @@ def run_mode(mode, args, xt, configs, random_ipeps):
         out["gemm_timed_flops"] = fl
         out["gemm_launches_blocks"] = nlog
         out["gemm_shapes"] = rt.gemm_log()
+        if args.autotune:
+            out["autotune"] = autotune_stats(rt)
     return out
 
 
@@ -238,6 +243,7 @@ def run_c4(args):
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
+                 **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
                  "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
         "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5),
@@ -334,6 +340,8 @@ def run_c3(args):
 
     L = _lib.lib()
     rt = xt.Runtime(0)
+    if args.autotune:
+        L.xh_set_gemm_autotune(rt.ctx(), 1)
     A, B = build_c3(rt, args.leg_dim, xt, configs)
     prec = args.precision
     a1, a2 = [-1, -2, 1, 2], [1, 2, -3, -4]
@@ -395,6 +403,8 @@ def run_c3(args):
     }
     if graph is not None:
         res["graph_replay"] = graph
+    if args.autotune:
+        res["autotune_entries_runs_hits_chose64"] = autotune_stats(rt)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_c3(args.leg_dim, args.cpu_seconds, configs)
     return res
@@ -430,6 +440,15 @@ def run_copy(args):
             "ms_per_step": round(1e3 * el / steps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (constant fill)", "config": {"workload": "copy of 2^27 doubles (1 GiB read + 1 GiB written per launch)"},
             "bytes_per_launch": 2 * 8 * n}
+
+
+def autotune_stats(rt):
+    """(cached signatures, tuning runs, cache hits, signatures that chose 64 x 64 tiles only)."""
+    import xped_amd.tensor as xt
+
+    v = [C.c_int64() for _ in range(4)]
+    xt.lib().xh_gemm_autotune_stats(rt.ctx(), *[C.byref(x) for x in v])
+    return [x.value for x in v]
 
 
 def measured_traffic(name):

@@ -97,6 +97,15 @@ int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, 
 int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan);
 int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
 int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops);
+/* GEMM tile autotuner (no counterpart in the reference, whose per-block BLAS calls have no tiles
+ * to choose): with enable != 0, the first FP64 call of a plan that has 128 x 128 tiles, with
+ * beta = 0, runs both tilings (128 x 128 + 64 x 64, and 64 x 64 only), timed with HIP events
+ * (one host synchronisation), and runs the faster last so that C holds exactly its result; the
+ * choice is cached per plan signature (transposes, split-K use, log2 buckets of item count, largest
+ * m, n, k and total flops) and reused by later plans of the same signature.  Off by default. */
+int xh_set_gemm_autotune(xh_ctx* ctx, int enable);
+/* autotuner state: cached signatures, tuning runs, cache hits, signatures that chose 64 x 64 only */
+int xh_gemm_autotune_stats(xh_ctx* ctx, int64_t* entries, int64_t* tuned, int64_t* hits, int64_t* chose_64);
 /* FP64 tiling of sectors with both output sides >= 192: 0 = 128 x 128 tiles (default), 1 = 64 x 64
  * tiles like the other sectors.  A tuning switch; no counterpart in the reference. */
 int xh_set_gemm_tiles(xh_ctx* ctx, int mode);

@@ -185,6 +185,62 @@ def test_gemm_timing_and_marker(rt):
         L.xh_free(ctx, d)
 
 
+def test_gemm_autotuner_picks_a_tiling_and_reuses_it(rt):
+    # items with both output sides >= 192 have 128 x 128 tiles, so both tilings are candidates
+    L = lib()
+    ctx = rt.ctx()
+    r = np.random.default_rng(9)
+    items_mnk = [(320, 256, 200), (200, 192, 64), (96, 80, 50)]
+    offs, a_tot, b_tot, c_tot = [], 0, 0, 0
+    for m, n, k in items_mnk:
+        offs.append((a_tot, b_tot, c_tot))
+        a_tot += m * k
+        b_tot += k * n
+        c_tot += m * n
+    A, B = r.uniform(-1, 1, a_tot), r.uniform(-1, 1, b_tot)
+    dA, dB, dC = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, sz in ((dA, a_tot), (dB, b_tot), (dC, c_tot)):
+        assert L.xh_alloc(ctx, 8 * sz, C.byref(d)) == 0
+    L.xh_upload(ctx, dA, A.ctypes.data, 8 * a_tot)
+    L.xh_upload(ctx, dB, B.ctypes.data, 8 * b_tot)
+    arr = (_lib.xh_gemm_item * 3)(*[_lib.xh_gemm_item(ao, bo, co, m, n, k) for (ao, bo, co), (m, n, k) in zip(offs, items_mnk)])
+    plan = C.c_void_p()
+    assert L.xh_gemm_plan_create(ctx, arr, 3, 0, 0, C.byref(plan)) == 0
+    stats = [C.c_int64() for _ in range(4)]
+
+    def run():
+        assert L.xh_gemm(ctx, plan, dA, dB, dC, 1.0, 0.0, _lib.XH_FP64) == 0
+        out = np.empty(c_tot)
+        L.xh_download(ctx, out.ctypes.data, dC, 8 * c_tot)
+        return out
+
+    assert L.xh_set_gemm_autotune(ctx, 1) == 0
+    try:
+        tuned = run()
+        assert L.xh_gemm_autotune_stats(ctx, *[C.byref(x) for x in stats]) == 0
+        entries, runs, hits, chose64 = (x.value for x in stats)
+        assert runs >= 1 and entries >= 1
+        again = run()
+        assert L.xh_gemm_autotune_stats(ctx, *[C.byref(x) for x in stats]) == 0
+        assert stats[2].value == hits + 1  # the second call reuses the cached choice
+        assert np.array_equal(again, tuned)
+    finally:
+        assert L.xh_set_gemm_autotune(ctx, 0) == 0
+    # the tuned result is exactly the chosen tiling's result, and a correct product
+    assert L.xh_set_gemm_tiles(ctx, 1 if chose64 else 0) == 0
+    try:
+        assert np.array_equal(run(), tuned)
+    finally:
+        assert L.xh_set_gemm_tiles(ctx, 0) == 0
+    for (ao, bo, co), (m, n, k) in zip(offs, items_mnk):
+        ref = A[ao:ao + m * k].reshape(k, m).T @ B[bo:bo + k * n].reshape(n, k).T
+        got = tuned[co:co + m * n].reshape(n, m).T
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max() * math.sqrt(k))
+    L.xh_gemm_plan_destroy(ctx, plan)
+    for d in (dA, dB, dC):
+        L.xh_free(ctx, d)
+
+
 @pytest.mark.parametrize("pcode", PCODES, ids=IDS)
 def test_mul_and_contract(rt, pcode):
     r = random.Random(6)

@@ -107,6 +107,8 @@ _SIGS = [
     ("xh_set_gemm_tiles", C.c_int, [_P, C.c_int]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),
     ("xh_event_elapsed_ms", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    ("xh_set_gemm_autotune", C.c_int, [_P, C.c_int]),
+    ("xh_gemm_autotune_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("xh_graph_begin", C.c_int, [_P]),
     ("xh_graph_end", C.c_int, [_P, _PP]),
     ("xh_graph_launch", C.c_int, [_P, _P]),

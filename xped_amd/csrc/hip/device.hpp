@@ -6,6 +6,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../../include/xped_hip.h"
@@ -37,6 +38,10 @@ struct xh_ctx {
     size_t gemm_events_used = 0;
     std::vector<double> gemm_event_flops;
     // the same for permute launches (xh_permute_timing), with the bytes each one moves
+    // GEMM tile autotuner: plan signature -> tiling (0 mixed, 1 64x64 only), and counters
+    bool gemm_autotune = false;
+    std::unordered_map<uint64_t, int> gemm_tune;
+    int64_t gemm_tune_runs = 0, gemm_tune_hits = 0;
     // set when a call refuses to run under graph capture (it would allocate or synchronise);
     // xh_graph_end then discards the capture
     bool capture_failed = false;

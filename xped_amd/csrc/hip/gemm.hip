@@ -373,6 +373,7 @@ struct xh_gemm_plan {
     int32_t nred = 0;
     int64_t ws_doubles = 0, max_red_mn = 0;
     int ta = 0, tb = 0;
+    uint64_t sig = 0;  // autotuner key: transposes, split-K use and log2 buckets of the plan's sizes
     double flops = 0.0;
 };
 
@@ -468,6 +469,24 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         plan->nred = static_cast<int32_t>(red.size());
         plan->ta = trans_a ? 1 : 0;
         plan->tb = trans_b ? 1 : 0;
+        {
+            auto lg = [](double x) {
+                uint64_t l = 0;
+                while(x > 1.0 && l < 63) {
+                    x *= 0.5;
+                    ++l;
+                }
+                return l;
+            };
+            double mm = 0, mn = 0, mk = 0;
+            for(const auto& it : v) {
+                mm = std::max(mm, double(it.m));
+                mn = std::max(mn, double(it.n));
+                mk = std::max(mk, double(it.k));
+            }
+            plan->sig = (uint64_t(plan->ta) << 62) | (uint64_t(plan->tb) << 61) | (lg(double(v.size())) << 48) | (lg(mm) << 40) | (lg(mn) << 32) |
+                        (lg(mk) << 24) | (lg(plan->flops) << 16) | (plan->ws_doubles > 0 ? 1u : 0u);
+        }
         xh::PackedUpload& up = plan->up;
         const size_t bytes = xh::PackedUpload::packed_bytes({v.size() * sizeof(xh_gemm_item), ext.size() * sizeof(ItemExt), t_big.size() * sizeof(Tile),
                                                              t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem)});
@@ -566,24 +585,78 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, 
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, ctx->stream, g);
             xh::launch_check(name);
         };
+        // tiles: 0 = FP64 128x128 tiles for the large items and 64x64 for the rest, 1 = 64x64 only
+        auto run = [&](int tiles) {
+            if(precision == XH_FP64 && tiles == 0) {
+                launch(plan->big128, k_gemm_f64_128, 2, G8_THREADS, "k_gemm_f64_128");
+                launch(plan->rest64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+            } else if(precision == XH_FP64) {
+                launch(plan->all64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+            } else {
+                launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");
+            }
+            if(plan->nred > 0) {
+                const unsigned gx = static_cast<unsigned>(std::min<int64_t>((plan->max_red_mn + 1023) / 1024, 256));
+                hipLaunchKernelGGL(k_splitk_reduce, dim3(std::max(gx, 1u), static_cast<unsigned>(plan->nred)), dim3(256), 0, ctx->stream, plan->d_red, ws, C,
+                                   alpha, beta);
+                xh::launch_check("k_splitk_reduce");
+            }
+        };
+        int tiles = ctx->gemm_tiles;
+        const bool tunable = ctx->gemm_autotune && precision == XH_FP64 && plan->big128.ntiles > 0 && !xh_graph_capturing(ctx);
         const bool timed = ctx->gemm_timing;
         if(timed) xh::gemm_timer_begin(ctx);
-        if(precision == XH_FP64 && ctx->gemm_tiles == 0) {
-            launch(plan->big128, k_gemm_f64_128, 2, G8_THREADS, "k_gemm_f64_128");
-            launch(plan->rest64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
-        } else if(precision == XH_FP64) {
-            launch(plan->all64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
-        } else {
-            launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");
+        if(tunable) {
+            auto hit = ctx->gemm_tune.find(plan->sig);
+            if(hit != ctx->gemm_tune.end()) {
+                tiles = hit->second;
+                ++ctx->gemm_tune_hits;
+            } else if(beta == 0.0 && C != A && C != B) {
+                // time both tilings on this call's operands (beta = 0: every run rewrites C) and
+                // run the faster one last, so that C holds exactly its result
+                if(!plan->all64_built) build_all64(ctx, plan);
+                hipEvent_t ev[3];
+                for(auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+                check(hipEventRecord(ev[0], ctx->stream), "hipEventRecord");
+                run(0);
+                check(hipEventRecord(ev[1], ctx->stream), "hipEventRecord");
+                run(1);
+                check(hipEventRecord(ev[2], ctx->stream), "hipEventRecord");
+                check(hipEventSynchronize(ev[2]), "hipEventSynchronize");
+                float t0 = 0.f, t1 = 0.f;
+                check(hipEventElapsedTime(&t0, ev[0], ev[1]), "hipEventElapsedTime");
+                check(hipEventElapsedTime(&t1, ev[1], ev[2]), "hipEventElapsedTime");
+                for(auto& e : ev) hipEventDestroy(e);
+                tiles = t0 < t1 ? 0 : 1;
+                ctx->gemm_tune[plan->sig] = tiles;
+                ++ctx->gemm_tune_runs;
+                if(tiles == 0) run(0);
+                if(timed) xh::gemm_timer_end(ctx, plan->flops);
+                if(ws) xh::dev_free(ctx, ws);
+                return;
+            }
         }
-        if(plan->nred > 0) {
-            const unsigned gx = static_cast<unsigned>(std::min<int64_t>((plan->max_red_mn + 1023) / 1024, 256));
-            hipLaunchKernelGGL(k_splitk_reduce, dim3(std::max(gx, 1u), static_cast<unsigned>(plan->nred)), dim3(256), 0, ctx->stream, plan->d_red, ws, C, alpha,
-                               beta);
-            xh::launch_check("k_splitk_reduce");
-        }
+        if(tiles != 0 && precision == XH_FP64 && !plan->all64_built) build_all64(ctx, plan);
+        run(tiles);
         if(timed) xh::gemm_timer_end(ctx, plan->flops);
         if(ws) xh::dev_free(ctx, ws);
+    });
+}
+
+extern "C" int xh_set_gemm_autotune(xh_ctx* ctx, int enable)
+{
+    return guard(ctx, [&] { ctx->gemm_autotune = enable != 0; });
+}
+
+extern "C" int xh_gemm_autotune_stats(xh_ctx* ctx, int64_t* entries, int64_t* tuned, int64_t* hits, int64_t* chose_64)
+{
+    return guard(ctx, [&] {
+        int64_t c64 = 0;
+        for(const auto& kv : ctx->gemm_tune) c64 += kv.second == 1;
+        if(entries) *entries = static_cast<int64_t>(ctx->gemm_tune.size());
+        if(tuned) *tuned = ctx->gemm_tune_runs;
+        if(hits) *hits = ctx->gemm_tune_hits;
+        if(chose_64) *chose_64 = c64;
     });
 }
 
