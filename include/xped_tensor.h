@@ -39,6 +39,12 @@ xh_ctx* xt_rt_ctx(xt_rt* rt);
 /* counters: total GEMM flops and permute bytes issued, number of launches */
 int xt_rt_counters(const xt_rt* rt, double* gemm_flops, double* permute_bytes, long* n_gemm, long* n_permute);
 int xt_rt_clear_caches(xt_rt* rt);
+/* bound of the plan caches (permute and GEMM plans, each): past it the least recently used half is
+ * dropped, device tables freed in stream order (no counterpart in the reference, which keeps no
+ * device plans); default 2048, at least 8 */
+int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans);
+/* current cached permute plans, GEMM plans, and plans dropped so far */
+int xt_rt_cache_sizes(const xt_rt* rt, int64_t* permute_plans, int64_t* gemm_plans, int64_t* evicted);
 /* host time (seconds) spent building permute and GEMM plans, and how many were built */
 int xt_rt_host_stats(const xt_rt* rt, double* permute_plan_s, double* gemm_plan_s, long* n_permute_plans, long* n_gemm_plans);
 /* breakdown of that host time (seconds): [0] numeric part of the permute plans on the host, [1]

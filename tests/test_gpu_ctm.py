@@ -85,3 +85,22 @@ def test_randomized_svd_step_matches_full_step(rt):
     n, nfb, ratio = rand.rsvd_stats()
     assert n == 16
     assert abs(Er - Ef) <= 1e-10 * abs(Ef), (Er, Ef, nfb, ratio)
+
+
+def test_bounded_plan_cache_gives_the_same_steps():
+    # a runtime whose plan caches hold at most 8 plans drops and rebuilds plans all through the
+    # steps (device tables freed in stream order while launches may still be queued); energies must
+    # equal those of a runtime with the default bound bit for bit
+    energies = []
+    evicted = []
+    for bound in (None, 8):
+        rt = xt.Runtime(0)
+        if bound:
+            rt.set_max_plans(bound)
+        oA, pA = make_ipeps(rt, [((0,), 1), ((1,), 1)], configs.c4_physical(), 7)
+        c = xt.CTM(rt, pA, 16).init()
+        energies.append([c.step() for _ in range(3)])
+        evicted.append(rt.cache_sizes()[2])
+        assert rt.cache_sizes()[0] <= (bound or 2048)
+    assert energies[0] == energies[1]
+    assert evicted[0] == 0 and evicted[1] > 0

@@ -288,3 +288,30 @@ def test_permute_plans_reused_across_dimension_changes(xlib, kinds, pcode):
                 np.testing.assert_allclose(plan["src_coeff"], [s[1] for s in srcs], rtol=0, atol=1e-13)
     st = rt.host_stats()
     assert st["n_permute_plans"] == 12 and st["symbolic_hits"] == 8
+
+
+def test_plan_cache_is_bounded_and_rebuilds_evicted_plans(xlib):
+    """The runtime drops the least recently used half of its permute plans past the bound; a
+    dropped plan is rebuilt on its next use and equals the oracle's plan."""
+    rt = xt.Runtime(-1)
+    rt.set_max_plans(8)
+    kinds, pcode = (U1,), xt.U1
+    rank, corank = 2, 2
+    legs = rand_legs(kinds, 9, rank + corank, 3, 3)
+    ob, pb = zip(*[both_bases(kinds, pcode, e) for e in legs])
+    ot = oten.Tensor(Sym(*kinds), list(ob[:rank]), list(ob[rank:]))
+    pt = xt.Tensor(rt, list(pb[:rank]), list(pb[rank:]))
+    cases = [(p, shift) for p in itertools.permutations(range(4)) for shift in (-1, 0, 1)][:40]
+    for p, shift in cases:
+        pt.permute_plan(shift, p)
+        assert rt.cache_sizes()[0] <= 8
+    nperm, _, evicted = rt.cache_sizes()
+    assert evicted >= len(cases) - 8
+    # the first plans were dropped; rebuilding them gives the oracle's plans
+    for p, shift in cases[:3]:
+        out, tasks = oracle_plan(ot, shift, p)
+        plan = pt.permute_plan(shift, p)
+        assert list(plan["task_dst"]) == [t[0] for t in tasks]
+        assert list(plan["src_off"]) == [s[0] for t in tasks for s in t[3]]
+    with pytest.raises(Exception):
+        rt.set_max_plans(2)

@@ -121,6 +121,8 @@ _SIGS = [
     ("xt_rt_ctx", _P, [_P]),
     ("xt_rt_counters", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_clear_caches", C.c_int, [_P]),
+    ("xt_rt_set_max_plans", C.c_int, [_P, C.c_int64]),
+    ("xt_rt_cache_sizes", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("xt_rt_host_stats", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_host_timers", C.c_int, [_P, _DP, C.c_int]),
     ("xt_rt_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_long)]),

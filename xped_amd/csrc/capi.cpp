@@ -180,6 +180,24 @@ int xt_rt_gemm_log(const xt_rt* rt, int32_t* mnk, long cap)
     return XH_OK;
 }
 
+int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans)
+{
+    return run(rt, [&] {
+        if(max_plans < 8) throw xa::Error(XH_EINVAL, "xt_rt_set_max_plans: the bound must be at least 8");
+        rt->rt.max_plans = static_cast<std::size_t>(max_plans);
+        rt->rt.trim_caches();
+    });
+}
+
+int xt_rt_cache_sizes(const xt_rt* rt, int64_t* permute_plans, int64_t* gemm_plans, int64_t* evicted)
+{
+    if(!rt) return XH_EINVAL;
+    if(permute_plans) *permute_plans = static_cast<int64_t>(rt->rt.permute_cache.size());
+    if(gemm_plans) *gemm_plans = static_cast<int64_t>(rt->rt.gemm_cache.size());
+    if(evicted) *evicted = rt->rt.n_evicted;
+    return XH_OK;
+}
+
 int xt_rt_clear_caches(xt_rt* rt)
 {
     return run(rt, [&] {
@@ -451,7 +469,8 @@ int xt_permute_plan_stats(const xt_tensor* t, int shift, const int* p, int np, i
         // the runtime's cached path; on a GPU runtime the cached plan gave its host tables to
         // the device plan, so they are rebuilt from the structure
         const std::vector<int> pv(p, p + np);
-        const xa::PermutePlan& cached = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv)->plan;
+        const auto held = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv);  // kept alive while read
+        const xa::PermutePlan& cached = held->plan;
         const xa::PermutePlan plan = cached.tasks.empty() && cached.moved_elems > 0 ? xa::make_permute_plan(*t->t.st, shift, pv) : cached;
         if(ntasks) *ntasks = static_cast<int64_t>(plan.tasks.size());
         if(nsources) *nsources = static_cast<int64_t>(plan.sources.size());
@@ -466,7 +485,8 @@ int xt_permute_plan_dump(const xt_tensor* t, int shift, const int* p, int np, in
         // the runtime's cached path (symbolic plans reused across dimension changes); on a GPU
         // runtime the host tables were handed to the device plan and are rebuilt here
         const std::vector<int> pv(p, p + np);
-        const xa::PermutePlan& cached = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv)->plan;
+        const auto held = xa::get_permute_plan(t->t.rt, *t->t.st, shift, pv);  // kept alive while read
+        const xa::PermutePlan& cached = held->plan;
         const xa::PermutePlan plan = cached.tasks.empty() && cached.moved_elems > 0 ? xa::make_permute_plan(*t->t.st, shift, pv) : cached;
         for(std::size_t i = 0; i < plan.tasks.size(); ++i) {
             task_dst[i] = plan.tasks[i].dst_off;

@@ -72,6 +72,44 @@ Runtime::~Runtime()
     }
 }
 
+namespace {
+// drop the least recently used half of a plan cache; entries still held elsewhere are kept
+template <class Map, class Destroy>
+long trim_lru(Map& m, std::size_t max_entries, Destroy destroy)
+{
+    if(m.size() <= max_entries) return 0;
+    std::vector<std::pair<std::uint64_t, typename Map::iterator>> byuse;
+    byuse.reserve(m.size());
+    for(auto it = m.begin(); it != m.end(); ++it) byuse.emplace_back(it->second->last_use, it);
+    std::sort(byuse.begin(), byuse.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    long n = 0;
+    const std::size_t drop = m.size() - max_entries / 2;
+    for(std::size_t i = 0; i < drop && i < byuse.size(); ++i) {
+        auto it = byuse[i].second;
+        if(it->second.use_count() > 1) continue;
+        destroy(*it->second);
+        m.erase(it);
+        ++n;
+    }
+    return n;
+}
+}  // namespace
+
+void Runtime::trim_caches()
+{
+    n_evicted += trim_lru(permute_cache, max_plans, [this](CachedPermute& c) {
+        if(ctx && c.dev) xh_permute_plan_destroy(ctx, c.dev);
+        c.dev = nullptr;
+    });
+    n_evicted += trim_lru(gemm_cache, max_plans, [this](CachedGemm& c) {
+        if(ctx && c.dev) xh_gemm_plan_destroy(ctx, c.dev);
+        c.dev = nullptr;
+    });
+    // host-only caches: dimension-free permute parts and shared structures
+    if(sym_permute_cache.size() > 2 * max_plans) sym_permute_cache.clear();
+    if(struct_cache.size() > 4 * max_plans) struct_cache.clear();
+}
+
 void Runtime::require_device() const
 {
     if(!ctx) throw Error(XH_ENODEV, "this runtime has no GPU (structure-only)");
@@ -173,7 +211,10 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     put_i(k, shift);
     for(int x : p) put_i(k, x);
     auto it = rt->permute_cache.find(k);
-    if(it != rt->permute_cache.end()) return it->second;
+    if(it != rt->permute_cache.end()) {
+        it->second->last_use = ++rt->tick;
+        return it->second;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     auto lap = [rt, t = t0](int slot) mutable {
         const auto now = std::chrono::steady_clock::now();
@@ -181,6 +222,7 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
         t = now;
     };
     auto cp = std::make_shared<CachedPermute>();
+    cp->last_use = ++rt->tick;
     // the dimension-free part is shared by all structures with the same charges (tensors whose
     // sector dimensions changed after a truncation); only offsets and extents are recomputed
     StructPtr out = permuted_structure(in, shift, p);
@@ -202,6 +244,7 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     lap(0);
     finish_permute_plan(rt, *cp, in, lap);
     rt->permute_cache.emplace(k, cp);
+    rt->trim_caches();
     rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rt->n_permute_plans++;
     return cp;
@@ -266,7 +309,10 @@ std::shared_ptr<CachedPermute> get_permute_adj_plan(Runtime* rt, const TensorStr
     put_i(k, shift);
     for(int x : p) put_i(k, x);
     auto it = rt->permute_cache.find(k);
-    if(it != rt->permute_cache.end()) return it->second;
+    if(it != rt->permute_cache.end()) {
+        it->second->last_use = ++rt->tick;
+        return it->second;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     auto lap = [rt, t = t0](int slot) mutable {
         const auto now = std::chrono::steady_clock::now();
@@ -274,11 +320,13 @@ std::shared_ptr<CachedPermute> get_permute_adj_plan(Runtime* rt, const TensorStr
         t = now;
     };
     auto cp = std::make_shared<CachedPermute>();
+    cp->last_use = ++rt->tick;
     StructPtr xs = permute_adj_structure(y, shift, p);
     cp->plan = make_permute_adj_plan(xs, y, shift, p);
     lap(0);
     finish_permute_plan(rt, *cp, y, lap);
     rt->permute_cache.emplace(k, cp);
+    rt->trim_caches();
     rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rt->n_permute_plans++;
     return cp;
@@ -328,7 +376,10 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
     k += "|";
     k += structure_key(r);
     auto it = rt->gemm_cache.find(k);
-    if(it != rt->gemm_cache.end()) return it->second;
+    if(it != rt->gemm_cache.end()) {
+        it->second->last_use = ++rt->tick;
+        return it->second;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     auto lap = [rt, t = t0](int slot) mutable {
         const auto now = std::chrono::steady_clock::now();
@@ -336,6 +387,7 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
         t = now;
     };
     auto cg = std::make_shared<CachedGemm>();
+    cg->last_use = ++rt->tick;
     cg->plan = make_gemm_plan(l, r);
     for(const auto& g : cg->plan.items) {
         const std::int64_t mk = static_cast<std::int64_t>(g.m) * g.k, kn = static_cast<std::int64_t>(g.k) * g.n, mn = static_cast<std::int64_t>(g.m) * g.n;
@@ -371,6 +423,7 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
     }
     lap(4);
     rt->gemm_cache.emplace(k, cg);
+    rt->trim_caches();
     rt->gemm_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rt->n_gemm_plans++;
     return cg;

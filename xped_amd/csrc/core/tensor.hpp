@@ -44,11 +44,13 @@ struct DeviceBuffer {
 struct CachedPermute {
     PermutePlan plan;
     xh_permute_plan* dev = nullptr;
+    std::uint64_t last_use = 0;  // runtime tick of the last lookup (cache eviction order)
 };
 struct CachedGemm {
     GemmPlan plan;
     xh_gemm_plan* dev = nullptr;
     std::vector<xh_copy_rect> zero_rects;  // output sectors not written by any GEMM
+    std::uint64_t last_use = 0;
 };
 
 // Device context, plan caches and counters.  ctx == nullptr is a structure-only runtime: every
@@ -65,6 +67,12 @@ struct Runtime {
     // counters for reporting
     double gemm_flops = 0.0, permute_bytes = 0.0;
     long n_gemm = 0, n_permute = 0;
+    // plan caches are bounded: past max_plans entries the least recently used half is dropped
+    // (device tables freed stream-ordered, so launches already queued still read them)
+    std::uint64_t tick = 0;
+    std::size_t max_plans = 2048;
+    long n_evicted = 0;
+    void trim_caches();
     // host time spent building plans (seconds) and the number built
     double permute_plan_s = 0.0, gemm_plan_s = 0.0;
     long n_permute_plans = 0, n_gemm_plans = 0;

@@ -145,6 +145,16 @@ class Runtime:
     def clear_caches(self):
         check(lib().xt_rt_clear_caches(self.h), self.h, "clear_caches")
 
+    def set_max_plans(self, n: int):
+        """Bound of the permute and GEMM plan caches (least recently used half dropped past it)."""
+        check(lib().xt_rt_set_max_plans(self.h, int(n)), self.h, "set_max_plans")
+
+    def cache_sizes(self):
+        """(cached permute plans, cached GEMM plans, plans dropped so far)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().xt_rt_cache_sizes(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h, "cache_sizes")
+        return a.value, b.value, c.value
+
     def __del__(self):
         h = getattr(self, "h", None)
         f = _lib.native("xt_rt_destroy") if (_lib is not None and h) else None
