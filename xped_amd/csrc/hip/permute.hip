@@ -107,11 +107,19 @@ __device__ __forceinline__ double gather_one(const PermDev& p, const TaskMeta& t
 // scaled stores.  0 + c v keeps the sign of zero of the general path's accumulation.  Plans with
 // several sources per task (recoupling sums) accumulate per element in the reference's order.
 // NLT: the plan's leg count when it is 2 .. 8, else 0 (read at run time).
+constexpr int MT_CAP = 64;  // tasks of a multi-task tile whose records are staged in LDS
+
 template <bool SINGLE_SRC, int NLT>
 __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const double* __restrict__ src, double* __restrict__ dst)
 {
     constexpr int PER = PT_TILE / PT_THREADS;
+    constexpr int SL = NLT > 0 ? NLT : 1;
     __shared__ int32_t start[PT_TILE + 1];  // tile-relative first element of tasks t_lo .. t_hi (clamped at 0)
+    // records of the tile's tasks (fixed leg count, at most MT_CAP tasks): read once per tile
+    __shared__ TaskMeta s_meta[MT_CAP];
+    __shared__ int64_t s_pre[MT_CAP];
+    __shared__ LegInfo s_legs[MT_CAP * SL];
+    __shared__ xh_permute_source s_src[SINGLE_SRC ? MT_CAP : 1];
     const int64_t e0 = int64_t(blockIdx.x) * PT_TILE;
     const int n_here = static_cast<int>(min(int64_t(PT_TILE), p.total - e0));
     const int t_lo = p.tile_first[blockIdx.x];
@@ -150,37 +158,78 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const 
         // tasks t_lo .. t_hi intersect the tile; t_hi - t_lo <= PT_TILE since every task holds an element
         const int nt = t_hi - t_lo + 1;
         for(int j = threadIdx.x; j < nt; j += PT_THREADS) start[j] = static_cast<int32_t>(max(int64_t(0), p.prefix[t_lo + j] - e0));
+        const bool staged = NLT > 0 && nt <= MT_CAP;  // uniform over the workgroup
+        if(staged) {
+            for(int j = threadIdx.x; j < nt; j += PT_THREADS) {
+                s_meta[j] = p.meta[t_lo + j];
+                s_pre[j] = p.prefix[t_lo + j];
+                if constexpr(SINGLE_SRC) s_src[j] = p.src[p.meta[t_lo + j].src_begin];
+            }
+            for(int j = threadIdx.x; j < nt * SL; j += PT_THREADS) s_legs[j] = p.legs[int64_t(t_lo) * SL + j];
+        }
         __syncthreads();
+        // the element's task by binary search over the staged offsets; its records come from LDS
+        // (staged) or global memory, in two loops so that each reads one address space
+        auto find_task = [&](int r) {
+            int lo = 0, hi = nt - 1;  // last j with start[j] <= r
+            while(lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if(start[mid] <= r)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            return lo;
+        };
+        if(staged) {
 #pragma unroll
-        for(int i = 0; i < PER; ++i) {
-            const int r = threadIdx.x + i * PT_THREADS;
-            dpos[i] = -1;
-            soff[i] = 0;
-            coef[i] = 0.0;
-            val[i] = 0.0;
-            if(r < n_here) {
-                int lo = 0, hi = nt - 1;  // last j with start[j] <= r
-                while(lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if(start[mid] <= r)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
+            for(int i = 0; i < PER; ++i) {
+                const int r = threadIdx.x + i * PT_THREADS;
+                dpos[i] = -1;
+                soff[i] = 0;
+                coef[i] = 0.0;
+                val[i] = 0.0;
+                if(r < n_here) {
+                    const int lo = find_task(r);
+                    const TaskMeta tm = s_meta[lo];
+                    const uint32_t el = static_cast<uint32_t>(e0 + r - s_pre[lo]);
+                    const uint32_t row = el % tm.rows, col = el / tm.rows;
+                    int64_t srow, scol;
+                    decode_src<NLT>(s_legs + lo * SL, p.nlegs, p.new_rank, row, col, srow, scol);
+                    dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
+                    if constexpr(SINGLE_SRC) {
+                        const xh_permute_source so = s_src[lo];
+                        soff[i] = so.src_off + srow + scol * so.src_ld;
+                        coef[i] = so.coeff;
+                    } else {
+                        val[i] = gather_one(p, tm, src, srow, scol);
+                    }
                 }
-                const int t = t_lo + lo;
-                const TaskMeta tm = p.meta[t];
-                const uint32_t el = static_cast<uint32_t>(e0 + r - p.prefix[t]);
-                const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
-                const uint32_t row = el % tm.rows, col = el / tm.rows;
-                int64_t srow, scol;
-                decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
-                dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
-                if constexpr(SINGLE_SRC) {
-                    const xh_permute_source so = p.src[tm.src_begin];
-                    soff[i] = so.src_off + srow + scol * so.src_ld;
-                    coef[i] = so.coeff;
-                } else {
-                    val[i] = gather_one(p, tm, src, srow, scol);
+            }
+        } else {
+#pragma unroll
+            for(int i = 0; i < PER; ++i) {
+                const int r = threadIdx.x + i * PT_THREADS;
+                dpos[i] = -1;
+                soff[i] = 0;
+                coef[i] = 0.0;
+                val[i] = 0.0;
+                if(r < n_here) {
+                    const int t = t_lo + find_task(r);
+                    const TaskMeta tm = p.meta[t];
+                    const uint32_t el = static_cast<uint32_t>(e0 + r - p.prefix[t]);
+                    const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
+                    const uint32_t row = el % tm.rows, col = el / tm.rows;
+                    int64_t srow, scol;
+                    decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
+                    dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
+                    if constexpr(SINGLE_SRC) {
+                        const xh_permute_source so = p.src[tm.src_begin];
+                        soff[i] = so.src_off + srow + scol * so.src_ld;
+                        coef[i] = so.coeff;
+                    } else {
+                        val[i] = gather_one(p, tm, src, srow, scol);
+                    }
                 }
             }
         }
