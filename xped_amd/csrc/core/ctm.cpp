@@ -159,114 +159,160 @@ DTensor CTM::contract_corner(int x, int y, int corner) const
 
 std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb, int site, int dir)
 {
-    // PEPSContractions.cpp:16-43: prod = T1 * T2, tSVD(chi, 1e-10, ., PRESERVE_MULTIPLETS = false),
-    // P1 = S^-1/2 U^dagger T1, P2 = T2 V S^-1/2
-    DTensor prod = mul(Ta, Tb, opt.precision);
-    {
-        const double m = max_norm(prod);
+    return std::move(decompose_batch({{Ta, Tb}}, {site}, dir)[0]);
+}
+
+std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<std::pair<DTensor, DTensor>>& in, const std::vector<int>& sites,
+                                                              int dir)
+{
+    // PEPSContractions.cpp:16-43, for every pair: prod = T1 * T2, tSVD(chi, 1e-10, .,
+    // PRESERVE_MULTIPLETS = false), P1 = S^-1/2 U^dagger T1, P2 = T2 V S^-1/2.  The pairs of one
+    // move are independent; their randomized SVDs share grouped launches.
+    const std::size_t nb = in.size();
+    if(sites.size() != nb) throw Error(XH_EINVAL, "CTM: one site per projector pair");
+    std::vector<DTensor> prods(nb);
+    for(std::size_t b = 0; b < nb; ++b) {
+        prods[b] = mul(in[b].first, in[b].second, opt.precision);
+        const double m = max_norm(prods[b]);
         if(!std::isfinite(m))
-            throw Error(XH_EINVAL, "CTM: non-finite projector product at site " + std::to_string(site) + ", direction " + std::to_string(dir));
+            throw Error(XH_EINVAL, "CTM: non-finite projector product at site " + std::to_string(sites[b]) + ", direction " + std::to_string(dir));
     }
     auto t0 = std::chrono::steady_clock::now();
-    SVDResult r;
-    if(opt.svd == SvdMode::Full) {
-        FullSVD f = full_svd(prod);
-        r = cutoff(prod, f, opt.chi, 1.e-10, false, 0.0, 0.0);
+    std::vector<SVDResult> rs(nb);
+    auto record = [&](std::size_t b, const SVDResult& r, const FullSVD* f) {
+        const DTensor& prod = prods[b];
         std::map<Q, int> kept;
         for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
-        last_kept[{site, dir}] = kept;
+        last_kept[{sites[b], dir}] = kept;
         if(opt.record_spectra) {
+            // a randomized SVD records the kept singular values (it computes no others exactly)
             SpectrumRecord rec;
-            rec.site = site;
+            rec.site = sites[b];
             rec.dir = dir;
-            for(const auto& [q, s] : f.all_sv) rec.sv[q].push_back(s);
+            if(f)
+                for(const auto& [q, sv] : f->all_sv) rec.sv[q].push_back(sv);
+            else
+                for(const auto& [q, v] : r.spec) rec.sv[q] = v;
             rec.kept = kept;
             for(std::size_t i = 0; i < prod.st->sector.size(); ++i)
                 rec.shape[prod.st->sector[i]] = {static_cast<int>(prod.st->rows[i]), static_cast<int>(prod.st->cols[i])};
             spectra.push_back(std::move(rec));
+        }
+    };
+    if(opt.svd == SvdMode::Full) {
+        for(std::size_t b = 0; b < nb; ++b) {
+            FullSVD f = full_svd(prods[b]);
+            rs[b] = cutoff(prods[b], f, opt.chi, 1.e-10, false, 0.0, 0.0);
+            record(b, rs[b], &f);
         }
     } else {
-        std::map<Q, int> ranks;
-        if(opt.rank_policy)
-            ranks = opt.rank_policy(prod, site, dir);
-        else if(predictor)
-            ranks = predictor->predict(prod, opt.chi, 0xfeedull + 64ull * static_cast<std::uint64_t>(n_rsvd) + 4ull * site + dir);
-        else {
-            auto it = last_kept.find({site, dir});
-            if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
-            ranks = it->second;
+        std::vector<std::map<Q, int>> ranks(nb);
+        std::vector<std::uint64_t> seeds(nb);
+        std::vector<const DTensor*> ptrs(nb);
+        for(std::size_t b = 0; b < nb; ++b) {
+            const int site = sites[b];
+            const std::uint64_t nr = static_cast<std::uint64_t>(n_rsvd) + b;
+            if(opt.rank_policy)
+                ranks[b] = opt.rank_policy(prods[b], site, dir);
+            else if(predictor)
+                ranks[b] = predictor->predict(prods[b], opt.chi, 0xfeedull + 64ull * nr + 4ull * site + dir);
+            else {
+                auto it = last_kept.find({site, dir});
+                if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
+                ranks[b] = it->second;
+            }
+            seeds[b] = 0x5eedull + 64ull * nr + 4ull * site + dir;
+            ptrs[b] = &prods[b];
         }
-        RandomizedSVD rs = tsvd_randomized(prod, ranks, opt.chi, 1.e-10, opt.rsvd, 0x5eedull + 64ull * static_cast<std::uint64_t>(n_rsvd) + 4ull * site + dir);
-        ++n_rsvd;
-        max_residual_ratio = std::max(max_residual_ratio, rs.max_residual_ratio);
-        if(rs.safe) {
-            r = std::move(rs.res);
-        } else {
-            // the full FP64 tSVD is the fallback and the judge
-            ++n_fallback;
-            FullSVD f = full_svd(prod);
-            r = cutoff(prod, f, opt.chi, 1.e-10, false, 0.0, 0.0);
-        }
-        std::map<Q, int> kept;
-        for(const auto& [q, v] : r.spec) kept[q] = static_cast<int>(v.size());
-        last_kept[{site, dir}] = kept;
-        if(opt.record_spectra) {
-            // the kept singular values (the randomized SVD computes no others exactly)
-            SpectrumRecord rec;
-            rec.site = site;
-            rec.dir = dir;
-            for(const auto& [q, v] : r.spec) rec.sv[q] = v;
-            rec.kept = kept;
-            for(std::size_t i = 0; i < prod.st->sector.size(); ++i)
-                rec.shape[prod.st->sector[i]] = {static_cast<int>(prod.st->rows[i]), static_cast<int>(prod.st->cols[i])};
-            spectra.push_back(std::move(rec));
+        std::vector<RandomizedSVD> rsv = tsvd_randomized_batch(ptrs, ranks, opt.chi, 1.e-10, opt.rsvd, seeds);
+        n_rsvd += static_cast<decltype(n_rsvd)>(nb);
+        for(std::size_t b = 0; b < nb; ++b) {
+            max_residual_ratio = std::max(max_residual_ratio, rsv[b].max_residual_ratio);
+            if(rsv[b].safe) {
+                rs[b] = std::move(rsv[b].res);
+            } else {
+                // the full FP64 tSVD is the fallback and the judge
+                ++n_fallback;
+                FullSVD f = full_svd(prods[b]);
+                rs[b] = cutoff(prods[b], f, opt.chi, 1.e-10, false, 0.0, 0.0);
+            }
+            record(b, rs[b], nullptr);
         }
     }
     if(rt->ctx) xh_synchronize(rt->ctx);
     const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     svd_ms += dt;
-    if(opt.verbose) {
-        std::size_t kept = 0;
-        for(const auto& [q, v] : r.spec) kept += v.size();
-        std::fprintf(stderr, "[ctm] site %d dir %d: %zu sectors, %zu kept, svd %.1f ms\n", site, dir, prod.st->sector.size(), kept, dt);
-        std::fflush(stderr);
+    std::vector<std::pair<DTensor, DTensor>> out(nb);
+    for(std::size_t b = 0; b < nb; ++b) {
+        const SVDResult& r = rs[b];
+        if(opt.verbose) {
+            std::size_t kept = 0;
+            for(const auto& [q, v] : r.spec) kept += v.size();
+            std::fprintf(stderr, "[ctm] site %d dir %d: %zu sectors, %zu kept, svd %.1f ms (batch of %zu)\n", sites[b], dir, prods[b].st->sector.size(), kept,
+                         dt, nb);
+            std::fflush(stderr);
+        }
+        const DTensor& Ta = in[b].first;
+        const DTensor& Tb = in[b].second;
+        DTensor isqrtS = diag_map(r.S, 2);  // S.diag_sqrt().diag_inv()
+        DTensor P1 = mul(mul(isqrtS, adjoint(r.U), opt.precision), Ta, opt.precision);
+        DTensor P2 = mul(mul(Tb, adjoint(r.Vdag), opt.precision), isqrtS, opt.precision);
+        if(opt.check_finite) {
+            const std::pair<const char*, const DTensor*> parts[] = {{"U", &r.U}, {"S", &r.S}, {"Vdag", &r.Vdag}, {"P1", &P1}, {"P2", &P2}};
+            for(const auto& [name, t] : parts)
+                if(!std::isfinite(max_norm(*t)))
+                    throw Error(XH_EINVAL, std::string("CTM: non-finite ") + name + " at site " + std::to_string(sites[b]) + ", direction " +
+                                               std::to_string(dir));
+        }
+        out[b] = {std::move(P1), std::move(P2)};
     }
-    DTensor isqrtS = diag_map(r.S, 2);  // S.diag_sqrt().diag_inv()
-    DTensor P1 = mul(mul(isqrtS, adjoint(r.U), opt.precision), Ta, opt.precision);
-    DTensor P2 = mul(mul(Tb, adjoint(r.Vdag), opt.precision), isqrtS, opt.precision);
-    if(opt.check_finite) {
-        const std::pair<const char*, const DTensor*> parts[] = {{"U", &r.U}, {"S", &r.S}, {"Vdag", &r.Vdag}, {"P1", &P1}, {"P2", &P2}};
-        for(const auto& [name, t] : parts)
-            if(!std::isfinite(max_norm(*t)))
-                throw Error(XH_EINVAL, std::string("CTM: non-finite ") + name + " at site " + std::to_string(site) + ", direction " + std::to_string(dir));
-    }
-    return {P1, P2};
+    return out;
 }
 
-std::pair<DTensor, DTensor> CTM::get_projectors(int x, int y, int dir)
+std::pair<DTensor, DTensor> CTM::projector_inputs(int x, int y, int dir)
 {
-    const int site = idx(x, y);
     switch(dir) {
     case D_LEFT: {
         DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
         DTensor Q4 = tw(contract_corner(x, y + 1, LOWER_LEFT), {3, 4, 5});
-        return decompose(Q4, Q1, site, dir);
+        return {Q4, Q1};
     }
     case D_RIGHT: {
         DTensor Q2 = contract_corner(x, y, UPPER_RIGHT);
         DTensor Q3 = contract_corner(x, y + 1, LOWER_RIGHT);
-        return decompose(Q2, Q3, site, dir);
+        return {Q2, Q3};
     }
     case D_TOP: {
         DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
         DTensor Q2 = contract_corner(x + 1, y, UPPER_RIGHT);
-        return decompose(Q1, Q2, site, dir);
+        return {Q1, Q2};
     }
     default: {
         DTensor Q4 = tw(contract_corner(x, y, LOWER_LEFT), {0, 1, 2});
         DTensor Q3 = contract_corner(x + 1, y, LOWER_RIGHT);
-        return decompose(Q3, Q4, site, dir);
+        return {Q3, Q4};
     }
+    }
+}
+
+std::pair<DTensor, DTensor> CTM::get_projectors(int x, int y, int dir)
+{
+    auto [Ta, Tb] = projector_inputs(x, y, dir);
+    return decompose(Ta, Tb, idx(x, y), dir);
+}
+
+void CTM::get_projectors_all(const std::vector<std::pair<int, int>>& xy, int dir, std::vector<DTensor>& P1, std::vector<DTensor>& P2)
+{
+    std::vector<std::pair<DTensor, DTensor>> in;
+    std::vector<int> sites;
+    for(const auto& [x, y] : xy) {
+        in.push_back(projector_inputs(x, y, dir));
+        sites.push_back(idx(x, y));
+    }
+    std::vector<std::pair<DTensor, DTensor>> out = decompose_batch(in, sites, dir);
+    for(std::size_t b = 0; b < out.size(); ++b) {
+        P1[sites[b]] = std::move(out[b].first);
+        P2[sites[b]] = std::move(out[b].second);
     }
 }
 
@@ -275,7 +321,11 @@ void CTM::left_move()
     // CTM.cpp:924-988; projector and new-tensor tables are indexed like the unit cell
     for(int x = 0; x < Lx; ++x) {
         std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C1n(Lx * Ly), T4n(Lx * Ly), C4n(Lx * Ly);
-        for(int y = 0; y < Ly; ++y) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_LEFT);
+        {
+            std::vector<std::pair<int, int>> xy;
+            for(int y = 0; y < Ly; ++y) xy.emplace_back(x, y);
+            get_projectors_all(xy, D_LEFT, P1, P2);
+        }
         for(int y = 0; y < Ly; ++y) {
             // renormalize_left (CTM.cpp:1339-1416)
             DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -4, -2, -3}, 3);
@@ -301,7 +351,11 @@ void CTM::right_move()
     for(int xs = Lx; xs > 0; --xs) {
         const int x = xs % Lx;
         std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C2n(Lx * Ly), T2n(Lx * Ly), C3n(Lx * Ly);
-        for(int y = 0; y < Ly; ++y) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_RIGHT);
+        {
+            std::vector<std::pair<int, int>> xy;
+            for(int y = 0; y < Ly; ++y) xy.emplace_back(x, y);
+            get_projectors_all(xy, D_RIGHT, P1, P2);
+        }
         for(int y = 0; y < Ly; ++y) {
             // renormalize_right (CTM.cpp:1418-1488)
             DTensor T1C2 = ct(T1[idx(x, y - 1)], {-1, 1, -3, -4}, C2[idx(x + 1, y - 1)], {1, -2}, 1);
@@ -326,7 +380,11 @@ void CTM::top_move()
     // CTM.cpp:1053-1117
     for(int y = 0; y < Ly; ++y) {
         std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C1n(Lx * Ly), T1n(Lx * Ly), C2n(Lx * Ly);
-        for(int x = 0; x < Lx; ++x) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_TOP);
+        {
+            std::vector<std::pair<int, int>> xy;
+            for(int x = 0; x < Lx; ++x) xy.emplace_back(x, y);
+            get_projectors_all(xy, D_TOP, P1, P2);
+        }
         for(int x = 0; x < Lx; ++x) {
             // renormalize_top (CTM.cpp:1490-1562)
             DTensor T4C1 = ct(T4[idx(x - 1, y)], {1, -1, -3, -4}, tw(C1[idx(x - 1, y - 1)], {0}), {1, -2}, 1);
@@ -352,7 +410,11 @@ void CTM::bottom_move()
     for(int ys = Ly; ys > 0; --ys) {
         const int y = ys % Ly;
         std::vector<DTensor> P1(Lx * Ly), P2(Lx * Ly), C4n(Lx * Ly), T3n(Lx * Ly), C3n(Lx * Ly);
-        for(int x = 0; x < Lx; ++x) std::tie(P1[idx(x, y)], P2[idx(x, y)]) = get_projectors(x, y, D_BOTTOM);
+        {
+            std::vector<std::pair<int, int>> xy;
+            for(int x = 0; x < Lx; ++x) xy.emplace_back(x, y);
+            get_projectors_all(xy, D_BOTTOM, P1, P2);
+        }
         for(int x = 0; x < Lx; ++x) {
             // renormalize_bottom (CTM.cpp:1564-1641)
             DTensor T4C4 = ct(T4[idx(x - 1, y)], {-4, 1, -2, -3}, C4[idx(x - 1, y + 1)], {1, -1}, 3);

@@ -124,7 +124,13 @@ private:
     Qbasis bra(int x, int y, int leg) const;
     DTensor contract_corner(int x, int y, int corner) const;
     std::pair<DTensor, DTensor> decompose(const DTensor& T1, const DTensor& T2, int site, int dir);
+    // decompose() of several (T1, T2) pairs of one move; the randomized SVDs run as one batch
+    std::vector<std::pair<DTensor, DTensor>> decompose_batch(const std::vector<std::pair<DTensor, DTensor>>& in, const std::vector<int>& sites,
+                                                             int dir);
+    std::pair<DTensor, DTensor> projector_inputs(int x, int y, int dir);
     std::pair<DTensor, DTensor> get_projectors(int x, int y, int dir);
+    // get_projectors for the sites of one move, with their SVDs batched
+    void get_projectors_all(const std::vector<std::pair<int, int>>& xy, int dir, std::vector<DTensor>& P1, std::vector<DTensor>& P2);
     void left_move();
     void right_move();
     void top_move();
@@ -145,5 +151,10 @@ struct RandomizedSVD {
 };
 RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
                               std::uint64_t seed);
+// The randomized SVDs of several tensors in one set of grouped launches (every GEMM, Cholesky and
+// block SVD launch holds the blocks of all of them); result i is what tsvd_randomized(*ts[i],
+// ranks[i], ..., seeds[i]) computes, up to the summation order of split-K GEMMs.
+std::vector<RandomizedSVD> tsvd_randomized_batch(const std::vector<const DTensor*>& ts, const std::vector<std::map<Q, int>>& ranks,
+                                                 std::size_t max_keep, double eps_svd, const RsvdOptions& o, const std::vector<std::uint64_t>& seeds);
 
 }  // namespace xa

@@ -70,6 +70,7 @@ struct Plan {
 };
 
 struct Sec {
+    int t;                   // input tensor of the batch
     int i;                   // sector index in the input tensor
     int64_t a, m, n, l;      // block offset and extents
     int64_t ym, zn, g, s;    // offsets: m x l panels, n x l panels, l x l blocks, l vectors
@@ -80,30 +81,52 @@ struct Sec {
 RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
                               std::uint64_t seed)
 {
-    t.rt->require_device();
-    xh_ctx* ctx = t.rt->ctx;
-    const TensorStructure& s = *t.st;
+    return std::move(tsvd_randomized_batch({&t}, {ranks}, max_keep, eps_svd, o, {seed})[0]);
+}
+
+std::vector<RandomizedSVD> tsvd_randomized_batch(const std::vector<const DTensor*>& ts, const std::vector<std::map<Q, int>>& ranks_of,
+                                                 std::size_t max_keep, double eps_svd, const RsvdOptions& o, const std::vector<std::uint64_t>& seeds)
+{
+    const int nt = static_cast<int>(ts.size());
+    if(nt == 0) return {};
+    if(ranks_of.size() != ts.size() || seeds.size() != ts.size()) throw Error(XH_EINVAL, "randomized SVD batch: one rank map and one seed per tensor");
+    Runtime* rt = ts[0]->rt;
+    rt->require_device();
+    for(const DTensor* t : ts)
+        if(t->rt != rt) throw Error(XH_EINVAL, "randomized SVD batch: tensors of different runtimes");
+    xh_ctx* ctx = rt->ctx;
+    // the grouped launches address every tensor's blocks from the lowest data address
+    const double* A = nullptr;
+    for(const DTensor* t : ts)
+        if(t->size() && (!A || t->data() < A)) A = t->data();
     std::vector<Sec> secs;
+    std::vector<int64_t> zbeg(nt + 1, 0);
     int64_t nY = 0, nZ = 0, nG = 0, nS = 0;
-    for(std::size_t i = 0; i < s.sector.size(); ++i) {
-        const int64_t m = static_cast<int64_t>(s.rows[i]), n = static_cast<int64_t>(s.cols[i]);
-        if(m == 0 || n == 0) continue;
-        auto it = ranks.find(s.sector[i]);
-        const int k = it == ranks.end() ? 0 : std::max(0, it->second);
-        const int64_t want = static_cast<int64_t>(std::ceil(o.rank_factor * k)) + o.oversample;
-        const int64_t l = std::min<int64_t>(std::max<int64_t>(want, 1), std::min(m, n));
-        secs.push_back(Sec{static_cast<int>(i), static_cast<int64_t>(s.offsets[i]), m, n, l, nY, nZ, nG, nS});
-        nY += m * l;
-        nZ += n * l;
-        nG += l * l;
-        nS += l;
+    for(int ti = 0; ti < nt; ++ti) {
+        const TensorStructure& s = *ts[ti]->st;
+        zbeg[ti] = nZ;
+        if(!ts[ti]->size()) continue;
+        const int64_t tbase = static_cast<int64_t>((reinterpret_cast<std::uintptr_t>(ts[ti]->data()) - reinterpret_cast<std::uintptr_t>(A)) / sizeof(double));
+        for(std::size_t i = 0; i < s.sector.size(); ++i) {
+            const int64_t m = static_cast<int64_t>(s.rows[i]), n = static_cast<int64_t>(s.cols[i]);
+            if(m == 0 || n == 0) continue;
+            auto it = ranks_of[ti].find(s.sector[i]);
+            const int k = it == ranks_of[ti].end() ? 0 : std::max(0, it->second);
+            const int64_t want = static_cast<int64_t>(std::ceil(o.rank_factor * k)) + o.oversample;
+            const int64_t l = std::min<int64_t>(std::max<int64_t>(want, 1), std::min(m, n));
+            secs.push_back(Sec{ti, static_cast<int>(i), tbase + static_cast<int64_t>(s.offsets[i]), m, n, l, nY, nZ, nG, nS});
+            nY += m * l;
+            nZ += n * l;
+            nG += l * l;
+            nS += l;
+        }
     }
-    RandomizedSVD out;
+    zbeg[nt] = nZ;
+    std::vector<RandomizedSVD> outs(nt);
     if(secs.empty()) {
-        out.safe = false;
-        return out;
+        for(auto& out : outs) out.safe = false;
+        return outs;
     }
-    const double* A = t.data();
     auto items = [&](auto f) {
         std::vector<xh_gemm_item> v;
         v.reserve(secs.size());
@@ -148,7 +171,9 @@ RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, s
         return b1;
     };
     // range finder
-    dchk(ctx, xh_fill_gaussian(ctx, Z0.p, static_cast<size_t>(nZ), seed), "gaussian test matrix");
+    for(int ti = 0; ti < nt; ++ti)
+        if(zbeg[ti + 1] > zbeg[ti])
+            dchk(ctx, xh_fill_gaussian(ctx, Z0.p + zbeg[ti], static_cast<size_t>(zbeg[ti + 1] - zbeg[ti]), seeds[ti]), "gaussian test matrix");
     p_sketch.run(A, Z0.p, Y0.p, o.sketch_precision);
     // inside the subspace iteration one shifted CholeskyQR pass keeps the panels well conditioned;
     // the basis that B is projected on gets all three passes
@@ -170,51 +195,60 @@ RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, s
     // singular values and the captured weights
     std::vector<double> sv(nS);
     dchk(ctx, xh_download(ctx, sv.data(), Sr.p, static_cast<size_t>(nS) * sizeof(double)), "download singular values");
-    std::vector<std::pair<Q, double>> all_sv;
-    for(const auto& c : secs)
-        for(int64_t j = 0; j < c.l; ++j) all_sv.emplace_back(s.sector[c.i], sv[c.s + j]);
-    TruncRule rule = truncation_rule(s.sym, all_sv, max_keep, eps_svd, false, 0.0, 0.0);
-    double sigma_cut = std::numeric_limits<double>::infinity();
-    for(const auto& [q, v] : rule.kept)
-        for(double x : v) sigma_cut = std::min(sigma_cut, x);
-    for(const auto& c : secs) {
-        if(c.l >= std::min(c.m, c.n)) continue;  // the sketch spans the whole block
-        const double ratio = sv[c.s + c.l - 1] / sigma_cut;
-        out.max_residual_ratio = std::max(out.max_residual_ratio, ratio);
-        if(!(o.safety * ratio < 1.0)) out.safe = false;
+    for(int ti = 0; ti < nt; ++ti) {
+        const TensorStructure& s = *ts[ti]->st;
+        RandomizedSVD& out = outs[ti];
+        if(!ts[ti]->size()) {
+            out.safe = false;
+            continue;
+        }
+        std::vector<std::pair<Q, double>> all_sv;
+        for(const auto& c : secs)
+            if(c.t == ti)
+                for(int64_t j = 0; j < c.l; ++j) all_sv.emplace_back(s.sector[c.i], sv[c.s + j]);
+        TruncRule rule = truncation_rule(s.sym, all_sv, max_keep, eps_svd, false, 0.0, 0.0);
+        double sigma_cut = std::numeric_limits<double>::infinity();
+        for(const auto& [q, v] : rule.kept)
+            for(double x : v) sigma_cut = std::min(sigma_cut, x);
+        for(const auto& c : secs) {
+            if(c.t != ti || c.l >= std::min(c.m, c.n)) continue;  // the sketch spans the whole block
+            const double ratio = sv[c.s + c.l - 1] / sigma_cut;
+            out.max_residual_ratio = std::max(out.max_residual_ratio, ratio);
+            if(!(o.safety * ratio < 1.0)) out.safe = false;
+        }
+        // U, Sigma, Vdag over the kept sectors (ascending q, as cutoff_matrices builds them)
+        Qbasis truncBasis(s.sym, 1);
+        std::vector<Q> order;
+        for(const auto& [q, v] : rule.kept) {
+            truncBasis.push_back(q, v.size());
+            order.push_back(q);
+        }
+        SVDResult& res = out.res;
+        res.trunc_weight = rule.trunc_weight;
+        res.entropy = rule.entropy;
+        res.spec = rule.kept;
+        res.n_states = all_sv.size();
+        res.U = DTensor::empty(rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {truncBasis}, order));
+        res.S = DTensor::empty(rt, TensorStructure::create_pushed(s.sym, {truncBasis}, {truncBasis}, order));
+        res.Vdag = DTensor::empty(rt, TensorStructure::create_pushed(s.sym, {truncBasis}, s.unc_cod, order));
+        std::vector<xh_copy_rect> ru, rv;
+        std::vector<double> hs(res.S.size(), 0.0);
+        for(std::size_t o2 = 0; o2 < order.size(); ++o2) {
+            const Q& q = order[o2];
+            const Sec* c = nullptr;
+            for(const auto& x : secs)
+                if(x.t == ti && s.sector[x.i] == q) c = &x;
+            if(!c) throw Error(XH_EINVAL, "randomized SVD: kept sector without a sketch");
+            const int64_t nret = static_cast<int64_t>(rule.kept[q].size());
+            ru.push_back(xh_copy_rect{c->ym, c->m, static_cast<int64_t>(res.U.st->offsets[o2]), c->m, I32(c->m), I32(nret), 1.0, 0});
+            rv.push_back(xh_copy_rect{c->zn, c->n, static_cast<int64_t>(res.Vdag.st->offsets[o2]), nret, I32(nret), I32(c->n), 1.0, 1});
+            for(int64_t k = 0; k < nret; ++k) hs[res.S.st->offsets[o2] + k + k * nret] = rule.kept[q][k];
+        }
+        dchk(ctx, xh_copy_rects(ctx, Y0.p, res.U.data(), ru.data(), static_cast<int64_t>(ru.size())), "truncate U");
+        dchk(ctx, xh_copy_rects(ctx, Z0.p, res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
+        if(res.S.size()) res.S.upload(hs.data(), hs.size());
     }
-    // U, Sigma, Vdag over the kept sectors (ascending q, as cutoff_matrices builds them)
-    Qbasis truncBasis(s.sym, 1);
-    std::vector<Q> order;
-    for(const auto& [q, v] : rule.kept) {
-        truncBasis.push_back(q, v.size());
-        order.push_back(q);
-    }
-    SVDResult& res = out.res;
-    res.trunc_weight = rule.trunc_weight;
-    res.entropy = rule.entropy;
-    res.spec = rule.kept;
-    res.n_states = all_sv.size();
-    res.U = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {truncBasis}, order));
-    res.S = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {truncBasis}, {truncBasis}, order));
-    res.Vdag = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {truncBasis}, s.unc_cod, order));
-    std::vector<xh_copy_rect> ru, rv;
-    std::vector<double> hs(res.S.size(), 0.0);
-    for(std::size_t o2 = 0; o2 < order.size(); ++o2) {
-        const Q& q = order[o2];
-        const Sec* c = nullptr;
-        for(const auto& x : secs)
-            if(s.sector[x.i] == q) c = &x;
-        if(!c) throw Error(XH_EINVAL, "randomized SVD: kept sector without a sketch");
-        const int64_t nret = static_cast<int64_t>(rule.kept[q].size());
-        ru.push_back(xh_copy_rect{c->ym, c->m, static_cast<int64_t>(res.U.st->offsets[o2]), c->m, I32(c->m), I32(nret), 1.0, 0});
-        rv.push_back(xh_copy_rect{c->zn, c->n, static_cast<int64_t>(res.Vdag.st->offsets[o2]), nret, I32(nret), I32(c->n), 1.0, 1});
-        for(int64_t k = 0; k < nret; ++k) hs[res.S.st->offsets[o2] + k + k * nret] = rule.kept[q][k];
-    }
-    dchk(ctx, xh_copy_rects(ctx, Y0.p, res.U.data(), ru.data(), static_cast<int64_t>(ru.size())), "truncate U");
-    dchk(ctx, xh_copy_rects(ctx, Z0.p, res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
-    if(res.S.size()) res.S.upload(hs.data(), hs.size());
-    return out;
+    return outs;
 }
 
 }  // namespace xa
