@@ -149,6 +149,12 @@ int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, 
 int xh_scale(xh_ctx* ctx, double* x, size_t n, double alpha);                          /* TensorBase.hpp:197-209 */
 int xh_axpby(xh_ctx* ctx, double* y, const double* x, size_t n, double a, double b);    /* y = a*x + b*y (CoeffBinaryOp); b == 0: y not read */
 int xh_max_abs(xh_ctx* ctx, const double* x, size_t n, double* out);                   /* maxNorm, TensorBase.cpp:48-56 */
+/* y = x / maxNorm(x), the normalisation of the reference's CTM moves (CTM.cpp:1372, 1383, ...:
+ * `C / C.maxNorm()`), without a host round trip: the norm is reduced and applied on the device.
+ * A vanished or non-finite norm copies x unchanged and is counted; xh_normalize_status returns
+ * the count since the last nonzero report and the first such norm (synchronous; resets). */
+int xh_normalize_max_abs(xh_ctx* ctx, const double* x, double* y, size_t n);
+int xh_normalize_status(xh_ctx* ctx, int64_t* count, double* first);
 int xh_dot(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out);       /* squaredNorm building block */
 /* Per block g (m_g x n_g at off_g): out[g] = trace of the block (TensorBase.cpp:26-46). */
 int xh_block_trace(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* x, double* out);

@@ -91,6 +91,8 @@ _SIGS = [
     ("xh_scale", C.c_int, [_P, _P, _SZ, C.c_double]),
     ("xh_axpby", C.c_int, [_P, _P, _P, _SZ, C.c_double, C.c_double]),
     ("xh_max_abs", C.c_int, [_P, _P, _SZ, _DP]),
+    ("xh_normalize_max_abs", C.c_int, [_P, _P, _P, _SZ]),
+    ("xh_normalize_status", C.c_int, [_P, C.POINTER(C.c_int64), _DP]),
     ("xh_dot", C.c_int, [_P, _P, _P, _SZ, _DP]),
     ("xh_mlp_create", C.c_int, [_P, C.c_int, C.c_int, C.c_uint64, _PP]),
     ("xh_mlp_destroy", C.c_int, [_P, _P]),

@@ -89,11 +89,26 @@ Qbasis CTM::bra(int x, int y, int leg) const
 
 DTensor CTM::normalized(const DTensor& t) const
 {
-    // the reference divides by maxNorm unchecked (CTM.cpp:1372, 1383, ...); a vanished or non-finite
-    // environment tensor is reported instead of being propagated as inf/NaN
-    const double m = max_norm(t);
-    if(!(m > 0.0) || !std::isfinite(m)) throw Error(XH_EINVAL, "CTM: environment tensor with max norm " + std::to_string(m) + " cannot be normalized");
-    return scaled(t, 1.0 / m);
+    // the reference divides by maxNorm unchecked (CTM.cpp:1372, 1383, ...).  Here the norm stays on
+    // the device; a vanished or non-finite one is counted there and check_normalized() reports it
+    // (before the next projector batch and at the end of every move) instead of propagating inf/NaN
+    if(!rt->ctx || !t.size()) {
+        const double m = max_norm(t);
+        if(!(m > 0.0) || !std::isfinite(m)) throw Error(XH_EINVAL, "CTM: environment tensor with max norm " + std::to_string(m) + " cannot be normalized");
+        return scaled(t, 1.0 / m);
+    }
+    DTensor out = DTensor::empty(t.rt, t.st);
+    if(xh_normalize_max_abs(rt->ctx, t.data(), out.data(), t.size()) != XH_OK) throw Error(XH_EHIP, std::string("CTM normalize: ") + xh_last_error(rt->ctx));
+    return out;
+}
+
+void CTM::check_normalized() const
+{
+    if(!rt->ctx) return;
+    int64_t count = 0;
+    double first = 0.0;
+    if(xh_normalize_status(rt->ctx, &count, &first) != XH_OK) throw Error(XH_EHIP, std::string("CTM normalize status: ") + xh_last_error(rt->ctx));
+    if(count) throw Error(XH_EINVAL, "CTM: environment tensor with max norm " + std::to_string(first) + " cannot be normalized");
 }
 
 void CTM::init_from_A()
@@ -170,6 +185,7 @@ std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<
     // move are independent; their randomized SVDs share grouped launches.
     const std::size_t nb = in.size();
     if(sites.size() != nb) throw Error(XH_EINVAL, "CTM: one site per projector pair");
+    check_normalized();
     std::vector<DTensor> prods(nb);
     for(std::size_t b = 0; b < nb; ++b) {
         prods[b] = mul(in[b].first, in[b].second, opt.precision);
@@ -437,9 +453,13 @@ void CTM::bottom_move()
 void CTM::grow_all()
 {
     left_move();
+    check_normalized();
     right_move();
+    check_normalized();
     top_move();
+    check_normalized();
     bottom_move();
+    check_normalized();
 }
 
 void CTM::compute_rdm()

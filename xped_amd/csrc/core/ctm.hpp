@@ -136,6 +136,7 @@ private:
     void top_move();
     void bottom_move();
     DTensor normalized(const DTensor& t) const;
+    void check_normalized() const;  // throws for a vanished or non-finite norm met by normalized()
 };
 
 // Randomized truncated block SVD (Halko, Martinsson & Tropp, SIAM Review 53 (2011), Alg. 4.4,

@@ -39,6 +39,7 @@ extern "C" int xh_create(int device, xh_ctx** out)
         check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr), "hipMemPoolSetAttribute");
         check(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scalar), 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
         check(hipMalloc(reinterpret_cast<void**>(&ctx->dev_scalar), (xh::RED_PARTS + 64) * sizeof(double)), "hipMalloc");
+        check(hipMemset(ctx->dev_scalar + xh::RED_PARTS, 0, 64 * sizeof(double)), "hipMemset");  // normalisation status
     });
     if(rc != XH_OK) {
         delete ctx;
@@ -297,6 +298,21 @@ __global__ void k_fill(double* __restrict__ x, size_t n, double v)
 __global__ void k_scale(double* __restrict__ x, size_t n, double a)
 {
     for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = a * x[i];
+}
+
+// y = x / m with m = *res, the max-abs of x folded on the device.  A vanished or non-finite m
+// copies x unchanged and is counted in status[0] (the first such m kept in status[1]).
+__global__ void k_scale_inv_dev(const double* __restrict__ x, double* __restrict__ y, size_t n, const double* __restrict__ res,
+                                double* __restrict__ status)
+{
+    const double m = *res;
+    const bool ok = m > 0.0 && m <= DBL_MAX;  // NaN fails both
+    if(!ok && blockIdx.x == 0 && threadIdx.x == 0) {
+        if(status[0] == 0.0) status[1] = m;
+        status[0] += 1.0;
+    }
+    const double a = ok ? 1.0 / m : 1.0;
+    for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = a * x[i];
 }
 
 // y = a x + b y; with b == 0, y is not read (it may be uninitialised, and 0 * NaN is NaN), and
@@ -605,6 +621,36 @@ static void reduce(xh_ctx* ctx, const double* x, const double* y, size_t n, doub
 extern "C" int xh_max_abs(xh_ctx* ctx, const double* x, size_t n, double* out)
 {
     return guard(ctx, [&] { reduce(ctx, x, nullptr, n, out, true); });
+}
+
+extern "C" int xh_normalize_max_abs(xh_ctx* ctx, const double* x, double* y, size_t n)
+{
+    // asynchronous: the norm never leaves the device
+    return guard(ctx, [&] {
+        if(n == 0) return;
+        const size_t want = (n + EW_THREADS - 1) / EW_THREADS;
+        const int nb = static_cast<int>(std::min<size_t>(xh::RED_PARTS, want));
+        double* res = ctx->dev_scalar + xh::RED_PARTS;
+        hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, n, ctx->dev_scalar);
+        xh::launch_check("k_maxabs");
+        hipLaunchKernelGGL(k_fold, dim3(1), dim3(EW_THREADS), 0, ctx->stream, ctx->dev_scalar, nb, res, 1);
+        xh::launch_check("fold");
+        hipLaunchKernelGGL(k_scale_inv_dev, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, x, y, n, res, res + 2);
+        xh::launch_check("k_scale_inv_dev");
+    });
+}
+
+extern "C" int xh_normalize_status(xh_ctx* ctx, int64_t* count, double* first)
+{
+    return guard(ctx, [&] {
+        xh::refuse_under_capture(ctx, "normalize status");
+        double* st = ctx->dev_scalar + xh::RED_PARTS + 2;
+        check(hipMemcpyAsync(ctx->host_scalar + 8, st, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        *count = static_cast<int64_t>(ctx->host_scalar[8]);
+        *first = ctx->host_scalar[9];
+        if(*count) check(hipMemsetAsync(st, 0, 2 * sizeof(double), ctx->stream), "hipMemsetAsync");
+    });
 }
 
 extern "C" int xh_dot(xh_ctx* ctx, const double* x, const double* y, size_t n, double* out)
