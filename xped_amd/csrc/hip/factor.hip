@@ -307,12 +307,19 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
 //       X_{i,k-1} = -sv[k-1] * sum_{l=i}^{k-2} X_il R_{l,k-1}  (X R = I by columns).
 // (a) writes row k-1 (columns >= k), (b) reads row k and writes rows > k, (c) reads rows < k-1 of
 // R and columns < k-1 of X and writes column k-1 of X, so one barrier per step suffices.
+// Blocks in the global working copy use a panel variant (PANEL = true) of the same steps,
+// described at its loop; in LDS the per-pivot barrier, not the trailing update's traffic, bounds
+// the time, and the plain loop is as fast.
 constexpr int CI_THREADS = 1024;
 constexpr int CI_LANES = 16;  // lanes per dot product in (c)
+constexpr int CI_PANEL = 4;   // pivots per trailing update of the panel variant
+constexpr int CI_MAX_N = 40000;  // packed indices of a block fit in 32 bits
 
 __host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * (n + 1) / 2 + 2 * n; }
 __device__ __forceinline__ int64_t pk(int i, int j) { return int64_t(j) * (j + 1) / 2 + i; }
+__device__ __forceinline__ int pk32(int i, int j) { return ((j * (j + 1)) >> 1) + i; }  // n <= CI_MAX_N
 
+template <bool PANEL>
 __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
 {
     extern __shared__ double smem[];
@@ -350,66 +357,160 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
             rd[k] = d;
         }
     }
-    for(int k = 0; k <= n; ++k) {
-        __syncthreads();
-        double inv_d = 0.0;
-        if(k < n) {
-            const double d = W[pk(k, k)];
-            const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
-            inv_d = ok ? 1.0 / d : 0.0;
-            if(tid == 0) sv[k] = ok ? 1.0 / sqrt(d) : 0.0;  // read from step k + 1 on
-        }
-        if(k >= 1) {
-            // (a) deferred scaling of row k - 1
-            const double s = sv[k - 1];
-            for(int j = k + tid; j < n; j += CI_THREADS) W[pk(k - 1, j)] *= s;
-        }
-        if(k + 1 < n && inv_d != 0.0) {
-            // (b) columns a = k+1+pp (rows k+1 .. a) and b = n-1-pp (rows k+1 .. b) as one list
-            const int t = n - k - 1;
-            const int npairs = (t + 1) / 2;
-            const int c2 = npairs <= CI_THREADS ? CI_THREADS / npairs : 1;
-            const int p0 = npairs <= CI_THREADS ? tid % npairs : tid;
-            const int s0 = npairs <= CI_THREADS ? tid / npairs : 0;
-            if(s0 < c2) {
-                for(int pp = p0; pp < npairs; pp += (npairs <= CI_THREADS ? npairs : CI_THREADS)) {
-                    const int a = pp, b = t - 1 - pp;
-                    const int la = a + 1;
-                    const int len = (a == b) ? la : la + b + 1;
-                    const double wka = W[pk(k, k + 1 + a)] * inv_d, wkb = W[pk(k, k + 1 + b)] * inv_d;
-                    for(int e = s0; e < len; e += c2) {
-                        const bool first = e < la;
-                        const int i = k + 1 + (first ? e : e - la);
-                        const int j = k + 1 + (first ? a : b);
-                        W[pk(i, j)] -= W[pk(k, i)] * (first ? wka : wkb);
+    if constexpr(!PANEL) {
+        for(int k = 0; k <= n; ++k) {
+            __syncthreads();
+            double inv_d = 0.0;
+            if(k < n) {
+                const double d = W[pk(k, k)];
+                const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
+                inv_d = ok ? 1.0 / d : 0.0;
+                if(tid == 0) sv[k] = ok ? 1.0 / sqrt(d) : 0.0;  // read from step k + 1 on
+            }
+            if(k >= 1) {
+                // (a) deferred scaling of row k - 1
+                const double s = sv[k - 1];
+                for(int j = k + tid; j < n; j += CI_THREADS) W[pk(k - 1, j)] *= s;
+            }
+            if(k + 1 < n && inv_d != 0.0) {
+                // (b) columns a = k+1+pp (rows k+1 .. a) and b = n-1-pp (rows k+1 .. b) as one list
+                const int t = n - k - 1;
+                const int npairs = (t + 1) / 2;
+                const int c2 = npairs <= CI_THREADS ? CI_THREADS / npairs : 1;
+                const int p0 = npairs <= CI_THREADS ? tid % npairs : tid;
+                const int s0 = npairs <= CI_THREADS ? tid / npairs : 0;
+                if(s0 < c2) {
+                    for(int pp = p0; pp < npairs; pp += (npairs <= CI_THREADS ? npairs : CI_THREADS)) {
+                        const int a = pp, b = t - 1 - pp;
+                        const int la = a + 1;
+                        const int len = (a == b) ? la : la + b + 1;
+                        const double wka = W[pk(k, k + 1 + a)] * inv_d, wkb = W[pk(k, k + 1 + b)] * inv_d;
+                        for(int e = s0; e < len; e += c2) {
+                            const bool first = e < la;
+                            const int i = k + 1 + (first ? e : e - la);
+                            const int j = k + 1 + (first ? a : b);
+                            W[pk(i, j)] -= W[pk(k, i)] * (first ? wka : wkb);
+                        }
                     }
                 }
             }
-        }
-        if(k >= 2) {
-            // (c) column j = k - 1 of X, rows i < j; X_il is read from X^T at X[l + i n]
-            const int j = k - 1;
-            constexpr int per = CI_THREADS / CI_LANES;
-            const double sj = sv[j];
-            for(int base = 0; base < j; base += per) {
-                const int i = base + tid / CI_LANES, part = tid & (CI_LANES - 1);
-                double acc0 = 0.0, acc1 = 0.0;
-                if(i < j) {
-                    int l = i + part;
-                    for(; l + CI_LANES < j; l += 2 * CI_LANES) {
-                        const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
-                        const double x1 = X[l + CI_LANES + int64_t(i) * n];
-                        acc0 = fma(x0, W[pk(l, j)], acc0);
-                        acc1 = fma(x1, W[pk(l + CI_LANES, j)], acc1);
+            if(k >= 2) {
+                // (c) column j = k - 1 of X, rows i < j; X_il is read from X^T at X[l + i n]
+                const int j = k - 1;
+                constexpr int per = CI_THREADS / CI_LANES;
+                const double sj = sv[j];
+                for(int base = 0; base < j; base += per) {
+                    const int i = base + tid / CI_LANES, part = tid & (CI_LANES - 1);
+                    double acc0 = 0.0, acc1 = 0.0;
+                    if(i < j) {
+                        int l = i + part;
+                        for(; l + CI_LANES < j; l += 2 * CI_LANES) {
+                            const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
+                            const double x1 = X[l + CI_LANES + int64_t(i) * n];
+                            acc0 = fma(x0, W[pk(l, j)], acc0);
+                            acc1 = fma(x1, W[pk(l + CI_LANES, j)], acc1);
+                        }
+                        if(l < j) {
+                            const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
+                            acc0 = fma(x0, W[pk(l, j)], acc0);
+                        }
                     }
-                    if(l < j) {
-                        const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
-                        acc0 = fma(x0, W[pk(l, j)], acc0);
+                    double acc = acc0 + acc1;
+                    for(int o = CI_LANES >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                    if(i < j && part == 0) X[j + int64_t(i) * n] = -acc * sj;
+                }
+            }
+        }
+    } else {
+        // panel variant (blocks in the global working copy): rows of W stay unscaled (row k of R is
+        // W's row k times sv[k]) and are never written after their pivot step.  Inside a panel of
+        // CI_PANEL pivots, (a) updates the panel's remaining rows by pivot k; at the panel's last
+        // pivot, (b) applies the rank-CI_PANEL update of all its pivots to the rows below, so the
+        // trailing triangle is read and written once per panel.  (c) is as above with R_lj =
+        // sv[l] W_lj.  1 / d_r of an earlier pivot of the panel is taken as sv[r]^2.
+        for(int k = 0; k <= n; ++k) {
+            __syncthreads();
+            if(k < n) {
+                const double d = W[pk32(k, k)];
+                const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
+                const double s = ok ? 1.0 / sqrt(d) : 0.0;
+                const double inv_d = s * s;
+                if(tid == 0) sv[k] = s;  // read from step k + 1 on
+                const int k0 = k - k % CI_PANEL;
+                const int kl = min(k0 + CI_PANEL, n) - 1;
+                const int t = n - k - 1;
+                if(k < kl) {
+                    if(inv_d != 0.0) {
+                        const int ne = (kl - k) * t;
+                        for(int e = tid; e < ne; e += CI_THREADS) {
+                            const int i = k + 1 + e / t, j = k + 1 + e % t;
+                            if(j >= i) W[pk32(i, j)] -= W[pk32(k, i)] * (W[pk32(k, j)] * inv_d);
+                        }
+                    }
+                } else if(t > 0) {
+                    // k == kl < n - 1 implies a full panel: k - k0 + 1 == CI_PANEL
+                    const int npairs = (t + 1) / 2;
+                    const int c2 = npairs <= CI_THREADS ? CI_THREADS / npairs : 1;
+                    const int p0 = npairs <= CI_THREADS ? tid % npairs : tid;
+                    const int s0 = npairs <= CI_THREADS ? tid / npairs : 0;
+                    if(s0 < c2) {
+                        double idr[CI_PANEL];
+#pragma unroll
+                        for(int r = 0; r < CI_PANEL - 1; ++r) idr[r] = sv[k0 + r] * sv[k0 + r];
+                        idr[CI_PANEL - 1] = inv_d;
+                        for(int pp = p0; pp < npairs; pp += (npairs <= CI_THREADS ? npairs : CI_THREADS)) {
+                            const int a = pp, b = t - 1 - pp;
+                            const int la = a + 1;
+                            const int len = (a == b) ? la : la + b + 1;
+                            const int ca = pk32(0, k + 1 + a), cb = pk32(0, k + 1 + b);
+                            double fa[CI_PANEL], fb[CI_PANEL];
+#pragma unroll
+                            for(int r = 0; r < CI_PANEL; ++r) {
+                                fa[r] = W[ca + k0 + r] * idr[r];
+                                fb[r] = W[cb + k0 + r] * idr[r];
+                            }
+                            for(int e = s0; e < len; e += c2) {
+                                const bool first = e < la;
+                                const int i = k + 1 + (first ? e : e - la);
+                                const int bi = pk32(k0, i);
+                                const int pos = (first ? ca : cb) + i;
+                                double acc = W[pos];
+#pragma unroll
+                                for(int r = 0; r < CI_PANEL; ++r) acc = fma(-W[bi + r], first ? fa[r] : fb[r], acc);
+                                W[pos] = acc;
+                            }
+                        }
                     }
                 }
-                double acc = acc0 + acc1;
-                for(int o = CI_LANES >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                if(i < j && part == 0) X[j + int64_t(i) * n] = -acc * sj;
+            }
+            if(k >= 2) {
+                const int j = k - 1;
+                const int cj = pk32(0, j);
+                constexpr int per = CI_THREADS / CI_LANES;
+                const double sj = sv[j];
+                for(int base = 0; base < j; base += per) {
+                    const int i = base + tid / CI_LANES, part = tid & (CI_LANES - 1);
+                    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+                    if(i < j) {
+                        const double* xr = X + int64_t(i) * n;
+                        int l = i + part;
+                        for(; l + 3 * CI_LANES < j; l += 4 * CI_LANES) {
+                            const double x0 = (l == i) ? sv[i] : xr[l];
+                            const double x1 = xr[l + CI_LANES], x2 = xr[l + 2 * CI_LANES], x3 = xr[l + 3 * CI_LANES];
+                            acc0 = fma(x0 * sv[l], W[cj + l], acc0);
+                            acc1 = fma(x1 * sv[l + CI_LANES], W[cj + l + CI_LANES], acc1);
+                            acc2 = fma(x2 * sv[l + 2 * CI_LANES], W[cj + l + 2 * CI_LANES], acc2);
+                            acc3 = fma(x3 * sv[l + 3 * CI_LANES], W[cj + l + 3 * CI_LANES], acc3);
+                        }
+                        for(; l < j; l += CI_LANES) {
+                            const double x0 = (l == i) ? sv[i] : xr[l];
+                            acc0 = fma(x0 * sv[l], W[cj + l], acc0);
+                        }
+                    }
+                    double acc = (acc0 + acc1) + (acc2 + acc3);
+                    for(int o = CI_LANES >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+                    if(i < j && part == 0) X[j + int64_t(i) * n] = -acc * sj;
+                }
             }
         }
     }
@@ -706,6 +807,7 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
             xh_factor_item it = items[i];
             if(it.m != it.n) throw xh::Error(XH_EINVAL, "chol_inv: blocks must be square");
             if(it.n < 0 || it.a_off < 0 || it.u_off < 0) throw xh::Error(XH_EINVAL, "chol_inv: negative extent or offset");
+            if(it.n > CI_MAX_N) throw xh::Error(XH_EINVAL, "chol_inv: block larger than 40000");
             if(it.n == 0) continue;
             it.s_off = static_cast<int64_t>(sh.size());
             sh.push_back(shift_rel ? shift_rel[i] : 0.0);
@@ -726,16 +828,16 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
         if(!small.empty()) {
             const size_t shmem = size_t(small_max) * sizeof(double);
             if(shmem > 64 * 1024)
-                check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol_inv), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
+                check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol_inv<false>), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
             FactorArgs fa{up.ptr<xh_factor_item>(o_small), G, X, d_sh, nullptr, nullptr, 0, 1, nullptr};
-            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(small.size())), dim3(CI_THREADS), shmem, ctx->stream, fa);
+            hipLaunchKernelGGL(k_chol_inv<false>, dim3(static_cast<unsigned>(small.size())), dim3(CI_THREADS), shmem, ctx->stream, fa);
             xh::launch_check("k_chol_inv");
         }
         if(!big.empty()) {
             double* scratch = nullptr;
             check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(big_max) * big.size() * sizeof(double), ctx->stream), "hipMallocAsync");
             FactorArgs fa{up.ptr<xh_factor_item>(o_big), G, X, d_sh, nullptr, scratch, big_max, 0, nullptr};
-            hipLaunchKernelGGL(k_chol_inv, dim3(static_cast<unsigned>(big.size())), dim3(CI_THREADS), 0, ctx->stream, fa);
+            hipLaunchKernelGGL(k_chol_inv<true>, dim3(static_cast<unsigned>(big.size())), dim3(CI_THREADS), 0, ctx->stream, fa);
             xh::launch_check("k_chol_inv");
             xh::dev_free(ctx, scratch);
         }
