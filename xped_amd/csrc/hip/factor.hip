@@ -312,7 +312,7 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
 // the time, and the plain loop is as fast.
 constexpr int CI_THREADS = 1024;
 constexpr int CI_LANES = 16;  // lanes per dot product in (c)
-constexpr int CI_PANEL = 4;   // pivots per trailing update of the panel variant
+constexpr int CI_PANEL = 8;   // pivots per trailing update of the panel variant
 constexpr int CI_MAX_N = 40000;  // packed indices of a block fit in 32 bits
 
 __host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * (n + 1) / 2 + 2 * n; }
