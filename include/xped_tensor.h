@@ -33,6 +33,8 @@ typedef struct xt_tensor xt_tensor;
 
 /* ---- runtime ------------------------------------------------------------------------- */
 int xt_rt_create(int device, xt_rt** out);
+/* Fails with XH_EINVAL (the runtime stays usable) while tensor or CTM handles made on it are
+ * alive: they refer to the runtime, so destroy them first.  Bases do not refer to a runtime. */
 int xt_rt_destroy(xt_rt* rt);
 const char* xt_last_error(const xt_rt* rt);
 xh_ctx* xt_rt_ctx(xt_rt* rt);

@@ -40,3 +40,26 @@ def test_structure_only_runtime_rejects_data_ops(rt_struct):
         t.download()
     with pytest.raises(XpedNoDevice):
         t.tSVD(2, 0.0)
+
+
+def test_runtime_destroy_is_refused_while_tensors_live(xlib):
+    import ctypes as C
+
+    import xped_amd.tensor as xt
+
+    rt = C.c_void_p()
+    assert xlib.xt_rt_create(-1, C.byref(rt)) == 0
+    b = xt.Qbasis(xt.U1).push_back(0, 2).push_back(1, 1)
+    arr = (C.c_void_p * 2)(b.h.value, b.h.value)
+    t1, t2 = C.c_void_p(), C.c_void_p()
+    assert xlib.xt_tensor_create(rt, 1, 1, arr, C.byref(t1)) == 0
+    assert xlib.xt_tensor_copy(t1, C.byref(t2)) == 0
+    assert xlib.xt_rt_destroy(rt) == _lib.XH_EINVAL
+    assert b"still alive" in xlib.xt_last_error(rt)
+    # the runtime is intact: structure queries on its tensors still work
+    r, c, n, s = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+    assert xlib.xt_tensor_shape(t2, C.byref(r), C.byref(c), C.byref(n), C.byref(s)) == 0 and s.value == 5
+    assert xlib.xt_tensor_destroy(t1) == 0
+    assert xlib.xt_rt_destroy(rt) == _lib.XH_EINVAL
+    assert xlib.xt_tensor_destroy(t2) == 0
+    assert xlib.xt_rt_destroy(rt) == 0

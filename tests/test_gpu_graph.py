@@ -80,3 +80,35 @@ def test_plan_creation_is_refused_under_capture(rt):
     want = A.download().reshape(-1)  # the stream works eagerly again
     assert want.size == A.size
     np.testing.assert_allclose(A.mul_into(B, out).download(), A.mul(B).download(), rtol=0, atol=0)
+
+
+def test_graph_keeps_its_plans_after_cache_eviction():
+    # an own runtime with an 8-plan bound, so that the captured product's plan is dropped from the
+    # cache while the graph is alive; the graph must keep the plan's device tables until it is
+    # destroyed, however much memory later plans take from the pool
+    rt2 = xt.Runtime(0)
+    rt2.set_max_plans(8)
+    A, B = _pair(rt2, (1536, 512, 96), (96, 64, 40), (1536, 512, 80), 41)
+    ref = A.mul(B).download()
+    out = A.mul(B).zeros_like()
+    rt2.synchronize()
+    rt2.graph_begin()
+    A.mul_into(B, out)
+    g = rt2.graph_end()
+    evicted0 = rt2.cache_sizes()[2]
+    keep = []
+    for i in range(24):  # new plans of other shapes push the captured one out of the cache
+        X, Y = _pair(rt2, (40 + i, 3), (17, 2), (30 + i, 5), 100 + i)
+        keep.append(X.mul(Y))
+    rt2.synchronize()
+    assert rt2.cache_sizes()[2] > evicted0
+    assert rt2.cache_sizes()[1] <= 8
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        out.upload(rng.uniform(-1, 1, out.size))
+        g.launch()
+        rt2.synchronize()
+        assert np.array_equal(out.download(), ref)
+    g.close()
+    # the plan was freed with the graph; the product plans anew and still agrees
+    assert np.array_equal(A.mul(B).download(), ref)

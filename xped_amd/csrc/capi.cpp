@@ -12,6 +12,7 @@
 struct xt_rt {
     xa::Runtime rt;
     std::string err;
+    long live = 0;  // tensor and CTM handles made on this runtime and not yet destroyed
 };
 struct xt_basis {
     xa::Qbasis b;
@@ -66,6 +67,7 @@ xt_tensor* wrap(xt_rt* owner, xa::DTensor t)
     auto* h = new xt_tensor();
     h->t = std::move(t);
     h->owner = owner;
+    if(owner) ++owner->live;
     return h;
 }
 
@@ -105,6 +107,11 @@ int xt_rt_create(int device, xt_rt** out)
 int xt_rt_destroy(xt_rt* rt)
 {
     if(!rt) return XH_OK;
+    if(rt->live > 0) {
+        // tensors and CTM objects keep raw pointers to the runtime: refuse rather than dangle
+        rt->err = g_err = "xt_rt_destroy: " + std::to_string(rt->live) + " tensor or CTM handle(s) of this runtime are still alive; destroy them first";
+        return XH_EINVAL;
+    }
     if(rt->rt.ctx) xh_synchronize(rt->rt.ctx);
     // the runtime destructor releases the cached plans; the context itself lives on until the
     // last tensor buffer that uses it is released
@@ -395,6 +402,7 @@ int xt_tensor_create(xt_rt* rt, int rank, int corank, const xt_basis* const* leg
 
 int xt_tensor_destroy(xt_tensor* t)
 {
+    if(t && t->owner) --t->owner->live;
     delete t;
     return XH_OK;
 }
@@ -763,12 +771,14 @@ int xt_ctm_create(xt_rt* rt, const xt_tensor* const* A, size_t chi, xt_ctm** out
         auto* c = new xt_ctm();
         c->ctm = std::make_unique<xa::CTM>(&rt->rt, a, chi);
         c->owner = rt;
+        ++rt->live;
         *out = c;
     });
 }
 
 int xt_ctm_destroy(xt_ctm* c)
 {
+    if(c && c->owner) --c->owner->live;
     delete c;
     return XH_OK;
 }

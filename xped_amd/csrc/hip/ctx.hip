@@ -125,6 +125,7 @@ extern "C" void* xh_stream(xh_ctx* ctx) { return ctx ? reinterpret_cast<void*>(c
 struct xh_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<xh::PlanPin> pins;  // plans whose device tables the graph's kernels read
 };
 
 extern "C" int xh_graph_capturing(xh_ctx* ctx)
@@ -143,11 +144,32 @@ void xh::refuse_under_capture(xh_ctx* ctx, const char* what)
     }
 }
 
+void xh::pin_for_capture(xh_ctx* ctx, PlanLife* plan, void (*release)(xh_ctx*, PlanLife*))
+{
+    if(!plan || !xh_graph_capturing(ctx)) return;
+    ++plan->pins;
+    ctx->capture_pins.push_back(PlanPin{plan, release});
+}
+
+void xh::unpin(xh_ctx* ctx, const PlanPin& p)
+{
+    if(--p.plan->pins == 0 && p.plan->dead) p.release(ctx, p.plan);
+}
+
+namespace {
+void unpin_all(xh_ctx* ctx, std::vector<xh::PlanPin>& pins)
+{
+    for(const auto& p : pins) xh::unpin(ctx, p);
+    pins.clear();
+}
+}  // namespace
+
 extern "C" int xh_graph_begin(xh_ctx* ctx)
 {
     return guard(ctx, [&] {
         if(xh_graph_capturing(ctx)) throw xh::Error(XH_EINVAL, "graph capture already active on this context");
         ctx->capture_failed = false;
+        unpin_all(ctx, ctx->capture_pins);
         check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     });
 }
@@ -160,7 +182,10 @@ extern "C" int xh_graph_end(xh_ctx* ctx, xh_graph** out)
         const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         const bool refused = ctx->capture_failed;
         ctx->capture_failed = false;
+        std::vector<xh::PlanPin> pins;
+        pins.swap(ctx->capture_pins);
         if(e != hipSuccess || !g || refused) {
+            unpin_all(ctx, pins);
             if(g) hipGraphDestroy(g);
             (void)hipGetLastError();
             if(refused) throw xh::Error(XH_EINVAL, "graph capture discarded: a captured call could not run under capture");
@@ -170,16 +195,19 @@ extern "C" int xh_graph_end(xh_ctx* ctx, xh_graph** out)
         gr->graph = g;
         const hipError_t ei = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
         if(ei != hipSuccess) {
+            unpin_all(ctx, pins);
             hipGraphDestroy(g);
             delete gr;
             throw xh::Error(XH_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
         }
         if(!out) {
+            unpin_all(ctx, pins);
             hipGraphExecDestroy(gr->exec);
             hipGraphDestroy(gr->graph);
             delete gr;
             throw xh::Error(XH_EINVAL, "xh_graph_end: null output");
         }
+        gr->pins = std::move(pins);
         *out = gr;
     });
 }
@@ -198,6 +226,9 @@ extern "C" int xh_graph_destroy(xh_ctx* ctx, xh_graph* g)
     return guard(ctx, [&] {
         if(g->exec) check(hipGraphExecDestroy(g->exec), "hipGraphExecDestroy");
         if(g->graph) check(hipGraphDestroy(g->graph), "hipGraphDestroy");
+        g->exec = nullptr;
+        g->graph = nullptr;
+        unpin_all(ctx, g->pins);  // frees the plans whose destroy was deferred for this graph
         delete g;
     });
 }

@@ -11,6 +11,20 @@
 
 #include "../../../include/xped_hip.h"
 
+namespace xh {
+// Lifetime of a plan that captured graphs may read: a destroy call on a plan still recorded by a
+// live graph is deferred until the last such graph is destroyed (xh_graph_destroy), because the
+// graph's kernel nodes hold the plan's device tables by address.
+struct PlanLife {
+    int pins = 0;       // live graphs (and the capture in progress) that recorded the plan
+    bool dead = false;  // destroy was requested while pinned
+};
+struct PlanPin {
+    PlanLife* plan;
+    void (*release)(xh_ctx*, PlanLife*);  // frees the plan for good
+};
+}  // namespace xh
+
 struct xh_ctx {
     int device = 0;
     int num_cu = 256;
@@ -45,6 +59,7 @@ struct xh_ctx {
     // set when a call refuses to run under graph capture (it would allocate or synchronise);
     // xh_graph_end then discards the capture
     bool capture_failed = false;
+    std::vector<xh::PlanPin> capture_pins;  // plans recorded by the capture in progress
     bool perm_timing = false;
     std::vector<hipEvent_t> perm_events;
     size_t perm_events_used = 0;
@@ -59,6 +74,10 @@ namespace xh {
 
 // partial results of one reduction launch (k_maxabs / k_dot) in the context scratch
 constexpr int RED_PARTS = 1024;
+
+// while the context stream is being captured: keep the plan alive until the graph is destroyed
+void pin_for_capture(xh_ctx* ctx, PlanLife* plan, void (*release)(xh_ctx*, PlanLife*));
+void unpin(xh_ctx* ctx, const PlanPin& p);
 
 // refuse work that cannot be recorded into a graph while the context stream is being captured
 void refuse_under_capture(xh_ctx* ctx, const char* what);

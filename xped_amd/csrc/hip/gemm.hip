@@ -358,7 +358,7 @@ struct TileList {
     int32_t part[NPART + 1] = {};
 };
 
-struct xh_gemm_plan {
+struct xh_gemm_plan : xh::PlanLife {
     xh::PackedUpload up;
     xh_gemm_item* d_items = nullptr;
     ItemExt* d_ext = nullptr;
@@ -516,13 +516,25 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
     return XH_OK;
 }
 
+namespace {
+void gemm_plan_release(xh_ctx* ctx, xh::PlanLife* p)
+{
+    auto* plan = static_cast<xh_gemm_plan*>(p);
+    plan->up.release(ctx);
+    plan->up_all.release(ctx);
+    delete plan;
+}
+}  // namespace
+
 extern "C" int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan)
 {
     if(!plan) return XH_OK;
     return guard(ctx, [&] {
-        plan->up.release(ctx);
-        plan->up_all.release(ctx);
-        delete plan;
+        if(plan->pins > 0) {
+            plan->dead = true;  // a live graph reads its tables: freed by the last xh_graph_destroy
+            return;
+        }
+        gemm_plan_release(ctx, plan);
     });
 }
 
@@ -574,6 +586,8 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, 
         double* ws = nullptr;
         if(plan->ws_doubles > 0) xh::refuse_under_capture(ctx, "a split-K GEMM plan (it allocates its partials)");
         if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
+        if(plan->dead) throw xh::Error(XH_EINVAL, "gemm plan used after xh_gemm_plan_destroy");
+        xh::pin_for_capture(ctx, plan, gemm_plan_release);
         GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
         // a multiple of NPART workgroups, at most per_cu per CU
         auto launch = [&](const TileList& L, void (*kernel)(GemmArgs), int per_cu, int threads, const char* name) {

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_transpose(PermDev p, con
 
 }  // namespace
 
-struct xh_permute_plan {
+struct xh_permute_plan : xh::PlanLife {
     PermDev dev{};
     xh::PackedUpload up;
     int64_t ngather_tiles = 0;
@@ -525,12 +525,24 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
     return XH_OK;
 }
 
+namespace {
+void permute_plan_release(xh_ctx* ctx, xh::PlanLife* p)
+{
+    auto* plan = static_cast<xh_permute_plan*>(p);
+    plan->up.release(ctx);
+    delete plan;
+}
+}  // namespace
+
 extern "C" int xh_permute_plan_destroy(xh_ctx* ctx, xh_permute_plan* plan)
 {
     if(!plan) return XH_OK;
     return guard(ctx, [&] {
-        plan->up.release(ctx);
-        delete plan;
+        if(plan->pins > 0) {
+            plan->dead = true;  // a live graph reads its tables: freed by the last xh_graph_destroy
+            return;
+        }
+        permute_plan_release(ctx, plan);
     });
 }
 
@@ -571,6 +583,8 @@ extern "C" int xh_permute(xh_ctx* ctx, const xh_permute_plan* plan, const double
 {
     return guard(ctx, [&] {
         if(!plan) throw xh::Error(XH_EINVAL, "null permute plan");
+        if(plan->dead) throw xh::Error(XH_EINVAL, "permute plan used after xh_permute_plan_destroy");
+        xh::pin_for_capture(ctx, const_cast<xh_permute_plan*>(plan), permute_plan_release);
         const bool timed = ctx->perm_timing && (plan->ngather_tiles > 0 || plan->dev.ntr_tiles > 0);
         if(timed) xh::perm_timer_begin(ctx);
         if(plan->ngather_tiles > 0) {
