@@ -97,6 +97,21 @@ int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, 
 int xh_gemm_plan_destroy(xh_ctx* ctx, xh_gemm_plan* plan);
 int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
 int xh_gemm_plan_flops(const xh_gemm_plan* plan, double* flops);
+/* Device bytes of the split-K partials one xh_gemm call of this plan needs (0: none). */
+int xh_gemm_plan_workspace(const xh_gemm_plan* plan, int64_t* bytes);
+/* xh_gemm that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs the same kernels as xh_gemm, with the same results bit for bit, but the
+ * split-K partials go to a fixed per-device array of XH_GEMM_SCRATCH_BYTES bytes.  The library
+ * holds that array on every GPU it runs on until the process ends.  A plan whose partials need more
+ * (xh_gemm_plan_workspace) fails with XH_EINVAL.  Every split-K chunk writes its whole partial
+ * block before the reduction reads it, so nothing in the array needs to start at zero.  All calls
+ * on one GPU share the array: calls that overlap in time on that GPU (several contexts or streams,
+ * or graphs replayed concurrently) race and give wrong results.  No timing events and no autotuning
+ * are used.  The 64 x 64 tile list that the FP32/BF16x3 modes (and the 64 x 64 FP64 tiling) need is
+ * built by a plan's first xh_gemm call in such a mode; without it the call fails with XH_EINVAL. */
+#define XH_GEMM_SCRATCH_BYTES ((size_t)32 << 20)
+int xh_gemm_noalloc(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
 /* GEMM tile autotuner (no counterpart in the reference, whose per-block BLAS calls have no tiles
  * to choose): with enable != 0, the first FP64 call of a plan that has 128 x 128 tiles, with
  * beta = 0, runs both tilings (128 x 128 + 64 x 64, and 64 x 64 only), timed with HIP events

@@ -83,6 +83,8 @@ _SIGS = [
     ("xh_gemm_plan_destroy", C.c_int, [_P, _P]),
     ("xh_gemm", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]),
     ("xh_gemm_plan_flops", C.c_int, [_P, _DP]),
+    ("xh_gemm_plan_workspace", C.c_int, [_P, C.POINTER(C.c_int64)]),
+    ("xh_gemm_noalloc", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]),
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
     ("xh_set_svd_backend", C.c_int, [_P, C.c_int]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),

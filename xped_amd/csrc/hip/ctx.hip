@@ -40,6 +40,7 @@ extern "C" int xh_create(int device, xh_ctx** out)
         check(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_scalar), 64 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
         check(hipMalloc(reinterpret_cast<void**>(&ctx->dev_scalar), (xh::RED_PARTS + 64) * sizeof(double)), "hipMalloc");
         check(hipMemset(ctx->dev_scalar + xh::RED_PARTS, 0, 64 * sizeof(double)), "hipMemset");  // normalisation status
+        xh::gemm_scratch_init(ctx);
     });
     if(rc != XH_OK) {
         delete ctx;

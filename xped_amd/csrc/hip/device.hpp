@@ -37,6 +37,7 @@ struct xh_ctx {
     void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
     int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
     int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 tiles for large sectors, 1 64x64 tiles only
+    double* gemm_scratch = nullptr;  // device address of this device's copy of the GEMM scratch array (xh_gemm_noalloc)
     // host staging buffers of asynchronous uploads, kept until the event after their copy fires
     std::vector<std::pair<hipEvent_t, std::vector<char>>> pending_host;
     // pinned host staging buffers for large plan uploads: free ones, and ones whose copy may still run
@@ -74,6 +75,9 @@ namespace xh {
 
 // partial results of one reduction launch (k_maxabs / k_dot) in the context scratch
 constexpr int RED_PARTS = 1024;
+
+// take the device address of the current device's copy of the GEMM scratch array (gemm.hip)
+void gemm_scratch_init(xh_ctx* ctx);
 
 // while the context stream is being captured: keep the plan alive until the graph is destroyed
 void pin_for_capture(xh_ctx* ctx, PlanLife* plan, void (*release)(xh_ctx*, PlanLife*));

@@ -18,7 +18,10 @@
 //             A*B ~ hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x16_bf16, FP32 accumulation
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <type_traits>
+#include <unordered_map>
 
 #include "device.hpp"
 
@@ -574,18 +577,55 @@ void build_all64(xh_ctx* ctx, xh_gemm_plan* plan)
 }
 }  // namespace
 
-extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const double* B, double* C, double alpha, double beta, int precision)
+// ---------------------------------------------------------------- allocation-free variant
+// Split-K partials of xh_gemm_noalloc live in this fixed array (one copy per device, held until
+// the process ends) instead of a per-call stream-ordered allocation.  Host code must never pass
+// `scratch` itself to a kernel: its device address is taken once per device with
+// hipGetSymbolAddress (gemm_scratch_init, called by xh_create) and kept in the context.
+static __device__ __align__(256) double scratch[XH_GEMM_SCRATCH_BYTES / sizeof(double)];
+
+namespace xh {
+void gemm_scratch_init(xh_ctx* ctx)
 {
-    return guard(ctx, [&] {
+    static std::mutex mu;
+    static std::unordered_map<int, double*> addr;  // device -> address of its copy of `scratch`
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(ctx->device);
+    if(it == addr.end()) {
+        void* p = nullptr;  // the caller has made ctx->device current
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(gemm scratch)");
+        if(!p) throw Error(XH_EHIP, "hipGetSymbolAddress(gemm scratch) returned a null address");
+        it = addr.emplace(ctx->device, static_cast<double*>(p)).first;
+    }
+    ctx->gemm_scratch = it->second;
+}
+}  // namespace xh
+
+namespace {
+// One grouped GEMM call.  ws: split-K partials (plan->ws_doubles, or null when the plan has no
+// split items).  With noalloc the call allocates nothing: no deferred tile list is built, and no
+// tuning or timing events are used.
+void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const double* B, double* C, double alpha, double beta, int precision, double* ws,
+              bool noalloc)
+{
+    {
         if(!cplan) throw xh::Error(XH_EINVAL, "null gemm plan");
         xh_gemm_plan* plan = const_cast<xh_gemm_plan*>(cplan);  // the deferred tile list is filled in on first use
+        if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
         if(!plan->any_tiles) return;
         const bool need_all = precision != XH_FP64 || ctx->gemm_tiles != 0;
-        if(need_all && !plan->all64_built) build_all64(ctx, plan);
-        if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
-        double* ws = nullptr;
-        if(plan->ws_doubles > 0) xh::refuse_under_capture(ctx, "a split-K GEMM plan (it allocates its partials)");
-        if(plan->ws_doubles > 0) check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
+        if(need_all && !plan->all64_built) {
+            if(noalloc)
+                throw xh::Error(XH_EINVAL, "xh_gemm_noalloc: the plan's 64 x 64 tile list (FP32/BF16x3 modes, 64 x 64 FP64 tiling) does not exist yet; "
+                                           "one xh_gemm call in this mode builds it");
+            build_all64(ctx, plan);
+        }
+        bool own_ws = false;
+        if(plan->ws_doubles > 0 && !ws) {
+            xh::refuse_under_capture(ctx, "a split-K GEMM plan (it allocates its partials; xh_gemm_noalloc does not)");
+            check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
+            own_ws = true;
+        }
         if(plan->dead) throw xh::Error(XH_EINVAL, "gemm plan used after xh_gemm_plan_destroy");
         xh::pin_for_capture(ctx, plan, gemm_plan_release);
         GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, A, B, C, ws, alpha, beta};
@@ -617,8 +657,8 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, 
             }
         };
         int tiles = ctx->gemm_tiles;
-        const bool tunable = ctx->gemm_autotune && precision == XH_FP64 && plan->big128.ntiles > 0 && !xh_graph_capturing(ctx);
-        const bool timed = ctx->gemm_timing;
+        const bool tunable = !noalloc && ctx->gemm_autotune && precision == XH_FP64 && plan->big128.ntiles > 0 && !xh_graph_capturing(ctx);
+        const bool timed = !noalloc && ctx->gemm_timing;
         if(timed) xh::gemm_timer_begin(ctx);
         if(tunable) {
             auto hit = ctx->gemm_tune.find(plan->sig);
@@ -646,15 +686,42 @@ extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, 
                 ++ctx->gemm_tune_runs;
                 if(tiles == 0) run(0);
                 if(timed) xh::gemm_timer_end(ctx, plan->flops);
-                if(ws) xh::dev_free(ctx, ws);
+                if(own_ws) xh::dev_free(ctx, ws);
                 return;
             }
         }
-        if(tiles != 0 && precision == XH_FP64 && !plan->all64_built) build_all64(ctx, plan);
+        if(tiles != 0 && !plan->all64_built) build_all64(ctx, plan);  // autotuned choice (never with noalloc)
         run(tiles);
         if(timed) xh::gemm_timer_end(ctx, plan->flops);
-        if(ws) xh::dev_free(ctx, ws);
+        if(own_ws) xh::dev_free(ctx, ws);
+    }
+}
+}  // namespace
+
+extern "C" int xh_gemm(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision)
+{
+    return guard(ctx, [&] { gemm_run(ctx, plan, A, B, C, alpha, beta, precision, nullptr, false); });
+}
+
+extern "C" int xh_gemm_noalloc(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta,
+                               int precision)
+{
+    return guard(ctx, [&] {
+        if(!plan) throw xh::Error(XH_EINVAL, "null gemm plan");
+        const size_t need = size_t(plan->ws_doubles) * sizeof(double);
+        if(need > XH_GEMM_SCRATCH_BYTES)
+            throw xh::Error(XH_EINVAL, "xh_gemm_noalloc: the plan's split-K partials need " + std::to_string(need) + " bytes, more than the " +
+                                           std::to_string(size_t(XH_GEMM_SCRATCH_BYTES)) + "-byte scratch array");
+        if(need > 0 && !ctx->gemm_scratch) throw xh::Error(XH_EHIP, "xh_gemm_noalloc: the scratch array's device address is not set");
+        gemm_run(ctx, plan, A, B, C, alpha, beta, precision, need > 0 ? ctx->gemm_scratch : nullptr, true);
     });
+}
+
+extern "C" int xh_gemm_plan_workspace(const xh_gemm_plan* plan, int64_t* bytes)
+{
+    if(!plan || !bytes) return XH_EINVAL;
+    *bytes = plan->ws_doubles * int64_t(sizeof(double));
+    return XH_OK;
 }
 
 extern "C" int xh_set_gemm_autotune(xh_ctx* ctx, int enable)
