@@ -16,7 +16,8 @@ import scipy.linalg
 from . import rng
 from .structure import Qbasis, build_fusion_tree, get_permutations, pair_permute, tree_permute
 from .stdsort import std_sort
-from .symmetry import Sym
+from .symmetry import SU2, Sym, phase
+from .wigner import wigner_3j
 
 PERMUTE_SKIP = 1.0e-10  # Tensor.cpp:127,193
 
@@ -306,29 +307,95 @@ def teigh(t: Tensor):
     return D, V
 
 
+def cgc(sym: Sym, q1, q2, q3) -> np.ndarray:
+    """Clebsch-Gordan tensor of one fusion vertex (SU2.cpp:70-89): indices (m1, m2, m3), each
+    running from the highest weight down (2m = q - 1 - 2i), entries 3j(q1 q2 q3; m1 m2 -m3) *
+    (-1)^((q1 - q2 + 2m3)/2) * sqrt(q3) with the sympy-pinned exact 3j symbols.  Abelian
+    components contribute the 1 x 1 x 1 unit."""
+    out = np.ones((1, 1, 1))
+    for c, k in enumerate(sym.kinds):
+        if k != SU2:
+            continue
+        d1, d2, d3 = q1[c], q2[c], q3[c]
+        v = np.zeros((d1, d2, d3))
+        for i1 in range(d1):
+            for i2 in range(d2):
+                for i3 in range(d3):
+                    m1, m2, m3 = d1 - 1 - 2 * i1, d2 - 1 - 2 * i2, d3 - 1 - 2 * i3
+                    v[i1, i2, i3] = wigner_3j(d1 - 1, d2 - 1, d3 - 1, m1, m2, -m3) * phase((d1 - d2 + m3) // 2) * math.sqrt(d3)
+        # product symmetries: the degeneracy index of a product irrep runs over the SU(2) factor
+        out = np.einsum("abc,ijk->aibjck", out, v).reshape(out.shape[0] * d1, out.shape[1] * d2, out.shape[2] * d3)
+    return out
+
+
+def one_j(sym: Sym, q) -> np.ndarray:
+    """The 1j tensor of a dual leg (SU2.cpp:41-61): entry (i, j) = (-1)^((q - 2m_i)/2) for
+    m_i = -m_j, truncating the halved exponent as C++ integer division does."""
+    out = np.ones((1, 1))
+    for c, k in enumerate(sym.kinds):
+        if k != SU2:
+            continue
+        d = q[c]
+        v = np.zeros((d, d))
+        for i in range(d):
+            for j in range(d):
+                im, jm = d - 1 - 2 * i, d - 1 - 2 * j
+                if im == -jm:
+                    v[i, j] = phase((d - im) // 2)
+        out = np.kron(out, v)
+    return out
+
+
+def tree_tensor(sym: Sym, tree) -> np.ndarray:
+    """FusionTree::asTensor (FusionTree.cpp:154-307): the fold of vertex tensors over the
+    intermediates, legs (m_1 .. m_r, m_coupled), with the 1j tensor applied to every dual leg."""
+    r = tree.rank
+    if r == 0:
+        return np.ones((1,))
+    if r == 1:
+        out = np.eye(sym.degeneracy(tree.unc[0]))
+    else:
+        outs = list(tree.inter) + [tree.coupled]
+        out = cgc(sym, tree.unc[0], tree.unc[1], outs[0])
+        for k in range(2, r):
+            out = np.tensordot(out, cgc(sym, outs[k - 2], tree.unc[k], outs[k - 1]), axes=([k], [0]))
+    for i in range(r):
+        if tree.dual[i]:
+            out = np.moveaxis(np.tensordot(one_j(sym, tree.unc[i]), out, axes=([1], [i])), 0, i)
+    return out
+
+
 def plain(t: Tensor) -> np.ndarray:
-    """Dense expansion for abelian symmetries (Tensor.cpp:1133-1310 with trivial Clebsch-Gordan
-    tensors): legs sorted ascending by q (conjugate-sorted legs keep the basis order)."""
-    assert not t.sym.non_abelian
+    """Dense expansion Tensor::plainTensor (Tensor.cpp:1133-1310).  Every leg is sorted; within a
+    leg, sector q starts at full_outer_num(q) and the index of (inner state a, weight m) is
+    a * degeneracy(q) + m.  A (domain tree, codomain tree) subblock of sector q contributes
+    sub[a.., b..] * sum_mc T1[m.., mc] T2[n.., mc] with T1, T2 the trees' Clebsch-Gordan
+    tensors (tree_tensor).  For abelian symmetries every tensor is the unit and this is the plain
+    placement of subblocks."""
     legs = t.unc_dom + t.unc_cod
     sorted_legs = []
     for b in legs:
         c = b.copy()
         c.sort()
         sorted_legs.append(c)
-    out = np.zeros([b.dim() for b in sorted_legs])
+    out = np.zeros([b.full_dim() for b in sorted_legs])
+    R, C = t.rank, t.corank
     for i, q in enumerate(t.sector):
-        r0 = 0
         for t1 in t.dom.tree(q):
-            c0 = 0
+            T1 = tree_tensor(t.sym, t1)
             for t2 in t.cod.tree(q):
                 sub = subblock(t, i, t1, t2)
-                idx = []
+                G = np.tensordot(T1, tree_tensor(t.sym, t2), axes=([R], [C]))  # (m.., n..)
+                full = np.multiply.outer(sub, G)
+                n = R + C
+                # interleave (a_i, m_i) -> a_i * deg_i + m_i on every leg
+                full = np.transpose(full, [x for i in range(n) for x in (i, n + i)]) if n else full
+                shape, idx = [], []
                 for leg, qq in enumerate(tuple(t1.unc) + tuple(t2.unc)):
                     b = sorted_legs[leg]
-                    o = b.data[b.find(qq)][1]
-                    idx.append(slice(o, o + b.inner_dim(qq)))
-                out[tuple(idx)] += sub
-                c0 += t2.dim
-            r0 += t1.dim
+                    deg = t.sym.degeneracy(qq)
+                    shape.append(b.inner_dim(qq) * deg)
+                    o = b.full_outer_num(qq)
+                    idx.append(slice(o, o + b.inner_dim(qq) * deg))
+                out[tuple(idx)] += full.reshape(shape) if n else full
     return out

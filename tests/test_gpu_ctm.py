@@ -104,3 +104,77 @@ def test_bounded_plan_cache_gives_the_same_steps():
         assert rt.cache_sizes()[0] <= (bound or 2048)
     assert energies[0] == energies[1]
     assert evicted[0] == 0 and evicted[1] > 0
+
+
+def _hip_bond_energies(c, oc):
+    """Bond energies of the HIP CTM's density matrices (the oracle's gate on HIP data; the
+    oracle CTM run alongside supplies the structures, which must match)."""
+    from oracle import ctm as octm_
+    from oracle import tensor as ot
+
+    out = []
+    for name, otab in (("rho_h", oc.rho_h), ("rho_v", oc.rho_v)):
+        for site in range(4):
+            r = c.tensor(name, site)
+            o = otab[site].copy()
+            assert structure_of(r) == oracle_structure(o)
+            o.data = r.download()
+            out.append(ot.trace(octm_.ct(octm_.tw(o, [0, 1]), [1, 2, 3, 4], oc._gate(o, 1.0, 1.0), [3, 4, 1, 2], 0)))
+    return out
+
+
+@pytest.mark.parametrize("orientation", ["h", "v"])
+@pytest.mark.parametrize("alpha,beta", [(-1.0, 1.0), (0.8, 0.3)], ids=["singlet", "magnetised"])
+def test_dimer_states_keep_exact_bond_energies(rt, orientation, alpha, beta):
+    # closed forms (helpers.dimer_bond_energies) through three HIP CTMRG steps
+    from .helpers import dimer_bond_energies, make_dimer
+
+    oA, pA = make_dimer(rt, orientation, alpha, beta)
+    oc = octm.CTM(oA, 16)
+    oc.init_from_A()
+    c = xt.CTM(rt, pA, 16).init()
+    want = np.array(dimer_bond_energies(orientation, alpha, beta))
+    for _ in range(3):
+        oc.step()
+        E = c.step()
+        assert abs(E - want.sum() / 4) <= 1e-12
+        assert np.abs(np.array(_hip_bond_energies(c, oc)) - want).max() <= 1e-12
+
+
+def test_truncating_d4_steps_match_oracle(rt):
+    # D = 4 over charges 0, 1, 2 and chi = 30: every projector truncates.  The double layer is
+    # symmetric under exchanging ket and bra, which maps the fused charge q to -q, so +q/-q sectors
+    # carry exactly degenerate singular values and a cut through such a pair keeps either partner
+    # (last-bit rounding decides, in the reference as here).  Compared: energies to 1e-10 relative,
+    # the kept singular values of every decomposition as a multiset to 1e-12 sigma_max, and the
+    # density matrices to 1e-9 in every step whose per-sector kept counts all agree.
+    virtual = [((0,), 2), ((1,), 1), ((2,), 1)]
+    oA, pA = make_ipeps(rt, virtual, configs.c4_physical(), 11)
+    chi = 30
+    oc = octm.CTM(oA, chi)
+    oc.init_from_A()
+    c = xt.CTM(rt, pA, chi).set_options(record_spectra=True).init()
+    same_counts = 0
+    for it in range(2):
+        oc.spectra = []
+        c.clear_spectra()
+        Eo = oc.step()
+        E = c.step()
+        assert abs(E - Eo) <= 1e-10 * abs(Eo), (it, E, Eo)
+        sp = c.spectra()
+        assert len(sp) == len(oc.spectra) == 16
+        all_same = True
+        for g, o in zip(sp, oc.spectra):
+            assert (g["site"], g["dir"]) == o[:2]
+            kept_g = {s["q"]: s["kept"] for s in g["sectors"] if s["kept"] > 0}
+            vals_g = sorted(x for s in g["sectors"] for x in np.sort(s["sv"])[::-1][: s["kept"]])
+            assert len(vals_g) == len(o[3]) == chi
+            np.testing.assert_allclose(vals_g, o[3], rtol=0, atol=1e-12 * max(vals_g))
+            all_same &= kept_g == {q[0]: n for q, n in o[2].items() if n > 0}
+        if all_same:
+            same_counts += 1
+            for site in range(4):
+                for name, otab in (("rho_h", oc.rho_h), ("rho_v", oc.rho_v)):
+                    r = c.tensor(name, site)
+                    assert structure_of(r) == oracle_structure(otab[site])
+                    np.testing.assert_allclose(r.download(), otab[site].data, rtol=0, atol=1e-9)
