@@ -128,6 +128,10 @@ int xh_set_gemm_tiles(xh_ctx* ctx, int mode);
  * the reference): enable (1) / disable (0) / keep (-1); returns the summed kernel time, the flops of
  * the timed launches and their count since the last reset (reset != 0 clears after reading). */
 int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* flops, int64_t* launches);
+/* one timed launch (0 .. launches - 1 since the last reset): its kernel time, the plan's transposes
+ * (trans[2]) and the (m, n, k) of its first `cap` items (mnk[3 * cap]); *nitems = all items
+ * (measurement, to replay a workload's launches; no counterpart in the reference) */
+int xh_gemm_launch_log(xh_ctx* ctx, int64_t launch, double* ms, int32_t* trans, int32_t* mnk, int64_t cap, int64_t* nitems);
 /* HIP-event timing of every xh_permute call, as xh_gemm_timing: summed kernel time, the bytes the
  * timed calls move (8 (elements read + written)) and their count (measurement; no counterpart in
  * the reference) */

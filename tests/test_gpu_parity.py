@@ -153,6 +153,44 @@ def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
         L.xh_free(ctx, d)
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_grouped_gemm_skinny_items(rt, ta, tb):
+    # items with a short output side (<= 32) and K <= 32 run on the memory-bound skinny kernel,
+    # mixed in one plan with items on the matrix-core tiles; FP64 to 1e-12
+    L = lib()
+    ctx = rt.ctx()
+    r = np.random.default_rng(9)
+    shapes = [(16, 5000, 9), (7000, 24, 13), (32, 300, 32), (31, 31, 31), (1, 2000, 1), (3000, 1, 5), (5, 7, 0), (200, 150, 40), (9, 9000, 8)]
+    items, ao, bo, co = [], 0, 0, 0
+    for m, n, k in shapes:
+        items.append({"a": ao, "b": bo, "c": co, "m": m, "n": n, "k": k})
+        ao += m * k
+        bo += k * n
+        co += m * n
+    A, B, C0 = r.uniform(-1, 1, ao), r.uniform(-1, 1, bo), r.uniform(-1, 1, co)
+    dA, dB, dC = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, n in ((dA, ao), (dB, bo), (dC, co)):
+        assert L.xh_alloc(ctx, 8 * max(n, 1), C.byref(d)) == 0
+    L.xh_upload(ctx, dA, A.ctypes.data, 8 * ao)
+    L.xh_upload(ctx, dB, B.ctypes.data, 8 * bo)
+    L.xh_upload(ctx, dC, C0.ctypes.data, 8 * co)
+    arr = (_lib.xh_gemm_item * len(items))(*[_lib.xh_gemm_item(it["a"], it["b"], it["c"], it["m"], it["n"], it["k"]) for it in items])
+    plan = C.c_void_p()
+    assert L.xh_gemm_plan_create(ctx, arr, len(items), ta, tb, C.byref(plan)) == 0
+    assert L.xh_gemm(ctx, plan, dA, dB, dC, 1.5, -0.5, _lib.XH_FP64) == 0
+    out = np.empty(co)
+    L.xh_download(ctx, out.ctypes.data, dC, 8 * co)
+    ref = _gemm_ref(A, B, items, ta, tb)
+    for it in items:
+        c, m, n = ref[it["c"]]
+        want = 1.5 * c - 0.5 * C0[it["c"]:it["c"] + m * n].reshape(n, m).T
+        got = out[it["c"]:it["c"] + m * n].reshape(n, m).T
+        assert np.abs(got - want).max(initial=0.0) <= 1e-12 * max(1.0, np.abs(want).max(initial=0.0)) * math.sqrt(max(it["k"], 1)), it
+    L.xh_gemm_plan_destroy(ctx, plan)
+    for d in (dA, dB, dC):
+        L.xh_free(ctx, d)
+
+
 def test_gemm_timing_and_marker(rt):
     L = lib()
     ctx = rt.ctx()

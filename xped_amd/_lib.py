@@ -107,6 +107,7 @@ _SIGS = [
     ("xh_add_at", C.c_int, [_P, _P, C.c_int64, C.c_double]),
     ("xh_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
     ("xh_trace_marker", C.c_int, [_P, C.c_int]),
+    ("xh_gemm_launch_log", C.c_int, [_P, C.c_int64, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]),
     ("xh_permute_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
     ("xh_set_gemm_tiles", C.c_int, [_P, C.c_int]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),

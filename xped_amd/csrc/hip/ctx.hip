@@ -234,6 +234,29 @@ extern "C" int xh_graph_destroy(xh_ctx* ctx, xh_graph* g)
     });
 }
 
+extern "C" int xh_gemm_launch_log(xh_ctx* ctx, int64_t launch, double* ms, int32_t* trans, int32_t* mnk, int64_t cap, int64_t* nitems)
+{
+    return guard(ctx, [&] {
+        const size_t n = ctx->gemm_events_used / 2;
+        if(launch < 0 || size_t(launch) >= n || size_t(launch) >= ctx->gemm_launch_items.size())
+            throw xh::Error(XH_EINVAL, "xh_gemm_launch_log: no such timed launch");
+        check(hipEventSynchronize(ctx->gemm_events[2 * launch + 1]), "hipEventSynchronize");
+        float t = 0.f;
+        check(hipEventElapsedTime(&t, ctx->gemm_events[2 * launch], ctx->gemm_events[2 * launch + 1]), "hipEventElapsedTime");
+        if(ms) *ms = t;
+        const auto& v = ctx->gemm_launch_items[launch];
+        const int64_t ni = (int64_t(v.size()) - 2) / 3;
+        if(trans) {
+            trans[0] = v[0];
+            trans[1] = v[1];
+        }
+        if(nitems) *nitems = ni;
+        if(mnk)
+            for(int64_t i = 0; i < std::min(ni, cap); ++i)
+                for(int j = 0; j < 3; ++j) mnk[3 * i + j] = v[2 + 3 * i + j];
+    });
+}
+
 extern "C" int xh_event_record(xh_ctx* ctx, int slot)
 {
     return guard(ctx, [&] {
@@ -260,6 +283,7 @@ extern "C" int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel
         if(reset) {
             ctx->gemm_events_used = 0;
             ctx->gemm_event_flops.clear();
+            ctx->gemm_launch_items.clear();
         }
         if(enable >= 0) ctx->gemm_timing = enable != 0;
     });

@@ -52,6 +52,8 @@ struct xh_ctx {
     std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
     size_t gemm_events_used = 0;
     std::vector<double> gemm_event_flops;
+    // per timed launch: transposes and the (m, n, k) of every item (xh_gemm_launch_log)
+    std::vector<std::vector<int32_t>> gemm_launch_items;
     // the same for permute launches (xh_permute_timing), with the bytes each one moves
     // GEMM tile autotuner: plan signature -> tiling (0 mixed, 1 64x64 only), and counters
     bool gemm_autotune = false;
@@ -130,8 +132,9 @@ inline void gemm_timer_begin(xh_ctx* ctx)
     }
     check(hipEventRecord(ctx->gemm_events[ctx->gemm_events_used], ctx->stream), "hipEventRecord");
 }
-inline void gemm_timer_end(xh_ctx* ctx, double flops)
+inline void gemm_timer_end(xh_ctx* ctx, double flops, std::vector<int32_t>&& items = {})
 {
+    ctx->gemm_launch_items.push_back(std::move(items));
     check(hipEventRecord(ctx->gemm_events[ctx->gemm_events_used + 1], ctx->stream), "hipEventRecord");
     ctx->gemm_events_used += 2;
     ctx->gemm_event_flops.push_back(flops);

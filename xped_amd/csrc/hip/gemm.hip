@@ -9,7 +9,10 @@
 // tiles of one sector run on one XCD and share its L2, and uneven sector sizes are balanced.
 // Launches with few output tiles (a sketch's Gram matrix: a small output with a long K) split K
 // into chunks that write FP64 partial products to a workspace; a second kernel sums the chunks
-// in a fixed order (deterministic) and applies alpha / beta.
+// in a fixed order (deterministic) and applies alpha / beta.  Items with one short output side
+// (<= 32) and a short K (<= 32), as the contractions of a site tensor's physical or virtual leg
+// with a large environment tensor give, are memory-bound and run on a third kernel (one thread per
+// index of the long side, FP64 FMA) in the same call.
 //
 // Precision modes (operands are always FP64 in memory, the output is FP64):
 //   XH_FP64   v_mfma_f64_16x16x4_f64, FP64 accumulation
@@ -125,46 +128,173 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedItem* __restrict
     }
 }
 
-// ---------------------------------------------------------------- FP64
-__global__ __launch_bounds__(G_THREADS) void k_gemm_f64(GemmArgs g)
+// ---------------------------------------------------------------- FP64, pipelined
+// BM x BN output tiles, 4 waves in 2 x 2, each owning (BM/2) x (BN/2) = MI x NI blocks of 16 x 16.
+// K steps of 16 run through two LDS stages: the operands of step t+1 are fetched into registers
+// while the matrix cores work on step t, then written to the other stage; one barrier per step.
+// Tiles that lie inside their item (all rows, columns and the whole K step in range) load without
+// bounds checks from per-thread offsets computed once per tile.
+template <int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_f64p(GemmArgs g)
 {
-    constexpr int BK = 16;
-    __shared__ double sA[BK][BM + 1];
-    __shared__ double sB[BK][BN + 1];
+    constexpr int NT = 64 * WGM * WGN;                 // threads
+    constexpr int WTM = BM / WGM, WTN = BN / WGN;       // wave tile
+    constexpr int MI = WTM / 16, NI = WTN / 16;
+    constexpr int RA = BM * BK / NT, RB = BN * BK / NT;  // operand elements per thread and step
+    __shared__ double sA[2][BK][BM + 1];
+    __shared__ double sB[2][BK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WGM, wn = wave / WGM;
     const TileRange tr = my_tiles(g);
     for(int t = tr.begin; t < tr.end; t += tr.stride) {
         const Tile tl = g.tiles[t];
         const xh_gemm_item it = g.items[tl.item];
         const ItemExt ex = g.ext[tl.item];
         const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
-        d4 acc[2][2];
-        for(int a = 0; a < 2; ++a)
-            for(int b = 0; b < 2; ++b) acc[a][b] = d4{0, 0, 0, 0};
-        for(int k0 = kbeg; k0 < kend; k0 += BK) {
-            for(int r = 0; r < 4; ++r) {
-                // lanes run along the contiguous direction of each operand in memory
-                const int idx = tid + G_THREADS * r;
-                const int i = g.ta ? idx >> 4 : idx & 63, kk = g.ta ? idx & 15 : idx >> 6;
-                sA[kk][i] = load_a(g, it, tl.m0 + i, k0 + kk, kend);
-                const int kb = g.tb ? idx >> 6 : idx & 15, j = g.tb ? idx & 63 : idx >> 4;
-                sB[kb][j] = load_b(g, it, k0 + kb, tl.n0 + j, kend);
+        const bool full_mn = tl.m0 + BM <= it.m && tl.n0 + BN <= it.n;
+        // element r of this thread: A (i, kk) and B (kk, j), lanes along each operand's contiguous direction
+        int ai[RA], ak[RA], bk[RB], bj[RB];
+        int64_t aoff[RA], boff[RB];  // offset of the element at k = 0 of the item
+        for(int r = 0; r < RA; ++r) {
+            const int idx = tid + NT * r;
+            ai[r] = g.ta ? idx / BK : idx % BM;
+            ak[r] = g.ta ? idx % BK : idx / BM;
+            aoff[r] = g.ta ? int64_t(tl.m0 + ai[r]) * it.k + ak[r] : int64_t(ak[r]) * it.m + tl.m0 + ai[r];
+        }
+        for(int r = 0; r < RB; ++r) {
+            const int idx = tid + NT * r;
+            bj[r] = g.tb ? idx % BN : idx / BK;
+            bk[r] = g.tb ? idx / BN : idx % BK;
+            boff[r] = g.tb ? int64_t(bk[r]) * it.n + tl.n0 + bj[r] : int64_t(tl.n0 + bj[r]) * it.k + bk[r];
+        }
+        const int64_t astep = g.ta ? 1 : it.m, bstep = g.tb ? it.n : 1;  // offset per unit of k
+        const double* A = g.A + it.a_off;
+        const double* B = g.B + it.b_off;
+        double ra[RA], rb[RB];
+        auto fetch = [&](int k0) {
+            if(full_mn && k0 + BK <= kend) {
+                for(int r = 0; r < RA; ++r) ra[r] = A[aoff[r] + int64_t(k0) * astep];
+                for(int r = 0; r < RB; ++r) rb[r] = B[boff[r] + int64_t(k0) * bstep];
+            } else {
+                for(int r = 0; r < RA; ++r)
+                    ra[r] = (tl.m0 + ai[r] < it.m && k0 + ak[r] < kend) ? A[aoff[r] + int64_t(k0) * astep] : 0.0;
+                for(int r = 0; r < RB; ++r)
+                    rb[r] = (tl.n0 + bj[r] < it.n && k0 + bk[r] < kend) ? B[boff[r] + int64_t(k0) * bstep] : 0.0;
             }
-            __syncthreads();
+        };
+        auto put = [&](int s) {
+            for(int r = 0; r < RA; ++r) sA[s][ak[r]][ai[r]] = ra[r];
+            for(int r = 0; r < RB; ++r) sB[s][bk[r]][bj[r]] = rb[r];
+        };
+        d4 acc[MI][NI];
+        for(int a = 0; a < MI; ++a)
+            for(int b = 0; b < NI; ++b) acc[a][b] = d4{0, 0, 0, 0};
+        __syncthreads();  // the previous tile's last reads of stage 0 are done
+        if(kbeg < kend) {
+            fetch(kbeg);
+            put(0);
+        }
+        __syncthreads();
+        int cur = 0;
+        for(int k0 = kbeg; k0 < kend; k0 += BK) {
+            const bool more = k0 + BK < kend;
+            if(more) fetch(k0 + BK);
             for(int kq = 0; kq < BK; kq += 4) {
                 const int kr = kq + (lane >> 4);
-                double av[2], bv[2];
-                for(int mi = 0; mi < 2; ++mi) av[mi] = sA[kr][wm * 32 + mi * 16 + (lane & 15)];
-                for(int ni = 0; ni < 2; ++ni) bv[ni] = sB[kr][wn * 32 + ni * 16 + (lane & 15)];
-                for(int mi = 0; mi < 2; ++mi)
-                    for(int ni = 0; ni < 2; ++ni) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+                double av[MI], bv[NI];
+                for(int mi = 0; mi < MI; ++mi) av[mi] = sA[cur][kr][wm * WTM + mi * 16 + (lane & 15)];
+                for(int ni = 0; ni < NI; ++ni) bv[ni] = sB[cur][kr][wn * WTN + ni * 16 + (lane & 15)];
+                for(int mi = 0; mi < MI; ++mi)
+                    for(int ni = 0; ni < NI; ++ni) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
             }
+            if(more) put(cur ^ 1);
             __syncthreads();
+            cur ^= 1;
         }
-        store_tile<true>(g, it, ex, tl, wm, wn, lane, acc);
+        const bool part = ex.split > 1;
+        double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
+        const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
+        for(int mi = 0; mi < MI; ++mi)
+            for(int ni = 0; ni < NI; ++ni)
+                for(int r = 0; r < 4; ++r) {
+                    const int row = tl.m0 + wm * WTM + mi * 16 + (lane >> 4) + 4 * r;
+                    const int col = tl.n0 + wn * WTN + ni * 16 + (lane & 15);
+                    if(row < it.m && col < it.n) {
+                        double* p = base + int64_t(col) * it.m + row;
+                        const double v = acc[mi][ni][r];
+                        *p = (beta == 0.0) ? alpha * v : alpha * v + beta * *p;
+                    }
+                }
     }
 }
+
+// ---------------------------------------------------------------- FP64, skinny items
+// Items whose output has one short side (S = min(m, n) <= SK_MAX) and a short K (<= SK_KMAX) are
+// bound by the bytes of the long operand and of C, not by arithmetic: a 16 x 16 matrix-core tile
+// would waste most of its work on them.  One thread per index of the long side (256 per
+// workgroup) reads that index's K entries of the long operand once, keeps the S sums in registers
+// with the small operand staged in LDS (read as broadcasts), and writes its S outputs.
+constexpr int SK_MAX = 32, SK_KMAX = 32, SK_THREADS = 256;
+
+template <int MS, int R>
+__global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny(GemmArgs g, const Tile* __restrict__ blocks, int nblocks)
+{
+    __shared__ double sS[SK_KMAX * MS];  // the small operand, [kk][s]
+    for(int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const Tile tl = blocks[b];
+        const xh_gemm_item it = g.items[tl.item];
+        const bool wide = it.m <= it.n;  // wide: A is the small operand (C has S rows); else B
+        const int S = wide ? it.m : it.n, K = it.k, Lng = wide ? it.n : it.m;
+        __syncthreads();  // the previous block's reads of sS are done
+        for(int e = threadIdx.x; e < K * MS; e += SK_THREADS) {
+            const int sx = e % MS, kk = e / MS;
+            double v = 0.0;
+            if(sx < S) {
+                if(wide)
+                    v = g.ta ? g.A[it.a_off + int64_t(sx) * K + kk] : g.A[it.a_off + int64_t(kk) * it.m + sx];
+                else
+                    v = g.tb ? g.B[it.b_off + int64_t(kk) * it.n + sx] : g.B[it.b_off + int64_t(sx) * K + kk];
+            }
+            sS[kk * MS + sx] = v;
+        }
+        __syncthreads();
+        // R long indices per thread, SK_THREADS apart, so that consecutive lanes stay consecutive; all
+        // K loads of a thread are issued before the first product, to keep many loads in flight
+#pragma unroll
+        for(int r = 0; r < R; ++r) {
+            const int L = tl.m0 + threadIdx.x + r * SK_THREADS;
+            if(L >= Lng) break;
+            double x[SK_KMAX];
+#pragma unroll
+            for(int kk = 0; kk < SK_KMAX; ++kk) {
+                x[kk] = 0.0;
+                if(kk < K) {
+                    if(wide)
+                        x[kk] = g.tb ? g.B[it.b_off + int64_t(kk) * it.n + L] : g.B[it.b_off + int64_t(L) * K + kk];
+                    else
+                        x[kk] = g.ta ? g.A[it.a_off + int64_t(L) * K + kk] : g.A[it.a_off + int64_t(kk) * it.m + L];
+                }
+            }
+            double acc[MS];
+#pragma unroll
+            for(int sx = 0; sx < MS; ++sx) acc[sx] = 0.0;
+#pragma unroll
+            for(int kk = 0; kk < SK_KMAX; ++kk)
+                if(kk < K)
+#pragma unroll
+                    for(int sx = 0; sx < MS; ++sx) acc[sx] = __builtin_fma(sS[kk * MS + sx], x[kk], acc[sx]);
+#pragma unroll
+            for(int sx = 0; sx < MS; ++sx)
+                if(sx < S) {
+                    double* p = g.C + it.c_off + (wide ? int64_t(L) * it.m + sx : int64_t(sx) * it.m + L);
+                    *p = (g.beta == 0.0) ? g.alpha * acc[sx] : g.alpha * acc[sx] + g.beta * *p;
+                }
+        }
+    }
+}
+
+// rows per thread of the skinny kernel for a register width ms
+constexpr int sk_rows(int ms) { return ms <= 16 ? 4 : 2; }
 
 // ---------------------------------------------------------------- FP64, 128 x 128 tiles
 // For items with both output sides >= 192: 8 waves in 2 x 4, each owning 64 x 32 (4 x 2 MFMA
@@ -368,6 +498,8 @@ struct xh_gemm_plan : xh::PlanLife {
     RedItem* d_red = nullptr;
     TileList all64;           // every item in 64 x 64 tiles (FP32 / BF16x3 modes), built on first use
     TileList big128, rest64;  // FP64: items with both sides >= BIG_MIN in 128 x 128 tiles, the others in 64 x 64
+    TileList skinny;          // FP64: blocks of 256 long-side indices of the skinny items (Tile.m0 = first index)
+    int sk_ms = 8;            // register width of the skinny kernel: 8, 16 or 32 >= every skinny item's short side
     bool all64_built = false, any_tiles = false;
     xh::PackedUpload up_all;
     std::vector<xh_gemm_item> h_items;  // host copies for the deferred list
@@ -397,6 +529,7 @@ void cut_parts(const std::vector<double>& work, int32_t part[NPART + 1])
 }
 
 bool is_big(const xh_gemm_item& it) { return it.m >= BIG_MIN && it.n >= BIG_MIN; }
+bool is_skinny(const xh_gemm_item& it) { return std::min(it.m, it.n) <= SK_MAX && it.k <= SK_KMAX; }
 }  // namespace
 
 extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64_t nitems, int trans_a, int trans_b, xh_gemm_plan** out)
@@ -414,7 +547,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         for(const auto& it : v) {
             if(it.m < 0 || it.n < 0 || it.k < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative extent");
             if(it.a_off < 0 || it.b_off < 0 || it.c_off < 0) throw xh::Error(XH_EINVAL, "gemm plan: negative offset");
-            if(it.m == 0 || it.n == 0) continue;
+            if(it.m == 0 || it.n == 0 || is_skinny(it)) continue;
             const int tm = is_big(it) ? BM2 : BM, tn = is_big(it) ? BN2 : BN;
             base_tiles += int64_t((it.m + tm - 1) / tm) * ((it.n + tn - 1) / tn);
         }
@@ -423,7 +556,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         const int64_t S = base_tiles > 0 && base_tiles < target ? (target + base_tiles - 1) / base_tiles : 1;
         std::vector<ItemExt> ext(v.size(), ItemExt{1, 0, 0});
         std::vector<RedItem> red;
-        std::vector<Tile> t_big, t_rest;
+        std::vector<Tile> t_big, t_rest, t_sk;
         std::vector<double> w_big, w_rest;
         for(int id : order) {
             const xh_gemm_item& it = v[id];
@@ -431,6 +564,10 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
             plan->flops += 2.0 * it.m * double(it.n) * it.k;
             ItemExt& e = ext[id];
             e.kc = std::max(it.k, 1);
+            if(is_skinny(it)) {  // never split: K <= SK_KMAX; blocks are made below
+                plan->any_tiles = true;
+                continue;
+            }
             if(S > 1 && it.k >= 256) {
                 int kc = static_cast<int>((it.k + S - 1) / S);
                 kc = std::max(128, (kc + 15) / 16 * 16);
@@ -464,7 +601,18 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
                 }
             }
         }
-        if(t_big.size() + t_rest.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
+        // skinny items: blocks of SK_THREADS * rows-per-thread long-side indices
+        for(const auto& it : v)
+            if(it.m > 0 && it.n > 0 && is_skinny(it))
+                while(plan->sk_ms < std::min(it.m, it.n)) plan->sk_ms *= 2;
+        for(int id : order) {
+            const xh_gemm_item& it = v[id];
+            if(it.m == 0 || it.n == 0 || !is_skinny(it)) continue;
+            const int step = SK_THREADS * sk_rows(plan->sk_ms);
+            for(int l0 = 0; l0 < std::max(it.m, it.n); l0 += step) t_sk.push_back(Tile{id, l0, 0, 0});
+        }
+        if(t_big.size() + t_rest.size() + t_sk.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gemm plan: too many tiles");
+        plan->skinny.ntiles = static_cast<int32_t>(t_sk.size());
         cut_parts(w_big, plan->big128.part);
         cut_parts(w_rest, plan->rest64.part);
         plan->big128.ntiles = static_cast<int32_t>(t_big.size());
@@ -492,7 +640,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         }
         xh::PackedUpload& up = plan->up;
         const size_t bytes = xh::PackedUpload::packed_bytes({v.size() * sizeof(xh_gemm_item), ext.size() * sizeof(ItemExt), t_big.size() * sizeof(Tile),
-                                                             t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem)});
+                                                             t_rest.size() * sizeof(Tile), red.size() * sizeof(RedItem), t_sk.size() * sizeof(Tile)});
         if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);  // large plans: straight into pinned staging
         auto put = [&up](const auto& vec) {
             using T = typename std::decay_t<decltype(vec)>::value_type;
@@ -500,7 +648,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
             if(!vec.empty()) std::memcpy(up.host_ptr<T>(o), vec.data(), vec.size() * sizeof(T));
             return o;
         };
-        const size_t oi = put(v), oe = put(ext), ob = put(t_big), orr_t = put(t_rest), orr = put(red);
+        const size_t oi = put(v), oe = put(ext), ob = put(t_big), orr_t = put(t_rest), orr = put(red), osk = put(t_sk);
         plan->up.upload(ctx);
         plan->d_items = plan->up.ptr<xh_gemm_item>(oi);
         plan->d_ext = plan->up.ptr<ItemExt>(oe);
@@ -509,6 +657,7 @@ extern "C" int xh_gemm_plan_create(xh_ctx* ctx, const xh_gemm_item* items, int64
         plan->h_order = std::move(order);
         plan->big128.d_tiles = plan->up.ptr<Tile>(ob);
         plan->rest64.d_tiles = plan->up.ptr<Tile>(orr_t);
+        plan->skinny.d_tiles = plan->up.ptr<Tile>(osk);
         plan->d_red = red.empty() ? nullptr : plan->up.ptr<RedItem>(orr);
     });
     if(rc != XH_OK) {
@@ -602,6 +751,19 @@ void gemm_scratch_init(xh_ctx* ctx)
 }  // namespace xh
 
 namespace {
+// the log record of a timed launch: ta, tb, then (m, n, k) per item
+std::vector<int32_t> launch_items(const xh_gemm_plan* plan)
+{
+    std::vector<int32_t> v{plan->ta, plan->tb};
+    v.reserve(2 + 3 * plan->h_items.size());
+    for(const auto& it : plan->h_items) {
+        v.push_back(it.m);
+        v.push_back(it.n);
+        v.push_back(it.k);
+    }
+    return v;
+}
+
 // One grouped GEMM call.  ws: split-K partials (plan->ws_doubles, or null when the plan has no
 // split items).  With noalloc the call allocates nothing: no deferred tile list is built, and no
 // tuning or timing events are used.
@@ -642,10 +804,16 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
         // tiles: 0 = FP64 128x128 tiles for the large items and 64x64 for the rest, 1 = 64x64 only
         auto run = [&](int tiles) {
             if(precision == XH_FP64 && tiles == 0) {
+                if(plan->skinny.ntiles > 0) {
+                    auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny<8, sk_rows(8)> : plan->sk_ms <= 16 ? k_gemm_skinny<16, sk_rows(16)> : k_gemm_skinny<32, sk_rows(32)>;
+                    const int grid = std::min<int>(plan->skinny.ntiles, ctx->num_cu * 8);
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, plan->skinny.d_tiles, plan->skinny.ntiles);
+                    xh::launch_check("k_gemm_skinny");
+                }
                 launch(plan->big128, k_gemm_f64_128, 2, G8_THREADS, "k_gemm_f64_128");
-                launch(plan->rest64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+                launch(plan->rest64, k_gemm_f64p<64, 64, 16, 2, 2>, 8, G_THREADS, "k_gemm_f64");
             } else if(precision == XH_FP64) {
-                launch(plan->all64, k_gemm_f64, 8, G_THREADS, "k_gemm_f64");
+                launch(plan->all64, k_gemm_f64p<64, 64, 16, 2, 2>, 8, G_THREADS, "k_gemm_f64");
             } else {
                 launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");
             }
@@ -685,14 +853,14 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
                 ctx->gemm_tune[plan->sig] = tiles;
                 ++ctx->gemm_tune_runs;
                 if(tiles == 0) run(0);
-                if(timed) xh::gemm_timer_end(ctx, plan->flops);
+                if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(plan));
                 if(own_ws) xh::dev_free(ctx, ws);
                 return;
             }
         }
         if(tiles != 0 && !plan->all64_built) build_all64(ctx, plan);  // autotuned choice (never with noalloc)
         run(tiles);
-        if(timed) xh::gemm_timer_end(ctx, plan->flops);
+        if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(plan));
         if(own_ws) xh::dev_free(ctx, ws);
     }
 }
