@@ -129,9 +129,13 @@ int xh_set_gemm_tiles(xh_ctx* ctx, int mode);
  * the timed launches and their count since the last reset (reset != 0 clears after reading). */
 int xh_gemm_timing(xh_ctx* ctx, int enable, int reset, double* kernel_ms, double* flops, int64_t* launches);
 /* one timed launch (0 .. launches - 1 since the last reset): its kernel time, the plan's transposes
- * (trans[2]) and the (m, n, k) of its first `cap` items (mnk[3 * cap]); *nitems = all items
- * (measurement, to replay a workload's launches; no counterpart in the reference) */
+ * and the caller's tag (trans[3] = {ta, tb, tag}) and the (m, n, k) of its first `cap` items
+ * (mnk[3 * cap]); *nitems = all items (measurement, to replay a workload's launches; no
+ * counterpart in the reference) */
 int xh_gemm_launch_log(xh_ctx* ctx, int64_t launch, double* ms, int32_t* trans, int32_t* mnk, int64_t cap, int64_t* nitems);
+/* the tag recorded with the following timed launches (the tensor layer tags its contraction
+ * products 1, everything else runs untagged, 0); measurement only */
+int xh_set_gemm_tag(xh_ctx* ctx, int tag);
 /* HIP-event timing of every xh_permute call, as xh_gemm_timing: summed kernel time, the bytes the
  * timed calls move (8 (elements read + written)) and their count (measurement; no counterpart in
  * the reference) */
@@ -152,8 +156,6 @@ typedef struct {
     int32_t m, n;
 } xh_factor_item;
 int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt);
-/* SVD backend for large blocks: 0 = own one-sided (block) Jacobi (default), 1 = rocSOLVER gesvd */
-int xh_set_svd_backend(xh_ctx* ctx, int backend);
 /* QR: Q (m x r) at u_off, R (r x n) at vt_off, Householder, R with the sign convention of
  * Eigen::HouseholderQR (diagonal may be negative). */
 int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* Q, double* R);
@@ -226,6 +228,17 @@ typedef struct {
 } xh_svd_bwd_item;
 int xh_svd_bwd(xh_ctx* ctx, const xh_svd_bwd_item* items, int64_t nitems, const double* U, const double* S, const double* V, const double* dU,
                const double* dS, const double* dV, double* dA);
+/* The adjoint of the per-sector thin QR A = Q R of xh_block_qr (no counterpart in the reference,
+ * whose AD tensor records no tQR): with k = min(m, n), Q m x k (ld m) at q_off, R k x n (ld k,
+ * upper triangular, nonzero diagonal) at r_off, the same offsets for dQ and dR, A and dA m x n
+ * (ld m) at a_off: dA += (dQ + Q copyltu(R dR^T - dQ^T Q)) R^-T for m >= n; for m < n the
+ * square part uses dQ + A[:, k:] dR[:, k:]^T and dA[:, k:] += Q dR[:, k:]. */
+typedef struct {
+    int64_t a_off, q_off, r_off;
+    int32_t m, n;
+} xh_qr_bwd_item;
+int xh_qr_bwd(xh_ctx* ctx, const xh_qr_bwd_item* items, int64_t nitems, const double* A, const double* Q, const double* R, const double* dQ,
+              const double* dR, double* dA);
 
 /* ---- learned rank predictor: a small FP32 network nin -> width -> width -> 1 (ReLU), MSE loss,
  * Adam; hand-written forward/backward kernels.  No counterpart in the reference (its SVD is always

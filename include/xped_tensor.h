@@ -126,7 +126,26 @@ int xt_contract(const xt_tensor* a, const int* a1, int n1, const xt_tensor* b, c
                 xt_tensor** out); /* Tensor.hpp:328-351 */
 int xt_get_permutations(const int* a1, int n1, int rank1, const int* a2, int n2, int rank2, int rankres, int* p1, int* shift1, int* p2, int* shift2,
                         int* pres, int* shiftres); /* Constfct.hpp:55-109 */
-/* Tensor::tSVD (Tensor.cpp:478-633).  *trunc_weight and *entropy are accumulated into (+=). */
+/* The SVspec argument of tSVD: std::map<qarray, VectorType> (Tensor.hpp:353-372), an ordered map
+ * from sector to its kept singular values (descending). */
+typedef struct xt_spec xt_spec;
+int xt_spec_create(xt_spec** out);
+int xt_spec_destroy(xt_spec* sp);
+int xt_spec_size(const xt_spec* sp, int* nsec);
+/* entry i in ascending q: its q (2 ints), the number of values and the first `cap` of them */
+int xt_spec_entry(const xt_spec* sp, int i, int* q, int* n, double* vals, int cap);
+/* Tensor::tSVD(maxKeep, eps_svd, truncWeight, entropy, SVspec, PRESERVE_MULTIPLETS, RETURN_SPEC)
+ * (Tensor.hpp:353-362, Tensor.cpp:478-633): *trunc_weight and *entropy are accumulated into
+ * (+=); with return_spec != 0 every kept sector's singular values are inserted into SVspec
+ * (an entry already present for a sector is kept, as std::map::insert does). */
+int xt_tsvd_spec(const xt_tensor* t, size_t max_keep, double eps_svd, double* trunc_weight, double* entropy, xt_spec* SVspec, int preserve_multiplets,
+                 int return_spec, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag);
+/* tQR reverse pass, accumulated into dA (structure of A): the QR adjoint per sector
+ * (xh_qr_bwd).  The reference's AD tensor records no tQR; this is the adjoint a recorded tQR
+ * needs.  Every sector needs the middle dimension min(rows, cols) and a full-rank R. */
+int xt_qr_bwd(const xt_tensor* A, const xt_tensor* Q, const xt_tensor* R, const xt_tensor* dQ, const xt_tensor* dR, xt_tensor* dA);
+/* Tensor::tSVD (Tensor.cpp:478-633) without SVspec (Tensor.hpp:364-372).  *trunc_weight and
+ * *entropy are accumulated into (+=). */
 int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_multiplets, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
             double* trunc_weight, double* entropy);
 /* Randomized truncated block SVD with the tSVD interface and truncation rule (Tensor.cpp:478-633,
@@ -174,7 +193,11 @@ int xt_trace_bwd(xt_tensor* dx, double g);
 /* maxNorm reverse pass (ADTensor.hpp:450-476): the first entry of largest modulus of x gets
  * dx += g * sign(x). */
 int xt_max_norm_bwd(const xt_tensor* x, xt_tensor* dx, double g);
-int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R);   /* Tensor.cpp:757-780 */
+/* Tensor::tQR(bool RETURN_LQ) (Tensor.hpp:390-391, Tensor.cpp:757-780): Q(uncoupledDomain |
+ * middle), R(middle | uncoupledCodomain), middle = the smaller coupled side.  return_lq != 0
+ * gives the reference's LQ mode: per sector Q^T and R^T in the same tensors, defined only when
+ * every sector block is square (other blocks: XH_EINVAL). */
+int xt_tqr(const xt_tensor* t, int return_lq, xt_tensor** Q, xt_tensor** R);
 int xt_teigh(const xt_tensor* t, xt_tensor** D, xt_tensor** V); /* Tensor.cpp:782-803 */
 int xt_adjoint(const xt_tensor* t, xt_tensor** out);            /* AdjointOp.hpp:25-69 */
 int xt_twist(const xt_tensor* t, int leg, xt_tensor** out);     /* Tensor.cpp:357-377 */

@@ -63,3 +63,28 @@ def test_runtime_destroy_is_refused_while_tensors_live(xlib):
     assert xlib.xt_rt_destroy(rt) == _lib.XH_EINVAL
     assert xlib.xt_tensor_destroy(t2) == 0
     assert xlib.xt_rt_destroy(rt) == 0
+
+
+def test_cpp_facade_compiles_and_runs_structure_queries(xlib):
+    """include/xped_amd.hpp (the C++ facade a maintainer binds Xped's Tensor to) compiled against
+    the headers and linked with the library: structure queries on a structure-only runtime match
+    the oracle, and a data operation without a device surfaces as std::runtime_error."""
+    from oracle import tensor as oten
+    from oracle.structure import Qbasis
+    from oracle.symmetry import U1, Sym
+
+    from .cpp_adapter import build_and_run
+
+    out = build_and_run("struct")
+    s = Sym(U1)
+    virt, phys = Qbasis(s), Qbasis(s)
+    for q in range(-2, 3):
+        virt.push_back((q,), 3 - abs(q))
+    phys.push_back((-1,), 1)
+    phys.push_back((1,), 1)
+    fused = virt.combine(phys)
+    assert out["virt_dim"] == virt.dim() and out["fused_dim"] == fused.dim() and out["fused_full"] == fused.full_dim()
+    assert out["fused"] == [[e[0][0], e[1], e[2]] for e in fused.data]
+    A = oten.Tensor(s, [virt, phys], [virt])
+    assert out["A"] == [[q[0], o, r, c] for q, o, r, c in A.structure()]
+    assert out["error_mapped"] is True

@@ -23,6 +23,7 @@ import xped_amd.tensor as xt
 from oracle import ad as oad
 from oracle import tensor as oten
 from oracle.symmetry import Sym
+from xped_amd._lib import XpedError
 from xped_amd.ad import ADTensor, Tape
 
 from .helpers import KINDS, both_basis, rand_entries, structure_of
@@ -231,3 +232,78 @@ def _svd_values(t):
 def test_tsvd_tape_vs_finite_differences(rt, pcode, f):
     t = _random_t(rt, pcode, 32, nsec=3, maxdim=3)
     check_ad_vs_fd(rt, f, t, rtol=2e-6)
+
+
+# ------------------------------------------------------------------ tQR: LQ mode, adjoint
+def _qr_t(rt, pcode, shape, seed):
+    """Rank (1, 1) tensor whose sectors are all tall (rows >= cols) or all wide, so that the
+    middle basis of tQR has min(rows, cols) in every sector."""
+    qs = [(-1,), (0,), (1,)] if pcode != xt.SU2 else [(1,), (2,), (3,)]
+    dims = {"tall": [(5, 3), (4, 4), (6, 2)], "wide": [(2, 5), (3, 3), (1, 4)], "square": [(3, 3), (4, 4), (2, 2)]}[shape]
+    _, t = sorted_both(rt, pcode, [[(q, d[0]) for q, d in zip(qs, dims)], [(q, d[1]) for q, d in zip(qs, dims)]], 1, seed)
+    return t
+
+
+def _blocks(t):
+    d = t.download()
+    return {q: d[o:o + r * c].reshape(c, r).T for q, o, r, c in t.sectors()}
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.SU2], ids=["U1", "SU2"])
+def test_tqr_lq_mode_returns_the_transposed_factors(rt, pcode):
+    t = _qr_t(rt, pcode, "square", 41)
+    Q, R = t.tQR()
+    Ql, Rl = t.tQR(RETURN_LQ=True)
+    assert structure_of(Ql) == structure_of(Q) and structure_of(Rl) == structure_of(R)
+    bq, br, bql, brl, bt = _blocks(Q), _blocks(R), _blocks(Ql), _blocks(Rl), _blocks(t)
+    for q in bq:
+        np.testing.assert_array_equal(bql[q], bq[q].T)
+        np.testing.assert_array_equal(brl[q], br[q].T)
+        # Q^T and R^T are the LQ factors of A^T: A^T = R^T Q^T with R^T lower triangular
+        np.testing.assert_allclose(brl[q] @ bql[q], bt[q].T, rtol=0, atol=1e-12 * np.abs(bt[q]).max())
+        assert np.abs(np.triu(brl[q], 1)).max(initial=0.0) == 0.0
+    with pytest.raises(XpedError):
+        _qr_t(rt, pcode, "tall", 42).tQR(RETURN_LQ=True)
+
+
+def _qr_f(which):
+    def f(x):
+        Q, R = x.tQR()
+        rng = np.random.default_rng(5)
+        wq = ADTensor(x.tape, Q.val.zeros_like().upload(rng.standard_normal(Q.val.size)))
+        wr = ADTensor(x.tape, R.val.zeros_like().upload(rng.standard_normal(R.val.size)))
+        fq = wq.adjoint().mul(Q).trace()
+        fr = wr.adjoint().mul(R).trace()
+        return {"Q": fq, "R": fr, "QR": fq + fr}[which]
+    return f
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.SU2], ids=["U1", "SU2"])
+@pytest.mark.parametrize("shape", ["tall", "wide", "square"])
+@pytest.mark.parametrize("which", ["Q", "R", "QR"])
+def test_tqr_tape_vs_finite_differences(rt, pcode, shape, which):
+    # the QR adjoint (xt_qr_bwd) through the tape: <Wq, Q(A)> + <Wr, R(A)> against central differences
+    t = _qr_t(rt, pcode, shape, 43)
+    check_ad_vs_fd(rt, _qr_f(which), t, rtol=2e-6)
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.SU2], ids=["U1", "SU2"])
+def test_tsvd_svspec(rt, pcode):
+    # tSVD's SVspec (Tensor.cpp:620-624): the kept singular values per sector, inserted into the map
+    r = random.Random(17)
+    legs = [rand_entries(pcode, r, 4, 5) for _ in range(3)]
+    o, t = sorted_both(rt, pcode, legs, 2, 19)
+    U, S, V, tw, en, spec = t.tSVD_spec(12, 1e-10, PRESERVE_MULTIPLETS=False)
+    bs = _blocks(S)
+    assert set(spec) == set(bs)
+    for q, b in bs.items():
+        np.testing.assert_array_equal(spec[q], np.diag(b))
+    oU, oS, oV, otw, oent, kept = oten.tsvd(o, 12, 1e-10, False)
+    for q, v in kept.items():
+        np.testing.assert_allclose(spec[tuple(q)], v, rtol=0, atol=1e-12 * oS.data.max())
+    # std::map::insert keeps an existing entry
+    q0 = next(iter(spec))
+    again = t.tSVD_spec(12, 1e-10, SVspec={q0: np.array([-1.0])}, PRESERVE_MULTIPLETS=False)[5]
+    assert list(again[q0]) == [-1.0]
+    # RETURN_SPEC = false leaves the map alone
+    assert t.tSVD_spec(12, 1e-10, SVspec={}, PRESERVE_MULTIPLETS=False, RETURN_SPEC=False)[5] == {}

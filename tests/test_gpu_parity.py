@@ -399,3 +399,44 @@ def test_axpby_with_zero_b_does_not_read_the_output(rt):
     L.xh_free(ctx, dx)
     L.xh_free(ctx, dy)
     assert np.array_equal(y, 2.5 * x) and np.signbit(y[7])
+
+
+def test_cpp_facade_runs_the_hot_path_on_the_gpu(rt):
+    """include/xped_amd.hpp driven from C++ (tests/cpp/adapter_test.cpp): setRandom, contract,
+    permute, tSVD with SVspec and tQR on the device, against the oracle on the same seeds."""
+    from oracle.structure import Qbasis
+    from oracle.symmetry import U1, Sym
+
+    from .cpp_adapter import build_and_run
+
+    out = build_and_run("gpu")
+    s = Sym(U1)
+    virt, phys = Qbasis(s), Qbasis(s)
+    for q in range(-2, 3):
+        virt.push_back((q,), 3 - abs(q))
+    phys.push_back((-1,), 1)
+    phys.push_back((1,), 1)
+    A = oten.Tensor(s, [virt, phys], [virt]).set_random(1)
+    B = oten.Tensor(s, [virt, phys], [virt]).set_random(2)
+    assert np.array_equal(np.array(out["A_data"]), A.data) and np.array_equal(np.array(out["B_data"]), B.data)
+    Cc = oten.contract(A, [-1, -2, 1], B, [1, -3, -4], 2)
+    assert out["C"] == [[q[0], o, r, c] for q, o, r, c in Cc.structure()]
+    np.testing.assert_allclose(out["C_data"], Cc.data, rtol=0, atol=1e-12 * np.abs(Cc.data).max())
+    P = oten.permute(Cc, -1, [0, 3, 1, 2])
+    assert out["P"] == [[q[0], o, r, c] for q, o, r, c in P.structure()]
+    np.testing.assert_allclose(out["P_data"], P.data, rtol=0, atol=1e-12 * np.abs(P.data).max())
+    U, S, V, tw, ent, kept = oten.tsvd(Cc, 8, 0.0, False)
+    assert out["S"] == [[q[0], o, r, c] for q, o, r, c in S.structure()]
+    assert abs(out["truncWeight"] - tw) <= 1e-12 * max(1.0, tw) and abs(out["entropy"] - ent) <= 1e-12 * max(1.0, abs(ent))
+    assert set(out["SVspec"]) == {str(q[0]) for q in kept}
+    for q, v in kept.items():
+        np.testing.assert_allclose(out["SVspec"][str(q[0])], v, rtol=0, atol=1e-12 * S.data.max())
+    # C's blocks are rank-deficient (rank <= the dimension of the contracted sector), so the
+    # trailing columns of Q are any orthonormal completion: compare the factorisation instead
+    Q, R = oten.tqr(Cc)
+    Q.data, R.data = np.array(out["Q_data"]), np.array(out["R_data"])
+    for i, q in enumerate(Cc.sector):
+        qb, rb = Q.block(Q.find(q)), R.block(R.find(q))
+        np.testing.assert_allclose(qb @ rb, Cc.block(i), rtol=0, atol=1e-12 * np.abs(Cc.data).max())
+        np.testing.assert_allclose(qb.T @ qb, np.eye(qb.shape[1]), rtol=0, atol=1e-12)
+        assert np.abs(np.tril(rb, -1)).max(initial=0.0) == 0.0

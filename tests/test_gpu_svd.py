@@ -35,8 +35,7 @@ def _blocks():
     return out
 
 
-@pytest.mark.parametrize("backend", [0, 1], ids=["jacobi", "rocsolver"])
-def test_grouped_block_svd_vs_lapack(rt, backend):
+def test_grouped_block_svd_vs_lapack(rt):
     L = lib()
     ctx = rt.ctx()
     blocks = _blocks()
@@ -56,11 +55,7 @@ def test_grouped_block_svd_vs_lapack(rt, backend):
     dA, dU, dS, dV = ptrs
     L.xh_upload(ctx, dA, Aall.ctypes.data, 8 * ao)
     arr = (_lib.xh_factor_item * len(items))(*items)
-    assert L.xh_set_svd_backend(ctx, backend) == 0
-    try:
-        assert L.xh_block_svd(ctx, arr, len(items), dA, dU, dS, dV) == 0, L.xh_last_error(ctx)
-    finally:
-        L.xh_set_svd_backend(ctx, 0)
+    assert L.xh_block_svd(ctx, arr, len(items), dA, dU, dS, dV) == 0, L.xh_last_error(ctx)
     U, S, V = np.empty(uo), np.empty(so), np.empty(vo)
     L.xh_download(ctx, U.ctypes.data, dU, 8 * uo)
     L.xh_download(ctx, S.ctypes.data, dS, 8 * so)

@@ -121,3 +121,31 @@ def test_fermionic_split_permutes_are_graded_shuffles(kinds):
             # a dropped sign would make the plain shuffle agree instead
             got = oten.plain(oten.permute(t, 0, [1, 0, 2, 3]))
             assert not np.array_equal(got, np.transpose(P, [1, 0, 2, 3]))
+
+
+def test_c2_reduced_contraction_orders_agree_in_the_oracle():
+    """The identity the workload-size C2 GPU test relies on, on the oracle: absorbing a site into
+    T in two contraction orders gives the same tensor."""
+    from xped_amd import configs
+    from xped_amd import su2_layer as L
+
+    from .c2_ops import OracleOps
+
+    s = Sym(SU2)
+    ops = OracleOps(s)
+    v, p, env = configs.c2_small_virtual(), configs.c2_physical(), configs.c2_env(9)
+    A = oten.Tensor(s, [obasis(s, v), obasis(s, v)], [obasis(s, v), obasis(s, v), obasis(s, p)]).set_random(7, 1)
+    T = oten.Tensor(s, [obasis(s, env)], [obasis(s, env), obasis(s, v), obasis(s, v)]).set_random(8, 1)
+    Ad = L.adag_of(ops, A)
+    X, Y = L.absorb(ops, T, A, Ad), L.absorb_other_order(ops, T, A, Ad)
+    assert X.sector == Y.sector
+    assert np.abs(X.data - Y.data).max() <= 1e-12 * np.abs(X.data).max()
+    # the double layer's dense expansion equals the dense ket-bra contraction joined by the dense
+    # expansions of the fusion isometries (spin SU(2) is bosonic, so the twists are identities)
+    a = L.contract_aadag(ops, A, Ad)
+    l, t, r, b = A.unc_dom[0], A.unc_dom[1], A.unc_cod[0], A.unc_cod[1]
+    lA, tA, rA, bA = Ad.unc_dom[0], Ad.unc_dom[1], Ad.unc_cod[0], Ad.unc_cod[1]
+    F = [oten.plain(ops.identity([ops.fuse(l, lA)], [l, lA])), oten.plain(ops.identity([ops.fuse(t, tA)], [t, tA])),
+         oten.plain(ops.identity([r, rA], [ops.fuse(r, rA)])), oten.plain(ops.identity([b, bA], [ops.fuse(b, bA)]))]
+    want = np.einsum("ltrbs,LTsRB,XlL,YtT,rRZ,bBW->XYZW", oten.plain(A), oten.plain(Ad), *F)
+    np.testing.assert_allclose(oten.plain(a), want, rtol=0, atol=1e-13 * np.abs(want).max())

@@ -86,7 +86,6 @@ _SIGS = [
     ("xh_gemm_plan_workspace", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("xh_gemm_noalloc", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]),
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
-    ("xh_set_svd_backend", C.c_int, [_P, C.c_int]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_eigh", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_chol_inv", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _DP]),
@@ -108,6 +107,7 @@ _SIGS = [
     ("xh_gemm_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
     ("xh_trace_marker", C.c_int, [_P, C.c_int]),
     ("xh_gemm_launch_log", C.c_int, [_P, C.c_int64, _DP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]),
+    ("xh_set_gemm_tag", C.c_int, [_P, C.c_int]),
     ("xh_permute_timing", C.c_int, [_P, C.c_int, C.c_int, _DP, _DP, C.POINTER(C.c_int64)]),
     ("xh_set_gemm_tiles", C.c_int, [_P, C.c_int]),
     ("xh_event_record", C.c_int, [_P, C.c_int]),
@@ -182,7 +182,14 @@ _SIGS = [
     ("xt_diag_bwd", C.c_int, [_P, _P, _P, C.c_int]),
     ("xt_trace_bwd", C.c_int, [_P, C.c_double]),
     ("xt_max_norm_bwd", C.c_int, [_P, _P, C.c_double]),
-    ("xt_tqr", C.c_int, [_P, _PP, _PP]),
+    ("xt_tqr", C.c_int, [_P, C.c_int, _PP, _PP]),
+    ("xt_qr_bwd", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("xt_spec_create", C.c_int, [_PP]),
+    ("xt_spec_destroy", C.c_int, [_P]),
+    ("xt_spec_size", C.c_int, [_P, _IP]),
+    ("xt_spec_entry", C.c_int, [_P, C.c_int, _IP, _IP, _DP, C.c_int]),
+    ("xt_tsvd_spec", C.c_int, [_P, _SZ, C.c_double, _DP, _DP, _P, C.c_int, C.c_int, _PP, _PP, _PP]),
+    ("xh_qr_bwd", C.c_int, [_P, _P, C.c_int64, _P, _P, _P, _P, _P, _P]),
     ("xt_teigh", C.c_int, [_P, _PP, _PP]),
     ("xt_adjoint", C.c_int, [_P, _PP]),
     ("xt_twist", C.c_int, [_P, C.c_int, _PP]),

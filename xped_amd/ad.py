@@ -16,14 +16,16 @@ Operations and their reverse passes (ADTensor.hpp line numbers):
 * ``tSVD`` = ``full_svd_impl`` (:237-329) + ``cutoff_matrices`` (:331-374);
 * ``norm`` (:492-499, ``sqrt((t * t.adjoint()).trace())``), ``maxNorm`` (:450-476),
   ``trace`` (:531-548), ``diag_sqrt`` / ``diag_inv`` (:560-590), ``twist`` (:592-602);
-* ``coeff`` (the scalar entry used by the reference's tests, ADTensor.hpp:152-169).
+* ``coeff`` (the scalar entry used by the reference's tests, ADTensor.hpp:152-169);
+* ``tQR``, which the reference does not record (its AD tensor has no tQR); its reverse pass is the
+  standard QR adjoint (``xt_qr_bwd``).
 """
 from __future__ import annotations
 
 import math
 from typing import Callable, List, Optional, Sequence
 
-from .tensor import Tensor, cutoff_bwd, diag_bwd, get_permutations, max_norm_bwd, mul_bwd, svd_bwd, trace_bwd
+from .tensor import Tensor, cutoff_bwd, diag_bwd, get_permutations, max_norm_bwd, mul_bwd, qr_bwd, svd_bwd, trace_bwd
 
 
 class Tape:
@@ -48,6 +50,16 @@ class Var:
 
     def __init__(self, tape: Tape, val: float):
         self.tape, self.val, self.adj = tape, float(val), 0.0
+
+    def __add__(self, other: "Var") -> "Var":
+        out = Var(self.tape, self.val + other.val)
+
+        def back(a=self, b=other, o=out):
+            a.adj += o.adj
+            b.adj += o.adj
+
+        self.tape.record(back)
+        return out
 
     def sqrt(self) -> "Var":
         out = Var(self.tape, math.sqrt(self.val))
@@ -159,6 +171,17 @@ class ADTensor:
         """ADTensor::tSVD (ADTensor.hpp:221-233): full_svd_impl then cutoff_matrices."""
         U, S, V = self.full_svd_impl()
         return self.cutoff_matrices(U, S, V, maxKeep, eps_svd, PRESERVE_MULTIPLETS)
+
+    # ---- QR (no AD counterpart in the reference: xt_qr_bwd is the adjoint a recorded tQR needs)
+    def tQR(self):
+        Q, R = self.val.tQR()
+        aQ, aR = self._new(Q), self._new(R)
+
+        def back(x=self, q=aQ, r=aR):
+            qr_bwd(x.val, q.val, r.val, q.adj, r.adj, x.adj)
+
+        self.tape.record(back)
+        return aQ, aR
 
     # ---- scalars
     def trace(self) -> Var:

@@ -69,7 +69,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
                     print(out)
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for _, o in steps], "-o", str(tmp),
-          "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
+          "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

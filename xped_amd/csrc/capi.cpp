@@ -1,6 +1,7 @@
 // C ABI of the tensor layer (include/xped_tensor.h) over the C++ structure engine and device layer.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 
@@ -20,6 +21,9 @@ struct xt_basis {
 struct xt_tensor {
     xa::DTensor t;
     xt_rt* owner = nullptr;
+};
+struct xt_spec {
+    std::map<xa::Q, std::vector<double>> m;  // ordered by q, as std::map<qarray, VectorType>
 };
 
 namespace {
@@ -556,6 +560,58 @@ int xt_tsvd(const xt_tensor* t, size_t max_keep, double eps_svd, int preserve_mu
     });
 }
 
+int xt_spec_create(xt_spec** out)
+{
+    *out = new xt_spec();
+    return XH_OK;
+}
+
+int xt_spec_destroy(xt_spec* sp)
+{
+    delete sp;
+    return XH_OK;
+}
+
+int xt_spec_size(const xt_spec* sp, int* nsec)
+{
+    if(!sp || !nsec) return XH_EINVAL;
+    *nsec = static_cast<int>(sp->m.size());
+    return XH_OK;
+}
+
+int xt_spec_entry(const xt_spec* sp, int i, int* q, int* n, double* vals, int cap)
+{
+    if(!sp || i < 0 || i >= static_cast<int>(sp->m.size())) return XH_EINVAL;
+    auto it = sp->m.begin();
+    std::advance(it, i);
+    if(q) q_to(it->first, q);
+    if(n) *n = static_cast<int>(it->second.size());
+    if(vals)
+        for(int k = 0; k < std::min(cap, static_cast<int>(it->second.size())); ++k) vals[k] = it->second[k];
+    return XH_OK;
+}
+
+int xt_tsvd_spec(const xt_tensor* t, size_t max_keep, double eps_svd, double* trunc_weight, double* entropy, xt_spec* SVspec, int preserve_multiplets,
+                 int return_spec, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        if(return_spec && !SVspec) throw xa::Error(XH_EINVAL, "tSVD: RETURN_SPEC needs an SVspec map");
+        xa::SVDResult r = xa::tsvd(t->t, max_keep, eps_svd, preserve_multiplets != 0, trunc_weight ? *trunc_weight : 0.0, entropy ? *entropy : 0.0);
+        *U = wrap(t->owner, r.U);
+        *S = wrap(t->owner, r.S);
+        *Vdag = wrap(t->owner, r.Vdag);
+        if(trunc_weight) *trunc_weight = r.trunc_weight;
+        if(entropy) *entropy = r.entropy;
+        if(return_spec)
+            for(const auto& [q, v] : r.spec) SVspec->m.insert({q, v});  // std::map::insert keeps an existing entry, as the reference's SVspec.insert
+    });
+}
+
+int xt_qr_bwd(const xt_tensor* A, const xt_tensor* Q, const xt_tensor* R, const xt_tensor* dQ, const xt_tensor* dR, xt_tensor* dA)
+{
+    return run(A ? A->owner : nullptr, [&] { xa::qr_bwd(A->t, Q->t, R->t, dQ->t, dR->t, dA->t); });
+}
+
 int xt_tsvd_randomized(const xt_tensor* t, size_t max_keep, double eps_svd, const int* qs, const int* ranks, int nsec, int oversample,
                        double rank_factor, int power_iters, int sketch_precision, uint64_t seed, xt_tensor** U, xt_tensor** S, xt_tensor** Vdag,
                        double* trunc_weight, int* safe, double* residual_ratio)
@@ -674,10 +730,10 @@ int xt_max_norm_bwd(const xt_tensor* x, xt_tensor* dx, double g)
     });
 }
 
-int xt_tqr(const xt_tensor* t, xt_tensor** Q, xt_tensor** R)
+int xt_tqr(const xt_tensor* t, int return_lq, xt_tensor** Q, xt_tensor** R)
 {
     return run(t ? t->owner : nullptr, [&] {
-        auto [q, r] = xa::tqr(t->t);
+        auto [q, r] = xa::tqr(t->t, return_lq != 0);
         *Q = wrap(t->owner, q);
         *R = wrap(t->owner, r);
     });

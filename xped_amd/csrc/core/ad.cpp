@@ -93,6 +93,28 @@ void svd_bwd(const DTensor& U, const DTensor& S, const DTensor& Vdag, const DTen
            "svd adjoint");
 }
 
+void qr_bwd(const DTensor& A, const DTensor& Q, const DTensor& R, const DTensor& dQ, const DTensor& dR, DTensor& dA)
+{
+    A.rt->require_device();
+    require_same_blocks(*Q.st, *dQ.st, "qr adjoint (Q)");
+    require_same_blocks(*R.st, *dR.st, "qr adjoint (R)");
+    require_same_blocks(*A.st, *dA.st, "qr adjoint (A)");
+    const TensorStructure &a = *A.st, &q = *Q.st, &r = *R.st;
+    std::vector<xh_qr_bwd_item> items;
+    for(std::size_t i = 0; i < a.sector.size(); ++i) {
+        const int jq = q.find_sector(a.sector[i]), jr = r.find_sector(a.sector[i]);
+        if(jq < 0 || jr < 0) throw Error(XH_EINVAL, "qr adjoint: Q or R lacks a sector of A");
+        const std::size_t m = a.rows[i], n = a.cols[i], k = std::min(m, n);
+        if(q.rows[jq] != m || q.cols[jq] != k || r.rows[jr] != k || r.cols[jr] != n)
+            throw Error(XH_EINVAL, "qr adjoint: every sector needs the middle dimension min(rows, cols)");
+        items.push_back(xh_qr_bwd_item{static_cast<int64_t>(a.offsets[i]), static_cast<int64_t>(q.offsets[jq]), static_cast<int64_t>(r.offsets[jr]),
+                                       static_cast<int32_t>(m), static_cast<int32_t>(n)});
+    }
+    if(items.empty()) return;
+    dcheck(A.rt->ctx, xh_qr_bwd(A.rt->ctx, items.data(), static_cast<int64_t>(items.size()), A.data(), Q.data(), R.data(), dQ.data(), dR.data(), dA.data()),
+           "qr adjoint");
+}
+
 // cutoff_matrices reverse pass (ADTensor.hpp:357-370): the truncated adjoints are added into the
 // leading columns of dU, the leading corner of dS and the leading rows of dVdag, sector by sector.
 void cutoff_bwd(const DTensor& dUt, const DTensor& dSt, const DTensor& dVt, DTensor& dU, DTensor& dS, DTensor& dVdag)

@@ -15,6 +15,11 @@ void svd_bwd(const DTensor& U, const DTensor& S, const DTensor& Vdag, const DTen
 // factors into the leading columns / corner / rows of the adjoints of the full factors.
 void cutoff_bwd(const DTensor& dUt, const DTensor& dSt, const DTensor& dVt, DTensor& dU, DTensor& dS, DTensor& dVdag);
 
+// tQR reverse pass (no counterpart in the reference, whose AD tensor records no tQR): with
+// (Q, R) = tQR(A), every sector needs the middle dimension min(rows, cols) and a full-rank R;
+// dA += the QR adjoint of the sector (xh_qr_bwd).
+void qr_bwd(const DTensor& A, const DTensor& Q, const DTensor& R, const DTensor& dQ, const DTensor& dR, DTensor& dA);
+
 // diag_sqrt (mode 0) / diag_inv (mode 1) reverse passes (ADTensor.hpp:560-590); y is the result.
 void diag_bwd(const DTensor& y, const DTensor& dy, DTensor& dx, int mode);
 // trace reverse pass (ADTensor.hpp:533-545): dx += g * WeightedIdentity.

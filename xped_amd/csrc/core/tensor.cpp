@@ -455,7 +455,10 @@ static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r,
             for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
         }
         Runtime* rt = l.rt;
-        dcheck(rt->ctx, xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision), "grouped gemm");
+        if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 1);  // timed contraction products carry tag 1
+        const int rc = xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision);
+        if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 0);
+        dcheck(rt->ctx, rc, "grouped gemm");
         if(rt->time_gemm)
             for(const auto& g : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(g.m), static_cast<int32_t>(g.n), static_cast<int32_t>(g.k)});
         rt->gemm_flops += cg->plan.flops;
@@ -726,10 +729,13 @@ SVDResult tsvd(const DTensor& t, std::size_t maxKeep, double eps_svd, bool prese
     return cutoff(t, f, maxKeep, eps_svd, preserve_multiplets, trunc_weight_in, entropy_in);
 }
 
-std::pair<DTensor, DTensor> tqr(const DTensor& t)
+std::pair<DTensor, DTensor> tqr(const DTensor& t, bool return_lq)
 {
     t.rt->require_device();
     const TensorStructure& s = *t.st;
+    if(return_lq)
+        for(std::size_t i = 0; i < s.sector.size(); ++i)
+            if(s.rows[i] != s.cols[i]) throw Error(XH_EINVAL, "tQR(RETURN_LQ): the adjoint factors fit the tensors of Q and R only for square sector blocks");
     Qbasis middle = s.dom().dim() > s.cod().dim() ? s.cod().forget_history() : s.dom().forget_history();
     DTensor Qt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, s.unc_dom, {middle}, s.sector));
     DTensor Rt = DTensor::empty(t.rt, TensorStructure::create_pushed(s.sym, {middle}, s.unc_cod, s.sector));
@@ -740,6 +746,20 @@ std::pair<DTensor, DTensor> tqr(const DTensor& t)
                                        static_cast<int64_t>(Rt.st->offsets[i]), static_cast<int32_t>(s.rows[i]), static_cast<int32_t>(s.cols[i])});
     }
     if(!items.empty()) dcheck(t.rt->ctx, xh_block_qr(t.rt->ctx, items.data(), static_cast<int64_t>(items.size()), t.data(), Qt.data(), Rt.data()), "block qr");
+    if(return_lq && !items.empty()) {
+        // Qmat = Q^T, Rmat = R^T per sector (Tensor.cpp:767-769); square blocks, same offsets
+        DTensor Ql = DTensor::empty(t.rt, Qt.st), Rl = DTensor::empty(t.rt, Rt.st);
+        std::vector<xh_copy_rect> rq, rr;
+        for(std::size_t i = 0; i < s.sector.size(); ++i) {
+            const int32_t n = static_cast<int32_t>(s.rows[i]);
+            if(n == 0) continue;
+            rq.push_back(xh_copy_rect{static_cast<int64_t>(Qt.st->offsets[i]), n, static_cast<int64_t>(Qt.st->offsets[i]), n, n, n, 1.0, 1});
+            rr.push_back(xh_copy_rect{static_cast<int64_t>(Rt.st->offsets[i]), n, static_cast<int64_t>(Rt.st->offsets[i]), n, n, n, 1.0, 1});
+        }
+        dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, Qt.data(), Ql.data(), rq.data(), static_cast<int64_t>(rq.size())), "transpose Q");
+        dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, Rt.data(), Rl.data(), rr.data(), static_cast<int64_t>(rr.size())), "transpose R");
+        return {Ql, Rl};
+    }
     return {Qt, Rt};
 }
 

@@ -148,7 +148,10 @@ struct FullSVD {
 FullSVD full_svd(const DTensor& t);
 SVDResult cutoff(const DTensor& t, const FullSVD& f, std::size_t max_keep, double eps_svd, bool preserve_multiplets, double trunc_weight_in,
                  double entropy_in);
-std::pair<DTensor, DTensor> tqr(const DTensor& t);    // Tensor.cpp:757-780 (QR, RETURN_LQ = false)
+// Tensor::tQR(RETURN_LQ) (Tensor.cpp:757-780).  With return_lq the reference returns the
+// adjoints of the per-sector factors (Q^T, R^T: the LQ factors of A^T) in the tensors of Q and R,
+// which is only well-formed when every sector block is square; other blocks raise XH_EINVAL.
+std::pair<DTensor, DTensor> tqr(const DTensor& t, bool return_lq = false);
 std::pair<DTensor, DTensor> teigh(const DTensor& t);  // Tensor.cpp:782-803
 
 // Host-side truncation rule of cutoff_matrices (Tensor.cpp:523-633) on an explicit list of

@@ -160,3 +160,115 @@ extern "C" int xh_svd_bwd(xh_ctx* ctx, const xh_svd_bwd_item* items, int64_t nit
         pu.run(Z, V, dA, 1.0);
     });
 }
+
+// ---------------------------------------------------------------- QR adjoint
+// The reference records no adjoint for tQR (include/Xped/AD/ADTensor.hpp has none); this is the
+// standard one.  Per sector, A = Q R with k = min(m, n), Q m x k, R k x n upper triangular with a
+// nonzero diagonal (full rank).  For m >= n (R square):
+//
+//   dA += (dQ + Q copyltu(M)) R^-T,   M = R dR^T - dQ^T Q,   copyltu(M)_ij = M_max(i,j),min(i,j),
+//
+// which follows from dA = dQ' R + Q dR' with Q^T dQ' antisymmetric and dR' upper triangular.
+// For m < n, A = Q [R1 R2] splits into the square factorisation X = Q R1 and Y = Q R2: Y's part
+// of dA is Q dR2, and X's part is the square formula with dQ + Y dR2^T in place of dQ.
+// One workgroup per sector in four phases (products with no data reuse: meant for the small blocks
+// of the reference's tQR callers).
+namespace {
+constexpr int QB_THREADS = 256;
+struct QbItem {
+    int64_t a_off, q_off, r_off, w_off, m_off;  // w_off: two m x k scratch blocks; m_off: k x k
+    int32_t m, n, k, pad;
+};
+__global__ __launch_bounds__(QB_THREADS) void k_qr_bwd(const QbItem* __restrict__ items, const double* __restrict__ A, const double* __restrict__ Q,
+                                                      const double* __restrict__ R, const double* __restrict__ dQ, const double* __restrict__ dR,
+                                                      double* __restrict__ dA, double* __restrict__ work)
+{
+    const QbItem it = items[blockIdx.x];
+    const int m = it.m, n = it.n, k = it.k, tid = threadIdx.x;
+    const double* a = A + it.a_off;
+    const double* q = Q + it.q_off;    // m x k, ld m
+    const double* r = R + it.r_off;    // k x n, ld k
+    const double* dq = dQ + it.q_off;
+    const double* dr = dR + it.r_off;
+    double* da = dA + it.a_off;        // m x n, ld m
+    double* W1 = work + it.w_off;      // dQ (+ Y dR2^T), m x k
+    double* W2 = W1 + int64_t(m) * k;  // dQ' + Q copyltu(M), then its solve, m x k
+    double* M = work + it.m_off;       // k x k
+    // phase 0: W1 = dQ + Y dR2^T, and dA[:, k:] += Q dR2 (wide blocks only)
+    for(int64_t e = tid; e < int64_t(m) * k; e += QB_THREADS) {
+        const int i = int(e % m), j = int(e / m);
+        double s = dq[e];
+        for(int l = k; l < n; ++l) s += a[int64_t(l) * m + i] * dr[int64_t(l) * k + j];
+        W1[e] = s;
+    }
+    for(int64_t e = tid; e < int64_t(m) * (n - k); e += QB_THREADS) {
+        const int i = int(e % m), l = k + int(e / m);
+        double s = 0.0;
+        for(int j = 0; j < k; ++j) s += q[int64_t(j) * m + i] * dr[int64_t(l) * k + j];
+        da[int64_t(l) * m + i] += s;
+    }
+    __syncthreads();
+    // phase 1: M = R1 dR1^T - W1^T Q  (R1, dR1: the first k columns)
+    for(int64_t e = tid; e < int64_t(k) * k; e += QB_THREADS) {
+        const int i = int(e % k), j = int(e / k);
+        double s = 0.0;
+        for(int c = i; c < k; ++c) s += r[int64_t(c) * k + i] * dr[int64_t(c) * k + j];
+        for(int x = 0; x < m; ++x) s -= W1[int64_t(i) * m + x] * q[int64_t(j) * m + x];
+        M[e] = s;
+    }
+    __syncthreads();
+    // phase 2: W2 = W1 + Q copyltu(M)
+    for(int64_t e = tid; e < int64_t(m) * k; e += QB_THREADS) {
+        const int x = int(e % m), j = int(e / m);
+        double s = W1[e];
+        for(int i = 0; i < k; ++i) s += q[int64_t(i) * m + x] * (i >= j ? M[int64_t(j) * k + i] : M[int64_t(i) * k + j]);
+        W2[e] = s;
+    }
+    __syncthreads();
+    // phase 3: row by row, z R1^T = w by back substitution (R1 upper), then dA[:, :k] += z
+    for(int x = tid; x < m; x += QB_THREADS) {
+        for(int i = k - 1; i >= 0; --i) {
+            double s = W2[int64_t(i) * m + x];
+            for(int j = i + 1; j < k; ++j) s -= r[int64_t(j) * k + i] * W2[int64_t(j) * m + x];
+            W2[int64_t(i) * m + x] = s / r[int64_t(i) * k + i];
+        }
+        for(int i = 0; i < k; ++i) da[int64_t(i) * m + x] += W2[int64_t(i) * m + x];
+    }
+}
+}  // namespace
+
+extern "C" int xh_qr_bwd(xh_ctx* ctx, const xh_qr_bwd_item* items, int64_t nitems, const double* A, const double* Q, const double* R, const double* dQ,
+                         const double* dR, double* dA)
+{
+    return guard(ctx, [&] {
+        if(nitems <= 0) return;
+        std::vector<QbItem> act;
+        int64_t work = 0;
+        for(int64_t g = 0; g < nitems; ++g) {
+            const xh_qr_bwd_item& it = items[g];
+            if(it.m < 0 || it.n < 0 || it.a_off < 0 || it.q_off < 0 || it.r_off < 0) throw xh::Error(XH_EINVAL, "qr adjoint: bad item");
+            if(it.m == 0 || it.n == 0) continue;
+            const int32_t k = std::min(it.m, it.n);
+            QbItem q{it.a_off, it.q_off, it.r_off, work, 0, it.m, it.n, k, 0};
+            work += 2 * int64_t(it.m) * k;
+            q.m_off = work;
+            work += int64_t(k) * k;
+            act.push_back(q);
+        }
+        if(act.empty()) return;
+        double* w = nullptr;
+        check(hipMallocAsync(reinterpret_cast<void**>(&w), size_t(work) * sizeof(double), ctx->stream), "hipMallocAsync(qr adjoint)");
+        QbItem* d_items = nullptr;
+        try {
+            d_items = xh::dev_upload(ctx, act);
+            hipLaunchKernelGGL(k_qr_bwd, dim3(static_cast<unsigned>(act.size())), dim3(QB_THREADS), 0, ctx->stream, d_items, A, Q, R, dQ, dR, dA, w);
+            xh::launch_check("k_qr_bwd");
+        } catch(...) {
+            xh::dev_free(ctx, d_items);
+            xh::dev_free(ctx, w);
+            throw;
+        }
+        xh::dev_free(ctx, d_items);
+        xh::dev_free(ctx, w);
+    });
+}

@@ -1,8 +1,7 @@
 // Context, buffers, events and the elementwise / reduction kernels of the device layer.
 #include <algorithm>
+#include <cfloat>
 #include <cstring>
-
-#include <rocblas/rocblas.h>
 
 #include "device.hpp"
 
@@ -63,7 +62,6 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     for(auto& ev : ctx->perm_events) hipEventDestroy(ev);
     if(ctx->host_scalar) hipHostFree(ctx->host_scalar);
     if(ctx->dev_scalar) hipFree(ctx->dev_scalar);
-    if(ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
     if(ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return XH_OK;
@@ -245,16 +243,22 @@ extern "C" int xh_gemm_launch_log(xh_ctx* ctx, int64_t launch, double* ms, int32
         check(hipEventElapsedTime(&t, ctx->gemm_events[2 * launch], ctx->gemm_events[2 * launch + 1]), "hipEventElapsedTime");
         if(ms) *ms = t;
         const auto& v = ctx->gemm_launch_items[launch];
-        const int64_t ni = (int64_t(v.size()) - 2) / 3;
+        const int64_t ni = (int64_t(v.size()) - 3) / 3;
         if(trans) {
-            trans[0] = v[0];
-            trans[1] = v[1];
+            trans[0] = v[1];
+            trans[1] = v[2];
+            trans[2] = v[0];
         }
         if(nitems) *nitems = ni;
         if(mnk)
             for(int64_t i = 0; i < std::min(ni, cap); ++i)
-                for(int j = 0; j < 3; ++j) mnk[3 * i + j] = v[2 + 3 * i + j];
+                for(int j = 0; j < 3; ++j) mnk[3 * i + j] = v[3 + 3 * i + j];
     });
+}
+
+extern "C" int xh_set_gemm_tag(xh_ctx* ctx, int tag)
+{
+    return guard(ctx, [&] { ctx->gemm_tag = tag; });
 }
 
 extern "C" int xh_event_record(xh_ctx* ctx, int slot)

@@ -34,8 +34,6 @@ struct xh_ctx {
     // small pinned host staging buffer for scalar reductions
     double* host_scalar = nullptr;
     double* dev_scalar = nullptr;  // RED_PARTS + 64 doubles of device scratch for reductions
-    void* blas = nullptr;          // rocblas_handle for the vendor FP64 SVD of large sector blocks
-    int svd_backend = 0;           // 0: own Jacobi kernels, 1: rocSOLVER gesvd for large blocks
     int gemm_tiles = 0;            // FP64 GEMM tiling: 0 128x128 tiles for large sectors, 1 64x64 tiles only
     double* gemm_scratch = nullptr;  // device address of this device's copy of the GEMM scratch array (xh_gemm_noalloc)
     // host staging buffers of asynchronous uploads, kept until the event after their copy fires
@@ -52,8 +50,9 @@ struct xh_ctx {
     std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
     size_t gemm_events_used = 0;
     std::vector<double> gemm_event_flops;
-    // per timed launch: transposes and the (m, n, k) of every item (xh_gemm_launch_log)
+    // per timed launch: tag, transposes and the (m, n, k) of every item (xh_gemm_launch_log)
     std::vector<std::vector<int32_t>> gemm_launch_items;
+    int gemm_tag = 0;  // recorded with each timed launch (xh_set_gemm_tag)
     // the same for permute launches (xh_permute_timing), with the bytes each one moves
     // GEMM tile autotuner: plan signature -> tiling (0 mixed, 1 64x64 only), and counters
     bool gemm_autotune = false;

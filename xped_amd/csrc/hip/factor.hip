@@ -12,8 +12,6 @@
 #include <cfloat>
 #include <cstring>
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "device.hpp"
 
@@ -700,70 +698,24 @@ void run_factor(xh_ctx* ctx, void (*kern)(FactorArgs), const std::vector<xh_fact
 
 }  // namespace
 
-// With the vendor backend selected (xh_set_svd_backend), blocks whose smaller side exceeds this use
-// rocSOLVER's FP64 gesvd (Householder bidiagonalisation + bidiagonal QR, the LAPACK algorithm the
-// reference reaches through Eigen and LAPACKE).
-constexpr int VENDOR_SVD_MIN = 384;
 // blocks with more columns than this (after transposition) use the block Jacobi path
 constexpr int BJ_MIN_COLS = 64;
-
-static void vendor_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& big, const double* A, double* U, double* S, double* Vt)
-{
-    if(big.empty()) return;
-    if(!ctx->blas) {
-        rocblas_handle h = nullptr;
-        if(rocblas_create_handle(&h) != rocblas_status_success) throw xh::Error(XH_EHIP, "rocblas_create_handle failed");
-        ctx->blas = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(ctx->blas);
-    if(rocblas_set_stream(h, ctx->stream) != rocblas_status_success) throw xh::Error(XH_EHIP, "rocblas_set_stream failed");
-    int64_t maxmn = 0, maxr = 0;
-    for(const auto& it : big) {
-        maxmn = std::max<int64_t>(maxmn, int64_t(it.m) * it.n);
-        maxr = std::max<int64_t>(maxr, std::min(it.m, it.n));
-    }
-    double *work = nullptr, *E = nullptr;
-    rocblas_int* info = nullptr;
-    check(hipMallocAsync(reinterpret_cast<void**>(&work), size_t(maxmn) * sizeof(double), ctx->stream), "hipMallocAsync");
-    check(hipMallocAsync(reinterpret_cast<void**>(&E), size_t(std::max<int64_t>(maxr, 1)) * sizeof(double), ctx->stream), "hipMallocAsync");
-    check(hipMallocAsync(reinterpret_cast<void**>(&info), sizeof(rocblas_int) * big.size(), ctx->stream), "hipMallocAsync");
-    for(size_t b = 0; b < big.size(); ++b) {
-        const auto& it = big[b];
-        const int r = std::min(it.m, it.n);
-        check(hipMemcpyAsync(work, A + it.a_off, size_t(it.m) * it.n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
-        rocblas_status st = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, it.m, it.n, work, it.m, S + it.s_off, U + it.u_off, it.m,
-                                             Vt + it.vt_off, r, E, rocblas_outofplace, info + b);
-        if(st != rocblas_status_success) throw xh::Error(XH_EHIP, "rocsolver_dgesvd failed");
-    }
-    std::vector<rocblas_int> hinfo(big.size());
-    check(hipMemcpyAsync(hinfo.data(), info, sizeof(rocblas_int) * big.size(), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
-    check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    xh::dev_free(ctx, work);
-    xh::dev_free(ctx, E);
-    xh::dev_free(ctx, info);
-    for(auto x : hinfo)
-        if(x != 0) throw xh::Error(XH_ENOTCONV, "gesvd: bidiagonal QR did not converge");
-}
 
 extern "C" int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t nitems, const double* A, double* U, double* S, double* Vt)
 {
     return guard(ctx, [&] {
         // small blocks: one workgroup each, in LDS; larger ones: block Jacobi over grouped GEMMs
-        // (or the vendor LAPACK-style path when selected)
-        std::vector<xh_factor_item> v, big, vendor;
+        std::vector<xh_factor_item> v, big;
         const size_t budget = lds_budget(ctx);
         for(int64_t i = 0; i < nitems; ++i) {
             const xh_factor_item& it = items[i];
             if(it.m <= 0 || it.n <= 0) continue;
             const int mn = std::min(it.m, it.n), mx = std::max(it.m, it.n);
-            if(ctx->svd_backend == 1 && mn > VENDOR_SVD_MIN)
-                vendor.push_back(it);
-            else if(mn > BJ_MIN_COLS || size_t(svd_work_doubles(mx, mn)) * sizeof(double) > budget)
+            if(mn > BJ_MIN_COLS || size_t(svd_work_doubles(mx, mn)) * sizeof(double) > budget)
                 big.push_back(it);
             else
                 v.push_back(it);
         }
-        vendor_svd(ctx, vendor, A, U, S, Vt);
         block_jacobi_svd(ctx, big, A, U, S, Vt);
         run_factor(
             ctx, k_svd_jacobi, v,
@@ -845,10 +797,3 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
     });
 }
 
-extern "C" int xh_set_svd_backend(xh_ctx* ctx, int backend)
-{
-    return guard(ctx, [&] {
-        if(backend != 0 && backend != 1) throw xh::Error(XH_EINVAL, "svd backend must be 0 (own Jacobi) or 1 (rocSOLVER gesvd)");
-        ctx->svd_backend = backend;
-    });
-}

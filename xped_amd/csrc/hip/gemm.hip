@@ -751,10 +751,10 @@ void gemm_scratch_init(xh_ctx* ctx)
 }  // namespace xh
 
 namespace {
-// the log record of a timed launch: ta, tb, then (m, n, k) per item
-std::vector<int32_t> launch_items(const xh_gemm_plan* plan)
+// the log record of a timed launch: tag, ta, tb, then (m, n, k) per item
+std::vector<int32_t> launch_items(const xh_ctx* ctx, const xh_gemm_plan* plan)
 {
-    std::vector<int32_t> v{plan->ta, plan->tb};
+    std::vector<int32_t> v{ctx->gemm_tag, plan->ta, plan->tb};
     v.reserve(2 + 3 * plan->h_items.size());
     for(const auto& it : plan->h_items) {
         v.push_back(it.m);
@@ -853,14 +853,14 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
                 ctx->gemm_tune[plan->sig] = tiles;
                 ++ctx->gemm_tune_runs;
                 if(tiles == 0) run(0);
-                if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(plan));
+                if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(ctx, plan));
                 if(own_ws) xh::dev_free(ctx, ws);
                 return;
             }
         }
         if(tiles != 0 && !plan->all64_built) build_all64(ctx, plan);  // autotuned choice (never with noalloc)
         run(tiles);
-        if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(plan));
+        if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(ctx, plan));
         if(own_ws) xh::dev_free(ctx, ws);
     }
 }

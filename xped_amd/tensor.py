@@ -284,6 +284,17 @@ class Tensor:
         leg = self.leg(0) if self.rank + self.corank else None
         self.sym = leg.sym if leg is not None else U1
 
+    @classmethod
+    def identity(cls, rt: Runtime, domain: Sequence[Qbasis], codomain: Sequence[Qbasis]) -> "Tensor":
+        """Tensor::Identity (Tensor.hpp:224-236): every block the rows x cols identity (ones on the
+        main diagonal), written on the host and uploaded."""
+        t = cls(rt, domain, codomain)
+        data = np.zeros(t.size)
+        for _, off, r, c in t.sectors():
+            k = min(r, c)
+            data[off + np.arange(k) * (r + 1)] = 1.0
+        return t.upload(data)
+
     def __del__(self):
         h = getattr(self, "h", None)
         f = _lib.native("xt_tensor_destroy") if (_lib is not None and h) else None
@@ -404,6 +415,32 @@ class Tensor:
                             C.byref(en)), self.rt.h, "tSVD")
         return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, en.value
 
+    def tSVD_spec(self, maxKeep: int, eps_svd: float, truncWeight: float = 0.0, entropy: float = 0.0, SVspec: Dict = None,
+                  PRESERVE_MULTIPLETS: bool = True, RETURN_SPEC: bool = True):
+        """The full tSVD overload (Tensor.hpp:353-362): returns (U, S, V, truncWeight, entropy, SVspec),
+        SVspec a dict {q: kept singular values} extended as std::map::insert does."""
+        L = lib()
+        sp = C.c_void_p()
+        check(L.xt_spec_create(C.byref(sp)), self.rt.h, "spec")
+        try:
+            U, S, V = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            tw, en = C.c_double(truncWeight), C.c_double(entropy)
+            check(L.xt_tsvd_spec(self.h, int(maxKeep), float(eps_svd), C.byref(tw), C.byref(en), sp, int(bool(PRESERVE_MULTIPLETS)),
+                                 int(bool(RETURN_SPEC)), C.byref(U), C.byref(S), C.byref(V)), self.rt.h, "tSVD")
+            out = dict(SVspec or {})
+            n = C.c_int()
+            L.xt_spec_size(sp, C.byref(n))
+            for i in range(n.value):
+                q, cnt = (C.c_int * 2)(), C.c_int()
+                L.xt_spec_entry(sp, i, q, C.byref(cnt), None, 0)
+                vals = (C.c_double * max(cnt.value, 1))()
+                L.xt_spec_entry(sp, i, q, C.byref(cnt), vals, cnt.value)
+                key = (q[0],) if self.sym[1] == 0 else (q[0], q[1])
+                out.setdefault(key, np.array(vals[:cnt.value]))
+        finally:
+            L.xt_spec_destroy(sp)
+        return self._wrap(U), self._wrap(S), self._wrap(V), tw.value, en.value, out
+
     def tSVD_randomized(self, maxKeep: int, eps_svd: float, ranks: Dict, oversample: int = 8, rank_factor: float = 2.0, power_iters: int = 1,
                         sketch_precision: str = "fp64", seed: int = 1):
         """Randomized truncated block SVD; returns (U, S, Vdag, truncWeight, safe, residual_ratio)."""
@@ -463,9 +500,10 @@ class Tensor:
         check(lib().xh_add_at(self.rt.ctx(), C.c_void_p(self.device_ptr()), int(index), float(value)), self.rt.h, "add_at")
         return self
 
-    def tQR(self):
+    def tQR(self, RETURN_LQ: bool = False):
+        """Tensor::tQR(RETURN_LQ) (Tensor.cpp:757-780)."""
         Q, R = C.c_void_p(), C.c_void_p()
-        check(lib().xt_tqr(self.h, C.byref(Q), C.byref(R)), self.rt.h, "tQR")
+        check(lib().xt_tqr(self.h, int(bool(RETURN_LQ)), C.byref(Q), C.byref(R)), self.rt.h, "tQR")
         return self._wrap(Q), self._wrap(R)
 
     def teigh(self):
@@ -520,6 +558,11 @@ class Tensor:
 
     def maxNorm(self) -> float:
         return self._scalar(lib().xt_max_norm)
+
+
+def qr_bwd(A: Tensor, Q: Tensor, R: Tensor, dQ: Tensor, dR: Tensor, dA: Tensor):
+    """tQR reverse pass (xt_qr_bwd): dA += the QR adjoint, sector by sector."""
+    check(lib().xt_qr_bwd(A.h, Q.h, R.h, dQ.h, dR.h, dA.h), A.rt.h, "qr adjoint")
 
 
 def mul_bwd(A: Tensor, B: Tensor, dC: Tensor, dA: Tensor = None, dB: Tensor = None, precision: str = "fp64"):
