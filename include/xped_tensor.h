@@ -110,6 +110,17 @@ const double* xt_tensor_device_ptr(const xt_tensor* t);
 /* Tensor::setRandom(engine) with std::mt19937(seed): dist 0 uniform[-1,1), 1 normal(0,1) */
 int xt_tensor_set_random(xt_tensor* t, uint32_t seed, int dist);
 
+/* Storage wire format: the fields of the reference's StorageType serialisation
+ * (StorageType_contiguous.hpp:191-204: data, sectors, uncoupled and coupled bases, offsets) in one
+ * binary layout ("XPEDTNS1", documented in csrc/core/storage.cpp); the reader rebuilds the coupled
+ * bases and the layout from the legs and refuses archives that disagree.  A structure-only runtime
+ * writes and reads the structure alone.  serialize: *n = the archive's size; the bytes are written
+ * when buf is non-null and cap >= *n. */
+int xt_tensor_save(const xt_tensor* t, const char* path);
+int xt_tensor_load(xt_rt* rt, const char* path, xt_tensor** out);
+int xt_tensor_serialize(const xt_tensor* t, void* buf, size_t cap, size_t* n);
+int xt_tensor_deserialize(xt_rt* rt, const void* buf, size_t n, xt_tensor** out);
+
 /* ---- operations ------------------------------------------------------------------------ */
 int xt_permute(const xt_tensor* t, int shift, const int* p, int np, xt_tensor** out);   /* Tensor.cpp:335-355 */
 /* structure-level description of the permute plan: number of output subblocks, sources, and the

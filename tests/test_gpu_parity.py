@@ -440,3 +440,20 @@ def test_cpp_facade_runs_the_hot_path_on_the_gpu(rt):
         np.testing.assert_allclose(qb @ rb, Cc.block(i), rtol=0, atol=1e-12 * np.abs(Cc.data).max())
         np.testing.assert_allclose(qb.T @ qb, np.eye(qb.shape[1]), rtol=0, atol=1e-12)
         assert np.abs(np.tril(rb, -1)).max(initial=0.0) == 0.0
+
+
+@pytest.mark.parametrize("pcode", [xt.U1, xt.SU2, xt.FU1xU1], ids=["U1", "SU2", "FU1xU1"])
+def test_storage_round_trip_with_data(rt, pcode, tmp_path):
+    # xt_tensor_save / load: data and structure bit for bit, and the loaded tensor computes as the original
+    r = random.Random(12)
+    legs = [rand_entries(pcode, r, 3, 4) for _ in range(4)]
+    o, t = both_tensor(rt, pcode, legs, 2, 33)
+    u = t.permute(1, (2, 0, 3, 1))
+    f = tmp_path / "u.xt"
+    u.save(f)
+    v = xt.Tensor.load(rt, f)
+    assert structure_of(v) == structure_of(u)
+    assert np.array_equal(v.download(), u.download())
+    assert np.array_equal(v.permute(-1, (1, 3, 0, 2)).download(), u.permute(-1, (1, 3, 0, 2)).download())
+    w = xt.Tensor.deserialize(rt, t.serialize())
+    assert np.array_equal(w.download(), o.data)

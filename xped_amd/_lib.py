@@ -162,6 +162,10 @@ _SIGS = [
     ("xt_tensor_download", C.c_int, [_P, _DP, _SZ]),
     ("xt_tensor_device_ptr", _P, [_P]),
     ("xt_tensor_set_random", C.c_int, [_P, C.c_uint32, C.c_int]),
+    ("xt_tensor_save", C.c_int, [_P, C.c_char_p]),
+    ("xt_tensor_load", C.c_int, [_P, C.c_char_p, _PP]),
+    ("xt_tensor_serialize", C.c_int, [_P, _P, _SZ, _SZP]),
+    ("xt_tensor_deserialize", C.c_int, [_P, _P, _SZ, _PP]),
     ("xt_permute", C.c_int, [_P, C.c_int, _IP, C.c_int, _PP]),
     ("xt_permute_plan_stats", C.c_int, [_P, C.c_int, _IP, C.c_int, _I64P, _I64P, _I64P]),
     ("xt_permute_plan_dump", C.c_int, [_P, C.c_int, _IP, C.c_int, _I64P, _I64P, _I64P, C.POINTER(C.c_int32), _I64P, _DP]),

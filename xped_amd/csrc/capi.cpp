@@ -7,6 +7,7 @@
 
 #include "../../include/xped_tensor.h"
 #include "core/ctm.hpp"
+#include "core/storage.hpp"
 #include "core/ad.hpp"
 #include "core/tensor.hpp"
 
@@ -604,6 +605,40 @@ int xt_tsvd_spec(const xt_tensor* t, size_t max_keep, double eps_svd, double* tr
         if(entropy) *entropy = r.entropy;
         if(return_spec)
             for(const auto& [q, v] : r.spec) SVspec->m.insert({q, v});  // std::map::insert keeps an existing entry, as the reference's SVspec.insert
+    });
+}
+
+int xt_tensor_save(const xt_tensor* t, const char* path)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        if(!t || !path) throw xa::Error(XH_EINVAL, "xt_tensor_save: null argument");
+        xa::save(t->t, path);
+    });
+}
+
+int xt_tensor_load(xt_rt* rt, const char* path, xt_tensor** out)
+{
+    return run(rt, [&] {
+        if(!rt || !path || !out) throw xa::Error(XH_EINVAL, "xt_tensor_load: null argument");
+        *out = wrap(rt, xa::load(&rt->rt, path));
+    });
+}
+
+int xt_tensor_serialize(const xt_tensor* t, void* buf, size_t cap, size_t* n)
+{
+    return run(t ? t->owner : nullptr, [&] {
+        if(!t || !n) throw xa::Error(XH_EINVAL, "xt_tensor_serialize: null argument");
+        const auto b = xa::serialize(t->t);
+        *n = b.size();
+        if(buf && cap >= b.size()) std::memcpy(buf, b.data(), b.size());
+    });
+}
+
+int xt_tensor_deserialize(xt_rt* rt, const void* buf, size_t n, xt_tensor** out)
+{
+    return run(rt, [&] {
+        if(!rt || !buf || !out) throw xa::Error(XH_EINVAL, "xt_tensor_deserialize: null argument");
+        *out = wrap(rt, xa::deserialize(&rt->rt, static_cast<const char*>(buf), n));
     });
 }
 

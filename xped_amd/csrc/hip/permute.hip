@@ -31,9 +31,15 @@ constexpr int TR_DIM = 64;     // transpose tile edge
 
 struct TaskMeta {
     int64_t dst_off, dst_ld;
-    uint32_t rows, cols;
+    uint32_t rows, rmagic, rshift, pad;  // rmagic / rshift: magic-number division by rows
     int32_t nsrc, src_begin;
 };
+// (row, column) of element el of a task's column-major output subblock, without an integer divide
+__device__ __forceinline__ void split_el(const TaskMeta& tm, uint32_t el, uint32_t& row, uint32_t& col)
+{
+    col = xh::fdiv(el, tm.rmagic, tm.rshift);
+    row = el - col * tm.rows;
+}
 
 // one output leg of one task: extent, magic division, and the stride of its source leg in the
 // source subblock's row (bit 8 of flags clear) or column (set) index; bits 0-7: division shift
@@ -142,7 +148,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const 
             val[i] = 0.0;
             if(r < n_here) {
                 const uint32_t el = static_cast<uint32_t>(base + r);
-                const uint32_t row = el % tm.rows, col = el / tm.rows;
+                uint32_t row, col;
+                split_el(tm, el, row, col);
                 int64_t srow, scol;
                 decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
                 dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
@@ -193,7 +200,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const 
                     const int lo = find_task(r);
                     const TaskMeta tm = s_meta[lo];
                     const uint32_t el = static_cast<uint32_t>(e0 + r - s_pre[lo]);
-                    const uint32_t row = el % tm.rows, col = el / tm.rows;
+                    uint32_t row, col;
+                split_el(tm, el, row, col);
                     int64_t srow, scol;
                     decode_src<NLT>(s_legs + lo * SL, p.nlegs, p.new_rank, row, col, srow, scol);
                     dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
@@ -219,7 +227,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_permute_gather(PermDev p, const 
                     const TaskMeta tm = p.meta[t];
                     const uint32_t el = static_cast<uint32_t>(e0 + r - p.prefix[t]);
                     const LegInfo* L = p.legs + int64_t(t) * p.nlegs;
-                    const uint32_t row = el % tm.rows, col = el / tm.rows;
+                    uint32_t row, col;
+                split_el(tm, el, row, col);
                     int64_t srow, scol;
                     decode_src<NLT>(L, p.nlegs, p.new_rank, row, col, srow, scol);
                     dpos[i] = tm.dst_off + row + int64_t(col) * tm.dst_ld;
@@ -473,7 +482,8 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                     e = e_end;
                 }
                 const int64_t idx = u > 0 ? ngather + slot[t] : slot[t];
-                meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), uint32_t(tk.cols), tk.nsrc, tk.src_begin};
+                const xh::FastDiv fd = xh::make_fastdiv(uint32_t(std::max<int64_t>(tk.rows, 1)));
+                meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), fd.magic, fd.shift, 0, tk.nsrc, tk.src_begin};
                 LegInfo* L = legs + idx * NL;
                 for(int k = 0; k < NL; ++k) {
                     const xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);

@@ -284,6 +284,29 @@ class Tensor:
         leg = self.leg(0) if self.rank + self.corank else None
         self.sym = leg.sym if leg is not None else U1
 
+    def serialize(self) -> bytes:
+        """The tensor's storage archive (xt_tensor_serialize)."""
+        n = C.c_size_t()
+        check(lib().xt_tensor_serialize(self.h, None, 0, C.byref(n)), self.rt.h, "serialize")
+        buf = (C.c_char * max(n.value, 1))()
+        check(lib().xt_tensor_serialize(self.h, buf, n.value, C.byref(n)), self.rt.h, "serialize")
+        return bytes(buf[:n.value])
+
+    @classmethod
+    def deserialize(cls, rt: Runtime, data: bytes) -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_tensor_deserialize(rt.h, data, len(data), C.byref(h)), rt.h, "deserialize")
+        return cls(rt, handle=h)
+
+    def save(self, path: str):
+        check(lib().xt_tensor_save(self.h, str(path).encode()), self.rt.h, "save")
+
+    @classmethod
+    def load(cls, rt: Runtime, path: str) -> "Tensor":
+        h = C.c_void_p()
+        check(lib().xt_tensor_load(rt.h, str(path).encode(), C.byref(h)), rt.h, "load")
+        return cls(rt, handle=h)
+
     @classmethod
     def identity(cls, rt: Runtime, domain: Sequence[Qbasis], codomain: Sequence[Qbasis]) -> "Tensor":
         """Tensor::Identity (Tensor.hpp:224-236): every block the rows x cols identity (ones on the
