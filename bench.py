@@ -46,6 +46,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = "contraction TFLOP/s + CTMRG step ms, mixed/learned vs FP64 GPU at ≤1e-10 err"
 PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}  # TFLOP/s, dense; BF16x3 counted as 3 BF16 products
 SCALE = 10.0  # site tensors N(0,1) * SCALE / sqrt(plain size), see run_c4
+# the CPU oracle's whole FP64 CTMRG step on this workload (oracle/ctm.py: numpy/OpenBLAS per block,
+# written by tests/golden/make_ctm_fixture.py): far beyond a bounded sample, so it is quoted from that
+# run rather than measured here
+ORACLE_STEP = {"seconds": 4171.0, "cores": 8, "where": "build container, tests/golden/make_ctm_fixture.py, second step (chi = 256 reached)"}
 
 
 def parse():
@@ -54,7 +58,7 @@ def parse():
     ap.add_argument("--workload", default="c4", choices=["c4", "c3", "copy"])
     ap.add_argument("--steps", type=int, default=3, help="timed fast-mode steps (c4) or contractions (c3)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed fast-mode steps after the common warm-up (c4)")
-    ap.add_argument("--fp64-steps", type=int, default=1, help="timed steps of the plain FP64 mode (c4)")
+    ap.add_argument("--fp64-steps", type=int, default=4, help="steps of the plain FP64 mode (c4), compared step by step with the fast mode's")
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--graph", action="store_true", help="c3: also time the product replayed from a captured HIP graph")
     ap.add_argument("--autotune", action="store_true", help="pick each FP64 GEMM plan's tiling by timing both on its first call")
@@ -64,12 +68,13 @@ def parse():
     ap.add_argument("--sketch-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the range finder (c4)")
     ap.add_argument("--ranks", default="learned", choices=["learned", "previous"],
                     help="target ranks of the randomized SVDs: the trained predictor or the previous step's kept counts")
-    ap.add_argument("--leg-dim", type=int, default=192, help="c3 leg dimension")
+    ap.add_argument("--leg-dim", type=int, default=512, help="c3 leg dimension")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the fast mode / c3")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
+    ap.add_argument("--no-previous", action="store_true", help="c4: skip the arm with the previous step's kept counts as target ranks")
     return ap.parse_args()
 
 
@@ -78,7 +83,32 @@ def log(msg):
 
 
 # ------------------------------------------------------------------------------------------ C4
-def run_mode(mode, args, xt, configs, random_ipeps):
+def launch_split(rt, xt):
+    """Kernel time and flops of the timed step's GEMM launches, split by the tag the tensor layer
+    gives its contraction products (1) against the randomized SVDs' sketch, Gram and factor
+    products (0)."""
+    import ctypes as C
+
+    L = xt.lib()
+    ctx = rt.ctx()
+    n = rt.gemm_launches()
+    out = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0]}
+    for i in range(n):
+        t, tr, ni = C.c_double(), (C.c_int32 * 3)(), C.c_int64()
+        L.xh_gemm_launch_log(ctx, i, C.byref(t), tr, None, 0, C.byref(ni))
+        buf = (C.c_int32 * (3 * max(ni.value, 1)))()
+        L.xh_gemm_launch_log(ctx, i, C.byref(t), tr, buf, ni.value, C.byref(ni))
+        fl = 0.0
+        for k in range(ni.value):
+            fl += 2.0 * buf[3 * k] * buf[3 * k + 1] * buf[3 * k + 2]
+        o = out[1 if tr[2] == 1 else 0]
+        o[0] += t.value
+        o[1] += fl
+        o[2] += 1
+    return out
+
+
+def run_mode(mode, args, xt, configs, random_ipeps, ranks=None):
     """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
     rt = xt.Runtime(0)
     if args.autotune and mode == "fast":
@@ -99,7 +129,8 @@ def run_mode(mode, args, xt, configs, random_ipeps):
     # the first step after the warm-up records its kept spectra (compared between the modes)
     configure(True)
     c.clear_spectra()
-    if mode == "fast" and args.ranks == "learned":
+    ranks = ranks or args.ranks
+    if mode == "fast" and ranks == "learned":
         from xped_amd import rank_predictor
 
         params, width = rank_predictor.load_weights()
@@ -144,6 +175,7 @@ def run_mode(mode, args, xt, configs, random_ipeps):
         c.step()
         rt.marker(2)
         ms, fl, nlog = rt.gemm_timing(False)
+        out["gemm_split"] = launch_split(rt, xt)
         out["gemm_launches"] = rt.gemm_launches()
         out["permute_ms"], out["permute_bytes"], out["permute_launches"] = rt.permute_timing(False, reset=True)
         out["gemm_kernel_ms"] = ms
@@ -220,8 +252,14 @@ def run_c4(args):
     from xped_amd.ipeps import random_ipeps
 
     fast = run_mode("fast", args, xt, configs, random_ipeps)
+    prev = None if (args.no_previous or args.ranks == "previous") else run_mode("fast", args, xt, configs, random_ipeps, ranks="previous")
     ref = None if args.no_fp64 else run_mode("fp64", args, xt, configs, random_ipeps)
-    value = fast["gemm_timed_flops"] / (fast["gemm_kernel_ms"] * 1e-3) / 1e12 if fast["gemm_kernel_ms"] > 0 else 0.0
+    # value: contraction products only (operator* of the step's contractions); all grouped GEMM
+    # launches of the step, the randomized SVDs' included, are reported beside it
+    con_ms, con_fl, con_n = fast["gemm_split"][1]
+    sk_ms, sk_fl, sk_n = fast["gemm_split"][0]
+    value = con_fl / (con_ms * 1e-3) / 1e12 if con_ms > 0 else 0.0
+    all_tf = fast["gemm_timed_flops"] / (fast["gemm_kernel_ms"] * 1e-3) / 1e12 if fast["gemm_kernel_ms"] > 0 else 0.0
     peak = PEAKS[args.precision]
     res = {
         "metric": METRIC,
@@ -248,9 +286,13 @@ def run_c4(args):
                  "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
         "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5),
                      "traffic": measured_traffic("k_gemm_f64_ctm_step_counters.md") if args.precision == "fp64" and args.chi == 256 else None,
-                     "kernel": "k_gemm_f64 + k_gemm_f64_128 (+ k_splitk_reduce)" if args.precision == "fp64" else f"k_gemm_{args.precision}",
-                     "launches_per_step": fast["gemm_launches"], "flops_per_step": fast["gemm_timed_flops"],
-                     "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3)},
+                     "kernel": ("grouped GEMM of the contractions: k_gemm_f64 + k_gemm_f64_128 + k_gemm_skinny (+ k_splitk_reduce)" if args.precision == "fp64"
+                                else f"k_gemm_{args.precision}"),
+                     "launches_per_step": con_n, "flops_per_step": con_fl, "kernel_ms_per_step": round(con_ms, 3)},
+        "all_gemm_launches": {"achieved": round(all_tf, 4), "unit": "TFLOP/s", "frac": round(all_tf / peak, 5), "launches_per_step": fast["gemm_launches"],
+                              "flops_per_step": fast["gemm_timed_flops"], "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3),
+                              "randomized_svd_products": {"achieved": round(sk_fl / (sk_ms * 1e-3) / 1e12, 4) if sk_ms > 0 else 0.0, "launches": sk_n,
+                                                          "flops": sk_fl, "kernel_ms": round(sk_ms, 3)}},
         "permute_roofline": {"bound": "hbm", "kernel": "k_permute_gather + k_permute_transpose",
                              "achieved": round(fast["permute_bytes"] / (fast["permute_ms"] * 1e-3) / 1e9, 1) if fast["permute_ms"] > 0 else 0.0,
                              "peak": 8000.0, "unit": "GB/s",
@@ -259,9 +301,17 @@ def run_c4(args):
                              "bytes_per_step": fast["permute_bytes"], "launches_per_step": fast["permute_launches"],
                              "kernel_ms_per_step": round(fast["permute_ms"], 3)},
     }
+    if prev is not None:
+        res["ranks_previous"] = {"ms_per_step": round(prev["ms_per_step"], 3), "svd_ms_per_step": round(prev["svd_ms_per_step"], 3),
+                                 "rsvd_calls_fallbacks_ratio": list(prev["rsvd"]), "energies": prev["energies"]}
     if ref is not None:
+        # step by step: the fast mode's warm-up and timed steps against as many FP64 steps
         nb = min(len(ref["energies"]), len(fast["energies"]))
         errs = [abs(fast["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nb)]
+        res["energy_rel_err_vs_fp64_per_step"] = errs
+        if prev is not None:
+            nbp = min(len(ref["energies"]), len(prev["energies"]))
+            res["ranks_previous"]["energy_rel_err_vs_fp64_per_step"] = [abs(prev["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nbp)]
         res["fp64"] = {"ms_per_step": round(ref["ms_per_step"], 3), "svd_ms_per_step": round(ref["svd_ms_per_step"], 3), "energies": ref["energies"],
                        "host_plan_ms_per_step": round(ref["host_plan_ms_per_step"], 1), "plans_per_step": ref["plans_per_step"]}
         res["speedup_vs_fp64"] = round(ref["ms_per_step"] / fast["ms_per_step"], 3)
@@ -269,6 +319,7 @@ def run_c4(args):
         res["spectra_vs_fp64"] = compare_spectra(fast["spectra"], ref["spectra"])
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_gemm_baseline(fast["gemm_shapes"], args.cpu_seconds)
+        res["cpu_baseline"]["oracle_ctm_step_s"] = ORACLE_STEP
     return res
 
 
