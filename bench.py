@@ -74,6 +74,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
+    ap.add_argument("--no-su", action="store_true", help="c4: skip the simple-update section")
     ap.add_argument("--no-previous", action="store_true", help="c4: skip the arm with the previous step's kept counts as target ranks")
     return ap.parse_args()
 
@@ -317,10 +318,35 @@ def run_c4(args):
         res["speedup_vs_fp64"] = round(ref["ms_per_step"] / fast["ms_per_step"], 3)
         res["energy_rel_err_vs_fp64"] = max(errs) if errs else None
         res["spectra_vs_fp64"] = compare_spectra(fast["spectra"], ref["spectra"])
+    if not args.no_su:
+        res["simple_update"] = run_simple_update(args, xt, configs, random_ipeps)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_gemm_baseline(fast["gemm_shapes"], args.cpu_seconds)
         res["cpu_baseline"]["oracle_ctm_step_s"] = ORACLE_STEP
     return res
+
+
+def run_simple_update(args, xt, configs, random_ipeps, steps=3):
+    """Simple-update steps (TimePropagator::t_step, 8 bond updates) of the Heisenberg model on the
+    C4 sites (D = 8, 2x2 cell), dt = 0.05, after one untimed step that builds the plans."""
+    rt = xt.Runtime(0)
+    A = random_ipeps(rt, configs.c4_virtual(), configs.c4_physical(), args.seed)
+    H = xt.heisenberg_bond(rt, A[0].leg(4))
+    tp = xt.TimePropagator(rt, [H] * 4, [H] * 4, 0.05)
+    psi = xt.WeightedPEPS(rt, A, 8)
+    tp.t_step(psi)
+    rt.synchronize()
+    c0 = rt.counters()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tp.t_step(psi)
+    rt.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = rt.counters()
+    log(f"[bench] simple update: {1e3 * wall / steps:.1f} ms per step")
+    return {"workload": "simple update t_step (8 bond updates), Heisenberg, 2x2 cell, D=8, dt=0.05", "steps": steps,
+            "ms_per_step": round(1e3 * wall / steps, 3), "ms_per_bond_update": round(1e3 * wall / steps / 8, 3),
+            "gemm_launches_per_step": (c1["n_gemm"] - c0["n_gemm"]) / steps, "permute_launches_per_step": (c1["n_permute"] - c0["n_permute"]) / steps}
 
 
 # ------------------------------------------------------------------------------------------ C3

@@ -211,9 +211,19 @@ int xh_add_rects(xh_ctx* ctx, const double* src, double* dst, const xh_copy_rect
  * The permute adjoint (Tensor::permute_adj, src/Core/Tensor.cpp:258-333) is a permute plan built
  * on the host, and the operator* adjoints (ADTensor.hpp:745-746) are xh_gemm plans with
  * trans_a / trans_b set. */
+/* exp(factor * M) of square column-major blocks (TensorBase::mexp, TensorBase.cpp:180-183; Eigen's
+ * matrix exponential per block, MatrixInterface_Eigen_impl.hpp:177-180): order m <= XH_MEXP_MAX,
+ * block at src + src_off, result at dst + dst_off (src == dst in place is allowed). */
+#define XH_MEXP_MAX 64
+typedef struct {
+    int64_t src_off, dst_off;
+    int32_t m, pad;
+} xh_mexp_item;
+int xh_block_mexp(xh_ctx* ctx, const xh_mexp_item* items, int64_t nitems, const double* src, double* dst, double factor);
 /* Diagonal-map reverse passes of square blocks (ADTensor.hpp:560-590; DiagCoeffBinaryOp acts on
  * the diagonal only): mode 0 for y = diag_sqrt(x): dx_ii += dy_ii * 0.5 / y_ii; mode 1 for
- * y = diag_inv(x): dx_ii += -dy_ii * y_ii^2. */
+ * y = diag_inv(x): dx_ii += -dy_ii * y_ii^2; mode 2 for y = diag_sqrt(x).diag_inv():
+ * dx_ii += -dy_ii * y_ii^3 / 2. */
 int xh_block_diag_bwd(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* y, const double* dy, double* dx, int mode);
 /* x_g += w_g * identity (rows_g x cols_g blocks): the trace reverse pass with WeightedIdentity
  * (ADTensor.hpp:533-545, Tensor.hpp:238-246). */

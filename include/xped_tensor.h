@@ -239,6 +239,13 @@ int xt_ctm_compute_rdm(xt_ctm* c);                  /* CTM::computeRDM (CTM.cpp:
 int xt_ctm_energy(xt_ctm* c, double Jxy, double Jz, double* E); /* avg + CTMSolver.cpp:60-62 */
 /* which: 0..7 = C1..C4, T1..T4; 8 = rho_h, 9 = rho_v; site = x*2 + y */
 int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out);
+/* the environment (C1..C4, T1..T4) of src becomes dst's (the reference's CTM copy `Jim = Jack`,
+ * CTMSolver.cpp:91-93); the tensors are shared, not copied */
+int xt_ctm_copy_env(xt_ctm* dst, const xt_ctm* src);
+/* energy after `steps` more grow_all moves and its gradient with respect to the four site tensors
+ * (CTMSolver::solve with AD, CTMSolver.cpp:80-137; each move a checkpoint, CTM.cpp:386-420): the
+ * environment on entry is a constant; needs svd_mode 0.  dA[site] are new tensors. */
+int xt_ctm_energy_grad(xt_ctm* c, int steps, double Jxy, double Jz, double* E, xt_tensor** dA);
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms);
 /* randomized SVD parameters of the CTM decompositions (see xt_tsvd_randomized) */
 int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_iters, int sketch_precision);
@@ -258,6 +265,48 @@ int xt_ctm_spectrum(const xt_ctm* c, int rec, int* site, int* dir, int* nsec, in
 int xt_ctm_clear_spectra(xt_ctm* c);
 /* target ranks for the next randomized SVD of (site, dir): nsec pairs (q[2], rank) */
 int xt_ctm_set_ranks(xt_ctm* c, int site, int dir, const int* qs, const int* ranks, int nsec);
+
+/* ---- reverse-mode recording (stan-math's reverse-pass callbacks of ADTensor.hpp) ---------- */
+/* While a tape records, xt_permute, xt_mul, xt_contract, xt_adjoint, xt_twist, xt_scale,
+ * xt_axpby, xt_diag_*, xt_tsvd (full SVD + cutoff), xt_tqr (QR, not LQ) and xt_div_trace push
+ * their reverse passes (ADTensor.hpp:138-602, 703-763); adjoints are keyed by the value's storage
+ * and accumulate.  Seed with xt_tape_seed (adj(y) += alpha * dy) or xt_tape_seed_trace (the
+ * adjoint of y.trace() times g), run xt_tape_backward, read adjoints with xt_tape_adjoint (zeros
+ * when none reached x).  One tape per runtime. */
+int xt_tape_begin(xt_rt* rt);
+int xt_tape_pause(xt_rt* rt, int paused); /* stop (1) or resume (0) recording */
+int xt_tape_seed(xt_rt* rt, const xt_tensor* y, const xt_tensor* dy, double alpha);
+int xt_tape_seed_trace(xt_rt* rt, const xt_tensor* y, double g);
+int xt_tape_backward(xt_rt* rt);
+int xt_tape_adjoint(xt_rt* rt, const xt_tensor* x, xt_tensor** out);
+int xt_tape_end(xt_rt* rt);
+/* x * (1 / s.trace()) with the degeneracy-weighted trace (TensorBase.cpp:26-37), recorded with
+ * the adjoints of x and s */
+int xt_div_trace(const xt_tensor* x, const xt_tensor* s, xt_tensor** out);
+
+/* ---- simple update on a 2x2 unit cell (src/PEPS/TimePropagator.cpp, nearest-neighbour terms) */
+/* TensorBase::mexp (TensorBase.cpp:180-183): exp(factor * block) of every square block (order <= 64) */
+int xt_tensor_mexp(const xt_tensor* t, double factor, xt_tensor** out);
+typedef struct xt_peps xt_peps;
+typedef struct xt_tprop xt_tprop;
+/* iPEPS state with bond weights: A as for xt_ctm_create (site = x*2 + y), Gs = A and identity
+ * weights (iPEPS::initWeightTensors, iPEPS.cpp:575-591); D is the bond dimension that the
+ * update truncates to (iPEPS::D) */
+int xt_peps_create(xt_rt* rt, const xt_tensor* const* A, size_t D, xt_peps** out);
+int xt_peps_destroy(xt_peps* p);
+/* which: 0 = Gs, 1 = whs (bond (x,y)-(x+1,y)), 2 = wvs (bond (x,y-1)-(x,y)), 3 = the weighted site
+ * tensors applyWeights(Gs, sqrt weights) (iPEPS::updateAtensors, iPEPS.cpp:593-602) */
+int xt_peps_tensor(const xt_peps* p, int which, int site, xt_tensor** out);
+/* TimePropagator(H, dt): Hh[x*2+y] and Hv[x*2+y] are the bond Hamiltonians Tensor<2,2>(p, p | p, p)
+ * of the bonds (x,y)-(x+1,y) and (x,y)-(x,y+1); the gates are exp(-dt H) (initU, :453-487) */
+int xt_tprop_create(xt_rt* rt, const xt_tensor* const* Hh, const xt_tensor* const* Hv, double dt, xt_tprop** out);
+int xt_tprop_destroy(xt_tprop* t);
+int xt_tprop_step(xt_tprop* t, xt_peps* p); /* t_step, nearest-neighbour branch (TimePropagator.cpp:78-105) */
+/* one bond update: t_step_h(x, y) (:113-190) or, with vertical != 0, t_step_v(x, y) (:192-226) */
+int xt_tprop_bond(xt_tprop* t, xt_peps* p, int vertical, int x, int y);
+/* which: 0 = horizontal gates, 1 = vertical gates, 2 = spectrum_h, 3 = spectrum_v (the last
+ * un-normalised weights of each bond) */
+int xt_tprop_tensor(const xt_tprop* t, int which, int site, xt_tensor** out);
 
 #ifdef __cplusplus
 }

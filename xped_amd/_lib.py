@@ -203,6 +203,25 @@ _SIGS = [
     ("xt_trace", C.c_int, [_P, _DP]),
     ("xt_squared_norm", C.c_int, [_P, _DP]),
     ("xt_max_norm", C.c_int, [_P, _DP]),
+    ("xt_tape_begin", C.c_int, [_P]),
+    ("xt_tape_pause", C.c_int, [_P, C.c_int]),
+    ("xt_tape_seed", C.c_int, [_P, _P, _P, C.c_double]),
+    ("xt_tape_seed_trace", C.c_int, [_P, _P, C.c_double]),
+    ("xt_tape_backward", C.c_int, [_P]),
+    ("xt_tape_adjoint", C.c_int, [_P, _P, _PP]),
+    ("xt_tape_end", C.c_int, [_P]),
+    ("xt_div_trace", C.c_int, [_P, _P, _PP]),
+    ("xt_ctm_copy_env", C.c_int, [_P, _P]),
+    ("xt_ctm_energy_grad", C.c_int, [_P, C.c_int, C.c_double, C.c_double, _DP, C.POINTER(C.c_void_p)]),
+    ("xt_tensor_mexp", C.c_int, [_P, C.c_double, _PP]),
+    ("xt_peps_create", C.c_int, [_P, C.POINTER(C.c_void_p), _SZ, _PP]),
+    ("xt_peps_destroy", C.c_int, [_P]),
+    ("xt_peps_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
+    ("xt_tprop_create", C.c_int, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_double, _PP]),
+    ("xt_tprop_destroy", C.c_int, [_P]),
+    ("xt_tprop_step", C.c_int, [_P, _P]),
+    ("xt_tprop_bond", C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
+    ("xt_tprop_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
     ("xt_ctm_create", C.c_int, [_P, C.POINTER(C.c_void_p), _SZ, _PP]),
     ("xt_ctm_destroy", C.c_int, [_P]),
     ("xt_ctm_set_options", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

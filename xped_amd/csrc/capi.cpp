@@ -7,6 +7,7 @@
 
 #include "../../include/xped_tensor.h"
 #include "core/ctm.hpp"
+#include "core/simple_update.hpp"
 #include "core/storage.hpp"
 #include "core/ad.hpp"
 #include "core/tensor.hpp"
@@ -15,6 +16,7 @@ struct xt_rt {
     xa::Runtime rt;
     std::string err;
     long live = 0;  // tensor and CTM handles made on this runtime and not yet destroyed
+    std::unique_ptr<xa::Tape> tape;  // set by xt_tape_begin
 };
 struct xt_basis {
     xa::Qbasis b;
@@ -847,6 +849,184 @@ int xt_truncation_rule(int kind0, int kind1, const int* qs, const double* svs, i
 
 }  // extern "C"
 
+extern "C" {
+
+int xt_tape_begin(xt_rt* rt)
+{
+    return run(rt, [&] {
+        if(rt->tape) throw xa::Error(XH_EINVAL, "a tape is already recording on this runtime");
+        rt->tape = std::make_unique<xa::Tape>();
+        rt->tape->active = true;
+        rt->rt.tape = rt->tape.get();
+    });
+}
+
+int xt_tape_pause(xt_rt* rt, int paused)
+{
+    return run(rt, [&] {
+        if(!rt->tape) throw xa::Error(XH_EINVAL, "no tape on this runtime");
+        rt->tape->active = paused == 0;
+    });
+}
+
+int xt_tape_seed(xt_rt* rt, const xt_tensor* y, const xt_tensor* dy, double alpha)
+{
+    return run(rt, [&] {
+        if(!rt->tape) throw xa::Error(XH_EINVAL, "no tape on this runtime");
+        rt->tape->accumulate(y->t, dy->t, alpha);
+    });
+}
+
+int xt_tape_seed_trace(xt_rt* rt, const xt_tensor* y, double g)
+{
+    return run(rt, [&] {
+        if(!rt->tape) throw xa::Error(XH_EINVAL, "no tape on this runtime");
+        xa::DTensor d = rt->tape->adjoint_of(y->t);
+        xa::trace_bwd(d, g);
+    });
+}
+
+int xt_tape_backward(xt_rt* rt)
+{
+    return run(rt, [&] {
+        if(!rt->tape) throw xa::Error(XH_EINVAL, "no tape on this runtime");
+        rt->tape->backward();
+    });
+}
+
+int xt_tape_adjoint(xt_rt* rt, const xt_tensor* x, xt_tensor** out)
+{
+    return run(rt, [&] {
+        if(!rt->tape) throw xa::Error(XH_EINVAL, "no tape on this runtime");
+        const xa::DTensor* d = rt->tape->find(x->t);
+        *out = wrap(rt, d ? *d : xa::DTensor::zeros(&rt->rt, x->t.st));
+    });
+}
+
+int xt_tape_end(xt_rt* rt)
+{
+    rt->rt.tape = nullptr;
+    rt->tape.reset();
+    return XH_OK;
+}
+
+int xt_div_trace(const xt_tensor* x, const xt_tensor* s, xt_tensor** out)
+{
+    return run(x->owner, [&] { *out = wrap(x->owner, xa::div_trace(x->t, s->t)); });
+}
+
+}  // extern "C"
+
+struct xt_peps {
+    xa::WeightedPEPS p;
+    xt_rt* owner = nullptr;
+};
+struct xt_tprop {
+    std::unique_ptr<xa::TimePropagator> tp;
+    xt_rt* owner = nullptr;
+};
+
+extern "C" {
+
+int xt_tensor_mexp(const xt_tensor* t, double factor, xt_tensor** out)
+{
+    return run(t->owner, [&] { *out = wrap(t->owner, xa::mexp(t->t, factor)); });
+}
+
+int xt_peps_create(xt_rt* rt, const xt_tensor* const* A, size_t D, xt_peps** out)
+{
+    return run(rt, [&] {
+        std::vector<xa::DTensor> a;
+        for(int i = 0; i < 4; ++i) a.push_back(A[i]->t);
+        auto* p = new xt_peps();
+        try {
+            p->p = xa::init_weight_tensors(&rt->rt, a, D);
+        } catch(...) {
+            delete p;
+            throw;
+        }
+        p->owner = rt;
+        ++rt->live;
+        *out = p;
+    });
+}
+
+int xt_peps_destroy(xt_peps* p)
+{
+    if(p && p->owner) --p->owner->live;
+    delete p;
+    return XH_OK;
+}
+
+int xt_peps_tensor(const xt_peps* p, int which, int site, xt_tensor** out)
+{
+    return run(p->owner, [&] {
+        if(site < 0 || site >= 4 || which < 0 || which > 3) throw xa::Error(XH_EINVAL, "peps tensor index out of range");
+        if(which == 3) {
+            *out = wrap(p->owner, xa::weighted_sites(p->p)[site]);
+            return;
+        }
+        const std::vector<xa::DTensor>* tabs[3] = {&p->p.Gs, &p->p.whs, &p->p.wvs};
+        *out = wrap(p->owner, (*tabs[which])[site]);
+    });
+}
+
+int xt_tprop_create(xt_rt* rt, const xt_tensor* const* Hh, const xt_tensor* const* Hv, double dt, xt_tprop** out)
+{
+    return run(rt, [&] {
+        std::vector<xa::DTensor> h, v;
+        for(int i = 0; i < 4; ++i) {
+            h.push_back(Hh[i]->t);
+            v.push_back(Hv[i]->t);
+        }
+        auto* t = new xt_tprop();
+        try {
+            t->tp = std::make_unique<xa::TimePropagator>(&rt->rt, h, v, dt);
+        } catch(...) {
+            delete t;
+            throw;
+        }
+        t->owner = rt;
+        ++rt->live;
+        *out = t;
+    });
+}
+
+int xt_tprop_destroy(xt_tprop* t)
+{
+    if(t && t->owner) --t->owner->live;
+    delete t;
+    return XH_OK;
+}
+
+int xt_tprop_step(xt_tprop* t, xt_peps* p)
+{
+    return run(t->owner, [&] { t->tp->t_step(p->p); });
+}
+
+int xt_tprop_bond(xt_tprop* t, xt_peps* p, int vertical, int x, int y)
+{
+    return run(t->owner, [&] {
+        if(vertical)
+            t->tp->t_step_v(p->p, x, y);
+        else
+            t->tp->t_step_h(p->p, x, y);
+    });
+}
+
+int xt_tprop_tensor(const xt_tprop* t, int which, int site, xt_tensor** out)
+{
+    return run(t->owner, [&] {
+        if(site < 0 || site >= 4 || which < 0 || which > 3) throw xa::Error(XH_EINVAL, "propagator tensor index out of range");
+        const std::vector<xa::DTensor>* tabs[4] = {&t->tp->Uh, &t->tp->Uv, &t->tp->spectrum_h, &t->tp->spectrum_v};
+        const xa::DTensor& r = (*tabs[which])[site];
+        if(!r.st) throw xa::Error(XH_EINVAL, "spectrum of a bond that was not updated yet");
+        *out = wrap(t->owner, r);
+    });
+}
+
+}  // extern "C"
+
 struct xt_ctm {
     std::unique_ptr<xa::CTM> ctm;
     xt_rt* owner = nullptr;
@@ -918,6 +1098,24 @@ int xt_ctm_tensor(xt_ctm* c, int which, int site, xt_tensor** out)
         const xa::DTensor& t = (*tabs[which])[site];
         if(!t.st) throw xa::Error(XH_EINVAL, "ctm tensor not computed yet");
         *out = wrap(c->owner, t);
+    });
+}
+
+int xt_ctm_copy_env(xt_ctm* dst, const xt_ctm* src)
+{
+    return run(dst->owner, [&] {
+        xa::CTM::State st = src->ctm->state();
+        if(st.C1.empty() || !st.C1[0].st) throw xa::Error(XH_EINVAL, "copy_env: the source has no environment");
+        dst->ctm->set_state(st);
+    });
+}
+
+int xt_ctm_energy_grad(xt_ctm* c, int steps, double Jxy, double Jz, double* E, xt_tensor** dA)
+{
+    return run(c->owner, [&] {
+        std::vector<xa::DTensor> g;
+        *E = c->ctm->energy_gradient(steps, Jxy, Jz, g);
+        for(int i = 0; i < 4; ++i) dA[i] = wrap(c->owner, g[i]);
     });
 }
 

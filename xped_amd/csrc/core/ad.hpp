@@ -20,7 +20,8 @@ void cutoff_bwd(const DTensor& dUt, const DTensor& dSt, const DTensor& dVt, DTen
 // dA += the QR adjoint of the sector (xh_qr_bwd).
 void qr_bwd(const DTensor& A, const DTensor& Q, const DTensor& R, const DTensor& dQ, const DTensor& dR, DTensor& dA);
 
-// diag_sqrt (mode 0) / diag_inv (mode 1) reverse passes (ADTensor.hpp:560-590); y is the result.
+// diag_sqrt (mode 0) / diag_inv (mode 1) / diag_sqrt().diag_inv() (mode 2) reverse passes
+// (ADTensor.hpp:560-590); y is the result.
 void diag_bwd(const DTensor& y, const DTensor& dy, DTensor& dx, int mode);
 // trace reverse pass (ADTensor.hpp:533-545): dx += g * WeightedIdentity.
 void trace_bwd(DTensor& dx, double g);

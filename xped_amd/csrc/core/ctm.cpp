@@ -5,6 +5,8 @@
 // identity for bosonic symmetries.
 #include "ctm.hpp"
 
+#include "ad.hpp"
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -52,15 +54,7 @@ DTensor CTM::ct(const DTensor& a, std::vector<int> a1, const DTensor& b, std::ve
 
 DTensor CTM::identity(const std::vector<Qbasis>& dom, const std::vector<Qbasis>& cod) const
 {
-    // Tensor::Identity / setIdentity (Tensor.hpp:229-235, Tensor.cpp:58-68): every block gets ones
-    // on its main diagonal
-    StructPtr st = cached_structure(rt, dom.empty() ? cod[0].sym : dom[0].sym, dom, cod);
-    DTensor t = DTensor::empty(rt, st);
-    std::vector<double> h(st->size, 0.0);
-    for(std::size_t i = 0; i < st->sector.size(); ++i)
-        for(std::size_t k = 0; k < std::min(st->rows[i], st->cols[i]); ++k) h[st->offsets[i] + k + k * st->rows[i]] = 1.0;
-    if(st->size) t.upload(h.data(), h.size());
-    return t;
+    return xa::identity(rt, dom, cod);
 }
 
 Qbasis CTM::ket(int x, int y, int leg) const
@@ -99,6 +93,20 @@ DTensor CTM::normalized(const DTensor& t) const
     }
     DTensor out = DTensor::empty(t.rt, t.st);
     if(xh_normalize_max_abs(rt->ctx, t.data(), out.data(), t.size()) != XH_OK) throw Error(XH_EHIP, std::string("CTM normalize: ") + xh_last_error(rt->ctx));
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out](Tape& tape) {
+            // t * (1 / t.maxNorm()) (ADTensor.hpp:450-476 and the scalar product): with m = |t_k| the
+            // largest modulus, dt += dy / m and dt_k += -<dy, t> / m^2 * sign(t_k)
+            const DTensor* d = tape.find(out);
+            if(!d) return;
+            const DTensor dd = *d;
+            const double m = max_norm(t);
+            double dot = 0.0;
+            if(xh_dot(t.rt->ctx, dd.data(), t.data(), t.size(), &dot) != XH_OK) throw Error(XH_EHIP, std::string("CTM normalize adjoint: ") + xh_last_error(t.rt->ctx));
+            tape.accumulate(t, dd, 1.0 / m);
+            DTensor dt = tape.adjoint_of(t);
+            max_norm_bwd(t, dt, -dot / (m * m));
+        });
     return out;
 }
 
@@ -465,9 +473,9 @@ void CTM::grow_all()
 void CTM::compute_rdm()
 {
     auto normalize_rho = [&](DTensor rho) {
+        // rho * (1 / rho.contract(Id2.twist(0).twist(1)).trace())
         DTensor Id2 = identity(rho.st->unc_cod, rho.st->unc_dom);
-        const double norm = trace(ct(rho, {1, 2, 3, 4}, tw(Id2, {0, 1}), {3, 4, 1, 2}, 0));
-        return scaled(rho, 1.0 / norm);
+        return div_trace(rho, ct(rho, {1, 2, 3, 4}, tw(Id2, {0, 1}), {3, 4, 1, 2}, 0));
     };
     for(int x = 0; x < Lx; ++x)
         for(int y = 0; y < Ly; ++y) {
@@ -505,7 +513,7 @@ void CTM::compute_rdm()
         }
 }
 
-double CTM::energy(double Jxy, double Jz) const
+std::vector<DTensor> CTM::bond_terms(double Jxy, double Jz) const
 {
     // Heisenberg gate Jz Sz.Sz + Jxy/2 (S+S- + S-S+) for spin 1/2 (Heisenberg.hpp:47-53), built on
     // the bases of the reduced density matrix; labels of a U(1) leg are 2 Sz (negated on a
@@ -540,11 +548,18 @@ double CTM::energy(double Jxy, double Jz) const
         }
         DTensor gate = DTensor::empty(rt, gs);
         if(gs->size) gate.upload(h.data(), h.size());
-        return trace(ct(tw(rho, {0, 1}), {1, 2, 3, 4}, gate, {3, 4, 1, 2}, 0));
+        return ct(tw(rho, {0, 1}), {1, 2, 3, 4}, gate, {3, 4, 1, 2}, 0);
     };
+    std::vector<DTensor> out;
+    for(int i = 0; i < Lx * Ly; ++i) out.push_back(bond(rho_h[i]));
+    for(int i = 0; i < Lx * Ly; ++i) out.push_back(bond(rho_v[i]));
+    return out;
+}
+
+double CTM::energy(double Jxy, double Jz) const
+{
     double e = 0.0;
-    for(int i = 0; i < Lx * Ly; ++i) e += bond(rho_h[i]);
-    for(int i = 0; i < Lx * Ly; ++i) e += bond(rho_v[i]);
+    for(const DTensor& t : bond_terms(Jxy, Jz)) e += trace(t);
     return e / (Lx * Ly);
 }
 
@@ -553,6 +568,96 @@ double CTM::step()
     grow_all();
     compute_rdm();
     return energy();
+}
+
+CTM::State CTM::state() const
+{
+    return State{C1, C2, C3, C4, T1, T2, T3, T4};
+}
+
+void CTM::set_state(const State& st)
+{
+    C1 = st.C1;
+    C2 = st.C2;
+    C3 = st.C3;
+    C4 = st.C4;
+    T1 = st.T1;
+    T2 = st.T2;
+    T3 = st.T3;
+    T4 = st.T4;
+}
+
+double CTM::energy_gradient(int steps, double Jxy, double Jz, std::vector<DTensor>& dA)
+{
+    // CTMSolver::solve with AD (CTMSolver.cpp:80-137): the environment reached so far is a constant;
+    // `steps` further grow_all moves, computeRDM and the energy are differentiated with respect to
+    // the site tensors.  Each move is a checkpoint (CTM.cpp:386-420 with CP): the forward pass keeps
+    // only the environment before every move, and the reverse pass recomputes one move at a time
+    // under the tape, seeds the adjoints of its outputs with those carried back from the next move,
+    // and runs its reverse passes.
+    if(steps < 1) throw Error(XH_EINVAL, "energy_gradient: at least one tracked step");
+    if(opt.svd != SvdMode::Full) throw Error(XH_EINVAL, "energy_gradient: the reverse pass needs the full block SVD");
+    if(rt->tape) throw Error(XH_EINVAL, "energy_gradient: the runtime is already recording");
+    if(C1.empty() || !C1[0].st) throw Error(XH_EINVAL, "energy_gradient: initialise the environment first");
+    std::vector<State> cps;
+    for(int k = 0; k < steps; ++k) {
+        cps.push_back(state());
+        grow_all();
+    }
+    const State final_state = state();
+    Tape tape;
+    struct Reset {
+        Runtime* r;
+        ~Reset() { r->tape = nullptr; }
+    } reset{rt};
+    rt->tape = &tape;
+    const std::vector<DTensor> Adag0 = Adags;
+    dA.assign(As.size(), DTensor());
+    for(std::size_t i = 0; i < As.size(); ++i) dA[i] = DTensor::zeros(rt, As[i].st);
+    State carried;  // adjoints of the environment after the move being reversed
+    double E = 0.0;
+    for(int k = steps - 1; k >= 0; --k) {
+        set_state(cps[k]);
+        tape.clear();
+        tape.active = true;
+        // iPEPS::updateAdags under the tape, so that the adjoint of A^dagger reaches A
+        for(std::size_t i = 0; i < As.size(); ++i) Adags[i] = permute(adjoint(As[i]), 0, {3, 4, 2, 0, 1});
+        grow_all();
+        if(k == steps - 1) {
+            compute_rdm();
+            const std::vector<DTensor> terms = bond_terms(Jxy, Jz);
+            tape.active = false;
+            for(const DTensor& t : terms) {
+                E += trace(t);
+                DTensor dt = tape.adjoint_of(t);
+                trace_bwd(dt, 1.0 / (Lx * Ly));
+            }
+            E /= (Lx * Ly);
+        } else {
+            tape.active = false;
+            const State now = state();
+            const std::vector<DTensor>* outs[8] = {&now.C1, &now.C2, &now.C3, &now.C4, &now.T1, &now.T2, &now.T3, &now.T4};
+            const std::vector<DTensor>* adjs[8] = {&carried.C1, &carried.C2, &carried.C3, &carried.C4, &carried.T1, &carried.T2, &carried.T3, &carried.T4};
+            for(int w = 0; w < 8; ++w)
+                for(std::size_t i = 0; i < outs[w]->size(); ++i)
+                    if((*adjs[w])[i].st) tape.accumulate((*outs[w])[i], (*adjs[w])[i]);
+        }
+        tape.backward();
+        const State& in = cps[k];
+        const std::vector<DTensor>* ins[8] = {&in.C1, &in.C2, &in.C3, &in.C4, &in.T1, &in.T2, &in.T3, &in.T4};
+        std::vector<DTensor>* cad[8] = {&carried.C1, &carried.C2, &carried.C3, &carried.C4, &carried.T1, &carried.T2, &carried.T3, &carried.T4};
+        for(int w = 0; w < 8; ++w) {
+            cad[w]->assign(ins[w]->size(), DTensor());
+            for(std::size_t i = 0; i < ins[w]->size(); ++i)
+                if(const DTensor* d = tape.find((*ins[w])[i])) (*cad[w])[i] = *d;
+        }
+        for(std::size_t i = 0; i < As.size(); ++i)
+            if(const DTensor* d = tape.find(As[i])) dA[i] = axpby(*d, dA[i], 1.0, 1.0);
+    }
+    tape.clear();
+    Adags = Adag0;
+    set_state(final_state);
+    return E;
 }
 
 }  // namespace xa

@@ -94,7 +94,20 @@ public:
     void grow_all();
     void compute_rdm();
     double energy(double Jxy = 1.0, double Jz = 1.0) const;  // (sum E_h + sum E_v) / 4
+    // the rank-0 tensors rho.contract(gate) of the 4 horizontal then 4 vertical bonds, whose
+    // traces are the bond energies
+    std::vector<DTensor> bond_terms(double Jxy = 1.0, double Jz = 1.0) const;
     double step();  // grow_all + computeRDM + energy
+    // energy after `steps` more grow_all moves and its gradient with respect to the four site
+    // tensors (dA[site], structures of A); the environment on entry is a constant.  Leaves the
+    // environment after the moves, as step() would (without the density matrices).
+    double energy_gradient(int steps, double Jxy, double Jz, std::vector<DTensor>& dA);
+
+    struct State {
+        std::vector<DTensor> C1, C2, C3, C4, T1, T2, T3, T4;
+    };
+    State state() const;
+    void set_state(const State& s);
 
     const DTensor& A(int x, int y) const { return As[idx(x, y)]; }
     const DTensor& Adag(int x, int y) const { return Adags[idx(x, y)]; }

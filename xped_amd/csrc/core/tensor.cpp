@@ -1,6 +1,8 @@
 // Device-resident tensor operations: structure on the host, data movement and arithmetic on the GPU.
 #include "tensor.hpp"
 
+#include "ad.hpp"
+
 #include "../common/pool.hpp"
 
 #include <cstddef>
@@ -365,6 +367,11 @@ DTensor permute(const DTensor& t, int shift, const std::vector<int>& p)
         t.rt->permute_bytes += b;
         t.rt->n_permute++;
     }
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out, shift, p](Tape& tape) {
+            // ADTensor.hpp:171-185: curr.adj() += res.adj().permute_adj<shift, p...>()
+            if(const DTensor* d = tape.find(out)) tape.accumulate(t, permute_adj(*d, shift, p));
+        });
     return out;
 }
 
@@ -436,6 +443,16 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision)
     auto cg = get_gemm_plan(l.rt, *l.st, *r.st);
     DTensor out = DTensor::empty(l.rt, cg->plan.out);
     launch_mul(cg.get(), l, r, out, precision);
+    if(Tape* tp = recording(l))
+        tp->backs.push_back([l, r, out, precision](Tape& tape) {
+            // ADTensor.hpp:745-746: left.adj() += res.adj() * right^dagger, right.adj() += left^dagger * res.adj()
+            const DTensor* d = tape.find(out);
+            if(!d) return;
+            const DTensor dc = *d;
+            DTensor dl = tape.adjoint_of(l);
+            DTensor dr = tape.adjoint_of(r);
+            mul_bwd(l, r, dc, &dl, &dr, precision);
+        });
     return out;
 }
 
@@ -495,6 +512,11 @@ DTensor adjoint(const DTensor& t)
         }
         dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, t.data(), out.data(), rects.data(), static_cast<int64_t>(rects.size())), "adjoint");
     }
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out](Tape& tape) {
+            // ADTensor.hpp:138-150
+            if(const DTensor* d = tape.find(out)) tape.accumulate(t, adjoint(*d));
+        });
     return out;
 }
 
@@ -528,6 +550,11 @@ DTensor twist(const DTensor& t, int leg)
         }
     }
     if(t.rt->ctx && !rects.empty()) dcheck(t.rt->ctx, xh_scale_rects(t.rt->ctx, out.data(), rects.data(), static_cast<int64_t>(rects.size())), "twist");
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out, leg](Tape& tape) {
+            // ADTensor.hpp:592-602: the twist is a diagonal sign map, its own transpose
+            if(const DTensor* d = tape.find(out)) tape.accumulate(t, twist(*d, leg));
+        });
     return out;
 }
 
@@ -535,6 +562,10 @@ DTensor scaled(const DTensor& t, double alpha)
 {
     DTensor out = DTensor::empty(t.rt, t.st);
     if(t.rt->ctx && out.size() > 0) dcheck(t.rt->ctx, xh_axpby(t.rt->ctx, out.data(), t.data(), t.size(), alpha, 0.0), "scale");
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out, alpha](Tape& tape) {
+            if(const DTensor* d = tape.find(out)) tape.accumulate(t, *d, alpha);
+        });
     return out;
 }
 
@@ -543,6 +574,14 @@ DTensor axpby(const DTensor& x, const DTensor& y, double a, double b)
     if(structure_key(*x.st) != structure_key(*y.st)) throw Error(XH_EINVAL, "binary op: tensors have different structure");
     DTensor out = copy_of(y);
     if(x.rt->ctx && out.size() > 0) dcheck(x.rt->ctx, xh_axpby(x.rt->ctx, out.data(), x.data(), x.size(), a, b), "axpby");
+    if(Tape* tp = recording(x))
+        tp->backs.push_back([x, y, out, a, b](Tape& tape) {
+            const DTensor* d = tape.find(out);
+            if(!d) return;
+            const DTensor dd = *d;
+            tape.accumulate(x, dd, a);
+            tape.accumulate(y, dd, b);
+        });
     return out;
 }
 
@@ -559,6 +598,43 @@ DTensor diag_map(const DTensor& t, int mode)
     // the reference's result is the diagonal matrix of the mapped diagonal (diagUnaryFunc)
     if(t.rt->ctx && !off.empty())
         dcheck(t.rt->ctx, xh_block_diag_map(t.rt->ctx, off.data(), m.data(), static_cast<int64_t>(off.size()), out.data(), mode | 4), "diag map");
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, out, mode](Tape& tape) {
+            // ADTensor.hpp:560-590 (diagonal entries only)
+            const DTensor* d = tape.find(out);
+            if(!d) return;
+            const DTensor dd = *d;
+            DTensor dt = tape.adjoint_of(t);
+            diag_bwd(out, dd, dt, mode);
+        });
+    return out;
+}
+
+DTensor identity(Runtime* rt, const std::vector<Qbasis>& dom, const std::vector<Qbasis>& cod)
+{
+    // Tensor::Identity / setIdentity (Tensor.hpp:229-235, Tensor.cpp:58-68): every block gets ones
+    // on its main diagonal; the pattern is written on the host and uploaded
+    if(dom.empty() && cod.empty()) throw Error(XH_EINVAL, "identity needs at least one leg");
+    StructPtr st = cached_structure(rt, dom.empty() ? cod[0].sym : dom[0].sym, dom, cod);
+    DTensor t = DTensor::empty(rt, st);
+    std::vector<double> h(st->size, 0.0);
+    for(std::size_t i = 0; i < st->sector.size(); ++i)
+        for(std::size_t k = 0; k < std::min(st->rows[i], st->cols[i]); ++k) h[st->offsets[i] + k + k * st->rows[i]] = 1.0;
+    if(st->size) t.upload(h.data(), h.size());
+    return t;
+}
+
+DTensor mexp(const DTensor& t, double factor)
+{
+    t.rt->require_device();
+    DTensor out = DTensor::empty(t.rt, t.st);
+    std::vector<xh_mexp_item> items;
+    for(std::size_t i = 0; i < t.st->sector.size(); ++i) {
+        if(t.st->rows[i] != t.st->cols[i]) throw Error(XH_EINVAL, "mexp needs square blocks");
+        if(t.st->rows[i] > XH_MEXP_MAX) throw Error(XH_EINVAL, "mexp: blocks larger than 64 x 64 are not supported");
+        items.push_back(xh_mexp_item{static_cast<int64_t>(t.st->offsets[i]), static_cast<int64_t>(t.st->offsets[i]), static_cast<int32_t>(t.st->rows[i]), 0});
+    }
+    if(!items.empty()) dcheck(t.rt->ctx, xh_block_mexp(t.rt->ctx, items.data(), static_cast<int64_t>(items.size()), t.data(), out.data(), factor), "mexp");
     return out;
 }
 
@@ -603,40 +679,44 @@ double max_norm(const DTensor& t)
 
 // ------------------------------------------------------------------ tSVD
 
-TruncRule truncation_rule(const Sym& s, std::vector<std::pair<Q, double>> allSV, std::size_t maxKeep, double eps_svd, bool preserve_multiplets,
-                          double truncWeight, double entropy)
+TruncRule truncation_rule(const Sym& s, std::vector<std::pair<Q, double>> spectrum, std::size_t max_keep, double eps_svd, bool preserve_multiplets,
+                          double trunc_weight, double entropy)
 {
-    TruncRule out;
-    const std::size_t numberOfStates = allSV.size();
-    if(numberOfStates == 0) throw Error(XH_EINVAL, "tSVD of a tensor without sectors");
+    // The selection of cutoff_matrices (Tensor.cpp:535-598).  The candidates are ordered by
+    // std::sort with a descending-value comparator, as there, so that equal values end up in
+    // libstdc++'s introsort order and the same ones are kept.
+    using Entry = std::pair<Q, double>;
+    if(spectrum.empty()) throw Error(XH_EINVAL, "tSVD of a tensor without sectors");
     // std::sort needs a strict weak order: NaN (from non-finite input data) would break it
-    for(const auto& e : allSV)
+    for(const Entry& e : spectrum)
         if(!std::isfinite(e.second)) throw Error(XH_EINVAL, "tSVD: non-finite singular value (non-finite tensor data)");
-    const auto first_entry = allSV[0];
-    std::sort(allSV.begin(), allSV.end(), [](const std::pair<Q, double>& a, const std::pair<Q, double>& b) { return a.second > b.second; });
-    for(std::size_t i = maxKeep; i < allSV.size(); ++i) truncWeight += s.degeneracy(allSV[i].first) * std::pow(std::abs(allSV[i].second), 2.);
-    allSV.resize(std::min(maxKeep, numberOfStates));
+    const Entry fallback = spectrum.front();  // kept alone when every value is cut
+    const std::size_t n_all = spectrum.size();
+    std::sort(spectrum.begin(), spectrum.end(), [](const Entry& a, const Entry& b) { return a.second > b.second; });
+    // discarded weight: every value past the first max_keep, times its multiplet degeneracy
+    for(std::size_t k = max_keep; k < n_all; ++k) trunc_weight += s.degeneracy(spectrum[k].first) * std::pow(std::abs(spectrum[k].second), 2.);
+    spectrum.resize(std::min(max_keep, n_all));
     if(eps_svd > 0.) {
-        allSV.erase(std::remove_if(allSV.begin(), allSV.end(), [eps_svd](const std::pair<Q, double>& sv) { return sv.second < eps_svd; }),
-                    allSV.end());
+        auto below = [eps_svd](const Entry& e) { return e.second < eps_svd; };
+        spectrum.erase(std::remove_if(spectrum.begin(), spectrum.end(), below), spectrum.end());
     }
     if(preserve_multiplets) {
-        int endOfMultiplet = -1;
-        for(int i = static_cast<int>(allSV.size()) - 1; i > 0; i--) {
-            double rel_diff = 2 * (allSV[i - 1].second - allSV[i].second) / (allSV[i - 1].second + allSV[i].second);
-            if(rel_diff > 0.1) {
-                endOfMultiplet = i;
+        // cut after the smallest-value position whose relative gap to the value above exceeds 10 %
+        for(std::size_t k = spectrum.size(); k-- > 1;) {
+            const double above = spectrum[k - 1].second, here = spectrum[k].second;
+            if(2 * (above - here) / (above + here) > 0.1) {
+                spectrum.resize(k);
                 break;
             }
         }
-        if(endOfMultiplet != -1) allSV.resize(endOfMultiplet);
     }
-    if(allSV.empty()) allSV.push_back(first_entry);
-    for(const auto& [q, sv] : allSV) {
-        out.kept[q].push_back(sv);
-        entropy += -s.degeneracy(q) * sv * sv * std::log(sv * sv);
+    if(spectrum.empty()) spectrum.push_back(fallback);
+    TruncRule out;
+    for(const auto& [q, sigma] : spectrum) {
+        out.kept[q].push_back(sigma);
+        entropy += -s.degeneracy(q) * sigma * sigma * std::log(sigma * sigma);
     }
-    out.trunc_weight = truncWeight;
+    out.trunc_weight = trunc_weight;
     out.entropy = entropy;
     return out;
 }
@@ -680,6 +760,14 @@ FullSVD full_svd(const DTensor& t)
         }
     }
     if(f.S.size()) f.S.upload(hs.data(), hs.size());
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, U = f.U, S = f.S, V = f.Vdag](Tape& tape) {
+            // ADTensor.hpp:262-320
+            if(!tape.find(U) && !tape.find(S) && !tape.find(V)) return;
+            const DTensor dU = tape.adjoint_of(U), dS = tape.adjoint_of(S), dV = tape.adjoint_of(V);
+            DTensor dt = tape.adjoint_of(t);
+            svd_bwd(U, S, V, dU, dS, dV, dt);
+        });
     return f;
 }
 
@@ -720,6 +808,14 @@ SVDResult cutoff(const DTensor& t, const FullSVD& f, std::size_t maxKeep, double
         dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, f.Vdag.data(), res.Vdag.data(), rv.data(), static_cast<int64_t>(rv.size())), "truncate Vdag");
     }
     if(res.S.size()) res.S.upload(hs.data(), hs.size());
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([U = f.U, S = f.S, V = f.Vdag, Ut = res.U, St = res.S, Vt = res.Vdag](Tape& tape) {
+            // ADTensor.hpp:357-370
+            if(!tape.find(Ut) && !tape.find(St) && !tape.find(Vt)) return;
+            const DTensor dUt = tape.adjoint_of(Ut), dSt = tape.adjoint_of(St), dVt = tape.adjoint_of(Vt);
+            DTensor dU = tape.adjoint_of(U), dS = tape.adjoint_of(S), dV = tape.adjoint_of(V);
+            cutoff_bwd(dUt, dSt, dVt, dU, dS, dV);
+        });
     return res;
 }
 
@@ -758,8 +854,17 @@ std::pair<DTensor, DTensor> tqr(const DTensor& t, bool return_lq)
         }
         dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, Qt.data(), Ql.data(), rq.data(), static_cast<int64_t>(rq.size())), "transpose Q");
         dcheck(t.rt->ctx, xh_copy_rects(t.rt->ctx, Rt.data(), Rl.data(), rr.data(), static_cast<int64_t>(rr.size())), "transpose R");
+        if(recording(t)) throw Error(XH_EINVAL, "tQR(RETURN_LQ) has no recorded reverse pass; record tQR of the adjoint instead");
         return {Ql, Rl};
     }
+    if(Tape* tp = recording(t))
+        tp->backs.push_back([t, Qt, Rt](Tape& tape) {
+            // the QR adjoint (the reference's AD tensor records no tQR)
+            if(!tape.find(Qt) && !tape.find(Rt)) return;
+            const DTensor dQ = tape.adjoint_of(Qt), dR = tape.adjoint_of(Rt);
+            DTensor dt = tape.adjoint_of(t);
+            qr_bwd(t, Qt, Rt, dQ, dR, dt);
+        });
     return {Qt, Rt};
 }
 
@@ -792,6 +897,77 @@ std::pair<DTensor, DTensor> teigh(const DTensor& t)
         for(std::size_t k = 0; k < s.rows[i]; ++k) hd[D.st->offsets[i] + k + k * s.rows[i]] = ev[ev_off[i] + k];
     if(D.size()) D.upload(hd.data(), hd.size());
     return {D, V};
+}
+
+// ------------------------------------------------------------------ reverse-mode recording
+
+const DTensor* Tape::find(const DTensor& x) const
+{
+    if(!x.buf) return nullptr;
+    auto it = adj.find(x.buf.get());
+    return it == adj.end() ? nullptr : &it->second.adj;
+}
+
+DTensor Tape::adjoint_of(const DTensor& x)
+{
+    if(!x.buf) throw Error(XH_EINVAL, "adjoint of a tensor without storage");
+    auto it = adj.find(x.buf.get());
+    if(it != adj.end()) return it->second.adj;
+    DTensor z = DTensor::zeros(x.rt, x.st);
+    adj.emplace(x.buf.get(), Entry{x, z});
+    return z;
+}
+
+void Tape::accumulate(const DTensor& x, const DTensor& dx, double alpha)
+{
+    if(!x.buf) return;
+    const TensorStructure &a = *x.st, &b = *dx.st;
+    if(a.sector != b.sector || a.rows != b.rows || a.cols != b.cols || a.size != b.size)
+        throw Error(XH_EINVAL, "reverse pass: adjoint does not have the block structure of its value");
+    auto it = adj.find(x.buf.get());
+    if(it == adj.end()) {
+        DTensor c = DTensor::empty(x.rt, x.st);
+        if(x.rt->ctx && c.size() > 0) dcheck(x.rt->ctx, xh_axpby(x.rt->ctx, c.data(), dx.data(), c.size(), alpha, 0.0), "adjoint");
+        adj.emplace(x.buf.get(), Entry{x, c});
+        return;
+    }
+    DTensor& d = it->second.adj;
+    if(x.rt->ctx && d.size() > 0) dcheck(x.rt->ctx, xh_axpby(x.rt->ctx, d.data(), dx.data(), d.size(), alpha, 1.0), "adjoint accumulation");
+}
+
+void Tape::backward()
+{
+    active = false;
+    for(std::size_t i = backs.size(); i-- > 0;) backs[i](*this);
+    backs.clear();
+}
+
+void Tape::clear()
+{
+    active = false;
+    backs.clear();
+    adj.clear();
+}
+
+DTensor div_trace(const DTensor& x, const DTensor& s)
+{
+    const double tr = trace(s);
+    if(!(tr != 0.0) || !std::isfinite(tr)) throw Error(XH_EINVAL, "division by a trace of " + std::to_string(tr));
+    DTensor out = DTensor::empty(x.rt, x.st);
+    if(x.rt->ctx && out.size() > 0) dcheck(x.rt->ctx, xh_axpby(x.rt->ctx, out.data(), x.data(), x.size(), 1.0 / tr, 0.0), "divide by trace");
+    if(Tape* tp = recording(x))
+        tp->backs.push_back([x, s, out, tr](Tape& tape) {
+            // y = x / tr(s): dx += dy / tr, ds += -<dy, x> / tr^2 * WeightedIdentity
+            const DTensor* d = tape.find(out);
+            if(!d) return;
+            const DTensor dd = *d;
+            double dot = 0.0;
+            if(x.size()) dcheck(x.rt->ctx, xh_dot(x.rt->ctx, dd.data(), x.data(), x.size(), &dot), "dot");
+            tape.accumulate(x, dd, 1.0 / tr);
+            DTensor ds = tape.adjoint_of(s);
+            trace_bwd(ds, -dot / (tr * tr));
+        });
+    return out;
 }
 
 }  // namespace xa

@@ -2,6 +2,7 @@
 // (include/Xped/Core/Tensor.hpp:59-444, TensorBase.hpp:47-135) on top of the HIP device layer.
 #pragma once
 #include <array>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -53,6 +54,8 @@ struct CachedGemm {
     std::uint64_t last_use = 0;
 };
 
+struct Tape;
+
 // Device context, plan caches and counters.  ctx == nullptr is a structure-only runtime: every
 // integer/structure operation works, data operations fail with XH_ENODEV.
 struct Runtime {
@@ -83,6 +86,8 @@ struct Runtime {
     // of their block shapes (m, n, k); off by default (it synchronises after every launch)
     bool time_gemm = false;
     std::vector<std::array<int32_t, 3>> gemm_log;
+    // reverse-mode recording: while tape->active, every differentiable operation pushes its reverse pass
+    Tape* tape = nullptr;
     ~Runtime();
     void require_device() const;
 };
@@ -111,6 +116,35 @@ public:
     void set_random(std::mt19937& engine, int dist);
 };
 
+// Reverse-mode recording of the tensor operations: the role of stan-math's reverse-pass callback
+// stack in the reference's AD tensor (include/Xped/AD/ADTensor.hpp, vari_value.hpp:97-165).  While
+// a Tape is set on a runtime and active, permute, operator*, adjoint, twist, scaled, axpby,
+// diag_map, full_svd, cutoff (and so contract and tsvd), tqr and div_trace push their reverse
+// passes; backward() runs them in reverse order.  Adjoints are keyed by the value's device buffer
+// and accumulate (+=) as the reference's callbacks do.  Operations that return their input (an
+// identity permutation, a bosonic twist) record nothing: value and adjoint are shared.
+struct Tape {
+    bool active = false;
+    std::vector<std::function<void(Tape&)>> backs;
+    struct Entry {
+        DTensor value, adj;
+    };
+    std::unordered_map<const DeviceBuffer*, Entry> adj;
+
+    const DTensor* find(const DTensor& x) const;  // nullptr when no adjoint reached x
+    DTensor adjoint_of(const DTensor& x);         // created as zeros on first use; shares the stored buffer
+    void accumulate(const DTensor& x, const DTensor& dx, double alpha = 1.0);  // adj(x) += alpha dx
+    void backward();                              // stops recording, runs the reverse passes, drops them
+    void clear();                                 // drops reverse passes and adjoints
+};
+inline Tape* recording(const DTensor& t)
+{
+    return (t.rt && t.rt->tape && t.rt->tape->active && t.buf) ? t.rt->tape : nullptr;
+}
+// x / trace(s) with s a square tensor (the weighted trace of TensorBase.cpp:26-37); recorded with
+// the adjoint of both x and s.
+DTensor div_trace(const DTensor& x, const DTensor& s);
+
 DTensor permute(const DTensor& t, int shift, const std::vector<int>& p);
 // Y.permute_adj<shift, p...>() (Tensor.cpp:258-333): the transpose of X -> X.permute<shift, p...>().
 DTensor permute_adj(const DTensor& y, int shift, const std::vector<int>& p);
@@ -125,6 +159,11 @@ DTensor twist(const DTensor& t, int leg);
 DTensor scaled(const DTensor& t, double alpha);
 DTensor axpby(const DTensor& x, const DTensor& y, double a, double b);  // a*x + b*y, same structure
 DTensor diag_map(const DTensor& t, int mode);                          // 0 sqrt, 1 inv, 2 inv sqrt of diagonal blocks
+// Tensor::Identity(dom, cod) (Tensor.hpp:229-235)
+DTensor identity(Runtime* rt, const std::vector<Qbasis>& dom, const std::vector<Qbasis>& cod);
+// TensorBase::mexp(factor) (TensorBase.cpp:180-183): exp(factor * block) of every square block
+// (order <= 64)
+DTensor mexp(const DTensor& t, double factor);
 double trace(const DTensor& t);                                         // degeneracy-weighted (TensorBase.cpp:26-37)
 double squared_norm(const DTensor& t);                                  // (T * T^dagger).trace()
 double max_norm(const DTensor& t);                                      // TensorBase.cpp:48-56

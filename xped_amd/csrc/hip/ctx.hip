@@ -499,6 +499,97 @@ __global__ void k_diag_map(const int64_t* __restrict__ off, const int32_t* __res
     }
 }
 
+// exp(factor * M) of square blocks of order <= XH_MEXP_MAX (TensorBase::mexp, TensorBase.cpp:180-183:
+// Eigen's matrix exponential of each block, MatrixInterface_Eigen_impl.hpp:177-180).  Scaling and
+// squaring: B = factor * M / 2^s with ||B||_1 <= 1/2, exp(B) by its Taylor sum to degree
+// MEXP_DEGREE (remainder below 0.5^19 / 19! ~ 1.6e-23 relative to ||exp(B)||), then s squarings.
+// One workgroup per block; the three m x m matrices live in LDS and each thread keeps its share of
+// a product in registers until the barrier that frees the operand it overwrites.
+constexpr int MEXP_THREADS = 256;
+constexpr int MEXP_PER_THREAD = XH_MEXP_MAX * XH_MEXP_MAX / MEXP_THREADS;
+constexpr int MEXP_DEGREE = 18;
+static_assert(XH_MEXP_MAX * XH_MEXP_MAX % MEXP_THREADS == 0, "mexp tile split");
+
+__device__ inline void mexp_product(const double* X, const double* Y, int m, double* acc)
+{
+    const int n2 = m * m;
+#pragma unroll
+    for(int r = 0; r < MEXP_PER_THREAD; ++r) {
+        const int e = threadIdx.x + r * MEXP_THREADS;
+        double v = 0.0;
+        if(e < n2) {
+            const int i = e % m, j = e / m;
+            for(int l = 0; l < m; ++l) v += X[i + l * m] * Y[l + j * m];
+        }
+        acc[r] = v;
+    }
+}
+
+__global__ __launch_bounds__(MEXP_THREADS) void k_block_mexp(const xh_mexp_item* __restrict__ items, const double* src, double* dst,
+                                                            double factor)
+{
+    __shared__ double B[XH_MEXP_MAX * XH_MEXP_MAX], T[XH_MEXP_MAX * XH_MEXP_MAX], S[XH_MEXP_MAX * XH_MEXP_MAX];
+    __shared__ double colsum[XH_MEXP_MAX];
+    __shared__ int nsq;
+    const xh_mexp_item it = items[blockIdx.x];
+    const int m = it.m, n2 = m * m;
+    for(int e = threadIdx.x; e < n2; e += MEXP_THREADS) B[e] = factor * src[it.src_off + e];
+    __syncthreads();
+    if(threadIdx.x < m) {
+        double c = 0.0;
+        for(int i = 0; i < m; ++i) c += fabs(B[i + threadIdx.x * m]);
+        colsum[threadIdx.x] = c;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+        double nrm = 0.0;
+        for(int j = 0; j < m; ++j) nrm = fmax(nrm, colsum[j]);
+        int s = 0;
+        if(nrm > 0.5 && isfinite(nrm)) {
+            int ex = 0;
+            frexp(nrm, &ex);  // nrm = f 2^ex, f in [1/2, 1): nrm / 2^(ex + 1) < 1/2
+            s = ex + 1;
+        }
+        nsq = s;
+    }
+    __syncthreads();
+    const double down = ldexp(1.0, -nsq);
+    for(int e = threadIdx.x; e < n2; e += MEXP_THREADS) {
+        const double b = B[e] * down;
+        B[e] = b;
+        T[e] = b;
+        S[e] = b + ((e % m) == (e / m) ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    double acc[MEXP_PER_THREAD];
+    for(int k = 2; k <= MEXP_DEGREE; ++k) {
+        mexp_product(T, B, m, acc);  // T_k = T_{k-1} B / k
+        __syncthreads();
+        const double inv_k = 1.0 / k;
+#pragma unroll
+        for(int r = 0; r < MEXP_PER_THREAD; ++r) {
+            const int e = threadIdx.x + r * MEXP_THREADS;
+            if(e < n2) {
+                const double t = acc[r] * inv_k;
+                T[e] = t;
+                S[e] += t;
+            }
+        }
+        __syncthreads();
+    }
+    for(int q = 0; q < nsq; ++q) {
+        mexp_product(S, S, m, acc);
+        __syncthreads();
+#pragma unroll
+        for(int r = 0; r < MEXP_PER_THREAD; ++r) {
+            const int e = threadIdx.x + r * MEXP_THREADS;
+            if(e < n2) S[e] = acc[r];
+        }
+        __syncthreads();
+    }
+    for(int e = threadIdx.x; e < n2; e += MEXP_THREADS) dst[it.dst_off + e] = S[e];
+}
+
 // Reverse passes of the diagonal maps (ADTensor.hpp:560-590), diagonal entries only, as the
 // reference's diagonal expressions are diagonal matrices: mode 0 (diag_sqrt, y = sqrt(x)):
 // dx_ii += dy_ii * 0.5 / y_ii;  mode 1 (diag_inv, y = 1/x): dx_ii += -dy_ii * y_ii^2.
@@ -511,7 +602,14 @@ __global__ void k_diag_bwd(const int64_t* __restrict__ off, const int32_t* __res
     for(int i = threadIdx.x; i < mm; i += blockDim.x) {
         const int64_t e = o + int64_t(i) * mm + i;
         const double yv = y[e], g1 = dy[e];
-        dx[e] += mode == 0 ? g1 * (1.0 / yv) * 0.5 : g1 * (-(yv * yv));
+        double d;
+        if(mode == 0)
+            d = g1 * (1.0 / yv) * 0.5;  // y = sqrt(x)
+        else if(mode == 1)
+            d = g1 * (-(yv * yv));  // y = 1 / x
+        else
+            d = g1 * (-0.5 * yv * yv * yv);  // y = 1 / sqrt(x)
+        dx[e] += d;
     }
 }
 
@@ -769,11 +867,33 @@ extern "C" int xh_block_diag_map(xh_ctx* ctx, const int64_t* off, const int32_t*
     });
 }
 
+extern "C" int xh_block_mexp(xh_ctx* ctx, const xh_mexp_item* items, int64_t nitems, const double* src, double* dst, double factor)
+{
+    return guard(ctx, [&] {
+        if(nitems <= 0) return;
+        std::vector<xh_mexp_item> v;
+        for(int64_t i = 0; i < nitems; ++i) {
+            const xh_mexp_item& it = items[i];
+            if(it.m < 0 || it.m > XH_MEXP_MAX) throw xh::Error(XH_EINVAL, "mexp: block order must be in [0, 64]");
+            if(it.src_off < 0 || it.dst_off < 0) throw xh::Error(XH_EINVAL, "mexp: negative offset");
+            if(it.m > 0) v.push_back(it);
+        }
+        if(v.empty()) return;
+        xh::PackedUpload up;
+        const size_t o = up.add(v);
+        up.upload(ctx);
+        hipLaunchKernelGGL(k_block_mexp, dim3(static_cast<unsigned>(v.size())), dim3(MEXP_THREADS), 0, ctx->stream, up.ptr<const xh_mexp_item>(o), src, dst,
+                           factor);
+        xh::launch_check("k_block_mexp");
+        up.release_deferred(ctx);
+    });
+}
+
 extern "C" int xh_block_diag_bwd(xh_ctx* ctx, const int64_t* off, const int32_t* m, int64_t nblocks, const double* y, const double* dy, double* dx, int mode)
 {
     return guard(ctx, [&] {
         if(nblocks <= 0) return;
-        if(mode != 0 && mode != 1) throw xh::Error(XH_EINVAL, "diag adjoint mode must be 0 (sqrt) or 1 (inverse)");
+        if(mode < 0 || mode > 2) throw xh::Error(XH_EINVAL, "diag adjoint mode must be 0 (sqrt), 1 (inverse) or 2 (inverse sqrt)");
         std::vector<int64_t> o(off, off + nblocks);
         std::vector<int32_t> mm(m, m + nblocks);
         xh::PackedUpload up;

@@ -318,6 +318,19 @@ class Tensor:
             data[off + np.arange(k) * (r + 1)] = 1.0
         return t.upload(data)
 
+    def div_trace(self, s: "Tensor") -> "Tensor":
+        """self * (1 / s.trace()) (degeneracy-weighted trace), recorded with both adjoints."""
+        h = C.c_void_p()
+        check(lib().xt_div_trace(self.h, s.h, C.byref(h)), self.rt.h, "div_trace")
+        return self._wrap(h)
+
+    def mexp(self, factor: float) -> "Tensor":
+        """TensorBase::mexp(factor) (TensorBase.cpp:180-183): exp(factor * block) of every square
+        block (order <= 64)."""
+        h = C.c_void_p()
+        check(lib().xt_tensor_mexp(self.h, float(factor), C.byref(h)), self.rt.h, "mexp")
+        return self._wrap(h)
+
     def __del__(self):
         h = getattr(self, "h", None)
         f = _lib.native("xt_tensor_destroy") if (_lib is not None and h) else None
@@ -719,6 +732,19 @@ class CTM:
         check(lib().xt_ctm_tensor(self.h, names.index(which), site, C.byref(h)), self.rt.h, "ctm tensor")
         return Tensor(self.rt, handle=h)
 
+    def copy_env(self, other: "CTM"):
+        """Take other's environment tensors (shared, not copied)."""
+        check(lib().xt_ctm_copy_env(self.h, other.h), self.rt.h, "copy_env")
+        return self
+
+    def energy_grad(self, steps: int = 1, Jxy: float = 1.0, Jz: float = 1.0):
+        """Energy after `steps` more grow_all moves and its gradient with respect to the four site
+        tensors (CTMSolver::solve with AD, each move a checkpoint).  Returns (E, [dA per site])."""
+        E = C.c_double()
+        out = (C.c_void_p * 4)()
+        check(lib().xt_ctm_energy_grad(self.h, int(steps), float(Jxy), float(Jz), C.byref(E), out), self.rt.h, "energy_grad")
+        return E.value, [Tensor(self.rt, handle=C.c_void_p(out[i])) for i in range(4)]
+
     def svd_ms(self) -> float:
         v = C.c_double()
         lib().xt_ctm_svd_ms(self.h, C.byref(v))
@@ -753,3 +779,161 @@ class CTM:
             qs[2 * i] = q if isinstance(q, int) else q[0]
             rk[i] = int(r)
         check(lib().xt_ctm_set_ranks(self.h, site, direction, qs, rk, n), self.rt.h, "set_ranks")
+
+
+def _handles(ts: Sequence[Tensor]):
+    return (C.c_void_p * len(ts))(*[t.h.value if isinstance(t.h, C.c_void_p) else t.h for t in ts])
+
+
+class WeightedPEPS:
+    """The weighted iPEPS state of the simple update: bare sites Gs, bond weights whs/wvs and the
+    bond dimension D (iPEPS::initWeightTensors, iPEPS.cpp:575-591).  Site index x * 2 + y."""
+
+    def __init__(self, rt: Runtime, A: Sequence[Tensor], D: int):
+        self.rt = rt
+        h = C.c_void_p()
+        check(lib().xt_peps_create(rt.h, _handles(A), int(D), C.byref(h)), rt.h, "iPEPS weights")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        f = _lib.native("xt_peps_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
+            self.h = None
+
+    def _get(self, which, site):
+        h = C.c_void_p()
+        check(lib().xt_peps_tensor(self.h, which, int(site), C.byref(h)), self.rt.h, "iPEPS tensor")
+        return Tensor(self.rt, handle=h)
+
+    def G(self, site) -> Tensor:
+        return self._get(0, site)
+
+    def wh(self, site) -> Tensor:
+        return self._get(1, site)
+
+    def wv(self, site) -> Tensor:
+        return self._get(2, site)
+
+    def A(self, site) -> Tensor:
+        """applyWeights(Gs, sqrt weights) (iPEPS::updateAtensors, iPEPS.cpp:593-602)."""
+        return self._get(3, site)
+
+
+class TimePropagator:
+    """Simple-update propagator exp(-dt H) for nearest-neighbour bond Hamiltonians on a 2x2 cell
+    (TimePropagator.cpp: initU :453-487, t_step :78-105, t_step_h :113-190, t_step_v :192-226).
+    Hh[x*2+y] acts on the bond (x,y)-(x+1,y), Hv[x*2+y] on (x,y)-(x,y+1)."""
+
+    def __init__(self, rt: Runtime, Hh: Sequence[Tensor], Hv: Sequence[Tensor], dt: float):
+        self.rt = rt
+        h = C.c_void_p()
+        check(lib().xt_tprop_create(rt.h, _handles(Hh), _handles(Hv), float(dt), C.byref(h)), rt.h, "TimePropagator")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        f = _lib.native("xt_tprop_destroy") if (_lib is not None and h) else None
+        if f is not None:
+            f(h)
+            self.h = None
+
+    def t_step(self, psi: WeightedPEPS):
+        check(lib().xt_tprop_step(self.h, psi.h), self.rt.h, "t_step")
+
+    def t_step_h(self, psi: WeightedPEPS, x: int, y: int):
+        check(lib().xt_tprop_bond(self.h, psi.h, 0, int(x), int(y)), self.rt.h, "t_step_h")
+
+    def t_step_v(self, psi: WeightedPEPS, x: int, y: int):
+        check(lib().xt_tprop_bond(self.h, psi.h, 1, int(x), int(y)), self.rt.h, "t_step_v")
+
+    def _get(self, which, site):
+        h = C.c_void_p()
+        check(lib().xt_tprop_tensor(self.h, which, int(site), C.byref(h)), self.rt.h, "propagator tensor")
+        return Tensor(self.rt, handle=h)
+
+    def gate_h(self, site) -> Tensor:
+        return self._get(0, site)
+
+    def gate_v(self, site) -> Tensor:
+        return self._get(1, site)
+
+    def spectrum_h(self, site) -> Tensor:
+        return self._get(2, site)
+
+    def spectrum_v(self, site) -> Tensor:
+        return self._get(3, site)
+
+
+def heisenberg_bond(rt: Runtime, phys: Qbasis, Jxy: float = 1.0, Jz: float = 1.0) -> Tensor:
+    """The spin-1/2 Heisenberg bond Hamiltonian Jz Sz.Sz + Jxy/2 (S+S- + S-S+) (Heisenberg.hpp:50-53)
+    as Tensor<2,2>(p, p | p, p) on a U(1) physical leg whose labels are 2 Sz.  Entries are written
+    on the host from the tree labels and uploaded (the model's input, like the reference's
+    operator tables)."""
+    H = Tensor(rt, [phys, phys], [phys, phys])
+    data = np.zeros(H.size)
+    cd, cc = H.coupledDomain(), H.coupledCodomain()
+    dq = {e[0]: i for i, e in enumerate(cd.entries())}
+    cq = {e[0]: i for i, e in enumerate(cc.entries())}
+
+    def sz(tree, leg):
+        lab = tree["unc"][leg][0]
+        return 0.5 * (-lab if tree["dual"][leg] else lab)
+
+    for q, off, r, c in H.sectors():
+        dom, cod = cd.trees(dq[q]), cc.trees(cq[q])
+        r0 = 0
+        for td in dom:
+            c0 = 0
+            for tc in cod:
+                if td["dim"] == 1 and tc["dim"] == 1:
+                    c1, c2, a1, a2 = sz(td, 0), sz(td, 1), sz(tc, 0), sz(tc, 1)
+                    v = 0.0
+                    if c1 == a1 and c2 == a2:
+                        v = Jz * a1 * a2
+                    if (c1 == a1 + 1 and c2 == a2 - 1) or (c1 == a1 - 1 and c2 == a2 + 1):
+                        v = 0.5 * Jxy
+                    data[off + r0 + c0 * r] = v
+                c0 += tc["dim"]
+            r0 += td["dim"]
+    return H.upload(data)
+
+
+class Tape:
+    """Reverse-mode recording of the native tensor operations on one runtime (xt_tape_*): the
+    stan-math reverse-pass stack of the reference's AD tensors (ADTensor.hpp)."""
+
+    def __init__(self, rt: Runtime):
+        self.rt = rt
+        check(lib().xt_tape_begin(rt.h), rt.h, "tape")
+        self.open = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def pause(self, paused: bool = True):
+        check(lib().xt_tape_pause(self.rt.h, int(bool(paused))), self.rt.h, "tape pause")
+
+    def seed(self, y: Tensor, dy: Tensor, alpha: float = 1.0):
+        check(lib().xt_tape_seed(self.rt.h, y.h, dy.h, float(alpha)), self.rt.h, "tape seed")
+
+    def seed_trace(self, y: Tensor, g: float = 1.0):
+        """Seed the adjoint of y.trace() (times g)."""
+        check(lib().xt_tape_seed_trace(self.rt.h, y.h, float(g)), self.rt.h, "tape seed")
+
+    def backward(self):
+        check(lib().xt_tape_backward(self.rt.h), self.rt.h, "tape backward")
+
+    def adjoint(self, x: Tensor) -> Tensor:
+        h = C.c_void_p()
+        check(lib().xt_tape_adjoint(self.rt.h, x.h, C.byref(h)), self.rt.h, "tape adjoint")
+        return Tensor(self.rt, handle=h)
+
+    def close(self):
+        if self.open:
+            lib().xt_tape_end(self.rt.h)
+            self.open = False
