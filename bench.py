@@ -44,7 +44,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "contraction TFLOP/s + CTMRG step ms, mixed/learned vs FP64 GPU at ≤1e-10 err"
-PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}  # TFLOP/s, dense; BF16x3 counted as 3 BF16 products
+PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0, "ozaki": 5000.0}  # TFLOP/s, dense; BF16x3 counted as 3 BF16 products
+# XH_OZAKI: 8 slices of 7 bits per row / column, the 36 slice products with s + t <= 9 on the int8
+# matrix cores (TOP/s against 5.0 POP/s dense int8)
+OZ_PAIRS = 36
+DTYPES = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32", "ozaki": "f64 via int8 slices (Ozaki)"}
 SCALE = 10.0  # site tensors N(0,1) * SCALE / sqrt(plain size), see run_c4
 # the CPU oracle's whole FP64 CTMRG step on this workload (oracle/ctm.py: numpy/OpenBLAS per block,
 # written by tests/golden/make_ctm_fixture.py): far beyond a bounded sample, so it is quoted from that
@@ -65,11 +69,11 @@ def parse():
     ap.add_argument("--power-iters", type=int, default=6)
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--rank-factor", type=float, default=2.0, help="sketch width l = ceil(rank_factor * k) + oversample")
-    ap.add_argument("--sketch-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the range finder (c4)")
+    ap.add_argument("--sketch-precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "ozaki"], help="GEMM precision of the range finder (c4)")
     ap.add_argument("--ranks", default="learned", choices=["learned", "previous"],
                     help="target ranks of the randomized SVDs: the trained predictor or the previous step's kept counts")
     ap.add_argument("--leg-dim", type=int, default=512, help="c3 leg dimension")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"], help="GEMM precision of the fast mode / c3")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "ozaki"], help="GEMM precision of the fast mode / c3")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -273,7 +277,7 @@ def run_c4(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": f"synthetic (site tensors N(0,1)*{SCALE:g}/sqrt(plain size), std::mt19937(seed + site))",
         "config": {"workload": f"U(1) iPEPS CTMRG step, 2x2 cell, D=8, chi={args.chi}, Heisenberg", "chi": args.chi, "D": 8,
                    "svd": f"randomized, l = {args.rank_factor:g} k + {args.oversample}, k from the {'learned predictor' if args.ranks == 'learned' else 'previous step'}, "
@@ -285,10 +289,12 @@ def run_c4(args):
                  **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
                  "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
-        "roofline": {"bound": "mfma", "achieved": round(value, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(value / peak, 5),
+        "roofline": {"bound": "mfma", "achieved": round(value * (OZ_PAIRS if args.precision == "ozaki" else 1), 4), "peak": peak,
+                     "unit": "TOP/s (int8)" if args.precision == "ozaki" else "TFLOP/s",
+                     "frac": round(value * (OZ_PAIRS if args.precision == "ozaki" else 1) / peak, 5),
                      "traffic": measured_traffic("k_gemm_f64_ctm_step_counters.md") if args.precision == "fp64" and args.chi == 256 else None,
                      "kernel": ("grouped GEMM of the contractions: k_gemm_f64 + k_gemm_f64_128 + k_gemm_skinny (+ k_splitk_reduce)" if args.precision == "fp64"
-                                else f"k_gemm_{args.precision}"),
+                                else "k_oz_split + k_gemm_oz" if args.precision == "ozaki" else f"k_gemm_{args.precision}"),
                      "launches_per_step": con_n, "flops_per_step": con_fl, "kernel_ms_per_step": round(con_ms, 3)},
         "all_gemm_launches": {"achieved": round(all_tf, 4), "unit": "TFLOP/s", "frac": round(all_tf / peak, 5), "launches_per_step": fast["gemm_launches"],
                               "flops_per_step": fast["gemm_timed_flops"], "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3),
@@ -441,8 +447,16 @@ def run_c3(args):
     flops_step = (c1["gemm_flops"] - c0["gemm_flops"]) / steps
     step_ms = ms.value / steps
     value = flops_step / (step_ms * 1e-3) / 1e12
+    # the product's error against the FP64 matrix cores on the same blocks (mixed modes)
+    vs_fp64 = None
+    if prec != "fp64":
+        ref = A.contract(B, a1, a2, 2, precision="fp64")
+        vs_fp64 = {"max_abs_diff_over_max_abs": (Cm - ref).maxNorm() / ref.maxNorm()}
+        del ref
     graph = None
-    if args.graph:
+    if args.graph and prec == "ozaki":
+        log("[bench] c3: no graph replay in the Ozaki mode (it allocates its slices per call)")
+    elif args.graph:
         # the same product replayed from a HIP graph captured once (mul_into on the output of the
         # last eager step: no allocation under capture); timed like the eager steps
         out = Cm
@@ -468,16 +482,25 @@ def run_c3(args):
     gemm_ms, gemm_fl, nblocks = rt.gemm_timing(False)
     achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
     peak = PEAKS[prec]
+    if prec == "ozaki":
+        fp64_equiv = achieved
+        achieved = OZ_PAIRS * achieved  # int8 TOP/s of the slice products
     res = {
         "metric": METRIC, "value": round(value, 4), "unit": "TFLOP/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": round(step_ms, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3/f32"}[prec], "data": "synthetic (seeded mt19937 uniform[-1,1) blocks)",
+        "dtype": DTYPES[prec], "data": "synthetic (seeded mt19937 uniform[-1,1) blocks)",
         "config": {"workload": f"U(1)xU(1) fermionic grouped contraction A(l1,l2|l3,l4).B(l3,l4|l5,l6), leg dim {args.leg_dim}",
                    "matched_blocks": nblocks // 20, "gflop_per_step": round(flops_step / 1e9, 4), "precision": prec},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
-                     "kernel": f"k_gemm_{'f64' if prec == 'fp64' else prec}", "kernel_ms": round(gemm_ms / 20, 5)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TOP/s (int8)" if prec == "ozaki" else "TFLOP/s",
+                     "frac": round(achieved / peak, 5), "traffic": None,
+                     "kernel": {"fp64": "k_gemm_f64 + k_gemm_f64_128", "ozaki": "k_oz_split + k_gemm_oz"}.get(prec, f"k_gemm_{prec}"),
+                     "kernel_ms": round(gemm_ms / 20, 5)},
         "wall_s_timed": round(wall, 4),
     }
+    if prec == "ozaki":
+        res["roofline"]["fp64_equivalent_tflops"] = round(fp64_equiv, 4)
+    if vs_fp64 is not None:
+        res["error_vs_fp64"] = vs_fp64
     if graph is not None:
         res["graph_replay"] = graph
     if args.autotune:
