@@ -8,6 +8,8 @@ satisfies X^T G X = I to 1e-10 on the independent columns.
 """
 import ctypes as C
 
+import math
+
 import numpy as np
 import pytest
 
@@ -145,20 +147,23 @@ def test_cholesky_inverse_kernel(rt):
         np.testing.assert_allclose(XtGX[np.ix_(live, live)], np.eye(len(live)), rtol=0, atol=1e-10)
 
 
-def test_large_block_full_svd(rt):
-    # blocks whose smaller side exceeds 384 take the LAPACK-style path (Householder
-    # bidiagonalisation + bidiagonal QR); singular values to 1e-12 of numpy's, reconstruction to 1e-12
+@pytest.mark.parametrize("shape", [(450, 400, 30, 20), (1842, 1844, 1200, 1900)], ids=["450x400", "c4-size"])
+def test_large_block_full_svd(rt, shape):
+    # blocks with more than 64 columns take the block-Jacobi path (bjsvd.hip), up to the largest
+    # projector-product blocks of the D = 8, chi = 256 CTMRG step (1842 x 1844); singular values to
+    # 1e-12 of numpy's, reconstruction to 1e-12
     r = np.random.default_rng(21)
+    m0, n0, m1, n1 = shape
     a, b = xt.Qbasis(xt.U1), xt.Qbasis(xt.U1)
-    a.push_back((0,), 450)
-    a.push_back((1,), 30)
-    b.push_back((0,), 400)
-    b.push_back((1,), 20)
+    a.push_back((0,), m0)
+    a.push_back((1,), m1)
+    b.push_back((0,), n0)
+    b.push_back((1,), n1)
     t = xt.Tensor(rt, [a], [b])
     data = r.uniform(-1, 1, t.size)
     t.upload(data)
     U, S, V, tw, _ = t.tSVD(10 ** 6, 0.0, PRESERVE_MULTIPLETS=False)
-    np.testing.assert_allclose(((U * S) * V).download(), data, rtol=0, atol=1e-12 * 50)
+    np.testing.assert_allclose(((U * S) * V).download(), data, rtol=0, atol=1e-12 * math.sqrt(max(m0, n0, m1, n1)) * 2)
     for q, off, rows, cols in t.sectors():
         blk = data[off:off + rows * cols].reshape(cols, rows).T
         ref = np.linalg.svd(blk, compute_uv=False)
