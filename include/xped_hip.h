@@ -29,7 +29,11 @@ enum {
     XH_ENODEV = 5    /* no usable GPU                                               */
 };
 
-enum { XH_FP64 = 0, XH_FP32 = 1, XH_BF16X3 = 2 };
+/* GEMM precision modes (operands and results FP64 in memory): FP64 matrix cores; FP32; BF16x3
+ * (three BF16 products of FP32-rounded operands); XH_OZAKI: FP64 accuracy from the int8 matrix
+ * cores by error-free splitting into 8 slices of 7 bits per row of A / column of B (no counterpart
+ * in the reference, whose products are FP64 BLAS calls) */
+enum { XH_FP64 = 0, XH_FP32 = 1, XH_BF16X3 = 2, XH_OZAKI = 3 };
 
 typedef struct xh_ctx xh_ctx;
 typedef struct xh_permute_plan xh_permute_plan;

@@ -103,7 +103,8 @@ def _gemm_ref(A, B, items, ta, tb):
     return C
 
 
-@pytest.mark.parametrize("precision,tol", [(_lib.XH_FP64, 1e-12), (_lib.XH_FP32, 1e-5), (_lib.XH_BF16X3, 5e-5)], ids=["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("precision,tol", [(_lib.XH_FP64, 1e-12), (_lib.XH_FP32, 1e-5), (_lib.XH_BF16X3, 5e-5), (_lib.XH_OZAKI, 1e-12)],
+                         ids=["fp64", "fp32", "bf16x3", "ozaki"])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
     L = lib()
@@ -151,6 +152,83 @@ def test_grouped_gemm_raw(rt, precision, tol, ta, tb):
     L.xh_gemm_plan_destroy(ctx, plan)
     for d in (dA, dB, dC):
         L.xh_free(ctx, d)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_ozaki_gemm_error_bound_and_nonfinite_rows(rt, ta, tb):
+    # XH_OZAKI against an extended-precision reference, on rows and columns scaled over 16 decades,
+    # K beyond one int32 fold (4096) and split K: every entry within the classical FP64 GEMM bound
+    # k u (|A| |B|)_ij (u = 2^-53), and the largest error relative to max|A_i.| max|B_.j| within 4x
+    # that of the FP64 matrix cores on the same products; a NaN in A spoils exactly its row, an inf
+    # in B exactly its column.  Items with a side or K below 64 (the second, fourth and fifth here)
+    # run on the FP64 kernels in the same call.
+    L = lib()
+    ctx = rt.ctx()
+    r = np.random.default_rng(9)
+    shapes = [(70, 90, 5000), (33, 65, 129), (16, 16, 9000), (1, 200, 64), (130, 1, 63)]
+    items, ao, bo, co = [], 0, 0, 0
+    for m, n, k in shapes:
+        items.append({"a": ao, "b": bo, "c": co, "m": m, "n": n, "k": k})
+        ao += m * k
+        bo += k * n
+        co += m * n
+    A = r.standard_normal(ao)
+    B = r.standard_normal(bo)
+    mats = []
+    for it in items:
+        m, n, k = it["m"], it["n"], it["k"]
+        a = 10.0 ** r.uniform(-8, 8, (m, 1)) * r.standard_normal((m, k))
+        b = r.standard_normal((k, n)) * 10.0 ** r.uniform(-8, 8, (1, n))
+        A[it["a"]:it["a"] + m * k] = (a if ta else a.T).ravel()
+        B[it["b"]:it["b"] + k * n] = (b if tb else b.T).ravel()
+        mats.append((a, b))
+    a0 = mats[0][0]
+    a0[3, 17] = np.nan
+    A[items[0]["a"]:items[0]["a"] + a0.size] = (a0 if ta else a0.T).ravel()
+    b1 = mats[1][1]
+    b1[5, 7] = np.inf
+    B[items[1]["b"]:items[1]["b"] + b1.size] = (b1 if tb else b1.T).ravel()
+    dA, dB, dC = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for d, n in ((dA, ao), (dB, bo), (dC, co)):
+        assert L.xh_alloc(ctx, 8 * max(n, 1), C.byref(d)) == 0
+    L.xh_upload(ctx, dA, A.ctypes.data, 8 * ao)
+    L.xh_upload(ctx, dB, B.ctypes.data, 8 * bo)
+    arr = (_lib.xh_gemm_item * len(items))(*[_lib.xh_gemm_item(it["a"], it["b"], it["c"], it["m"], it["n"], it["k"]) for it in items])
+    plan = C.c_void_p()
+    assert L.xh_gemm_plan_create(ctx, arr, len(items), ta, tb, C.byref(plan)) == 0
+    outs = {}
+    for prec in (_lib.XH_OZAKI, _lib.XH_FP64):
+        L.xh_fill(ctx, dC, co, 0.0)
+        assert L.xh_gemm(ctx, plan, dA, dB, dC, 1.0, 0.0, prec) == 0
+        out = np.empty(co)
+        L.xh_download(ctx, out.ctypes.data, dC, 8 * co)
+        outs[prec] = out
+    L.xh_gemm_plan_destroy(ctx, plan)
+    for d in (dA, dB, dC):
+        L.xh_free(ctx, d)
+    for idx, (it, (a, b)) in enumerate(zip(items, mats)):
+        m, n, k = it["m"], it["n"], it["k"]
+        got = outs[_lib.XH_OZAKI][it["c"]:it["c"] + m * n].reshape(n, m).T
+        f64 = outs[_lib.XH_FP64][it["c"]:it["c"] + m * n].reshape(n, m).T
+        bad = np.zeros((m, n), bool)
+        if idx == 0:
+            bad[3, :] = True
+        if idx == 1:
+            bad[:, 7] = True
+        assert not np.any(np.isfinite(got[bad])), "a non-finite input must spoil its row / column"
+        ok = ~bad
+        with np.errstate(invalid="ignore"):
+            ref = (a.astype(np.longdouble) @ b.astype(np.longdouble)).astype(np.float64)
+            af = np.where(np.isfinite(a), np.abs(a), 0.0)
+            bf = np.where(np.isfinite(b), np.abs(b), 0.0)
+            amax = af.max(axis=1)[:, None]
+            bmax = bf.max(axis=0)[None, :]
+        bound = k * 2.0 ** -53 * (af @ bf) + 2.0 ** -53 * np.abs(ref)
+        err = np.abs(got - ref)
+        assert np.all(err[ok] <= bound[ok]), (it, float(np.max(err[ok] / bound[ok])))
+        # no worse than the FP64 matrix cores on the same products (relative to |A_i.| |B_.j|)
+        scale = amax * bmax
+        assert np.max(err[ok] / scale[ok]) <= 4.0 * max(np.max(np.abs(f64 - ref)[ok] / scale[ok]), 2.0 ** -53)
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])

@@ -13,7 +13,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libxped_hip.so"
 
 XH_OK, XH_EINVAL, XH_ENOMEM, XH_EHIP, XH_ENOTCONV, XH_ENODEV = 0, 1, 2, 3, 4, 5
-XH_FP64, XH_FP32, XH_BF16X3 = 0, 1, 2
+XH_FP64, XH_FP32, XH_BF16X3, XH_OZAKI = 0, 1, 2, 3
 XH_MAXLEGS = 20
 
 

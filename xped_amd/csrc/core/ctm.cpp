@@ -233,14 +233,23 @@ std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<
         std::vector<std::map<Q, int>> ranks(nb);
         std::vector<std::uint64_t> seeds(nb);
         std::vector<const DTensor*> ptrs(nb);
+        if(!opt.rank_policy && predictor) {
+            // the predictor's feature sketches of the move's decompositions in one batch
+            std::vector<std::uint64_t> pseeds(nb);
+            std::vector<const DTensor*> pp(nb);
+            for(std::size_t b = 0; b < nb; ++b) {
+                const std::uint64_t nr = static_cast<std::uint64_t>(n_rsvd) + b;
+                pseeds[b] = 0xfeedull + 64ull * nr + 4ull * sites[b] + dir;
+                pp[b] = &prods[b];
+            }
+            ranks = predictor->predict_batch(pp, opt.chi, pseeds);
+        }
         for(std::size_t b = 0; b < nb; ++b) {
             const int site = sites[b];
             const std::uint64_t nr = static_cast<std::uint64_t>(n_rsvd) + b;
-            if(opt.rank_policy)
+            if(opt.rank_policy) {
                 ranks[b] = opt.rank_policy(prods[b], site, dir);
-            else if(predictor)
-                ranks[b] = predictor->predict(prods[b], opt.chi, 0xfeedull + 64ull * nr + 4ull * site + dir);
-            else {
+            } else if(!predictor) {  // (with a predictor the ranks were set above)
                 auto it = last_kept.find({site, dir});
                 if(it == last_kept.end()) throw Error(XH_EINVAL, "randomized SVD needs target ranks (no rank policy and no previous full SVD)");
                 ranks[b] = it->second;

@@ -66,6 +66,8 @@ constexpr int RANK_TOP = 16;
 constexpr int RANK_NFEAT = 19;
 void rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* f);
 
+struct RandomizedSVD;
+
 struct RankPredictor {
     xh_ctx* ctx = nullptr;
     xh_mlp* net = nullptr;
@@ -77,6 +79,10 @@ struct RankPredictor {
     RankPredictor& operator=(const RankPredictor&) = delete;
     // target ranks for prod at bond dimension chi (sketch + network + cut search)
     std::map<Q, int> predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const;
+    // predict() of several tensors with their sketches batched into shared grouped launches
+    std::vector<std::map<Q, int>> predict_batch(const std::vector<const DTensor*>& prods, std::size_t chi, const std::vector<std::uint64_t>& seeds) const;
+    // the network and cut search on a finished sketch
+    std::map<Q, int> ranks_from_sketch(const DTensor& prod, const RandomizedSVD& r, std::size_t chi) const;
     // predicted counts above the cut 10^log_cut * s_glob for explicit sketches (row-major nsec x ntop)
     std::vector<double> counts(const std::vector<std::vector<double>>& top, const std::vector<int>& dims, double s_glob, double log_cut) const;
     // the same for several cuts at once (cut-major rows)

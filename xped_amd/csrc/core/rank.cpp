@@ -74,17 +74,45 @@ std::vector<double> RankPredictor::counts_grid(const std::vector<std::vector<dou
     return out;
 }
 
-std::map<Q, int> RankPredictor::predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const
+namespace {
+// the sketch the features are read from: the RANK_TOP leading singular values of every sector
+RsvdOptions sketch_options()
 {
-    const TensorStructure& s = *prod.st;
-    // sketch: the RANK_TOP leading singular values of every sector
-    std::map<Q, int> sk;
-    for(const Q& q : s.sector) sk[q] = RANK_TOP;
     RsvdOptions o;
     o.oversample = 4;
     o.rank_factor = 1.0;
     o.power_iters = 2;
-    RandomizedSVD r = tsvd_randomized(prod, sk, static_cast<std::size_t>(-1) / 2, 0.0, o, seed);
+    return o;
+}
+std::map<Q, int> sketch_ranks(const TensorStructure& s)
+{
+    std::map<Q, int> sk;
+    for(const Q& q : s.sector) sk[q] = RANK_TOP;
+    return sk;
+}
+}  // namespace
+
+std::map<Q, int> RankPredictor::predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const
+{
+    RandomizedSVD r = tsvd_randomized(prod, sketch_ranks(*prod.st), static_cast<std::size_t>(-1) / 2, 0.0, sketch_options(), seed);
+    return ranks_from_sketch(prod, r, chi);
+}
+
+std::vector<std::map<Q, int>> RankPredictor::predict_batch(const std::vector<const DTensor*>& prods, std::size_t chi,
+                                                           const std::vector<std::uint64_t>& seeds) const
+{
+    // the sketches of several tensors share grouped launches (tsvd_randomized_batch)
+    std::vector<std::map<Q, int>> sk;
+    for(const DTensor* p : prods) sk.push_back(sketch_ranks(*p->st));
+    std::vector<RandomizedSVD> rs = tsvd_randomized_batch(prods, sk, static_cast<std::size_t>(-1) / 2, 0.0, sketch_options(), seeds);
+    std::vector<std::map<Q, int>> out;
+    for(std::size_t b = 0; b < prods.size(); ++b) out.push_back(ranks_from_sketch(*prods[b], rs[b], chi));
+    return out;
+}
+
+std::map<Q, int> RankPredictor::ranks_from_sketch(const DTensor& prod, const RandomizedSVD& r, std::size_t chi) const
+{
+    const TensorStructure& s = *prod.st;
     std::vector<std::vector<double>> top;
     std::vector<int> dims;
     std::vector<Q> qs;

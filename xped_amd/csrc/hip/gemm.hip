@@ -19,6 +19,10 @@
 //   XH_FP32   operands rounded to FP32, v_mfma_f32_16x16x4_f32, FP32 accumulation
 //   XH_BF16X3 operands rounded to FP32 and split x = hi + lo with hi, lo in BF16;
 //             A*B ~ hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x16_bf16, FP32 accumulation
+//   XH_OZAKI  FP64 accuracy from the int8 matrix cores (error-free splitting, Ozaki scheme):
+//             every row of A and column of B is scaled by a power of two to |x| < 1 and cut into
+//             OZ_NS slices of 7 bits (int8); the slice products s + t <= OZ_NS + 1 run on
+//             v_mfma_i32_16x16x64_i8 with exact int32 sums, and are combined in FP64
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -369,6 +373,275 @@ __global__ __launch_bounds__(G8_THREADS) void k_gemm_f64_128(GemmArgs g)
     }
 }
 
+// ---------------------------------------------------------------- FP64 through int8 slices
+// Ozaki, Ogita, Oishi and Rump, Numer. Algorithms 59 (2012) 95-118 (error-free splitting of a
+// matrix product), with int8 slices as in Ootomo, Ozaki and Yokota, IJHPCA (2024).  Row i of A is
+// 2^ea_i sum_s a_s,i 2^-7(s+1) with |a_s| <= 127 (s = 0..OZ_NS-1), column j of B likewise; every
+// slice product is an exact integer matrix, and the products of one level d = s + t (weight
+// 2^-7(d+2)) are summed in int32, exact while OZ_NS * OZ_FLUSH * OZ_KB * 127^2 < 2^31; every
+// OZ_FLUSH K steps the levels are folded into FP64 accumulators, smallest weight first.  The
+// levels d >= OZ_NS and the slicing remainder are dropped: both are below k 2^-7 OZ_NS max|A_i.|
+// max|B_.j| (2^-56 per term), under FP64 GEMM's own k 2^-53 bound.
+constexpr int OZ_NS = 8;
+constexpr int OZ_KB = 64;            // the K of one v_mfma_i32_16x16x64_i8
+constexpr int OZ_PITCH = OZ_KB + 16;  // LDS row pitch in bytes
+constexpr int OZ_FLUSH = 64;          // K steps between int32 -> FP64 folds
+constexpr int32_t OZ_NONFINITE = -0x40000000;
+static_assert(double(OZ_NS) * OZ_FLUSH * OZ_KB * 127 * 127 < 2147483648.0, "int32 level sums must stay exact");
+
+struct OzItem {
+    int64_t sa_off, sb_off;  // byte offsets of the item's A and B slices (OZ_NS x rows x kp)
+    int64_t ea_off, eb_off;  // offsets of its row and column exponents
+    int32_t kp, pad;         // K rounded up to OZ_KB (zero padded)
+};
+// one slicing job: rows [row0, row0 + nrows) of operand op (0: rows of A, 1: columns of B) of an
+// item; g = lanes per row for operands whose K direction is contiguous in memory
+struct OzJob {
+    int32_t item, op, row0, g;
+};
+struct OzArgs {
+    const OzItem* oz;
+    int8_t* sl;
+    int32_t* ex;
+};
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+// the row's base pointer and K stride, and the number of rows, of operand op of an item
+__device__ __forceinline__ const double* oz_row(const GemmArgs& g, const xh_gemm_item& it, int op, int r, int64_t& step)
+{
+    if(op == 0) {
+        step = g.ta ? 1 : it.m;
+        return g.ta ? g.A + it.a_off + int64_t(r) * it.k : g.A + it.a_off + r;
+    }
+    step = g.tb ? it.n : 1;
+    return g.tb ? g.B + it.b_off + r : g.B + it.b_off + int64_t(r) * it.k;
+}
+
+__device__ __forceinline__ int oz_exponent(double mx, int bad)
+{
+    int e = 0;
+    if(bad) return OZ_NONFINITE;
+    if(mx > 0.0) frexp(mx, &e);  // mx = f 2^e with f in [1/2, 1): every |x| / 2^e < 1
+    return e;
+}
+
+// 8 slices of x (|x| < 1): x = sum_s q_s 2^-7(s+1) + remainder, |q_s| <= 127
+__device__ __forceinline__ void oz_cut(double x, int8_t (&q)[OZ_NS])
+{
+#pragma unroll
+    for(int s = 0; s < OZ_NS; ++s) {
+        x *= 128.0;  // exact
+        const double t = trunc(x);
+        x -= t;  // exact: |x - t| < 1 and both carry at most 53 bits
+        q[s] = static_cast<int8_t>(static_cast<int>(t));
+    }
+}
+
+// operands whose rows are contiguous (A not transposed, B transposed): one thread per row, K
+// walked with coalesced loads across the workgroup's rows; slices written 4 K entries at a time
+__global__ __launch_bounds__(256) void k_oz_split_rows(GemmArgs g, OzArgs o, const OzJob* __restrict__ jobs)
+{
+    const OzJob jb = jobs[blockIdx.x];
+    const xh_gemm_item it = g.items[jb.item];
+    const OzItem oi = o.oz[jb.item];
+    const int rows = jb.op == 0 ? it.m : it.n;
+    const int r = jb.row0 + int(threadIdx.x);
+    if(r >= rows) return;
+    int64_t step;
+    const double* src = oz_row(g, it, jb.op, r, step);
+    double mx = 0.0;
+    int bad = 0;
+    for(int kk = 0; kk < it.k; ++kk) {
+        const double v = src[kk * step];
+        bad |= !isfinite(v);
+        mx = fmax(mx, fabs(v));
+    }
+    const int e = oz_exponent(mx, bad);
+    o.ex[(jb.op == 0 ? oi.ea_off : oi.eb_off) + r] = e;
+    const int es = bad ? 0 : e;
+    int8_t* out = o.sl + (jb.op == 0 ? oi.sa_off : oi.sb_off) + int64_t(r) * oi.kp;
+    const int64_t sstride = int64_t(rows) * oi.kp;
+    for(int kk = 0; kk < oi.kp; kk += 4) {
+        int8_t q[4][OZ_NS];
+#pragma unroll
+        for(int u = 0; u < 4; ++u) oz_cut((kk + u < it.k && !bad) ? ldexp(src[(kk + u) * step], -es) : 0.0, q[u]);
+#pragma unroll
+        for(int s = 0; s < OZ_NS; ++s) {
+            const uint32_t w = uint32_t(uint8_t(q[0][s])) | (uint32_t(uint8_t(q[1][s])) << 8) | (uint32_t(uint8_t(q[2][s])) << 16) |
+                               (uint32_t(uint8_t(q[3][s])) << 24);
+            *reinterpret_cast<uint32_t*>(out + s * sstride + kk) = w;
+        }
+    }
+}
+
+// operands whose K direction is contiguous (A transposed, B not): a group of g lanes per row
+__global__ __launch_bounds__(256) void k_oz_split_k(GemmArgs g, OzArgs o, const OzJob* __restrict__ jobs)
+{
+    const OzJob jb = jobs[blockIdx.x];
+    const xh_gemm_item it = g.items[jb.item];
+    const OzItem oi = o.oz[jb.item];
+    const int rows = jb.op == 0 ? it.m : it.n;
+    const int gs = jb.g, lig = int(threadIdx.x) % gs;
+    const int r = jb.row0 + int(threadIdx.x) / gs;
+    const bool live = r < rows;
+    int64_t step = 0;
+    const double* src = live ? oz_row(g, it, jb.op, r, step) : nullptr;
+    double mx = 0.0;
+    int bad = 0;
+    if(live)
+        for(int kk = lig; kk < it.k; kk += gs) {
+            const double v = src[kk];
+            bad |= !isfinite(v);
+            mx = fmax(mx, fabs(v));
+        }
+    // reduce within the lane group (gs divides 64: a group never spans two waves)
+    for(int off = gs >> 1; off > 0; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off));
+        bad |= __shfl_xor(bad, off);
+    }
+    if(!live) return;
+    const int e = oz_exponent(mx, bad);
+    if(lig == 0) o.ex[(jb.op == 0 ? oi.ea_off : oi.eb_off) + r] = e;
+    const int es = bad ? 0 : e;
+    int8_t* out = o.sl + (jb.op == 0 ? oi.sa_off : oi.sb_off) + int64_t(r) * oi.kp;
+    const int64_t sstride = int64_t(rows) * oi.kp;
+    for(int kk = lig; kk < oi.kp; kk += gs) {
+        int8_t q[OZ_NS];
+        oz_cut((kk < it.k && !bad) ? ldexp(src[kk], -es) : 0.0, q);
+#pragma unroll
+        for(int s = 0; s < OZ_NS; ++s) out[s * sstride + kk] = q[s];
+    }
+}
+
+
+constexpr int OZ_THREADS = 512;                       // 8 waves: 2 x 4 wave tiles of 32 x 16
+constexpr int OZ_UNITS = OZ_NS * 64 * 4 / OZ_THREADS;  // 16-byte units per thread and operand per K step
+
+// the 16-byte units of one K step that thread tid stages: unit u = tid + OZ_THREADS v is slice
+// u / 256, tile row (u / 4) % 64, bytes 16 (u % 4) .. of the step; bytes outside [kbeg, kend)
+// (a K chunk's ends) are zeroed
+__device__ __forceinline__ void oz_fetch(const OzArgs& o, const OzItem& oi, const xh_gemm_item& it, const Tile& tl, int k0, int kbeg, int kend,
+                                         i4v (&va)[OZ_UNITS], i4v (&vb)[OZ_UNITS])
+{
+#pragma unroll
+    for(int v = 0; v < OZ_UNITS; ++v) {
+        const int u = int(threadIdx.x) + OZ_THREADS * v;
+        const int sl = u >> 8, row = (u >> 2) & 63, c = u & 3;
+        const int i = tl.m0 + row, j = tl.n0 + row, kk0 = k0 + 16 * c;
+        va[v] = i4v{0, 0, 0, 0};
+        vb[v] = i4v{0, 0, 0, 0};
+        if(i < it.m) va[v] = *reinterpret_cast<const i4v*>(o.sl + oi.sa_off + (int64_t(sl) * it.m + i) * oi.kp + kk0);
+        if(j < it.n) vb[v] = *reinterpret_cast<const i4v*>(o.sl + oi.sb_off + (int64_t(sl) * it.n + j) * oi.kp + kk0);
+        if(kk0 < kbeg || kk0 + 16 > kend) {
+            const int lo = kbeg - kk0, hi = kend - kk0;  // keep bytes b with lo <= b < hi
+#pragma unroll
+            for(int w = 0; w < 4; ++w) {
+                uint32_t m = 0;
+#pragma unroll
+                for(int b = 0; b < 4; ++b)
+                    if(4 * w + b >= lo && 4 * w + b < hi) m |= 0xffu << (8 * b);
+                va[v][w] = int(uint32_t(va[v][w]) & m);
+                vb[v][w] = int(uint32_t(vb[v][w]) & m);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void oz_fold(i4v (&acc)[OZ_NS][2], d4 (&facc)[2])
+{
+#pragma unroll
+    for(int d = OZ_NS - 1; d >= 0; --d) {
+        const double w = ldexp(1.0, -7 * (d + 2));
+#pragma unroll
+        for(int mi = 0; mi < 2; ++mi) {
+#pragma unroll
+            for(int r = 0; r < 4; ++r) facc[mi][r] += w * static_cast<double>(acc[d][mi][r]);
+            acc[d][mi] = i4v{0, 0, 0, 0};
+        }
+    }
+}
+
+// 64 x 64 output tiles, 8 waves in 2 x 4, each 2 MFMA tiles of 16 x 16 (32 x 16); K steps of 64 with
+// all OZ_NS slices of both operands staged in LDS (80 KiB); the next step's slices are fetched
+// into registers while the matrix cores work on the current one
+__global__ __launch_bounds__(OZ_THREADS) void k_gemm_oz(GemmArgs g, OzArgs o)
+{
+    static_assert(OZ_NS * 64 * 4 == OZ_UNITS * OZ_THREADS, "a K step is OZ_UNITS 16-byte units per thread and operand");
+    __shared__ __attribute__((aligned(16))) int8_t sA[OZ_NS][BM][OZ_PITCH];
+    __shared__ __attribute__((aligned(16))) int8_t sB[OZ_NS][BN][OZ_PITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const TileRange tr = my_tiles(g);
+    for(int t = tr.begin; t < tr.end; t += tr.stride) {
+        const Tile tl = g.tiles[t];
+        const xh_gemm_item it = g.items[tl.item];
+        const ItemExt ex = g.ext[tl.item];
+        const OzItem oi = o.oz[tl.item];
+        const int kbeg = tl.ks * ex.kc, kend = min(it.k, kbeg + ex.kc);
+        d4 facc[2];
+        i4v acc[OZ_NS][2];
+#pragma unroll
+        for(int mi = 0; mi < 2; ++mi) {
+            facc[mi] = d4{0, 0, 0, 0};
+#pragma unroll
+            for(int d = 0; d < OZ_NS; ++d) acc[d][mi] = i4v{0, 0, 0, 0};
+        }
+        int nsteps = 0;
+        const int kfirst = kbeg - (kbeg % OZ_KB);
+        i4v va[OZ_UNITS], vb[OZ_UNITS];
+        if(kfirst < kend) oz_fetch(o, oi, it, tl, kfirst, kbeg, kend, va, vb);
+        for(int k0 = kfirst; k0 < kend; k0 += OZ_KB) {
+#pragma unroll
+            for(int v = 0; v < OZ_UNITS; ++v) {
+                const int u = tid + OZ_THREADS * v;
+                const int sl = u >> 8, row = (u >> 2) & 63, c = u & 3;
+                *reinterpret_cast<i4v*>(&sA[sl][row][16 * c]) = va[v];
+                *reinterpret_cast<i4v*>(&sB[sl][row][16 * c]) = vb[v];
+            }
+            __syncthreads();
+            if(k0 + OZ_KB < kend) oz_fetch(o, oi, it, tl, k0 + OZ_KB, kbeg, kend, va, vb);
+            i4v bf[OZ_NS];
+#pragma unroll
+            for(int tt = 0; tt < OZ_NS; ++tt) bf[tt] = *reinterpret_cast<const i4v*>(&sB[tt][wn * 16 + (lane & 15)][16 * (lane >> 4)]);
+#pragma unroll
+            for(int sa = 0; sa < OZ_NS; ++sa) {
+                i4v af[2];
+#pragma unroll
+                for(int mi = 0; mi < 2; ++mi) af[mi] = *reinterpret_cast<const i4v*>(&sA[sa][wm * 32 + mi * 16 + (lane & 15)][16 * (lane >> 4)]);
+#pragma unroll
+                for(int tt = 0; tt + sa < OZ_NS; ++tt)
+#pragma unroll
+                    for(int mi = 0; mi < 2; ++mi) acc[sa + tt][mi] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[mi], bf[tt], acc[sa + tt][mi], 0, 0, 0);
+            }
+            __syncthreads();
+            if(++nsteps == OZ_FLUSH) {
+                oz_fold(acc, facc);
+                nsteps = 0;
+            }
+        }
+        oz_fold(acc, facc);
+        // scale by 2^(ea_i + eb_j) and store; register r of lane l holds row 4 (l / 16) + r,
+        // column l % 16 of its 16 x 16 tile
+        const bool part = ex.split > 1;
+        double* base = part ? g.ws + ex.ws_off + int64_t(tl.ks) * it.m * it.n : g.C + it.c_off;
+        const double alpha = part ? 1.0 : g.alpha, beta = part ? 0.0 : g.beta;
+        const int col = tl.n0 + wn * 16 + (lane & 15);
+        const int eb = col < it.n ? o.ex[oi.eb_off + col] : 0;
+#pragma unroll
+        for(int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for(int r = 0; r < 4; ++r) {
+                const int row = tl.m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + r;
+                if(row < it.m && col < it.n) {
+                    const int ea = o.ex[oi.ea_off + row];
+                    const double v = (ea == OZ_NONFINITE || eb == OZ_NONFINITE) ? __builtin_nan("") : ldexp(facc[mi][r], ea + eb);
+                    double* pc = base + int64_t(col) * it.m + row;
+                    *pc = (beta == 0.0) ? alpha * v : alpha * v + beta * *pc;
+                }
+            }
+    }
+}
+
 // ---------------------------------------------------------------- FP32
 __global__ __launch_bounds__(G_THREADS) void k_gemm_f32(GemmArgs g)
 {
@@ -502,6 +775,16 @@ struct xh_gemm_plan : xh::PlanLife {
     int sk_ms = 8;            // register width of the skinny kernel: 8, 16 or 32 >= every skinny item's short side
     bool all64_built = false, any_tiles = false;
     xh::PackedUpload up_all;
+    // XH_OZAKI, built on first use: the slice layout of the items that take the int8 path, their
+    // tiles and slicing jobs, and the 64 x 64 FP64 tiles of the other non-skinny items
+    bool oz_built = false;
+    xh::PackedUpload up_oz;
+    OzItem* d_oz = nullptr;
+    TileList oz_tiles, oz_rest;
+    OzJob* d_jobs_rows = nullptr;
+    OzJob* d_jobs_k = nullptr;
+    int32_t n_jobs_rows = 0, n_jobs_k = 0;
+    int64_t oz_slice_bytes = 0, oz_exps = 0;
     std::vector<xh_gemm_item> h_items;  // host copies for the deferred list
     std::vector<ItemExt> h_ext;
     std::vector<int> h_order;
@@ -724,6 +1007,78 @@ void build_all64(xh_ctx* ctx, xh_gemm_plan* plan)
     plan->all64.d_tiles = plan->up_all.ptr<Tile>(o);
     plan->all64_built = true;
 }
+
+// XH_OZAKI takes the int8 path for items whose output sides and K are at least OZ_MIN (the matrix
+// cores' tiles are then mostly full); skinny items keep the FP64 skinny kernel and the others the
+// 64 x 64 FP64 tiles, in the same call.  Both are FP64-accurate.
+constexpr int OZ_MIN = 64;
+bool oz_item(const xh_gemm_item& it) { return it.m >= OZ_MIN && it.n >= OZ_MIN && it.k >= OZ_MIN; }
+
+void build_oz(xh_ctx* ctx, xh_gemm_plan* plan)
+{
+    std::vector<OzItem> v(plan->h_items.size(), OzItem{});
+    std::vector<OzJob> jr, jk;
+    int64_t bytes = 0, exps = 0;
+    for(size_t id = 0; id < plan->h_items.size(); ++id) {
+        const xh_gemm_item& it = plan->h_items[id];
+        if(!oz_item(it)) continue;
+        OzItem& o = v[id];
+        o.kp = (it.k + OZ_KB - 1) / OZ_KB * OZ_KB;
+        o.sa_off = bytes;
+        bytes += int64_t(OZ_NS) * it.m * o.kp;
+        o.sb_off = bytes;
+        bytes += int64_t(OZ_NS) * it.n * o.kp;
+        o.ea_off = exps;
+        exps += it.m;
+        o.eb_off = exps;
+        exps += it.n;
+        for(int op = 0; op < 2; ++op) {
+            const int rows = op == 0 ? it.m : it.n;
+            const bool row_contig = op == 0 ? !plan->ta : plan->tb;
+            if(row_contig) {
+                for(int r0 = 0; r0 < rows; r0 += 256) jr.push_back(OzJob{int32_t(id), op, r0, 1});
+            } else {
+                int gsz = 4;
+                while(gsz < 64 && gsz < it.k) gsz *= 2;
+                for(int r0 = 0; r0 < rows; r0 += 256 / gsz) jk.push_back(OzJob{int32_t(id), op, r0, gsz});
+            }
+        }
+    }
+    std::vector<Tile> t_oz, t_rest;
+    std::vector<double> w_oz, w_rest;
+    for(int id : plan->h_order) {
+        const xh_gemm_item& it = plan->h_items[id];
+        if(it.m == 0 || it.n == 0 || is_skinny(it)) continue;
+        const ItemExt& e = plan->h_ext[id];
+        const bool oz = oz_item(it);
+        for(int ks = 0; ks < e.split; ++ks) {
+            const double kw = std::min(it.k - ks * e.kc, e.kc) + 8.0;
+            for(int n0 = 0; n0 < it.n; n0 += BN)
+                for(int m0 = 0; m0 < it.m; m0 += BM) {
+                    (oz ? t_oz : t_rest).push_back(Tile{id, m0, n0, ks});
+                    (oz ? w_oz : w_rest).push_back(kw);
+                }
+        }
+    }
+    if(t_oz.size() > 0x7fffffff || t_rest.size() > 0x7fffffff || jr.size() > 0x7fffffff || jk.size() > 0x7fffffff)
+        throw xh::Error(XH_EINVAL, "gemm plan: too many tiles or slicing jobs");
+    cut_parts(w_oz, plan->oz_tiles.part);
+    cut_parts(w_rest, plan->oz_rest.part);
+    plan->oz_tiles.ntiles = static_cast<int32_t>(t_oz.size());
+    plan->oz_rest.ntiles = static_cast<int32_t>(t_rest.size());
+    const size_t ov = plan->up_oz.add(v), oto = plan->up_oz.add(t_oz), otr = plan->up_oz.add(t_rest), ojr = plan->up_oz.add(jr), ojk = plan->up_oz.add(jk);
+    plan->up_oz.upload(ctx);
+    plan->d_oz = plan->up_oz.ptr<OzItem>(ov);
+    plan->oz_tiles.d_tiles = plan->up_oz.ptr<Tile>(oto);
+    plan->oz_rest.d_tiles = plan->up_oz.ptr<Tile>(otr);
+    plan->d_jobs_rows = plan->up_oz.ptr<OzJob>(ojr);
+    plan->d_jobs_k = plan->up_oz.ptr<OzJob>(ojk);
+    plan->n_jobs_rows = static_cast<int32_t>(jr.size());
+    plan->n_jobs_k = static_cast<int32_t>(jk.size());
+    plan->oz_slice_bytes = bytes;
+    plan->oz_exps = exps;
+    plan->oz_built = true;
+}
 }  // namespace
 
 // ---------------------------------------------------------------- allocation-free variant
@@ -773,9 +1128,10 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
     {
         if(!cplan) throw xh::Error(XH_EINVAL, "null gemm plan");
         xh_gemm_plan* plan = const_cast<xh_gemm_plan*>(cplan);  // the deferred tile list is filled in on first use
-        if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3) throw xh::Error(XH_EINVAL, "unknown precision mode");
+        if(precision != XH_FP64 && precision != XH_FP32 && precision != XH_BF16X3 && precision != XH_OZAKI) throw xh::Error(XH_EINVAL, "unknown precision mode");
+        if(precision == XH_OZAKI && noalloc) throw xh::Error(XH_EINVAL, "xh_gemm_noalloc: XH_OZAKI needs a slice workspace; use xh_gemm");
         if(!plan->any_tiles) return;
-        const bool need_all = precision != XH_FP64 || ctx->gemm_tiles != 0;
+        const bool need_all = (precision != XH_FP64 && precision != XH_OZAKI) || ctx->gemm_tiles != 0;
         if(need_all && !plan->all64_built) {
             if(noalloc)
                 throw xh::Error(XH_EINVAL, "xh_gemm_noalloc: the plan's 64 x 64 tile list (FP32/BF16x3 modes, 64 x 64 FP64 tiling) does not exist yet; "
@@ -814,6 +1170,39 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
                 launch(plan->rest64, k_gemm_f64p<64, 64, 16, 2, 2>, 8, G_THREADS, "k_gemm_f64");
             } else if(precision == XH_FP64) {
                 launch(plan->all64, k_gemm_f64p<64, 64, 16, 2, 2>, 8, G_THREADS, "k_gemm_f64");
+            } else if(precision == XH_OZAKI) {
+                if(!plan->oz_built) build_oz(ctx, plan);
+                if(plan->skinny.ntiles > 0) {
+                    auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny<8, sk_rows(8)> : plan->sk_ms <= 16 ? k_gemm_skinny<16, sk_rows(16)> : k_gemm_skinny<32, sk_rows(32)>;
+                    const int grid = std::min<int>(plan->skinny.ntiles, ctx->num_cu * 8);
+                    hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, plan->skinny.d_tiles, plan->skinny.ntiles);
+                    xh::launch_check("k_gemm_skinny");
+                }
+                launch(plan->oz_rest, k_gemm_f64p<64, 64, 16, 2, 2>, 8, G_THREADS, "k_gemm_f64");
+                if(plan->oz_tiles.ntiles > 0) {
+                    int8_t* sl = nullptr;
+                    const size_t ex_off = (size_t(plan->oz_slice_bytes) + 255) / 256 * 256;
+                    const size_t bytes = ex_off + size_t(plan->oz_exps) * sizeof(int32_t);
+                    xh::refuse_under_capture(ctx, "an XH_OZAKI GEMM (it allocates its slices)");
+                    check(hipMallocAsync(reinterpret_cast<void**>(&sl), bytes, ctx->stream), "hipMallocAsync(Ozaki slices)");
+                    OzArgs oa{plan->d_oz, sl, reinterpret_cast<int32_t*>(sl + ex_off)};
+                    if(plan->n_jobs_rows > 0) {
+                        hipLaunchKernelGGL(k_oz_split_rows, dim3(static_cast<unsigned>(plan->n_jobs_rows)), dim3(256), 0, ctx->stream, g, oa, plan->d_jobs_rows);
+                        xh::launch_check("k_oz_split_rows");
+                    }
+                    if(plan->n_jobs_k > 0) {
+                        hipLaunchKernelGGL(k_oz_split_k, dim3(static_cast<unsigned>(plan->n_jobs_k)), dim3(256), 0, ctx->stream, g, oa, plan->d_jobs_k);
+                        xh::launch_check("k_oz_split_k");
+                    }
+                    const TileList& L = plan->oz_tiles;
+                    g.tiles = L.d_tiles;
+                    for(int x = 0; x <= NPART; ++x) g.part[x] = L.part[x];
+                    int grid = std::min<int>(L.ntiles, ctx->num_cu);
+                    grid = (grid + NPART - 1) / NPART * NPART;
+                    hipLaunchKernelGGL(k_gemm_oz, dim3(grid), dim3(OZ_THREADS), 0, ctx->stream, g, oa);
+                    xh::launch_check("k_gemm_oz");
+                    check(hipFreeAsync(sl, ctx->stream), "hipFreeAsync(Ozaki slices)");
+                }
             } else {
                 launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");
             }

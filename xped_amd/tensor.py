@@ -12,7 +12,7 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import XH_BF16X3, XH_FP32, XH_FP64, check, lib
+from ._lib import XH_BF16X3, XH_FP32, XH_FP64, XH_OZAKI, check, lib
 
 # symmetry kinds (component codes of the C ABI)
 U1 = (1, 0)          # U1<SpinU1> / U1<ChargeU1>
@@ -21,7 +21,7 @@ SU2 = (3, 0)         # SU2<SpinSU2>
 FU1xU1 = (2, 1)      # Combined<U1<FChargeU1>, U1<SpinU1>>
 U1xU1 = (1, 1)
 
-PRECISION = {"fp64": XH_FP64, "fp32": XH_FP32, "bf16x3": XH_BF16X3}
+PRECISION = {"fp64": XH_FP64, "fp32": XH_FP32, "bf16x3": XH_BF16X3, "ozaki": XH_OZAKI}
 
 
 def _q2(q) -> C.Array:
