@@ -79,7 +79,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
     ap.add_argument("--no-su", action="store_true", help="c4: skip the simple-update section")
-    ap.add_argument("--no-previous", action="store_true", help="c4: skip the arm with the previous step's kept counts as target ranks")
+    ap.add_argument("--no-previous", action="store_true", help="c4: skip the arms with the previous step's kept counts as target ranks and with freshly sketched predictor features")
     return ap.parse_args()
 
 
@@ -113,7 +113,7 @@ def launch_split(rt, xt):
     return out
 
 
-def run_mode(mode, args, xt, configs, random_ipeps, ranks=None):
+def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
     rt = xt.Runtime(0)
     if args.autotune and mode == "fast":
@@ -126,7 +126,8 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None):
     log(f"[bench] {mode}: warm-up grow_all {time.perf_counter() - t0:.1f} s")
     def configure(record):
         if mode == "fast":
-            c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters, record_spectra=record)
+            c.set_options(precision=args.precision, svd="randomized", oversample=args.oversample, power_iters=args.power_iters, record_spectra=record,
+                          fresh_predictor_sketch=fresh)
             c.set_rsvd(oversample=args.oversample, rank_factor=args.rank_factor, power_iters=args.power_iters, sketch_precision=args.sketch_precision)
         else:
             c.set_options(precision="fp64", svd="full", record_spectra=record)
@@ -169,6 +170,7 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None):
            "plans_per_step": ((h1["n_permute_plans"] - h0["n_permute_plans"]) + (h1["n_gemm_plans"] - h0["n_gemm_plans"])) / nsteps,
            "host_breakdown_ms_per_step": [round(1e3 * (b - a) / nsteps, 1) for a, b in zip(h0["breakdown_s"], h1["breakdown_s"])],
            "symbolic_plan_hits_per_step": (h1["symbolic_hits"] - h0["symbolic_hits"]) / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
+           "predictor_features_previous_sketch": list(c.predictor_stats()),
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
     if mode == "fast":
@@ -258,6 +260,7 @@ def run_c4(args):
 
     fast = run_mode("fast", args, xt, configs, random_ipeps)
     prev = None if (args.no_previous or args.ranks == "previous") else run_mode("fast", args, xt, configs, random_ipeps, ranks="previous")
+    fresh = None if (args.no_previous or args.ranks == "previous") else run_mode("fast", args, xt, configs, random_ipeps, fresh=True)
     ref = None if args.no_fp64 else run_mode("fp64", args, xt, configs, random_ipeps)
     # value: contraction products only (operator* of the step's contractions); all grouped GEMM
     # launches of the step, the randomized SVDs' included, are reported beside it
@@ -286,6 +289,7 @@ def run_c4(args):
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
+                 "predictor_features_previous_sketch": fast["predictor_features_previous_sketch"],
                  **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
                  "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
@@ -308,6 +312,9 @@ def run_c4(args):
                              "bytes_per_step": fast["permute_bytes"], "launches_per_step": fast["permute_launches"],
                              "kernel_ms_per_step": round(fast["permute_ms"], 3)},
     }
+    if fresh is not None:
+        res["learned_fresh_sketch"] = {"ms_per_step": round(fresh["ms_per_step"], 3), "svd_ms_per_step": round(fresh["svd_ms_per_step"], 3),
+                                       "rsvd_calls_fallbacks_ratio": list(fresh["rsvd"]), "energies": fresh["energies"]}
     if prev is not None:
         res["ranks_previous"] = {"ms_per_step": round(prev["ms_per_step"], 3), "svd_ms_per_step": round(prev["svd_ms_per_step"], 3),
                                  "rsvd_calls_fallbacks_ratio": list(prev["rsvd"]), "energies": prev["energies"]}
@@ -316,6 +323,10 @@ def run_c4(args):
         nb = min(len(ref["energies"]), len(fast["energies"]))
         errs = [abs(fast["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nb)]
         res["energy_rel_err_vs_fp64_per_step"] = errs
+        if fresh is not None:
+            nbf = min(len(ref["energies"]), len(fresh["energies"]))
+            res["learned_fresh_sketch"]["energy_rel_err_vs_fp64_per_step"] = [abs(fresh["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i])
+                                                                             for i in range(nbf)]
         if prev is not None:
             nbp = min(len(ref["energies"]), len(prev["energies"]))
             res["ranks_previous"]["energy_rel_err_vs_fp64_per_step"] = [abs(prev["energies"][i] - ref["energies"][i]) / abs(ref["energies"][i]) for i in range(nbp)]

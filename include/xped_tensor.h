@@ -231,7 +231,9 @@ int xt_ctm_destroy(xt_ctm* c);
 /* precision: XH_FP64/XH_FP32/XH_BF16X3 for the contractions; svd_mode 0 = full FP64 block SVD,
  * 1 = randomized block SVD with per-sector target ranks; flags: bit 0 records the full spectra,
  * bit 1 checks every projector factor for non-finite entries, bit 2 prints one line per
- * projector decomposition to stderr */
+ * projector decomposition to stderr, bit 3 makes the learned ranks always sketch their features
+ * afresh (by default they reuse the previous decomposition's spectrum of the same site and
+ * direction when it covers every sector) */
 int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int flags);
 int xt_ctm_init(xt_ctm* c);                         /* CTM::init, FROM_A (CTM.cpp:137-240) */
 int xt_ctm_grow_all(xt_ctm* c);                     /* CTM::grow_all (CTM.cpp:386-420) */
@@ -246,6 +248,8 @@ int xt_ctm_copy_env(xt_ctm* dst, const xt_ctm* src);
  * (CTMSolver::solve with AD, CTMSolver.cpp:80-137; each move a checkpoint, CTM.cpp:386-420): the
  * environment on entry is a constant; needs svd_mode 0.  dA[site] are new tensors. */
 int xt_ctm_energy_grad(xt_ctm* c, int steps, double Jxy, double Jz, double* E, xt_tensor** dA);
+/* decompositions whose learned ranks used the previous spectrum / a fresh sketch as features */
+int xt_ctm_predictor_stats(const xt_ctm* c, long* from_previous, long* from_sketch);
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms);
 /* randomized SVD parameters of the CTM decompositions (see xt_tsvd_randomized) */
 int xt_ctm_set_rsvd(xt_ctm* c, int oversample, double rank_factor, int power_iters, int sketch_precision);

@@ -230,6 +230,7 @@ _SIGS = [
     ("xt_ctm_compute_rdm", C.c_int, [_P]),
     ("xt_ctm_energy", C.c_int, [_P, C.c_double, C.c_double, _DP]),
     ("xt_ctm_tensor", C.c_int, [_P, C.c_int, C.c_int, _PP]),
+    ("xt_ctm_predictor_stats", C.c_int, [_P, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_ctm_svd_ms", C.c_int, [_P, _DP]),
     ("xt_ctm_set_rsvd", C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.c_int]),
     ("xt_ctm_set_predictor", C.c_int, [_P, C.POINTER(C.c_float), C.c_int64, C.c_int, C.c_double]),

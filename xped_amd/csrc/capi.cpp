@@ -1065,6 +1065,7 @@ int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, i
         o.record_spectra = (flags & 1) != 0;
         o.check_finite = (flags & 2) != 0;
         o.verbose = (flags & 4) != 0;
+        o.predictor_reuses_spectra = (flags & 8) == 0;
         c->ctm->set_options(o);
     });
 }
@@ -1117,6 +1118,13 @@ int xt_ctm_energy_grad(xt_ctm* c, int steps, double Jxy, double Jz, double* E, x
         *E = c->ctm->energy_gradient(steps, Jxy, Jz, g);
         for(int i = 0; i < 4; ++i) dA[i] = wrap(c->owner, g[i]);
     });
+}
+
+int xt_ctm_predictor_stats(const xt_ctm* c, long* from_previous, long* from_sketch)
+{
+    *from_previous = c->ctm->n_predicted_from_previous;
+    *from_sketch = c->ctm->n_predicted_from_sketch;
+    return XH_OK;
 }
 
 int xt_ctm_svd_ms(const xt_ctm* c, double* ms)

@@ -150,34 +150,68 @@ void CTM::init_from_A()
         }
 }
 
+ContractJob CTM::corner_stage(int x, int y, int corner, int stage, const DTensor& prev) const
+{
+    // contractCorner (CTM.cpp:1643-1760): four contractions per corner; stage k contracts the
+    // result of stage k - 1 (prev) with the next environment or site tensor
+    switch(corner) {
+    case UPPER_LEFT:
+        switch(stage) {
+        case 0: return {C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -2, -3, -4}, 1};                // C1T1
+        case 1: return {T4[idx(x - 1, y)], {1, -1, -2, -3}, tw(prev, {0}), {1, -4, -5, -6}, 3};                 // T4C1T1
+        case 2: return {prev, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4};                         // T4C1T1A
+        default: return {prev, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3};
+        }
+    case LOWER_LEFT:
+        switch(stage) {
+        case 0: return {C4[idx(x - 1, y + 1)], {-1, 1}, T3[idx(x, y + 1)], {-2, -3, 1, -4}, 1};                // C4T3
+        case 1: return {T4[idx(x - 1, y)], {-1, 1, -2, -3}, prev, {1, -4, -5, -6}, 3};                        // T4C4T3
+        case 2: return {prev, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {3}), {1, -5, -6, 2, -7}, 4};                // T4C4T3A
+        default: return {prev, {-4, 1, 2, -1, -5, -2, 3}, tw(Adag(x, y), {3}), {1, -6, 3, -3, 2}, 3};
+        }
+    case UPPER_RIGHT:
+        switch(stage) {
+        case 0: return {T1[idx(x, y - 1)], {-1, 1, -2, -3}, C2[idx(x + 1, y - 1)], {1, -4}, 3};                // T1C2
+        case 1: return {prev, {-1, -2, -3, 1}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3};                        // T1C2T2
+        case 2: return {prev, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4};                // T1C2T2A
+        default: return {prev, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3};
+        }
+    default:
+        switch(stage) {
+        case 0: return {tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 1};       // C3T3
+        case 1: return {T2[idx(x + 1, y)], {-1, -2, -3, 1}, prev, {1, -4, -5, -6}, 3};                        // T2C3T3
+        case 2: return {prev, {1, -1, -2, 2, -3, -4}, tw(A(x, y), {2, 3}), {-5, -6, 1, 2, -7}, 4};             // T2C3T3A
+        default: return {prev, {1, -1, 2, -4, -5, -2, 3}, Adag(x, y), {-6, -3, 3, 1, 2}, 3};
+        }
+    }
+}
+
+std::vector<DTensor> CTM::contract_corners(const std::vector<std::array<int, 3>>& req) const
+{
+    // the corners of several sites in lockstep: each stage's contractions share one GEMM launch
+    std::vector<DTensor> cur(req.size());
+    for(int stage = 0; stage < 4; ++stage) {
+        std::vector<ContractJob> jobs;
+        for(std::size_t i = 0; i < req.size(); ++i) jobs.push_back(corner_stage(req[i][0], req[i][1], req[i][2], stage, cur[i]));
+        cur = contract_batch(jobs, opt.precision);
+    }
+    return cur;
+}
+
 DTensor CTM::contract_corner(int x, int y, int corner) const
 {
-    switch(corner) {
-    case UPPER_LEFT: {
-        DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -2, -3, -4}, 1);
-        DTensor T4C1T1 = ct(T4[idx(x - 1, y)], {1, -1, -2, -3}, tw(C1T1, {0}), {1, -4, -5, -6}, 3);
-        DTensor T4C1T1A = ct(T4C1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
-        return ct(T4C1T1A, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3);
+    return contract_corners({{x, y, corner}})[0];
+}
+
+void CTM::keep_spectrum(int site, int dir, const FullSVD& f)
+{
+    // the leading singular values per sector (all_sv lists each sector's values in descending order)
+    std::map<Q, std::vector<double>> m;
+    for(const auto& [q, sv] : f.all_sv) {
+        auto& v = m[q];
+        if(v.size() < static_cast<std::size_t>(RANK_TOP)) v.push_back(sv);
     }
-    case LOWER_LEFT: {
-        DTensor C4T3 = ct(C4[idx(x - 1, y + 1)], {-1, 1}, T3[idx(x, y + 1)], {-2, -3, 1, -4}, 1);
-        DTensor T4C4T3 = ct(T4[idx(x - 1, y)], {-1, 1, -2, -3}, C4T3, {1, -4, -5, -6}, 3);
-        DTensor T4C4T3A = ct(T4C4T3, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {3}), {1, -5, -6, 2, -7}, 4);
-        return ct(T4C4T3A, {-4, 1, 2, -1, -5, -2, 3}, tw(Adag(x, y), {3}), {1, -6, 3, -3, 2}, 3);
-    }
-    case UPPER_RIGHT: {
-        DTensor T1C2 = ct(T1[idx(x, y - 1)], {-1, 1, -2, -3}, C2[idx(x + 1, y - 1)], {1, -4}, 3);
-        DTensor T1C2T2 = ct(T1C2, {-1, -2, -3, 1}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3);
-        DTensor T1C2T2A = ct(T1C2T2, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4);
-        return ct(T1C2T2A, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3);
-    }
-    default: {
-        DTensor C3T3 = ct(tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 1);
-        DTensor T2C3T3 = ct(T2[idx(x + 1, y)], {-1, -2, -3, 1}, C3T3, {1, -4, -5, -6}, 3);
-        DTensor T2C3T3A = ct(T2C3T3, {1, -1, -2, 2, -3, -4}, tw(A(x, y), {2, 3}), {-5, -6, 1, 2, -7}, 4);
-        return ct(T2C3T3A, {1, -1, 2, -4, -5, -2, 3}, Adag(x, y), {-6, -3, 3, 1, 2}, 3);
-    }
-    }
+    last_spectra[{site, dir}] = std::move(m);
 }
 
 std::pair<DTensor, DTensor> CTM::decompose(const DTensor& Ta, const DTensor& Tb, int site, int dir)
@@ -228,21 +262,44 @@ std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<
             FullSVD f = full_svd(prods[b]);
             rs[b] = cutoff(prods[b], f, opt.chi, 1.e-10, false, 0.0, 0.0);
             record(b, rs[b], &f);
+            keep_spectrum(sites[b], dir, f);
         }
     } else {
         std::vector<std::map<Q, int>> ranks(nb);
         std::vector<std::uint64_t> seeds(nb);
         std::vector<const DTensor*> ptrs(nb);
         if(!opt.rank_policy && predictor) {
-            // the predictor's feature sketches of the move's decompositions in one batch
-            std::vector<std::uint64_t> pseeds(nb);
-            std::vector<const DTensor*> pp(nb);
+            // the network's features: the previous decomposition's leading singular values of the
+            // same (site, direction) when they cover every sector, else a fresh sketch (the sketches
+            // of the move's remaining decompositions in one batch)
+            std::vector<std::uint64_t> pseeds;
+            std::vector<const DTensor*> pp;
+            std::vector<std::size_t> need;
             for(std::size_t b = 0; b < nb; ++b) {
-                const std::uint64_t nr = static_cast<std::uint64_t>(n_rsvd) + b;
-                pseeds[b] = 0xfeedull + 64ull * nr + 4ull * sites[b] + dir;
-                pp[b] = &prods[b];
+                const TensorStructure& st = *prods[b].st;
+                auto it = opt.predictor_reuses_spectra ? last_spectra.find({sites[b], dir}) : last_spectra.end();
+                bool covered = it != last_spectra.end();
+                for(std::size_t i = 0; covered && i < st.sector.size(); ++i) {
+                    const std::size_t dim = std::min(st.rows[i], st.cols[i]);
+                    if(dim == 0) continue;
+                    auto sv = it->second.find(st.sector[i]);
+                    covered = sv != it->second.end() && sv->second.size() >= std::min<std::size_t>(dim, RANK_TOP);
+                }
+                if(covered) {
+                    ranks[b] = predictor->ranks_from_spectra(st, it->second, opt.chi);
+                    ++n_predicted_from_previous;
+                } else {
+                    const std::uint64_t nr = static_cast<std::uint64_t>(n_rsvd) + b;
+                    pseeds.push_back(0xfeedull + 64ull * nr + 4ull * sites[b] + dir);
+                    pp.push_back(&prods[b]);
+                    need.push_back(b);
+                }
             }
-            ranks = predictor->predict_batch(pp, opt.chi, pseeds);
+            if(!need.empty()) {
+                std::vector<std::map<Q, int>> r = predictor->predict_batch(pp, opt.chi, pseeds);
+                for(std::size_t j = 0; j < need.size(); ++j) ranks[need[j]] = std::move(r[j]);
+                n_predicted_from_sketch += static_cast<long>(need.size());
+            }
         }
         for(std::size_t b = 0; b < nb; ++b) {
             const int site = sites[b];
@@ -263,11 +320,13 @@ std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<
             max_residual_ratio = std::max(max_residual_ratio, rsv[b].max_residual_ratio);
             if(rsv[b].safe) {
                 rs[b] = std::move(rsv[b].res);
+                last_spectra[{sites[b], dir}] = std::move(rsv[b].ritz);
             } else {
                 // the full FP64 tSVD is the fallback and the judge
                 ++n_fallback;
                 FullSVD f = full_svd(prods[b]);
                 rs[b] = cutoff(prods[b], f, opt.chi, 1.e-10, false, 0.0, 0.0);
+                keep_spectrum(sites[b], dir, f);
             }
             record(b, rs[b], nullptr);
         }
@@ -302,30 +361,45 @@ std::vector<std::pair<DTensor, DTensor>> CTM::decompose_batch(const std::vector<
     return out;
 }
 
+std::vector<std::pair<DTensor, DTensor>> CTM::projector_inputs_all(const std::vector<std::pair<int, int>>& xy, int dir)
+{
+    // get_projectors (CTM.cpp:1185-1337): the two corners of each site, all sites' corners batched
+    std::vector<std::array<int, 3>> req;
+    for(const auto& [x, y] : xy) switch(dir) {
+        case D_LEFT:
+            req.push_back({x, y, UPPER_LEFT});
+            req.push_back({x, y + 1, LOWER_LEFT});
+            break;
+        case D_RIGHT:
+            req.push_back({x, y, UPPER_RIGHT});
+            req.push_back({x, y + 1, LOWER_RIGHT});
+            break;
+        case D_TOP:
+            req.push_back({x, y, UPPER_LEFT});
+            req.push_back({x + 1, y, UPPER_RIGHT});
+            break;
+        default:
+            req.push_back({x, y, LOWER_LEFT});
+            req.push_back({x + 1, y, LOWER_RIGHT});
+            break;
+        }
+    const std::vector<DTensor> Q = contract_corners(req);
+    std::vector<std::pair<DTensor, DTensor>> out;
+    for(std::size_t s = 0; s < xy.size(); ++s) {
+        const DTensor &a = Q[2 * s], &b = Q[2 * s + 1];
+        switch(dir) {
+        case D_LEFT: out.emplace_back(tw(b, {3, 4, 5}), a); break;  // (Q4, Q1)
+        case D_RIGHT: out.emplace_back(a, b); break;                // (Q2, Q3)
+        case D_TOP: out.emplace_back(a, b); break;                  // (Q1, Q2)
+        default: out.emplace_back(b, tw(a, {0, 1, 2})); break;      // (Q3, Q4)
+        }
+    }
+    return out;
+}
+
 std::pair<DTensor, DTensor> CTM::projector_inputs(int x, int y, int dir)
 {
-    switch(dir) {
-    case D_LEFT: {
-        DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
-        DTensor Q4 = tw(contract_corner(x, y + 1, LOWER_LEFT), {3, 4, 5});
-        return {Q4, Q1};
-    }
-    case D_RIGHT: {
-        DTensor Q2 = contract_corner(x, y, UPPER_RIGHT);
-        DTensor Q3 = contract_corner(x, y + 1, LOWER_RIGHT);
-        return {Q2, Q3};
-    }
-    case D_TOP: {
-        DTensor Q1 = contract_corner(x, y, UPPER_LEFT);
-        DTensor Q2 = contract_corner(x + 1, y, UPPER_RIGHT);
-        return {Q1, Q2};
-    }
-    default: {
-        DTensor Q4 = tw(contract_corner(x, y, LOWER_LEFT), {0, 1, 2});
-        DTensor Q3 = contract_corner(x + 1, y, LOWER_RIGHT);
-        return {Q3, Q4};
-    }
-    }
+    return projector_inputs_all({{x, y}}, dir)[0];
 }
 
 std::pair<DTensor, DTensor> CTM::get_projectors(int x, int y, int dir)
@@ -336,17 +410,33 @@ std::pair<DTensor, DTensor> CTM::get_projectors(int x, int y, int dir)
 
 void CTM::get_projectors_all(const std::vector<std::pair<int, int>>& xy, int dir, std::vector<DTensor>& P1, std::vector<DTensor>& P2)
 {
-    std::vector<std::pair<DTensor, DTensor>> in;
+    std::vector<std::pair<DTensor, DTensor>> in = projector_inputs_all(xy, dir);
     std::vector<int> sites;
-    for(const auto& [x, y] : xy) {
-        in.push_back(projector_inputs(x, y, dir));
-        sites.push_back(idx(x, y));
-    }
+    for(const auto& [x, y] : xy) sites.push_back(idx(x, y));
     std::vector<std::pair<DTensor, DTensor>> out = decompose_batch(in, sites, dir);
     for(std::size_t b = 0; b < out.size(); ++b) {
         P1[sites[b]] = std::move(out[b].first);
         P2[sites[b]] = std::move(out[b].second);
     }
+}
+
+void CTM::run_chains(std::vector<Chain>& chains)
+{
+    std::size_t len = 0;
+    for(const Chain& c : chains) len = std::max(len, c.stages.size());
+    std::vector<DTensor> cur(chains.size());
+    for(std::size_t k = 0; k < len; ++k) {
+        std::vector<ContractJob> jobs;
+        std::vector<std::size_t> who;
+        for(std::size_t i = 0; i < chains.size(); ++i)
+            if(k < chains[i].stages.size()) {
+                jobs.push_back(chains[i].stages[k](cur[i]));
+                who.push_back(i);
+            }
+        std::vector<DTensor> res = contract_batch(jobs, opt.precision);
+        for(std::size_t j = 0; j < who.size(); ++j) cur[who[j]] = std::move(res[j]);
+    }
+    for(std::size_t i = 0; i < chains.size(); ++i) *chains[i].out = normalized(cur[i]);
 }
 
 void CTM::left_move()
@@ -359,17 +449,22 @@ void CTM::left_move()
             for(int y = 0; y < Ly; ++y) xy.emplace_back(x, y);
             get_projectors_all(xy, D_LEFT, P1, P2);
         }
+        std::vector<Chain> chains;
         for(int y = 0; y < Ly; ++y) {
             // renormalize_left (CTM.cpp:1339-1416)
-            DTensor C1T1 = ct(C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -4, -2, -3}, 3);
-            C1n[idx(x, y - 1)] = normalized(ct(tw(P1[idx(x, y - 1)], {1, 2, 3}), {-1, 1, 2, 3}, C1T1, {1, 2, 3, -2}, 0));
-            DTensor C4T3 = ct(C4[idx(x - 1, y + 1)], {-2, 1}, T3[idx(x, y + 1)], {-3, -4, 1, -1}, 1);
-            C4n[idx(x, y + 1)] = normalized(ct(C4T3, {-2, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -1}, 1));
-            DTensor P1T4 = ct(tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, T4[idx(x - 1, y)], {-4, 1, -5, -6}, 3);
-            DTensor P1T4A = ct(P1T4, {-1, 2, -2, -3, 1, -4}, A(x, y), {1, -5, -6, 2, -7}, 4);
-            DTensor P1T4AAdag = ct(P1T4A, {-1, 2, -4, 1, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, -6, 3, -3, 2}, 3);
-            T4n[idx(x, y)] = normalized(ct(P1T4AAdag, {-2, -3, -4, 1, 2, 3}, tw(P2[idx(x, y - 1)], {0, 1, 2}), {1, 2, 3, -1}, 1));
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{C1[idx(x - 1, y - 1)], {-1, 1}, T1[idx(x, y - 1)], {1, -4, -2, -3}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{tw(P1[idx(x, y - 1)], {1, 2, 3}), {-1, 1, 2, 3}, p, {1, 2, 3, -2}, 0}; }},
+                             &C1n[idx(x, y - 1)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{C4[idx(x - 1, y + 1)], {-2, 1}, T3[idx(x, y + 1)], {-3, -4, 1, -1}, 1}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-2, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -1}, 1}; }},
+                             &C4n[idx(x, y + 1)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, T4[idx(x - 1, y)], {-4, 1, -5, -6}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 2, -2, -3, 1, -4}, A(x, y), {1, -5, -6, 2, -7}, 4}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 2, -4, 1, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, -6, 3, -3, 2}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-2, -3, -4, 1, 2, 3}, tw(P2[idx(x, y - 1)], {0, 1, 2}), {1, 2, 3, -1}, 1}; }},
+                             &T4n[idx(x, y)]});
         }
+        run_chains(chains);
         for(int y = 0; y < Ly; ++y) {
             C1[idx(x, y)] = C1n[idx(x, y)];
             T4[idx(x, y)] = T4n[idx(x, y)];
@@ -389,17 +484,22 @@ void CTM::right_move()
             for(int y = 0; y < Ly; ++y) xy.emplace_back(x, y);
             get_projectors_all(xy, D_RIGHT, P1, P2);
         }
+        std::vector<Chain> chains;
         for(int y = 0; y < Ly; ++y) {
             // renormalize_right (CTM.cpp:1418-1488)
-            DTensor T1C2 = ct(T1[idx(x, y - 1)], {-1, 1, -3, -4}, C2[idx(x + 1, y - 1)], {1, -2}, 1);
-            C2n[idx(x, y - 1)] = normalized(ct(T1C2, {-1, 1, 2, 3}, P2[idx(x, y - 1)], {1, 2, 3, -2}, 1));
-            DTensor C3T3 = ct(tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 3);
-            C3n[idx(x, y + 1)] = normalized(ct(P1[idx(x, y)], {-1, 1, 2, 3}, C3T3, {1, 2, 3, -2}, 2));
-            DTensor P1T2 = ct(P1[idx(x, y - 1)], {-1, 1, -2, -3}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3);
-            DTensor P1T2A = ct(P1T2, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4);
-            DTensor P1T2AAdag = ct(P1T2A, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3);
-            T2n[idx(x, y)] = normalized(ct(P1T2AAdag, {-3, -1, -2, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -4}, 3));
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{T1[idx(x, y - 1)], {-1, 1, -3, -4}, C2[idx(x + 1, y - 1)], {1, -2}, 1}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, 2, 3}, P2[idx(x, y - 1)], {1, 2, 3, -2}, 1}; }},
+                             &C2n[idx(x, y - 1)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{tw(C3[idx(x + 1, y + 1)], {1}), {-1, 1}, T3[idx(x, y + 1)], {-2, -3, -4, 1}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{P1[idx(x, y)], {-1, 1, 2, 3}, p, {1, 2, 3, -2}, 2}; }},
+                             &C3n[idx(x, y + 1)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{P1[idx(x, y - 1)], {-1, 1, -2, -3}, T2[idx(x + 1, y)], {-4, -5, 1, -6}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, -2, 2, -3, -4}, tw(A(x, y), {2}), {-5, 1, 2, -6, -7}, 4}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, 2, -4, -2, -5, 3}, tw(Adag(x, y), {4}), {-3, 1, 3, 2, -6}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-3, -1, -2, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -4}, 3}; }},
+                             &T2n[idx(x, y)]});
         }
+        run_chains(chains);
         for(int y = 0; y < Ly; ++y) {
             C2[idx(x, y)] = C2n[idx(x, y)];
             T2[idx(x, y)] = T2n[idx(x, y)];
@@ -418,17 +518,22 @@ void CTM::top_move()
             for(int x = 0; x < Lx; ++x) xy.emplace_back(x, y);
             get_projectors_all(xy, D_TOP, P1, P2);
         }
+        std::vector<Chain> chains;
         for(int x = 0; x < Lx; ++x) {
             // renormalize_top (CTM.cpp:1490-1562)
-            DTensor T4C1 = ct(T4[idx(x - 1, y)], {1, -1, -3, -4}, tw(C1[idx(x - 1, y - 1)], {0}), {1, -2}, 1);
-            C1n[idx(x - 1, y)] = normalized(ct(T4C1, {-1, 1, 2, 3}, P2[idx(x - 1, y)], {1, 2, 3, -2}, 0));
-            DTensor C2T2 = ct(C2[idx(x + 1, y - 1)], {-1, 1}, T2[idx(x + 1, y)], {-2, -3, 1, -4}, 3);
-            C2n[idx(x + 1, y)] = normalized(ct(P1[idx(x, y)], {-1, 1, 2, 3}, C2T2, {1, 2, 3, -2}, 1));
-            DTensor P1T1 = ct(P1[idx(x - 1, y)], {-1, 1, -2, -3}, T1[idx(x, y - 1)], {1, -4, -5, -6}, 3);
-            DTensor P1T1A = ct(P1T1, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4);
-            DTensor P1T1AAdag = ct(P1T1A, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3);
-            T1n[idx(x, y)] = normalized(ct(P1T1AAdag, {-1, -3, -4, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -2}, 1));
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{T4[idx(x - 1, y)], {1, -1, -3, -4}, tw(C1[idx(x - 1, y - 1)], {0}), {1, -2}, 1}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, 2, 3}, P2[idx(x - 1, y)], {1, 2, 3, -2}, 0}; }},
+                             &C1n[idx(x - 1, y)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{C2[idx(x + 1, y - 1)], {-1, 1}, T2[idx(x + 1, y)], {-2, -3, 1, -4}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{P1[idx(x, y)], {-1, 1, 2, 3}, p, {1, 2, 3, -2}, 1}; }},
+                             &C2n[idx(x + 1, y)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{P1[idx(x - 1, y)], {-1, 1, -2, -3}, T1[idx(x, y - 1)], {1, -4, -5, -6}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, -2, -3, 2, -4}, A(x, y), {1, 2, -5, -6, -7}, 4}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, -4, 2, -5, -2, 3}, tw(Adag(x, y), {3, 4}), {1, 2, 3, -6, -3}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, -3, -4, 1, 2, 3}, P2[idx(x, y)], {1, 2, 3, -2}, 1}; }},
+                             &T1n[idx(x, y)]});
         }
+        run_chains(chains);
         for(int x = 0; x < Lx; ++x) {
             C1[idx(x, y)] = C1n[idx(x, y)];
             T1[idx(x, y)] = T1n[idx(x, y)];
@@ -448,17 +553,22 @@ void CTM::bottom_move()
             for(int x = 0; x < Lx; ++x) xy.emplace_back(x, y);
             get_projectors_all(xy, D_BOTTOM, P1, P2);
         }
+        std::vector<Chain> chains;
         for(int x = 0; x < Lx; ++x) {
             // renormalize_bottom (CTM.cpp:1564-1641)
-            DTensor T4C4 = ct(T4[idx(x - 1, y)], {-4, 1, -2, -3}, C4[idx(x - 1, y + 1)], {1, -1}, 3);
-            C4n[idx(x - 1, y)] = normalized(ct(tw(P1[idx(x - 1, y)], {1, 2, 3}), {-2, 1, 2, 3}, T4C4, {1, 2, 3, -1}, 1));
-            DTensor T2C3 = ct(T2[idx(x + 1, y)], {-3, -4, -1, 1}, C3[idx(x + 1, y + 1)], {1, -2}, 1);
-            C3n[idx(x + 1, y)] = normalized(ct(T2C3, {-1, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -2}, 2));
-            DTensor P1T3 = ct(tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, tw(T3[idx(x, y + 1)], {0}), {-4, -5, -6, 1}, 3);
-            DTensor P1T3A = ct(P1T3, {-1, 1, -2, 2, -3, -4}, A(x, y), {-5, -6, 1, 2, -7}, 4);
-            DTensor P1T3AAdag = ct(P1T3A, {-1, 1, 2, -4, -5, -2, 3}, tw(Adag(x, y), {3}), {-6, -3, 3, 1, 2}, 3);
-            T3n[idx(x, y)] = normalized(ct(P1T3AAdag, {-4, -1, -2, 1, 2, 3}, tw(P2[idx(x - 1, y)], {0, 1, 2}), {1, 2, 3, -3}, 3));
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{T4[idx(x - 1, y)], {-4, 1, -2, -3}, C4[idx(x - 1, y + 1)], {1, -1}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{tw(P1[idx(x - 1, y)], {1, 2, 3}), {-2, 1, 2, 3}, p, {1, 2, 3, -1}, 1}; }},
+                             &C4n[idx(x - 1, y)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{T2[idx(x + 1, y)], {-3, -4, -1, 1}, C3[idx(x + 1, y + 1)], {1, -2}, 1}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, 2, 3}, tw(P2[idx(x, y)], {0, 1, 2}), {1, 2, 3, -2}, 2}; }},
+                             &C3n[idx(x + 1, y)]});
+            chains.push_back({{[&, x, y](const DTensor&) { return ContractJob{tw(P1[idx(x, y)], {1, 2, 3}), {-1, 1, -2, -3}, tw(T3[idx(x, y + 1)], {0}), {-4, -5, -6, 1}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, -2, 2, -3, -4}, A(x, y), {-5, -6, 1, 2, -7}, 4}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-1, 1, 2, -4, -5, -2, 3}, tw(Adag(x, y), {3}), {-6, -3, 3, 1, 2}, 3}; },
+                              [&, x, y](const DTensor& p) { return ContractJob{p, {-4, -1, -2, 1, 2, 3}, tw(P2[idx(x - 1, y)], {0, 1, 2}), {1, 2, 3, -3}, 3}; }},
+                             &T3n[idx(x, y)]});
         }
+        run_chains(chains);
         for(int x = 0; x < Lx; ++x) {
             C4[idx(x, y)] = C4n[idx(x, y)];
             T3[idx(x, y)] = T3n[idx(x, y)];

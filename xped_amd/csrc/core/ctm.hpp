@@ -40,6 +40,9 @@ struct CTMOptions {
     // SVD of the same (site, direction) are used
     std::function<std::map<Q, int>(const DTensor& prod, int site, int dir)> rank_policy;
     bool record_spectra = false;
+    // learned ranks: read the network's features from the previous decomposition of the same
+    // (site, direction) when it covers every sector (no sketch of its own), else from a fresh sketch
+    bool predictor_reuses_spectra = true;
     bool check_finite = false;  // check every projector factor for non-finite entries (diagnostics)
     bool verbose = false;       // one line per projector decomposition on stderr
 };
@@ -81,8 +84,9 @@ struct RankPredictor {
     std::map<Q, int> predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const;
     // predict() of several tensors with their sketches batched into shared grouped launches
     std::vector<std::map<Q, int>> predict_batch(const std::vector<const DTensor*>& prods, std::size_t chi, const std::vector<std::uint64_t>& seeds) const;
-    // the network and cut search on a finished sketch
-    std::map<Q, int> ranks_from_sketch(const DTensor& prod, const RandomizedSVD& r, std::size_t chi) const;
+    // the network and cut search on given leading singular values per sector (a fresh sketch's,
+    // or those of the previous decomposition of the same tensor)
+    std::map<Q, int> ranks_from_spectra(const TensorStructure& s, const std::map<Q, std::vector<double>>& spec, std::size_t chi) const;
     // predicted counts above the cut 10^log_cut * s_glob for explicit sketches (row-major nsec x ntop)
     std::vector<double> counts(const std::vector<std::vector<double>>& top, const std::vector<int>& dims, double s_glob, double log_cut) const;
     // the same for several cuts at once (cut-major rows)
@@ -120,6 +124,9 @@ public:
     std::vector<DTensor> C1, C2, C3, C4, T1, T2, T3, T4, rho_h, rho_v;
     std::vector<SpectrumRecord> spectra;
     std::map<std::pair<int, int>, std::map<Q, int>> last_kept;  // (site, dir) -> kept counts of the last full SVD
+    // (site, dir) -> the leading computed singular values per sector of its last decomposition
+    std::map<std::pair<int, int>, std::map<Q, std::vector<double>>> last_spectra;
+    long n_predicted_from_previous = 0, n_predicted_from_sketch = 0;
     std::shared_ptr<RankPredictor> predictor;  // learned target ranks (randomized mode) when set
     double svd_ms = 0.0;  // host-timed SVD phase (diagnostics)
     long n_rsvd = 0, n_fallback = 0;
@@ -142,6 +149,11 @@ private:
     Qbasis ket(int x, int y, int leg) const;
     Qbasis bra(int x, int y, int leg) const;
     DTensor contract_corner(int x, int y, int corner) const;
+    // stage (0..3) of contractCorner for (x, y, corner), given the previous stage's result
+    ContractJob corner_stage(int x, int y, int corner, int stage, const DTensor& prev) const;
+    // contract_corner of several (x, y, corner) with each stage's products in one launch
+    std::vector<DTensor> contract_corners(const std::vector<std::array<int, 3>>& req) const;
+    std::vector<std::pair<DTensor, DTensor>> projector_inputs_all(const std::vector<std::pair<int, int>>& xy, int dir);
     std::pair<DTensor, DTensor> decompose(const DTensor& T1, const DTensor& T2, int site, int dir);
     // decompose() of several (T1, T2) pairs of one move; the randomized SVDs run as one batch
     std::vector<std::pair<DTensor, DTensor>> decompose_batch(const std::vector<std::pair<DTensor, DTensor>>& in, const std::vector<int>& sites,
@@ -155,6 +167,15 @@ private:
     void top_move();
     void bottom_move();
     DTensor normalized(const DTensor& t) const;
+    void keep_spectrum(int site, int dir, const FullSVD& f);
+    // a chain of contractions, stage k contracting stage k - 1's result, whose normalized result
+    // goes to *out; run_chains advances independent chains in lockstep, each stage's products in
+    // one GEMM launch
+    struct Chain {
+        std::vector<std::function<ContractJob(const DTensor&)>> stages;
+        DTensor* out;
+    };
+    void run_chains(std::vector<Chain>& chains);
     void check_normalized() const;  // throws for a vanished or non-finite norm met by normalized()
 };
 
@@ -168,6 +189,7 @@ struct RandomizedSVD {
     SVDResult res;
     bool safe = true;
     double max_residual_ratio = 0.0;  // max_q sigma_l(B_q) / sigma_cut over the sketched sectors
+    std::map<Q, std::vector<double>> ritz;  // every computed singular value per sector (descending)
 };
 RandomizedSVD tsvd_randomized(const DTensor& t, const std::map<Q, int>& ranks, std::size_t max_keep, double eps_svd, const RsvdOptions& o,
                               std::uint64_t seed);

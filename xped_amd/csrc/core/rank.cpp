@@ -95,7 +95,7 @@ std::map<Q, int> sketch_ranks(const TensorStructure& s)
 std::map<Q, int> RankPredictor::predict(const DTensor& prod, std::size_t chi, std::uint64_t seed) const
 {
     RandomizedSVD r = tsvd_randomized(prod, sketch_ranks(*prod.st), static_cast<std::size_t>(-1) / 2, 0.0, sketch_options(), seed);
-    return ranks_from_sketch(prod, r, chi);
+    return ranks_from_spectra(*prod.st, r.res.spec, chi);
 }
 
 std::vector<std::map<Q, int>> RankPredictor::predict_batch(const std::vector<const DTensor*>& prods, std::size_t chi,
@@ -106,13 +106,12 @@ std::vector<std::map<Q, int>> RankPredictor::predict_batch(const std::vector<con
     for(const DTensor* p : prods) sk.push_back(sketch_ranks(*p->st));
     std::vector<RandomizedSVD> rs = tsvd_randomized_batch(prods, sk, static_cast<std::size_t>(-1) / 2, 0.0, sketch_options(), seeds);
     std::vector<std::map<Q, int>> out;
-    for(std::size_t b = 0; b < prods.size(); ++b) out.push_back(ranks_from_sketch(*prods[b], rs[b], chi));
+    for(std::size_t b = 0; b < prods.size(); ++b) out.push_back(ranks_from_spectra(*prods[b]->st, rs[b].res.spec, chi));
     return out;
 }
 
-std::map<Q, int> RankPredictor::ranks_from_sketch(const DTensor& prod, const RandomizedSVD& r, std::size_t chi) const
+std::map<Q, int> RankPredictor::ranks_from_spectra(const TensorStructure& s, const std::map<Q, std::vector<double>>& spec, std::size_t chi) const
 {
-    const TensorStructure& s = *prod.st;
     std::vector<std::vector<double>> top;
     std::vector<int> dims;
     std::vector<Q> qs;
@@ -121,9 +120,12 @@ std::map<Q, int> RankPredictor::ranks_from_sketch(const DTensor& prod, const Ran
         const int dim = static_cast<int>(std::min(s.rows[i], s.cols[i]));
         if(dim == 0) continue;
         std::vector<double> t;
-        auto it = r.res.spec.find(s.sector[i]);
-        if(it != r.res.spec.end()) t.assign(it->second.begin(), it->second.begin() + std::min<std::size_t>(it->second.size(), RANK_TOP));
-        std::sort(t.begin(), t.end(), std::greater<double>());
+        auto it = spec.find(s.sector[i]);
+        if(it != spec.end()) {
+            t = it->second;
+            std::sort(t.begin(), t.end(), std::greater<double>());
+            if(t.size() > static_cast<std::size_t>(RANK_TOP)) t.resize(RANK_TOP);
+        }
         if(!t.empty()) s_glob = std::max(s_glob, t[0]);
         top.push_back(std::move(t));
         dims.push_back(dim);

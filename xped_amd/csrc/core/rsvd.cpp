@@ -205,7 +205,10 @@ std::vector<RandomizedSVD> tsvd_randomized_batch(const std::vector<const DTensor
         std::vector<std::pair<Q, double>> all_sv;
         for(const auto& c : secs)
             if(c.t == ti)
-                for(int64_t j = 0; j < c.l; ++j) all_sv.emplace_back(s.sector[c.i], sv[c.s + j]);
+                for(int64_t j = 0; j < c.l; ++j) {
+                    all_sv.emplace_back(s.sector[c.i], sv[c.s + j]);
+                    out.ritz[s.sector[c.i]].push_back(sv[c.s + j]);
+                }
         TruncRule rule = truncation_rule(s.sym, all_sv, max_keep, eps_svd, false, 0.0, 0.0);
         double sigma_cut = std::numeric_limits<double>::infinity();
         for(const auto& [q, v] : rule.kept)

@@ -456,6 +456,93 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision)
     return out;
 }
 
+std::vector<DTensor> mul_batch(const std::vector<std::pair<DTensor, DTensor>>& pairs, int precision)
+{
+    // operator* of independent pairs in one grouped launch: every pair's items, with their offsets
+    // taken from the lowest data address of the left operands, right operands and outputs
+    const std::size_t n = pairs.size();
+    std::vector<DTensor> outs(n);
+    if(n == 0) return outs;
+    if(n == 1) {
+        outs[0] = mul(pairs[0].first, pairs[0].second, precision);
+        return outs;
+    }
+    Runtime* rt = pairs[0].first.rt;
+    std::vector<std::shared_ptr<CachedGemm>> cgs(n);
+    for(std::size_t i = 0; i < n; ++i) {
+        if(pairs[i].first.rt != rt || pairs[i].second.rt != rt) throw Error(XH_EINVAL, "mul_batch: tensors of different runtimes");
+        cgs[i] = get_gemm_plan(rt, *pairs[i].first.st, *pairs[i].second.st);
+        outs[i] = DTensor::empty(rt, cgs[i]->plan.out);
+    }
+    if(rt->ctx) {
+        const double *A0 = nullptr, *B0 = nullptr;
+        double* C0 = nullptr;
+        for(std::size_t i = 0; i < n; ++i) {
+            if(cgs[i]->plan.items.empty()) continue;
+            const double *a = pairs[i].first.data(), *b = pairs[i].second.data();
+            double* c = outs[i].data();
+            if(!A0 || a < A0) A0 = a;
+            if(!B0 || b < B0) B0 = b;
+            if(!C0 || c < C0) C0 = c;
+        }
+        std::vector<xh_gemm_item> items;
+        double flops = 0.0;
+        for(std::size_t i = 0; i < n; ++i) {
+            const CachedGemm& cg = *cgs[i];
+            if(outs[i].size() > 0)
+                for(const auto& z : cg.zero_rects)
+                    dcheck(rt->ctx, xh_fill(rt->ctx, outs[i].data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
+            if(cg.plan.items.empty()) continue;
+            const int64_t da = pairs[i].first.data() - A0, db = pairs[i].second.data() - B0, dc = outs[i].data() - C0;
+            for(const auto& g : cg.plan.items) {
+                items.push_back(xh_gemm_item{g.a_off + da, g.b_off + db, g.c_off + dc, g.m, g.n, g.k});
+                flops += 2.0 * g.m * double(g.n) * g.k;
+                if(rt->time_gemm) rt->gemm_log.push_back({static_cast<int32_t>(g.m), static_cast<int32_t>(g.n), static_cast<int32_t>(g.k)});
+            }
+        }
+        if(!items.empty()) {
+            xh_gemm_plan* plan = nullptr;
+            dcheck(rt->ctx, xh_gemm_plan_create(rt->ctx, items.data(), static_cast<int64_t>(items.size()), 0, 0, &plan), "batched gemm plan");
+            if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 1);  // timed contraction products carry tag 1
+            const int rc = xh_gemm(rt->ctx, plan, A0, B0, C0, 1.0, 0.0, precision);
+            if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 0);
+            xh_gemm_plan_destroy(rt->ctx, plan);  // its tables are freed after the launch
+            dcheck(rt->ctx, rc, "batched grouped gemm");
+            rt->gemm_flops += flops;
+            rt->n_gemm++;
+        }
+    }
+    for(std::size_t i = 0; i < n; ++i)
+        if(Tape* tp = recording(pairs[i].first))
+            tp->backs.push_back([l = pairs[i].first, r = pairs[i].second, out = outs[i], precision](Tape& tape) {
+                const DTensor* d = tape.find(out);
+                if(!d) return;
+                const DTensor dc = *d;
+                DTensor dl = tape.adjoint_of(l);
+                DTensor dr = tape.adjoint_of(r);
+                mul_bwd(l, r, dc, &dl, &dr, precision);
+            });
+    return outs;
+}
+
+std::vector<DTensor> contract_batch(const std::vector<ContractJob>& jobs, int precision)
+{
+    // Tensor::contract of independent jobs: their permutes one by one, the products in one launch
+    std::vector<std::pair<DTensor, DTensor>> pairs;
+    std::vector<ContractPerms> perms;
+    for(const ContractJob& j : jobs) {
+        if(static_cast<int>(j.a1.size()) != j.a.st->rank + j.a.st->corank || static_cast<int>(j.a2.size()) != j.b.st->rank + j.b.st->corank)
+            throw Error(XH_EINVAL, "contract: index list length does not match the tensor rank");
+        ContractPerms cp = get_permutations(j.a1, j.a.st->rank, j.a2, j.b.st->rank, j.rank);
+        pairs.emplace_back(permute(j.a, cp.shift1, cp.p1), permute(j.b, cp.shift2, cp.p2));
+        perms.push_back(cp);
+    }
+    std::vector<DTensor> prods = mul_batch(pairs, precision);
+    std::vector<DTensor> out;
+    for(std::size_t i = 0; i < jobs.size(); ++i) out.push_back(permute(prods[i], perms[i].shiftres, perms[i].pres));
+    return out;
+}
+
 void mul_into(const DTensor& l, const DTensor& r, DTensor& out, int precision)
 {
     auto cg = get_gemm_plan(l.rt, *l.st, *r.st);

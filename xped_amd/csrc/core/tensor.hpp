@@ -154,6 +154,17 @@ DTensor mul(const DTensor& l, const DTensor& r, int precision);
 // can be captured in a graph once the plan exists
 void mul_into(const DTensor& l, const DTensor& r, DTensor& out, int precision);
 DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision);
+// operator* of independent pairs in one grouped GEMM launch (each product as mul computes it)
+std::vector<DTensor> mul_batch(const std::vector<std::pair<DTensor, DTensor>>& pairs, int precision);
+struct ContractJob {
+    DTensor a;
+    std::vector<int> a1;
+    DTensor b;
+    std::vector<int> a2;
+    int rank = 0;
+};
+// contract() of independent jobs with their products in one grouped GEMM launch
+std::vector<DTensor> contract_batch(const std::vector<ContractJob>& jobs, int precision);
 DTensor adjoint(const DTensor& t);
 DTensor twist(const DTensor& t, int leg);
 DTensor scaled(const DTensor& t, double alpha);

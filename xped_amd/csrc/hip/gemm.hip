@@ -297,8 +297,10 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny(GemmArgs g, const Ti
     }
 }
 
-// rows per thread of the skinny kernel for a register width ms
-constexpr int sk_rows(int ms) { return ms <= 16 ? 4 : 2; }
+// long-side indices per thread of the skinny kernel: one.  (Two or four, processed one after the
+// other, halved the workgroups without adding loads in flight: on single items of 262144 x 20 x 20
+// and 65536 x 20 x 20 one index per thread took 28.8 and 13.3 us against 30.5 and 21.0 us.)
+constexpr int sk_rows(int) { return 1; }
 
 // ---------------------------------------------------------------- FP64, 128 x 128 tiles
 // For items with both output sides >= 192: 8 waves in 2 x 4, each owning 64 x 32 (4 x 2 MFMA

@@ -681,8 +681,9 @@ class CTM:
             f(h)
             self.h = None
 
-    def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=1, record_spectra=False, check_finite=False, verbose=False):
-        flags = int(bool(record_spectra)) | (2 if check_finite else 0) | (4 if verbose else 0)
+    def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=1, record_spectra=False, check_finite=False, verbose=False,
+                    fresh_predictor_sketch=False):
+        flags = int(bool(record_spectra)) | (2 if check_finite else 0) | (4 if verbose else 0) | (8 if fresh_predictor_sketch else 0)
         check(lib().xt_ctm_set_options(self.h, PRECISION[precision], 1 if svd == "randomized" else 0, oversample, power_iters, flags),
               self.rt.h, "set_options")
         return self
@@ -699,6 +700,12 @@ class CTM:
         p = np.ascontiguousarray(params, np.float32)
         check(lib().xt_ctm_set_predictor(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size, int(width), float(margin)), self.rt.h, "set_predictor")
         return self
+
+    def predictor_stats(self):
+        """(decompositions whose learned ranks read the previous spectrum, ones that sketched afresh)"""
+        a, b = C.c_long(), C.c_long()
+        lib().xt_ctm_predictor_stats(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def rsvd_stats(self):
         """(randomized SVDs run, full-SVD fallbacks, largest missed-weight ratio)"""
