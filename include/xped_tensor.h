@@ -233,7 +233,8 @@ int xt_ctm_destroy(xt_ctm* c);
  * bit 1 checks every projector factor for non-finite entries, bit 2 prints one line per
  * projector decomposition to stderr, bit 3 makes the learned ranks always sketch their features
  * afresh (by default they reuse the previous decomposition's spectrum of the same site and
- * direction when it covers every sector) */
+ * direction when it covers every sector), bit 4 runs the independent contractions of a move in
+ * lockstep with shared GEMM launches (by default they run one at a time) */
 int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, int power_iters, int flags);
 int xt_ctm_init(xt_ctm* c);                         /* CTM::init, FROM_A (CTM.cpp:137-240) */
 int xt_ctm_grow_all(xt_ctm* c);                     /* CTM::grow_all (CTM.cpp:386-420) */

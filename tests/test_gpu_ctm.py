@@ -87,6 +87,32 @@ def test_randomized_svd_step_matches_full_step(rt):
     assert abs(Er - Ef) <= 1e-10 * abs(Ef), (Er, Ef, nfb, ratio)
 
 
+def test_batched_move_contractions_give_the_same_steps(rt):
+    # the independent contractions of a move in lockstep, each stage's products in one grouped GEMM
+    # launch (mul_batch: offsets from the lowest data addresses of several tensors), against one
+    # contraction at a time.  The split-K chunking of a shared launch differs, so results differ in
+    # the last bits; energies must agree to 1e-10 while every decomposition keeps the same counts,
+    # and a first difference in kept counts may only be a +q/-q partner swap at the cut
+    _, A = make_ipeps(rt, configs.c4_small_virtual(), configs.c4_physical(), 7)
+    chi = 24
+    runs = []
+    for batch in (False, True):
+        c = xt.CTM(rt, A, chi).set_options(batch_contractions=batch, record_spectra=True).init()
+        E = [c.step() for _ in range(3)]
+        kept = [{s["q"]: s["kept"] for s in r["sectors"] if s["kept"] > 0} for r in c.spectra()]
+        runs.append((E, kept))
+    (E0, k0), (E1, k1) = runs
+    assert len(k0) == len(k1) == 48
+    first = next((i for i, (a, b) in enumerate(zip(k0, k1)) if a != b), None)
+    if first is not None:
+        diff = {q: k1[first].get(q, 0) - k0[first].get(q, 0) for q in set(k0[first]) | set(k1[first])}
+        moved = sorted((d, q) for q, d in diff.items() if d)
+        assert len(moved) == 2 and moved[0][0] == -1 and moved[1][0] == 1 and moved[0][1] == -moved[1][1], moved
+    for step, (a, b) in enumerate(zip(E0, E1)):
+        same = first is None or first >= 16 * (step + 1)
+        assert abs(a - b) <= (1e-10 if same else 1e-6) * abs(a), (step, E0, E1, first)
+
+
 def test_bounded_plan_cache_gives_the_same_steps():
     # a runtime whose plan caches hold at most 8 plans drops and rebuilds plans all through the
     # steps (device tables freed in stream order while launches may still be queued); energies must

@@ -1066,6 +1066,7 @@ int xt_ctm_set_options(xt_ctm* c, int precision, int svd_mode, int oversample, i
         o.check_finite = (flags & 2) != 0;
         o.verbose = (flags & 4) != 0;
         o.predictor_reuses_spectra = (flags & 8) == 0;
+        o.batch_contractions = (flags & 16) != 0;
         c->ctm->set_options(o);
     });
 }

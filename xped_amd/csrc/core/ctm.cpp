@@ -186,6 +186,14 @@ ContractJob CTM::corner_stage(int x, int y, int corner, int stage, const DTensor
     }
 }
 
+std::vector<DTensor> CTM::batched(const std::vector<ContractJob>& jobs) const
+{
+    if(opt.batch_contractions) return contract_batch(jobs, opt.precision);
+    std::vector<DTensor> out;
+    for(const ContractJob& j : jobs) out.push_back(contract(j.a, j.a1, j.b, j.a2, j.rank, opt.precision));
+    return out;
+}
+
 std::vector<DTensor> CTM::contract_corners(const std::vector<std::array<int, 3>>& req) const
 {
     // the corners of several sites in lockstep: each stage's contractions share one GEMM launch
@@ -193,7 +201,7 @@ std::vector<DTensor> CTM::contract_corners(const std::vector<std::array<int, 3>>
     for(int stage = 0; stage < 4; ++stage) {
         std::vector<ContractJob> jobs;
         for(std::size_t i = 0; i < req.size(); ++i) jobs.push_back(corner_stage(req[i][0], req[i][1], req[i][2], stage, cur[i]));
-        cur = contract_batch(jobs, opt.precision);
+        cur = batched(jobs);
     }
     return cur;
 }
@@ -433,7 +441,7 @@ void CTM::run_chains(std::vector<Chain>& chains)
                 jobs.push_back(chains[i].stages[k](cur[i]));
                 who.push_back(i);
             }
-        std::vector<DTensor> res = contract_batch(jobs, opt.precision);
+        std::vector<DTensor> res = batched(jobs);
         for(std::size_t j = 0; j < who.size(); ++j) cur[who[j]] = std::move(res[j]);
     }
     for(std::size_t i = 0; i < chains.size(); ++i) *chains[i].out = normalized(cur[i]);

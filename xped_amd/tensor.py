@@ -682,8 +682,9 @@ class CTM:
             self.h = None
 
     def set_options(self, precision="fp64", svd="full", oversample=8, power_iters=1, record_spectra=False, check_finite=False, verbose=False,
-                    fresh_predictor_sketch=False):
-        flags = int(bool(record_spectra)) | (2 if check_finite else 0) | (4 if verbose else 0) | (8 if fresh_predictor_sketch else 0)
+                    fresh_predictor_sketch=False, batch_contractions=False):
+        flags = (int(bool(record_spectra)) | (2 if check_finite else 0) | (4 if verbose else 0) | (8 if fresh_predictor_sketch else 0)
+                 | (16 if batch_contractions else 0))
         check(lib().xt_ctm_set_options(self.h, PRECISION[precision], 1 if svd == "randomized" else 0, oversample, power_iters, flags),
               self.rt.h, "set_options")
         return self
