@@ -175,7 +175,8 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
     if mode == "fast":
         # one more step with every GEMM and permute launch timed by HIP events on the context stream; marker
-        # kernels of 1 and 2 workgroups delimit it in a kernel trace
+        # kernels of 1 and 2 workgroups delimit it in a kernel trace, and inside it markers of 3 and 4
+        # workgroups bracket each contraction product (the tensor layer's tagged GEMM calls)
         rt.gemm_timing(True, reset=True)
         rt.permute_timing(True, reset=True)
         rt.marker(1)

@@ -438,6 +438,21 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
 
 static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r, DTensor& out, int precision);
 
+// While GEMM launches are timed, contraction products carry tag 1 in the launch log and are
+// bracketed in a kernel trace by marker kernels of 3 (before) and 4 (after) workgroups, so that
+// per-dispatch counters can be split between contractions and the other grouped GEMM calls.
+static void tag_contraction(Runtime* rt, bool begin)
+{
+    if(!rt->time_gemm) return;
+    if(begin) {
+        dcheck(rt->ctx, xh_trace_marker(rt->ctx, 3), "trace marker");
+        xh_set_gemm_tag(rt->ctx, 1);
+    } else {
+        xh_set_gemm_tag(rt->ctx, 0);
+        dcheck(rt->ctx, xh_trace_marker(rt->ctx, 4), "trace marker");
+    }
+}
+
 DTensor mul(const DTensor& l, const DTensor& r, int precision)
 {
     auto cg = get_gemm_plan(l.rt, *l.st, *r.st);
@@ -503,9 +518,9 @@ std::vector<DTensor> mul_batch(const std::vector<std::pair<DTensor, DTensor>>& p
         if(!items.empty()) {
             xh_gemm_plan* plan = nullptr;
             dcheck(rt->ctx, xh_gemm_plan_create(rt->ctx, items.data(), static_cast<int64_t>(items.size()), 0, 0, &plan), "batched gemm plan");
-            if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 1);  // timed contraction products carry tag 1
+            tag_contraction(rt, true);
             const int rc = xh_gemm(rt->ctx, plan, A0, B0, C0, 1.0, 0.0, precision);
-            if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 0);
+            tag_contraction(rt, false);
             xh_gemm_plan_destroy(rt->ctx, plan);  // its tables are freed after the launch
             dcheck(rt->ctx, rc, "batched grouped gemm");
             rt->gemm_flops += flops;
@@ -559,9 +574,9 @@ static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r,
             for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
         }
         Runtime* rt = l.rt;
-        if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 1);  // timed contraction products carry tag 1
+        tag_contraction(rt, true);
         const int rc = xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision);
-        if(rt->time_gemm) xh_set_gemm_tag(rt->ctx, 0);
+        tag_contraction(rt, false);
         dcheck(rt->ctx, rc, "grouped gemm");
         if(rt->time_gemm)
             for(const auto& g : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(g.m), static_cast<int32_t>(g.n), static_cast<int32_t>(g.k)});
