@@ -79,6 +79,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
     ap.add_argument("--no-su", action="store_true", help="c4: skip the simple-update section")
+    ap.add_argument("--no-fuse", action="store_true", help="c4: materialise every permuted operand (no gather GEMM); same results bit for bit")
     ap.add_argument("--no-previous", action="store_true", help="c4: skip the arms with the previous step's kept counts as target ranks and with freshly sketched predictor features")
     return ap.parse_args()
 
@@ -116,6 +117,7 @@ def launch_split(rt, xt):
 def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
     rt = xt.Runtime(0)
+    rt.fuse_permute(not args.no_fuse)
     if args.autotune and mode == "fast":
         xt.lib().xh_set_gemm_autotune(rt.ctx(), 1)
     A = [a.scale(SCALE) for a in random_ipeps(rt, configs.c4_virtual(), configs.c4_physical(), args.seed)]
@@ -154,6 +156,7 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     svd0 = c.svd_ms()
     c0 = rt.counters()
     h0 = rt.host_stats()
+    f0 = rt.fuse_permute(None)
     t0 = time.perf_counter()
     for k in range(nsteps):
         energies.append(c.step())
@@ -162,6 +165,7 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     wall = time.perf_counter() - t0
     c1 = rt.counters()
     h1 = rt.host_stats()
+    f1 = rt.fuse_permute(None)
     if spectra is None:  # no warm-up step: the first timed step recorded (16 decompositions per step)
         spectra = c.spectra()[:16]
         configure(False)
@@ -172,7 +176,8 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
            "symbolic_plan_hits_per_step": (h1["symbolic_hits"] - h0["symbolic_hits"]) / nsteps, "svd_ms_per_step": (c.svd_ms() - svd0) / nsteps, "energies": energies,
            "predictor_features_previous_sketch": list(c.predictor_stats()),
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
-           "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats()}
+           "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats(),
+           "fused_contractions_per_step": (f1 - f0) / nsteps}
     if mode == "fast":
         # one more step with every GEMM and permute launch timed by HIP events on the context stream; marker
         # kernels of 1 and 2 workgroups delimit it in a kernel trace, and inside it markers of 3 and 4
@@ -290,6 +295,7 @@ def run_c4(args):
         "fast": {"ms_per_step": round(fast["ms_per_step"], 3), "svd_ms_per_step": round(fast["svd_ms_per_step"], 3),
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
+                 "fused_contractions_per_step": fast["fused_contractions_per_step"],
                  "predictor_features_previous_sketch": fast["predictor_features_previous_sketch"],
                  **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],

@@ -38,6 +38,7 @@ enum { XH_FP64 = 0, XH_FP32 = 1, XH_BF16X3 = 2, XH_OZAKI = 3 };
 typedef struct xh_ctx xh_ctx;
 typedef struct xh_permute_plan xh_permute_plan;
 typedef struct xh_gemm_plan xh_gemm_plan;
+typedef struct xh_gather_plan xh_gather_plan;
 
 /* ---- context and buffers ------------------------------------------------------------ */
 int xh_create(int device, xh_ctx** out);
@@ -116,6 +117,37 @@ int xh_gemm_plan_workspace(const xh_gemm_plan* plan, int64_t* bytes);
  * built by a plan's first xh_gemm call in such a mode; without it the call fails with XH_EINVAL. */
 #define XH_GEMM_SCRATCH_BYTES ((size_t)32 << 20)
 int xh_gemm_noalloc(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, const double* B, double* C, double alpha, double beta, int precision);
+/* ---- permute fused into the grouped GEMM (gather GEMM) -----------------------------------
+ * Tensor::contract (Tensor.hpp:328-351) permutes its left operand and then multiplies
+ * (TensorBase.cpp:262-279).  xh_gemm_gather computes C = alpha * permute(src) * B + beta * C for
+ * every item of a GEMM plan without materialising permute(src): each item's A block is read
+ * element by element through the permute's task tables.  No counterpart in the reference.
+ *   - perm: the permute of src (as for xh_permute_plan_create); every task must have exactly one
+ *     source with coefficient +1 or -1 (abelian symmetries);
+ *   - items[j], for GEMM item j: its A block is the permute's output sector whose tasks are
+ *     task_base + a * ncod + b (domain tree a < ndom, codomain tree b < ncod); starts[starts_off ..]
+ *     holds the ndom + 1 domain-tree row starts, then the ncod + 1 codomain-tree column starts;
+ *   - src_elems: size of src in doubles (every source access is checked against it here).
+ * Only plans whose every item is a skinny FP64 item with A the long operand (k <= 32, n <= 32 < m)
+ * and no transposes qualify (xh_gemm_gather_eligible); the result equals
+ * xh_permute + xh_gemm(XH_FP64) bit for bit. */
+typedef struct {
+    int32_t task_base, ndom, ncod, pad;
+    int64_t starts_off;
+} xh_gather_item;
+typedef struct {
+    xh_permute_desc perm;
+    int64_t nitems;
+    const xh_gather_item* items;
+    int64_t nstarts;
+    const int32_t* starts;
+    int64_t src_elems;
+} xh_gather_desc;
+int xh_gemm_gather_eligible(const xh_gemm_plan* plan, int* ok);
+int xh_gather_plan_create(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather_desc* desc, xh_gather_plan** out);
+int xh_gather_plan_destroy(xh_ctx* ctx, xh_gather_plan* plan);
+int xh_gemm_gather(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather_plan* gather, const double* src, int64_t src_elems, const double* B, double* C,
+                   double alpha, double beta);
 /* GEMM tile autotuner (no counterpart in the reference, whose per-block BLAS calls have no tiles
  * to choose): with enable != 0, the first FP64 call of a plan that has 128 x 128 tiles, with
  * beta = 0, runs both tilings (128 x 128 + 64 x 64, and 64 x 64 only), timed with HIP events

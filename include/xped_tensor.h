@@ -45,6 +45,13 @@ int xt_rt_clear_caches(xt_rt* rt);
  * dropped, device tables freed in stream order (no counterpart in the reference, which keeps no
  * device plans); default 2048, at least 8 */
 int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans);
+/* contract() with the permute of its left operand read inside the product (xh_gemm_gather)
+ * wherever the product qualifies (FP64, every sector block a skinny item with the permuted operand
+ * on its long side, single-source +-1 permute, no tape recording): enable = 1 (the default) /
+ * 0 / -1 keeps the setting; *n_fused (may be null) = contractions run fused so far.  Results are
+ * the same bit for bit either way; the reference always materialises the permuted operand
+ * (Tensor.hpp:328-351). */
+int xt_rt_set_fuse_permute(xt_rt* rt, int enable, long* n_fused);
 /* current cached permute plans, GEMM plans, and plans dropped so far */
 int xt_rt_cache_sizes(const xt_rt* rt, int64_t* permute_plans, int64_t* gemm_plans, int64_t* evicted);
 /* host time (seconds) spent building permute and GEMM plans, and how many were built */

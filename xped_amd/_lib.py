@@ -85,6 +85,10 @@ _SIGS = [
     ("xh_gemm_plan_flops", C.c_int, [_P, _DP]),
     ("xh_gemm_plan_workspace", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("xh_gemm_noalloc", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]),
+    ("xh_gemm_gather_eligible", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("xh_gather_plan_create", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    ("xh_gather_plan_destroy", C.c_int, [_P, _P]),
+    ("xh_gemm_gather", C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, C.c_double, C.c_double]),
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_eigh", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
@@ -127,6 +131,7 @@ _SIGS = [
     ("xt_rt_counters", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_clear_caches", C.c_int, [_P]),
     ("xt_rt_set_max_plans", C.c_int, [_P, C.c_int64]),
+    ("xt_rt_set_fuse_permute", C.c_int, [_P, C.c_int, C.POINTER(C.c_long)]),
     ("xt_rt_cache_sizes", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("xt_rt_host_stats", C.c_int, [_P, _DP, _DP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("xt_rt_host_timers", C.c_int, [_P, _DP, C.c_int]),

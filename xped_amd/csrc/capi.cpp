@@ -203,6 +203,15 @@ int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans)
     });
 }
 
+int xt_rt_set_fuse_permute(xt_rt* rt, int enable, long* n_fused)
+{
+    return run(rt, [&] {
+        if(enable < -1 || enable > 1) throw xa::Error(XH_EINVAL, "xt_rt_set_fuse_permute: enable must be -1, 0 or 1");
+        if(enable >= 0) rt->rt.fuse_permute = enable != 0;
+        if(n_fused) *n_fused = rt->rt.n_fused;
+    });
+}
+
 int xt_rt_cache_sizes(const xt_rt* rt, int64_t* permute_plans, int64_t* gemm_plans, int64_t* evicted)
 {
     if(!rt) return XH_EINVAL;
@@ -216,6 +225,10 @@ int xt_rt_clear_caches(xt_rt* rt)
 {
     return run(rt, [&] {
         if(rt->rt.ctx) xh_synchronize(rt->rt.ctx);
+        // gather plans first: each refers to a GEMM plan of the cache
+        for(auto& [k, v] : rt->rt.gather_cache)
+            if(v->dev) xh_gather_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
+        rt->rt.gather_cache.clear();
         for(auto& [k, v] : rt->rt.permute_cache)
             if(v->dev) xh_permute_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
         for(auto& [k, v] : rt->rt.gemm_cache)

@@ -1,4 +1,5 @@
 // Device-resident tensor operations: structure on the host, data movement and arithmetic on the GPU.
+#include <cstdlib>
 #include "tensor.hpp"
 
 #include "ad.hpp"
@@ -12,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <sstream>
+#include <unordered_map>
 #include <stdexcept>
 
 namespace xa {
@@ -67,6 +69,8 @@ std::string charges_key(const TensorStructure& t)
 Runtime::~Runtime()
 {
     if(ctx) {
+        for(auto& [k, v] : gather_cache)  // before the GEMM plans they refer to
+            if(v->dev) xh_gather_plan_destroy(ctx, v->dev);
         for(auto& [k, v] : permute_cache)
             if(v->dev) xh_permute_plan_destroy(ctx, v->dev);
         for(auto& [k, v] : gemm_cache)
@@ -99,6 +103,10 @@ long trim_lru(Map& m, std::size_t max_entries, Destroy destroy)
 
 void Runtime::trim_caches()
 {
+    n_evicted += trim_lru(gather_cache, max_plans, [this](CachedGather& c) {
+        if(ctx && c.dev) xh_gather_plan_destroy(ctx, c.dev);
+        c.dev = nullptr;
+    });
     n_evicted += trim_lru(permute_cache, max_plans, [this](CachedPermute& c) {
         if(ctx && c.dev) xh_permute_plan_destroy(ctx, c.dev);
         c.dev = nullptr;
@@ -207,6 +215,25 @@ void DTensor::set_random(std::mt19937& engine, int dist)
 template <class Lap>
 static void finish_permute_plan(Runtime* rt, CachedPermute& cpr, const TensorStructure& in, Lap& lap);
 
+// The dimension-free part of a permute plan, shared by all structures with the same charges
+// (tensors whose sector dimensions changed after a truncation); only offsets and extents are
+// recomputed per structure.
+static std::shared_ptr<const SymbolicPermute> symbolic_permute(Runtime* rt, const TensorStructure& in, const TensorStructure& out, int shift,
+                                                               const std::vector<int>& p)
+{
+    std::string sk = charges_key(in);
+    put_i(sk, shift);
+    for(int x : p) put_i(sk, x);
+    auto si = rt->sym_permute_cache.find(sk);
+    if(si != rt->sym_permute_cache.end()) {
+        rt->n_symbolic_hits++;
+        return si->second;
+    }
+    auto sp = std::make_shared<const SymbolicPermute>(make_symbolic_permute(in, out, shift, p));
+    rt->sym_permute_cache.emplace(sk, sp);
+    return sp;
+}
+
 std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p)
 {
     std::string k = structure_key(in);
@@ -225,22 +252,9 @@ std::shared_ptr<CachedPermute> get_permute_plan(Runtime* rt, const TensorStructu
     };
     auto cp = std::make_shared<CachedPermute>();
     cp->last_use = ++rt->tick;
-    // the dimension-free part is shared by all structures with the same charges (tensors whose
-    // sector dimensions changed after a truncation); only offsets and extents are recomputed
     StructPtr out = permuted_structure(in, shift, p);
     lap(5);
-    std::string sk = charges_key(in);
-    put_i(sk, shift);
-    for(int x : p) put_i(sk, x);
-    std::shared_ptr<const SymbolicPermute> sp;
-    auto si = rt->sym_permute_cache.find(sk);
-    if(si != rt->sym_permute_cache.end()) {
-        sp = si->second;
-        rt->n_symbolic_hits++;
-    } else {
-        sp = std::make_shared<const SymbolicPermute>(make_symbolic_permute(in, *out, shift, p));
-        rt->sym_permute_cache.emplace(sk, sp);
-    }
+    std::shared_ptr<const SymbolicPermute> sp = symbolic_permute(rt, in, *out, shift, p);
     lap(6);
     cp->plan = numeric_permute_plan(*sp, in, out, p);
     lap(0);
@@ -585,12 +599,136 @@ static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r,
     }
 }
 
+// The permute of contract()'s left operand fused into the product (CachedGather), built on the
+// first use of (left structure, shift, p, right structure).  It qualifies when every GEMM item is a
+// skinny FP64 item with the permuted operand on its long side and the permute is single-source
+// with coefficients +-1 (abelian symmetries); otherwise dev stays null.
+static std::shared_ptr<CachedGather> get_gather(Runtime* rt, const TensorStructure& in, int shift, const std::vector<int>& p, const TensorStructure& right)
+{
+    std::string k = structure_key(in);
+    put_i(k, shift);
+    for(int x : p) put_i(k, x);
+    k += "|";
+    k += structure_key(right);
+    auto it = rt->gather_cache.find(k);
+    if(it != rt->gather_cache.end()) {
+        it->second->last_use = ++rt->tick;
+        return it->second;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    auto g = std::make_shared<CachedGather>();
+    g->last_use = ++rt->tick;
+    StructPtr out = permuted_structure(in, shift, p);
+    {
+        // share the structure with the permute path (same leg sets give the same structure)
+        std::string sk = legs_key(out->sym, out->unc_dom, out->unc_cod);
+        auto si = rt->struct_cache.find(sk);
+        if(si == rt->struct_cache.end())
+            rt->struct_cache.emplace(sk, out);
+        else if(si->second->sector == out->sector)
+            out = si->second;
+    }
+    g->permuted = out;
+    g->gemm = get_gemm_plan(rt, *out, right);
+    int ok = 0;
+    if(rt->ctx && g->gemm->dev) dcheck(rt->ctx, xh_gemm_gather_eligible(g->gemm->dev, &ok), "gather eligibility");
+    if(ok) {
+        std::shared_ptr<const SymbolicPermute> sp = symbolic_permute(rt, in, *out, shift, p);
+        PermutePlan pl = numeric_permute_plan(*sp, in, out, p);
+        for(const PermuteTask& t : pl.tasks) {
+            if(t.nsrc != 1) {
+                ok = 0;
+                break;
+            }
+            const double c = pl.sources[t.src_begin].coeff;
+            if(c != 1.0 && c != -1.0) {
+                ok = 0;
+                break;
+            }
+        }
+        if(ok) {
+            // per GEMM item: the permuted sector that is its A block, its task grid and tree starts
+            std::unordered_map<std::int64_t, std::size_t> sector_at;
+            for(std::size_t s = 0; s < out->sector.size(); ++s)
+                if(out->rows[s] > 0 && out->cols[s] > 0) sector_at.emplace(static_cast<std::int64_t>(out->offsets[s]), s);
+            const auto& items = g->gemm->plan.items;
+            std::vector<xh_gather_item> gi(items.size(), xh_gather_item{0, 0, 0, 0, 0});
+            std::vector<std::int32_t> starts;
+            for(std::size_t j = 0; j < items.size(); ++j) {
+                if(items[j].m == 0 || items[j].n == 0) continue;
+                auto f = sector_at.find(items[j].a_off);
+                if(f == sector_at.end()) throw Error(XH_EINVAL, "gather plan: a GEMM item's A block is not a sector of the permuted tensor");
+                const std::size_t s = f->second;
+                const auto& dt = out->dom().tree(out->sector[s]);
+                const auto& ct = out->cod().tree(out->sector[s]);
+                gi[j] = xh_gather_item{sp->task_base[s], static_cast<std::int32_t>(dt.size()), static_cast<std::int32_t>(ct.size()), 0,
+                                       static_cast<std::int64_t>(starts.size())};
+                std::int64_t acc = 0;
+                starts.push_back(0);
+                for(const auto& t : dt) starts.push_back(static_cast<std::int32_t>(acc += static_cast<std::int64_t>(t.dim)));
+                acc = 0;
+                starts.push_back(0);
+                for(const auto& t : ct) starts.push_back(static_cast<std::int32_t>(acc += static_cast<std::int64_t>(t.dim)));
+            }
+            xh_gather_desc desc{};
+            desc.perm = xh_permute_desc{pl.nlegs, pl.new_rank, static_cast<int64_t>(pl.tasks.size()), static_cast<int64_t>(pl.sources.size()),
+                                        reinterpret_cast<const xh_permute_task*>(pl.tasks.data()), reinterpret_cast<const xh_permute_source*>(pl.sources.data())};
+            desc.nitems = static_cast<int64_t>(gi.size());
+            desc.items = gi.data();
+            desc.nstarts = static_cast<int64_t>(starts.size());
+            desc.starts = starts.data();
+            desc.src_elems = static_cast<int64_t>(in.size);
+            dcheck(rt->ctx, xh_gather_plan_create(rt->ctx, g->gemm->dev, &desc, &g->dev), "gather plan");
+        }
+        recycle_plan_tables(pl);
+    }
+    rt->gather_cache.emplace(k, g);
+    rt->trim_caches();
+    rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->n_permute_plans++;
+    return g;
+}
+
+// the product of a gather plan: out = permute(a) * pb with permute(a) read inside the GEMM
+static void launch_gather(const CachedGather* g, const DTensor& a, const DTensor& pb, DTensor& out)
+{
+    Runtime* rt = a.rt;
+    if(out.size() == 0) return;
+    const CachedGemm* cg = g->gemm.get();
+    // sectors without a matching pair stay zero, as in operator*
+    for(const auto& z : cg->zero_rects) dcheck(rt->ctx, xh_fill(rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
+    tag_contraction(rt, true);
+    const int rc = xh_gemm_gather(rt->ctx, cg->dev, g->dev, a.data(), static_cast<int64_t>(a.size()), pb.data(), out.data(), 1.0, 0.0);
+    tag_contraction(rt, false);
+    dcheck(rt->ctx, rc, "gather gemm");
+    if(rt->time_gemm)
+        for(const auto& it : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(it.m), static_cast<int32_t>(it.n), static_cast<int32_t>(it.k)});
+    rt->gemm_flops += cg->plan.flops;
+    rt->n_gemm++;
+    rt->n_fused++;
+}
+
 DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b, const std::vector<int>& a2, int rankres, int precision)
 {
     // Tensor::contract (Tensor.hpp:328-351)
     if(static_cast<int>(a1.size()) != a.st->rank + a.st->corank || static_cast<int>(a2.size()) != b.st->rank + b.st->corank)
         throw Error(XH_EINVAL, "contract: index list length does not match the tensor rank");
     ContractPerms cp = get_permutations(a1, a.st->rank, a2, b.st->rank, rankres);
+    Runtime* rt = a.rt;
+    bool ident = cp.shift1 == 0;
+    for(std::size_t i = 0; ident && i < cp.p1.size(); ++i) ident = cp.p1[i] == static_cast<int>(i);
+    if(rt->fuse_permute && rt->ctx && precision == XH_FP64 && !ident && a.size() > 0 && b.rt == rt && !recording(a) && !recording(b)) {
+        // the left permute read inside the product when the product qualifies (xh_gemm_gather)
+        DTensor pb = permute(b, cp.shift2, cp.p2);
+        auto g = get_gather(rt, *a.st, cp.shift1, cp.p1, *pb.st);
+        if(g->dev) {
+            DTensor prod = DTensor::empty(rt, g->gemm->plan.out);
+            launch_gather(g.get(), a, pb, prod);
+            return permute(prod, cp.shiftres, cp.pres);
+        }
+        DTensor pa = permute(a, cp.shift1, cp.p1);
+        return permute(mul(pa, pb, precision), cp.shiftres, cp.pres);
+    }
     DTensor pa = permute(a, cp.shift1, cp.p1);
     DTensor pb = permute(b, cp.shift2, cp.p2);
     DTensor prod = mul(pa, pb, precision);

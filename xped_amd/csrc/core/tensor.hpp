@@ -54,6 +54,16 @@ struct CachedGemm {
     std::uint64_t last_use = 0;
 };
 
+// contract() with the permute of its left operand fused into the product (xh_gemm_gather): the
+// permuted structure, the GEMM plan of (permuted left, permuted right) and the device gather plan;
+// dev is null when the product does not qualify (then contract() permutes and multiplies)
+struct CachedGather {
+    StructPtr permuted;
+    std::shared_ptr<CachedGemm> gemm;
+    xh_gather_plan* dev = nullptr;
+    std::uint64_t last_use = 0;
+};
+
 struct Tape;
 
 // Device context, plan caches and counters.  ctx == nullptr is a structure-only runtime: every
@@ -66,6 +76,11 @@ struct Runtime {
     std::unordered_map<std::string, std::shared_ptr<const SymbolicPermute>> sym_permute_cache;
     long n_symbolic_hits = 0;
     std::unordered_map<std::string, std::shared_ptr<CachedGemm>> gemm_cache;
+    std::unordered_map<std::string, std::shared_ptr<CachedGather>> gather_cache;
+    // contract(): read the left operand through its permute inside skinny FP64 products instead of
+    // materialising the permuted tensor (on by default; off gives the same results bit for bit)
+    bool fuse_permute = true;
+    long n_fused = 0;
     std::unordered_map<std::string, StructPtr> struct_cache;
     // counters for reporting
     double gemm_flops = 0.0, permute_bytes = 0.0;

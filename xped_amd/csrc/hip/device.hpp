@@ -173,6 +173,22 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t x, uint32_t magic, uint32_t sh
     return (__umulhi(x, magic) + x) >> shift;
 }
 
+// One output leg of a permute task (shared by the permute kernels and the GEMM that reads its A
+// operand through a permute): extent, magic division by the extent, and the stride of the leg's
+// source leg in the source subblock's row (bit 8 of flags clear) or column (set) index; bits 0-7
+// of flags: division shift.
+struct LegInfo {
+    uint32_t ext, magic, flags, stride;
+};
+__host__ __device__ __forceinline__ uint32_t leg_shift(const LegInfo& l) { return l.flags & 255u; }
+__host__ __device__ __forceinline__ int64_t leg_rstride(const LegInfo& l) { return (l.flags & 256u) ? 0 : int64_t(l.stride); }
+__host__ __device__ __forceinline__ int64_t leg_cstride(const LegInfo& l) { return (l.flags & 256u) ? int64_t(l.stride) : 0; }
+inline LegInfo make_leg(uint32_t ext, int64_t sstride, bool is_col)
+{
+    const FastDiv f = make_fastdiv(ext);
+    return LegInfo{ext, f.magic, f.shift | (is_col ? 256u : 0u), static_cast<uint32_t>(sstride)};
+}
+
 template <typename T>
 T* dev_upload(xh_ctx* ctx, const std::vector<T>& v)
 {

@@ -31,6 +31,7 @@
 #include <unordered_map>
 
 #include "device.hpp"
+#include "../common/pool.hpp"
 
 using xh::check;
 using xh::guard;
@@ -301,6 +302,148 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny(GemmArgs g, const Ti
 // other, halved the workgroups without adding loads in flight: on single items of 262144 x 20 x 20
 // and 65536 x 20 x 20 one index per thread took 28.8 and 13.3 us against 30.5 and 21.0 us.)
 constexpr int sk_rows(int) { return 1; }
+
+// ---------------------------------------------------------------- FP64, skinny items, A through a permute
+// The skinny kernel with its long operand A read through a single-source permute of src, instead
+// of from the materialised permuted tensor (xh_gemm_gather).  Row r of an item's A block lies in
+// domain tree a and column kk in codomain tree b; the element is entry (r - start_a, kk - start_b)
+// of permute task task_base + a * ncod + b, and its source offset is
+//   src_off(t) + sum over legs of (leg index) * (leg stride in the source, times src_ld for a
+//   source column leg),
+// the domain legs' indices peeled off the row, the codomain legs' off the column.  The plan cuts
+// every item's rows into blocks of at most 256 rows that meet at most GA_SEG domain trees.  Per
+// (block tree, column) the block tabulates in LDS everything but the domain legs' terms (the
+// column's offset) and the domain legs' strides; a thread then adds (index x stride) over its
+// row's domain legs per column.  Coefficients are +1 or -1, so the operand is the source element,
+// negated for -1: the value the permute writes, up to the sign of a zero, which sums that start
+// from +0 do not see.  The sums run in the skinny kernel's order, so the product equals
+// permute + xh_gemm bit for bit.
+constexpr int GA_MAXR = 8;  // domain legs of a gather plan's permuted tensor (checked at plan creation)
+constexpr int GA_SEG = 4;   // domain trees one block may meet
+struct GatherBlock {
+    int32_t item, row0, nrows, nseg;
+    int32_t a[GA_SEG];      // domain tree of each segment
+    int32_t start[GA_SEG];  // first row of that tree (segments ascend)
+};
+struct GatherItem {
+    int32_t task_base, ncod;
+    int64_t cols_off;  // per column kk < K: codomain tree b | (column inside it << 16)
+};
+struct GatherSrc {
+    int64_t off, ld;   // source subblock offset and leading dimension
+    int32_t neg, pad;  // 1: coefficient -1
+};
+struct GatherDev {
+    const GatherBlock* blocks;
+    const GatherItem* items;  // per GEMM item
+    const int32_t* cols;
+    const xh::LegInfo* legs;  // ntasks * nlegs
+    const GatherSrc* src;     // per task
+    int32_t nlegs, new_rank, nblocks;
+};
+
+template <int MS>
+__global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, GatherDev gd, const double* __restrict__ src)
+{
+    __shared__ double sS[SK_KMAX * MS];                     // B, [kk][s]
+    __shared__ int64_t s_off[GA_SEG][SK_KMAX];              // per tree and column: the column's offset
+    __shared__ int64_t s_rs[GA_SEG][SK_KMAX][GA_MAXR];      // per tree and column: the domain legs' strides
+    __shared__ xh::LegInfo s_rleg[GA_SEG][GA_MAXR];         // per tree: its domain legs (extents, divisions)
+    __shared__ uint32_t s_neg[GA_SEG];                      // bit kk: the column's task has coefficient -1
+    const int R = gd.new_rank, NL = gd.nlegs;
+    for(int blk = blockIdx.x; blk < gd.nblocks; blk += gridDim.x) {
+        const GatherBlock bl = gd.blocks[blk];
+        const xh_gemm_item it = g.items[bl.item];
+        const GatherItem gi = gd.items[bl.item];
+        const int S = it.n, K = it.k;
+        __syncthreads();  // the previous block's reads of the shared tables are done
+        for(int e = threadIdx.x; e < K * MS; e += SK_THREADS) {
+            const int sx = e % MS, kk = e / MS;
+            double v = 0.0;
+            if(sx < S) v = g.tb ? g.B[it.b_off + int64_t(kk) * it.n + sx] : g.B[it.b_off + int64_t(sx) * K + kk];
+            sS[kk * MS + sx] = v;
+        }
+        if(int(threadIdx.x) < GA_SEG) s_neg[threadIdx.x] = 0;
+        __syncthreads();
+        for(int e = threadIdx.x; e < bl.nseg * K; e += SK_THREADS) {
+            const int sg = e / K, kk = e % K;
+            const int32_t cb = gd.cols[gi.cols_off + kk];
+            const int t = gi.task_base + gd.blocks[blk].a[sg] * gi.ncod + (cb & 0xffff);
+            const GatherSrc so = gd.src[t];
+            const xh::LegInfo* Lg = gd.legs + int64_t(t) * NL;
+            // the codomain legs' terms of column kk (its index inside tree b peeled leg by leg)
+            uint32_t x = static_cast<uint32_t>(cb >> 16);
+            int64_t off = so.off;
+            for(int k = R; k < NL; ++k) {
+                const xh::LegInfo li = Lg[k];
+                const uint32_t q = xh::fdiv(x, li.magic, xh::leg_shift(li));
+                const uint32_t j = x - q * li.ext;
+                x = q;
+                off += (li.flags & 256u) ? int64_t(j) * li.stride * so.ld : int64_t(j) * li.stride;
+            }
+            s_off[sg][kk] = off;
+            for(int k = 0; k < R; ++k) {
+                const xh::LegInfo li = Lg[k];
+                s_rs[sg][kk][k] = (li.flags & 256u) ? int64_t(li.stride) * so.ld : int64_t(li.stride);
+                if(kk == 0) s_rleg[sg][k] = li;
+            }
+            if(so.neg) atomicOr(&s_neg[sg], 1u << kk);
+        }
+        __syncthreads();
+        if(int(threadIdx.x) >= bl.nrows) continue;  // no barrier follows before the next block's first one
+        const int L = bl.row0 + threadIdx.x;
+        int sg = 0, first = bl.start[0];
+#pragma unroll
+        for(int q = 1; q < GA_SEG; ++q)
+            if(q < bl.nseg && bl.start[q] <= L) {
+                sg = q;
+                first = bl.start[q];
+            }
+        // the row's domain-leg indices inside its tree
+        uint32_t x = static_cast<uint32_t>(L - first);
+        uint32_t dig[GA_MAXR];
+#pragma unroll
+        for(int k = 0; k < GA_MAXR; ++k) {
+            dig[k] = 0;
+            if(k < R) {
+                const xh::LegInfo li = s_rleg[sg][k];
+                const uint32_t q = xh::fdiv(x, li.magic, xh::leg_shift(li));
+                dig[k] = x - q * li.ext;
+                x = q;
+            }
+        }
+        double xv[SK_KMAX];
+#pragma unroll
+        for(int kk = 0; kk < SK_KMAX; ++kk) {
+            xv[kk] = 0.0;
+            if(kk < K) {
+                int64_t off = s_off[sg][kk];
+#pragma unroll
+                for(int k = 0; k < GA_MAXR; ++k)
+                    if(k < R) off += int64_t(dig[k]) * s_rs[sg][kk][k];
+                xv[kk] = src[off];
+            }
+        }
+        const uint32_t negm = s_neg[sg];
+#pragma unroll
+        for(int kk = 0; kk < SK_KMAX; ++kk)
+            if((negm >> kk) & 1u) xv[kk] = -xv[kk];
+        double acc[MS];
+#pragma unroll
+        for(int sx = 0; sx < MS; ++sx) acc[sx] = 0.0;
+#pragma unroll
+        for(int kk = 0; kk < SK_KMAX; ++kk)
+            if(kk < K)
+#pragma unroll
+                for(int sx = 0; sx < MS; ++sx) acc[sx] = __builtin_fma(sS[kk * MS + sx], xv[kk], acc[sx]);
+#pragma unroll
+        for(int sx = 0; sx < MS; ++sx)
+            if(sx < S) {
+                double* p = g.C + it.c_off + int64_t(sx) * it.m + L;
+                *p = (g.beta == 0.0) ? g.alpha * acc[sx] : g.alpha * acc[sx] + g.beta * *p;
+            }
+    }
+}
 
 // ---------------------------------------------------------------- FP64, 128 x 128 tiles
 // For items with both output sides >= 192: 8 waves in 2 x 4, each owning 64 x 32 (4 x 2 MFMA
@@ -1305,5 +1448,199 @@ extern "C" int xh_set_gemm_tiles(xh_ctx* ctx, int mode)
     return guard(ctx, [&] {
         if(mode < 0 || mode > 1) throw xh::Error(XH_EINVAL, "gemm tile mode must be 0 or 1");
         ctx->gemm_tiles = mode;
+    });
+}
+
+// ---------------------------------------------------------------- gather GEMM (permute fused into the product)
+
+struct xh_gather_plan : xh::PlanLife {
+    xh::PackedUpload up;
+    GatherDev dev{};
+    const xh_gemm_plan* gemm = nullptr;
+    int64_t src_elems = 0;
+};
+
+namespace {
+bool gather_eligible(const xh_gemm_plan* plan)
+{
+    if(plan->ta != 0 || plan->skinny.ntiles == 0) return false;
+    for(const auto& it : plan->h_items)
+        if(it.m > 0 && it.n > 0 && !(is_skinny(it) && it.m > it.n)) return false;
+    return true;
+}
+
+void gather_plan_release(xh_ctx* ctx, xh::PlanLife* p)
+{
+    auto* plan = static_cast<xh_gather_plan*>(p);
+    plan->up.release(ctx);
+    delete plan;
+}
+}  // namespace
+
+extern "C" int xh_gemm_gather_eligible(const xh_gemm_plan* plan, int* ok)
+{
+    if(!plan || !ok) return XH_EINVAL;
+    *ok = gather_eligible(plan) ? 1 : 0;
+    return XH_OK;
+}
+
+extern "C" int xh_gather_plan_create(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather_desc* d, xh_gather_plan** out)
+{
+    *out = nullptr;
+    auto* plan = new xh_gather_plan();
+    const int rc = guard(ctx, [&] {
+        xh::refuse_under_capture(ctx, "plan creation");
+        if(!gemm || !d) throw xh::Error(XH_EINVAL, "gather plan: null argument");
+        if(!gather_eligible(gemm)) throw xh::Error(XH_EINVAL, "gather plan: the GEMM plan has an item that is not a skinny item with A the long operand");
+        const xh_permute_desc& pd = d->perm;
+        const int NL = pd.nlegs, NR = pd.new_rank;
+        if(NL < 1 || NL > XH_MAXLEGS || NR < 0 || NR > NL || NR > GA_MAXR) throw xh::Error(XH_EINVAL, "gather plan: invalid leg counts");
+        const int64_t NT = pd.ntasks;
+        if(NT < 0 || NT > 0x7fffffff || pd.nsources < 0) throw xh::Error(XH_EINVAL, "gather plan: invalid task or source count");
+        if(d->nitems != static_cast<int64_t>(gemm->h_items.size())) throw xh::Error(XH_EINVAL, "gather plan: one gather item per GEMM item");
+        if(d->nstarts < 0 || d->src_elems < 0) throw xh::Error(XH_EINVAL, "gather plan: invalid sizes");
+        // every task: one source, coefficient +-1, extents matching its subblock, every source
+        // element inside src
+        xp::parallel_ranges(size_t(NT), 4096, [&](size_t t0, size_t t1) {
+            for(size_t t = t0; t < t1; ++t) {
+                const xh_permute_task& tk = pd.tasks[t];
+                if(tk.nsrc != 1 || tk.src_begin < 0 || tk.src_begin >= pd.nsources) throw xh::Error(XH_EINVAL, "gather plan: every task needs exactly one source");
+                const xh_permute_source& so = pd.sources[tk.src_begin];
+                if(so.coeff != 1.0 && so.coeff != -1.0) throw xh::Error(XH_EINVAL, "gather plan: source coefficients must be +1 or -1");
+                int64_t rprod = 1, cprod = 1, rmax = 0, cmax = 0;
+                for(int k = 0; k < NL; ++k) {
+                    if(tk.ext[k] == 0 || tk.sstride[k] < 0 || tk.sstride[k] >= (int64_t(1) << 31)) throw xh::Error(XH_EINVAL, "gather plan: invalid leg");
+                    (k < NR ? rprod : cprod) *= tk.ext[k];
+                    (tk.is_col[k] ? cmax : rmax) += (int64_t(tk.ext[k]) - 1) * tk.sstride[k];
+                }
+                if(rprod != tk.rows || cprod != tk.cols || tk.rows >= (int64_t(1) << 31) || tk.cols >= (int64_t(1) << 31))
+                    throw xh::Error(XH_EINVAL, "gather plan: leg extents do not match the subblock shape");
+                if(so.src_off < 0 || rmax >= so.src_ld || rmax >= (int64_t(1) << 31) || cmax >= (int64_t(1) << 31) || so.src_off + rmax + cmax * so.src_ld >= d->src_elems)
+                    throw xh::Error(XH_EINVAL, "gather plan: source subblock out of bounds");
+            }
+        });
+        // every item: its trees tile the A block, and its tasks have the trees' shapes
+        xp::parallel_ranges(size_t(d->nitems), 1, [&](size_t j0, size_t j1) {
+            for(size_t j = j0; j < j1; ++j) {
+                const xh_gemm_item& it = gemm->h_items[j];
+                const xh_gather_item& gi = d->items[j];
+                if(it.m == 0 || it.n == 0) continue;
+                if(gi.ndom < 1 || gi.ncod < 1 || gi.task_base < 0 || int64_t(gi.task_base) + int64_t(gi.ndom) * gi.ncod > NT || gi.starts_off < 0 ||
+                   gi.starts_off + gi.ndom + gi.ncod + 2 > d->nstarts)
+                    throw xh::Error(XH_EINVAL, "gather plan: item tables out of range");
+                const int32_t* ds = d->starts + gi.starts_off;
+                const int32_t* cs = ds + gi.ndom + 1;
+                if(ds[0] != 0 || ds[gi.ndom] != it.m || cs[0] != 0 || cs[gi.ncod] != it.k) throw xh::Error(XH_EINVAL, "gather plan: tree starts do not cover the A block");
+                if(gi.ncod > 0xffff) throw xh::Error(XH_EINVAL, "gather plan: too many codomain trees");
+                // the kernel decodes a row with the domain-leg extents of one task of its tree
+                for(int a = 0; a < gi.ndom; ++a) {
+                    const xh_permute_task& t0 = pd.tasks[gi.task_base + int64_t(a) * gi.ncod];
+                    for(int b = 0; b < gi.ncod; ++b) {
+                        const xh_permute_task& tk = pd.tasks[gi.task_base + int64_t(a) * gi.ncod + b];
+                        if(ds[a + 1] - ds[a] != tk.rows || cs[b + 1] - cs[b] != tk.cols) throw xh::Error(XH_EINVAL, "gather plan: task shape does not match its trees");
+                        for(int k = 0; k < NR; ++k)
+                            if(tk.ext[k] != t0.ext[k]) throw xh::Error(XH_EINVAL, "gather plan: domain-leg extents differ within a domain tree");
+                    }
+                }
+            }
+        });
+        // blocks of at most SK_THREADS rows meeting at most GA_SEG domain trees, in the GEMM plan's
+        // item order; per item the codomain tree and in-tree column of every column
+        std::vector<GatherBlock> blocks;
+        std::vector<GatherItem> items(size_t(d->nitems), GatherItem{0, 0, 0});
+        std::vector<int32_t> cols;
+        for(int64_t j = 0; j < d->nitems; ++j) {
+            const xh_gemm_item& it = gemm->h_items[j];
+            const xh_gather_item& g = d->items[j];
+            if(it.m == 0 || it.n == 0) continue;
+            const int32_t* ds = d->starts + g.starts_off;
+            const int32_t* cs = ds + g.ndom + 1;
+            items[j] = GatherItem{g.task_base, g.ncod, int64_t(cols.size())};
+            for(int b = 0; b < g.ncod; ++b)
+                for(int c = cs[b]; c < cs[b + 1]; ++c) cols.push_back(b | ((c - cs[b]) << 16));
+            int a = 0;
+            for(int r = 0; r < it.m;) {
+                GatherBlock bl{int32_t(j), r, 0, 0, {}, {}};
+                const int end = std::min(it.m, r + SK_THREADS);
+                while(r < end && bl.nseg < GA_SEG) {
+                    while(ds[a + 1] <= r) ++a;
+                    bl.a[bl.nseg] = a;
+                    bl.start[bl.nseg] = ds[a];
+                    ++bl.nseg;
+                    r = std::min(end, ds[a + 1]);
+                }
+                bl.nrows = r - bl.row0;
+                blocks.push_back(bl);
+            }
+        }
+        if(blocks.size() > 0x7fffffff) throw xh::Error(XH_EINVAL, "gather plan: too many blocks");
+        xh::PackedUpload& up = plan->up;
+        const size_t bytes = xh::PackedUpload::packed_bytes({blocks.size() * sizeof(GatherBlock), items.size() * sizeof(GatherItem), cols.size() * sizeof(int32_t),
+                                                             size_t(NT * NL) * sizeof(xh::LegInfo), size_t(NT) * sizeof(GatherSrc)});
+        if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);
+        auto put = [&up](const auto& vec) {
+            using T = typename std::decay_t<decltype(vec)>::value_type;
+            const size_t o = up.reserve<T>(static_cast<int64_t>(vec.size()));
+            if(!vec.empty()) std::memcpy(up.host_ptr<T>(o), vec.data(), vec.size() * sizeof(T));
+            return o;
+        };
+        const size_t o_bl = put(blocks), o_it = put(items), o_co = put(cols);
+        const size_t o_lg = up.reserve<xh::LegInfo>(NT * NL), o_src = up.reserve<GatherSrc>(NT);
+        xh::LegInfo* lg = up.host_ptr<xh::LegInfo>(o_lg);
+        GatherSrc* gs = up.host_ptr<GatherSrc>(o_src);
+        xp::parallel_ranges(size_t(NT), 4096, [&](size_t t0, size_t t1) {
+            for(size_t t = t0; t < t1; ++t) {
+                const xh_permute_task& tk = pd.tasks[t];
+                for(int k = 0; k < NL; ++k) lg[t * NL + k] = xh::make_leg(tk.ext[k], tk.sstride[k], tk.is_col[k] != 0);
+                const xh_permute_source& so = pd.sources[tk.src_begin];
+                gs[t] = GatherSrc{so.src_off, so.src_ld, so.coeff < 0.0 ? 1 : 0, 0};
+            }
+        });
+        up.upload(ctx);
+        plan->dev = GatherDev{up.ptr<GatherBlock>(o_bl), up.ptr<GatherItem>(o_it), up.ptr<int32_t>(o_co), up.ptr<xh::LegInfo>(o_lg), up.ptr<GatherSrc>(o_src),
+                              NL, NR, static_cast<int32_t>(blocks.size())};
+        plan->gemm = gemm;
+        plan->src_elems = d->src_elems;
+    });
+    if(rc != XH_OK) {
+        gather_plan_release(ctx, plan);
+        return rc;
+    }
+    *out = plan;
+    return XH_OK;
+}
+
+extern "C" int xh_gather_plan_destroy(xh_ctx* ctx, xh_gather_plan* plan)
+{
+    if(!plan) return XH_OK;
+    return guard(ctx, [&] {
+        if(plan->pins > 0) {
+            plan->dead = true;  // a live graph reads its tables: freed by the last xh_graph_destroy
+            return;
+        }
+        gather_plan_release(ctx, plan);
+    });
+}
+
+extern "C" int xh_gemm_gather(xh_ctx* ctx, const xh_gemm_plan* plan, const xh_gather_plan* gp, const double* src, int64_t src_elems, const double* B, double* C,
+                              double alpha, double beta)
+{
+    return guard(ctx, [&] {
+        if(!plan || !gp) throw xh::Error(XH_EINVAL, "xh_gemm_gather: null plan");
+        if(gp->gemm != plan) throw xh::Error(XH_EINVAL, "xh_gemm_gather: the gather plan was built for another GEMM plan");
+        if(plan->dead || gp->dead) throw xh::Error(XH_EINVAL, "xh_gemm_gather: plan used after destroy");
+        if(src_elems != gp->src_elems) throw xh::Error(XH_EINVAL, "xh_gemm_gather: the source size differs from the plan's");
+        auto* mp = const_cast<xh_gemm_plan*>(plan);
+        auto* mg = const_cast<xh_gather_plan*>(gp);
+        xh::pin_for_capture(ctx, mp, gemm_plan_release);
+        xh::pin_for_capture(ctx, mg, gather_plan_release);
+        const bool timed = ctx->gemm_timing;
+        if(timed) xh::gemm_timer_begin(ctx);
+        GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, nullptr, B, C, nullptr, alpha, beta};
+        auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny_gather<8> : plan->sk_ms <= 16 ? k_gemm_skinny_gather<16> : k_gemm_skinny_gather<32>;
+        const int grid = std::min<int>(gp->dev.nblocks, ctx->num_cu * 8);
+        if(grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, gp->dev, src);
+        xh::launch_check("k_gemm_skinny_gather");
+        if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(ctx, plan));
     });
 }

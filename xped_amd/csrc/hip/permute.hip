@@ -41,14 +41,10 @@ __device__ __forceinline__ void split_el(const TaskMeta& tm, uint32_t el, uint32
     row = el - col * tm.rows;
 }
 
-// one output leg of one task: extent, magic division, and the stride of its source leg in the
-// source subblock's row (bit 8 of flags clear) or column (set) index; bits 0-7: division shift
-struct LegInfo {
-    uint32_t ext, magic, flags, stride;
-};
-__host__ __device__ __forceinline__ uint32_t leg_shift(const LegInfo& l) { return l.flags & 255u; }
-__host__ __device__ __forceinline__ int64_t leg_rstride(const LegInfo& l) { return (l.flags & 256u) ? 0 : int64_t(l.stride); }
-__host__ __device__ __forceinline__ int64_t leg_cstride(const LegInfo& l) { return (l.flags & 256u) ? int64_t(l.stride) : 0; }
+using xh::leg_cstride;
+using xh::leg_rstride;
+using xh::leg_shift;
+using xh::LegInfo;
 
 struct PermDev {
     int nlegs, new_rank;
@@ -485,10 +481,7 @@ extern "C" int xh_permute_plan_create(xh_ctx* ctx, const xh_permute_desc* desc, 
                 const xh::FastDiv fd = xh::make_fastdiv(uint32_t(std::max<int64_t>(tk.rows, 1)));
                 meta[idx] = TaskMeta{tk.dst_off, tk.dst_ld, uint32_t(tk.rows), fd.magic, fd.shift, 0, tk.nsrc, tk.src_begin};
                 LegInfo* L = legs + idx * NL;
-                for(int k = 0; k < NL; ++k) {
-                    const xh::FastDiv f = xh::make_fastdiv(tk.ext[k]);
-                    L[k] = LegInfo{tk.ext[k], f.magic, f.shift | (tk.is_col[k] ? 256u : 0u), static_cast<uint32_t>(tk.sstride[k])};
-                }
+                for(int k = 0; k < NL; ++k) L[k] = xh::make_leg(tk.ext[k], tk.sstride[k], tk.is_col[k] != 0);
                 if(u > 0) {
                     int64_t rest = 1;
                     for(int k = 0; k < NL; ++k)

@@ -149,6 +149,15 @@ class Runtime:
         """Bound of the permute and GEMM plan caches (least recently used half dropped past it)."""
         check(lib().xt_rt_set_max_plans(self.h, int(n)), self.h, "set_max_plans")
 
+    def fuse_permute(self, enable=None):
+        """contract() with the left operand's permute read inside skinny FP64 products (the gather
+        GEMM, on by default; bit-identical either way).  enable: True / False / None (keep).
+        Returns the number of contractions run fused so far."""
+        n = C.c_long()
+        e = -1 if enable is None else (1 if enable else 0)
+        check(lib().xt_rt_set_fuse_permute(self.h, e, C.byref(n)), self.h, "fuse_permute")
+        return n.value
+
     def cache_sizes(self):
         """(cached permute plans, cached GEMM plans, plans dropped so far)."""
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
