@@ -9,11 +9,12 @@ Heisenberg energy.  Two modes run on fresh runtimes (no shared plan caches):
          products with per-sector target ranks from the learned predictor (or, with
          --ranks previous, the previous step's kept counts); full FP64 SVD as fallback
 Both start from the same warm-up (init from the double layer + one full-SVD grow_all reaching
-chi = 256).  `value` is the GEMM throughput of the fast step: the summed algorithmic flops of
-all its grouped GEMM launches (contractions, sketch and Gram products) over their HIP-event-timed
-kernel time (k_gemm_f64 plus the split-K reduction); `ms_per_step` is the
-fast step's wall time; the fp64 step time, the speedup and the energy difference are reported
-beside them.
+chi = 256).  `value` is the contraction GEMM throughput of the fast step: the summed algorithmic
+flops of its contraction products (operator* of every contract) over their HIP-event-timed kernel
+time; `roofline` reports the dominant kernel, the contraction launches with matrix-core tiles,
+and `skinny_contractions` the memory-bound ones (bytes against HBM), the sketch and Gram products
+of the randomized SVDs are reported beside them; `ms_per_step` is the fast step's wall time; the
+fp64 step time, the speedup and the energy difference are reported beside them.
 
 --workload c3 times the grouped U(1)xU(1) fermionic contraction A(l1,l2|l3,l4).B(l3,l4|l5,l6)
 (leg dim 192, 313 matched sector GEMMs) instead.
@@ -89,28 +90,38 @@ def log(msg):
 
 
 # ------------------------------------------------------------------------------------------ C4
+SKINNY_MAX = 32  # an item with min(m, n) <= 32 and k <= 32 runs in the memory-bound skinny kernel
+
+
 def launch_split(rt, xt):
-    """Kernel time and flops of the timed step's GEMM launches, split by the tag the tensor layer
-    gives its contraction products (1) against the randomized SVDs' sketch, Gram and factor
-    products (0)."""
+    """Kernel time, flops, algorithmic bytes and count of the timed step's GEMM launches, split by
+    the tag the tensor layer gives its contraction products against the randomized SVDs' sketch,
+    Gram and factor products (key 0); contraction launches with at least one matrix-core item
+    (key 1) apart from those whose every item is a skinny item (key 2, memory-bound: the fused
+    gather products and the other site-tensor products)."""
     import ctypes as C
 
     L = xt.lib()
     ctx = rt.ctx()
     n = rt.gemm_launches()
-    out = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0]}
+    out = {0: [0.0, 0.0, 0, 0.0], 1: [0.0, 0.0, 0, 0.0], 2: [0.0, 0.0, 0, 0.0]}
     for i in range(n):
         t, tr, ni = C.c_double(), (C.c_int32 * 3)(), C.c_int64()
         L.xh_gemm_launch_log(ctx, i, C.byref(t), tr, None, 0, C.byref(ni))
         buf = (C.c_int32 * (3 * max(ni.value, 1)))()
         L.xh_gemm_launch_log(ctx, i, C.byref(t), tr, buf, ni.value, C.byref(ni))
-        fl = 0.0
+        fl = by = 0.0
+        skinny = True
         for k in range(ni.value):
-            fl += 2.0 * buf[3 * k] * buf[3 * k + 1] * buf[3 * k + 2]
-        o = out[1 if tr[2] == 1 else 0]
+            m, nn, kk = buf[3 * k], buf[3 * k + 1], buf[3 * k + 2]
+            fl += 2.0 * m * nn * kk
+            by += 8.0 * (m * kk + kk * nn + m * nn)
+            skinny = skinny and (min(m, nn) <= SKINNY_MAX and kk <= SKINNY_MAX)
+        o = out[0 if tr[2] != 1 else (2 if skinny else 1)]
         o[0] += t.value
         o[1] += fl
         o[2] += 1
+        o[3] += by
     return out
 
 
@@ -270,9 +281,12 @@ def run_c4(args):
     ref = None if args.no_fp64 else run_mode("fp64", args, xt, configs, random_ipeps)
     # value: contraction products only (operator* of the step's contractions); all grouped GEMM
     # launches of the step, the randomized SVDs' included, are reported beside it
-    con_ms, con_fl, con_n = fast["gemm_split"][1]
-    sk_ms, sk_fl, sk_n = fast["gemm_split"][0]
+    mf_ms, mf_fl, mf_n, _ = fast["gemm_split"][1]
+    sn_ms, sn_fl, sn_n, sn_by = fast["gemm_split"][2]
+    sk_ms, sk_fl, sk_n, _ = fast["gemm_split"][0]
+    con_ms, con_fl, con_n = mf_ms + sn_ms, mf_fl + sn_fl, mf_n + sn_n
     value = con_fl / (con_ms * 1e-3) / 1e12 if con_ms > 0 else 0.0
+    mf_tf = mf_fl / (mf_ms * 1e-3) / 1e12 if mf_ms > 0 else 0.0
     all_tf = fast["gemm_timed_flops"] / (fast["gemm_kernel_ms"] * 1e-3) / 1e12 if fast["gemm_kernel_ms"] > 0 else 0.0
     peak = PEAKS[args.precision]
     res = {
@@ -300,13 +314,21 @@ def run_c4(args):
                  **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],
                  "host_breakdown_ms_per_step": fast["host_breakdown_ms_per_step"], "symbolic_plan_hits_per_step": fast["symbolic_plan_hits_per_step"]},
-        "roofline": {"bound": "mfma", "achieved": round(value * (OZ_PAIRS if args.precision == "ozaki" else 1), 4), "peak": peak,
+        # the dominant kernel: the matrix-core GEMM of the contraction launches with tiled items
+        "roofline": {"bound": "mfma", "achieved": round(mf_tf * (OZ_PAIRS if args.precision == "ozaki" else 1), 4), "peak": peak,
                      "unit": "TOP/s (int8)" if args.precision == "ozaki" else "TFLOP/s",
-                     "frac": round(value * (OZ_PAIRS if args.precision == "ozaki" else 1) / peak, 5),
+                     "frac": round(mf_tf * (OZ_PAIRS if args.precision == "ozaki" else 1) / peak, 5),
                      "traffic": measured_traffic("k_gemm_f64_ctm_step_counters.md") if args.precision == "fp64" and args.chi == 256 else None,
-                     "kernel": ("grouped GEMM of the contractions: k_gemm_f64 + k_gemm_f64_128 + k_gemm_skinny (+ k_splitk_reduce)" if args.precision == "fp64"
-                                else "k_oz_split + k_gemm_oz" if args.precision == "ozaki" else f"k_gemm_{args.precision}"),
-                     "launches_per_step": con_n, "flops_per_step": con_fl, "kernel_ms_per_step": round(con_ms, 3)},
+                     "kernel": ("contraction GEMM launches with matrix-core items: k_gemm_f64 + k_gemm_f64_128 (+ k_splitk_reduce, and the skinny items of the same launches)"
+                                if args.precision == "fp64" else "k_oz_split + k_gemm_oz" if args.precision == "ozaki" else f"k_gemm_{args.precision}"),
+                     "launches_per_step": mf_n, "flops_per_step": mf_fl, "bytes_per_step": fast["gemm_split"][1][3], "kernel_ms_per_step": round(mf_ms, 3)},
+        # contraction launches whose every item is skinny (memory-bound), the gather GEMMs among them
+        "skinny_contractions": {"bound": "hbm", "kernel": "k_gemm_skinny + k_gemm_skinny_gather",
+                                "achieved": round(sn_by / (sn_ms * 1e-3) / 1e9, 1) if sn_ms > 0 else 0.0, "peak": 8000.0, "unit": "GB/s",
+                                "frac": round(sn_by / (sn_ms * 1e-3) / 1e9 / 8000.0, 4) if sn_ms > 0 else 0.0,
+                                "bytes_per_step": sn_by, "flops_per_step": sn_fl, "launches_per_step": sn_n, "kernel_ms_per_step": round(sn_ms, 3)},
+        "contractions_all": {"achieved": round(value, 4), "unit": "TFLOP/s", "launches_per_step": con_n, "flops_per_step": con_fl,
+                             "kernel_ms_per_step": round(con_ms, 3)},
         "all_gemm_launches": {"achieved": round(all_tf, 4), "unit": "TFLOP/s", "frac": round(all_tf / peak, 5), "launches_per_step": fast["gemm_launches"],
                               "flops_per_step": fast["gemm_timed_flops"], "kernel_ms_per_step": round(fast["gemm_kernel_ms"], 3),
                               "randomized_svd_products": {"achieved": round(sk_fl / (sk_ms * 1e-3) / 1e12, 4) if sk_ms > 0 else 0.0, "launches": sk_n,
