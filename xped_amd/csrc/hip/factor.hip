@@ -52,10 +52,23 @@ struct FactorArgs {
     int* status;
 };
 
+// The working set of a workgroup: dynamic LDS, or its slice of the global scratch.  Resolved at
+// compile time, so that LDS accesses compile to LDS instructions (a pointer that may point to
+// either space compiles to flat loads and stores, which wait on both memory counters).
+template <bool LDS>
+__device__ __forceinline__ double* work_base(double* smem, const FactorArgs& fa)
+{
+    if constexpr(LDS)
+        return smem;
+    else
+        return fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+}
+
 // Working set of the Jacobi SVD for a block of mm x nn (mm >= nn):
 //   W (mm*nn), V (nn*nn), sigma (nn), order (nn ints)
 __host__ __device__ inline int64_t svd_work_doubles(int64_t mm, int64_t nn) { return mm * nn + nn * nn + nn + (nn + 1) / 2 + 2; }
 
+template <bool LDS>
 __global__ __launch_bounds__(F_THREADS) void k_svd_jacobi(FactorArgs fa)
 {
     extern __shared__ double smem[];
@@ -65,7 +78,7 @@ __global__ __launch_bounds__(F_THREADS) void k_svd_jacobi(FactorArgs fa)
     if(m == 0 || n == 0) return;
     const bool tr = m < n;
     const int mm = tr ? n : m, nn = tr ? m : n;
-    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* W = work_base<LDS>(smem, fa);
     double* V = W + int64_t(mm) * nn;
     double* sig = V + int64_t(nn) * nn;
     int* ord = reinterpret_cast<int*>(sig + nn);
@@ -200,6 +213,7 @@ __global__ __launch_bounds__(F_THREADS) void k_svd_jacobi(FactorArgs fa)
 // Q has `q_cols` columns (the tensor's middle dimension; columns beyond m are zero) and R has
 // `q_cols` rows (rows beyond min(m, n) are zero).  Item fields: u_off -> Q, vt_off -> R,
 // s_off -> q_cols (reused as an integer).
+template <bool LDS>
 __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
 {
     extern __shared__ double smem[];
@@ -210,7 +224,7 @@ __global__ __launch_bounds__(F_THREADS) void k_qr(FactorArgs fa)
     const int qc = static_cast<int>(it.s_off);
     if(m == 0 || n == 0) return;
     const int kmax = min(m, n);
-    double* Wk = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;  // m x n working copy
+    double* Wk = work_base<LDS>(smem, fa);  // m x n working copy
     double* tau = Wk + int64_t(m) * n;                                                      // kmax
     const int tid = threadIdx.x;
     for(int64_t e = tid; e < int64_t(m) * n; e += F_THREADS) Wk[e] = fa.A[it.a_off + e];
@@ -313,7 +327,7 @@ constexpr int CI_LANES = 16;  // lanes per dot product in (c)
 constexpr int CI_PANEL = 8;   // pivots per trailing update of the panel variant
 constexpr int CI_MAX_N = 40000;  // packed indices of a block fit in 32 bits
 
-__host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * (n + 1) / 2 + 2 * n; }
+__host__ __device__ inline int64_t chol_work_doubles(int64_t n) { return n * (n + 1) / 2 + 4 * n; }
 __device__ __forceinline__ int64_t pk(int i, int j) { return int64_t(j) * (j + 1) / 2 + i; }
 __device__ __forceinline__ int pk32(int i, int j) { return ((j * (j + 1)) >> 1) + i; }  // n <= CI_MAX_N
 
@@ -325,10 +339,11 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
     const xh_factor_item it = fa.items[blockIdx.x];
     const int n = it.n;
     if(n == 0) return;
-    double* W = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* W = work_base<!PANEL>(smem, fa);
     const int64_t np = int64_t(n) * (n + 1) / 2;
     double* rd = W + np;  // shifted original diagonal (dependence threshold)
     double* sv = rd + n;  // 1 / r_kk, or 0
+    double* Y = sv + n;   // LDS variant: two columns of X in flight
     double* X = fa.U + it.u_off;
     const int tid = threadIdx.x;
     const double* G = fa.A + it.a_off;
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
             __syncthreads();
             double inv_d = 0.0;
             if(k < n) {
-                const double d = W[pk(k, k)];
+                const double d = W[pk32(k, k)];
                 const bool ok = d > 64.0 * DBL_EPSILON * rd[k] && d <= DBL_MAX;
                 inv_d = ok ? 1.0 / d : 0.0;
                 if(tid == 0) sv[k] = ok ? 1.0 / sqrt(d) : 0.0;  // read from step k + 1 on
@@ -368,7 +383,7 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
             if(k >= 1) {
                 // (a) deferred scaling of row k - 1
                 const double s = sv[k - 1];
-                for(int j = k + tid; j < n; j += CI_THREADS) W[pk(k - 1, j)] *= s;
+                for(int j = k + tid; j < n; j += CI_THREADS) W[pk32(k - 1, j)] *= s;
             }
             if(k + 1 < n && inv_d != 0.0) {
                 // (b) columns a = k+1+pp (rows k+1 .. a) and b = n-1-pp (rows k+1 .. b) as one list
@@ -382,19 +397,23 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                         const int a = pp, b = t - 1 - pp;
                         const int la = a + 1;
                         const int len = (a == b) ? la : la + b + 1;
-                        const double wka = W[pk(k, k + 1 + a)] * inv_d, wkb = W[pk(k, k + 1 + b)] * inv_d;
+                        const double wka = W[pk32(k, k + 1 + a)] * inv_d, wkb = W[pk32(k, k + 1 + b)] * inv_d;
                         for(int e = s0; e < len; e += c2) {
                             const bool first = e < la;
                             const int i = k + 1 + (first ? e : e - la);
                             const int j = k + 1 + (first ? a : b);
-                            W[pk(i, j)] -= W[pk(k, i)] * (first ? wka : wkb);
+                            W[pk32(i, j)] -= W[pk32(k, i)] * (first ? wka : wkb);
                         }
                     }
                 }
             }
-            if(k >= 2) {
-                // (c) column j = k - 1 of X, rows i < j; X_il is read from X^T at X[l + i n]
+            if(k >= 1) {
+                // (c) column j = k - 1 of X into Yc, rows i < j, from the columns of X before it
+                // (in place in W, the last one still in Yp) and column j of R; then column j - 1
+                // of X (computed last step) goes from Yp into W, over R's column j - 1
                 const int j = k - 1;
+                const double* Yp = Y + ((k - 1) & 1) * n;
+                double* Yc = Y + (k & 1) * n;
                 constexpr int per = CI_THREADS / CI_LANES;
                 const double sj = sv[j];
                 for(int base = 0; base < j; base += per) {
@@ -403,22 +422,37 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                     if(i < j) {
                         int l = i + part;
                         for(; l + CI_LANES < j; l += 2 * CI_LANES) {
-                            const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
-                            const double x1 = X[l + CI_LANES + int64_t(i) * n];
-                            acc0 = fma(x0, W[pk(l, j)], acc0);
-                            acc1 = fma(x1, W[pk(l + CI_LANES, j)], acc1);
+                            const double x0 = W[pk32(i, l)];  // l < j - 1: X in place (its diagonal is sv)
+                            const double x1 = (l + CI_LANES == j - 1) ? Yp[i] : W[pk32(i, l + CI_LANES)];
+                            acc0 = fma(x0, W[pk32(l, j)], acc0);
+                            acc1 = fma(x1, W[pk32(l + CI_LANES, j)], acc1);
                         }
                         if(l < j) {
-                            const double x0 = (l == i) ? sv[i] : X[l + int64_t(i) * n];
-                            acc0 = fma(x0, W[pk(l, j)], acc0);
+                            const double x0 = (l == j - 1) ? Yp[i] : W[pk32(i, l)];
+                            acc0 = fma(x0, W[pk32(l, j)], acc0);
                         }
                     }
                     double acc = acc0 + acc1;
                     for(int o = CI_LANES >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                    if(i < j && part == 0) X[j + int64_t(i) * n] = -acc * sj;
+                    if(i < j && part == 0) Yc[i] = -acc * sj;
                 }
+                if(tid == 0) Yc[j] = sj;
+                if(k >= 2)
+                    for(int i = tid; i <= j - 1; i += CI_THREADS) W[pk32(i, j - 1)] = Yp[i];
             }
         }
+        // the last column of X into W, then X (upper triangle of W, zeros below) to the output
+        __syncthreads();
+        {
+            const double* Yl = Y + (n & 1) * n;
+            for(int i = tid; i < n; i += CI_THREADS) W[pk32(i, n - 1)] = Yl[i];
+        }
+        __syncthreads();
+        for(int64_t e = tid; e < nn; e += CI_THREADS) {
+            const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+            X[e] = i <= j ? W[pk32(i, j)] : 0.0;
+        }
+        return;
     } else {
         // panel variant (blocks in the global working copy): rows of W stay unscaled (row k of R is
         // W's row k times sv[k]) and are never written after their pivot step.  Inside a panel of
@@ -493,7 +527,8 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                         const double* xr = X + int64_t(i) * n;
                         int l = i + part;
                         for(; l + 3 * CI_LANES < j; l += 4 * CI_LANES) {
-                            const double x0 = (l == i) ? sv[i] : xr[l];
+                            const double xg = xr[l], xs = sv[i];  // both read: selecting a value, not an address space
+                            const double x0 = (l == i) ? xs : xg;
                             const double x1 = xr[l + CI_LANES], x2 = xr[l + 2 * CI_LANES], x3 = xr[l + 3 * CI_LANES];
                             acc0 = fma(x0 * sv[l], W[cj + l], acc0);
                             acc1 = fma(x1 * sv[l + CI_LANES], W[cj + l + CI_LANES], acc1);
@@ -501,7 +536,8 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                             acc3 = fma(x3 * sv[l + 3 * CI_LANES], W[cj + l + 3 * CI_LANES], acc3);
                         }
                         for(; l < j; l += CI_LANES) {
-                            const double x0 = (l == i) ? sv[i] : xr[l];
+                            const double xg = xr[l], xs = sv[i];  // both read: selecting a value, not an address space
+                            const double x0 = (l == i) ? xs : xg;
                             acc0 = fma(x0 * sv[l], W[cj + l], acc0);
                         }
                     }
@@ -528,6 +564,7 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
 
 // Symmetric eigen-decomposition by parallel cyclic Jacobi.  Item: a_off (n x n), s_off -> eigenvalues,
 // u_off -> eigenvectors (columns), ascending order.
+template <bool LDS>
 __global__ __launch_bounds__(F_THREADS) void k_eigh(FactorArgs fa)
 {
     extern __shared__ double smem[];
@@ -535,7 +572,7 @@ __global__ __launch_bounds__(F_THREADS) void k_eigh(FactorArgs fa)
     const xh_factor_item it = fa.items[blockIdx.x];
     const int n = it.m;
     if(n == 0) return;
-    double* Am = fa.use_lds ? smem : fa.scratch + int64_t(blockIdx.x) * fa.scratch_stride;
+    double* Am = work_base<LDS>(smem, fa);
     double* V = Am + int64_t(n) * n;
     double* cs = V + int64_t(n) * n;  // per pair: c, s  (n entries)
     int* pr = reinterpret_cast<int*>(cs + n + 1);  // per pair: p, q (n ints)
@@ -652,8 +689,8 @@ size_t lds_budget(xh_ctx* ctx)
 }
 
 template <typename Need>
-void run_factor(xh_ctx* ctx, void (*kern)(FactorArgs), const std::vector<xh_factor_item>& items, Need need_doubles, const double* A, double* U,
-                double* S, double* Vt)
+void run_factor(xh_ctx* ctx, void (*kern_lds)(FactorArgs), void (*kern_global)(FactorArgs), const std::vector<xh_factor_item>& items, Need need_doubles,
+                const double* A, double* U, double* S, double* Vt)
 {
     const size_t budget = lds_budget(ctx);
     std::vector<xh_factor_item> small, big;
@@ -679,6 +716,7 @@ void run_factor(xh_ctx* ctx, void (*kern)(FactorArgs), const std::vector<xh_fact
         double* scratch = nullptr;
         if(!lds) check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(maxd) * v.size() * sizeof(double), ctx->stream), "hipMallocAsync");
         FactorArgs fa{d_items, A, U, S, Vt, scratch, maxd, lds ? 1 : 0, d_status};
+        auto* kern = lds ? kern_lds : kern_global;
         const size_t shmem = lds ? size_t(maxd) * sizeof(double) : 0;
         if(lds && shmem > 64 * 1024) check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, int(shmem)), "hipFuncSetAttribute");
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(v.size())), dim3(F_THREADS), shmem, ctx->stream, fa);
@@ -718,7 +756,7 @@ extern "C" int xh_block_svd(xh_ctx* ctx, const xh_factor_item* items, int64_t ni
         }
         block_jacobi_svd(ctx, big, A, U, S, Vt);
         run_factor(
-            ctx, k_svd_jacobi, v,
+            ctx, k_svd_jacobi<true>, k_svd_jacobi<false>, v,
             [](const xh_factor_item& it) {
                 const int64_t mm = std::max(it.m, it.n), nn = std::min(it.m, it.n);
                 return svd_work_doubles(mm, nn);
@@ -732,7 +770,7 @@ extern "C" int xh_block_qr(xh_ctx* ctx, const xh_factor_item* items, int64_t nit
     return guard(ctx, [&] {
         std::vector<xh_factor_item> v(items, items + nitems);
         run_factor(
-            ctx, k_qr, v, [](const xh_factor_item& it) { return int64_t(it.m) * it.n + std::min(it.m, it.n) + 1; }, A, Q, nullptr, R);
+            ctx, k_qr<true>, k_qr<false>, v, [](const xh_factor_item& it) { return int64_t(it.m) * it.n + std::min(it.m, it.n) + 1; }, A, Q, nullptr, R);
     });
 }
 
@@ -743,7 +781,7 @@ extern "C" int xh_block_eigh(xh_ctx* ctx, const xh_factor_item* items, int64_t n
         for(auto& it : v)
             if(it.m != it.n) throw xh::Error(XH_EINVAL, "eigh: blocks must be square");
         run_factor(
-            ctx, k_eigh, v, [](const xh_factor_item& it) { return 2 * int64_t(it.m) * it.m + 2 * int64_t(it.m) + 4; }, A, V, W, nullptr);
+            ctx, k_eigh<true>, k_eigh<false>, v, [](const xh_factor_item& it) { return 2 * int64_t(it.m) * it.m + 2 * int64_t(it.m) + 4; }, A, V, W, nullptr);
     });
 }
 
