@@ -398,12 +398,12 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                         const int la = a + 1;
                         const int len = (a == b) ? la : la + b + 1;
                         const double wka = W[pk32(k, k + 1 + a)] * inv_d, wkb = W[pk32(k, k + 1 + b)] * inv_d;
-                        for(int e = s0; e < len; e += c2) {
-                            const bool first = e < la;
-                            const int i = k + 1 + (first ? e : e - la);
-                            const int j = k + 1 + (first ? a : b);
-                            W[pk32(i, j)] -= W[pk32(k, i)] * (first ? wka : wkb);
-                        }
+                        // entries s0, s0 + c2, ... of the list: column a's rows, then column b's;
+                        // a column's rows are contiguous in the packed triangle
+                        const int ca = pk32(k + 1, k + 1 + a), cb = pk32(k + 1, k + 1 + b);
+                        int e = s0;
+                        for(; e < la; e += c2) W[ca + e] -= W[pk32(k, k + 1 + e)] * wka;
+                        for(; e < len; e += c2) W[cb + e - la] -= W[pk32(k, k + 1 + e - la)] * wkb;
                     }
                 }
             }
@@ -420,15 +420,24 @@ __global__ __launch_bounds__(CI_THREADS) void k_chol_inv(FactorArgs fa)
                     const int i = base + tid / CI_LANES, part = tid & (CI_LANES - 1);
                     double acc0 = 0.0, acc1 = 0.0;
                     if(i < j) {
+                        // X_il for l < j - 1 in place (its diagonal is sv), X_i,j-1 in Yp.  For
+                        // l = j - 1 (column j - 1 of W is being written back this step) the load
+                        // reads R's column j instead and its value is not used, so that the select
+                        // is a value select, not a branch
+                        const double yp = Yp[i];
                         int l = i + part;
                         for(; l + CI_LANES < j; l += 2 * CI_LANES) {
-                            const double x0 = W[pk32(i, l)];  // l < j - 1: X in place (its diagonal is sv)
-                            const double x1 = (l + CI_LANES == j - 1) ? Yp[i] : W[pk32(i, l + CI_LANES)];
+                            const bool last = l + CI_LANES == j - 1;
+                            const double x0 = W[pk32(i, l)];
+                            const double w1 = W[last ? pk32(l + CI_LANES, j) : pk32(i, l + CI_LANES)];
+                            const double x1 = last ? yp : w1;
                             acc0 = fma(x0, W[pk32(l, j)], acc0);
                             acc1 = fma(x1, W[pk32(l + CI_LANES, j)], acc1);
                         }
                         if(l < j) {
-                            const double x0 = (l == j - 1) ? Yp[i] : W[pk32(i, l)];
+                            const bool last = l == j - 1;
+                            const double w0 = W[last ? pk32(l, j) : pk32(i, l)];
+                            const double x0 = last ? yp : w0;
                             acc0 = fma(x0, W[pk32(l, j)], acc0);
                         }
                     }
