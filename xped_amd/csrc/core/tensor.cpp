@@ -615,7 +615,14 @@ static std::shared_ptr<CachedGather> get_gather(Runtime* rt, const TensorStructu
         it->second->last_use = ++rt->tick;
         return it->second;
     }
-    const auto t0 = std::chrono::steady_clock::now();
+    auto t0 = std::chrono::steady_clock::now();
+    auto tl = t0;  // end of the last lap
+    auto lap = [rt, &tl](int slot) {
+        const auto now = std::chrono::steady_clock::now();
+        rt->host_t[slot] += std::chrono::duration<double>(now - tl).count();
+        tl = now;
+    };
+    double own_s = 0.0;  // this plan's host time, without the GEMM plan's (counted there)
     auto g = std::make_shared<CachedGather>();
     g->last_use = ++rt->tick;
     StructPtr out = permuted_structure(in, shift, p);
@@ -629,12 +636,17 @@ static std::shared_ptr<CachedGather> get_gather(Runtime* rt, const TensorStructu
             out = si->second;
     }
     g->permuted = out;
+    lap(5);
+    own_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     g->gemm = get_gemm_plan(rt, *out, right);
+    t0 = tl = std::chrono::steady_clock::now();
     int ok = 0;
     if(rt->ctx && g->gemm->dev) dcheck(rt->ctx, xh_gemm_gather_eligible(g->gemm->dev, &ok), "gather eligibility");
     if(ok) {
         std::shared_ptr<const SymbolicPermute> sp = symbolic_permute(rt, in, *out, shift, p);
+        lap(6);
         PermutePlan pl = numeric_permute_plan(*sp, in, out, p);
+        lap(0);
         for(const PermuteTask& t : pl.tasks) {
             if(t.nsrc != 1) {
                 ok = 0;
@@ -678,13 +690,16 @@ static std::shared_ptr<CachedGather> get_gather(Runtime* rt, const TensorStructu
             desc.nstarts = static_cast<int64_t>(starts.size());
             desc.starts = starts.data();
             desc.src_elems = static_cast<int64_t>(in.size);
+            lap(1);
             dcheck(rt->ctx, xh_gather_plan_create(rt->ctx, g->gemm->dev, &desc, &g->dev), "gather plan");
+            lap(2);
         }
         recycle_plan_tables(pl);
     }
     rt->gather_cache.emplace(k, g);
     rt->trim_caches();
-    rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    own_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->permute_plan_s += own_s;
     rt->n_permute_plans++;
     return g;
 }

@@ -241,6 +241,36 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_f64p(GemmArgs g)
 // with the small operand staged in LDS (read as broadcasts), and writes its S outputs.
 constexpr int SK_MAX = 32, SK_KMAX = 32, SK_THREADS = 256;
 
+// The S <= MS sums of one long-side index against the small operand in LDS (sS, [kk][s]) and
+// their stores (output s at base + s * stride), in passes of at most 16 outputs: the K loaded
+// elements stay in registers and only 16 accumulators are live (the gather kernel: 145 instead of
+// 169 VGPRs at MS = 32, three waves per SIMD instead of two).  Every sum runs over kk in order, as
+// one pass would.
+template <int MS>
+__device__ __forceinline__ void skinny_sums(const double* __restrict__ sS, const double (&x)[SK_KMAX], int K, int S, double* base, int64_t stride, double alpha,
+                                            double beta)
+{
+    constexpr int MH = MS > 16 ? 16 : MS;
+#pragma unroll
+    for(int h = 0; h < MS; h += MH) {
+        if(h >= S) break;
+        double acc[MH];
+#pragma unroll
+        for(int c = 0; c < MH; ++c) acc[c] = 0.0;
+#pragma unroll
+        for(int kk = 0; kk < SK_KMAX; ++kk)
+            if(kk < K)
+#pragma unroll
+                for(int c = 0; c < MH; ++c) acc[c] = __builtin_fma(sS[kk * MS + h + c], x[kk], acc[c]);
+#pragma unroll
+        for(int c = 0; c < MH; ++c)
+            if(h + c < S) {
+                double* p = base + int64_t(h + c) * stride;
+                *p = (beta == 0.0) ? alpha * acc[c] : alpha * acc[c] + beta * *p;
+            }
+    }
+}
+
 template <int MS, int R>
 __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny(GemmArgs g, const Tile* __restrict__ blocks, int nblocks)
 {
@@ -428,20 +458,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, G
 #pragma unroll
         for(int kk = 0; kk < SK_KMAX; ++kk)
             if((negm >> kk) & 1u) xv[kk] = -xv[kk];
-        double acc[MS];
-#pragma unroll
-        for(int sx = 0; sx < MS; ++sx) acc[sx] = 0.0;
-#pragma unroll
-        for(int kk = 0; kk < SK_KMAX; ++kk)
-            if(kk < K)
-#pragma unroll
-                for(int sx = 0; sx < MS; ++sx) acc[sx] = __builtin_fma(sS[kk * MS + sx], xv[kk], acc[sx]);
-#pragma unroll
-        for(int sx = 0; sx < MS; ++sx)
-            if(sx < S) {
-                double* p = g.C + it.c_off + int64_t(sx) * it.m + L;
-                *p = (g.beta == 0.0) ? g.alpha * acc[sx] : g.alpha * acc[sx] + g.beta * *p;
-            }
+        skinny_sums<MS>(sS, xv, K, S, g.C + it.c_off + L, it.m, g.alpha, g.beta);
     }
 }
 
