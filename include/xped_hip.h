@@ -39,6 +39,7 @@ typedef struct xh_ctx xh_ctx;
 typedef struct xh_permute_plan xh_permute_plan;
 typedef struct xh_gemm_plan xh_gemm_plan;
 typedef struct xh_gather_plan xh_gather_plan;
+typedef struct xh_scatter_plan xh_scatter_plan;
 
 /* ---- context and buffers ------------------------------------------------------------ */
 int xh_create(int device, xh_ctx** out);
@@ -148,6 +149,34 @@ int xh_gather_plan_create(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather
 int xh_gather_plan_destroy(xh_ctx* ctx, xh_gather_plan* plan);
 int xh_gemm_gather(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather_plan* gather, const double* src, int64_t src_elems, const double* B, double* C,
                    double alpha, double beta);
+/* The product's own permute folded into the same launch (the result permute of contract): each
+ * product element is written straight to its place in permute(product), as 0 + c v with c = +-1,
+ * and the product is never materialised.  No counterpart in the reference.
+ *   - perm: the permute of the product (output subblocks as tasks, one source each, coefficient
+ *     +-1), p its leg permutation (output leg k is product leg p[k]);
+ *   - per GEMM item: grid_off, ncod and cstarts_off locate its task grid (task of the product
+ *     subblock in domain tree a and codomain tree c: grid[grid_off + a * ncod + c]) and its
+ *     ncod + 1 column starts in cstarts; the domain trees are the gather plan's;
+ *   - dst_elems: size of the permuted product (every destination access is checked against it).
+ * Sectors of the product that no GEMM item writes are not touched: the caller zeroes them.  The
+ * result equals xh_gemm_gather (beta = 0) + xh_permute bit for bit. */
+typedef struct {
+    xh_permute_desc perm;
+    const int32_t* p;
+    int64_t nitems;
+    const int64_t* grid_off;
+    const int32_t* ncod;
+    const int64_t* cstarts_off;
+    int64_t ngrid;
+    const int32_t* grid;
+    int64_t ncstarts;
+    const int32_t* cstarts;
+    int64_t dst_elems;
+} xh_scatter_desc;
+int xh_scatter_plan_create(xh_ctx* ctx, const xh_gather_plan* gather, const xh_scatter_desc* desc, xh_scatter_plan** out);
+int xh_scatter_plan_destroy(xh_ctx* ctx, xh_scatter_plan* plan);
+int xh_gemm_gather_scatter(xh_ctx* ctx, const xh_gemm_plan* gemm, const xh_gather_plan* gather, const xh_scatter_plan* scatter, const double* src,
+                           int64_t src_elems, const double* B, double* dst, int64_t dst_elems, double alpha);
 /* GEMM tile autotuner (no counterpart in the reference, whose per-block BLAS calls have no tiles
  * to choose): with enable != 0, the first FP64 call of a plan that has 128 x 128 tiles, with
  * beta = 0, runs both tilings (128 x 128 + 64 x 64, and 64 x 64 only), timed with HIP events

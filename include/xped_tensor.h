@@ -47,10 +47,11 @@ int xt_rt_clear_caches(xt_rt* rt);
 int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans);
 /* contract() with the permute of its left operand read inside the product (xh_gemm_gather)
  * wherever the product qualifies (FP64, every sector block a skinny item with the permuted operand
- * on its long side, single-source +-1 permute, no tape recording): enable = 1 (the default) /
- * 0 / -1 keeps the setting; *n_fused (may be null) = contractions run fused so far.  Results are
- * the same bit for bit either way; the reference always materialises the permuted operand
- * (Tensor.hpp:328-351). */
+ * on its long side, single-source +-1 permute, no tape recording): enable = 1 (the default);
+ * 2 also writes such a product straight into its own permuted layout when that permute is
+ * single-source +-1 (xh_gemm_gather_scatter); 0 materialises every permuted tensor; -1 keeps the
+ * setting.  *n_fused (may be null) = contractions run fused so far.  Results are the same bit for
+ * bit in every mode; the reference always materialises the permuted operands (Tensor.hpp:328-351). */
 int xt_rt_set_fuse_permute(xt_rt* rt, int enable, long* n_fused);
 /* current cached permute plans, GEMM plans, and plans dropped so far */
 int xt_rt_cache_sizes(const xt_rt* rt, int64_t* permute_plans, int64_t* gemm_plans, int64_t* evicted);

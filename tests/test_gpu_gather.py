@@ -12,6 +12,9 @@ materialising the permuted copy (xh_gemm_gather).  Checks:
 * products that do not qualify (a wide right operand, SU(2) recoupling sums) take the materialising
   path and still match the oracle;
 * CTMRG moves with and without the fusion give the same environment and energy bit for bit.
+
+Each check runs with the gather alone (mode 1, the default) and with the product's own permute
+folded into its stores as well (mode 2, xh_gemm_gather_scatter).
 """
 import random
 
@@ -57,8 +60,8 @@ def _pair(rt, pcode, big, small, open_b, seed):
     return oA, A, oB, B
 
 
-def _fused_and_plain(rt, A, B):
-    n0 = rt.fuse_permute(True)
+def _fused_and_plain(rt, A, B, mode=1):
+    n0 = rt.fuse_permute(mode)
     Cf = A.contract(B, A1, A2, 3)
     n1 = rt.fuse_permute(False)
     Cp = A.contract(B, A1, A2, 3)
@@ -67,30 +70,36 @@ def _fused_and_plain(rt, A, B):
     return Cf, Cp, n1 - n0
 
 
+@pytest.mark.parametrize("mode", [1, 2], ids=["gather", "gather+scatter"])
 @pytest.mark.parametrize("pcode", [xt.U1, xt.FU1, xt.FU1xU1], ids=["U1", "FU1", "FU1xU1"])
-def test_gather_gemm_is_the_permute_then_the_product(rt, pcode):
+def test_gather_gemm_is_the_permute_then_the_product(rt, pcode, mode):
+    # mode 2 also writes the product straight into its permuted layout (the result permute of
+    # this contraction, (0, 3, 5 | 1, 2, 4), is not the identity)
     big = [(-2, 3), (0, 5), (2, 4)]
     small = [(0, 1), (1, 2)]
     oA, A, oB, B = _pair(rt, pcode, big, small, [(-1, 2), (1, 1)], 7)
-    Cf, Cp, nf = _fused_and_plain(rt, A, B)
+    Cf, Cp, nf = _fused_and_plain(rt, A, B, mode)
     assert nf == 1, "the product did not run fused"
     assert structure_of(Cf) == structure_of(Cp)
     assert np.array_equal(Cf.download(), Cp.download())
     oC = oten.contract(oA, A1, oB, A2, 3)
     assert structure_of(Cf) == oracle_structure(oC)
     np.testing.assert_allclose(Cf.download(), oC.data, rtol=0, atol=1e-12 * np.abs(oC.data).max())
-    # a second call reuses the cached gather plan
+    # a second call reuses the cached plans
+    rt.fuse_permute(mode)
     assert np.array_equal(A.contract(B, A1, A2, 3).download(), Cf.download())
+    rt.fuse_permute(True)
 
 
-def test_gather_gemm_rows_across_many_domain_trees(rt):
+@pytest.mark.parametrize("mode", [1, 2], ids=["gather", "gather+scatter"])
+def test_gather_gemm_rows_across_many_domain_trees(rt, mode):
     # many small domain trees: the plan's row blocks meet up to four trees each, and blocks end
     # early where a fifth would begin (the first test's trees are longer than a block instead)
     r = random.Random(3)
     big = [(q, r.randint(1, 3)) for q in range(-3, 4)]
     small = [(-1, 2), (0, 1), (1, 1)]
     oA, A, oB, B = _pair(rt, xt.U1, big, small, [(-1, 1), (0, 2), (1, 1)], 11)
-    Cf, Cp, nf = _fused_and_plain(rt, A, B)
+    Cf, Cp, nf = _fused_and_plain(rt, A, B, mode)
     assert nf == 1
     assert np.array_equal(Cf.download(), Cp.download())
     oC = oten.contract(oA, A1, oB, A2, 3)
@@ -117,7 +126,8 @@ def test_products_that_do_not_qualify_are_materialised(rt):
     np.testing.assert_allclose(Cf.download(), oC.data, rtol=0, atol=1e-12 * np.abs(oC.data).max())
 
 
-def test_ctm_moves_with_and_without_the_fusion_agree_bit_for_bit():
+@pytest.mark.parametrize("mode", [1, 2], ids=["gather", "gather+scatter"])
+def test_ctm_moves_with_and_without_the_fusion_agree_bit_for_bit(mode):
     def run(fuse):
         rt = xt.Runtime(0)
         rt.fuse_permute(fuse)
@@ -129,7 +139,7 @@ def test_ctm_moves_with_and_without_the_fusion_agree_bit_for_bit():
         env = [c.tensor(w, s).download() for w in ("C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4") for s in range(4)]
         return c.energy(), env, rt.fuse_permute(None)
 
-    Ef, envf, nf = run(True)
+    Ef, envf, nf = run(mode)
     Ep, envp, np_ = run(False)
     assert nf > 0 and np_ == 0
     assert Ef == Ep

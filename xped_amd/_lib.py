@@ -89,6 +89,9 @@ _SIGS = [
     ("xh_gather_plan_create", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
     ("xh_gather_plan_destroy", C.c_int, [_P, _P]),
     ("xh_gemm_gather", C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, C.c_double, C.c_double]),
+    ("xh_scatter_plan_create", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    ("xh_scatter_plan_destroy", C.c_int, [_P, _P]),
+    ("xh_gemm_gather_scatter", C.c_int, [_P, _P, _P, _P, _P, C.c_int64, _P, _P, C.c_int64, C.c_double]),
     ("xh_block_svd", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P, _P]),
     ("xh_block_qr", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),
     ("xh_block_eigh", C.c_int, [_P, C.POINTER(xh_factor_item), C.c_int64, _P, _P, _P]),

@@ -206,8 +206,8 @@ int xt_rt_set_max_plans(xt_rt* rt, int64_t max_plans)
 int xt_rt_set_fuse_permute(xt_rt* rt, int enable, long* n_fused)
 {
     return run(rt, [&] {
-        if(enable < -1 || enable > 1) throw xa::Error(XH_EINVAL, "xt_rt_set_fuse_permute: enable must be -1, 0 or 1");
-        if(enable >= 0) rt->rt.fuse_permute = enable != 0;
+        if(enable < -1 || enable > 2) throw xa::Error(XH_EINVAL, "xt_rt_set_fuse_permute: enable must be -1, 0, 1 or 2");
+        if(enable >= 0) rt->rt.fuse_permute = enable;
         if(n_fused) *n_fused = rt->rt.n_fused;
     });
 }
@@ -227,7 +227,7 @@ int xt_rt_clear_caches(xt_rt* rt)
         if(rt->rt.ctx) xh_synchronize(rt->rt.ctx);
         // gather plans first: each refers to a GEMM plan of the cache
         for(auto& [k, v] : rt->rt.gather_cache)
-            if(v->dev) xh_gather_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;
+            if(rt->rt.ctx) v->release(rt->rt.ctx);
         rt->rt.gather_cache.clear();
         for(auto& [k, v] : rt->rt.permute_cache)
             if(v->dev) xh_permute_plan_destroy(rt->rt.ctx, v->dev), v->dev = nullptr;

@@ -69,8 +69,7 @@ std::string charges_key(const TensorStructure& t)
 Runtime::~Runtime()
 {
     if(ctx) {
-        for(auto& [k, v] : gather_cache)  // before the GEMM plans they refer to
-            if(v->dev) xh_gather_plan_destroy(ctx, v->dev);
+        for(auto& [k, v] : gather_cache) v->release(ctx);  // before the GEMM plans they refer to
         for(auto& [k, v] : permute_cache)
             if(v->dev) xh_permute_plan_destroy(ctx, v->dev);
         for(auto& [k, v] : gemm_cache)
@@ -104,8 +103,7 @@ long trim_lru(Map& m, std::size_t max_entries, Destroy destroy)
 void Runtime::trim_caches()
 {
     n_evicted += trim_lru(gather_cache, max_plans, [this](CachedGather& c) {
-        if(ctx && c.dev) xh_gather_plan_destroy(ctx, c.dev);
-        c.dev = nullptr;
+        if(ctx) c.release(ctx);
     });
     n_evicted += trim_lru(permute_cache, max_plans, [this](CachedPermute& c) {
         if(ctx && c.dev) xh_permute_plan_destroy(ctx, c.dev);
@@ -118,6 +116,15 @@ void Runtime::trim_caches()
     // host-only caches: dimension-free permute parts and shared structures
     if(sym_permute_cache.size() > 2 * max_plans) sym_permute_cache.clear();
     if(struct_cache.size() > 4 * max_plans) struct_cache.clear();
+}
+
+void CachedGather::release(xh_ctx* ctx)
+{
+    for(auto& [k, sc] : scatter)
+        if(sc.dev) xh_scatter_plan_destroy(ctx, sc.dev);
+    scatter.clear();
+    if(dev) xh_gather_plan_destroy(ctx, dev);
+    dev = nullptr;
 }
 
 void Runtime::require_device() const
@@ -704,6 +711,121 @@ static std::shared_ptr<CachedGather> get_gather(Runtime* rt, const TensorStructu
     return g;
 }
 
+// The permute of a gather product folded into its stores (CachedGather::Scatter), built on the
+// first use of (shift, p) for that product.  It qualifies when the permute is single-source with
+// coefficients +-1 and every sector of the product is written by a GEMM item.
+static const CachedGather::Scatter& get_scatter(Runtime* rt, CachedGather& g, int shift, const std::vector<int>& p)
+{
+    std::string k;
+    put_i(k, shift);
+    for(int x : p) put_i(k, x);
+    auto it = g.scatter.find(k);
+    if(it != g.scatter.end()) return it->second;
+    const auto t0 = std::chrono::steady_clock::now();
+    CachedGather::Scatter sc;
+    const TensorStructure& prod = *g.gemm->plan.out;
+    StructPtr out = permuted_structure(prod, shift, p);
+    {
+        std::string sk = legs_key(out->sym, out->unc_dom, out->unc_cod);
+        auto si = rt->struct_cache.find(sk);
+        if(si == rt->struct_cache.end())
+            rt->struct_cache.emplace(sk, out);
+        else if(si->second->sector == out->sector)
+            out = si->second;
+    }
+    sc.out = out;
+    // every product sector must be some item's output (no zero-filled sectors, no empty items)
+    const auto& items = g.gemm->plan.items;
+    std::unordered_map<std::int64_t, std::size_t> item_at;  // product sector offset -> GEMM item
+    bool ok = g.dev != nullptr && g.gemm->zero_rects.empty();
+    for(std::size_t j = 0; ok && j < items.size(); ++j) {
+        if(items[j].m == 0 || items[j].n == 0) ok = false;
+        item_at.emplace(items[j].c_off, j);
+    }
+    std::vector<std::size_t> sector_item(prod.sector.size());
+    for(std::size_t s = 0; ok && s < prod.sector.size(); ++s) {
+        auto f = item_at.find(static_cast<std::int64_t>(prod.offsets[s]));
+        if(f == item_at.end() || prod.rows[s] == 0 || prod.cols[s] == 0)
+            ok = false;
+        else
+            sector_item[s] = f->second;
+    }
+    if(ok) {
+        std::shared_ptr<const SymbolicPermute> sp = symbolic_permute(rt, prod, *out, shift, p);
+        PermutePlan pl = numeric_permute_plan(*sp, prod, out, p);
+        for(const PermuteTask& t : pl.tasks) {
+            const double c = t.nsrc == 1 ? pl.sources[t.src_begin].coeff : 0.0;
+            if(t.nsrc != 1 || (c != 1.0 && c != -1.0)) {
+                ok = false;
+                break;
+            }
+        }
+        // per item the task grid of its product block and its column starts
+        std::vector<std::int64_t> grid_off(items.size(), 0), cstarts_off(items.size(), 0);
+        std::vector<std::int32_t> ncod(items.size(), 0), grid, cstarts;
+        if(ok) {
+            for(std::size_t s = 0; s < prod.sector.size(); ++s) {
+                const std::size_t j = sector_item[s];
+                const auto& dt = prod.dom().tree(prod.sector[s]);
+                const auto& ct = prod.cod().tree(prod.sector[s]);
+                grid_off[j] = static_cast<std::int64_t>(grid.size());
+                ncod[j] = static_cast<std::int32_t>(ct.size());
+                grid.resize(grid.size() + dt.size() * ct.size(), -1);
+                cstarts_off[j] = static_cast<std::int64_t>(cstarts.size());
+                std::int64_t acc = 0;
+                cstarts.push_back(0);
+                for(const auto& t : ct) cstarts.push_back(static_cast<std::int32_t>(acc += static_cast<std::int64_t>(t.dim)));
+            }
+            for(std::size_t t = 0; t < pl.tasks.size(); ++t) {
+                const auto& src = sp->sources[static_cast<std::size_t>(sp->src_begin[t])];
+                const std::size_t j = sector_item[static_cast<std::size_t>(src.sector)];
+                grid[static_cast<std::size_t>(grid_off[j] + static_cast<std::int64_t>(src.t1) * ncod[j] + src.t2)] = static_cast<std::int32_t>(t);
+            }
+            for(std::int32_t t : grid)
+                if(t < 0) ok = false;  // a product subblock that no task reads: leave the permute to permute()
+        }
+        if(ok) {
+            xh_scatter_desc desc{};
+            desc.perm = xh_permute_desc{pl.nlegs, pl.new_rank, static_cast<int64_t>(pl.tasks.size()), static_cast<int64_t>(pl.sources.size()),
+                                        reinterpret_cast<const xh_permute_task*>(pl.tasks.data()), reinterpret_cast<const xh_permute_source*>(pl.sources.data())};
+            std::vector<std::int32_t> p32(p.begin(), p.end());
+            desc.p = p32.data();
+            desc.nitems = static_cast<int64_t>(items.size());
+            desc.grid_off = grid_off.data();
+            desc.ncod = ncod.data();
+            desc.cstarts_off = cstarts_off.data();
+            desc.ngrid = static_cast<int64_t>(grid.size());
+            desc.grid = grid.data();
+            desc.ncstarts = static_cast<int64_t>(cstarts.size());
+            desc.cstarts = cstarts.data();
+            desc.dst_elems = static_cast<int64_t>(out->size);
+            dcheck(rt->ctx, xh_scatter_plan_create(rt->ctx, g.dev, &desc, &sc.dev), "scatter plan");
+        }
+        recycle_plan_tables(pl);
+    }
+    rt->permute_plan_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rt->n_permute_plans++;
+    return g.scatter.emplace(k, sc).first->second;
+}
+
+// the product of a gather plan with its permute folded in: out = permute(permute(a) * pb)
+static void launch_gather_scatter(const CachedGather* g, const CachedGather::Scatter& sc, const DTensor& a, const DTensor& pb, DTensor& out)
+{
+    Runtime* rt = a.rt;
+    if(out.size() == 0) return;
+    const CachedGemm* cg = g->gemm.get();
+    tag_contraction(rt, true);
+    const int rc = xh_gemm_gather_scatter(rt->ctx, cg->dev, g->dev, sc.dev, a.data(), static_cast<int64_t>(a.size()), pb.data(), out.data(),
+                                          static_cast<int64_t>(out.size()), 1.0);
+    tag_contraction(rt, false);
+    dcheck(rt->ctx, rc, "gather gemm with the product's permute");
+    if(rt->time_gemm)
+        for(const auto& it : cg->plan.items) rt->gemm_log.push_back({static_cast<int32_t>(it.m), static_cast<int32_t>(it.n), static_cast<int32_t>(it.k)});
+    rt->gemm_flops += cg->plan.flops;
+    rt->n_gemm++;
+    rt->n_fused++;
+}
+
 // the product of a gather plan: out = permute(a) * pb with permute(a) read inside the GEMM
 static void launch_gather(const CachedGather* g, const DTensor& a, const DTensor& pb, DTensor& out)
 {
@@ -732,11 +854,21 @@ DTensor contract(const DTensor& a, const std::vector<int>& a1, const DTensor& b,
     Runtime* rt = a.rt;
     bool ident = cp.shift1 == 0;
     for(std::size_t i = 0; ident && i < cp.p1.size(); ++i) ident = cp.p1[i] == static_cast<int>(i);
-    if(rt->fuse_permute && rt->ctx && precision == XH_FP64 && !ident && a.size() > 0 && b.rt == rt && !recording(a) && !recording(b)) {
+    if(rt->fuse_permute > 0 && rt->ctx && precision == XH_FP64 && !ident && a.size() > 0 && b.rt == rt && !recording(a) && !recording(b)) {
         // the left permute read inside the product when the product qualifies (xh_gemm_gather)
         DTensor pb = permute(b, cp.shift2, cp.p2);
         auto g = get_gather(rt, *a.st, cp.shift1, cp.p1, *pb.st);
         if(g->dev) {
+            bool rid = cp.shiftres == 0;
+            for(std::size_t i = 0; rid && i < cp.pres.size(); ++i) rid = cp.pres[i] == static_cast<int>(i);
+            if(!rid && rt->fuse_permute >= 2) {
+                const CachedGather::Scatter& sc = get_scatter(rt, *g, cp.shiftres, cp.pres);
+                if(sc.dev) {
+                    DTensor out = DTensor::empty(rt, sc.out);
+                    launch_gather_scatter(g.get(), sc, a, pb, out);
+                    return out;
+                }
+            }
             DTensor prod = DTensor::empty(rt, g->gemm->plan.out);
             launch_gather(g.get(), a, pb, prod);
             return permute(prod, cp.shiftres, cp.pres);

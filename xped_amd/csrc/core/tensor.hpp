@@ -62,6 +62,15 @@ struct CachedGather {
     std::shared_ptr<CachedGemm> gemm;
     xh_gather_plan* dev = nullptr;
     std::uint64_t last_use = 0;
+    // the product's own permute folded into the same launch (xh_gemm_gather_scatter), per
+    // (shift, p) of that permute: the permuted product's structure and the device plan (null when
+    // the permute cannot be folded; contract() then permutes the product)
+    struct Scatter {
+        StructPtr out;
+        xh_scatter_plan* dev = nullptr;
+    };
+    std::unordered_map<std::string, Scatter> scatter;
+    void release(xh_ctx* ctx);  // device plans, scatter plans first
 };
 
 struct Tape;
@@ -77,9 +86,10 @@ struct Runtime {
     long n_symbolic_hits = 0;
     std::unordered_map<std::string, std::shared_ptr<CachedGemm>> gemm_cache;
     std::unordered_map<std::string, std::shared_ptr<CachedGather>> gather_cache;
-    // contract(): read the left operand through its permute inside skinny FP64 products instead of
-    // materialising the permuted tensor (on by default; off gives the same results bit for bit)
-    bool fuse_permute = true;
+    // contract(): 1 (default) reads the left operand through its permute inside skinny FP64
+    // products instead of materialising the permuted tensor; 2 also writes the product straight
+    // into its own permuted layout; 0 materialises both.  All give the same results bit for bit.
+    int fuse_permute = 1;
     long n_fused = 0;
     std::unordered_map<std::string, StructPtr> struct_cache;
     // counters for reporting

@@ -242,13 +242,12 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_f64p(GemmArgs g)
 constexpr int SK_MAX = 32, SK_KMAX = 32, SK_THREADS = 256;
 
 // The S <= MS sums of one long-side index against the small operand in LDS (sS, [kk][s]) and
-// their stores (output s at base + s * stride), in passes of at most 16 outputs: the K loaded
+// hands output s to store(s, sum), in passes of at most 16 outputs: the K loaded
 // elements stay in registers and only 16 accumulators are live (the gather kernel: 145 instead of
 // 169 VGPRs at MS = 32, three waves per SIMD instead of two).  Every sum runs over kk in order, as
 // one pass would.
-template <int MS>
-__device__ __forceinline__ void skinny_sums(const double* __restrict__ sS, const double (&x)[SK_KMAX], int K, int S, double* base, int64_t stride, double alpha,
-                                            double beta)
+template <int MS, class Store>
+__device__ __forceinline__ void skinny_sums(const double* __restrict__ sS, const double (&x)[SK_KMAX], int K, int S, Store&& store)
 {
     constexpr int MH = MS > 16 ? 16 : MS;
 #pragma unroll
@@ -264,10 +263,7 @@ __device__ __forceinline__ void skinny_sums(const double* __restrict__ sS, const
                 for(int c = 0; c < MH; ++c) acc[c] = __builtin_fma(sS[kk * MS + h + c], x[kk], acc[c]);
 #pragma unroll
         for(int c = 0; c < MH; ++c)
-            if(h + c < S) {
-                double* p = base + int64_t(h + c) * stride;
-                *p = (beta == 0.0) ? alpha * acc[c] : alpha * acc[c] + beta * *p;
-            }
+            if(h + c < S) store(h + c, acc[c]);
     }
 }
 
@@ -372,9 +368,32 @@ struct GatherDev {
     int32_t nlegs, new_rank, nblocks;
 };
 
-template <int MS>
-__global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, GatherDev gd, const double* __restrict__ src)
+// The permute of the product folded into the stores (SCATTER): every product subblock (domain tree
+// a, codomain tree c) is the single source of one task of that permute, which fixes its place in
+// the output.  Per (block tree, product column) the block tabulates the column's destination
+// offset and the destination strides of the product's domain legs, which are the permuted left
+// operand's domain legs, so the row's indices are the ones decoded for the gather.
+struct ScatterItem {
+    int64_t grid_off, cols_off;  // product subblock (a, c) -> task grid[grid_off + a * ncod + c]; columns as GatherItem's
+    int32_t ncod, pad;
+};
+struct ScatterDev {
+    const ScatterItem* items;  // per GEMM item
+    const int32_t* grid;
+    const int32_t* cols;       // per product column: codomain tree | (column inside it << 16)
+    const xh::LegInfo* legs;   // per task and product leg j: extent, division and destination stride of leg j
+    const int64_t* dst_off;    // per task: offset of its output subblock
+    const int32_t* neg;        // per task: 1 for coefficient -1
+    int32_t nlegs;             // product legs (its domain legs first)
+};
+
+template <int MS, bool SCATTER>
+__global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, GatherDev gd, const double* __restrict__ src, ScatterDev sd)
 {
+    constexpr int DS = SCATTER ? GA_SEG : 1, DC = SCATTER ? SK_MAX : 1, DR = SCATTER ? GA_MAXR : 1;
+    __shared__ int64_t s_doff[DS][DC];      // per tree and product column: destination offset of the column
+    __shared__ int64_t s_drs[DS][DC][DR];   // per tree and product column: destination strides of the domain legs
+    __shared__ uint32_t s_dneg[DS];         // bit c: the column's task has coefficient -1
     __shared__ double sS[SK_KMAX * MS];                     // B, [kk][s]
     __shared__ int64_t s_off[GA_SEG][SK_KMAX];              // per tree and column: the column's offset
     __shared__ int64_t s_rs[GA_SEG][SK_KMAX][GA_MAXR];      // per tree and column: the domain legs' strides
@@ -394,6 +413,8 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, G
             sS[kk * MS + sx] = v;
         }
         if(int(threadIdx.x) < GA_SEG) s_neg[threadIdx.x] = 0;
+        if constexpr(SCATTER)
+            if(int(threadIdx.x) < GA_SEG) s_dneg[threadIdx.x] = 0;
         __syncthreads();
         for(int e = threadIdx.x; e < bl.nseg * K; e += SK_THREADS) {
             const int sg = e / K, kk = e % K;
@@ -418,6 +439,27 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, G
                 if(kk == 0) s_rleg[sg][k] = li;
             }
             if(so.neg) atomicOr(&s_neg[sg], 1u << kk);
+        }
+        if constexpr(SCATTER) {
+            const ScatterItem si = sd.items[bl.item];
+            const int NP = sd.nlegs;
+            for(int e = threadIdx.x; e < bl.nseg * S; e += SK_THREADS) {
+                const int sg = e / S, c = e % S;
+                const int32_t cb = sd.cols[si.cols_off + c];
+                const int t = sd.grid[si.grid_off + int64_t(gd.blocks[blk].a[sg]) * si.ncod + (cb & 0xffff)];
+                const xh::LegInfo* Lg = sd.legs + int64_t(t) * NP;
+                uint32_t x = static_cast<uint32_t>(cb >> 16);
+                int64_t off = sd.dst_off[t];
+                for(int k = R; k < NP; ++k) {
+                    const xh::LegInfo li = Lg[k];
+                    const uint32_t q = xh::fdiv(x, li.magic, xh::leg_shift(li));
+                    off += int64_t(x - q * li.ext) * li.stride;
+                    x = q;
+                }
+                s_doff[sg][c] = off;
+                for(int k = 0; k < R; ++k) s_drs[sg][c][k] = Lg[k].stride;
+                if(sd.neg[t]) atomicOr(&s_dneg[sg], 1u << c);
+            }
         }
         __syncthreads();
         if(int(threadIdx.x) >= bl.nrows) continue;  // no barrier follows before the next block's first one
@@ -458,7 +500,27 @@ __global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, G
 #pragma unroll
         for(int kk = 0; kk < SK_KMAX; ++kk)
             if((negm >> kk) & 1u) xv[kk] = -xv[kk];
-        skinny_sums<MS>(sS, xv, K, S, g.C + it.c_off + L, it.m, g.alpha, g.beta);
+        if constexpr(!SCATTER) {
+            double* base = g.C + it.c_off + L;
+            const int64_t ld = it.m;
+            skinny_sums<MS>(sS, xv, K, S, [&](int c, double a) {
+                double* p = base + int64_t(c) * ld;
+                *p = (g.beta == 0.0) ? g.alpha * a : g.alpha * a + g.beta * *p;
+            });
+        } else {
+            // each output straight to its place in the permuted product: the column's offset plus
+            // (domain-leg index x destination stride) over the row's legs; 0 + c v with c = +-1,
+            // as the permute writes it (beta is 0)
+            const uint32_t dneg = s_dneg[sg];
+            skinny_sums<MS>(sS, xv, K, S, [&](int c, double a) {
+                int64_t off = s_doff[sg][c];
+#pragma unroll
+                for(int k = 0; k < GA_MAXR; ++k)
+                    if(k < R) off += int64_t(dig[k]) * s_drs[sg][c][k];
+                const double v = g.alpha * a;
+                g.C[off] = ((dneg >> c) & 1u) ? 0.0 - v : 0.0 + v;
+            });
+        }
     }
 }
 
@@ -1475,6 +1537,18 @@ struct xh_gather_plan : xh::PlanLife {
     GatherDev dev{};
     const xh_gemm_plan* gemm = nullptr;
     int64_t src_elems = 0;
+    // host copies for the checks of a scatter plan: per item its domain tree starts (offset in
+    // h_dstarts, ndom + 1 entries) and per tree the domain legs' extents (offset in h_dext)
+    std::vector<int64_t> h_dstarts_off, h_dext_off;
+    std::vector<int32_t> h_ndom, h_dstarts;
+    std::vector<uint32_t> h_dext;
+};
+
+struct xh_scatter_plan : xh::PlanLife {
+    xh::PackedUpload up;
+    ScatterDev dev{};
+    const xh_gather_plan* gather = nullptr;
+    int64_t dst_elems = 0;
 };
 
 namespace {
@@ -1561,6 +1635,20 @@ extern "C" int xh_gather_plan_create(xh_ctx* ctx, const xh_gemm_plan* gemm, cons
                 }
             }
         });
+        plan->h_ndom.assign(size_t(d->nitems), 0);
+        plan->h_dstarts_off.assign(size_t(d->nitems), 0);
+        plan->h_dext_off.assign(size_t(d->nitems), 0);
+        for(int64_t j = 0; j < d->nitems; ++j) {
+            const xh_gemm_item& it = gemm->h_items[j];
+            const xh_gather_item& g = d->items[j];
+            if(it.m == 0 || it.n == 0) continue;
+            plan->h_ndom[j] = g.ndom;
+            plan->h_dstarts_off[j] = int64_t(plan->h_dstarts.size());
+            plan->h_dext_off[j] = int64_t(plan->h_dext.size());
+            plan->h_dstarts.insert(plan->h_dstarts.end(), d->starts + g.starts_off, d->starts + g.starts_off + g.ndom + 1);
+            for(int a = 0; a < g.ndom; ++a)
+                for(int k = 0; k < NR; ++k) plan->h_dext.push_back(pd.tasks[g.task_base + int64_t(a) * g.ncod].ext[k]);
+        }
         // blocks of at most SK_THREADS rows meeting at most GA_SEG domain trees, in the GEMM plan's
         // item order; per item the codomain tree and in-tree column of every column
         std::vector<GatherBlock> blocks;
@@ -1654,9 +1742,169 @@ extern "C" int xh_gemm_gather(xh_ctx* ctx, const xh_gemm_plan* plan, const xh_ga
         const bool timed = ctx->gemm_timing;
         if(timed) xh::gemm_timer_begin(ctx);
         GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, nullptr, B, C, nullptr, alpha, beta};
-        auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny_gather<8> : plan->sk_ms <= 16 ? k_gemm_skinny_gather<16> : k_gemm_skinny_gather<32>;
+        auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny_gather<8, false> : plan->sk_ms <= 16 ? k_gemm_skinny_gather<16, false> : k_gemm_skinny_gather<32, false>;
         const int grid = std::min<int>(gp->dev.nblocks, ctx->num_cu * 8);
-        if(grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, gp->dev, src);
+        if(grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, gp->dev, src, ScatterDev{});
+        xh::launch_check("k_gemm_skinny_gather");
+        if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(ctx, plan));
+    });
+}
+
+// ---------------------------------------------------------------- the product's permute folded into the gather GEMM's stores
+
+namespace {
+void scatter_plan_release(xh_ctx* ctx, xh::PlanLife* p)
+{
+    auto* plan = static_cast<xh_scatter_plan*>(p);
+    plan->up.release(ctx);
+    delete plan;
+}
+}  // namespace
+
+extern "C" int xh_scatter_plan_create(xh_ctx* ctx, const xh_gather_plan* gather, const xh_scatter_desc* d, xh_scatter_plan** out)
+{
+    *out = nullptr;
+    auto* plan = new xh_scatter_plan();
+    const int rc = guard(ctx, [&] {
+        xh::refuse_under_capture(ctx, "plan creation");
+        if(!gather || !d || !d->p) throw xh::Error(XH_EINVAL, "scatter plan: null argument");
+        const xh_gemm_plan* gemm = gather->gemm;
+        const xh_permute_desc& pd = d->perm;
+        const int NP = pd.nlegs, NR2 = pd.new_rank, R = gather->dev.new_rank;
+        if(NP < 1 || NP > XH_MAXLEGS || NR2 < 0 || NR2 > NP || R > NP) throw xh::Error(XH_EINVAL, "scatter plan: invalid leg counts");
+        std::vector<int> pinv(size_t(NP), -1);
+        for(int k = 0; k < NP; ++k) {
+            if(d->p[k] < 0 || d->p[k] >= NP || pinv[size_t(d->p[k])] >= 0) throw xh::Error(XH_EINVAL, "scatter plan: p is not a permutation");
+            pinv[size_t(d->p[k])] = k;
+        }
+        const int64_t NT = pd.ntasks;
+        if(NT < 0 || NT > 0x7fffffff || pd.nsources < 0) throw xh::Error(XH_EINVAL, "scatter plan: invalid task or source count");
+        if(d->nitems != static_cast<int64_t>(gemm->h_items.size())) throw xh::Error(XH_EINVAL, "scatter plan: one entry per GEMM item");
+        if(d->ngrid < 0 || d->ncstarts < 0 || d->dst_elems < 0) throw xh::Error(XH_EINVAL, "scatter plan: invalid sizes");
+        // per task: one source, coefficient +-1; its destination strides by product leg, and every
+        // destination element inside dst
+        std::vector<xh::LegInfo> legs(static_cast<size_t>(NT * NP));
+        std::vector<int64_t> dst_off(static_cast<size_t>(NT));
+        std::vector<int32_t> neg(static_cast<size_t>(NT));
+        xp::parallel_ranges(size_t(NT), 4096, [&](size_t t0, size_t t1) {
+            for(size_t t = t0; t < t1; ++t) {
+                const xh_permute_task& tk = pd.tasks[t];
+                if(tk.nsrc != 1 || tk.src_begin < 0 || tk.src_begin >= pd.nsources) throw xh::Error(XH_EINVAL, "scatter plan: every task needs exactly one source");
+                const double c = pd.sources[tk.src_begin].coeff;
+                if(c != 1.0 && c != -1.0) throw xh::Error(XH_EINVAL, "scatter plan: source coefficients must be +1 or -1");
+                if(tk.dst_off < 0 || tk.dst_ld < 1) throw xh::Error(XH_EINVAL, "scatter plan: invalid destination subblock");
+                int64_t rs = 1, cs = tk.dst_ld, dmax = tk.dst_off;
+                for(int k = 0; k < NP; ++k) {
+                    if(tk.ext[k] == 0) throw xh::Error(XH_EINVAL, "scatter plan: zero extent");
+                    int64_t& st = k < NR2 ? rs : cs;
+                    if(st >= (int64_t(1) << 31)) throw xh::Error(XH_EINVAL, "scatter plan: destination stride out of range");
+                    const int j = d->p[k];  // output leg k is product leg j
+                    legs[t * NP + size_t(j)] = xh::make_leg(tk.ext[k], st, false);
+                    dmax += (int64_t(tk.ext[k]) - 1) * st;
+                    st *= tk.ext[k];
+                }
+                if(dmax >= d->dst_elems || dmax >= (int64_t(1) << 62)) throw xh::Error(XH_EINVAL, "scatter plan: destination subblock out of bounds");
+                dst_off[t] = tk.dst_off;
+                neg[t] = c < 0.0 ? 1 : 0;
+            }
+        });
+        // per item: its task grid covers the product block's subblocks (a, c), each task reads
+        // exactly the subblock it is listed for, with the domain-leg extents the gather decodes
+        std::vector<int32_t> seen(static_cast<size_t>(NT), 0);
+        std::vector<ScatterItem> items(size_t(d->nitems), ScatterItem{0, 0, 0, 0});
+        std::vector<int32_t> cols;
+        for(int64_t j = 0; j < d->nitems; ++j) {
+            const xh_gemm_item& it = gemm->h_items[j];
+            if(it.m == 0 || it.n == 0) continue;
+            const int ndom = gather->h_ndom[j], ncod = d->ncod[j];
+            const int32_t* ds = gather->h_dstarts.data() + gather->h_dstarts_off[j];
+            const uint32_t* dext = gather->h_dext.data() + gather->h_dext_off[j];
+            if(ncod < 1 || ncod > 0xffff || d->grid_off[j] < 0 || d->grid_off[j] + int64_t(ndom) * ncod > d->ngrid || d->cstarts_off[j] < 0 ||
+               d->cstarts_off[j] + ncod + 1 > d->ncstarts)
+                throw xh::Error(XH_EINVAL, "scatter plan: item tables out of range");
+            const int32_t* cs = d->cstarts + d->cstarts_off[j];
+            if(cs[0] != 0 || cs[ncod] != it.n) throw xh::Error(XH_EINVAL, "scatter plan: column starts do not cover the product block");
+            for(int a = 0; a < ndom; ++a)
+                for(int c = 0; c < ncod; ++c) {
+                    const int32_t t = d->grid[d->grid_off[j] + int64_t(a) * ncod + c];
+                    if(t < 0 || t >= NT || seen[size_t(t)]++) throw xh::Error(XH_EINVAL, "scatter plan: every task must appear once in the grids");
+                    const xh_permute_task& tk = pd.tasks[t];
+                    const xh_permute_source& so = pd.sources[tk.src_begin];
+                    if(so.src_off != it.c_off + ds[a] + int64_t(cs[c]) * it.m || so.src_ld != it.m)
+                        throw xh::Error(XH_EINVAL, "scatter plan: a task does not read the product subblock it is listed for");
+                    int64_t rp = 1, cp = 1;
+                    for(int jj = 0; jj < NP; ++jj) {
+                        const uint32_t e = tk.ext[pinv[size_t(jj)]];
+                        if(jj < R) {
+                            if(e != dext[int64_t(a) * R + jj]) throw xh::Error(XH_EINVAL, "scatter plan: domain-leg extents differ from the gather's");
+                            rp *= e;
+                        } else {
+                            cp *= e;
+                        }
+                    }
+                    if(rp != ds[a + 1] - ds[a] || cp != cs[c + 1] - cs[c]) throw xh::Error(XH_EINVAL, "scatter plan: task shape does not match its product subblock");
+                }
+            items[j] = ScatterItem{d->grid_off[j], int64_t(cols.size()), ncod, 0};
+            for(int c = 0; c < ncod; ++c)
+                for(int x = cs[c]; x < cs[c + 1]; ++x) cols.push_back(c | ((x - cs[c]) << 16));
+        }
+        for(int64_t t = 0; t < NT; ++t)
+            if(!seen[size_t(t)]) throw xh::Error(XH_EINVAL, "scatter plan: a task is in no grid");
+        xh::PackedUpload& up = plan->up;
+        const size_t bytes = xh::PackedUpload::packed_bytes({items.size() * sizeof(ScatterItem), size_t(d->ngrid) * sizeof(int32_t), cols.size() * sizeof(int32_t),
+                                                             legs.size() * sizeof(xh::LegInfo), dst_off.size() * sizeof(int64_t), neg.size() * sizeof(int32_t)});
+        if(bytes >= (size_t(1) << 18)) up.begin_pinned(ctx, bytes);
+        auto put = [&up](const auto* ptr, size_t n) {
+            using T = std::decay_t<decltype(*ptr)>;
+            const size_t o = up.reserve<T>(static_cast<int64_t>(n));
+            if(n > 0) std::memcpy(up.host_ptr<T>(o), ptr, n * sizeof(T));
+            return o;
+        };
+        const size_t o_it = put(items.data(), items.size()), o_gr = put(d->grid, size_t(d->ngrid)), o_co = put(cols.data(), cols.size()),
+                     o_lg = put(legs.data(), legs.size()), o_do = put(dst_off.data(), dst_off.size()), o_ng = put(neg.data(), neg.size());
+        up.upload(ctx);
+        plan->dev = ScatterDev{up.ptr<ScatterItem>(o_it), up.ptr<int32_t>(o_gr), up.ptr<int32_t>(o_co), up.ptr<xh::LegInfo>(o_lg), up.ptr<int64_t>(o_do),
+                               up.ptr<int32_t>(o_ng), NP};
+        plan->gather = gather;
+        plan->dst_elems = d->dst_elems;
+    });
+    if(rc != XH_OK) {
+        scatter_plan_release(ctx, plan);
+        return rc;
+    }
+    *out = plan;
+    return XH_OK;
+}
+
+extern "C" int xh_scatter_plan_destroy(xh_ctx* ctx, xh_scatter_plan* plan)
+{
+    if(!plan) return XH_OK;
+    return guard(ctx, [&] {
+        if(plan->pins > 0) {
+            plan->dead = true;  // a live graph reads its tables: freed by the last xh_graph_destroy
+            return;
+        }
+        scatter_plan_release(ctx, plan);
+    });
+}
+
+extern "C" int xh_gemm_gather_scatter(xh_ctx* ctx, const xh_gemm_plan* plan, const xh_gather_plan* gp, const xh_scatter_plan* sp, const double* src,
+                                      int64_t src_elems, const double* B, double* dst, int64_t dst_elems, double alpha)
+{
+    return guard(ctx, [&] {
+        if(!plan || !gp || !sp) throw xh::Error(XH_EINVAL, "xh_gemm_gather_scatter: null plan");
+        if(gp->gemm != plan || sp->gather != gp) throw xh::Error(XH_EINVAL, "xh_gemm_gather_scatter: the plans were built for one another's others");
+        if(plan->dead || gp->dead || sp->dead) throw xh::Error(XH_EINVAL, "xh_gemm_gather_scatter: plan used after destroy");
+        if(src_elems != gp->src_elems || dst_elems != sp->dst_elems) throw xh::Error(XH_EINVAL, "xh_gemm_gather_scatter: buffer sizes differ from the plans'");
+        xh::pin_for_capture(ctx, const_cast<xh_gemm_plan*>(plan), gemm_plan_release);
+        xh::pin_for_capture(ctx, const_cast<xh_gather_plan*>(gp), gather_plan_release);
+        xh::pin_for_capture(ctx, const_cast<xh_scatter_plan*>(sp), scatter_plan_release);
+        const bool timed = ctx->gemm_timing;
+        if(timed) xh::gemm_timer_begin(ctx);
+        GemmArgs g{plan->d_items, plan->d_ext, nullptr, {}, plan->ta, plan->tb, nullptr, B, dst, nullptr, alpha, 0.0};
+        auto* kern = plan->sk_ms <= 8 ? k_gemm_skinny_gather<8, true> : plan->sk_ms <= 16 ? k_gemm_skinny_gather<16, true> : k_gemm_skinny_gather<32, true>;
+        const int grid = std::min<int>(gp->dev.nblocks, ctx->num_cu * 8);
+        if(grid > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(SK_THREADS), 0, ctx->stream, g, gp->dev, src, sp->dev);
         xh::launch_check("k_gemm_skinny_gather");
         if(timed) xh::gemm_timer_end(ctx, plan->flops, launch_items(ctx, plan));
     });

@@ -151,10 +151,11 @@ class Runtime:
 
     def fuse_permute(self, enable=None):
         """contract() with the left operand's permute read inside skinny FP64 products (the gather
-        GEMM, on by default; bit-identical either way).  enable: True / False / None (keep).
+        GEMM, on by default; bit-identical either way).  enable: True / 1 (gather), 2 (gather and
+        the product's own permute folded into its stores), False / 0 (off), None (keep).
         Returns the number of contractions run fused so far."""
         n = C.c_long()
-        e = -1 if enable is None else (1 if enable else 0)
+        e = -1 if enable is None else (2 if enable == 2 else (1 if enable else 0))
         check(lib().xt_rt_set_fuse_permute(self.h, e, C.byref(n)), self.h, "fuse_permute")
         return n.value
 
