@@ -129,7 +129,7 @@ int xh_gemm_noalloc(xh_ctx* ctx, const xh_gemm_plan* plan, const double* A, cons
  *     task_base + a * ncod + b (domain tree a < ndom, codomain tree b < ncod); starts[starts_off ..]
  *     holds the ndom + 1 domain-tree row starts, then the ncod + 1 codomain-tree column starts;
  *   - src_elems: size of src in doubles (every source access is checked against it here).
- * Only plans whose every item is a skinny FP64 item with A the long operand (k <= 32, n <= 32 < m)
+ * Only plans whose every item is a skinny FP64 item with A the long operand (k <= 32, n <= 32, n < m)
  * and no transposes qualify (xh_gemm_gather_eligible); the result equals
  * xh_permute + xh_gemm(XH_FP64) bit for bit. */
 typedef struct {

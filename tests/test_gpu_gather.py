@@ -2,7 +2,7 @@
 
 contract(a, a1, b, a2) permutes a and b, multiplies, and permutes the product (Tensor.hpp:328-351).
 When every sector block of the product is a skinny FP64 item with the permuted a on its long side
-(k <= 32, n <= 32 < m) and the permute of a is single-source with coefficients +-1 (U(1),
+(k <= 32, n <= 32, n < m) and the permute of a is single-source with coefficients +-1 (U(1),
 fermionic U(1), U(1)xU(1)), the product reads a through the permute's tasks instead of
 materialising the permuted copy (xh_gemm_gather).  Checks:
 
