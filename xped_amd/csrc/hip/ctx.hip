@@ -667,19 +667,46 @@ __global__ void k_add_at(double* __restrict__ x, int64_t i, double v)
     if(threadIdx.x == 0) x[i] += v;
 }
 
+// Rectangles in tiles of 32 x 32, blockIdx.y the rectangle, the tiles spread over blockIdx.x:
+// straight copies read and write along the rectangle's columns; transposed ones stage the tile in
+// LDS so that both the reads (along the source's columns) and the writes are contiguous.
+constexpr int CR_TILE = 32, CR_THREADS = 256;
 template <bool ACC>
-__global__ void k_copy_rects(const double* __restrict__ src, double* __restrict__ dst, const xh_copy_rect* __restrict__ rects)
+__global__ __launch_bounds__(CR_THREADS) void k_copy_rects(const double* __restrict__ src, double* __restrict__ dst, const xh_copy_rect* __restrict__ rects)
 {
-    const xh_copy_rect r = rects[blockIdx.x];
-    const int64_t n = int64_t(r.rows) * r.cols;
-    for(int64_t e = threadIdx.x; e < n; e += blockDim.x) {
-        int64_t i = e % r.rows, j = e / r.rows;
-        double v = r.transpose ? src[r.src_off + j + i * r.src_ld] : src[r.src_off + i + j * r.src_ld];
+    __shared__ double tile[CR_TILE][CR_TILE + 1];
+    const xh_copy_rect r = rects[blockIdx.y];
+    const int ntr = (r.rows + CR_TILE - 1) / CR_TILE, ntc = (r.cols + CR_TILE - 1) / CR_TILE;
+    const int tx = threadIdx.x % CR_TILE, ty = threadIdx.x / CR_TILE;
+    constexpr int STEP = CR_THREADS / CR_TILE;
+    auto put = [&](int i, int j, double v) {
         v = r.factor == 1.0 ? v : r.factor * v;
+        double* p = dst + r.dst_off + i + int64_t(j) * r.dst_ld;
         if constexpr(ACC)
-            dst[r.dst_off + i + j * r.dst_ld] += v;
+            *p += v;
         else
-            dst[r.dst_off + i + j * r.dst_ld] = v;
+            *p = v;
+    };
+    for(int t = blockIdx.x; t < ntr * ntc; t += gridDim.x) {
+        const int i0 = (t % ntr) * CR_TILE, j0 = (t / ntr) * CR_TILE;
+        if(!r.transpose) {
+            for(int q = ty; q < CR_TILE; q += STEP) {
+                const int i = i0 + tx, j = j0 + q;
+                if(i < r.rows && j < r.cols) put(i, j, src[r.src_off + i + int64_t(j) * r.src_ld]);
+            }
+        } else {
+            // dst (i, j) = src (j, i); the source is cols x rows
+            __syncthreads();  // the previous tile's reads of the LDS tile are done
+            for(int q = ty; q < CR_TILE; q += STEP) {
+                const int j = j0 + tx, i = i0 + q;
+                tile[q][tx] = (i < r.rows && j < r.cols) ? src[r.src_off + j + int64_t(i) * r.src_ld] : 0.0;
+            }
+            __syncthreads();
+            for(int q = ty; q < CR_TILE; q += STEP) {
+                const int i = i0 + tx, j = j0 + q;
+                if(i < r.rows && j < r.cols) put(i, j, tile[tx][q]);
+            }
+        }
     }
 }
 
@@ -958,11 +985,19 @@ static int copy_rects_impl(bool acc, xh_ctx* ctx, const double* src, double* dst
 {
     return guard(ctx, [&] {
         if(nrects <= 0) return;
+        if(nrects > 65535) throw xh::Error(XH_EINVAL, "copy_rects: at most 65535 rectangles per call");
         std::vector<xh_copy_rect> r(rects, rects + nrects);
+        int64_t tiles = 0;
+        for(const auto& x : r) {
+            if(x.rows < 0 || x.cols < 0 || x.src_ld < 0 || x.dst_ld < 0) throw xh::Error(XH_EINVAL, "copy_rects: negative extent or leading dimension");
+            tiles = std::max(tiles, int64_t((x.rows + CR_TILE - 1) / CR_TILE) * ((x.cols + CR_TILE - 1) / CR_TILE));
+        }
+        if(tiles == 0) return;
         xh::PackedUpload up;
         const size_t o = up.add(r);
         up.upload(ctx);
-        hipLaunchKernelGGL(acc ? k_copy_rects<true> : k_copy_rects<false>, dim3(static_cast<unsigned>(nrects)), dim3(EW_THREADS), 0, ctx->stream, src, dst, up.ptr<const xh_copy_rect>(o));
+        const dim3 grid(static_cast<unsigned>(std::min<int64_t>(tiles, 64)), static_cast<unsigned>(nrects));
+        hipLaunchKernelGGL(acc ? k_copy_rects<true> : k_copy_rects<false>, grid, dim3(CR_THREADS), 0, ctx->stream, src, dst, up.ptr<const xh_copy_rect>(o));
         xh::launch_check("k_copy_rects");
         up.release_deferred(ctx);
     });
