@@ -168,6 +168,7 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     c0 = rt.counters()
     h0 = rt.host_stats()
     f0 = rt.fuse_permute(None)
+    d0 = rt.device_cache()
     t0 = time.perf_counter()
     for k in range(nsteps):
         energies.append(c.step())
@@ -177,6 +178,7 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     c1 = rt.counters()
     h1 = rt.host_stats()
     f1 = rt.fuse_permute(None)
+    d1 = rt.device_cache()
     if spectra is None:  # no warm-up step: the first timed step recorded (16 decompositions per step)
         spectra = c.spectra()[:16]
         configure(False)
@@ -188,7 +190,9 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
            "predictor_features_previous_sketch": list(c.predictor_stats()),
            "gemm_gflop_per_step": (c1["gemm_flops"] - c0["gemm_flops"]) / nsteps / 1e9,
            "permute_gb_per_step": (c1["permute_bytes"] - c0["permute_bytes"]) / nsteps / 1e9, "rsvd": c.rsvd_stats(), "host": rt.host_stats(),
-           "fused_contractions_per_step": (f1 - f0) / nsteps}
+           "fused_contractions_per_step": (f1 - f0) / nsteps,
+           # device allocations per step served by the context's block cache / passed to the HIP runtime
+           "device_allocs_cached_missed_per_step": [(d1["hits"] - d0["hits"]) / nsteps, (d1["misses"] - d0["misses"]) / nsteps]}
     if mode == "fast":
         # one more step with every GEMM and permute launch timed by HIP events on the context stream; marker
         # kernels of 1 and 2 workgroups delimit it in a kernel trace, and inside it markers of 3 and 4
@@ -310,6 +314,7 @@ def run_c4(args):
                  "gemm_gflop_per_step": round(fast["gemm_gflop_per_step"], 3), "permute_gb_per_step": round(fast["permute_gb_per_step"], 3),
                  "rsvd_calls_fallbacks_ratio": list(fast["rsvd"]), "energies": fast["energies"],
                  "fused_contractions_per_step": fast["fused_contractions_per_step"],
+                 "device_allocs_cached_missed_per_step": fast["device_allocs_cached_missed_per_step"],
                  "predictor_features_previous_sketch": fast["predictor_features_previous_sketch"],
                  **({"autotune_entries_runs_hits_chose64": fast["autotune"]} if "autotune" in fast else {}),
                  "host_plan_ms_per_step": round(fast["host_plan_ms_per_step"], 1), "plans_per_step": fast["plans_per_step"],

@@ -48,6 +48,15 @@ const char* xh_last_error(const xh_ctx* ctx);
 int xh_device_count(int* count);
 int xh_alloc(xh_ctx* ctx, size_t bytes, void** dptr);
 int xh_free(xh_ctx* ctx, void* dptr);
+/* Device blocks released by xh_free (and the library's own workspaces and plan tables) are kept
+ * per context in size classes (four per power of two) and handed out again, in stream order, by
+ * later allocations of the same class; idle blocks beyond `bytes` (default 4 GiB; 0 keeps none)
+ * go back to the HIP runtime, as do all of them when an allocation fails or the context is
+ * destroyed.  No counterpart in the reference, whose storage is host memory
+ * (StorageType_contiguous.hpp:91-96). */
+int xh_set_device_cache(xh_ctx* ctx, size_t bytes);
+/* allocations served by the cache, allocations that reached the runtime, bytes held idle */
+int xh_device_cache_stats(xh_ctx* ctx, int64_t* hits, int64_t* misses, size_t* idle_bytes);
 int xh_upload(xh_ctx* ctx, void* dst, const void* src, size_t bytes);   /* host -> device, synchronous */
 int xh_download(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device -> host, synchronous */
 int xh_copy(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device to device, async on the stream */

@@ -67,6 +67,8 @@ _SIGS = [
     ("xh_device_count", C.c_int, [_IP]),
     ("xh_alloc", C.c_int, [_P, _SZ, _PP]),
     ("xh_free", C.c_int, [_P, _P]),
+    ("xh_set_device_cache", C.c_int, [_P, _SZ]),
+    ("xh_device_cache_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(_SZ)]),
     ("xh_upload", C.c_int, [_P, _P, _P, _SZ]),
     ("xh_download", C.c_int, [_P, _P, _P, _SZ]),
     ("xh_copy", C.c_int, [_P, _P, _P, _SZ]),

@@ -132,7 +132,7 @@ extern "C" int xh_svd_bwd(xh_ctx* ctx, const xh_svd_bwd_item* items, int64_t nit
         if(act.empty()) return;
         if(max_e >= (int64_t(1) << 40)) throw xh::Error(XH_EINVAL, "svd adjoint: block too large");
         double *X = nullptr, *Y = nullptr, *M = nullptr, *W = nullptr, *Z = nullptr;
-        auto alloc = [&](double** p, int64_t n) { check(hipMallocAsync(reinterpret_cast<void**>(p), size_t(std::max<int64_t>(n, 1)) * sizeof(double), ctx->stream), "hipMallocAsync"); };
+        auto alloc = [&](double** p, int64_t n) { check(xh::dmalloc(ctx, reinterpret_cast<void**>(p), size_t(std::max<int64_t>(n, 1)) * sizeof(double)), "hipMallocAsync"); };
         struct Free {
             xh_ctx* c;
             double** p[5];
@@ -257,7 +257,7 @@ extern "C" int xh_qr_bwd(xh_ctx* ctx, const xh_qr_bwd_item* items, int64_t nitem
         }
         if(act.empty()) return;
         double* w = nullptr;
-        check(hipMallocAsync(reinterpret_cast<void**>(&w), size_t(work) * sizeof(double), ctx->stream), "hipMallocAsync(qr adjoint)");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&w), size_t(work) * sizeof(double)), "hipMallocAsync(qr adjoint)");
         QbItem* d_items = nullptr;
         try {
             d_items = xh::dev_upload(ctx, act);

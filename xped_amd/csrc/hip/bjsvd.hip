@@ -340,9 +340,9 @@ void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, con
     };
     PlanSet ps{ctx};
     try {
-        check(hipMallocAsync(reinterpret_cast<void**>(&buf[0]), size_t(nw) * sizeof(double), ctx->stream), "hipMallocAsync");
-        check(hipMallocAsync(reinterpret_cast<void**>(&buf[1]), size_t(nw) * sizeof(double), ctx->stream), "hipMallocAsync");
-        check(hipMallocAsync(reinterpret_cast<void**>(&d_flags), sizeof(int) * secs.size(), ctx->stream), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&buf[0]), size_t(nw) * sizeof(double)), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&buf[1]), size_t(nw) * sizeof(double)), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&d_flags), sizeof(int) * secs.size()), "hipMallocAsync");
         // W = A (or A^T) in the layout of step 0, V = the matching column permutation of I
         std::vector<xh_copy_rect> rects;
         std::vector<int64_t> ones;
@@ -396,8 +396,8 @@ void block_jacobi_svd(xh_ctx* ctx, const std::vector<xh_factor_item>& items, con
                 if(gsize > gcap) {
                     xh::dev_free(ctx, G);
                     xh::dev_free(ctx, Jm);
-                    check(hipMallocAsync(reinterpret_cast<void**>(&G), size_t(gsize) * sizeof(double), ctx->stream), "hipMallocAsync");
-                    check(hipMallocAsync(reinterpret_cast<void**>(&Jm), size_t(gsize) * sizeof(double), ctx->stream), "hipMallocAsync");
+                    check(xh::dmalloc(ctx, reinterpret_cast<void**>(&G), size_t(gsize) * sizeof(double)), "hipMallocAsync");
+                    check(xh::dmalloc(ctx, reinterpret_cast<void**>(&Jm), size_t(gsize) * sizeof(double)), "hipMallocAsync");
                     gcap = gsize;
                 }
                 rebuild = false;

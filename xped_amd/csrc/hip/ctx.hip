@@ -56,6 +56,8 @@ extern "C" int xh_destroy(xh_ctx* ctx)
     if(ctx->stream) hipStreamSynchronize(ctx->stream);
     xh::collect_staging(ctx, true);
     xh::pinned_destroy_all(ctx);
+    xh::dcache_release(ctx);
+    if(ctx->stream) hipStreamSynchronize(ctx->stream);
     for(auto& ev : ctx->events)
         if(ev) hipEventDestroy(ev);
     for(auto& ev : ctx->gemm_events) hipEventDestroy(ev);
@@ -74,15 +76,94 @@ extern "C" int xh_alloc(xh_ctx* ctx, size_t bytes, void** dptr)
     return guard(ctx, [&] {
         *dptr = nullptr;
         if(bytes == 0) return;
-        check(hipMallocAsync(dptr, bytes, ctx->stream), "hipMallocAsync");
+        check(xh::dmalloc(ctx, dptr, bytes), "hipMallocAsync");
     });
 }
 
 extern "C" int xh_free(xh_ctx* ctx, void* dptr)
 {
     return guard(ctx, [&] {
-        if(dptr) check(hipFreeAsync(dptr, ctx->stream), "hipFreeAsync");
+        if(dptr) check(xh::dfree(ctx, dptr), "hipFreeAsync");
     });
+}
+
+namespace {
+// size class of a request: 1 KiB, or the request rounded up to a quarter of its power of two
+// (2^e < b <= 2^(e+1) -> a multiple of 2^(e-2): 1.25, 1.5, 1.75 or 2 x 2^e)
+size_t dclass(size_t b)
+{
+    if(b <= 1024) return 1024;
+    const int e = 63 - __builtin_clzll(static_cast<unsigned long long>(b - 1));
+    const size_t q = size_t(1) << (e - 2);
+    return (b + q - 1) / q * q;
+}
+}  // namespace
+
+hipError_t xh::dmalloc(xh_ctx* ctx, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if(xh_graph_capturing(ctx)) return hipMallocAsync(p, bytes, ctx->stream);  // owned by the capture
+    const size_t c = dclass(bytes);
+    auto it = ctx->dblock_idle.find(c);
+    if(it != ctx->dblock_idle.end() && !it->second.empty()) {
+        *p = it->second.back();
+        it->second.pop_back();
+        ctx->dcache_bytes -= c;
+        ctx->dblock_live.emplace(*p, c);
+        ++ctx->dcache_hits;
+        return hipSuccess;
+    }
+    hipError_t e = hipMallocAsync(p, c, ctx->stream);
+    if((e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) && ctx->dcache_bytes > 0) {
+        (void)hipGetLastError();
+        dcache_release(ctx);
+        e = hipMallocAsync(p, c, ctx->stream);
+    }
+    if(e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    ctx->dblock_live.emplace(*p, c);
+    ++ctx->dcache_misses;
+    return hipSuccess;
+}
+
+hipError_t xh::dfree(xh_ctx* ctx, void* p)
+{
+    if(!p) return hipSuccess;
+    auto it = ctx->dblock_live.find(p);
+    if(it == ctx->dblock_live.end()) return hipFreeAsync(p, ctx->stream);  // not from the cache
+    const size_t c = it->second;
+    ctx->dblock_live.erase(it);
+    if(xh_graph_capturing(ctx) || ctx->dcache_bytes + c > ctx->dcache_limit) return hipFreeAsync(p, ctx->stream);
+    ctx->dblock_idle[c].push_back(p);
+    ctx->dcache_bytes += c;
+    return hipSuccess;
+}
+
+void xh::dcache_release(xh_ctx* ctx)
+{
+    for(auto& kv : ctx->dblock_idle)
+        for(void* p : kv.second) hipFreeAsync(p, ctx->stream);
+    ctx->dblock_idle.clear();
+    ctx->dcache_bytes = 0;
+}
+
+extern "C" int xh_set_device_cache(xh_ctx* ctx, size_t bytes)
+{
+    return guard(ctx, [&] {
+        ctx->dcache_limit = bytes;
+        if(ctx->dcache_bytes > bytes) xh::dcache_release(ctx);
+    });
+}
+
+extern "C" int xh_device_cache_stats(xh_ctx* ctx, int64_t* hits, int64_t* misses, size_t* idle_bytes)
+{
+    if(!ctx) return XH_EINVAL;
+    if(hits) *hits = ctx->dcache_hits;
+    if(misses) *misses = ctx->dcache_misses;
+    if(idle_bytes) *idle_bytes = ctx->dcache_bytes;
+    return XH_OK;
 }
 
 extern "C" int xh_upload(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
@@ -852,7 +933,7 @@ extern "C" int xh_block_trace(xh_ctx* ctx, const int64_t* off, const int32_t* m,
         int64_t* d_off = xh::dev_upload(ctx, o);
         int32_t* d_m = xh::dev_upload(ctx, mm);
         double* d_out = nullptr;
-        check(hipMallocAsync(reinterpret_cast<void**>(&d_out), nblocks * sizeof(double), ctx->stream), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&d_out), nblocks * sizeof(double)), "hipMallocAsync");
         hipLaunchKernelGGL(k_block_trace, dim3(static_cast<unsigned>(nblocks)), dim3(EW_THREADS), 0, ctx->stream, d_off, d_m, x, d_out);
         xh::launch_check("k_block_trace");
         check(hipMemcpyAsync(out, d_out, nblocks * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
@@ -958,7 +1039,7 @@ extern "C" int xh_max_abs_index(xh_ctx* ctx, const double* x, size_t n, double* 
         if(n == 0) return;
         const unsigned nb = std::min(ew_grid(n), 1024u);
         MaxIdx* part = nullptr;
-        check(hipMallocAsync(reinterpret_cast<void**>(&part), (nb + 1) * sizeof(MaxIdx), ctx->stream), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&part), (nb + 1) * sizeof(MaxIdx)), "hipMallocAsync");
         hipLaunchKernelGGL(k_maxabs_idx, dim3(nb), dim3(EW_THREADS), 0, ctx->stream, x, n, part);
         xh::launch_check("k_maxabs_idx");
         hipLaunchKernelGGL(k_fold_idx, dim3(1), dim3(EW_THREADS), 0, ctx->stream, part, static_cast<int>(nb));
@@ -1030,7 +1111,7 @@ extern "C" int xh_range_sumsq(xh_ctx* ctx, const int64_t* off, const int64_t* le
         v.insert(v.end(), len, len + nranges);
         int64_t* d = xh::dev_upload(ctx, v);
         double* d_out = nullptr;
-        check(hipMallocAsync(reinterpret_cast<void**>(&d_out), size_t(nranges) * sizeof(double), ctx->stream), "hipMallocAsync");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&d_out), size_t(nranges) * sizeof(double)), "hipMallocAsync");
         hipLaunchKernelGGL(k_range_sumsq, dim3(static_cast<unsigned>(nranges)), dim3(EW_THREADS), 0, ctx->stream, d, d + nranges, x, d_out);
         xh::launch_check("k_range_sumsq");
         check(hipMemcpyAsync(out, d_out, size_t(nranges) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");

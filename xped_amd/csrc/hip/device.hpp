@@ -45,6 +45,17 @@ struct xh_ctx {
         hipEvent_t ev = nullptr;
     };
     std::vector<PinnedBuf> pinned_free, pinned_busy;
+    // pinned slabs that small uploads are sent from: the current one (filled up to slab_used),
+    // full ones whose copies may still run, and idle ones
+    PinnedBuf slab{};
+    size_t slab_used = 0;
+    std::vector<PinnedBuf> slab_free, slab_busy;
+    // device blocks (xh::dmalloc / xh::dfree): the size class of every block handed out, idle
+    // blocks by size class, the bytes they hold and their bound, and counters
+    std::unordered_map<void*, size_t> dblock_live;
+    std::unordered_map<size_t, std::vector<void*>> dblock_idle;
+    size_t dcache_bytes = 0, dcache_limit = size_t(4) << 30;
+    int64_t dcache_hits = 0, dcache_misses = 0;
     // HIP-event timing of every grouped GEMM launch (xh_gemm_timing): event pairs in launch order
     bool gemm_timing = false;
     std::vector<hipEvent_t> gemm_events;  // pool, 2 per timed launch
@@ -189,19 +200,20 @@ inline LegInfo make_leg(uint32_t ext, int64_t sstride, bool is_col)
     return LegInfo{ext, f.magic, f.shift | (is_col ? 256u : 0u), static_cast<uint32_t>(sstride)};
 }
 
-template <typename T>
-T* dev_upload(xh_ctx* ctx, const std::vector<T>& v)
-{
-    if(v.empty()) return nullptr;
-    T* p = nullptr;
-    check(hipMallocAsync(reinterpret_cast<void**>(&p), v.size() * sizeof(T), ctx->stream), "hipMallocAsync(plan)");
-    check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
-    return p;
-}
+// Stream-ordered device memory of the context (ctx.hip).  hipFreeAsync took 90 us of host time
+// per call on the MI355X (212 ms in one CTMRG step of 2361 frees), so freed blocks are kept in
+// size classes (four per power of two) and handed out again by later requests of the same class.
+// Every launch of a context runs on its one stream, so a block freed after the launches that read
+// it were enqueued may serve the next launch at once, as with hipFreeAsync.  Idle blocks beyond
+// dcache_limit bytes go back with hipFreeAsync; blocks requested under graph capture bypass the
+// cache.
+hipError_t dmalloc(xh_ctx* ctx, void** p, size_t bytes);
+hipError_t dfree(xh_ctx* ctx, void* p);
+void dcache_release(xh_ctx* ctx);  // every idle block back to the runtime
 
 inline void dev_free(xh_ctx* ctx, void* p)
 {
-    if(p) hipFreeAsync(p, ctx->stream);
+    if(p) dfree(ctx, p);
 }
 
 // Pinned staging buffers (hipHostMalloc), reused once the copy that read them has completed.
@@ -250,20 +262,106 @@ inline void pinned_put(xh_ctx* ctx, const xh_ctx::PinnedBuf& b)
 }
 inline void pinned_destroy_all(xh_ctx* ctx)
 {
-    for(auto* v : {&ctx->pinned_free, &ctx->pinned_busy})
+    if(ctx->slab.p) ctx->slab_free.push_back(ctx->slab);
+    ctx->slab = {};
+    for(auto* v : {&ctx->pinned_free, &ctx->pinned_busy, &ctx->slab_free, &ctx->slab_busy})
         for(auto& b : *v) {
             hipHostFree(b.p);
             hipEventDestroy(b.ev);
         }
     ctx->pinned_free.clear();
     ctx->pinned_busy.clear();
+    ctx->slab_free.clear();
+    ctx->slab_busy.clear();
+}
+
+// A host-to-device copy from pageable memory may return only once the work queued before it on
+// the stream has run (a 200 KB copy behind 30 ms of queued kernels returned after 8 ms on the
+// MI355X, the same copy from pinned memory after 0.01 ms), which starves the GPU while plans are
+// built between launches.  Small uploads are therefore copied into a pinned slab and sent from
+// there: each takes a 256-byte aligned region of the current slab; a full slab is retired behind
+// an event recorded after its last copy and reused once that event has completed.
+constexpr size_t SLAB_BYTES = size_t(4) << 20;
+inline void* slab_take(xh_ctx* ctx, size_t bytes)
+{
+    if(bytes > SLAB_BYTES) throw Error(XH_EINVAL, "slab upload too large");
+    size_t off = (ctx->slab_used + 255) & ~size_t(255);
+    if(!ctx->slab.p || off + bytes > ctx->slab.cap) {
+        if(ctx->slab.p) {
+            check(hipEventRecord(ctx->slab.ev, ctx->stream), "hipEventRecord");
+            ctx->slab_busy.push_back(ctx->slab);
+            ctx->slab = {};
+        }
+        auto& busy = ctx->slab_busy;
+        std::size_t keep = 0;
+        for(std::size_t i = 0; i < busy.size(); ++i) {
+            if(hipEventQuery(busy[i].ev) == hipSuccess)
+                ctx->slab_free.push_back(busy[i]);
+            else
+                busy[keep++] = busy[i];
+        }
+        busy.resize(keep);
+        while(ctx->slab_free.size() > 8) {  // keep at most 8 idle slabs
+            hipHostFree(ctx->slab_free.back().p);
+            hipEventDestroy(ctx->slab_free.back().ev);
+            ctx->slab_free.pop_back();
+        }
+        if(!ctx->slab_free.empty()) {
+            ctx->slab = ctx->slab_free.back();
+            ctx->slab_free.pop_back();
+        } else {
+            xh_ctx::PinnedBuf b;
+            b.cap = SLAB_BYTES;
+            check(hipHostMalloc(&b.p, b.cap, hipHostMallocDefault), "hipHostMalloc(slab)");
+            if(hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) {
+                hipHostFree(b.p);
+                throw Error(XH_EHIP, "hipEventCreate(slab)");
+            }
+            ctx->slab = b;
+        }
+        off = 0;
+    }
+    ctx->slab_used = off + bytes;
+    return static_cast<char*>(ctx->slab.p) + off;
+}
+
+// Asynchronous copy of host memory to the device, ordered on the context stream; the caller may
+// reuse or free src as soon as this returns.  Not for a stream under graph capture (the graph
+// would copy from the staging memory again at every replay).
+inline void upload_async(xh_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if(bytes == 0) return;
+    if(bytes <= SLAB_BYTES / 4) {
+        void* s = slab_take(ctx, bytes);
+        std::memcpy(s, src, bytes);
+        check(hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(upload)");
+        return;
+    }
+    xh_ctx::PinnedBuf b = pinned_get(ctx, bytes);
+    std::memcpy(b.p, src, bytes);
+    const hipError_t e = hipMemcpyAsync(dst, b.p, bytes, hipMemcpyHostToDevice, ctx->stream);
+    pinned_put(ctx, b);
+    check(e, "hipMemcpyAsync(upload)");
+}
+
+// a device copy of v, freed by the caller (dev_free) in stream order
+template <typename T>
+T* dev_upload(xh_ctx* ctx, const std::vector<T>& v)
+{
+    if(v.empty()) return nullptr;
+    refuse_under_capture(ctx, "a host upload");
+    T* p = nullptr;
+    check(xh::dmalloc(ctx, reinterpret_cast<void**>(&p), v.size() * sizeof(T)), "hipMallocAsync(plan)");
+    upload_async(ctx, p, v.data(), v.size() * sizeof(T));
+    return p;
 }
 
 // Host arrays packed into one device allocation and sent with one asynchronous copy on the
-// context stream, without a host synchronisation.  The host copy is kept until release(), which
-// first waits for the event recorded after the copy.  Large uploads whose size is known up front
-// (begin_pinned) are written straight into a pinned staging buffer that returns to the context's
-// pool when the copy has completed.
+// context stream, without a host synchronisation.  The host arrays are sent through upload_async
+// (pinned staging) and dropped; only under graph capture is the host copy kept until release(),
+// which first waits for the event recorded after the copy.  Large uploads whose size is known up
+// front (begin_pinned) are written straight into a pinned staging buffer that returns to the
+// context's pool when the copy has completed.
 struct PackedUpload {
     std::vector<char> host;
     void* dev = nullptr;
@@ -326,7 +424,7 @@ struct PackedUpload {
         if(pin.p) {
             owner = ctx;
             if(pin_used > 0) {
-                check(hipMallocAsync(&dev, pin_used, ctx->stream), "hipMallocAsync(plan)");
+                check(xh::dmalloc(ctx, &dev, pin_used), "hipMallocAsync(plan)");
                 check(hipMemcpyAsync(dev, pin.p, pin_used, hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
             }
             pinned_put(ctx, pin);
@@ -335,7 +433,13 @@ struct PackedUpload {
         }
         if(host.empty()) return;
         owner = ctx;
-        check(hipMallocAsync(&dev, host.size(), ctx->stream), "hipMallocAsync(plan)");
+        check(xh::dmalloc(ctx, &dev, host.size()), "hipMallocAsync(plan)");
+        if(!xh_graph_capturing(ctx)) {
+            upload_async(ctx, dev, host.data(), host.size());
+            std::vector<char>().swap(host);
+            return;
+        }
+        // under capture the graph's copy node reads the host copy, kept until release()
         check(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync(plan)");
         check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
         check(hipEventRecord(done, ctx->stream), "hipEventRecord");
@@ -352,7 +456,7 @@ struct PackedUpload {
             hipEventDestroy(done);
             done = nullptr;
         }
-        if(dev) hipFreeAsync(dev, ctx->stream);
+        if(dev) xh::dfree(ctx, dev);
         dev = nullptr;
         host.clear();
     }
@@ -384,7 +488,7 @@ inline void collect_staging(xh_ctx* ctx, bool wait)
 
 inline void PackedUpload::release_deferred(xh_ctx* ctx)
 {
-    if(dev) hipFreeAsync(dev, ctx->stream);
+    if(dev) xh::dfree(ctx, dev);
     dev = nullptr;
     if(done) ctx->pending_host.emplace_back(done, std::move(host));
     done = nullptr;

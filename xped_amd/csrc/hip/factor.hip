@@ -717,13 +717,13 @@ void run_factor(xh_ctx* ctx, void (*kern_lds)(FactorArgs), void (*kern_global)(F
         }
     }
     int* d_status = nullptr;
-    check(hipMallocAsync(reinterpret_cast<void**>(&d_status), sizeof(int), ctx->stream), "hipMallocAsync");
+    check(xh::dmalloc(ctx, reinterpret_cast<void**>(&d_status), sizeof(int)), "hipMallocAsync");
     check(hipMemsetAsync(d_status, 0, sizeof(int), ctx->stream), "hipMemsetAsync");
     auto go = [&](const std::vector<xh_factor_item>& v, bool lds, int64_t maxd) {
         if(v.empty()) return;
         xh_factor_item* d_items = xh::dev_upload(ctx, v);
         double* scratch = nullptr;
-        if(!lds) check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(maxd) * v.size() * sizeof(double), ctx->stream), "hipMallocAsync");
+        if(!lds) check(xh::dmalloc(ctx, reinterpret_cast<void**>(&scratch), size_t(maxd) * v.size() * sizeof(double)), "hipMallocAsync");
         FactorArgs fa{d_items, A, U, S, Vt, scratch, maxd, lds ? 1 : 0, d_status};
         auto* kern = lds ? kern_lds : kern_global;
         const size_t shmem = lds ? size_t(maxd) * sizeof(double) : 0;
@@ -834,7 +834,7 @@ extern "C" int xh_block_chol_inv(xh_ctx* ctx, const xh_factor_item* items, int64
         }
         if(!big.empty()) {
             double* scratch = nullptr;
-            check(hipMallocAsync(reinterpret_cast<void**>(&scratch), size_t(big_max) * big.size() * sizeof(double), ctx->stream), "hipMallocAsync");
+            check(xh::dmalloc(ctx, reinterpret_cast<void**>(&scratch), size_t(big_max) * big.size() * sizeof(double)), "hipMallocAsync");
             FactorArgs fa{up.ptr<xh_factor_item>(o_big), G, X, d_sh, nullptr, scratch, big_max, 0, nullptr};
             hipLaunchKernelGGL(k_chol_inv<true>, dim3(static_cast<unsigned>(big.size())), dim3(CI_THREADS), 0, ctx->stream, fa);
             xh::launch_check("k_chol_inv");

@@ -1365,7 +1365,7 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
         bool own_ws = false;
         if(plan->ws_doubles > 0 && !ws) {
             xh::refuse_under_capture(ctx, "a split-K GEMM plan (it allocates its partials; xh_gemm_noalloc does not)");
-            check(hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double), ctx->stream), "hipMallocAsync(split K)");
+            check(xh::dmalloc(ctx, reinterpret_cast<void**>(&ws), size_t(plan->ws_doubles) * sizeof(double)), "hipMallocAsync(split K)");
             own_ws = true;
         }
         if(plan->dead) throw xh::Error(XH_EINVAL, "gemm plan used after xh_gemm_plan_destroy");
@@ -1408,7 +1408,7 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
                     const size_t ex_off = (size_t(plan->oz_slice_bytes) + 255) / 256 * 256;
                     const size_t bytes = ex_off + size_t(plan->oz_exps) * sizeof(int32_t);
                     xh::refuse_under_capture(ctx, "an XH_OZAKI GEMM (it allocates its slices)");
-                    check(hipMallocAsync(reinterpret_cast<void**>(&sl), bytes, ctx->stream), "hipMallocAsync(Ozaki slices)");
+                    check(xh::dmalloc(ctx, reinterpret_cast<void**>(&sl), bytes), "hipMallocAsync(Ozaki slices)");
                     OzArgs oa{plan->d_oz, sl, reinterpret_cast<int32_t*>(sl + ex_off)};
                     if(plan->n_jobs_rows > 0) {
                         hipLaunchKernelGGL(k_oz_split_rows, dim3(static_cast<unsigned>(plan->n_jobs_rows)), dim3(256), 0, ctx->stream, g, oa, plan->d_jobs_rows);
@@ -1425,7 +1425,7 @@ void gemm_run(xh_ctx* ctx, const xh_gemm_plan* cplan, const double* A, const dou
                     grid = (grid + NPART - 1) / NPART * NPART;
                     hipLaunchKernelGGL(k_gemm_oz, dim3(grid), dim3(OZ_THREADS), 0, ctx->stream, g, oa);
                     xh::launch_check("k_gemm_oz");
-                    check(hipFreeAsync(sl, ctx->stream), "hipFreeAsync(Ozaki slices)");
+                    check(xh::dfree(ctx, sl), "hipFreeAsync(Ozaki slices)");
                 }
             } else {
                 launch(plan->all64, precision == XH_FP32 ? k_gemm_f32 : k_gemm_bf16x3, 8, G_THREADS, "k_gemm");

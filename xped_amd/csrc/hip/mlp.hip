@@ -172,7 +172,7 @@ void ensure_capacity(xh_ctx* ctx, xh_mlp* net, int64_t B)
     const int64_t w = net->width;
     auto a = [&](int64_t n) {
         float* p = nullptr;
-        check(hipMallocAsync(reinterpret_cast<void**>(&p), size_t(std::max<int64_t>(n, 1)) * sizeof(float), ctx->stream), "hipMallocAsync(mlp)");
+        check(xh::dmalloc(ctx, reinterpret_cast<void**>(&p), size_t(std::max<int64_t>(n, 1)) * sizeof(float)), "hipMallocAsync(mlp)");
         return p;
     };
     net->h1 = a(B * w);
@@ -224,7 +224,7 @@ extern "C" int xh_mlp_create(xh_ctx* ctx, int nin, int width, uint64_t seed, xh_
         for(int64_t i = 0; i < int64_t(width) * width; ++i) h[L.W2 + i] = float(normal() * std::sqrt(2.0 / width));
         for(int64_t i = 0; i < width; ++i) h[L.w3 + i] = float(normal() * std::sqrt(1.0 / width));
         for(float** p : {&net->params, &net->m, &net->v})
-            check(hipMallocAsync(reinterpret_cast<void**>(p), size_t(L.n) * sizeof(float), ctx->stream), "hipMallocAsync(mlp)");
+            check(xh::dmalloc(ctx, reinterpret_cast<void**>(p), size_t(L.n) * sizeof(float)), "hipMallocAsync(mlp)");
         check(hipMemcpyAsync(net->params, h.data(), size_t(L.n) * sizeof(float), hipMemcpyHostToDevice, ctx->stream), "hipMemcpyAsync");
         check(hipMemsetAsync(net->m, 0, size_t(L.n) * sizeof(float), ctx->stream), "hipMemsetAsync");
         check(hipMemsetAsync(net->v, 0, size_t(L.n) * sizeof(float), ctx->stream), "hipMemsetAsync");

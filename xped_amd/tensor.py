@@ -97,6 +97,16 @@ class Runtime:
         return {"permute_plan_s": a.value, "gemm_plan_s": b.value, "n_permute_plans": na.value, "n_gemm_plans": nb.value,
                 "breakdown_s": [round(x, 4) for x in list(t[:5]) + list(t[6:8])], "symbolic_hits": int(t[5])}
 
+    def device_cache(self, limit_bytes=None):
+        """Counters of the context's cache of freed device blocks: allocations it served (hits), those
+        that reached the HIP runtime (misses) and the bytes it holds idle.  With limit_bytes, first
+        bounds the idle bytes (0 keeps none; the default bound is 4 GiB)."""
+        if limit_bytes is not None:
+            _check_xh(lib().xh_set_device_cache(self.ctx(), int(limit_bytes)), self.ctx(), "device cache")
+        h, m, b = C.c_int64(), C.c_int64(), C.c_size_t()
+        _check_xh(lib().xh_device_cache_stats(self.ctx(), C.byref(h), C.byref(m), C.byref(b)), self.ctx(), "device cache")
+        return {"hits": h.value, "misses": m.value, "idle_bytes": b.value}
+
     def graph_begin(self):
         """Start capturing the runtime's stream into a HIP graph (see Graph)."""
         _check_xh(lib().xh_graph_begin(self.ctx()), self.ctx(), "graph capture")
