@@ -6,17 +6,37 @@
 
 namespace xa {
 
-void rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* f)
+namespace {
+// the features of a sector that do not depend on the cut (f[0..16]) and the two logarithms of f[18]
+struct FeatureBase {
+    float f[RANK_NFEAT];
+    double lg_glob, lg_last;
+};
+double lg10(double x) { return x > 0.0 ? std::log10(x) : -300.0; }
+FeatureBase feature_base(const double* top, int ntop, int dim, double s_glob)
 {
     // decades, scaled by 1/4 so that the inputs are O(1)
+    FeatureBase b{};
     const double s1 = ntop > 0 ? top[0] : 0.0;
-    auto lg = [](double x) { return x > 0.0 ? std::log10(x) : -300.0; };
-    f[0] = static_cast<float>(0.25 * std::max(-16.0, lg(s1) - lg(s_glob)));
-    for(int j = 1; j < RANK_TOP; ++j) f[j] = static_cast<float>(0.25 * (j < ntop ? std::max(-16.0, lg(top[j]) - lg(s1)) : -16.0));
-    f[16] = static_cast<float>(0.25 * std::log10(std::max(1, dim)));
-    f[17] = static_cast<float>(0.25 * log_cut);
+    b.f[0] = static_cast<float>(0.25 * std::max(-16.0, lg10(s1) - lg10(s_glob)));
+    for(int j = 1; j < RANK_TOP; ++j) b.f[j] = static_cast<float>(0.25 * (j < ntop ? std::max(-16.0, lg10(top[j]) - lg10(s1)) : -16.0));
+    b.f[16] = static_cast<float>(0.25 * std::log10(std::max(1, dim)));
     const double slast = ntop > 0 ? top[std::min(ntop, RANK_TOP) - 1] : 0.0;
-    f[18] = static_cast<float>(0.25 * std::max(-16.0, std::min(16.0, log_cut + lg(s_glob) - lg(slast))));
+    b.lg_glob = lg10(s_glob);
+    b.lg_last = lg10(slast);
+    return b;
+}
+void features_at(const FeatureBase& b, double log_cut, float* f)
+{
+    std::copy(b.f, b.f + 17, f);
+    f[17] = static_cast<float>(0.25 * log_cut);
+    f[18] = static_cast<float>(0.25 * std::max(-16.0, std::min(16.0, log_cut + b.lg_glob - b.lg_last)));
+}
+}  // namespace
+
+void rank_features(const double* top, int ntop, int dim, double s_glob, double log_cut, float* f)
+{
+    features_at(feature_base(top, ntop, dim, s_glob), log_cut, f);
 }
 
 namespace {
@@ -56,9 +76,11 @@ std::vector<double> RankPredictor::counts_grid(const std::vector<std::vector<dou
     const int nsec = static_cast<int>(top.size());
     const int64_t B = static_cast<int64_t>(nsec) * static_cast<int64_t>(log_cuts.size());
     std::vector<float> X(static_cast<std::size_t>(B) * RANK_NFEAT);
+    // the logarithms of a sector's values once, then per cut its two cut features
+    std::vector<FeatureBase> base(static_cast<std::size_t>(nsec));
+    for(int q = 0; q < nsec; ++q) base[q] = feature_base(top[q].data(), static_cast<int>(top[q].size()), dims[q], s_glob);
     for(std::size_t c = 0; c < log_cuts.size(); ++c)
-        for(int q = 0; q < nsec; ++q)
-            rank_features(top[q].data(), static_cast<int>(top[q].size()), dims[q], s_glob, log_cuts[c], X.data() + (c * nsec + q) * RANK_NFEAT);
+        for(int q = 0; q < nsec; ++q) features_at(base[q], log_cuts[c], X.data() + (c * nsec + q) * RANK_NFEAT);
     float *dX = nullptr, *dY = nullptr;
     rchk(ctx, xh_alloc(ctx, X.size() * sizeof(float), reinterpret_cast<void**>(&dX)), "alloc");
     rchk(ctx, xh_alloc(ctx, std::max<int64_t>(B, 1) * sizeof(float), reinterpret_cast<void**>(&dY)), "alloc");
