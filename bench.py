@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--leg-dim", type=int, default=512, help="c3 leg dimension")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "ozaki"], help="GEMM precision of the fast mode / c3")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--device-cache-gib", type=float, default=None,
+                    help="bound of the idle device blocks each runtime keeps for reuse (library default 4; 0 returns every freed block)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 mode (c4)")
@@ -129,6 +131,8 @@ def run_mode(mode, args, xt, configs, random_ipeps, ranks=None, fresh=False):
     """One CTMRG mode on a fresh runtime; returns timings, energies and GEMM statistics."""
     rt = xt.Runtime(0)
     rt.fuse_permute(not args.no_fuse)
+    if args.device_cache_gib is not None:
+        rt.device_cache(int(args.device_cache_gib * 2**30))
     if args.autotune and mode == "fast":
         xt.lib().xh_set_gemm_autotune(rt.ctx(), 1)
     A = [a.scale(SCALE) for a in random_ipeps(rt, configs.c4_virtual(), configs.c4_physical(), args.seed)]
