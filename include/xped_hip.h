@@ -63,6 +63,10 @@ int xh_copy(xh_ctx* ctx, void* dst, const void* src, size_t bytes); /* device to
 int xh_synchronize(xh_ctx* ctx);
 void* xh_stream(xh_ctx* ctx); /* the context's hipStream_t */
 int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value);
+/* dst[off_r .. off_r + len_r) = value for the nranges (off_r, len_r) pairs of ranges[2 nranges],
+ * each inside dst[0, n), in one launch: the zero sectors of a product that no GEMM item writes
+ * (Tout.setZero() before the products, TensorBase.cpp:256-258) */
+int xh_fill_ranges(xh_ctx* ctx, double* dst, size_t n, const int64_t* ranges, int64_t nranges, double value);
 /* Standard normal entries from a counter-based generator (splitmix64 + Box-Muller), used for the
  * test matrices of the randomized block SVD; no counterpart in the reference. */
 int xh_fill_gaussian(xh_ctx* ctx, double* dst, size_t n, uint64_t seed);

@@ -68,6 +68,7 @@ _SIGS = [
     ("xh_alloc", C.c_int, [_P, _SZ, _PP]),
     ("xh_free", C.c_int, [_P, _P]),
     ("xh_set_device_cache", C.c_int, [_P, _SZ]),
+    ("xh_fill_ranges", C.c_int, [_P, _P, _SZ, _P, C.c_int64, C.c_double]),
     ("xh_device_cache_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(_SZ)]),
     ("xh_upload", C.c_int, [_P, _P, _P, _SZ]),
     ("xh_download", C.c_int, [_P, _P, _P, _SZ]),

@@ -442,6 +442,14 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
             xh_copy_rect r{0, 0, static_cast<int64_t>(o.offsets[s]), static_cast<int64_t>(o.rows[s]), static_cast<int32_t>(o.rows[s]),
                            static_cast<int32_t>(o.cols[s]), 0.0, 0};
             cg->zero_rects.push_back(r);
+            const std::int64_t off = static_cast<std::int64_t>(o.offsets[s]), len = static_cast<std::int64_t>(o.rows[s] * o.cols[s]);
+            auto& zr = cg->zero_ranges;
+            if(!zr.empty() && zr[zr.size() - 2] + zr.back() == off)
+                zr.back() += len;
+            else {
+                zr.push_back(off);
+                zr.push_back(len);
+            }
         }
     lap(3);
     if(rt->ctx) {
@@ -458,6 +466,15 @@ static std::shared_ptr<CachedGemm> get_gemm_plan(Runtime* rt, const TensorStruct
 }
 
 static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r, DTensor& out, int precision);
+
+// sectors without a matching pair stay zero (Tout.setZero(), TensorBase.cpp:256-258): one launch
+// over the plan's merged ranges of unwritten sectors
+static void zero_unwritten(Runtime* rt, const CachedGemm& cg, DTensor& out)
+{
+    if(cg.zero_ranges.empty() || out.size() == 0) return;
+    dcheck(rt->ctx, xh_fill_ranges(rt->ctx, out.data(), out.size(), cg.zero_ranges.data(), static_cast<int64_t>(cg.zero_ranges.size() / 2), 0.0),
+           "zero fill");
+}
 
 // While GEMM launches are timed, contraction products carry tag 1 in the launch log and are
 // bracketed in a kernel trace by marker kernels of 3 (before) and 4 (after) workgroups, so that
@@ -525,9 +542,7 @@ std::vector<DTensor> mul_batch(const std::vector<std::pair<DTensor, DTensor>>& p
         double flops = 0.0;
         for(std::size_t i = 0; i < n; ++i) {
             const CachedGemm& cg = *cgs[i];
-            if(outs[i].size() > 0)
-                for(const auto& z : cg.zero_rects)
-                    dcheck(rt->ctx, xh_fill(rt->ctx, outs[i].data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
+            zero_unwritten(rt, cg, outs[i]);
             if(cg.plan.items.empty()) continue;
             const int64_t da = pairs[i].first.data() - A0, db = pairs[i].second.data() - B0, dc = outs[i].data() - C0;
             for(const auto& g : cg.plan.items) {
@@ -590,11 +605,8 @@ void mul_into(const DTensor& l, const DTensor& r, DTensor& out, int precision)
 static void launch_mul(const CachedGemm* cg, const DTensor& l, const DTensor& r, DTensor& out, int precision)
 {
     if(l.rt->ctx && out.size() > 0) {
-        if(!cg->zero_rects.empty()) {
-            // sectors without a matching pair stay zero (Tout.setZero(), TensorBase.cpp:256-258)
-            for(const auto& z : cg->zero_rects) dcheck(l.rt->ctx, xh_fill(l.rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
-        }
         Runtime* rt = l.rt;
+        zero_unwritten(rt, *cg, out);
         tag_contraction(rt, true);
         const int rc = xh_gemm(rt->ctx, cg->dev, l.data(), r.data(), out.data(), 1.0, 0.0, precision);
         tag_contraction(rt, false);
@@ -832,8 +844,7 @@ static void launch_gather(const CachedGather* g, const DTensor& a, const DTensor
     Runtime* rt = a.rt;
     if(out.size() == 0) return;
     const CachedGemm* cg = g->gemm.get();
-    // sectors without a matching pair stay zero, as in operator*
-    for(const auto& z : cg->zero_rects) dcheck(rt->ctx, xh_fill(rt->ctx, out.data() + z.dst_off, size_t(z.rows) * z.cols, 0.0), "zero fill");
+    zero_unwritten(rt, *cg, out);  // sectors without a matching pair stay zero, as in operator*
     tag_contraction(rt, true);
     const int rc = xh_gemm_gather(rt->ctx, cg->dev, g->dev, a.data(), static_cast<int64_t>(a.size()), pb.data(), out.data(), 1.0, 0.0);
     tag_contraction(rt, false);

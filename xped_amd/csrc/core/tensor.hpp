@@ -51,6 +51,7 @@ struct CachedGemm {
     GemmPlan plan;
     xh_gemm_plan* dev = nullptr;
     std::vector<xh_copy_rect> zero_rects;  // output sectors not written by any GEMM
+    std::vector<std::int64_t> zero_ranges;  // the same as (offset, length) pairs, adjacent sectors merged
     std::uint64_t last_use = 0;
 };
 

@@ -436,6 +436,20 @@ __global__ void k_fill(double* __restrict__ x, size_t n, double v)
     for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = v;
 }
 
+// ranges r = blockIdx.y of (offset, length): from the kernel arguments (up to FR_ARGS ranges) or
+// from a device table (off, len pairs)
+constexpr int FR_ARGS = 8;
+struct FillRanges {
+    int64_t off[FR_ARGS], len[FR_ARGS];
+};
+__global__ void k_fill_ranges(double* __restrict__ x, FillRanges fr, const int64_t* __restrict__ table, double v)
+{
+    const int r = blockIdx.y;
+    const int64_t off = table ? table[2 * r] : fr.off[r], len = table ? table[2 * r + 1] : fr.len[r];
+    double* p = x + off;
+    for(int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < len; e += int64_t(gridDim.x) * blockDim.x) p[e] = v;
+}
+
 __global__ void k_scale(double* __restrict__ x, size_t n, double a)
 {
     for(size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) x[i] = a * x[i];
@@ -840,6 +854,46 @@ extern "C" int xh_fill(xh_ctx* ctx, double* dst, size_t n, double value)
         if(n == 0) return;
         hipLaunchKernelGGL(k_fill, dim3(ew_grid(n)), dim3(EW_THREADS), 0, ctx->stream, dst, n, value);
         xh::launch_check("k_fill");
+    });
+}
+
+extern "C" int xh_fill_ranges(xh_ctx* ctx, double* dst, size_t n, const int64_t* ranges, int64_t nranges, double value)
+{
+    return guard(ctx, [&] {
+        if(nranges < 0) throw xh::Error(XH_EINVAL, "fill ranges: negative count");
+        int64_t maxlen = 0;
+        for(int64_t r = 0; r < nranges; ++r) {
+            const int64_t off = ranges[2 * r], len = ranges[2 * r + 1];
+            if(off < 0 || len < 0 || static_cast<uint64_t>(off) + static_cast<uint64_t>(len) > n)
+                throw xh::Error(XH_EINVAL, "fill ranges: a range lies outside the buffer");
+            maxlen = std::max(maxlen, len);
+        }
+        if(maxlen == 0) return;
+        const dim3 block(EW_THREADS);
+        const unsigned gx = std::min(ew_grid(static_cast<size_t>(maxlen)), 1024u);
+        for(int64_t r0 = 0; r0 < nranges;) {
+            if(nranges - r0 <= FR_ARGS) {
+                FillRanges fr{};
+                const int nr = static_cast<int>(nranges - r0);
+                for(int i = 0; i < nr; ++i) {
+                    fr.off[i] = ranges[2 * (r0 + i)];
+                    fr.len[i] = ranges[2 * (r0 + i) + 1];
+                }
+                hipLaunchKernelGGL(k_fill_ranges, dim3(gx, nr), block, 0, ctx->stream, dst, fr, static_cast<const int64_t*>(nullptr), value);
+                xh::launch_check("k_fill_ranges");
+                break;
+            }
+            // more ranges than fit in the arguments: a table of at most 65535 per launch
+            const int64_t nr = std::min<int64_t>(nranges - r0, 65535);
+            std::vector<int64_t> t(ranges + 2 * r0, ranges + 2 * (r0 + nr));
+            xh::PackedUpload up;
+            const size_t o = up.add(t);
+            up.upload(ctx);
+            hipLaunchKernelGGL(k_fill_ranges, dim3(gx, static_cast<unsigned>(nr)), block, 0, ctx->stream, dst, FillRanges{}, up.ptr<const int64_t>(o), value);
+            xh::launch_check("k_fill_ranges");
+            up.release_deferred(ctx);
+            r0 += nr;
+        }
     });
 }
 
