@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const RedItem* __restrict
 // Tiles that lie inside their item (all rows, columns and the whole K step in range) load without
 // bounds checks from per-thread offsets computed once per tile.
 template <int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_f64p(GemmArgs g)
+__global__ __launch_bounds__(64 * WGM * WGN, 4) void k_gemm_f64p(GemmArgs g)
 {
     constexpr int NT = 64 * WGM * WGN;                 // threads
     constexpr int WTM = BM / WGM, WTN = BN / WGN;       // wave tile
@@ -533,7 +533,7 @@ constexpr int BM2 = 128, BN2 = 128;
 constexpr int BIG_MIN = 192;
 
 constexpr int G8_THREADS = 512;
-__global__ __launch_bounds__(G8_THREADS) void k_gemm_f64_128(GemmArgs g)
+__global__ __launch_bounds__(G8_THREADS, 4) void k_gemm_f64_128(GemmArgs g)
 {
     constexpr int BK = 16;
     __shared__ double sA[BK][BM2 + 1];
