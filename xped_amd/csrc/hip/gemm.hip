@@ -388,7 +388,7 @@ struct ScatterDev {
 };
 
 template <int MS, bool SCATTER>
-__global__ __launch_bounds__(SK_THREADS) void k_gemm_skinny_gather(GemmArgs g, GatherDev gd, const double* __restrict__ src, ScatterDev sd)
+__global__ __launch_bounds__(SK_THREADS, SCATTER ? 1 : 4) void k_gemm_skinny_gather(GemmArgs g, GatherDev gd, const double* __restrict__ src, ScatterDev sd)
 {
     constexpr int DS = SCATTER ? GA_SEG : 1, DC = SCATTER ? SK_MAX : 1, DR = SCATTER ? GA_MAXR : 1;
     __shared__ int64_t s_doff[DS][DC];      // per tree and product column: destination offset of the column
